@@ -1,0 +1,132 @@
+/*
+ * sufkit_hip.h -- C ABI of the MI355X (gfx950) suffix-index construction
+ * backend.  Plain pointers and sizes only.
+ *
+ * This is the third construction backend beside the reference's private
+ * adapters
+ *   detail::BuildDivsufsort32/64  (src/divsufsort_backend.hpp:27-33)
+ *   detail::BuildCaps32/64        (src/caps_backend.hpp:93-99)
+ * which SuffixArray::Build selects by (SaBackend, CoordinateWidth)
+ * (src/suffix_array.cpp:4182-4236).  Like them it receives the logical text
+ * (encoded reference bytes followed by the sentinel 0, suffix_array.cpp:
+ * 4107-4117) and returns the complete SA, its ISA and the raw LCP of the Fast
+ * profile.  Unlike them it validates the permutation/inverse on the device
+ * (the checks of RepackSuffixArray, suffix_array.cpp:852-881) and writes the
+ * SA/ISA directly in the requested stored width, so the caller need not redo
+ * O(n) random CPU work.
+ *
+ * Status codes are the reference's ErrorCode (include/sufkit/types.hpp:
+ * 708-722) plus one: SFK_HIP_OK = 0, otherwise ErrorCode + 1.
+ */
+#ifndef SUFKIT_HIP_H_
+#define SUFKIT_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFK_HIP_ABI_VERSION 1
+
+enum sfk_hip_status {
+  SFK_HIP_OK = 0,
+  SFK_HIP_INVALID_INPUT = 1,       /* ErrorCode::kInvalidInput */
+  SFK_HIP_IO_ERROR = 2,            /* ErrorCode::kIoError */
+  SFK_HIP_UNSUPPORTED_BACKEND = 3, /* ErrorCode::kUnsupportedBackend */
+  SFK_HIP_CORRUPT_INDEX = 4,       /* ErrorCode::kCorruptIndex */
+  SFK_HIP_VERSION_MISMATCH = 5,    /* ErrorCode::kVersionMismatch */
+  SFK_HIP_BUILD_FAILURE = 6        /* ErrorCode::kBuildFailure */
+};
+
+/* Build flags (retain_isa / retain_lcp of divsufsort_backend.hpp:27-33). */
+#define SFK_HIP_RETAIN_ISA 1u
+#define SFK_HIP_RETAIN_LCP 2u
+/* Record per-kernel HIP events (for roofline accounting); slower. */
+#define SFK_HIP_PROFILE_KERNELS 4u
+
+/* Per-build statistics.  Times are HIP-event times on the build stream. */
+typedef struct sfk_hip_stats {
+  double sa_seconds;        /* pack + prefix doubling (SA and ranks=ISA) */
+  double isa_seconds;       /* ISA validation / narrowing */
+  double lcp_seconds;       /* PLCP + LCP permutation */
+  double device_seconds;    /* whole device build */
+  double h2d_seconds;       /* host text -> device (host entry point only) */
+  double d2h_seconds;       /* device results -> host (host entry point only) */
+  uint64_t refine_steps;    /* sort+compact steps */
+  uint64_t doubling_rounds; /* rounds with h >= 32 */
+  uint64_t active_total;    /* sum over steps of active suffixes */
+  uint64_t sort_passes;     /* LSD digit passes actually run */
+  uint64_t lce_long_items;  /* LCEs handed to the wave-wide comparer */
+  uint64_t lce_long_words;  /* 32-symbol windows compared by those waves */
+  uint64_t algorithmic_bytes_sa;  /* HBM byte model, SA phase */
+  uint64_t algorithmic_bytes_isa; /* HBM byte model, ISA phase */
+  uint64_t algorithmic_bytes_lcp; /* HBM byte model, LCP phase */
+  /* Dominant kernel (LSD scatter) when SFK_HIP_PROFILE_KERNELS is set. */
+  double sort_scatter_seconds;
+  uint64_t sort_scatter_launches;
+  uint64_t sort_scatter_bytes;
+} sfk_hip_stats;
+
+typedef struct sfk_hip_builder sfk_hip_builder;
+
+/* ABI version of the loaded library (SFK_HIP_ABI_VERSION). */
+int sfk_hip_abi_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int sfk_hip_device_count(void);
+
+/* A builder owns one HIP stream and a grow-only device workspace on
+ * `device`.  Builders are independent: two builders may build concurrently
+ * from two host threads (reentrancy as in test_caps_backend.cpp:373-404). */
+int sfk_hip_builder_create(int device, sfk_hip_builder** out, char* error,
+                           size_t error_capacity);
+void sfk_hip_builder_destroy(sfk_hip_builder* builder);
+/* The builder's stream (a hipStream_t), for callers that order their own
+ * work after a device build. */
+void* sfk_hip_builder_stream(sfk_hip_builder* builder);
+/* Bytes of device workspace a build of n symbols at construction_bits needs
+ * (excluding the caller's input/output buffers). */
+uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits);
+
+/*
+ * Device-resident build.  d_text: n device bytes (values 0..6, a single 0 at
+ * position n-1).  Outputs are device buffers of n elements:
+ *   d_sa  of sa_bits  (32 or 64), required
+ *   d_isa of isa_bits (32 or 64), or NULL
+ *   d_lcp of lcp_bits (32 or 64), or NULL
+ * construction_bits: 32 (n <= 2^32 - 1) or 64.  A stored width narrower than
+ * the construction width is checked on the device (range error ->
+ * SFK_HIP_BUILD_FAILURE).  Synchronous: returns after the stream drains.
+ */
+int sfk_hip_build_device(sfk_hip_builder* builder, const uint8_t* d_text,
+                         uint64_t n, uint32_t construction_bits, uint32_t flags,
+                         void* d_sa, uint32_t sa_bits, void* d_isa,
+                         uint32_t isa_bits, void* d_lcp, uint32_t lcp_bits,
+                         sfk_hip_stats* stats, char* error,
+                         size_t error_capacity);
+
+/*
+ * Host build: the same, from host text into host output buffers (each n
+ * elements of the given width; isa/lcp may be NULL).  This is the entry
+ * point a SuffixArray::Build backend branch calls in place of
+ * detail::BuildDivsufsort32/64 (divsufsort_backend.cpp:24-156).
+ */
+int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits,
+                  uint32_t flags, void* sa, uint32_t sa_bits, void* isa,
+                  uint32_t isa_bits, void* lcp, uint32_t lcp_bits,
+                  sfk_hip_stats* stats, char* error, size_t error_capacity);
+
+/* Device memory helpers so callers without a device allocator of their own
+ * can stage buffers (tests, benchmarks). */
+int sfk_hip_malloc(int device, uint64_t bytes, void** out);
+int sfk_hip_free(void* ptr);
+int sfk_hip_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUFKIT_HIP_H_ */

@@ -1,0 +1,194 @@
+"""CPU oracle for the suffix-index construction path.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, and only as the checker; the product
+(sufkit_amd/) never imports it.
+
+Two layers:
+  * liboracle.so (oracle.c): a plain-C restatement of the reference's
+    normalization, SA order, ISA scatter, generalised Kasai LCP, FNV-1a-64
+    and CRC32, each citing the reference lines it follows;
+  * _ref/libsufkit_ref.so: the reference's own divsufsort backend
+    (detail::BuildDivsufsort32/64 over libdivsufsort 2.0.2) compiled from
+    the reference sources by oracle/Makefile, when those sources were
+    present at build time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+_REF = None
+
+U64P = ctypes.POINTER(ctypes.c_uint64)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build(ref: str | None = "/root/reference") -> None:
+    """Compile liboracle.so and, when the reference sources exist, _ref/."""
+    env = dict(os.environ)
+    args = ["make", "-s", "-C", str(HERE), "oracle"]
+    subprocess.run(args, check=True, env=env)
+    if ref and Path(ref, "src", "divsufsort_backend.cpp").exists():
+        subprocess.run(["make", "-s", "-j8", "-C", str(HERE), "ref", f"REF={ref}"], check=True, env=env)
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            build(ref=None)
+        L = ctypes.CDLL(str(path))
+        L.sfk_oracle_encode_records.argtypes = [U8P, U64P, ctypes.c_uint64, U8P, U64P]
+        L.sfk_oracle_encode_records.restype = ctypes.c_uint64
+        L.sfk_oracle_fingerprint.argtypes = [U8P, ctypes.c_uint64]
+        L.sfk_oracle_fingerprint.restype = ctypes.c_uint64
+        L.sfk_oracle_sa_naive.argtypes = [U8P, ctypes.c_uint64, U64P]
+        L.sfk_oracle_sa_naive.restype = None
+        L.sfk_oracle_sa_doubling.argtypes = [U8P, ctypes.c_uint64, U64P]
+        L.sfk_oracle_sa_doubling.restype = ctypes.c_int
+        L.sfk_oracle_isa.argtypes = [U64P, ctypes.c_uint64, ctypes.c_uint32, U64P]
+        L.sfk_oracle_isa.restype = None
+        L.sfk_oracle_lcp_kasai.argtypes = [U8P, ctypes.c_uint64, U64P, U64P, ctypes.c_uint64,
+                                           ctypes.c_uint32, U64P]
+        L.sfk_oracle_lcp_kasai.restype = None
+        L.sfk_oracle_lcp_brute.argtypes = [U8P, ctypes.c_uint64, U64P, ctypes.c_uint64, U64P]
+        L.sfk_oracle_lcp_brute.restype = None
+        L.sfk_oracle_crc32.argtypes = [ctypes.c_uint32, U8P, ctypes.c_uint64]
+        L.sfk_oracle_crc32.restype = ctypes.c_uint32
+        L.sfk_oracle_digest_u64.argtypes = [U64P, ctypes.c_uint64]
+        L.sfk_oracle_digest_u64.restype = ctypes.c_uint64
+        _LIB = L
+    return _LIB
+
+
+def ref_available() -> bool:
+    return (HERE / "_ref" / "libsufkit_ref.so").exists()
+
+
+def ref() -> ctypes.CDLL:
+    global _REF
+    if _REF is None:
+        path = HERE / "_ref" / "libsufkit_ref.so"
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (reference sources absent at build time)")
+        R = ctypes.CDLL(str(path))
+        R.sfk_ref_build_fast.argtypes = [U8P, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
+                                         ctypes.c_uint32, U64P, U64P, U64P,
+                                         ctypes.POINTER(ctypes.c_double)]
+        R.sfk_ref_build_fast.restype = ctypes.c_longlong
+        R.sfk_ref_divsufsort.argtypes = [U8P, ctypes.c_uint64, ctypes.c_int, U64P]
+        R.sfk_ref_divsufsort.restype = ctypes.c_int
+        _REF = R
+    return _REF
+
+
+def _u8(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(U8P)
+
+
+def _u64(a):
+    return a.ctypes.data_as(U64P)
+
+
+def encode_records(sequences: list[bytes]) -> tuple[np.ndarray, int]:
+    """Logical text of FromRecords + sentinel; returns (text, ambiguous)."""
+    raw = np.frombuffer(b"".join(sequences), dtype=np.uint8)
+    raw, raw_p = _u8(raw if raw.size else np.zeros(1, np.uint8))
+    lengths = np.array([len(s) for s in sequences], dtype=np.uint64)
+    out = np.empty(int(lengths.sum()) + len(sequences) + 1, dtype=np.uint8)
+    amb = ctypes.c_uint64(0)
+    n = lib().sfk_oracle_encode_records(raw_p, _u64(lengths), len(sequences),
+                                        out.ctypes.data_as(U8P), ctypes.byref(amb))
+    assert n == out.size
+    return out, int(amb.value)
+
+
+def fingerprint(data: np.ndarray) -> int:
+    data, p = _u8(data)
+    return int(lib().sfk_oracle_fingerprint(p, data.size))
+
+
+def sa_naive(text: np.ndarray) -> np.ndarray:
+    text, p = _u8(text)
+    sa = np.empty(text.size, dtype=np.uint64)
+    lib().sfk_oracle_sa_naive(p, text.size, _u64(sa))
+    return sa
+
+
+def sa_doubling(text: np.ndarray) -> np.ndarray:
+    text, p = _u8(text)
+    sa = np.empty(text.size, dtype=np.uint64)
+    if lib().sfk_oracle_sa_doubling(p, text.size, _u64(sa)) != 0:
+        raise MemoryError("oracle doubling allocation failed")
+    return sa
+
+
+def isa(sa: np.ndarray, sampling_rate: int = 1) -> np.ndarray:
+    sa = np.ascontiguousarray(sa, dtype=np.uint64)
+    out = np.empty(sa.size, dtype=np.uint64)
+    lib().sfk_oracle_isa(_u64(sa), sa.size, sampling_rate, _u64(out))
+    return out
+
+
+def lcp_kasai(text: np.ndarray, sa: np.ndarray, isa_: np.ndarray, sampling_rate: int = 1) -> np.ndarray:
+    text, p = _u8(text)
+    sa = np.ascontiguousarray(sa, dtype=np.uint64)
+    isa_ = np.ascontiguousarray(isa_, dtype=np.uint64)
+    out = np.empty(sa.size, dtype=np.uint64)
+    lib().sfk_oracle_lcp_kasai(p, text.size, _u64(sa), _u64(isa_), sa.size, sampling_rate, _u64(out))
+    return out
+
+
+def lcp_brute(text: np.ndarray, sa: np.ndarray) -> np.ndarray:
+    text, p = _u8(text)
+    sa = np.ascontiguousarray(sa, dtype=np.uint64)
+    out = np.empty(sa.size, dtype=np.uint64)
+    lib().sfk_oracle_lcp_brute(p, text.size, _u64(sa), sa.size, _u64(out))
+    return out
+
+
+def crc32(data: bytes | np.ndarray, crc: int = 0) -> int:
+    arr = np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray)) else data
+    arr, p = _u8(arr if arr.size else np.zeros(1, np.uint8))
+    size = len(data) if isinstance(data, (bytes, bytearray)) else data.size
+    return int(lib().sfk_oracle_crc32(crc, p, size))
+
+
+def digest(values: np.ndarray) -> int:
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    return int(lib().sfk_oracle_digest_u64(_u64(values), values.size))
+
+
+def fast_build(text: np.ndarray, method: str = "auto"):
+    """SA, ISA, LCP (uint64) of the Fast profile.
+
+    method "ref32"/"ref64": the compiled reference (BuildDivsufsort32/64);
+    "doubling"/"naive": the restatement; "auto": reference if built, else
+    doubling.
+    """
+    text, p = _u8(text)
+    n = text.size
+    if method == "auto":
+        method = ("ref32" if n <= 0x7FFFFFFF else "ref64") if ref_available() else "doubling"
+    if method in ("ref32", "ref64"):
+        sa = np.empty(n, np.uint64)
+        isa_ = np.empty(n, np.uint64)
+        lcp = np.empty(n, np.uint64)
+        secs = (ctypes.c_double * 3)()
+        rows = ref().sfk_ref_build_fast(p, n, 32 if method == "ref32" else 64, 1, 1,
+                                        _u64(sa), _u64(isa_), _u64(lcp), secs)
+        if rows != n:
+            raise RuntimeError("reference build failed")
+        return sa, isa_, lcp
+    sa = sa_naive(text) if method == "naive" else sa_doubling(text)
+    isa_ = isa(sa)
+    return sa, isa_, lcp_kasai(text, sa, isa_)

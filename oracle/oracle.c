@@ -1,0 +1,244 @@
+/*
+ * CPU oracle for the suffix-index construction path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (sufkit_amd/) links,
+ * loads or calls this file.  It is the checker that tests/, the smoke entry
+ * point and the CPU-baseline leg of bench.py compare the HIP backend against.
+ *
+ * Every function restates the reference algorithm it names, in plain C:
+ *
+ *   sfk_oracle_encode_records  genome_reference.cpp:30-46 (encode table),
+ *                              :131-160 (separator 1 after EVERY contig),
+ *                              suffix_array.cpp:4107-4117 (sentinel 0 appended)
+ *   sfk_oracle_fingerprint     reference_data.hpp:46-54 (FNV-1a-64)
+ *   sfk_oracle_sa_naive        the unique suffix order of text (byte compare),
+ *                              as produced by divsufsort() (divsufsort.c:332)
+ *   sfk_oracle_sa_doubling     same output, O(n log^2 n) rank doubling so
+ *                              medium texts finish in seconds
+ *   sfk_oracle_isa             divsufsort_backend.cpp:81-119 and
+ *                              suffix_array.cpp:977-1022 (isa[SA[r]/K] = r)
+ *   sfk_oracle_lcp_kasai       divsufsort_backend.cpp:121-150 (generalised
+ *                              Kasai with carry common -= K, byte equality,
+ *                              LCP[0] = 0)
+ *   sfk_oracle_crc32           zlib crc32 (IEEE 802.3, reflected 0xEDB88320)
+ *                              as used by serialization.cpp:60-141
+ *
+ * Pinning (what makes each function trustworthy, checked in tests/):
+ *   - naive SA, doubling SA: the reference's own golden vectors decoded
+ *     from tests/data/legacy-sa-v1.{1,3}.fixture, each other on random and
+ *     run-heavy texts, and the compiled reference (oracle/_ref) where built;
+ *   - ISA: closed form (inverse permutation, checked both ways);
+ *   - Kasai: brute-force adjacent compare on small texts + fixture LCPs;
+ *   - CRC32: the standard check value crc32("123456789") = 0xCBF43926 and
+ *     the header/section CRCs stored in the legacy fixtures;
+ *   - FNV-1a-64: the fixture fingerprint 0x0e8d89af19be03e1.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SFK_SENTINEL 0u
+#define SFK_SEPARATOR 1u
+#define SFK_A 2u
+#define SFK_C 3u
+#define SFK_G 4u
+#define SFK_T 5u
+#define SFK_N 6u
+
+static uint8_t encode_base(uint8_t raw) {
+  switch (raw) {
+    case 'A': case 'a': return SFK_A;
+    case 'C': case 'c': return SFK_C;
+    case 'G': case 'g': return SFK_G;
+    case 'T': case 't': return SFK_T;
+    default: return SFK_N;
+  }
+}
+
+/* Encode `count` raw records laid out back to back in `raw` with lengths in
+ * `lengths`.  Writes sum(lengths) + count + 1 bytes (the sentinel included)
+ * to `out` and returns that length.  `ambiguous` (may be NULL) receives the
+ * number of bases normalised to N. */
+uint64_t sfk_oracle_encode_records(const uint8_t* raw, const uint64_t* lengths,
+                                   uint64_t count, uint8_t* out,
+                                   uint64_t* ambiguous) {
+  uint64_t in = 0, o = 0, amb = 0;
+  for (uint64_t r = 0; r < count; ++r) {
+    for (uint64_t k = 0; k < lengths[r]; ++k) {
+      uint8_t v = encode_base(raw[in++]);
+      amb += (v == SFK_N);
+      out[o++] = v;
+    }
+    out[o++] = SFK_SEPARATOR;
+  }
+  out[o++] = SFK_SENTINEL;
+  if (ambiguous) *ambiguous = amb;
+  return o;
+}
+
+uint64_t sfk_oracle_fingerprint(const uint8_t* data, uint64_t size) {
+  uint64_t hash = 14695981039346656037ULL;
+  for (uint64_t i = 0; i < size; ++i) {
+    hash ^= data[i];
+    hash *= 1099511628211ULL;
+  }
+  return hash;
+}
+
+/* ---------------------------------------------------------------- SA ---- */
+
+static const uint8_t* g_text;
+static uint64_t g_n;
+
+static int cmp_suffix(const void* a, const void* b) {
+  uint64_t i = *(const uint64_t*)a, j = *(const uint64_t*)b;
+  while (i < g_n && j < g_n) {
+    if (g_text[i] != g_text[j]) return g_text[i] < g_text[j] ? -1 : 1;
+    ++i;
+    ++j;
+  }
+  /* unreachable for a text with a unique terminal sentinel */
+  return i < j ? 1 : (i > j ? -1 : 0);
+}
+
+/* Brute-force suffix sort: qsort of suffix start positions under byte-wise
+ * lexicographic comparison.  Quadratic on repetitive text; use for tiny
+ * inputs only. */
+void sfk_oracle_sa_naive(const uint8_t* text, uint64_t n, uint64_t* sa) {
+  for (uint64_t i = 0; i < n; ++i) sa[i] = i;
+  g_text = text;
+  g_n = n;
+  qsort(sa, n, sizeof(uint64_t), cmp_suffix);
+}
+
+typedef struct {
+  uint64_t key0, key1, idx;
+} triple_t;
+
+static int cmp_triple(const void* a, const void* b) {
+  const triple_t* x = (const triple_t*)a;
+  const triple_t* y = (const triple_t*)b;
+  if (x->key0 != y->key0) return x->key0 < y->key0 ? -1 : 1;
+  if (x->key1 != y->key1) return x->key1 < y->key1 ? -1 : 1;
+  return 0;
+}
+
+/* Manber-Myers rank doubling with comparison sorts: rank_h(i) orders the
+ * h-prefixes; (rank_h(i), rank_h(i+h)) orders the 2h-prefixes.  Positions
+ * past the end rank below everything (they never decide the order here
+ * because the terminal sentinel is unique).  Returns 0 on success, -1 on
+ * allocation failure. */
+int sfk_oracle_sa_doubling(const uint8_t* text, uint64_t n, uint64_t* sa) {
+  if (n == 0) return 0;
+  triple_t* t = (triple_t*)malloc(n * sizeof(triple_t));
+  uint64_t* rank = (uint64_t*)malloc(n * sizeof(uint64_t));
+  if (!t || !rank) {
+    free(t);
+    free(rank);
+    return -1;
+  }
+  for (uint64_t i = 0; i < n; ++i) rank[i] = (uint64_t)text[i] + 1;
+  for (uint64_t h = 1;; h <<= 1) {
+    for (uint64_t i = 0; i < n; ++i) {
+      t[i].key0 = rank[i];
+      t[i].key1 = (i + h < n) ? rank[i + h] : 0;
+      t[i].idx = i;
+    }
+    qsort(t, n, sizeof(triple_t), cmp_triple);
+    uint64_t groups = 1;
+    rank[t[0].idx] = 1;
+    for (uint64_t r = 1; r < n; ++r) {
+      if (t[r].key0 != t[r - 1].key0 || t[r].key1 != t[r - 1].key1) ++groups;
+      rank[t[r].idx] = groups;
+    }
+    if (groups == n || h >= n) break;
+  }
+  for (uint64_t r = 0; r < n; ++r) sa[r] = t[r].idx;
+  free(t);
+  free(rank);
+  return 0;
+}
+
+/* ISA of a (possibly sampled) SA: isa[SA[r] / K] = r. */
+void sfk_oracle_isa(const uint64_t* sa, uint64_t count, uint32_t sampling_rate,
+                    uint64_t* isa) {
+  for (uint64_t r = 0; r < count; ++r) isa[sa[r] / sampling_rate] = r;
+}
+
+/* Generalised Kasai over sampled positions K apart, byte equality, exactly
+ * as divsufsort_backend.cpp:121-150: LCP[row] = lcp(SA[row-1], SA[row]),
+ * LCP[0] = 0, carry common -= K (floored at 0) between samples. */
+void sfk_oracle_lcp_kasai(const uint8_t* text, uint64_t n, const uint64_t* sa,
+                          const uint64_t* isa, uint64_t count,
+                          uint32_t sampling_rate, uint64_t* lcp) {
+  uint64_t common = 0;
+  memset(lcp, 0, count * sizeof(uint64_t));
+  for (uint64_t sample = 0; sample < count; ++sample) {
+    uint64_t suffix = sample * sampling_rate;
+    uint64_t row = isa[sample];
+    if (row != 0) {
+      uint64_t previous = sa[row - 1];
+      if (suffix + common < n && previous + common < n) {
+        uint64_t a = suffix + common, b = previous + common;
+        uint64_t remaining = (n - a) < (n - b) ? (n - a) : (n - b);
+        uint64_t k = 0;
+        while (k < remaining && text[a + k] == text[b + k]) ++k;
+        common += k;
+      }
+      lcp[row] = common;
+    }
+    common = common > sampling_rate ? common - sampling_rate : 0;
+  }
+}
+
+/* Brute-force LCP: compare every adjacent SA pair from scratch. */
+void sfk_oracle_lcp_brute(const uint8_t* text, uint64_t n, const uint64_t* sa,
+                          uint64_t count, uint64_t* lcp) {
+  for (uint64_t r = 0; r < count; ++r) {
+    if (r == 0) {
+      lcp[0] = 0;
+      continue;
+    }
+    uint64_t a = sa[r - 1], b = sa[r], k = 0;
+    while (a + k < n && b + k < n && text[a + k] == text[b + k]) ++k;
+    lcp[r] = k;
+  }
+}
+
+/* --------------------------------------------------------------- CRC ---- */
+
+static uint32_t g_crc_table[256];
+static int g_crc_ready;
+
+static void crc_init(void) {
+  for (uint32_t i = 0; i < 256; ++i) {
+    uint32_t c = i;
+    for (int k = 0; k < 8; ++k) c = (c & 1) ? (0xEDB88320u ^ (c >> 1)) : (c >> 1);
+    g_crc_table[i] = c;
+  }
+  g_crc_ready = 1;
+}
+
+/* zlib-compatible running CRC32: crc32(crc32(0, a), b) == crc32(0, a||b). */
+uint32_t sfk_oracle_crc32(uint32_t crc, const uint8_t* data, uint64_t size) {
+  if (!g_crc_ready) crc_init();
+  crc = ~crc;
+  for (uint64_t i = 0; i < size; ++i)
+    crc = g_crc_table[(crc ^ data[i]) & 0xFFu] ^ (crc >> 8);
+  return ~crc;
+}
+
+/* Order-sensitive 64-bit digest of an integer array (splitmix64 finaliser
+ * over value and index), used to compare large SA/ISA/LCP arrays without
+ * shipping them. */
+uint64_t sfk_oracle_digest_u64(const uint64_t* values, uint64_t count) {
+  uint64_t h = 0x9E3779B97F4A7C15ULL ^ count;
+  for (uint64_t i = 0; i < count; ++i) {
+    uint64_t z = values[i] + 0x9E3779B97F4A7C15ULL * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    h = (h ^ z) * 0x100000001B3ULL + (h >> 29);
+  }
+  return h;
+}

@@ -1,0 +1,203 @@
+"""sufkit_amd: MI355X-native suffix-index construction (SA + ISA + raw LCP).
+
+The product is native code: the HIP/gfx950 backend behind the C ABI in
+include/sufkit_hip.h (sufkit_amd/lib/libsufkit_hip.so) and the C++17 host
+library around it (sufkit_amd/lib/libsufkit.so).  This module only loads
+them with ctypes and converts numpy buffers; it has no fallback: when the
+native library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_DIR = PKG / "lib"
+
+SFK_HIP_RETAIN_ISA = 1
+SFK_HIP_RETAIN_LCP = 2
+SFK_HIP_PROFILE_KERNELS = 4
+
+# Reference ErrorCode + 1 (include/sufkit/types.hpp:708-722).
+STATUS_NAMES = {
+    0: "ok",
+    1: "invalid_input",
+    2: "io_error",
+    3: "unsupported_backend",
+    4: "corrupt_index",
+    5: "version_mismatch",
+    6: "build_failure",
+}
+
+
+class SufkitError(RuntimeError):
+    """Raised for a non-zero status; `code` is the reference ErrorCode name."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.code = STATUS_NAMES.get(status, str(status))
+
+
+class HipStats(ctypes.Structure):
+    _fields_ = [
+        ("sa_seconds", ctypes.c_double),
+        ("isa_seconds", ctypes.c_double),
+        ("lcp_seconds", ctypes.c_double),
+        ("device_seconds", ctypes.c_double),
+        ("h2d_seconds", ctypes.c_double),
+        ("d2h_seconds", ctypes.c_double),
+        ("refine_steps", ctypes.c_uint64),
+        ("doubling_rounds", ctypes.c_uint64),
+        ("active_total", ctypes.c_uint64),
+        ("sort_passes", ctypes.c_uint64),
+        ("lce_long_items", ctypes.c_uint64),
+        ("lce_long_words", ctypes.c_uint64),
+        ("algorithmic_bytes_sa", ctypes.c_uint64),
+        ("algorithmic_bytes_isa", ctypes.c_uint64),
+        ("algorithmic_bytes_lcp", ctypes.c_uint64),
+        ("sort_scatter_seconds", ctypes.c_double),
+        ("sort_scatter_launches", ctypes.c_uint64),
+        ("sort_scatter_bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_HIP = None
+
+
+def hip_library_path() -> Path:
+    return LIB_DIR / "libsufkit_hip.so"
+
+
+def load_hip() -> ctypes.CDLL:
+    """Load libsufkit_hip.so (built by sufkit_amd.build); raise if absent."""
+    global _HIP
+    if _HIP is not None:
+        return _HIP
+    path = hip_library_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m sufkit_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(str(path))
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    vp = ctypes.c_void_p
+    lib.sfk_hip_abi_version.restype = ctypes.c_int
+    lib.sfk_hip_device_count.restype = ctypes.c_int
+    lib.sfk_hip_builder_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_builder_create.restype = ctypes.c_int
+    lib.sfk_hip_builder_destroy.argtypes = [vp]
+    lib.sfk_hip_builder_destroy.restype = None
+    lib.sfk_hip_builder_stream.argtypes = [vp]
+    lib.sfk_hip_builder_stream.restype = vp
+    lib.sfk_hip_workspace_bytes.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+    lib.sfk_hip_workspace_bytes.restype = ctypes.c_uint64
+    lib.sfk_hip_build_device.argtypes = [
+        vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+        ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_build_device.restype = ctypes.c_int
+    lib.sfk_hip_build.argtypes = [
+        u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+        ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_build.restype = ctypes.c_int
+    lib.sfk_hip_malloc.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
+    lib.sfk_hip_malloc.restype = ctypes.c_int
+    lib.sfk_hip_free.argtypes = [vp]
+    lib.sfk_hip_free.restype = ctypes.c_int
+    lib.sfk_hip_memcpy_h2d.argtypes = [vp, vp, ctypes.c_uint64]
+    lib.sfk_hip_memcpy_h2d.restype = ctypes.c_int
+    lib.sfk_hip_memcpy_d2h.argtypes = [vp, vp, ctypes.c_uint64]
+    lib.sfk_hip_memcpy_d2h.restype = ctypes.c_int
+    _HIP = lib
+    return lib
+
+
+def _dtype(bits: int):
+    return np.uint32 if bits == 32 else np.uint64
+
+
+def hip_build(text: np.ndarray, construction_bits: int = 32, *, sa_bits: int | None = None,
+              isa_bits: int | None = None, lcp_bits: int | None = None,
+              retain_isa: bool = True, retain_lcp: bool = True, profile: bool = False):
+    """Host-buffer build through sfk_hip_build.
+
+    `text` is the logical text (encoded bytes 0..6 with the sentinel 0 last).
+    Returns (sa, isa or None, lcp or None, stats dict).
+    """
+    lib = load_hip()
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    n = int(text.size)
+    narrow = n - 1 <= 0xFFFFFFFF
+    sa_bits = sa_bits or (32 if narrow else 64)
+    isa_bits = isa_bits or (32 if narrow else 64)
+    lcp_bits = lcp_bits or (32 if narrow else 64)
+    sa = np.empty(n, dtype=_dtype(sa_bits))
+    isa = np.empty(n, dtype=_dtype(isa_bits)) if retain_isa else None
+    lcp = np.empty(n, dtype=_dtype(lcp_bits)) if retain_lcp else None
+    flags = (SFK_HIP_RETAIN_ISA if retain_isa else 0) | (SFK_HIP_RETAIN_LCP if retain_lcp else 0)
+    if profile:
+        flags |= SFK_HIP_PROFILE_KERNELS
+    stats = HipStats()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_build(
+        text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits, flags,
+        sa.ctypes.data, sa_bits,
+        isa.ctypes.data if isa is not None else None, isa_bits,
+        lcp.ctypes.data if lcp is not None else None, lcp_bits,
+        ctypes.byref(stats), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+    return sa, isa, lcp, stats.as_dict()
+
+
+class DeviceBuilder:
+    """A builder handle (one HIP stream + grow-only workspace)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_hip()
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        status = self.lib.sfk_hip_builder_create(device, ctypes.byref(handle), err, len(err))
+        if status != 0:
+            raise SufkitError(status, err.value.decode(errors="replace"))
+        self.handle = handle
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.sfk_hip_builder_stream(self.handle) or 0)
+
+    def build(self, d_text: int, n: int, construction_bits: int, d_sa: int, sa_bits: int,
+              d_isa: int | None, isa_bits: int, d_lcp: int | None, lcp_bits: int,
+              flags: int = SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP) -> dict:
+        stats = HipStats()
+        err = ctypes.create_string_buffer(512)
+        status = self.lib.sfk_hip_build_device(
+            self.handle, d_text, n, construction_bits, flags, d_sa, sa_bits,
+            d_isa, isa_bits, d_lcp, lcp_bits, ctypes.byref(stats), err, len(err))
+        if status != 0:
+            raise SufkitError(status, err.value.decode(errors="replace"))
+        return stats.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sfk_hip_builder_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,168 @@
+// Device-side building blocks shared by every suffix-index kernel (gfx950).
+//
+// Text layout in HBM (one copy per build):
+//   codes[w]  uint64, symbols 32w..32w+31, symbol 32w+j in bits [63-2j, 62-2j]
+//             (most significant first, so integer order = lexicographic order)
+//   spec[w]   uint32, bit (31-j) set when symbol 32w+j is not a base
+// A base A/C/G/T is code 0..3 with its spec bit clear.  A special symbol has
+// its spec bit set and its kind in the code slot: 0 sentinel, 1 separator,
+// 3 N.  (code, spec) is therefore a lossless encoding of the 7-symbol
+// alphabet of the encoded text: 0 sentinel < 1 separator < A < C < G < T < N.
+// Two zero padding words follow the last symbol so that any 32-symbol window
+// starting at a valid position can be read with two word loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sfk {
+
+constexpr int kWave = 64;
+
+struct PackedText {
+  const std::uint64_t* codes;
+  const std::uint32_t* spec;
+  std::uint64_t n;
+};
+
+// Number of 32-symbol words for a text of n symbols, padding included.
+__host__ __device__ inline std::uint64_t PackedWords(std::uint64_t n) {
+  return (n + 31) / 32 + 2;
+}
+
+// The 32-symbol window starting at position i: 2-bit codes and spec mask.
+__device__ inline void LoadWindow(const PackedText& t, std::uint64_t i,
+                                  std::uint64_t& codes, std::uint32_t& spec) {
+  const std::uint64_t w = i >> 5;
+  const std::uint32_t o = static_cast<std::uint32_t>(i & 31);
+  const std::uint64_t c0 = t.codes[w];
+  const std::uint64_t c1 = t.codes[w + 1];
+  codes = o ? (c0 << (2 * o)) | (c1 >> (64 - 2 * o)) : c0;
+  const std::uint64_t s01 =
+      (static_cast<std::uint64_t>(t.spec[w]) << 32) | t.spec[w + 1];
+  spec = static_cast<std::uint32_t>((s01 << o) >> 32);
+}
+
+// Code and spec bit of the single symbol at position p.
+__device__ inline std::uint32_t SymbolAt(const PackedText& t, std::uint64_t p) {
+  const std::uint64_t w = p >> 5;
+  const std::uint32_t o = static_cast<std::uint32_t>(p & 31);
+  const std::uint32_t code =
+      static_cast<std::uint32_t>(t.codes[w] >> (62 - 2 * o)) & 3u;
+  const std::uint32_t sp = (t.spec[w] >> (31 - o)) & 1u;
+  return code | (sp << 2);
+}
+
+// 64-bit key of a 32-symbol window: the 2-bit codes, exact when the window
+// holds only bases.  From the first special symbol on, the key is filled with
+// zeros (sentinel, separator: below every base) or ones (N: above every
+// base).  The map is monotone: window x < window y implies key(x) <= key(y).
+// Equal keys are exact ties unless the group holds a window with a special.
+__device__ inline std::uint64_t FillKey(std::uint64_t codes, std::uint32_t spec) {
+  if (spec == 0) return codes;
+  const int p = __clz(spec);  // first special symbol in the window
+  const std::uint32_t kind =
+      static_cast<std::uint32_t>(codes >> (62 - 2 * p)) & 3u;
+  const std::uint64_t tail = ~0ull >> (2 * p);
+  return kind == 3u ? (codes | tail) : (codes & ~tail);
+}
+
+// Exact 96-bit key of a 32-symbol window, 3 bits per symbol with the
+// 7-symbol order (0 sentinel, 1 separator, 2..5 bases, 6 N), returned as the
+// big-endian 32-bit word `word` (0, 1 or 2).
+__device__ inline std::uint32_t ExactKeyWord(std::uint64_t codes,
+                                             std::uint32_t spec, int word) {
+  std::uint64_t hi = 0;
+  std::uint32_t lo = 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const std::uint32_t c = static_cast<std::uint32_t>(codes >> (62 - 2 * j)) & 3u;
+    const bool special = (spec >> (31 - j)) & 1u;
+    const std::uint32_t v = special ? (c == 3u ? 6u : c) : c + 2u;
+    hi = (hi << 3) | (lo >> 29);
+    lo = (lo << 3) | v;
+  }
+  if (word == 0) return static_cast<std::uint32_t>(hi >> 32);
+  if (word == 1) return static_cast<std::uint32_t>(hi);
+  return lo;
+}
+
+// Length of the common prefix of two 32-symbol windows (32 if equal).
+__device__ inline std::uint32_t WindowMismatch(std::uint64_t ca, std::uint32_t sa,
+                                               std::uint64_t cb, std::uint32_t sb) {
+  const std::uint64_t xc = ca ^ cb;
+  const std::uint32_t xs = sa ^ sb;
+  const std::uint32_t mc = xc ? static_cast<std::uint32_t>(__clzll(xc)) >> 1 : 32u;
+  const std::uint32_t ms = xs ? static_cast<std::uint32_t>(__clz(xs)) : 32u;
+  return mc < ms ? mc : ms;
+}
+
+__device__ inline std::uint32_t LaneId() { return __lane_id(); }
+
+__device__ inline std::uint64_t LaneMaskLt() {
+  return (1ull << LaneId()) - 1ull;
+}
+
+// Lanes of this wave whose `bits`-bit digit equals this lane's, among lanes
+// with `valid` set.  Every lane of the wave must call this together.
+template <int kBits>
+__device__ inline std::uint64_t MatchDigit(std::uint32_t digit, bool valid) {
+  std::uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kBits; ++b) {
+    const bool bit = (digit >> b) & 1u;
+    const std::uint64_t vote = __ballot(bit);
+    peers &= bit ? vote : ~vote;
+  }
+  return peers;
+}
+
+template <typename T>
+__device__ inline T WaveShfl(T value, int src) {
+  return __shfl(value, src, kWave);
+}
+
+__device__ inline std::uint64_t WaveShfl(std::uint64_t value, int src) {
+  const std::uint32_t lo = __shfl(static_cast<std::uint32_t>(value), src, kWave);
+  const std::uint32_t hi =
+      __shfl(static_cast<std::uint32_t>(value >> 32), src, kWave);
+  return (static_cast<std::uint64_t>(hi) << 32) | lo;
+}
+
+// Inclusive scan across the 64 lanes of a wave (sum).
+template <typename T>
+__device__ inline T WaveInclusiveSum(T value) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const T other = __shfl_up(value, d, kWave);
+    if (static_cast<int>(LaneId()) >= d) value += other;
+  }
+  return value;
+}
+
+__device__ inline std::uint64_t WaveInclusiveSum(std::uint64_t value) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const std::uint32_t lo = __shfl_up(static_cast<std::uint32_t>(value), d, kWave);
+    const std::uint32_t hi =
+        __shfl_up(static_cast<std::uint32_t>(value >> 32), d, kWave);
+    const std::uint64_t other = (static_cast<std::uint64_t>(hi) << 32) | lo;
+    if (static_cast<int>(LaneId()) >= d) value += other;
+  }
+  return value;
+}
+
+__device__ inline std::uint64_t WaveInclusiveMax(std::uint64_t value) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const std::uint32_t lo = __shfl_up(static_cast<std::uint32_t>(value), d, kWave);
+    const std::uint32_t hi =
+        __shfl_up(static_cast<std::uint32_t>(value >> 32), d, kWave);
+    const std::uint64_t other = (static_cast<std::uint64_t>(hi) << 32) | lo;
+    if (static_cast<int>(LaneId()) >= d && other > value) value = other;
+  }
+  return value;
+}
+
+}  // namespace sfk

@@ -1,0 +1,734 @@
+// MI355X suffix-index construction backend: host orchestration + C ABI.
+//
+// Build pipeline for one text of n symbols (encoded bytes + sentinel):
+//   K0  PackTextKernel           2-bit codes + special bitmap
+//   round 0 (exact 32-symbol ranks from 64-bit packed keys):
+//       refine by FillKey bits 63..32 (all n suffixes)
+//       refine by FillKey bits 31..0  (suffixes tied on the high half)
+//       refine by the exact 96-bit key, 32 bits at a time, only when a tied
+//       group holds a window with a separator, N or the sentinel
+//   rounds h = 32, 64, 128, ...: refine by rank[suffix + h]
+//   K9  ValidateKernel           ISA[SA[r]] == r (rank array == ISA)
+//   K6-K8 Φ/PLCP/LCP             irreducible LCEs + max-scan + permutation
+// Every refine step is one stable LSD radix sort of (segment, secondary) keys
+// over the still-active suffixes, then boundary detection and compaction
+// (sa_kernels.hpp).  The host reads back three counters per step to size the
+// next one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "radix_sort.hpp"
+#include "sa_kernels.hpp"
+#include "sufkit_hip.h"
+
+namespace sfk {
+namespace {
+
+struct Failure {
+  int status;
+  std::string message;
+};
+
+[[noreturn]] void Fail(int status, const std::string& message) {
+  throw Failure{status, message};
+}
+
+#define SFK_HIP_CHECK(expr)                                                  \
+  do {                                                                       \
+    const hipError_t sfk_err_ = (expr);                                      \
+    if (sfk_err_ != hipSuccess) {                                            \
+      ::sfk::Fail(SFK_HIP_BUILD_FAILURE,                                     \
+                  std::string(#expr) + ": " + hipGetErrorString(sfk_err_));  \
+    }                                                                        \
+  } while (0)
+
+int BitsFor(std::uint64_t max_value) {
+  int bits = 0;
+  while (bits < 64 && (max_value >> bits) != 0) ++bits;
+  return bits;
+}
+
+std::uint64_t CeilDiv(std::uint64_t a, std::uint64_t b) { return (a + b - 1) / b; }
+
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { Release(); }
+
+  void Reserve(std::uint64_t bytes) {
+    if (bytes <= bytes_) return;
+    Release();
+    void* ptr = nullptr;
+    const hipError_t err = hipMalloc(&ptr, bytes);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      Fail(SFK_HIP_BUILD_FAILURE, "device allocation of " + std::to_string(bytes) +
+                                      " bytes failed: " + hipGetErrorString(err));
+    }
+    ptr_ = ptr;
+    bytes_ = bytes;
+  }
+  void Release() {
+    if (ptr_ != nullptr) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template <typename T>
+  T* As() const {
+    return static_cast<T*>(ptr_);
+  }
+  std::uint64_t Bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  std::uint64_t bytes_ = 0;
+};
+
+struct EventPair {
+  hipEvent_t begin = nullptr;
+  hipEvent_t end = nullptr;
+  std::uint64_t bytes = 0;
+};
+
+}  // namespace
+}  // namespace sfk
+
+struct sfk_hip_builder {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sfk::DeviceBuffer codes, spec, rank, sa, act_idx, act_seg, keys0, keys1,
+      vals0, vals1, seg_head0, seg_head1, seg_start0, seg_start1, radix_counts,
+      partial, counters, narrow;
+  sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
+  std::vector<sfk::EventPair> events;             // per-kernel profiling
+  std::size_t events_used = 0;
+  hipEvent_t phase[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int radix_grid = 512;
+};
+
+namespace sfk {
+namespace {
+
+constexpr int kRefineItems = 8;
+constexpr int kScanItems = 8;
+
+struct BuildContext {
+  sfk_hip_builder* b;
+  std::uint64_t n;
+  bool profile;
+  sfk_hip_stats stats;
+};
+
+void ResetCounters(sfk_hip_builder* b) {
+  DeviceCounters init{};
+  init.key_or = 0;
+  init.key_and = ~0ull;
+  *b->host_counters = init;
+  SFK_HIP_CHECK(hipMemcpyAsync(b->counters.As<DeviceCounters>(), b->host_counters,
+                               sizeof(DeviceCounters), hipMemcpyHostToDevice, b->stream));
+}
+
+// Reset only the per-step key reduction fields, keeping error flags.
+__global__ void ResetKeyCountersKernel(DeviceCounters* c) {
+  c->key_or = 0;
+  c->key_and = ~0ull;
+  c->special_active = 0;
+}
+
+const DeviceCounters& ReadCounters(sfk_hip_builder* b) {
+  SFK_HIP_CHECK(hipMemcpyAsync(b->host_counters, b->counters.As<DeviceCounters>(),
+                               sizeof(DeviceCounters), hipMemcpyDeviceToHost, b->stream));
+  SFK_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return *b->host_counters;
+}
+
+EventPair* BeginProfiled(BuildContext& ctx, std::uint64_t bytes) {
+  if (!ctx.profile) return nullptr;
+  sfk_hip_builder* b = ctx.b;
+  if (b->events_used == b->events.size()) {
+    EventPair p;
+    SFK_HIP_CHECK(hipEventCreate(&p.begin));
+    SFK_HIP_CHECK(hipEventCreate(&p.end));
+    b->events.push_back(p);
+  }
+  EventPair* p = &b->events[b->events_used++];
+  p->bytes = bytes;
+  SFK_HIP_CHECK(hipEventRecord(p->begin, b->stream));
+  return p;
+}
+
+void EndProfiled(BuildContext& ctx, EventPair* p) {
+  if (p != nullptr) SFK_HIP_CHECK(hipEventRecord(p->end, ctx.b->stream));
+}
+
+struct SortGeometry {
+  int grid;
+  std::uint64_t chunk;
+};
+
+SortGeometry Geometry(std::uint64_t m, int tile, int max_grid) {
+  const std::uint64_t tiles = std::max<std::uint64_t>(1, CeilDiv(m, tile));
+  std::uint64_t grid = std::min<std::uint64_t>(tiles, static_cast<std::uint64_t>(max_grid));
+  const std::uint64_t per = CeilDiv(tiles, grid);
+  grid = CeilDiv(tiles, per);
+  return SortGeometry{static_cast<int>(grid), per * static_cast<std::uint64_t>(tile)};
+}
+
+// Stable LSD sort of m (key, value) pairs over the digits in which `varying`
+// has a set bit.  On return keys/vals point at the sorted buffers.
+template <typename K, typename V>
+void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
+               std::uint64_t m, std::uint64_t varying, int total_bits) {
+  sfk_hip_builder* b = ctx.b;
+  constexpr int kItems = RadixItems<K, V>::kValue;
+  constexpr int kTile = RadixTile::kBlock * kItems;
+  const SortGeometry g = Geometry(m, kTile, b->radix_grid);
+  std::uint64_t* counts = b->radix_counts.As<std::uint64_t>();
+  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) {
+    if (((varying >> shift) & (RadixTile::kRadix - 1)) == 0) continue;
+    RadixUpsweep<K, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
+        keys, m, g.chunk, shift, counts, g.grid);
+    RadixSpine<<<1, 1024, 0, b->stream>>>(counts, static_cast<std::uint32_t>(RadixTile::kRadix * g.grid));
+    EventPair* ev = BeginProfiled(ctx, 2ull * m * (sizeof(K) + sizeof(V)));
+    RadixDownsweep<K, V, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
+        keys, vals, keys_alt, vals_alt, m, g.chunk, shift, counts, g.grid);
+    EndProfiled(ctx, ev);
+    SFK_HIP_CHECK(hipGetLastError());
+    std::swap(keys, keys_alt);
+    std::swap(vals, vals_alt);
+    ctx.stats.sort_passes += 1;
+    ctx.stats.algorithmic_bytes_sa += m * sizeof(K) + 2ull * m * (sizeof(K) + sizeof(V));
+  }
+}
+
+template <typename Index>
+class Doubling {
+ public:
+  Doubling(BuildContext& ctx, Index* sa, Index* rank, PackedText text)
+      : ctx_(ctx), b_(ctx.b), n_(ctx.n), sa_(sa), rank_(rank), text_(text) {}
+
+  void Run() {
+    m_ = n_;
+    segments_ = 1;
+    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_head0.As<Index>(), 0, sizeof(Index), b_->stream));
+    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_start0.As<Index>(), 0, sizeof(Index), b_->stream));
+    Step(kSecFillHi, 32, 0, /*iota=*/true);
+    if (m_ > 0) Step(kSecFillLo, 32, 0, false);
+    if (m_ > 0) {
+      ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(b_->counters.As<DeviceCounters>());
+      CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(m_, 256)), 256, 0, b_->stream>>>(
+          b_->act_idx.As<Index>(), m_, text_, b_->counters.As<DeviceCounters>());
+      const DeviceCounters& c = ReadCounters(b_);
+      if (c.special_active > 0) {
+        Step(kSecExact0, 32, 0, false);
+        if (m_ > 0) Step(kSecExact1, 32, 0, false);
+        if (m_ > 0) Step(kSecExact2, 32, 0, false);
+      }
+    }
+    const int rank_bits = std::max(1, BitsFor(n_ - 1));
+    for (std::uint64_t h = 32; m_ > 0; h <<= 1) {
+      if (h >= n_) Fail(SFK_HIP_BUILD_FAILURE, "prefix doubling did not converge");
+      Step(kSecRank, rank_bits, h, false);
+      ctx_.stats.doubling_rounds += 1;
+    }
+  }
+
+ private:
+  void Step(int sec_kind, int sec_bits, std::uint64_t h, bool iota) {
+    const int seg_bits = segments_ > 1 ? BitsFor(segments_ - 1) : 0;
+    const int total_bits = seg_bits + sec_bits;
+    if (total_bits > 64) {
+      Fail(SFK_HIP_UNSUPPORTED_BACKEND,
+           "segment and rank bits exceed one 64-bit sort key");
+    }
+    ctx_.stats.refine_steps += 1;
+    ctx_.stats.active_total += m_;
+    if (total_bits <= 32) {
+      StepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota);
+    } else {
+      StepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota);
+    }
+  }
+
+  template <typename K>
+  void StepTyped(int sec_kind, int sec_bits, int seg_bits, std::uint64_t h, bool iota) {
+    const std::uint64_t m = m_;
+    const int total_bits = seg_bits + sec_bits;
+    K* keys = b_->keys0.As<K>();
+    K* keys_alt = b_->keys1.As<K>();
+    Index* vals = b_->vals0.As<Index>();
+    Index* vals_alt = b_->vals1.As<Index>();
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    constexpr std::uint64_t w = sizeof(Index);
+
+    ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+    KeygenKernel<Index, K><<<static_cast<unsigned>(CeilDiv(m, 256)), 256, 0, b_->stream>>>(
+        b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), m, sec_kind, sec_bits, seg_bits,
+        text_, rank_, h, iota, keys, vals, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    ctx_.stats.algorithmic_bytes_sa +=
+        m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (sec_kind == kSecRank ? w : 24) +
+             sizeof(K) + w);
+    const DeviceCounters& c = ReadCounters(b_);
+    if (c.error_flags != 0) {
+      Fail(SFK_HIP_INVALID_INPUT, c.error_flags & kFlagBadSymbol
+                                      ? "text holds a symbol outside 0..6"
+                                      : "text must end with its only 0 (sentinel)");
+    }
+    const std::uint64_t varying = (c.key_or ^ c.key_and) &
+                                  (total_bits >= 64 ? ~0ull : ((1ull << total_bits) - 1));
+    SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, varying, total_bits);
+
+    // Boundaries, singletons, compaction.
+    const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
+    RefinePartial* partial = b_->partial.As<RefinePartial>();
+    Index* head_cur = (flip_ ? b_->seg_head1 : b_->seg_head0).template As<Index>();
+    Index* start_cur = (flip_ ? b_->seg_start1 : b_->seg_start0).template As<Index>();
+    Index* head_next = (flip_ ? b_->seg_head0 : b_->seg_head1).template As<Index>();
+    Index* start_next = (flip_ ? b_->seg_start0 : b_->seg_start1).template As<Index>();
+    RefineReduceKernel<K><<<g.grid, 256, 0, b_->stream>>>(keys, m, g.chunk, partial);
+    RefineSpineKernel<<<1, 64, 0, b_->stream>>>(partial, g.grid, dc);
+    RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
+        keys, vals, m, g.chunk, sec_bits, head_cur, start_cur, partial, n_, sa_, rank_,
+        b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), head_next, start_next);
+    SFK_HIP_CHECK(hipGetLastError());
+    const DeviceCounters& after = ReadCounters(b_);
+    const std::uint64_t survivors = after.survivors;
+    ctx_.stats.algorithmic_bytes_sa +=
+        m * (2 * sizeof(K) + w) + (m - survivors) * 2 * w + survivors * 3 * w;
+    m_ = survivors;
+    segments_ = after.segments;
+    flip_ = !flip_;
+  }
+
+  BuildContext& ctx_;
+  sfk_hip_builder* b_;
+  std::uint64_t n_;
+  Index* sa_;
+  Index* rank_;
+  PackedText text_;
+  std::uint64_t m_ = 0;
+  std::uint64_t segments_ = 1;
+  bool flip_ = false;
+};
+
+template <typename Index, typename Out>
+void LaunchLcp(BuildContext& ctx, PackedText text, const Index* sa, const Index* isa,
+               Out* lcp) {
+  sfk_hip_builder* b = ctx.b;
+  const std::uint64_t n = ctx.n;
+  DeviceCounters* dc = b->counters.As<DeviceCounters>();
+  Index* val = b->keys0.As<Index>();
+  LongLce* queue = b->keys1.As<LongLce>();
+  const std::uint64_t capacity = b->keys1.Bytes() / sizeof(LongLce);
+  constexpr std::uint64_t w = sizeof(Index);
+  constexpr std::uint32_t kWordLimit = 8;
+  PlcpIrreducibleKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+      text, sa, isa, n, kWordLimit, val, queue, capacity, dc);
+  SFK_HIP_CHECK(hipGetLastError());
+  const DeviceCounters& c = ReadCounters(b);
+  const std::uint64_t items = c.long_items;
+  if (items > capacity) {
+    Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
+  }
+  if (items > 0) {
+    PlcpLongKernel<Index><<<static_cast<unsigned>(CeilDiv(items * kWave, 256)), 256, 0, b->stream>>>(
+        text, n, queue, items, val, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+  }
+  const SortGeometry g = Geometry(n, 256 * kScanItems, 1024);
+  std::uint64_t* partial = b->partial.As<std::uint64_t>();
+  MaxReduceKernel<Index><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
+  MaxSpineKernel<<<1, 64, 0, b->stream>>>(partial, g.grid);
+  MaxScanToPlcpKernel<Index, kScanItems><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
+  LcpPermuteKernel<Index, Out><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+      val, sa, n, lcp);
+  SFK_HIP_CHECK(hipGetLastError());
+  ctx.stats.algorithmic_bytes_lcp +=
+      n * (w + w + 8 + 24 + w)        // ISA, SA[r-1], two symbols, first LCE windows, val
+      + n * (3 * w)                   // max-scan: two reads, one write
+      + n * (w + w + sizeof(Out));    // SA, PLCP[SA], LCP
+}
+
+template <typename In, typename Out>
+void Convert(sfk_hip_builder* b, const In* in, std::uint64_t n, Out* out) {
+  NarrowKernel<In, Out><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+      in, n, out, b->counters.As<DeviceCounters>());
+  SFK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename In>
+void ConvertTo(sfk_hip_builder* b, const In* in, std::uint64_t n, void* out, std::uint32_t bits) {
+  if (bits == 32) {
+    Convert<In, std::uint32_t>(b, in, n, static_cast<std::uint32_t*>(out));
+  } else {
+    Convert<In, std::uint64_t>(b, in, n, static_cast<std::uint64_t*>(out));
+  }
+}
+
+template <typename Index>
+void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t flags,
+                void* d_sa, std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits,
+                void* d_lcp, std::uint32_t lcp_bits) {
+  sfk_hip_builder* b = ctx.b;
+  const std::uint64_t n = ctx.n;
+  constexpr std::uint32_t kIndexBits = 8 * sizeof(Index);
+  const std::uint64_t words = PackedWords(n);
+
+  // Workspace (grow-only).
+  b->codes.Reserve(words * 8);
+  b->spec.Reserve(words * 4);
+  const bool sa_direct = sa_bits == kIndexBits;
+  const bool isa_direct = d_isa != nullptr && isa_bits == kIndexBits;
+  if (!sa_direct) b->sa.Reserve(n * sizeof(Index));
+  if (!isa_direct) b->rank.Reserve(n * sizeof(Index));
+  b->act_idx.Reserve(n * sizeof(Index));
+  b->act_seg.Reserve(n * sizeof(Index));
+  b->keys0.Reserve(std::max<std::uint64_t>(n * 8, 64));
+  b->keys1.Reserve(std::max<std::uint64_t>(n * 8, 64));
+  b->vals0.Reserve(n * sizeof(Index));
+  b->vals1.Reserve(n * sizeof(Index));
+  const std::uint64_t seg_cap = n / 2 + 2;
+  b->seg_head0.Reserve(seg_cap * sizeof(Index));
+  b->seg_head1.Reserve(seg_cap * sizeof(Index));
+  b->seg_start0.Reserve(seg_cap * sizeof(Index));
+  b->seg_start1.Reserve(seg_cap * sizeof(Index));
+  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * 1024 * 8);
+  b->partial.Reserve(1024 * sizeof(RefinePartial));
+
+  Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
+  Index* rank = isa_direct ? static_cast<Index*>(d_isa) : b->rank.As<Index>();
+  PackedText text{b->codes.As<std::uint64_t>(), b->spec.As<std::uint32_t>(), n};
+
+  ResetCounters(b);
+  SFK_HIP_CHECK(hipEventRecord(b->phase[0], b->stream));
+  PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
+      d_text, n, words, b->codes.As<std::uint64_t>(), b->spec.As<std::uint32_t>(),
+      (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, b->counters.As<DeviceCounters>());
+  SFK_HIP_CHECK(hipGetLastError());
+  ctx.stats.algorithmic_bytes_sa += n + words * 12;
+
+  Doubling<Index>(ctx, sa, rank, text).Run();
+  SFK_HIP_CHECK(hipEventRecord(b->phase[1], b->stream));
+
+  // K9: permutation + inverse check on the device; ISA in its stored width.
+  ValidateKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+      sa, rank, n, b->counters.As<DeviceCounters>());
+  SFK_HIP_CHECK(hipGetLastError());
+  ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
+  if (d_isa != nullptr && !isa_direct) {
+    ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
+    ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + isa_bits / 8);
+  }
+  SFK_HIP_CHECK(hipEventRecord(b->phase[2], b->stream));
+  {
+    // The LCP kernels index SA and ISA with each other's values: only run
+    // them on a validated permutation/inverse pair.
+    const DeviceCounters& c = ReadCounters(b);
+    if (c.error_flags != 0) {
+      Fail(SFK_HIP_BUILD_FAILURE, "suffix array failed device validation (flags " +
+                                      std::to_string(c.error_flags) + ")");
+    }
+  }
+
+  if (d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP)) {
+    if (lcp_bits == 32) {
+      LaunchLcp<Index, std::uint32_t>(ctx, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
+    } else {
+      LaunchLcp<Index, std::uint64_t>(ctx, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
+    }
+  }
+  SFK_HIP_CHECK(hipEventRecord(b->phase[3], b->stream));
+  if (!sa_direct) {
+    ConvertTo<Index>(b, sa, n, d_sa, sa_bits);
+    ctx.stats.algorithmic_bytes_sa += n * (sizeof(Index) + sa_bits / 8);
+  }
+  SFK_HIP_CHECK(hipEventRecord(b->phase[4], b->stream));
+  const DeviceCounters& c = ReadCounters(b);
+  if (c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) {
+    Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
+  }
+  if (c.error_flags & kFlagRange) {
+    Fail(SFK_HIP_BUILD_FAILURE, "value does not fit the requested stored width");
+  }
+  ctx.stats.lce_long_items = c.long_items;
+  ctx.stats.lce_long_words = c.long_words;
+  float ms = 0.f;
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[0], b->phase[1]));
+  ctx.stats.sa_seconds = ms * 1e-3;
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[1], b->phase[2]));
+  ctx.stats.isa_seconds = ms * 1e-3;
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[2], b->phase[3]));
+  ctx.stats.lcp_seconds = ms * 1e-3;
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[0], b->phase[4]));
+  ctx.stats.device_seconds = ms * 1e-3;
+  if (ctx.profile) {
+    for (std::size_t e = 0; e < b->events_used; ++e) {
+      SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->events[e].begin, b->events[e].end));
+      ctx.stats.sort_scatter_seconds += ms * 1e-3;
+      ctx.stats.sort_scatter_launches += 1;
+      ctx.stats.sort_scatter_bytes += b->events[e].bytes;
+    }
+  }
+}
+
+void CopyError(const std::string& message, char* error, size_t capacity) {
+  if (error == nullptr || capacity == 0) return;
+  const std::size_t len = std::min(message.size(), capacity - 1);
+  std::memcpy(error, message.data(), len);
+  error[len] = '\0';
+}
+
+int ValidateArgs(std::uint64_t n, std::uint32_t construction_bits, std::uint32_t sa_bits,
+                 const void* isa, std::uint32_t isa_bits, const void* lcp,
+                 std::uint32_t lcp_bits, std::uint32_t flags, std::string& message) {
+  if (construction_bits != 32 && construction_bits != 64) {
+    message = "construction width must be 32 or 64 bits";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (n < 1) {
+    message = "reference text is empty";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (construction_bits == 32 && n > 0xFFFFFFFFull) {
+    message = "reference is too large for the 32-bit HIP backend";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (sa_bits != 32 && sa_bits != 64) {
+    message = "SA output width must be 32 or 64 bits";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (isa != nullptr && isa_bits != 32 && isa_bits != 64) {
+    message = "ISA output width must be 32 or 64 bits";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP) && lcp_bits != 32 && lcp_bits != 64) {
+    message = "LCP output width must be 32 or 64 bits";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  const std::uint64_t max_value = n - 1;
+  if ((sa_bits == 32 && max_value > 0xFFFFFFFFull) ||
+      (isa != nullptr && isa_bits == 32 && max_value > 0xFFFFFFFFull) ||
+      (lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP) && lcp_bits == 32 && max_value > 0xFFFFFFFFull)) {
+    message = "requested output width cannot represent the text coordinates";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  return SFK_HIP_OK;
+}
+
+int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t n,
+                   std::uint32_t construction_bits, std::uint32_t flags, void* d_sa,
+                   std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits, void* d_lcp,
+                   std::uint32_t lcp_bits, sfk_hip_stats* stats, std::string& message) {
+  BuildContext ctx{b, n, (flags & SFK_HIP_PROFILE_KERNELS) != 0, sfk_hip_stats{}};
+  b->events_used = 0;
+  SFK_HIP_CHECK(hipSetDevice(b->device));
+  if (construction_bits == 32) {
+    BuildTyped<std::uint32_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
+  } else {
+    BuildTyped<std::uint64_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
+  }
+  if (stats != nullptr) {
+    const double h2d = stats->h2d_seconds, d2h = stats->d2h_seconds;
+    *stats = ctx.stats;
+    stats->h2d_seconds = h2d;
+    stats->d2h_seconds = d2h;
+  }
+  (void)message;
+  return SFK_HIP_OK;
+}
+
+}  // namespace
+}  // namespace sfk
+
+extern "C" {
+
+int sfk_hip_abi_version(void) { return SFK_HIP_ABI_VERSION; }
+
+int sfk_hip_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return count;
+}
+
+int sfk_hip_builder_create(int device, sfk_hip_builder** out, char* error,
+                           size_t error_capacity) {
+  if (out == nullptr) return SFK_HIP_INVALID_INPUT;
+  *out = nullptr;
+  sfk_hip_builder* b = nullptr;
+  try {
+    if (sfk_hip_device_count() <= device || device < 0) {
+      sfk::Fail(SFK_HIP_UNSUPPORTED_BACKEND, "no HIP device " + std::to_string(device));
+    }
+    b = new sfk_hip_builder();
+    b->device = device;
+    SFK_HIP_CHECK(hipSetDevice(device));
+    SFK_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    SFK_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_counters),
+                                sizeof(sfk::DeviceCounters), hipHostMallocDefault));
+    b->counters.Reserve(sizeof(sfk::DeviceCounters));
+    for (auto& e : b->phase) SFK_HIP_CHECK(hipEventCreate(&e));
+    *out = b;
+    return SFK_HIP_OK;
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    sfk_hip_builder_destroy(b);
+    return f.status;
+  } catch (const std::bad_alloc&) {
+    sfk::CopyError("host allocation failed", error, error_capacity);
+    sfk_hip_builder_destroy(b);
+    return SFK_HIP_BUILD_FAILURE;
+  }
+}
+
+void sfk_hip_builder_destroy(sfk_hip_builder* b) {
+  if (b == nullptr) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream != nullptr) (void)hipStreamSynchronize(b->stream);
+  for (auto& e : b->events) {
+    (void)hipEventDestroy(e.begin);
+    (void)hipEventDestroy(e.end);
+  }
+  for (auto& e : b->phase) {
+    if (e != nullptr) (void)hipEventDestroy(e);
+  }
+  if (b->host_counters != nullptr) (void)hipHostFree(b->host_counters);
+  if (b->stream != nullptr) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void* sfk_hip_builder_stream(sfk_hip_builder* b) {
+  return b == nullptr ? nullptr : static_cast<void*>(b->stream);
+}
+
+uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
+  const uint64_t w = construction_bits == 64 ? 8 : 4;
+  const uint64_t words = sfk::PackedWords(n);
+  return words * 12 + n * w * 2 /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
+         (n / 2 + 2) * w * 4 /*segments*/ + n * w * 2 /*sa, rank when not direct*/ +
+         sfk::RadixTile::kRadix * 1024 * 8 + 1024 * sizeof(sfk::RefinePartial);
+}
+
+int sfk_hip_build_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,
+                         uint32_t construction_bits, uint32_t flags, void* d_sa,
+                         uint32_t sa_bits, void* d_isa, uint32_t isa_bits, void* d_lcp,
+                         uint32_t lcp_bits, sfk_hip_stats* stats, char* error,
+                         size_t error_capacity) {
+  std::string message;
+  if (b == nullptr || d_text == nullptr || d_sa == nullptr) {
+    sfk::CopyError("builder, text and SA buffers are required", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  int status = sfk::ValidateArgs(n, construction_bits, sa_bits, d_isa, isa_bits, d_lcp,
+                                 lcp_bits, flags, message);
+  if (status != SFK_HIP_OK) {
+    sfk::CopyError(message, error, error_capacity);
+    return status;
+  }
+  try {
+    return sfk::RunDeviceBuild(b, d_text, n, construction_bits, flags, d_sa, sa_bits, d_isa,
+                               isa_bits, d_lcp, lcp_bits, stats, message);
+  } catch (const sfk::Failure& f) {
+    (void)hipStreamSynchronize(b->stream);
+    sfk::CopyError(f.message, error, error_capacity);
+    return f.status;
+  } catch (const std::bad_alloc&) {
+    sfk::CopyError("host allocation failed", error, error_capacity);
+    return SFK_HIP_BUILD_FAILURE;
+  }
+}
+
+int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t flags,
+                  void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
+                  uint32_t lcp_bits, sfk_hip_stats* stats, char* error, size_t error_capacity) {
+  std::string message;
+  if (text == nullptr || sa == nullptr) {
+    sfk::CopyError("text and SA buffers are required", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  int status = sfk::ValidateArgs(n, construction_bits, sa_bits, isa, isa_bits, lcp, lcp_bits,
+                                 flags, message);
+  if (status != SFK_HIP_OK) {
+    sfk::CopyError(message, error, error_capacity);
+    return status;
+  }
+  sfk_hip_builder* b = nullptr;
+  status = sfk_hip_builder_create(0, &b, error, error_capacity);
+  if (status != SFK_HIP_OK) return status;
+  sfk::DeviceBuffer d_text, d_sa, d_isa, d_lcp;
+  const bool want_isa = isa != nullptr;
+  const bool want_lcp = lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
+  try {
+    d_text.Reserve(n + 64);
+    d_sa.Reserve(n * (sa_bits / 8));
+    if (want_isa) d_isa.Reserve(n * (isa_bits / 8));
+    if (want_lcp) d_lcp.Reserve(n * (lcp_bits / 8));
+    const auto t0 = std::chrono::steady_clock::now();
+    SFK_HIP_CHECK(hipMemcpy(d_text.As<void>(), text, n, hipMemcpyHostToDevice));
+    const auto t1 = std::chrono::steady_clock::now();
+    sfk_hip_stats local{};
+    local.h2d_seconds = std::chrono::duration<double>(t1 - t0).count();
+    status = sfk::RunDeviceBuild(b, d_text.As<uint8_t>(), n, construction_bits, flags,
+                                 d_sa.As<void>(), sa_bits, want_isa ? d_isa.As<void>() : nullptr,
+                                 isa_bits, want_lcp ? d_lcp.As<void>() : nullptr, lcp_bits,
+                                 &local, message);
+    const auto t2 = std::chrono::steady_clock::now();
+    SFK_HIP_CHECK(hipMemcpy(sa, d_sa.As<void>(), n * (sa_bits / 8), hipMemcpyDeviceToHost));
+    if (want_isa) SFK_HIP_CHECK(hipMemcpy(isa, d_isa.As<void>(), n * (isa_bits / 8), hipMemcpyDeviceToHost));
+    if (want_lcp) SFK_HIP_CHECK(hipMemcpy(lcp, d_lcp.As<void>(), n * (lcp_bits / 8), hipMemcpyDeviceToHost));
+    local.d2h_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
+    if (stats != nullptr) *stats = local;
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    status = f.status;
+  }
+  d_text.Release();
+  d_sa.Release();
+  d_isa.Release();
+  d_lcp.Release();
+  sfk_hip_builder_destroy(b);
+  return status;
+}
+
+int sfk_hip_malloc(int device, uint64_t bytes, void** out) {
+  if (out == nullptr) return SFK_HIP_INVALID_INPUT;
+  if (hipSetDevice(device) != hipSuccess) return SFK_HIP_UNSUPPORTED_BACKEND;
+  if (hipMalloc(out, bytes == 0 ? 1 : bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return SFK_HIP_BUILD_FAILURE;
+  }
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_free(void* ptr) {
+  return hipFree(ptr) == hipSuccess ? SFK_HIP_OK : SFK_HIP_BUILD_FAILURE;
+}
+
+int sfk_hip_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? SFK_HIP_OK
+                                                                         : SFK_HIP_BUILD_FAILURE;
+}
+
+int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? SFK_HIP_OK
+                                                                         : SFK_HIP_BUILD_FAILURE;
+}
+
+}  // extern "C"
