@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark: SA + ISA + raw LCP construction (Fast profile) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2]
+
+A step is one complete device build (pack, prefix doubling, ISA validation,
+Φ/PLCP/LCP) of the workload's logical text, already resident in HBM, into
+device SA/ISA/LCP buffers.  The default workload is the 64 Mbp multi-contig
+reference (200 contigs, GC 45%, 1% N runs, 30% diverged repeats, 1 Mbp
+poly-A/period-2/period-7 block), 32-bit construction, uint32 SA/ISA/LCP.
+
+Prints one JSON line.  `value` is bases / 1e6 / seconds-per-step.  The
+`roofline` object is for the dominant kernel (the LSD radix scatter pass),
+timed with HIP events on the build stream during the timed steps;
+`cpu_baseline` is the reference's own divsufsort backend (SA + ISA + Kasai
+LCP, 1 thread) timed on this host on the same text, and its output is
+compared with the GPU's element by element.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+WORKLOADS = {
+    1: "4 Mbp single contig, i.i.d. uniform A/C/G/T",
+    2: ("64 Mbp multi-contig reference: 200 log-normal contigs (median 200 kbp), GC 45%, "
+        "1% N runs of 10-5000, 30% repeats (300 bp-6 kbp families, 1-15% substitutions), "
+        "1 Mbp poly-A/period-2/period-7 block"),
+    3: ("3.1 Gbp human-scale genome: 24 contigs of 50-250 Mbp, GC 41%, 45% repeats, "
+        "3% satellites, N gaps up to 10 Mbp"),
+}
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS))
+    ap.add_argument("--scale", type=float, default=1.0, help="workload size multiplier")
+    ap.add_argument("--construction-bits", type=int, default=None, choices=(32, 64))
+    ap.add_argument("--seed", type=int, default=20260822)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-symbols", type=int, default=0,
+                    help="symbols of the text timed on the CPU (0 = whole text)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time without per-kernel HIP events (no roofline)")
+    return ap.parse_args()
+
+
+def cpu_info() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(text: np.ndarray, sample_symbols: int):
+    """Reference divsufsort32 Fast build (SA+ISA+LCP, 1 thread) on a bounded
+    sample of the same text, or the oracle restatement when the compiled
+    reference is absent."""
+    import oracle
+
+    n = text.size
+    if sample_symbols and sample_symbols < n:
+        sample = np.concatenate([text[: sample_symbols - 1], np.zeros(1, np.uint8)])
+        # A prefix may cut inside a contig: keep it a valid logical text.
+        sample[sample == 0] = 1
+        sample[-1] = 0
+    else:
+        sample = text
+    bases = int(np.count_nonzero(sample > 1))
+    if oracle.ref_available():
+        kind = "reference"
+        method = "ref32" if sample.size <= 0x7FFFFFFF else "ref64"
+    else:
+        kind = "port"
+        method = "doubling"
+    t0 = time.perf_counter()
+    sa, isa, lcp = oracle.fast_build(sample, method)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(bases / 1e6 / dt, 3), "unit": "Mbp/s", "cores": 1, "kind": kind,
+        "sample": (f"{'whole' if sample is text else 'prefix of the'} workload text, "
+                   f"{sample.size} symbols ({bases} bases); "
+                   f"{'divsufsort32' if method == 'ref32' else method} SA + ISA + Kasai LCP "
+                   f"(detail::BuildDivsufsort, 1 thread) on {cpu_info()}"),
+        "seconds": round(dt, 3),
+    }, (sa, isa, lcp, sample is text)
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); a text's build is "
+              "not sharded across GPUs in this implementation", file=sys.stderr)
+        return 2
+    import sufkit_amd
+
+    text, contigs = sufkit_amd.synthetic_text(args.workload, args.seed, 0, args.scale)
+    n = int(text.size)
+    bases = int(contigs.sum())
+    bits = args.construction_bits or (32 if n - 1 <= 0xFFFFFFFF else 64)
+    out_bits = 32 if n - 1 <= 0xFFFFFFFF else 64
+    hip = sufkit_amd.load_hip()
+
+    def dmalloc(nbytes):
+        ptr = ctypes.c_void_p()
+        if hip.sfk_hip_malloc(0, nbytes, ctypes.byref(ptr)) != 0:
+            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+        return ptr.value
+
+    d_text = dmalloc(n + 64)
+    d_sa = dmalloc(n * out_bits // 8)
+    d_isa = dmalloc(n * out_bits // 8)
+    d_lcp = dmalloc(n * out_bits // 8)
+    t0 = time.perf_counter()
+    hip.sfk_hip_memcpy_h2d(d_text, text.ctypes.data, n)
+    h2d = time.perf_counter() - t0
+
+    builder = sufkit_amd.DeviceBuilder(0)
+    flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
+    if not args.no_profile:
+        flags |= sufkit_amd.SFK_HIP_PROFILE_KERNELS
+
+    def step():
+        return builder.build(d_text, n, bits, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits, flags)
+
+    for _ in range(args.warmup):
+        step()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())   # synchronous: returns after the build stream drains
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # Results back to the host once (PCIe, outside the timed region).
+    dt = np.uint32 if out_bits == 32 else np.uint64
+    sa = np.empty(n, dt)
+    isa = np.empty(n, dt)
+    lcp = np.empty(n, dt)
+    t0 = time.perf_counter()
+    for arr, ptr in ((sa, d_sa), (isa, d_isa), (lcp, d_lcp)):
+        hip.sfk_hip_memcpy_d2h(arr.ctypes.data, ptr, arr.nbytes)
+    d2h = time.perf_counter() - t0
+
+    last = stats[-1]
+    scatter_s = sum(s["sort_scatter_seconds"] for s in stats)
+    scatter_n = sum(s["sort_scatter_launches"] for s in stats)
+    scatter_b = sum(s["sort_scatter_bytes"] for s in stats)
+    device_s = sum(s["device_seconds"] for s in stats) / len(stats)
+    algo_bytes = last["algorithmic_bytes_sa"] + last["algorithmic_bytes_isa"] + last["algorithmic_bytes_lcp"]
+    roofline = None
+    if scatter_n:
+        achieved = scatter_b / scatter_s / 1e9
+        roofline = {
+            "bound": "hbm", "kernel": "RadixDownsweep (LSD scatter pass)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "launches_per_step": scatter_n // len(stats),
+            "avg_launch_us": round(scatter_s / scatter_n * 1e6, 2),
+            "bytes_per_launch": scatter_b // scatter_n,
+        }
+
+    result = {
+        "metric": "SA+ISA+LCP build Mbp/s + HBM roofline frac, 1 MI355X vs divsufsort64 on host",
+        "value": round(bases / 1e6 / (ms_per_step / 1e3), 2),
+        "unit": "Mbp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"u{bits}",
+        "data": "synthetic (deterministic splitmix64 generator, seed %d)" % args.seed,
+        "config": {
+            "workload": WORKLOADS[args.workload] + (f" x{args.scale}" if args.scale != 1.0 else ""),
+            "text_symbols": n, "bases": bases, "contigs": int(contigs.size),
+            "construction_bits": bits, "stored_bits": out_bits,
+            "profile": "fast (SA+ISA+raw LCP)",
+        },
+        "roofline": roofline,
+        "phases": {
+            "device_seconds": round(device_s, 6),
+            "sa_seconds": round(last["sa_seconds"], 6),
+            "isa_seconds": round(last["isa_seconds"], 6),
+            "lcp_seconds": round(last["lcp_seconds"], 6),
+            "algorithmic_bytes": algo_bytes,
+            "whole_build_GBps": round(algo_bytes / device_s / 1e9, 1),
+            "whole_build_frac": round(algo_bytes / device_s / 1e9 / HBM_PEAK_GBPS, 4),
+            "refine_steps": last["refine_steps"], "doubling_rounds": last["doubling_rounds"],
+            "active_total": last["active_total"], "sort_passes": last["sort_passes"],
+            "lce_long_items": last["lce_long_items"],
+        },
+        "end_to_end_mbps_incl_pcie": round(bases / 1e6 / (ms_per_step / 1e3 + h2d + d2h), 2),
+        "pcie_seconds": {"h2d": round(h2d, 4), "d2h": round(d2h, 4)},
+    }
+
+    if not args.no_cpu_baseline:
+        base, (rsa, risa, rlcp, whole) = cpu_baseline(text, args.cpu_sample_symbols)
+        result["cpu_baseline"] = base
+        if whole:
+            ok = (np.array_equal(sa.astype(np.uint64), rsa) and np.array_equal(isa.astype(np.uint64), risa)
+                  and np.array_equal(lcp.astype(np.uint64), rlcp))
+            result["parity_vs_cpu_baseline"] = "bit-exact" if ok else "MISMATCH"
+    else:
+        result["cpu_baseline"] = None
+
+    builder.close()
+    for ptr in (d_text, d_sa, d_isa, d_lcp):
+        hip.sfk_hip_free(ptr)
+    print(json.dumps(result), flush=True)
+    return 0 if result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact" else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

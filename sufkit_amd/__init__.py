@@ -201,3 +201,47 @@ class DeviceBuilder:
             self.close()
         except Exception:
             pass
+
+
+_HOST = None
+
+
+def host_library_path() -> Path:
+    return LIB_DIR / "libsufkit.so"
+
+
+def load_host() -> ctypes.CDLL:
+    """Load libsufkit.so (host C++ library); raise if absent."""
+    global _HOST
+    if _HOST is not None:
+        return _HOST
+    path = host_library_path()
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build it with `python -m sufkit_amd.build`")
+    lib = ctypes.CDLL(str(path))
+    lib.sfk_synth_plan.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                   ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.sfk_synth_plan.restype = ctypes.c_uint64
+    lib.sfk_synth_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                       ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint]
+    lib.sfk_synth_generate.restype = ctypes.c_int
+    _HOST = lib
+    return lib
+
+
+DEFAULT_SEED = 20260822
+
+
+def synthetic_text(workload: int, seed: int = DEFAULT_SEED, ref: int = 0, scale: float = 1.0,
+                   threads: int = 8) -> tuple[np.ndarray, np.ndarray]:
+    """(logical text, contig lengths) of a benchmark workload (1..5)."""
+    lib = load_host()
+    count = ctypes.c_uint64()
+    lens = np.zeros(4096, dtype=np.uint64)
+    n = lib.sfk_synth_plan(workload, seed, ref, scale, lens.ctypes.data, lens.size, ctypes.byref(count))
+    if n == 0:
+        raise ValueError(f"unknown workload {workload}")
+    text = np.empty(n, dtype=np.uint8)
+    if lib.sfk_synth_generate(workload, seed, ref, scale, text.ctypes.data, n, threads) != 0:
+        raise RuntimeError("synthetic generation failed")
+    return text, lens[: count.value].copy()

@@ -64,23 +64,34 @@ __global__ __launch_bounds__(RadixTile::kBlock) void RadixUpsweep(
   }
 }
 
-// Exclusive scan, in place, of `len` 64-bit counts by one block of 1024.
-__global__ __launch_bounds__(1024) void RadixSpine(std::uint64_t* data,
-                                                   std::uint32_t len) {
-  constexpr int kBlock = 1024;
+// Spine, one block per digit: exclusive scan (in place) of that digit's row
+// of per-block counts, and the digit's total in digit_total[digit].  The
+// downsweep adds the exclusive prefix of digit totals itself.
+__global__ __launch_bounds__(256) void RadixSpine(std::uint64_t* __restrict__ counts,
+                                                  int grid,
+                                                  std::uint64_t* __restrict__ digit_total) {
+  constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  const std::uint32_t per = (len + kBlock - 1) / kBlock;
-  const std::uint32_t begin = threadIdx.x * per;
-  const std::uint32_t end = begin + per < len ? begin + per : len;
+  std::uint64_t* row = counts + static_cast<std::uint64_t>(blockIdx.x) * grid;
+  const int per = (grid + kBlock - 1) / kBlock;  // grid <= kMaxGrid = 1024 -> per <= 4
+  const int begin = threadIdx.x * per;
+  std::uint64_t vals[4];
   std::uint64_t local = 0;
-  for (std::uint32_t i = begin; i < end; ++i) local += data[i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    vals[i] = (i < per && begin + i < grid) ? row[begin + i] : 0ull;
+    local += vals[i];
+  }
   std::uint64_t total;
   std::uint64_t running = BlockExclusiveSum<kBlock>(local, scratch, total);
-  for (std::uint32_t i = begin; i < end; ++i) {
-    const std::uint64_t v = data[i];
-    data[i] = running;
-    running += v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i < per && begin + i < grid) {
+      row[begin + i] = running;
+      running += vals[i];
+    }
   }
+  if (threadIdx.x == 0) digit_total[blockIdx.x] = total;
 }
 
 template <typename K, typename V, int kItems>
@@ -88,7 +99,7 @@ __global__ __launch_bounds__(RadixTile::kBlock) void RadixDownsweep(
     const K* __restrict__ keys_in, const V* __restrict__ vals_in,
     K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m,
     std::uint64_t chunk, int shift, const std::uint64_t* __restrict__ offsets,
-    int grid) {
+    const std::uint64_t* __restrict__ digit_total, int grid) {
   constexpr int kBlock = RadixTile::kBlock;
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
@@ -103,8 +114,14 @@ __global__ __launch_bounds__(RadixTile::kBlock) void RadixDownsweep(
 
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  for (int d = threadIdx.x; d < kRadix; d += kBlock) {
-    run_off[d] = offsets[static_cast<std::uint64_t>(d) * grid + blockIdx.x];
+  {
+    static_assert(kRadix == kBlock, "one digit per thread");
+    __shared__ std::uint64_t base_scratch[kBlock / kWave];
+    std::uint64_t total;
+    const std::uint64_t digit_base =
+        BlockExclusiveSum<kBlock>(digit_total[threadIdx.x], base_scratch, total);
+    run_off[threadIdx.x] =
+        digit_base + offsets[static_cast<std::uint64_t>(threadIdx.x) * grid + blockIdx.x];
   }
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < m ? begin + chunk : m;

@@ -196,14 +196,15 @@ void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
   constexpr int kTile = RadixTile::kBlock * kItems;
   const SortGeometry g = Geometry(m, kTile, b->radix_grid);
   std::uint64_t* counts = b->radix_counts.As<std::uint64_t>();
+  std::uint64_t* digit_total = counts + static_cast<std::uint64_t>(RadixTile::kRadix) * RadixTile::kMaxGrid;
   for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) {
     if (((varying >> shift) & (RadixTile::kRadix - 1)) == 0) continue;
     RadixUpsweep<K, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
         keys, m, g.chunk, shift, counts, g.grid);
-    RadixSpine<<<1, 1024, 0, b->stream>>>(counts, static_cast<std::uint32_t>(RadixTile::kRadix * g.grid));
+    RadixSpine<<<RadixTile::kRadix, 256, 0, b->stream>>>(counts, g.grid, digit_total);
     EventPair* ev = BeginProfiled(ctx, 2ull * m * (sizeof(K) + sizeof(V)));
     RadixDownsweep<K, V, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
-        keys, vals, keys_alt, vals_alt, m, g.chunk, shift, counts, g.grid);
+        keys, vals, keys_alt, vals_alt, m, g.chunk, shift, counts, digit_total, g.grid);
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
     std::swap(keys, keys_alt);
@@ -273,22 +274,28 @@ class Doubling {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
     constexpr std::uint64_t w = sizeof(Index);
 
-    ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
-    KeygenKernel<Index, K><<<static_cast<unsigned>(CeilDiv(m, 256)), 256, 0, b_->stream>>>(
+    const std::uint64_t all_bits = total_bits >= 64 ? ~0ull : ((1ull << total_bits) - 1);
+    std::uint64_t varying = all_bits;
+    const bool rank_step = sec_kind == kSecRank;
+    if (!rank_step) ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+    const unsigned keygen_grid = static_cast<unsigned>(std::min<std::uint64_t>(CeilDiv(m, 256), 2048));
+    KeygenKernel<Index, K><<<keygen_grid, 256, 0, b_->stream>>>(
         b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), m, sec_kind, sec_bits, seg_bits,
         text_, rank_, h, iota, keys, vals, dc);
     SFK_HIP_CHECK(hipGetLastError());
     ctx_.stats.algorithmic_bytes_sa +=
-        m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (sec_kind == kSecRank ? w : 24) +
-             sizeof(K) + w);
-    const DeviceCounters& c = ReadCounters(b_);
-    if (c.error_flags != 0) {
-      Fail(SFK_HIP_INVALID_INPUT, c.error_flags & kFlagBadSymbol
-                                      ? "text holds a symbol outside 0..6"
-                                      : "text must end with its only 0 (sentinel)");
+        m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
+    if (!rank_step) {
+      // Text-derived keys: skip digits that are constant over the active set
+      // (and surface the packing errors before any rank gather).
+      const DeviceCounters& c = ReadCounters(b_);
+      if (c.error_flags != 0) {
+        Fail(SFK_HIP_INVALID_INPUT, c.error_flags & kFlagBadSymbol
+                                        ? "text holds a symbol outside 0..6"
+                                        : "text must end with its only 0 (sentinel)");
+      }
+      varying = (c.key_or ^ c.key_and) & all_bits;
     }
-    const std::uint64_t varying = (c.key_or ^ c.key_and) &
-                                  (total_bits >= 64 ? ~0ull : ((1ull << total_bits) - 1));
     SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, varying, total_bits);
 
     // Boundaries, singletons, compaction.
@@ -299,7 +306,7 @@ class Doubling {
     Index* head_next = (flip_ ? b_->seg_head0 : b_->seg_head1).template As<Index>();
     Index* start_next = (flip_ ? b_->seg_start0 : b_->seg_start1).template As<Index>();
     RefineReduceKernel<K><<<g.grid, 256, 0, b_->stream>>>(keys, m, g.chunk, partial);
-    RefineSpineKernel<<<1, 64, 0, b_->stream>>>(partial, g.grid, dc);
+    RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
     RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
         keys, vals, m, g.chunk, sec_bits, head_cur, start_cur, partial, n_, sa_, rank_,
         b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), head_next, start_next);
@@ -339,6 +346,9 @@ void LaunchLcp(BuildContext& ctx, PackedText text, const Index* sa, const Index*
       text, sa, isa, n, kWordLimit, val, queue, capacity, dc);
   SFK_HIP_CHECK(hipGetLastError());
   const DeviceCounters& c = ReadCounters(b);
+  if (c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) {
+    Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
+  }
   const std::uint64_t items = c.long_items;
   if (items > capacity) {
     Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
@@ -351,7 +361,7 @@ void LaunchLcp(BuildContext& ctx, PackedText text, const Index* sa, const Index*
   const SortGeometry g = Geometry(n, 256 * kScanItems, 1024);
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
   MaxReduceKernel<Index><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
-  MaxSpineKernel<<<1, 64, 0, b->stream>>>(partial, g.grid);
+  MaxSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
   MaxScanToPlcpKernel<Index, kScanItems><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
   LcpPermuteKernel<Index, Out><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
       val, sa, n, lcp);
@@ -405,7 +415,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   b->seg_head1.Reserve(seg_cap * sizeof(Index));
   b->seg_start0.Reserve(seg_cap * sizeof(Index));
   b->seg_start1.Reserve(seg_cap * sizeof(Index));
-  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * 1024 * 8);
+  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * (RadixTile::kMaxGrid + 1) * 8);
   b->partial.Reserve(1024 * sizeof(RefinePartial));
 
   Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
@@ -423,27 +433,21 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   Doubling<Index>(ctx, sa, rank, text).Run();
   SFK_HIP_CHECK(hipEventRecord(b->phase[1], b->stream));
 
-  // K9: permutation + inverse check on the device; ISA in its stored width.
-  ValidateKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
-      sa, rank, n, b->counters.As<DeviceCounters>());
-  SFK_HIP_CHECK(hipGetLastError());
-  ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
+  // K9: permutation + inverse check on the device (fused into the PLCP pass
+  // when the LCP is built); ISA in its stored width.
+  const bool want_lcp = d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
+  if (!want_lcp) {
+    ValidateKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+        sa, rank, n, b->counters.As<DeviceCounters>());
+    SFK_HIP_CHECK(hipGetLastError());
+    ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
+  }
   if (d_isa != nullptr && !isa_direct) {
     ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
     ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + isa_bits / 8);
   }
   SFK_HIP_CHECK(hipEventRecord(b->phase[2], b->stream));
-  {
-    // The LCP kernels index SA and ISA with each other's values: only run
-    // them on a validated permutation/inverse pair.
-    const DeviceCounters& c = ReadCounters(b);
-    if (c.error_flags != 0) {
-      Fail(SFK_HIP_BUILD_FAILURE, "suffix array failed device validation (flags " +
-                                      std::to_string(c.error_flags) + ")");
-    }
-  }
-
-  if (d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP)) {
+  if (want_lcp) {
     if (lcp_bits == 32) {
       LaunchLcp<Index, std::uint32_t>(ctx, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
     } else {

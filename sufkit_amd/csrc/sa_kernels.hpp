@@ -118,11 +118,12 @@ __global__ __launch_bounds__(256) void KeygenKernel(
     const Index* __restrict__ rank, std::uint64_t h, bool iota,
     K* __restrict__ keys, Index* __restrict__ vals, DeviceCounters* counters) {
   constexpr int kBlock = 256;
-  __shared__ unsigned long long s_or[kBlock / kWave];
-  __shared__ unsigned long long s_and[kBlock / kWave];
-  const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  unsigned long long kor = 0, kand = ~0ull;
-  if (k < m) {
+  __shared__ std::uint64_t s_or[kBlock / kWave];
+  __shared__ std::uint64_t s_and[kBlock / kWave];
+  std::uint64_t kor = 0, kand = ~0ull;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kBlock;
+  for (std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x; k < m;
+       k += stride) {
     const Index v = iota ? static_cast<Index>(k) : act_idx[k];
     std::uint64_t sec;
     if (sec_kind == kSecRank) {
@@ -151,13 +152,13 @@ __global__ __launch_bounds__(256) void KeygenKernel(
     const std::uint64_t key = (seg << sec_bits) | sec;
     keys[k] = static_cast<K>(key);
     vals[k] = v;
-    kor = key;
-    kand = key;
+    kor |= key;
+    kand &= key;
   }
-  // wave reduce
+  // One atomic pair per block: wave butterfly, then LDS across waves.
   for (int d = 32; d >= 1; d >>= 1) {
-    kor |= WaveShfl(static_cast<std::uint64_t>(kor), static_cast<int>(LaneId() ^ d));
-    kand &= WaveShfl(static_cast<std::uint64_t>(kand), static_cast<int>(LaneId() ^ d));
+    kor |= WaveShfl(kor, static_cast<int>(LaneId() ^ d));
+    kand &= WaveShfl(kand, static_cast<int>(LaneId() ^ d));
   }
   const int wave = threadIdx.x / kWave;
   if (LaneId() == 0) {
@@ -166,13 +167,13 @@ __global__ __launch_bounds__(256) void KeygenKernel(
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long o = 0, a = ~0ull;
+    std::uint64_t o = 0, a = ~0ull;
     for (int w = 0; w < kBlock / kWave; ++w) {
       o |= s_or[w];
       a &= s_and[w];
     }
-    atomicOr(&counters->key_or, o);
-    atomicAnd(&counters->key_and, a);
+    atomicOr(&counters->key_or, static_cast<unsigned long long>(o));
+    atomicAnd(&counters->key_and, static_cast<unsigned long long>(a));
   }
 }
 
@@ -240,22 +241,26 @@ __global__ __launch_bounds__(256) void RefineReduceKernel(
   }
 }
 
-// Serial exclusive scan of at most kMaxGrid partials; one thread.
-__global__ void RefineSpineKernel(RefinePartial* partial, int grid,
-                                  DeviceCounters* counters) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  std::uint64_t surv = 0, segs = 0, last = 0;
-  for (int b = 0; b < grid; ++b) {
-    const RefinePartial p = partial[b];
-    partial[b].survivors = surv;
-    partial[b].newsegs = segs;
-    partial[b].last_boundary = last;
-    surv += p.survivors;
-    segs += p.newsegs;
-    if (p.last_boundary > last) last = p.last_boundary;
+// Exclusive scan of at most 1024 block partials by one block of 1024.
+__global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial, int grid,
+                                                          DeviceCounters* counters) {
+  constexpr int kBlock = 1024;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  const RefinePartial p = valid ? partial[threadIdx.x] : RefinePartial{0, 0, 0, 0};
+  std::uint64_t tot_surv, tot_segs, tot_last;
+  const std::uint64_t surv = BlockExclusiveSum<kBlock>(p.survivors, scratch, tot_surv);
+  const std::uint64_t segs = BlockExclusiveSum<kBlock>(p.newsegs, scratch, tot_segs);
+  const std::uint64_t last = BlockExclusiveMax<kBlock>(p.last_boundary, scratch, tot_last);
+  if (valid) {
+    partial[threadIdx.x].survivors = surv;
+    partial[threadIdx.x].newsegs = segs;
+    partial[threadIdx.x].last_boundary = last;
   }
-  counters->survivors = surv;
-  counters->segments = segs;
+  if (threadIdx.x == 0) {
+    counters->survivors = tot_surv;
+    counters->segments = tot_segs;
+  }
 }
 
 // Each thread owns kItems consecutive sorted slots of one tile.
@@ -355,11 +360,23 @@ __global__ __launch_bounds__(256) void PlcpIrreducibleKernel(
   const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const std::uint64_t r = isa[i];
+  // Validation fused here: ISA[i] < n and SA[ISA[i]] == i for every i makes
+  // ISA a bijection and SA its inverse (the permutation check of the SA).
+  if (r >= n || static_cast<std::uint64_t>(sa[r]) != i) {
+    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotInverse));
+    val[i] = 0;
+    return;
+  }
   if (r == 0) {
     val[i] = static_cast<Index>(i);
     return;
   }
   const std::uint64_t j = sa[r - 1];
+  if (j >= n) {  // only possible for a corrupted SA; keeps every load in bounds
+    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotPermutation));
+    val[i] = 0;
+    return;
+  }
   if (i > 0 && j > 0 && SymbolAt(t, i - 1) == SymbolAt(t, j - 1)) {
     val[i] = 0;
     return;
@@ -445,14 +462,14 @@ __global__ __launch_bounds__(256) void MaxReduceKernel(
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-__global__ void MaxSpineKernel(std::uint64_t* partial, int grid) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  std::uint64_t run = 0;
-  for (int b = 0; b < grid; ++b) {
-    const std::uint64_t v = partial[b];
-    partial[b] = run;
-    run = v > run ? v : run;
-  }
+__global__ __launch_bounds__(1024) void MaxSpineKernel(std::uint64_t* partial, int grid) {
+  constexpr int kBlock = 1024;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  std::uint64_t total;
+  const std::uint64_t excl =
+      BlockExclusiveMax<kBlock>(valid ? partial[threadIdx.x] : 0ull, scratch, total);
+  if (valid) partial[threadIdx.x] = excl;
 }
 
 // Inclusive max-scan apply phase; each thread owns kItems consecutive items,
