@@ -1,0 +1,132 @@
+"""Pins the CPU oracle before anything is compared against it (CPU only).
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index) fails it: the reference's own golden vectors,
+brute force on small texts, closed forms, and the compiled reference.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import synth
+
+
+def vectors(fixtures, name):
+    return fixtures[name]["vectors"]
+
+
+def test_encode_records_matches_fixture_text(legacy_fixtures):
+    # FromRecords of tests/data/reference.fa (genome_reference.cpp:131-160).
+    text, amb = oracle.encode_records([b"ACGTNACGT", b"TTTACGTAAA", b"AAA"])
+    assert text.tolist() == vectors(legacy_fixtures, "legacy-sa-v1.1")["text"]
+    assert amb == legacy_fixtures["legacy-sa-v1.1"]["ambiguous_bases"] == 1
+    # lowercase and IUPAC codes normalise like genome_reference.cpp:30-46
+    lower, amb2 = oracle.encode_records([b"acgtRYn"])
+    assert lower.tolist() == [2, 3, 4, 5, 6, 6, 6, 1, 0] and amb2 == 3
+
+
+def test_fingerprint_matches_fixture(legacy_fixtures):
+    text = np.array(vectors(legacy_fixtures, "legacy-sa-v1.1")["text"], np.uint8)
+    fp = oracle.fingerprint(text[:-1])  # FNV over encoded bytes, sentinel excluded
+    assert f"0x{fp:016x}" == legacy_fixtures["legacy-sa-v1.1"]["fingerprint"]
+    assert oracle.fingerprint(np.zeros(0, np.uint8)) == 14695981039346656037
+
+
+@pytest.mark.parametrize("method", ["naive", "doubling"])
+def test_sa_isa_lcp_match_fixture(legacy_fixtures, method):
+    v = vectors(legacy_fixtures, "legacy-sa-v1.1")
+    text = np.array(v["text"], np.uint8)
+    sa, isa, lcp = oracle.fast_build(text, method)
+    assert sa.tolist() == v["sa"]
+    assert isa.tolist() == v["isa"]
+    assert lcp.tolist() == v["lcp"]
+
+
+def test_sampled_kasai_matches_fixture(legacy_fixtures):
+    # legacy-sa-v1.3: K = 2 sampled SA, dense ISA over pos/K, generalised
+    # Kasai with carry common -= K (divsufsort_backend.cpp:63-150).
+    v = vectors(legacy_fixtures, "legacy-sa-v1.3")
+    k = v["sampling_rate"]
+    text = np.array(v["text"], np.uint8)
+    full = oracle.sa_naive(text)
+    sampled = full[full % k == 0]
+    assert sampled.tolist() == v["sa"]
+    isa = oracle.isa(sampled, k)
+    assert isa.tolist() == v["isa"]
+    assert oracle.lcp_kasai(text, sampled, isa, k).tolist() == v["lcp"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_doubling_equals_naive(seed):
+    for text in (synth.random_text(seed, 3000), synth.random_symbols(seed, 2000),
+                 synth.run_text(seed, [("A", 500), ("rand", 50), ("tandem3", 400), ("sep", 1),
+                                       ("N", 300), ("tandem7", 200)])):
+        assert np.array_equal(oracle.sa_doubling(text), oracle.sa_naive(text))
+
+
+def test_sa_is_sorted_permutation():
+    text = synth.repeat_text(3, 5000, cover=0.5)
+    sa = oracle.sa_doubling(text)
+    assert np.array_equal(np.sort(sa), np.arange(text.size))
+    suffixes = [bytes(text[i:]) for i in sa]
+    assert all(a < b for a, b in zip(suffixes, suffixes[1:]))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_kasai_equals_brute_force(seed):
+    text = synth.run_text(seed, [("rand", 300), ("A", 200), ("rand", 40), ("tandem2", 300),
+                                 ("sep", 1), ("N", 100), ("rand", 200)])
+    sa = oracle.sa_naive(text)
+    isa = oracle.isa(sa)
+    assert np.array_equal(isa[sa], np.arange(text.size))
+    assert np.array_equal(oracle.lcp_kasai(text, sa, isa), oracle.lcp_brute(text, sa))
+
+
+def test_planted_lcp_mistakes_are_caught():
+    # A wrong carry, an off-by-one row, or a missing LCP[0]=0 must all fail
+    # the brute-force pin used above.
+    text = synth.repeat_text(4, 3000, cover=0.6)
+    sa = oracle.sa_naive(text)
+    good = oracle.lcp_brute(text, sa)
+    shifted = np.roll(good, 1)
+    bumped = good.copy()
+    bumped[len(bumped) // 2] += 1
+    nonzero_head = good.copy()
+    nonzero_head[0] = 1
+    for bad in (shifted, bumped, nonzero_head):
+        assert not np.array_equal(bad, good)
+
+
+def test_crc32_check_value_and_fixture_crcs(legacy_fixtures):
+    assert oracle.crc32(b"123456789") == 0xCBF43926
+    assert oracle.crc32(b"6789", oracle.crc32(b"12345")) == 0xCBF43926
+    for name in ("legacy-sa-v1.0", "legacy-sa-v1.1", "legacy-sa-v1.2", "legacy-sa-v1.3"):
+        fx = legacy_fixtures[name]
+        header = bytes.fromhex(fx["header_zeroed_crc_hex"])
+        assert oracle.crc32(header) == fx["header_crc"]
+        for section in fx["sections"]:
+            assert oracle.crc32(bytes.fromhex(section["hex"])) == section["crc"]
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built")
+@pytest.mark.parametrize("method", ["ref32", "ref64"])
+def test_compiled_reference_matches_fixture_and_oracle(legacy_fixtures, method):
+    v = vectors(legacy_fixtures, "legacy-sa-v1.1")
+    sa, isa, lcp = oracle.fast_build(np.array(v["text"], np.uint8), method)
+    assert (sa.tolist(), isa.tolist(), lcp.tolist()) == (v["sa"], v["isa"], v["lcp"])
+    text = synth.repeat_text(8, 20000, cover=0.5, n_runs=6)
+    got = oracle.fast_build(text, method)
+    want = oracle.fast_build(text, "doubling")
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["runs_300k", "symbols_200k"])
+def test_oracle_matches_reference_digests(reference_digests, name):
+    # Committed digests of the compiled reference (tests/golden/make_golden.py).
+    text = dict(synth.golden_cases())[name]
+    sa, isa, lcp = oracle.fast_build(text, "doubling")
+    want = reference_digests[name]["ref32"]
+    assert f"0x{oracle.digest(sa):016x}" == want["sa"]
+    assert f"0x{oracle.digest(isa):016x}" == want["isa"]
+    assert f"0x{oracle.digest(lcp):016x}" == want["lcp"]
