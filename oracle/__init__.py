@@ -36,7 +36,8 @@ def build(ref: str | None = "/root/reference") -> None:
     args = ["make", "-s", "-C", str(HERE), "oracle"]
     subprocess.run(args, check=True, env=env)
     if ref and Path(ref, "src", "divsufsort_backend.cpp").exists():
-        subprocess.run(["make", "-s", "-j8", "-C", str(HERE), "ref", f"REF={ref}"], check=True, env=env)
+        subprocess.run(["make", "-s", "-j8", "-C", str(HERE), "ref", "ref-full", f"REF={ref}"],
+                       check=True, env=env)
 
 
 def lib() -> ctypes.CDLL:
@@ -192,3 +193,26 @@ def fast_build(text: np.ndarray, method: str = "auto"):
     sa = sa_naive(text) if method == "naive" else sa_doubling(text)
     isa_ = isa(sa)
     return sa, isa_, lcp_kasai(text, sa, isa_)
+
+
+def ref_writer_available() -> bool:
+    return (HERE / "_ref" / "libsufkit_refwriter.so").exists()
+
+
+def ref_write_sufidx(records: list[tuple[str, bytes]], path: str, coordinate_width: int = 0,
+                     storage_width: int = 0, acceleration: int = 3) -> None:
+    """Build with the reference's own SuffixArray::Build (divsufsort) and Save
+    a .sufidx 1.4 file (test infrastructure: golden files only)."""
+    W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
+    W.sfk_ref_write_sufidx.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                       U64P, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    W.sfk_ref_write_sufidx.restype = ctypes.c_int
+    names = (ctypes.c_char_p * len(records))(*[r[0].encode() for r in records])
+    seqs = (ctypes.c_char_p * len(records))(*[r[1] for r in records])
+    lens = np.array([len(r[1]) for r in records], dtype=np.uint64)
+    err = ctypes.create_string_buffer(512)
+    rc = W.sfk_ref_write_sufidx(names, seqs, _u64(lens), len(records), str(path).encode(),
+                                coordinate_width, storage_width, acceleration, err, len(err))
+    if rc != 0:
+        raise RuntimeError(err.value.decode())

@@ -1,0 +1,220 @@
+// Internal building blocks of the host library: symbol codes, width
+// policies, coordinate/LCP storage, the .sufidx 1.4 section codecs and the
+// CRC-protected container.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <filesystem>
+#include <functional>
+#include <iosfwd>
+#include <limits>
+#include <string>
+#include <variant>
+#include <vector>
+
+#include "sufkit.hpp"
+
+namespace sufkit::detail {
+
+// reference_data.hpp:20-27
+constexpr std::uint8_t kSentinel = 0;
+constexpr std::uint8_t kSeparator = 1;
+constexpr std::uint8_t kA = 2;
+constexpr std::uint8_t kC = 3;
+constexpr std::uint8_t kG = 4;
+constexpr std::uint8_t kT = 5;
+constexpr std::uint8_t kN = 6;
+constexpr std::uint8_t kNormalizationId = 1;
+
+constexpr std::uint64_t kMaxCoordinate40 = (1ULL << 40) - 1;
+constexpr std::uint64_t kMaxCoordinate48 = (1ULL << 48) - 1;
+
+// Library version written into .sufidx headers: the reference format
+// version 0.3.0 this writer follows (version.hpp:10-16).
+constexpr std::uint8_t kVersionMajor = 0;
+constexpr std::uint8_t kVersionMinor = 3;
+constexpr std::uint8_t kVersionPatch = 0;
+
+// Stored backend IDs (serialization.hpp:25-34); 5 and 6 are the HIP builds.
+enum class StoredBackend : std::uint8_t {
+  kDivsufsort32 = 1,
+  kDivsufsort64 = 2,
+  kCaps32 = 3,
+  kCaps64 = 4,
+  kHip32 = 5,
+  kHip64 = 6,
+};
+
+enum class SectionType : std::uint32_t {
+  kMetadata = 1,
+  kText = 2,
+  kSuffixArray = 3,
+  kSdslCsa = 4,
+  kInverseSuffixArray = 5,
+  kLcp = 6,
+  kChild = 7,
+  kLearnedSa = 8,
+  kSaSampling = 9,
+};
+
+[[noreturn]] void Throw(ErrorCode code, const std::string& message);
+
+std::uint64_t ContentFingerprint(const std::uint8_t* data, std::size_t size);
+
+// ------------------------------------------------------------- policies ---
+// caps_backend.hpp:25-81, suffix_array.cpp:806-831, coordinate_storage.cpp:
+// 363-417, with the HIP backend's limits (32-bit construction up to
+// UINT32_MAX symbols, like CaPS-SA's uint32_t).
+SaBackend ResolveSaBackend(SaBackend requested, std::uint64_t text_symbols,
+                           std::uint32_t threads, bool hip_available) noexcept;
+CoordinateWidth ResolveSaCoordinateWidth(SaBackend backend, CoordinateWidth requested,
+                                         std::uint64_t text_symbols) noexcept;
+std::uint64_t MaximumSaConstructionTextSymbols(SaBackend backend,
+                                               CoordinateWidth width) noexcept;
+bool SaConstructionCanRepresent(SaBackend backend, CoordinateWidth width,
+                                std::uint64_t text_symbols) noexcept;
+CoordinateStorageWidth SelectCoordinateStorageWidth(std::uint64_t symbol_count,
+                                                    ErrorCode error_code = ErrorCode::kInvalidInput);
+CoordinateStorageWidth ResolveCoordinateStorageWidth(CoordinateStorageWidth requested,
+                                                     std::uint64_t symbol_count,
+                                                     ErrorCode error_code = ErrorCode::kInvalidInput);
+std::uint64_t MaxCoordinateForWidth(CoordinateStorageWidth width,
+                                    ErrorCode error_code = ErrorCode::kInvalidInput);
+CoordinateStorageWidth ResolveProfileStorageWidth(CoordinateStorageWidth requested,
+                                                  SaResourceProfile profile,
+                                                  std::uint64_t symbol_count);
+CoordinateStorageWidth ResolveAuxiliaryStorageWidth(CoordinateStorageWidth preferred,
+                                                    std::uint64_t symbol_count);
+
+// ----------------------------------------------------- coordinate storage ---
+struct Coordinate40 {
+  std::vector<std::uint32_t> low;
+  std::vector<std::uint8_t> high;
+};
+struct Coordinate48 {
+  std::vector<std::uint32_t> low;
+  std::vector<std::uint16_t> high;
+};
+
+// Immutable coordinates in one of four physical widths; every value is
+// range-checked against the symbol domain when built (coordinate_storage.cpp:
+// 439-598 semantics).
+class CoordinateStorage {
+ public:
+  using Storage = std::variant<std::monostate, std::vector<std::uint32_t>, std::vector<std::uint64_t>,
+                               Coordinate40, Coordinate48>;
+  CoordinateStorage() = default;
+  static CoordinateStorage FromUInt32(std::vector<std::uint32_t>&& values,
+                                      CoordinateStorageWidth width, std::uint64_t symbol_count,
+                                      ErrorCode error_code, const char* label);
+  static CoordinateStorage FromUInt64(std::vector<std::uint64_t>&& values,
+                                      CoordinateStorageWidth width, std::uint64_t symbol_count,
+                                      ErrorCode error_code, const char* label);
+  static CoordinateStorage FromPacked40(Coordinate40&& values, std::uint64_t symbol_count,
+                                        ErrorCode error_code, const char* label);
+  static CoordinateStorage FromPacked48(Coordinate48&& values, std::uint64_t symbol_count,
+                                        ErrorCode error_code, const char* label);
+  bool Empty() const noexcept { return storage_.index() == 0; }
+  std::uint64_t Size() const noexcept;
+  CoordinateStorageWidth Width() const noexcept { return width_; }
+  std::uint64_t Bytes() const noexcept;
+  std::uint64_t At(std::uint64_t index) const noexcept;
+  const Storage& Raw() const noexcept { return storage_; }
+
+ private:
+  Storage storage_;
+  CoordinateStorageWidth width_ = CoordinateStorageWidth::kAutoSelect;
+};
+
+// Raw LCP (the Fast profile's layout): one 32- or 64-bit value per row,
+// LCP[0] == 0 (lcp_storage.cpp:508-528).
+class LcpStorage {
+ public:
+  LcpStorage() = default;
+  static LcpStorage FromRaw32(std::vector<std::uint32_t>&& values, std::uint32_t sampling_rate,
+                              ErrorCode error_code);
+  static LcpStorage FromRaw64(std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
+                              ErrorCode error_code);
+  bool Empty() const noexcept { return width_ == 0; }
+  std::uint64_t Size() const noexcept;
+  std::uint8_t Width() const noexcept { return width_; }
+  std::uint32_t SamplingRate() const noexcept { return sampling_rate_; }
+  std::uint64_t At(std::uint64_t row) const noexcept;
+  std::uint64_t Bytes() const noexcept { return Size() * (width_ / 8u); }
+  const std::vector<std::uint32_t>& Raw32() const noexcept { return raw32_; }
+  const std::vector<std::uint64_t>& Raw64() const noexcept { return raw64_; }
+  // lcp_storage.cpp:693-770 for raw LCP: row count, LCP[0]==0, bounds.
+  void Validate(const CoordinateStorage& sa, std::uint64_t text_symbols,
+                ErrorCode error_code) const;
+
+ private:
+  std::vector<std::uint32_t> raw32_;
+  std::vector<std::uint64_t> raw64_;
+  std::uint8_t width_ = 0;
+  std::uint32_t sampling_rate_ = 1;
+};
+
+// ------------------------------------------------------------- codecs ---
+void WriteCoordinateSectionV14(std::ostream& out, const CoordinateStorage& storage,
+                               std::uint64_t symbol_count);
+CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t section_size,
+                                           std::uint64_t expected_elements,
+                                           std::uint64_t expected_symbols, const char* label);
+void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t text_symbols);
+LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size,
+                             std::uint64_t expected_rows, std::uint32_t sampling_rate,
+                             std::uint64_t text_symbols);
+
+// ---------------------------------------------------------- container ---
+struct ContainerSpec {
+  std::uint16_t format_minor = 4;
+  std::uint8_t kind = 1;  // IndexKind::kSuffixArray
+  StoredBackend backend = StoredBackend::kHip32;
+  std::uint8_t coordinate_width = 32;
+  std::uint8_t library_major = kVersionMajor;
+  std::uint8_t library_minor = kVersionMinor;
+  std::uint8_t library_patch = kVersionPatch;
+  std::uint8_t sdsl_major = 0, sdsl_minor = 0, sdsl_patch = 0;
+  SaResourceProfile profile = SaResourceProfile::kFast;
+  std::uint64_t sequence_count = 0;
+  std::uint64_t total_bases = 0;
+  std::uint64_t text_symbols = 0;
+  std::uint64_t ambiguous_bases = 0;
+  std::uint64_t fingerprint = 0;
+};
+
+struct SectionDescriptor {
+  SectionType type = SectionType::kMetadata;
+  std::uint32_t flags = 1;
+  std::uint64_t offset = 0;
+  std::uint64_t size = 0;
+  std::uint32_t crc32 = 0;
+};
+
+struct SectionWriter {
+  SectionType type;
+  std::function<void(std::ostream&)> write;
+};
+
+struct ParsedContainer {
+  std::filesystem::path path;
+  ContainerSpec spec;
+  std::vector<SectionDescriptor> sections;
+  std::uint64_t file_size = 0;
+  const SectionDescriptor* Find(SectionType type) const noexcept;
+  const SectionDescriptor& Require(SectionType type) const;
+};
+
+std::uint32_t Crc32(std::uint32_t crc, const void* data, std::size_t size);
+std::vector<std::uint8_t> BuildHeader(const ContainerSpec& spec,
+                                      const std::vector<SectionDescriptor>& sections);
+void WriteContainer(const std::filesystem::path& path, const SaveOptions& options,
+                    const ContainerSpec& spec, const std::vector<SectionWriter>& writers);
+ParsedContainer ReadContainer(const std::filesystem::path& path);
+std::vector<std::uint8_t> ReadSection(const ParsedContainer& container, SectionType type);
+
+void WriteU32(std::ostream& out, std::uint32_t value);
+void WriteU64(std::ostream& out, std::uint64_t value);
+
+}  // namespace sufkit::detail

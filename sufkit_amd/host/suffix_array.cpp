@@ -1,0 +1,595 @@
+// SuffixArray build / persist / load around the HIP construction backend.
+//
+// Build follows SuffixArray::Build (suffix_array.cpp:4086-4390): the same
+// stage notifications, the same backend/width/storage/LCP-width policies,
+// text = encoded ++ [sentinel], then one device build that returns the
+// device-validated SA (permutation + inverse checked on the GPU, so the
+// host does not repeat RepackSuffixArray's O(n) random bitmap pass), the ISA
+// and the raw LCP directly in their resolved widths.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <string>
+
+#include "internal.hpp"
+#include "sufkit_hip.h"
+
+namespace sufkit {
+
+SuffixArrayBuildOptions FastSuffixArrayBuildOptions() { return SuffixArrayBuildOptions{}; }
+
+const char* StoredBackendName(std::uint8_t id) noexcept {
+  switch (id) {
+    case 1: return "divsufsort32";
+    case 2: return "divsufsort64";
+    case 3: return "caps32";
+    case 4: return "caps64";
+    case 5: return "hip32";
+    case 6: return "hip64";
+    default: return "unknown";
+  }
+}
+
+const char* StoredBackendSignature(std::uint8_t id) noexcept {
+  switch (id) {
+    case 1: return "libdivsufsort-2.0.2/saidx_t";
+    case 2: return "libdivsufsort-2.0.2/saidx64_t";
+    case 3: return "CaPS-SA@2597b373/uint32_t+ParlayLib@e1f1dc0";
+    case 4: return "CaPS-SA@2597b373/uint64_t+ParlayLib@e1f1dc0";
+    case 5: return "sufkit-amd-hip-gfx950-prefix-doubling/uint32_t";
+    case 6: return "sufkit-amd-hip-gfx950-prefix-doubling/uint64_t";
+    default: return "unknown";
+  }
+}
+
+bool HipBackendAvailable() noexcept { return sfk_hip_device_count() > 0; }
+
+struct SuffixArray::Impl {
+  std::vector<SequenceInfo> sequences;
+  std::uint64_t total_bases = 0;
+  std::uint64_t ambiguous_bases = 0;
+  std::uint64_t fingerprint = 0;
+  std::vector<std::uint8_t> text;
+  detail::StoredBackend backend = detail::StoredBackend::kHip32;
+  std::uint8_t construction_width = 32;
+  std::uint32_t sampling_rate = 1;
+  SaAcceleration acceleration = SaAcceleration::kLcpSuffixLink;
+  detail::CoordinateStorage sa;
+  detail::CoordinateStorage isa;
+  detail::LcpStorage lcp;
+  IndexInfo info;
+
+  void RefreshInfo() {
+    info.format_version = "1.4";
+    info.library_version = "0.3.0";
+    info.backend = StoredBackendName(static_cast<std::uint8_t>(backend));
+    info.backend_signature = StoredBackendSignature(static_cast<std::uint8_t>(backend));
+    info.coordinate_width = construction_width;
+    info.stored_coordinate_width = static_cast<std::uint8_t>(sa.Width());
+    info.sequence_count = sequences.size();
+    info.total_bases = total_bases;
+    info.text_symbols = text.size();
+    info.suffix_count = sa.Size();
+    info.sa_sampling_rate = sampling_rate;
+    info.ambiguous_bases = ambiguous_bases;
+    info.fingerprint = fingerprint;
+    info.sa_acceleration = acceleration;
+    info.lcp_encoding = lcp.Empty() ? SaLcpEncoding::kNone : SaLcpEncoding::kRaw;
+    info.text_bytes = text.size();
+    info.sa_bytes = sa.Bytes();
+    info.isa_bytes = isa.Bytes();
+    info.lcp_bytes = lcp.Bytes();
+    info.resident_core_bytes = info.text_bytes + info.sa_bytes + info.isa_bytes + info.lcp_bytes;
+  }
+};
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double Since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+ErrorCode FromStatus(int status) {
+  switch (status) {
+    case SFK_HIP_INVALID_INPUT: return ErrorCode::kInvalidInput;
+    case SFK_HIP_IO_ERROR: return ErrorCode::kIoError;
+    case SFK_HIP_UNSUPPORTED_BACKEND: return ErrorCode::kUnsupportedBackend;
+    case SFK_HIP_CORRUPT_INDEX: return ErrorCode::kCorruptIndex;
+    case SFK_HIP_VERSION_MISMATCH: return ErrorCode::kVersionMismatch;
+    default: return ErrorCode::kBuildFailure;
+  }
+}
+
+template <class T>
+detail::CoordinateStorage Store(std::vector<T>&& values, CoordinateStorageWidth width,
+                                std::uint64_t domain, ErrorCode code, const char* label) {
+  if constexpr (std::is_same_v<T, std::uint32_t>) {
+    return detail::CoordinateStorage::FromUInt32(std::move(values), width, domain, code, label);
+  } else {
+    return detail::CoordinateStorage::FromUInt64(std::move(values), width, domain, code, label);
+  }
+}
+
+// Sampled SA (K > 1) from the complete SA and LCP: keep rows whose text
+// position is a multiple of K; the LCP of adjacent kept rows is the minimum
+// complete-SA LCP over the interval between them (suffix_array.cpp:933-966).
+template <class S, class L>
+void SampleSaLcp(std::vector<S>& sa, std::vector<L>* lcp, std::uint32_t k) {
+  std::size_t out = 0;
+  std::uint64_t interval_min = std::numeric_limits<std::uint64_t>::max();
+  for (std::size_t row = 0; row < sa.size(); ++row) {
+    if (lcp != nullptr && row != 0) interval_min = std::min<std::uint64_t>(interval_min, (*lcp)[row]);
+    if (static_cast<std::uint64_t>(sa[row]) % k != 0) continue;
+    sa[out] = sa[row];
+    if (lcp != nullptr) (*lcp)[out] = static_cast<L>(out == 0 ? 0 : interval_min);
+    ++out;
+    interval_min = std::numeric_limits<std::uint64_t>::max();
+  }
+  sa.resize(out);
+  if (lcp != nullptr) lcp->resize(out);
+}
+
+template <class SaT, class IsaT, class LcpT>
+void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& options,
+                  CoordinateStorageWidth stored_width, bool retain_isa, bool retain_lcp,
+                  SuffixArrayBuildStatistics& stats, const std::function<void(const char*)>& notify) {
+  const std::uint64_t n = impl.text.size();
+  std::vector<SaT> sa(n);
+  std::vector<IsaT> isa(retain_isa || options.sampling_rate > 1 ? n : 0);
+  std::vector<LcpT> lcp(retain_lcp ? n : 0);
+  sfk_hip_stats hs{};
+  char err[512] = {0};
+  const std::uint32_t flags = (retain_isa ? SFK_HIP_RETAIN_ISA : 0u) | (retain_lcp ? SFK_HIP_RETAIN_LCP : 0u);
+  const int status = sfk_hip_build(impl.text.data(), n, impl.construction_width, flags, sa.data(),
+                                   8 * sizeof(SaT), isa.empty() ? nullptr : isa.data(), 8 * sizeof(IsaT),
+                                   lcp.empty() ? nullptr : lcp.data(), 8 * sizeof(LcpT), &hs, err,
+                                   sizeof(err));
+  if (status != SFK_HIP_OK) detail::Throw(FromStatus(status), std::string("HIP backend: ") + err);
+  stats.sa_seconds = hs.sa_seconds;
+  stats.isa_seconds = hs.isa_seconds;
+  stats.lcp_seconds = hs.lcp_seconds;
+  stats.device_seconds = hs.device_seconds;
+  stats.transfer_seconds = hs.h2d_seconds + hs.d2h_seconds;
+
+  const std::uint32_t k = options.sampling_rate;
+  if (k > 1) {
+    SampleSaLcp(sa, retain_lcp ? &lcp : nullptr, k);
+    isa.assign(sa.size(), 0);
+    for (std::size_t r = 0; r < sa.size(); ++r) isa[static_cast<std::size_t>(sa[r] / k)] = static_cast<IsaT>(r);
+    if (!retain_isa) std::vector<IsaT>().swap(isa);
+  }
+  const auto compaction = Clock::now();
+  notify("index-storage-layout");
+  const std::uint64_t rows = sa.size();
+  impl.sa = Store(std::move(sa), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
+  stats.storage_compaction_seconds = Since(compaction);
+  if (retain_isa) {
+    notify("index-isa");
+    const auto aux = detail::ResolveAuxiliaryStorageWidth(stored_width, rows);
+    impl.isa = Store(std::move(isa), aux, rows, ErrorCode::kBuildFailure, "inverse suffix array");
+  }
+  if (retain_lcp) {
+    notify("index-lcp-layout");
+    const auto fin = Clock::now();
+    if constexpr (std::is_same_v<LcpT, std::uint32_t>) {
+      impl.lcp = detail::LcpStorage::FromRaw32(std::move(lcp), k, ErrorCode::kBuildFailure);
+    } else {
+      impl.lcp = detail::LcpStorage::FromRaw64(std::move(lcp), k, ErrorCode::kBuildFailure);
+    }
+    stats.lcp_finalize_seconds = Since(fin);
+  }
+}
+
+}  // namespace
+
+SuffixArray::SuffixArray(std::unique_ptr<Impl> impl) : impl_(std::move(impl)) {}
+SuffixArray::SuffixArray(SuffixArray&&) noexcept = default;
+SuffixArray& SuffixArray::operator=(SuffixArray&&) noexcept = default;
+SuffixArray::~SuffixArray() = default;
+
+SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArrayBuildOptions& options) {
+  const auto total_begin = Clock::now();
+  const std::function<void(const char*)> notify = [&](const char* stage) {
+    if (options.stage_callback) options.stage_callback(stage, options.stage_context);
+  };
+  notify("index-text-prepare");
+  if (options.threads == 0) {
+    detail::Throw(ErrorCode::kInvalidInput, "suffix-array thread count must be greater than zero");
+  }
+  if (options.sampling_rate == 0) {
+    detail::Throw(ErrorCode::kInvalidInput, "SA sampling rate must be greater than zero");
+  }
+  if (static_cast<std::uint8_t>(options.acceleration) > static_cast<std::uint8_t>(SaAcceleration::kFull)) {
+    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array acceleration");
+  }
+  SuffixArrayBuildStatistics local_stats;
+  SuffixArrayBuildStatistics& stats = options.statistics ? *options.statistics : local_stats;
+  stats = {};
+  auto impl = std::make_unique<Impl>();
+  const auto& encoded = reference.Encoded();
+  impl->sequences = reference.Sequences();
+  impl->total_bases = reference.TotalBases();
+  impl->ambiguous_bases = reference.AmbiguousBases();
+  impl->fingerprint = reference.Fingerprint();
+  impl->text.reserve(encoded.size() + 1);
+  impl->text.insert(impl->text.end(), encoded.begin(), encoded.end());
+  impl->text.push_back(detail::kSentinel);
+  if (impl->text.size() < 2) detail::Throw(ErrorCode::kInvalidInput, "reference text is empty");
+  const std::uint64_t n = impl->text.size();
+
+  const auto backend = detail::ResolveSaBackend(options.backend, n, options.threads, HipBackendAvailable());
+  if (backend == SaBackend::kDivsufsort || backend == SaBackend::kCaps) {
+    detail::Throw(ErrorCode::kUnsupportedBackend,
+                  "the divsufsort and CaPS-SA engines are not part of this build; use SaBackend::kHip");
+  }
+  if (backend != SaBackend::kHip) detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array backend");
+  if (!HipBackendAvailable()) detail::Throw(ErrorCode::kUnsupportedBackend, "no HIP device is visible");
+  const auto width = detail::ResolveSaCoordinateWidth(backend, options.coordinate_width, n);
+  if (width != CoordinateWidth::kBits32 && width != CoordinateWidth::kBits64) {
+    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array coordinate width");
+  }
+  if (!detail::SaConstructionCanRepresent(backend, width, n)) {
+    detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
+  }
+  const auto stored_width = detail::ResolveProfileStorageWidth(options.storage_width, options.resource_profile, n);
+  if (options.resource_profile == SaResourceProfile::kLowMemory) {
+    detail::Throw(ErrorCode::kUnsupportedBackend,
+                  "the Low-memory profile (byte-coded LCP) is not part of this build");
+  }
+  if (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull) {
+    detail::Throw(ErrorCode::kUnsupportedBackend, "CHILD arrays are not part of this build");
+  }
+  const bool wide_lcp = n - 1U > std::numeric_limits<std::uint32_t>::max();
+  const bool retain_lcp = options.acceleration != SaAcceleration::kNone;
+  const bool retain_isa = options.acceleration == SaAcceleration::kLcpSuffixLink;
+  impl->construction_width = static_cast<std::uint8_t>(width);
+  impl->backend = width == CoordinateWidth::kBits32 ? detail::StoredBackend::kHip32 : detail::StoredBackend::kHip64;
+  impl->sampling_rate = options.sampling_rate;
+  impl->acceleration = options.acceleration;
+  stats.text_prepare_seconds = Since(total_begin);
+
+  notify("index-sa-lcp");
+  const bool narrow_sa = stored_width == CoordinateStorageWidth::kBits32;
+  const std::uint64_t rows = 1 + (n - 1) / options.sampling_rate;
+  const bool narrow_isa =
+      detail::ResolveAuxiliaryStorageWidth(stored_width, rows) == CoordinateStorageWidth::kBits32;
+  // Device outputs already in the resident widths: SA 32 or 64 (40/48 are
+  // split on the host from 64), ISA in the auxiliary width, LCP 32/64.
+  if (narrow_sa && narrow_isa && !wide_lcp) {
+    BuildWithHip<std::uint32_t, std::uint32_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
+                                                              retain_lcp, stats, notify);
+  } else if (narrow_sa && !wide_lcp) {
+    BuildWithHip<std::uint32_t, std::uint64_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
+                                                              retain_lcp, stats, notify);
+  } else if (!wide_lcp) {
+    BuildWithHip<std::uint64_t, std::uint64_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
+                                                              retain_lcp, stats, notify);
+  } else {
+    BuildWithHip<std::uint64_t, std::uint64_t, std::uint64_t>(*impl, options, stored_width, retain_isa,
+                                                              retain_lcp, stats, notify);
+  }
+  impl->RefreshInfo();
+  impl->info.sa_resource_profile = options.resource_profile;
+  notify("index-prefix-directory");
+  stats.total_seconds = Since(total_begin);
+  notify("index-build-return");
+  return SuffixArray(std::move(impl));
+}
+
+void SuffixArray::Save(const std::filesystem::path& path, const SaveOptions& options) const {
+  detail::ContainerSpec spec;
+  spec.format_minor = 4;
+  spec.kind = 1;
+  spec.backend = impl_->backend;
+  spec.coordinate_width = impl_->construction_width;
+  spec.profile = impl_->info.sa_resource_profile;
+  spec.sequence_count = impl_->sequences.size();
+  spec.total_bases = impl_->total_bases;
+  spec.text_symbols = impl_->text.size();
+  spec.ambiguous_bases = impl_->ambiguous_bases;
+  spec.fingerprint = impl_->fingerprint;
+  const Impl& impl = *impl_;
+  std::vector<detail::SectionWriter> writers{
+      {detail::SectionType::kMetadata,
+       [&](std::ostream& out) {
+         // serialization.cpp:785-799
+         detail::WriteU32(out, static_cast<std::uint32_t>(impl.sequences.size()));
+         for (const auto& s : impl.sequences) {
+           detail::WriteU32(out, s.id);
+           detail::WriteU64(out, s.global_offset);
+           detail::WriteU64(out, s.length);
+           detail::WriteU64(out, s.ambiguous_bases);
+           detail::WriteU32(out, static_cast<std::uint32_t>(s.name.size()));
+           out.write(s.name.data(), static_cast<std::streamsize>(s.name.size()));
+           detail::WriteU32(out, static_cast<std::uint32_t>(s.description.size()));
+           out.write(s.description.data(), static_cast<std::streamsize>(s.description.size()));
+         }
+       }},
+      {detail::SectionType::kText,
+       [&](std::ostream& out) {
+         out.write(reinterpret_cast<const char*>(impl.text.data()), static_cast<std::streamsize>(impl.text.size()));
+       }},
+      {detail::SectionType::kSuffixArray,
+       [&](std::ostream& out) { detail::WriteCoordinateSectionV14(out, impl.sa, impl.text.size()); }}};
+  if (!impl.isa.Empty()) {
+    writers.push_back({detail::SectionType::kInverseSuffixArray, [&](std::ostream& out) {
+                         detail::WriteCoordinateSectionV14(out, impl.isa, impl.sa.Size());
+                       }});
+  }
+  if (!impl.lcp.Empty()) {
+    writers.push_back({detail::SectionType::kLcp, [&](std::ostream& out) {
+                         detail::WriteLcpSectionV14(out, impl.lcp, impl.text.size());
+                       }});
+  }
+  if (impl.sampling_rate > 1) {
+    writers.push_back({detail::SectionType::kSaSampling, [&](std::ostream& out) {
+                         detail::WriteU32(out, impl.sampling_rate);
+                         detail::WriteU64(out, impl.sa.Size());
+                       }});
+  }
+  detail::WriteContainer(path, options, spec, writers);
+}
+
+namespace {
+
+struct ByteReader {
+  const std::vector<std::uint8_t>& bytes;
+  std::size_t pos = 0;
+  void Need(std::size_t n, const char* what) const {
+    if (bytes.size() - pos < n) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " is truncated");
+  }
+  std::uint32_t U32(const char* what) {
+    Need(4, what);
+    std::uint32_t v = 0;
+    for (int i = 0; i < 4; ++i) v |= static_cast<std::uint32_t>(bytes[pos + i]) << (8 * i);
+    pos += 4;
+    return v;
+  }
+  std::uint64_t U64(const char* what) {
+    Need(8, what);
+    std::uint64_t v = 0;
+    for (int i = 0; i < 8; ++i) v |= static_cast<std::uint64_t>(bytes[pos + i]) << (8 * i);
+    pos += 8;
+    return v;
+  }
+  std::string Str(const char* what) {
+    const auto len = U32(what);
+    Need(len, what);
+    std::string s(reinterpret_cast<const char*>(bytes.data() + pos), len);
+    pos += len;
+    return s;
+  }
+};
+
+// Legacy (format 1.0-1.3) coordinate sections: u64 count [+ u8 width] +
+// values (suffix_array.cpp:1778-1830).
+std::vector<std::uint64_t> ReadLegacyValues(const std::vector<std::uint8_t>& bytes, bool has_width,
+                                            std::uint8_t default_width, const char* what) {
+  ByteReader r{bytes};
+  const std::uint64_t count = r.U64(what);
+  std::uint8_t width = default_width;
+  if (has_width) {
+    r.Need(1, what);
+    width = bytes[r.pos++];
+  }
+  if (width != 32 && width != 64) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " has an invalid width");
+  const std::size_t w = width / 8;
+  if ((bytes.size() - r.pos) != count * w) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " size mismatch");
+  std::vector<std::uint64_t> out(count);
+  for (std::uint64_t i = 0; i < count; ++i) {
+    std::uint64_t v = 0;
+    for (std::size_t b = 0; b < w; ++b) v |= static_cast<std::uint64_t>(bytes[r.pos + i * w + b]) << (8 * b);
+    out[i] = v;
+  }
+  return out;
+}
+
+}  // namespace
+
+SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
+  const auto container = detail::ReadContainer(path);
+  if (container.spec.kind != 1) detail::Throw(ErrorCode::kCorruptIndex, "index does not contain a suffix array");
+  const auto backend_id = static_cast<std::uint8_t>(container.spec.backend);
+  if (backend_id < 1 || backend_id > 6) {
+    detail::Throw(ErrorCode::kUnsupportedBackend, "unsupported suffix-array payload");
+  }
+  auto impl = std::make_unique<Impl>();
+  impl->backend = container.spec.backend;
+  impl->construction_width = container.spec.coordinate_width;
+  impl->fingerprint = container.spec.fingerprint;
+
+  // Metadata (serialization.cpp:801-870).
+  {
+    const auto bytes = detail::ReadSection(container, detail::SectionType::kMetadata);
+    ByteReader r{bytes};
+    const auto count = r.U32("metadata sequence count");
+    if (count == 0 || count != container.spec.sequence_count) {
+      detail::Throw(ErrorCode::kCorruptIndex, "metadata sequence count mismatch");
+    }
+    std::uint64_t expected_offset = 0;
+    for (std::uint32_t i = 0; i < count; ++i) {
+      SequenceInfo s;
+      s.id = r.U32("sequence id");
+      s.global_offset = r.U64("sequence global offset");
+      s.length = r.U64("sequence length");
+      s.ambiguous_bases = r.U64("sequence ambiguous bases");
+      s.name = r.Str("sequence name");
+      s.description = r.Str("sequence description");
+      if (s.id != i || s.name.empty() || s.length == 0 || s.global_offset != expected_offset ||
+          s.ambiguous_bases > s.length) {
+        detail::Throw(ErrorCode::kCorruptIndex, "invalid sequence metadata");
+      }
+      expected_offset += s.length + 1;
+      impl->total_bases += s.length;
+      impl->ambiguous_bases += s.ambiguous_bases;
+      impl->sequences.push_back(std::move(s));
+    }
+    if (r.pos != bytes.size()) detail::Throw(ErrorCode::kCorruptIndex, "metadata section has trailing bytes");
+    if (impl->total_bases != container.spec.total_bases ||
+        impl->ambiguous_bases != container.spec.ambiguous_bases ||
+        expected_offset + 1 != container.spec.text_symbols) {
+      detail::Throw(ErrorCode::kCorruptIndex, "metadata totals do not match the index header");
+    }
+  }
+  // Text: sentinel, separators, fingerprint (suffix_array.cpp:4945-4969).
+  impl->text = detail::ReadSection(container, detail::SectionType::kText);
+  const std::uint64_t n = impl->text.size();
+  if (n != container.spec.text_symbols || n < 2 || impl->text.back() != detail::kSentinel ||
+      std::find(impl->text.begin(), impl->text.end() - 1, detail::kSentinel) != impl->text.end() - 1) {
+    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text has an invalid sentinel");
+  }
+  for (const auto& s : impl->sequences) {
+    const auto sep = s.global_offset + s.length;
+    if (sep >= n || impl->text[sep] != detail::kSeparator) {
+      detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text has an invalid contig separator");
+    }
+  }
+  if (detail::ContentFingerprint(impl->text.data(), n - 1) != impl->fingerprint) {
+    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text fingerprint mismatch");
+  }
+  if (const auto* sampling = container.Find(detail::SectionType::kSaSampling)) {
+    (void)sampling;
+    const auto bytes = detail::ReadSection(container, detail::SectionType::kSaSampling);
+    ByteReader r{bytes};
+    impl->sampling_rate = r.U32("SA sampling rate");
+    const auto recorded = r.U64("sampled suffix count");
+    if (impl->sampling_rate <= 1 || recorded != 1 + (n - 1) / impl->sampling_rate || r.pos != bytes.size()) {
+      detail::Throw(ErrorCode::kCorruptIndex, "invalid sampled SA metadata");
+    }
+  }
+  const std::uint32_t k = impl->sampling_rate;
+  const std::uint64_t count = 1 + (n - 1) / k;
+  const bool backend_is_32 = backend_id == 1 || backend_id == 3 || backend_id == 5;
+  if (container.spec.coordinate_width != (backend_is_32 ? 32 : 64)) {
+    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array backend and construction width disagree");
+  }
+  const bool v14 = container.spec.format_minor >= 4;
+  // SA section + sampled-permutation check (suffix_array.cpp:4999-5010).
+  {
+    const auto& d = container.Require(detail::SectionType::kSuffixArray);
+    if (v14) {
+      std::ifstream in(path, std::ios::binary);
+      in.seekg(static_cast<std::streamoff>(d.offset));
+      impl->sa = detail::ReadCoordinateSectionV14(in, d.size, count, n, "suffix array");
+    } else {
+      auto values = ReadLegacyValues(detail::ReadSection(container, detail::SectionType::kSuffixArray), false,
+                                     container.spec.coordinate_width, "suffix array");
+      if (values.size() != count) detail::Throw(ErrorCode::kCorruptIndex, "suffix-array length mismatch");
+      impl->sa = detail::CoordinateStorage::FromUInt64(
+          std::move(values), backend_is_32 ? CoordinateStorageWidth::kBits32 : CoordinateStorageWidth::kBits64, n,
+          ErrorCode::kCorruptIndex, "suffix array");
+    }
+    std::vector<bool> seen(count, false);
+    for (std::uint64_t r = 0; r < count; ++r) {
+      const auto v = impl->sa.At(r);
+      if (v >= n || v % k != 0) detail::Throw(ErrorCode::kCorruptIndex, "suffix array contains a non-sampled position");
+      if (seen[v / k]) detail::Throw(ErrorCode::kCorruptIndex, "suffix array is not a sampled permutation");
+      seen[v / k] = true;
+    }
+  }
+  const auto* isa_d = container.Find(detail::SectionType::kInverseSuffixArray);
+  const auto* lcp_d = container.Find(detail::SectionType::kLcp);
+  if (isa_d != nullptr) {
+    if (v14) {
+      std::ifstream in(path, std::ios::binary);
+      in.seekg(static_cast<std::streamoff>(isa_d->offset));
+      impl->isa = detail::ReadCoordinateSectionV14(in, isa_d->size, count, count, "inverse suffix array");
+    } else {
+      auto values = ReadLegacyValues(detail::ReadSection(container, detail::SectionType::kInverseSuffixArray),
+                                     true, 64, "inverse suffix array");
+      if (values.size() != count) detail::Throw(ErrorCode::kCorruptIndex, "ISA length mismatch");
+      impl->isa = detail::CoordinateStorage::FromUInt64(std::move(values), CoordinateStorageWidth::kAutoSelect,
+                                                        count, ErrorCode::kCorruptIndex, "inverse suffix array");
+    }
+    for (std::uint64_t r = 0; r < count; ++r) {  // suffix_array.cpp:5120-5131
+      if (impl->isa.At(impl->sa.At(r) / k) != r) {
+        detail::Throw(ErrorCode::kCorruptIndex, "inverse suffix array does not invert the suffix array");
+      }
+    }
+  }
+  if (lcp_d != nullptr) {
+    if (v14) {
+      std::ifstream in(path, std::ios::binary);
+      in.seekg(static_cast<std::streamoff>(lcp_d->offset));
+      impl->lcp = detail::ReadLcpSectionV14(in, lcp_d->size, count, k, n);
+    } else {
+      auto values = ReadLegacyValues(detail::ReadSection(container, detail::SectionType::kLcp), true, 64, "LCP");
+      if (values.size() != count) detail::Throw(ErrorCode::kCorruptIndex, "LCP length mismatch");
+      impl->lcp = detail::LcpStorage::FromRaw64(std::move(values), k, ErrorCode::kCorruptIndex);
+    }
+    impl->lcp.Validate(impl->sa, n, ErrorCode::kCorruptIndex);
+  }
+  impl->acceleration = lcp_d == nullptr ? SaAcceleration::kNone
+                                        : (isa_d != nullptr ? SaAcceleration::kLcpSuffixLink : SaAcceleration::kLcp);
+  impl->RefreshInfo();
+  impl->info.sa_resource_profile = container.spec.profile;
+  impl->info.format_version = "1." + std::to_string(container.spec.format_minor);
+  impl->info.library_version = std::to_string(container.spec.library_major) + "." +
+                               std::to_string(container.spec.library_minor) + "." +
+                               std::to_string(container.spec.library_patch);
+  impl->info.serialized_bytes = container.file_size;
+  return SuffixArray(std::move(impl));
+}
+
+const IndexInfo& SuffixArray::Info() const noexcept { return impl_->info; }
+std::uint64_t SuffixArray::Size() const noexcept { return impl_->sa.Size(); }
+std::uint64_t SuffixArray::TextSymbols() const noexcept { return impl_->text.size(); }
+Position SuffixArray::SuffixAt(std::uint64_t row) const {
+  if (row >= impl_->sa.Size()) detail::Throw(ErrorCode::kInvalidInput, "suffix-array row out of range");
+  return impl_->sa.At(row);
+}
+bool SuffixArray::HasIsa() const noexcept { return !impl_->isa.Empty(); }
+bool SuffixArray::HasLcp() const noexcept { return !impl_->lcp.Empty(); }
+std::uint64_t SuffixArray::IsaAt(std::uint64_t sample) const {
+  if (sample >= impl_->isa.Size()) detail::Throw(ErrorCode::kInvalidInput, "ISA index out of range");
+  return impl_->isa.At(sample);
+}
+std::uint64_t SuffixArray::LcpAt(std::uint64_t row) const {
+  if (row >= impl_->lcp.Size()) detail::Throw(ErrorCode::kInvalidInput, "LCP row out of range");
+  return impl_->lcp.At(row);
+}
+const std::vector<std::uint8_t>& SuffixArray::Text() const noexcept { return impl_->text; }
+
+std::uint64_t SuffixArray::Count(std::string_view pattern) const {
+  if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "pattern must not be empty");
+  if (impl_->sampling_rate != 1) {
+    detail::Throw(ErrorCode::kUnsupportedBackend, "count over a sampled suffix array is not part of this build");
+  }
+  std::vector<std::uint8_t> p(pattern.size());
+  for (std::size_t i = 0; i < pattern.size(); ++i) {
+    switch (pattern[i]) {
+      case 'A': case 'a': p[i] = detail::kA; break;
+      case 'C': case 'c': p[i] = detail::kC; break;
+      case 'G': case 'g': p[i] = detail::kG; break;
+      case 'T': case 't': p[i] = detail::kT; break;
+      default: detail::Throw(ErrorCode::kInvalidInput, "pattern contains a non-ACGT character");
+    }
+  }
+  const auto& text = impl_->text;
+  const std::uint64_t n = text.size();
+  // compare suffix at position s with the pattern: <0, 0 (prefix match), >0
+  auto cmp = [&](std::uint64_t s) {
+    for (std::size_t i = 0; i < p.size(); ++i) {
+      if (s + i >= n) return -1;
+      if (text[s + i] != p[i]) return text[s + i] < p[i] ? -1 : 1;
+    }
+    return 0;
+  };
+  std::uint64_t lo = 0, hi = impl_->sa.Size();
+  while (lo < hi) {  // first row with suffix >= pattern
+    const auto mid = lo + (hi - lo) / 2;
+    if (cmp(impl_->sa.At(mid)) < 0) lo = mid + 1; else hi = mid;
+  }
+  const auto first = lo;
+  hi = impl_->sa.Size();
+  while (lo < hi) {  // first row with suffix > pattern (prefix matches count as equal)
+    const auto mid = lo + (hi - lo) / 2;
+    if (cmp(impl_->sa.At(mid)) <= 0) lo = mid + 1; else hi = mid;
+  }
+  return lo - first;
+}
+
+}  // namespace sufkit

@@ -1,0 +1,579 @@
+// .sufidx 1.4 persistence: SKC1 coordinate and SKL1 LCP section codecs
+// (sa_codec.cpp:466-690) and the CRC-protected outer container with atomic,
+// no-clobber publication (serialization.cpp:246-362, 529-765).  All integers
+// are little-endian.
+#include <fcntl.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <fstream>
+#include <istream>
+#include <ostream>
+#include <set>
+#include <system_error>
+
+#include "internal.hpp"
+
+namespace sufkit::detail {
+namespace {
+
+constexpr std::uint32_t kCoordinateMagic = 0x31434b53U;  // "SKC1"
+constexpr std::uint32_t kLcpMagic = 0x314c4b53U;         // "SKL1"
+constexpr std::uint32_t kCodecVersion = 1;
+constexpr std::uint32_t kLcpGuideBlockShift = 12;
+constexpr std::uint64_t kCoordinateHeaderBytes = 56;
+constexpr std::uint64_t kLcpHeaderBytes = 80;
+constexpr std::array<std::uint8_t, 8> kMagic{{'S', 'U', 'F', 'K', 'I', 'D', 'X', 0}};
+constexpr std::size_t kBaseHeaderSize = 80;
+constexpr std::size_t kSectionEntrySize = 32;
+constexpr std::size_t kHeaderCrcOffset = 72;
+constexpr std::uint32_t kMaxSections = 16;
+
+enum class CoordinateCodec : std::uint32_t { kNative32 = 0, kNative64 = 1, kSplit40 = 2, kSplit48 = 3 };
+
+std::atomic<std::uint64_t> g_temp_counter{0};
+
+void WriteBytes(std::ostream& out, const void* data, std::size_t size) {
+  out.write(static_cast<const char*>(data), static_cast<std::streamsize>(size));
+  if (!out) Throw(ErrorCode::kIoError, "cannot write index section");
+}
+
+template <class T>
+void WritePlane(std::ostream& out, const std::vector<T>& values) {
+  // Little-endian host: the in-memory layout is the file layout.
+  constexpr std::size_t kBlock = 1u << 18;
+  for (std::size_t begin = 0; begin < values.size(); begin += kBlock) {
+    const std::size_t count = std::min(kBlock, values.size() - begin);
+    WriteBytes(out, values.data() + begin, count * sizeof(T));
+  }
+}
+
+void ReadExact(std::istream& in, void* data, std::size_t size, const char* what) {
+  in.read(static_cast<char*>(data), static_cast<std::streamsize>(size));
+  if (!in || static_cast<std::size_t>(in.gcount()) != size) {
+    Throw(ErrorCode::kCorruptIndex, std::string("truncated ") + what);
+  }
+}
+
+std::uint32_t ReadU32(std::istream& in, const char* what) {
+  std::uint8_t b[4];
+  ReadExact(in, b, 4, what);
+  return static_cast<std::uint32_t>(b[0]) | (static_cast<std::uint32_t>(b[1]) << 8) |
+         (static_cast<std::uint32_t>(b[2]) << 16) | (static_cast<std::uint32_t>(b[3]) << 24);
+}
+
+std::uint64_t ReadU64(std::istream& in, const char* what) {
+  std::uint8_t b[8];
+  ReadExact(in, b, 8, what);
+  std::uint64_t v = 0;
+  for (int i = 0; i < 8; ++i) v |= static_cast<std::uint64_t>(b[i]) << (8 * i);
+  return v;
+}
+
+template <class T>
+std::vector<T> ReadPlane(std::istream& in, std::uint64_t count, const char* what) {
+  std::vector<T> values;
+  try {
+    values.resize(static_cast<std::size_t>(count));
+  } catch (const std::exception&) {
+    Throw(ErrorCode::kCorruptIndex, std::string("cannot allocate ") + what);
+  }
+  if (count) ReadExact(in, values.data(), static_cast<std::size_t>(count) * sizeof(T), what);
+  return values;
+}
+
+std::uint64_t CheckedMul(std::uint64_t a, std::uint64_t b, ErrorCode code) {
+  if (b != 0 && a > std::numeric_limits<std::uint64_t>::max() / b) Throw(code, "byte count overflows");
+  return a * b;
+}
+
+void AppendU16(std::vector<std::uint8_t>& h, std::uint16_t v) {
+  h.push_back(static_cast<std::uint8_t>(v));
+  h.push_back(static_cast<std::uint8_t>(v >> 8));
+}
+void AppendU32(std::vector<std::uint8_t>& h, std::uint32_t v) {
+  for (int i = 0; i < 4; ++i) h.push_back(static_cast<std::uint8_t>(v >> (8 * i)));
+}
+void AppendU64(std::vector<std::uint8_t>& h, std::uint64_t v) {
+  for (int i = 0; i < 8; ++i) h.push_back(static_cast<std::uint8_t>(v >> (8 * i)));
+}
+std::uint16_t GetU16(const std::uint8_t* p) { return static_cast<std::uint16_t>(p[0] | (p[1] << 8)); }
+std::uint32_t GetU32(const std::uint8_t* p) {
+  return static_cast<std::uint32_t>(p[0]) | (static_cast<std::uint32_t>(p[1]) << 8) |
+         (static_cast<std::uint32_t>(p[2]) << 16) | (static_cast<std::uint32_t>(p[3]) << 24);
+}
+std::uint64_t GetU64(const std::uint8_t* p) {
+  std::uint64_t v = 0;
+  for (int i = 0; i < 8; ++i) v |= static_cast<std::uint64_t>(p[i]) << (8 * i);
+  return v;
+}
+
+// Stream buffer that forwards to a file buffer and accumulates the CRC32 of
+// everything written, so no payload is reread to fill the section table.
+class CrcBuffer : public std::streambuf {
+ public:
+  explicit CrcBuffer(std::streambuf* sink) : sink_(sink) { setp(buffer_, buffer_ + sizeof(buffer_)); }
+  bool Finish() { return Flush() == 0; }
+  std::uint32_t Crc() const { return crc_; }
+
+ protected:
+  int_type overflow(int_type ch) override {
+    if (Flush() != 0) return traits_type::eof();
+    if (!traits_type::eq_int_type(ch, traits_type::eof())) {
+      *pptr() = traits_type::to_char_type(ch);
+      pbump(1);
+    }
+    return traits_type::not_eof(ch);
+  }
+  int sync() override { return Flush(); }
+  std::streamsize xsputn(const char* s, std::streamsize n) override {
+    if (n >= static_cast<std::streamsize>(sizeof(buffer_))) {
+      if (Flush() != 0) return 0;
+      crc_ = Crc32(crc_, s, static_cast<std::size_t>(n));
+      return sink_->sputn(s, n);
+    }
+    return std::streambuf::xsputn(s, n);
+  }
+
+ private:
+  int Flush() {
+    const std::ptrdiff_t n = pptr() - pbase();
+    if (n > 0) {
+      crc_ = Crc32(crc_, pbase(), static_cast<std::size_t>(n));
+      if (sink_->sputn(pbase(), n) != n) return -1;
+      pbump(static_cast<int>(-n));
+    }
+    return 0;
+  }
+  std::streambuf* sink_;
+  char buffer_[1 << 16];
+  std::uint32_t crc_ = 0;
+};
+
+std::filesystem::path TemporaryPathFor(const std::filesystem::path& target) {
+  for (unsigned attempt = 0; attempt < 100; ++attempt) {
+    const auto counter = g_temp_counter.fetch_add(1, std::memory_order_relaxed);
+    const auto ticks = static_cast<std::uint64_t>(
+        std::chrono::steady_clock::now().time_since_epoch().count());
+    const std::filesystem::path temp(target.string() + ".partial." +
+                                     std::to_string(static_cast<long long>(getpid())) + "." +
+                                     std::to_string(ticks ^ counter));
+    const int fd = open(temp.c_str(), O_CREAT | O_EXCL | O_RDWR | O_CLOEXEC, 0666);
+    if (fd >= 0) {
+      close(fd);
+      return temp;
+    }
+    if (errno != EEXIST) {
+      Throw(ErrorCode::kIoError, "cannot create temporary index: " +
+                                     std::error_code(errno, std::generic_category()).message());
+    }
+  }
+  Throw(ErrorCode::kIoError, "cannot allocate a unique temporary index path");
+}
+
+void Publish(const std::filesystem::path& temp, const std::filesystem::path& target, bool overwrite) {
+  if (overwrite) {
+    std::filesystem::rename(temp, target);
+    return;
+  }
+  // Same-directory hard link: atomic publication that never replaces a
+  // target created concurrently by another writer.
+  if (link(temp.c_str(), target.c_str()) != 0) {
+    if (errno == EEXIST) Throw(ErrorCode::kIoError, "index already exists: " + target.string());
+    Throw(ErrorCode::kIoError, "cannot publish index: " +
+                                   std::error_code(errno, std::generic_category()).message());
+  }
+  std::error_code ignored;
+  std::filesystem::remove(temp, ignored);
+}
+
+}  // namespace
+
+std::uint32_t Crc32(std::uint32_t crc, const void* data, std::size_t size) {
+  const auto* p = static_cast<const unsigned char*>(data);
+  uLong c = crc;
+  while (size > 0) {
+    const auto chunk = static_cast<uInt>(std::min<std::size_t>(size, 1u << 30));
+    c = crc32(c, p, chunk);
+    p += chunk;
+    size -= chunk;
+  }
+  return static_cast<std::uint32_t>(c);
+}
+
+void WriteU32(std::ostream& out, std::uint32_t v) {
+  std::uint8_t b[4];
+  for (int i = 0; i < 4; ++i) b[i] = static_cast<std::uint8_t>(v >> (8 * i));
+  WriteBytes(out, b, 4);
+}
+
+void WriteU64(std::ostream& out, std::uint64_t v) {
+  std::uint8_t b[8];
+  for (int i = 0; i < 8; ++i) b[i] = static_cast<std::uint8_t>(v >> (8 * i));
+  WriteBytes(out, b, 8);
+}
+
+// ------------------------------------------------------------- codecs ---
+
+void WriteCoordinateSectionV14(std::ostream& out, const CoordinateStorage& storage,
+                               std::uint64_t symbol_count) {
+  (void)ResolveCoordinateStorageWidth(storage.Width(), symbol_count, ErrorCode::kBuildFailure);
+  const std::uint64_t count = storage.Size();
+  CoordinateCodec codec = CoordinateCodec::kNative32;
+  std::uint64_t low = 0, high = 0;
+  switch (storage.Width()) {
+    case CoordinateStorageWidth::kBits32: codec = CoordinateCodec::kNative32; low = count * 4; break;
+    case CoordinateStorageWidth::kBits64: codec = CoordinateCodec::kNative64; low = count * 8; break;
+    case CoordinateStorageWidth::kBits40: codec = CoordinateCodec::kSplit40; low = count * 4; high = count; break;
+    case CoordinateStorageWidth::kBits48: codec = CoordinateCodec::kSplit48; low = count * 4; high = count * 2; break;
+    default: Throw(ErrorCode::kBuildFailure, "coordinate storage width is not resolved");
+  }
+  WriteU32(out, kCoordinateMagic);
+  WriteU32(out, kCodecVersion);
+  WriteU32(out, static_cast<std::uint32_t>(codec));
+  WriteU32(out, 0);
+  WriteU64(out, symbol_count);
+  WriteU64(out, count);
+  WriteU64(out, low);
+  WriteU64(out, high);
+  WriteU64(out, 0);
+  const auto& raw = storage.Raw();
+  switch (raw.index()) {
+    case 1: WritePlane(out, std::get<1>(raw)); break;
+    case 2: WritePlane(out, std::get<2>(raw)); break;
+    case 3: WritePlane(out, std::get<3>(raw).low); WritePlane(out, std::get<3>(raw).high); break;
+    case 4: WritePlane(out, std::get<4>(raw).low); WritePlane(out, std::get<4>(raw).high); break;
+    default: break;
+  }
+}
+
+CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t section_size,
+                                           std::uint64_t expected_elements,
+                                           std::uint64_t expected_symbols, const char* label) {
+  if (ReadU32(in, "coordinate codec magic") != kCoordinateMagic) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: wrong coordinate codec magic");
+  }
+  if (ReadU32(in, "coordinate codec version") != kCodecVersion) {
+    Throw(ErrorCode::kVersionMismatch, "SA section codec: unsupported coordinate codec header version");
+  }
+  const std::uint32_t raw_codec = ReadU32(in, "coordinate codec identifier");
+  if (raw_codec > 3) Throw(ErrorCode::kVersionMismatch, "SA section codec: unknown coordinate codec identifier");
+  if (ReadU32(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: reserved field is non-zero");
+  const std::uint64_t symbols = ReadU64(in, "coordinate symbol count");
+  const std::uint64_t count = ReadU64(in, "coordinate element count");
+  const std::uint64_t low = ReadU64(in, "coordinate low-plane size");
+  const std::uint64_t high = ReadU64(in, "coordinate high-plane size");
+  if (ReadU64(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: reserved field is non-zero");
+  const auto codec = static_cast<CoordinateCodec>(raw_codec);
+  std::uint64_t want_low = 0, want_high = 0;
+  CoordinateStorageWidth width = CoordinateStorageWidth::kBits32;
+  switch (codec) {
+    case CoordinateCodec::kNative32: want_low = CheckedMul(count, 4, ErrorCode::kCorruptIndex); width = CoordinateStorageWidth::kBits32; break;
+    case CoordinateCodec::kNative64: want_low = CheckedMul(count, 8, ErrorCode::kCorruptIndex); width = CoordinateStorageWidth::kBits64; break;
+    case CoordinateCodec::kSplit40: want_low = CheckedMul(count, 4, ErrorCode::kCorruptIndex); want_high = count; width = CoordinateStorageWidth::kBits40; break;
+    case CoordinateCodec::kSplit48: want_low = CheckedMul(count, 4, ErrorCode::kCorruptIndex); want_high = CheckedMul(count, 2, ErrorCode::kCorruptIndex); width = CoordinateStorageWidth::kBits48; break;
+  }
+  if (symbols == 0 || low != want_low || high != want_high ||
+      kCoordinateHeaderBytes + low + high != section_size) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: coordinate plane lengths disagree with the codec");
+  }
+  if (count != expected_elements || symbols != expected_symbols) {
+    Throw(ErrorCode::kCorruptIndex, std::string("SA section codec: ") + label +
+                                        " metadata disagrees with the container");
+  }
+  switch (codec) {
+    case CoordinateCodec::kNative32:
+      return CoordinateStorage::FromUInt32(ReadPlane<std::uint32_t>(in, count, label), width, symbols,
+                                           ErrorCode::kCorruptIndex, label);
+    case CoordinateCodec::kNative64:
+      return CoordinateStorage::FromUInt64(ReadPlane<std::uint64_t>(in, count, label), width, symbols,
+                                           ErrorCode::kCorruptIndex, label);
+    case CoordinateCodec::kSplit40: {
+      Coordinate40 v;
+      v.low = ReadPlane<std::uint32_t>(in, count, label);
+      v.high = ReadPlane<std::uint8_t>(in, count, label);
+      return CoordinateStorage::FromPacked40(std::move(v), symbols, ErrorCode::kCorruptIndex, label);
+    }
+    case CoordinateCodec::kSplit48: {
+      Coordinate48 v;
+      v.low = ReadPlane<std::uint32_t>(in, count, label);
+      v.high = ReadPlane<std::uint16_t>(in, count, label);
+      return CoordinateStorage::FromPacked48(std::move(v), symbols, ErrorCode::kCorruptIndex, label);
+    }
+  }
+  Throw(ErrorCode::kCorruptIndex, "SA section codec: unreachable");
+}
+
+void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t text_symbols) {
+  if (lcp.Empty()) Throw(ErrorCode::kBuildFailure, "SA section codec: cannot serialize an empty LCP");
+  if (text_symbols == 0 || lcp.Size() != 1 + (text_symbols - 1) / lcp.SamplingRate()) {
+    Throw(ErrorCode::kBuildFailure, "SA section codec: LCP metadata is inconsistent before serialization");
+  }
+  WriteU32(out, kLcpMagic);
+  WriteU32(out, kCodecVersion);
+  WriteU32(out, 0);  // raw
+  WriteU32(out, lcp.Width());
+  WriteU32(out, lcp.SamplingRate());
+  WriteU32(out, kLcpGuideBlockShift);
+  WriteU64(out, text_symbols);
+  WriteU64(out, lcp.Size());
+  WriteU64(out, lcp.Size() * (lcp.Width() / 8u));
+  WriteU64(out, 0);
+  WriteU64(out, 0);
+  WriteU64(out, 0);
+  WriteU64(out, 0);
+  if (lcp.Width() == 32) {
+    WritePlane(out, lcp.Raw32());
+  } else {
+    WritePlane(out, lcp.Raw64());
+  }
+}
+
+LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size, std::uint64_t expected_rows,
+                             std::uint32_t sampling_rate, std::uint64_t text_symbols) {
+  if (ReadU32(in, "LCP codec magic") != kLcpMagic) Throw(ErrorCode::kCorruptIndex, "SA section codec: wrong LCP codec magic");
+  if (ReadU32(in, "LCP codec version") != kCodecVersion) {
+    Throw(ErrorCode::kVersionMismatch, "SA section codec: unsupported LCP codec header version");
+  }
+  const std::uint32_t codec = ReadU32(in, "LCP codec identifier");
+  if (codec > 1) Throw(ErrorCode::kVersionMismatch, "SA section codec: unknown LCP codec identifier");
+  if (codec == 1) {
+    Throw(ErrorCode::kUnsupportedBackend, "byte-coded (Low-memory) LCP sections are not supported by this build");
+  }
+  const std::uint32_t width = ReadU32(in, "LCP coordinate width");
+  const std::uint32_t rate = ReadU32(in, "LCP sampling rate");
+  if (ReadU32(in, "LCP guide block shift") != kLcpGuideBlockShift) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: invalid LCP guide block shift");
+  }
+  const std::uint64_t symbols = ReadU64(in, "LCP text-symbol count");
+  const std::uint64_t rows = ReadU64(in, "LCP row count");
+  const std::uint64_t primary = ReadU64(in, "LCP primary-plane size");
+  const std::uint64_t anchors = ReadU64(in, "LCP anchor count");
+  const std::uint64_t anchor_pos = ReadU64(in, "LCP anchor-position size");
+  const std::uint64_t anchor_val = ReadU64(in, "LCP anchor-value size");
+  if (ReadU64(in, "LCP reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP codec reserved field is non-zero");
+  if (rate == 0 || (width != 32 && width != 64) || symbols == 0) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: invalid LCP header");
+  }
+  if (rows != 1 + (symbols - 1) / rate || primary != CheckedMul(rows, width / 8u, ErrorCode::kCorruptIndex) ||
+      anchors != 0 || anchor_pos != 0 || anchor_val != 0 || kLcpHeaderBytes + primary != section_size) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: raw LCP payload lengths are inconsistent");
+  }
+  if (rows != expected_rows || rate != sampling_rate || symbols != text_symbols) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP metadata disagrees with the container");
+  }
+  if (width == 32) return LcpStorage::FromRaw32(ReadPlane<std::uint32_t>(in, rows, "raw32 LCP"), rate, ErrorCode::kCorruptIndex);
+  return LcpStorage::FromRaw64(ReadPlane<std::uint64_t>(in, rows, "raw64 LCP"), rate, ErrorCode::kCorruptIndex);
+}
+
+// ---------------------------------------------------------- container ---
+
+const SectionDescriptor* ParsedContainer::Find(SectionType type) const noexcept {
+  for (const auto& s : sections) {
+    if (s.type == type) return &s;
+  }
+  return nullptr;
+}
+
+const SectionDescriptor& ParsedContainer::Require(SectionType type) const {
+  const auto* s = Find(type);
+  if (s == nullptr) Throw(ErrorCode::kCorruptIndex, "required index section is missing");
+  return *s;
+}
+
+std::vector<std::uint8_t> BuildHeader(const ContainerSpec& spec,
+                                      const std::vector<SectionDescriptor>& sections) {
+  std::vector<std::uint8_t> h;
+  h.reserve(kBaseHeaderSize + sections.size() * kSectionEntrySize);
+  h.insert(h.end(), kMagic.begin(), kMagic.end());
+  AppendU16(h, 1);
+  AppendU16(h, spec.format_minor);
+  h.push_back(1);  // little endian
+  h.push_back(spec.kind);
+  h.push_back(static_cast<std::uint8_t>(spec.backend));
+  h.push_back(spec.coordinate_width);
+  h.push_back(kNormalizationId);
+  h.push_back(spec.library_major);
+  h.push_back(spec.library_minor);
+  h.push_back(spec.library_patch);
+  h.push_back(spec.sdsl_major);
+  h.push_back(spec.sdsl_minor);
+  h.push_back(spec.sdsl_patch);
+  h.push_back(spec.format_minor >= 4 ? static_cast<std::uint8_t>(spec.profile) : 0);
+  AppendU32(h, static_cast<std::uint32_t>(sections.size()));
+  AppendU32(h, static_cast<std::uint32_t>(kBaseHeaderSize + sections.size() * kSectionEntrySize));
+  AppendU64(h, spec.sequence_count);
+  AppendU64(h, spec.total_bases);
+  AppendU64(h, spec.text_symbols);
+  AppendU64(h, spec.ambiguous_bases);
+  AppendU64(h, spec.fingerprint);
+  AppendU32(h, 0);  // header CRC, patched below
+  AppendU32(h, 0);
+  for (const auto& s : sections) {
+    AppendU32(h, static_cast<std::uint32_t>(s.type));
+    AppendU32(h, s.flags);
+    AppendU64(h, s.offset);
+    AppendU64(h, s.size);
+    AppendU32(h, s.crc32);
+    AppendU32(h, 0);
+  }
+  const std::uint32_t crc = Crc32(0, h.data(), h.size());
+  for (int i = 0; i < 4; ++i) h[kHeaderCrcOffset + i] = static_cast<std::uint8_t>(crc >> (8 * i));
+  return h;
+}
+
+void WriteContainer(const std::filesystem::path& path, const SaveOptions& options,
+                    const ContainerSpec& spec, const std::vector<SectionWriter>& writers) {
+  if (writers.empty() || writers.size() > kMaxSections) {
+    Throw(ErrorCode::kBuildFailure, "invalid number of index sections");
+  }
+  if (path.empty()) Throw(ErrorCode::kInvalidInput, "index output path must not be empty");
+  if (!options.overwrite && std::filesystem::exists(path)) {
+    Throw(ErrorCode::kIoError, "index already exists: " + path.string());
+  }
+  const auto temp = TemporaryPathFor(path);
+  try {
+    const std::size_t header_size = kBaseHeaderSize + writers.size() * kSectionEntrySize;
+    std::fstream out(temp, std::ios::binary | std::ios::in | std::ios::out | std::ios::trunc);
+    if (!out) Throw(ErrorCode::kIoError, "cannot create temporary index: " + temp.string());
+    std::vector<char> placeholder(header_size, '\0');
+    out.write(placeholder.data(), static_cast<std::streamsize>(placeholder.size()));
+    std::vector<SectionDescriptor> sections;
+    std::set<SectionType> types;
+    for (const auto& w : writers) {
+      if (!types.insert(w.type).second) Throw(ErrorCode::kBuildFailure, "duplicate index section type");
+      const auto begin = out.tellp();
+      CrcBuffer crc(out.rdbuf());
+      std::ostream section(&crc);
+      w.write(section);
+      section.flush();
+      if (!section || !crc.Finish() || !out) Throw(ErrorCode::kIoError, "failed while writing index section");
+      const auto end = out.tellp();
+      SectionDescriptor d;
+      d.type = w.type;
+      d.flags = 1;
+      d.offset = static_cast<std::uint64_t>(begin);
+      d.size = static_cast<std::uint64_t>(end - begin);
+      d.crc32 = crc.Crc();
+      sections.push_back(d);
+    }
+    out.flush();
+    const auto header = BuildHeader(spec, sections);
+    out.seekp(0);
+    out.write(reinterpret_cast<const char*>(header.data()), static_cast<std::streamsize>(header.size()));
+    out.flush();
+    out.close();
+    if (!out) Throw(ErrorCode::kIoError, "failed to finalize temporary index");
+    (void)ReadContainer(temp);  // self-validate before publication
+    Publish(temp, path, options.overwrite);
+  } catch (...) {
+    std::error_code ignored;
+    std::filesystem::remove(temp, ignored);
+    throw;
+  }
+}
+
+ParsedContainer ReadContainer(const std::filesystem::path& path) {
+  std::error_code ec;
+  const auto file_size = std::filesystem::file_size(path, ec);
+  if (ec) Throw(ErrorCode::kIoError, "cannot stat index: " + path.string());
+  if (file_size < kBaseHeaderSize) Throw(ErrorCode::kCorruptIndex, "index is smaller than its fixed header");
+  std::ifstream in(path, std::ios::binary);
+  if (!in) Throw(ErrorCode::kIoError, "cannot open index: " + path.string());
+  std::vector<std::uint8_t> head(kBaseHeaderSize);
+  ReadExact(in, head.data(), head.size(), "index header");
+  if (!std::equal(kMagic.begin(), kMagic.end(), head.begin())) {
+    Throw(ErrorCode::kCorruptIndex, "index magic is invalid");
+  }
+  const auto major = GetU16(&head[8]);
+  const auto minor = GetU16(&head[10]);
+  if (major != 1 || minor > 4) Throw(ErrorCode::kVersionMismatch, "unsupported sufidx format version");
+  if (head[12] != 1) Throw(ErrorCode::kVersionMismatch, "unsupported index endianness");
+  const std::uint32_t count = GetU32(&head[24]);
+  const std::uint32_t header_size = GetU32(&head[28]);
+  if (count == 0 || count > kMaxSections || header_size != kBaseHeaderSize + count * kSectionEntrySize ||
+      header_size > file_size) {
+    Throw(ErrorCode::kCorruptIndex, "index section table is invalid");
+  }
+  head.resize(header_size);
+  ReadExact(in, head.data() + kBaseHeaderSize, header_size - kBaseHeaderSize, "section table");
+  const std::uint32_t stored_crc = GetU32(&head[kHeaderCrcOffset]);
+  std::vector<std::uint8_t> zeroed = head;
+  std::memset(&zeroed[kHeaderCrcOffset], 0, 4);
+  if (Crc32(0, zeroed.data(), zeroed.size()) != stored_crc) {
+    Throw(ErrorCode::kCorruptIndex, "index header checksum mismatch");
+  }
+  ParsedContainer c;
+  c.path = path;
+  c.file_size = file_size;
+  c.spec.format_minor = minor;
+  c.spec.kind = head[13];
+  c.spec.backend = static_cast<StoredBackend>(head[14]);
+  c.spec.coordinate_width = head[15];
+  if (head[16] != kNormalizationId) Throw(ErrorCode::kVersionMismatch, "unsupported normalization");
+  c.spec.library_major = head[17];
+  c.spec.library_minor = head[18];
+  c.spec.library_patch = head[19];
+  c.spec.sdsl_major = head[20];
+  c.spec.sdsl_minor = head[21];
+  c.spec.sdsl_patch = head[22];
+  c.spec.profile = static_cast<SaResourceProfile>(head[23]);
+  c.spec.sequence_count = GetU64(&head[32]);
+  c.spec.total_bases = GetU64(&head[40]);
+  c.spec.text_symbols = GetU64(&head[48]);
+  c.spec.ambiguous_bases = GetU64(&head[56]);
+  c.spec.fingerprint = GetU64(&head[64]);
+  std::set<std::uint32_t> types;
+  for (std::uint32_t i = 0; i < count; ++i) {
+    const std::uint8_t* e = &head[kBaseHeaderSize + i * kSectionEntrySize];
+    SectionDescriptor d;
+    d.type = static_cast<SectionType>(GetU32(e));
+    d.flags = GetU32(e + 4);
+    d.offset = GetU64(e + 8);
+    d.size = GetU64(e + 16);
+    d.crc32 = GetU32(e + 24);
+    if (!types.insert(GetU32(e)).second) Throw(ErrorCode::kCorruptIndex, "duplicate index section");
+    if (d.offset < header_size || d.offset > file_size || d.size > file_size - d.offset) {
+      Throw(ErrorCode::kCorruptIndex, "index section lies outside the file");
+    }
+    c.sections.push_back(d);
+  }
+  auto sorted = c.sections;
+  std::sort(sorted.begin(), sorted.end(), [](const auto& a, const auto& b) { return a.offset < b.offset; });
+  for (std::size_t i = 1; i < sorted.size(); ++i) {
+    if (sorted[i - 1].offset + sorted[i - 1].size > sorted[i].offset) {
+      Throw(ErrorCode::kCorruptIndex, "index sections overlap");
+    }
+  }
+  // Every section's CRC32, streamed.
+  std::vector<char> buffer(1 << 20);
+  for (const auto& d : c.sections) {
+    in.seekg(static_cast<std::streamoff>(d.offset));
+    std::uint64_t remaining = d.size;
+    std::uint32_t crc = 0;
+    while (remaining > 0) {
+      const auto n = static_cast<std::size_t>(std::min<std::uint64_t>(remaining, buffer.size()));
+      ReadExact(in, buffer.data(), n, "index section");
+      crc = Crc32(crc, buffer.data(), n);
+      remaining -= n;
+    }
+    if (crc != d.crc32) Throw(ErrorCode::kCorruptIndex, "index section checksum mismatch");
+  }
+  return c;
+}
+
+std::vector<std::uint8_t> ReadSection(const ParsedContainer& container, SectionType type) {
+  const auto& d = container.Require(type);
+  std::ifstream in(container.path, std::ios::binary);
+  if (!in) Throw(ErrorCode::kIoError, "cannot open index: " + container.path.string());
+  in.seekg(static_cast<std::streamoff>(d.offset));
+  std::vector<std::uint8_t> bytes(static_cast<std::size_t>(d.size));
+  if (d.size) ReadExact(in, bytes.data(), bytes.size(), "index section");
+  return bytes;
+}
+
+}  // namespace sufkit::detail

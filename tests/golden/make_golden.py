@@ -100,6 +100,50 @@ def reference_digests() -> dict:
     return out
 
 
+def golden_references() -> dict[str, list[tuple[str, bytes]]]:
+    """Small references for the .sufidx golden files (records as FASTA
+    text would give them)."""
+    import numpy as np
+
+    rng = np.random.default_rng(20260822)
+    alphabet = np.frombuffer(b"ACGT", np.uint8)
+    def rand(k):
+        return alphabet[rng.integers(0, 4, size=k)].tobytes()
+    mixed = []
+    for c in range(5):
+        body = bytearray(rand(int(rng.integers(500, 4000))))
+        pos = int(rng.integers(0, len(body) - 200))
+        body[pos:pos + 150] = b"N" * 150                     # N run
+        body[10:90] = b"A" * 80                              # poly-A
+        body.extend(b"acgtRYKM" + rand(300))                 # lowercase + IUPAC -> N
+        mixed.append((f"chr{c + 1}", bytes(body)))
+    return {
+        "fixture": [("chr1", b"ACGTNACGT"), ("chr2", b"TTTACGTAAA"), ("chr3", b"AAA")],
+        "mixed": mixed,
+    }
+
+
+# (coordinate width, storage width, acceleration) variants written per reference
+SUFIDX_VARIANTS = [
+    (0, 0, 3), (64, 0, 3), (0, 40, 3), (0, 48, 3), (0, 64, 3), (0, 0, 1), (0, 0, 0), (64, 48, 1),
+]
+
+
+def sufidx_golden() -> dict:
+    import oracle
+
+    out_dir = HERE / "sufidx"
+    out_dir.mkdir(exist_ok=True)
+    index = {}
+    for name, records in golden_references().items():
+        for cw, sw, accel in SUFIDX_VARIANTS:
+            fname = f"{name}_c{cw or 'auto'}_s{sw or 'auto'}_a{accel}.sufidx"
+            oracle.ref_write_sufidx(records, str(out_dir / fname), cw, sw, accel)
+            index[fname] = {"reference": name, "coordinate_width": cw, "storage_width": sw,
+                            "acceleration": accel}
+    return index
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
@@ -109,6 +153,8 @@ def main() -> None:
     import oracle
     oracle.build(str(ref))
     (HERE / "reference_digests.json").write_text(json.dumps(reference_digests(), indent=1) + "\n")
+    if oracle.ref_writer_available():
+        (HERE / "sufidx_index.json").write_text(json.dumps(sufidx_golden(), indent=1) + "\n")
 
 
 if __name__ == "__main__":

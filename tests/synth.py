@@ -21,6 +21,13 @@ def finish(contigs: list[np.ndarray]) -> np.ndarray:
     return np.concatenate(parts)
 
 
+def contig_lengths(text: np.ndarray) -> np.ndarray:
+    """Contig lengths of a logical text (bases between separators)."""
+    seps = np.flatnonzero(text == SEP)
+    starts = np.r_[0, seps[:-1] + 1]
+    return (seps - starts).astype(np.uint64)
+
+
 def random_bases(rng: np.random.Generator, length: int) -> np.ndarray:
     return rng.integers(A, T + 1, size=length, dtype=np.uint8)
 
