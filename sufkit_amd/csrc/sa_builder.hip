@@ -109,7 +109,7 @@ struct sfk_hip_builder {
   int device = 0;
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer codes, spec, rank, sa, act_idx, act_seg, keys0, keys1,
-      vals0, vals1, seg_head0, seg_head1, seg_start0, seg_start1, radix_counts,
+      vals0, vals1, seg_delta0, seg_delta1, radix_counts,
       partial, counters, narrow;
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
@@ -223,8 +223,7 @@ class Doubling {
   void Run() {
     m_ = n_;
     segments_ = 1;
-    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_head0.As<Index>(), 0, sizeof(Index), b_->stream));
-    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_start0.As<Index>(), 0, sizeof(Index), b_->stream));
+    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_delta0.As<Index>(), 0, sizeof(Index), b_->stream));
     Step(kSecFillHi, 32, 0, /*iota=*/true);
     if (m_ > 0) Step(kSecFillLo, 32, 0, false);
     if (m_ > 0) {
@@ -280,7 +279,7 @@ class Doubling {
     if (!rank_step) ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
     const unsigned keygen_grid = static_cast<unsigned>(std::min<std::uint64_t>(CeilDiv(m, 256), 2048));
     KeygenKernel<Index, K><<<keygen_grid, 256, 0, b_->stream>>>(
-        b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), m, sec_kind, sec_bits, seg_bits,
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), m, sec_kind, sec_bits, seg_bits,
         text_, rank_, h, iota, keys, vals, dc);
     SFK_HIP_CHECK(hipGetLastError());
     ctx_.stats.algorithmic_bytes_sa +=
@@ -301,15 +300,13 @@ class Doubling {
     // Boundaries, singletons, compaction.
     const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
     RefinePartial* partial = b_->partial.As<RefinePartial>();
-    Index* head_cur = (flip_ ? b_->seg_head1 : b_->seg_head0).template As<Index>();
-    Index* start_cur = (flip_ ? b_->seg_start1 : b_->seg_start0).template As<Index>();
-    Index* head_next = (flip_ ? b_->seg_head0 : b_->seg_head1).template As<Index>();
-    Index* start_next = (flip_ ? b_->seg_start0 : b_->seg_start1).template As<Index>();
+    Index* delta_cur = (flip_ ? b_->seg_delta1 : b_->seg_delta0).template As<Index>();
+    Index* delta_next = (flip_ ? b_->seg_delta0 : b_->seg_delta1).template As<Index>();
     RefineReduceKernel<K><<<g.grid, 256, 0, b_->stream>>>(keys, m, g.chunk, partial);
     RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
     RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
-        keys, vals, m, g.chunk, sec_bits, head_cur, start_cur, partial, n_, sa_, rank_,
-        b_->act_idx.As<Index>(), b_->act_seg.As<Index>(), head_next, start_next);
+        keys, vals, m, g.chunk, sec_bits, delta_cur, partial, n_, sa_, rank_,
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next);
     SFK_HIP_CHECK(hipGetLastError());
     const DeviceCounters& after = ReadCounters(b_);
     const std::uint64_t survivors = after.survivors;
@@ -405,16 +402,14 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   if (!sa_direct) b->sa.Reserve(n * sizeof(Index));
   if (!isa_direct) b->rank.Reserve(n * sizeof(Index));
   b->act_idx.Reserve(n * sizeof(Index));
-  b->act_seg.Reserve(n * sizeof(Index));
+  b->act_seg.Reserve(n * sizeof(std::uint32_t));
   b->keys0.Reserve(std::max<std::uint64_t>(n * 8, 64));
   b->keys1.Reserve(std::max<std::uint64_t>(n * 8, 64));
   b->vals0.Reserve(n * sizeof(Index));
   b->vals1.Reserve(n * sizeof(Index));
   const std::uint64_t seg_cap = n / 2 + 2;
-  b->seg_head0.Reserve(seg_cap * sizeof(Index));
-  b->seg_head1.Reserve(seg_cap * sizeof(Index));
-  b->seg_start0.Reserve(seg_cap * sizeof(Index));
-  b->seg_start1.Reserve(seg_cap * sizeof(Index));
+  b->seg_delta0.Reserve(seg_cap * sizeof(Index));
+  b->seg_delta1.Reserve(seg_cap * sizeof(Index));
   b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * (RadixTile::kMaxGrid + 1) * 8);
   b->partial.Reserve(1024 * sizeof(RefinePartial));
 
@@ -505,6 +500,10 @@ int ValidateArgs(std::uint64_t n, std::uint32_t construction_bits, std::uint32_t
   if (n < 1) {
     message = "reference text is empty";
     return SFK_HIP_INVALID_INPUT;
+  }
+  if (n >= (1ull << 33)) {
+    message = "texts of 2^33 symbols or more are not supported by the HIP backend";
+    return SFK_HIP_UNSUPPORTED_BACKEND;
   }
   if (construction_bits == 32 && n > 0xFFFFFFFFull) {
     message = "reference is too large for the 32-bit HIP backend";
@@ -623,8 +622,8 @@ void* sfk_hip_builder_stream(sfk_hip_builder* b) {
 uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
   const uint64_t w = construction_bits == 64 ? 8 : 4;
   const uint64_t words = sfk::PackedWords(n);
-  return words * 12 + n * w * 2 /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
-         (n / 2 + 2) * w * 4 /*segments*/ + n * w * 2 /*sa, rank when not direct*/ +
+  return words * 12 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
+         (n / 2 + 2) * w * 2 /*segment deltas*/ + n * w * 2 /*sa, rank when not direct*/ +
          sfk::RadixTile::kRadix * 1024 * 8 + 1024 * sizeof(sfk::RefinePartial);
 }
 

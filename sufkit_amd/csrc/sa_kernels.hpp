@@ -3,9 +3,11 @@
 // Prefix doubling over "segments": a segment is a run of SA rows whose
 // suffixes share their first h symbols (a group) and still hold more than
 // one suffix.  The active list keeps, per still-unsorted suffix, its text
-// position (act_idx) and its dense segment id (act_seg); the segment table
-// keeps the segment's first SA row (seg_head) and its first active slot
-// (seg_start).  One refine step:
+// position (act_idx) and its dense 32-bit segment id (act_seg); the segment
+// table keeps one value per segment, seg_delta = (first SA row of the
+// segment) - (its first active slot), so the SA row of active slot k is
+// seg_delta[seg] + k (slots only ever move left when survivors are
+// compacted, rows never do, so the delta is never negative).  One step:
 //   Keygen        key = (segment id << sec_bits) | secondary(suffix)
 //   radix sort    stable LSD over the varying digits of the key
 //   RefineReduce/RefineSpine/RefineApply
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(256) void PackTextKernel(
 
 template <typename Index, typename K>
 __global__ __launch_bounds__(256) void KeygenKernel(
-    const Index* __restrict__ act_idx, const Index* __restrict__ act_seg,
+    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
     const Index* __restrict__ rank, std::uint64_t h, bool iota,
     K* __restrict__ keys, Index* __restrict__ vals, DeviceCounters* counters) {
@@ -267,11 +269,10 @@ __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial
 template <typename Index, typename K, int kItems>
 __global__ __launch_bounds__(256) void RefineApplyKernel(
     const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m,
-    std::uint64_t chunk, int sec_bits, const Index* __restrict__ seg_head,
-    const Index* __restrict__ seg_start, const RefinePartial* __restrict__ partial,
+    std::uint64_t chunk, int sec_bits, const Index* __restrict__ seg_delta,
+    const RefinePartial* __restrict__ partial,
     std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
-    Index* __restrict__ out_seg, Index* __restrict__ out_head,
-    Index* __restrict__ out_start) {
+    std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta) {
   constexpr int kBlock = 256;
   constexpr int kTile = kBlock * kItems;
   __shared__ std::uint64_t scratch[kBlock / kWave];
@@ -308,9 +309,8 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
       const K key = keys[k];
       const Index v = vals[k];
       const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
-      const std::uint64_t shead = static_cast<std::uint64_t>(seg_head[seg]);
-      const std::uint64_t sstart = static_cast<std::uint64_t>(seg_start[seg]);
-      const std::uint64_t row = shead + (k - sstart);
+      const std::uint64_t delta = static_cast<std::uint64_t>(seg_delta[seg]);
+      const std::uint64_t row = delta + k;
       if (starts[i]) cur_last = k + 1;
       if (row >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
       if (single[i]) {
@@ -319,15 +319,14 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
       } else {
         // The group start lies in the same segment: every segment start is a
         // key change, hence a group start.
-        const std::uint64_t head_row = shead + (cur_last - 1 - sstart);
+        const std::uint64_t head_row = delta + (cur_last - 1);
         rank[v] = static_cast<Index>(head_row);
         if (starts[i]) {
-          out_head[segc] = static_cast<Index>(row);
-          out_start[segc] = static_cast<Index>(pos);
+          out_delta[segc] = static_cast<Index>(row - pos);
           ++segc;
         }
         out_idx[pos] = v;
-        out_seg[pos] = static_cast<Index>(segc - 1);
+        out_seg[pos] = static_cast<std::uint32_t>(segc - 1);
         ++pos;
       }
     }
