@@ -40,6 +40,8 @@ WORKLOADS = {
         "1 Mbp poly-A/period-2/period-7 block"),
     3: ("3.1 Gbp human-scale genome: 24 contigs of 50-250 Mbp, GC 41%, 45% repeats, "
         "3% satellites, N gaps up to 10 Mbp"),
+    4: ("batched build: 512 bacterial-like references, log-normal 0.5-12 Mbp (median 4 Mbp), "
+        "GC 30-70%, 5% shared mobile elements; one batched device build vs one build per reference"),
 }
 
 
@@ -52,6 +54,7 @@ def parse_args():
     ap.add_argument("--scale", type=float, default=1.0, help="workload size multiplier")
     ap.add_argument("--construction-bits", type=int, default=None, choices=(32, 64))
     ap.add_argument("--seed", type=int, default=20260822)
+    ap.add_argument("--refs", type=int, default=512, help="references in the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-symbols", type=int, default=0,
                     help="symbols of the text timed on the CPU (0 = whole text)")
@@ -128,12 +131,81 @@ def sample_check(text: np.ndarray, sa_rows: np.ndarray, sa_prev: np.ndarray, lcp
     return bad
 
 
+def run_batched(args) -> int:
+    """Workload 4: the batched build against plain per-reference builds."""
+    import sufkit_amd
+
+    refs = args.refs
+    texts, bases = [], 0
+    for r in range(refs):
+        t, lens = sufkit_amd.synthetic_text(4, args.seed, r, args.scale)
+        texts.append(t)
+        bases += int(lens.sum())
+    offsets = np.zeros(refs + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([t.size for t in texts])
+    n = int(offsets[-1])
+    d_text = sufkit_amd.DeviceArray(n + 64)
+    d_text.upload(np.concatenate(texts))
+    del texts
+    outs = {k: [sufkit_amd.DeviceArray(n * 4) for _ in range(3)] for k in ("batch", "plain")}
+    builder = sufkit_amd.DeviceBuilder(0)
+    flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
+
+    def batched():
+        sa, isa, lcp = outs["batch"]
+        return builder.build_batch(d_text.ptr, n, offsets, 32, sa.ptr, 32, isa.ptr, 32, lcp.ptr, 32, flags)
+
+    def plain():
+        sa, isa, lcp = outs["plain"]
+        for a, b in zip(offsets[:-1].tolist(), offsets[1:].tolist()):
+            builder.build(d_text.ptr + a, b - a, 32, sa.ptr + 4 * a, 32, isa.ptr + 4 * a, 32,
+                          lcp.ptr + 4 * a, 32, flags)
+
+    for _ in range(args.warmup):
+        batched()
+        plain()
+    t0 = time.perf_counter()
+    stats = [batched() for _ in range(args.steps)]
+    t_batch = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        plain()
+    t_plain = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    identical = True
+    chunk = 1 << 26
+    for i in range(3):
+        for start in range(0, n, chunk):
+            cnt = min(chunk, n - start)
+            if not np.array_equal(outs["batch"][i].download(np.uint32, cnt, start),
+                                  outs["plain"][i].download(np.uint32, cnt, start)):
+                identical = False
+    last = stats[-1]
+    result = {
+        "metric": "SA+ISA+LCP build Mbp/s + HBM roofline frac, 1 MI355X vs divsufsort64 on host",
+        "value": round(bases / 1e6 / t_batch, 2), "unit": "Mbp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (deterministic splitmix64 generator, seed %d)" % args.seed,
+        "config": {"workload": WORKLOADS[4], "references": refs, "text_symbols": n, "bases": bases},
+        "plain_per_reference_mbps": round(bases / 1e6 / t_plain, 2),
+        "plain_per_reference_ms": round(t_plain * 1e3, 3),
+        "batched_over_plain": round(t_plain / t_batch, 3),
+        "outputs_identical": identical,
+        "phases": {k: last[k] for k in ("refine_steps", "doubling_rounds", "active_total", "sort_passes")},
+    }
+    builder.close()
+    print(json.dumps(result), flush=True)
+    return 0 if identical else 1
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); a text's build is "
               "not sharded across GPUs in this implementation", file=sys.stderr)
         return 2
+    if args.workload == 4:
+        return run_batched(args)
     import sufkit_amd
 
     text, contigs = sufkit_amd.synthetic_text(args.workload, args.seed, 0, args.scale)

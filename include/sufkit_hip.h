@@ -108,6 +108,26 @@ int sfk_hip_build_device(sfk_hip_builder* builder, const uint8_t* d_text,
                          size_t error_capacity);
 
 /*
+ * Batched device build of `refs` independent references laid out back to
+ * back in one device text: reference r is the logical text (its encoded
+ * bytes followed by its own sentinel 0) at [offsets[r], offsets[r+1]);
+ * offsets (host array of refs + 1 entries) starts at 0 and ends at n.  The
+ * outputs use the same layout: rows [offsets[r], offsets[r+1]) of d_sa /
+ * d_isa / d_lcp hold reference r's SA, ISA and LCP in its own coordinates,
+ * identical to building it alone.  All references are refined together: the
+ * segment list of still-unsorted groups is one work queue over all of them,
+ * so one set of launches serves the whole batch.  No counterpart in the
+ * reference, which builds one reference per SuffixArray::Build call.
+ */
+int sfk_hip_build_batch_device(sfk_hip_builder* builder, const uint8_t* d_text,
+                               uint64_t n, const uint64_t* offsets, uint32_t refs,
+                               uint32_t construction_bits, uint32_t flags,
+                               void* d_sa, uint32_t sa_bits, void* d_isa,
+                               uint32_t isa_bits, void* d_lcp, uint32_t lcp_bits,
+                               sfk_hip_stats* stats, char* error,
+                               size_t error_capacity);
+
+/*
  * Host build: the same, from host text into host output buffers (each n
  * elements of the given width; isa/lcp may be NULL).  This is the entry
  * point a SuffixArray::Build backend branch calls in place of

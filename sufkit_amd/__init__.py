@@ -102,6 +102,11 @@ def load_hip() -> ctypes.CDLL:
         vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
         ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_build_device.restype = ctypes.c_int
+    lib.sfk_hip_build_batch_device.argtypes = [
+        vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+        ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+        ctypes.c_uint32, ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_build_batch_device.restype = ctypes.c_int
     lib.sfk_hip_build.argtypes = [
         u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
         vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
@@ -185,6 +190,20 @@ class DeviceBuilder:
             raise SufkitError(status, err.value.decode(errors="replace"))
         return stats.as_dict()
 
+    def build_batch(self, d_text: int, n: int, offsets: np.ndarray, construction_bits: int,
+                    d_sa: int, sa_bits: int, d_isa: int | None, isa_bits: int, d_lcp: int | None,
+                    lcp_bits: int, flags: int = SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP) -> dict:
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        stats = HipStats()
+        err = ctypes.create_string_buffer(512)
+        status = self.lib.sfk_hip_build_batch_device(
+            self.handle, d_text, n, offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            offsets.size - 1, construction_bits, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp,
+            lcp_bits, ctypes.byref(stats), err, len(err))
+        if status != 0:
+            raise SufkitError(status, err.value.decode(errors="replace"))
+        return stats.as_dict()
+
     def close(self) -> None:
         if self.handle:
             self.lib.sfk_hip_builder_destroy(self.handle)
@@ -245,3 +264,65 @@ def synthetic_text(workload: int, seed: int = DEFAULT_SEED, ref: int = 0, scale:
     if lib.sfk_synth_generate(workload, seed, ref, scale, text.ctypes.data, n, threads) != 0:
         raise RuntimeError("synthetic generation failed")
     return text, lens[: count.value].copy()
+
+
+class DeviceArray:
+    """A device allocation through the C ABI (for tests and the benchmark)."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        self.lib = load_hip()
+        ptr = ctypes.c_void_p()
+        if self.lib.sfk_hip_malloc(device, max(int(nbytes), 1), ctypes.byref(ptr)) != 0:
+            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+        self.ptr = int(ptr.value)
+        self.nbytes = int(nbytes)
+
+    def upload(self, array: np.ndarray) -> None:
+        array = np.ascontiguousarray(array)
+        assert array.nbytes <= self.nbytes
+        if self.lib.sfk_hip_memcpy_h2d(self.ptr, array.ctypes.data, array.nbytes) != 0:
+            raise RuntimeError("host to device copy failed")
+
+    def download(self, dtype, count: int, offset_elems: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if count and self.lib.sfk_hip_memcpy_d2h(
+                out.ctypes.data, self.ptr + offset_elems * out.itemsize, out.nbytes) != 0:
+            raise RuntimeError("device to host copy failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.lib.sfk_hip_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def hip_build_batch(texts: list[np.ndarray], construction_bits: int = 32, out_bits: int = 32,
+                    builder: "DeviceBuilder | None" = None):
+    """Batched build of several logical texts; returns [(sa, isa, lcp)] in
+    each reference's own coordinates, and the stats."""
+    offsets = np.zeros(len(texts) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([t.size for t in texts])
+    n = int(offsets[-1])
+    joined = np.concatenate([np.asarray(t, np.uint8) for t in texts])
+    d_text = DeviceArray(n + 64)
+    d_text.upload(joined)
+    w = out_bits // 8
+    d_sa, d_isa, d_lcp = DeviceArray(n * w), DeviceArray(n * w), DeviceArray(n * w)
+    own = builder is None
+    builder = builder or DeviceBuilder(0)
+    try:
+        stats = builder.build_batch(d_text.ptr, n, offsets, construction_bits, d_sa.ptr, out_bits,
+                                    d_isa.ptr, out_bits, d_lcp.ptr, out_bits)
+        dt = _dtype(out_bits)
+        sa, isa, lcp = d_sa.download(dt, n), d_isa.download(dt, n), d_lcp.download(dt, n)
+    finally:
+        if own:
+            builder.close()
+    parts = [(sa[a:b], isa[a:b], lcp[a:b]) for a, b in zip(offsets[:-1], offsets[1:])]
+    return parts, stats

@@ -110,7 +110,7 @@ struct sfk_hip_builder {
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer codes, spec, rank, sa, act_idx, act_seg, keys0, keys1,
       vals0, vals1, seg_delta0, seg_delta1, radix_counts,
-      partial, counters, narrow;
+      partial, counters, narrow, ref_offsets;
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
   std::size_t events_used = 0;
@@ -129,6 +129,9 @@ struct BuildContext {
   std::uint64_t n;
   bool profile;
   sfk_hip_stats stats;
+  // Batch build: reference r owns [offsets[r], offsets[r+1]) (device copy).
+  const std::uint64_t* d_offsets = nullptr;
+  std::uint32_t refs = 0;
 };
 
 void ResetCounters(sfk_hip_builder* b) {
@@ -222,9 +225,18 @@ class Doubling {
 
   void Run() {
     m_ = n_;
-    segments_ = 1;
-    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_delta0.As<Index>(), 0, sizeof(Index), b_->stream));
+    segments_ = ctx_.refs > 0 ? ctx_.refs : 1;
+    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_delta0.As<Index>(), 0, segments_ * sizeof(Index), b_->stream));
+    if (ctx_.refs > 0) {
+      // One initial segment per reference: its rows are its own positions.
+      InitSegmentsKernel<<<static_cast<unsigned>(CeilDiv(n_, 256)), 256, 0, b_->stream>>>(
+          ctx_.d_offsets, ctx_.refs, n_, b_->act_seg.As<std::uint32_t>());
+      SFK_HIP_CHECK(hipGetLastError());
+    }
     Step(kSecFillHi, 32, 0, /*iota=*/true);
+    if (ctx_.refs > 0 && b_->host_counters->zero_count != ctx_.refs) {
+      Fail(SFK_HIP_INVALID_INPUT, "every reference of a batch must end with its only 0 (sentinel)");
+    }
     if (m_ > 0) Step(kSecFillLo, 32, 0, false);
     if (m_ > 0) {
       ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(b_->counters.As<DeviceCounters>());
@@ -421,25 +433,27 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   SFK_HIP_CHECK(hipEventRecord(b->phase[0], b->stream));
   PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
       d_text, n, words, b->codes.As<std::uint64_t>(), b->spec.As<std::uint32_t>(),
-      (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, b->counters.As<DeviceCounters>());
+      (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, ctx.refs > 0,
+      b->counters.As<DeviceCounters>());
   SFK_HIP_CHECK(hipGetLastError());
+  if (ctx.refs > 0) {
+    CheckReferenceEndsKernel<<<static_cast<unsigned>(CeilDiv(ctx.refs, 256)), 256, 0, b->stream>>>(
+        d_text, ctx.d_offsets, ctx.refs, b->counters.As<DeviceCounters>());
+    SFK_HIP_CHECK(hipGetLastError());
+  }
   ctx.stats.algorithmic_bytes_sa += n + words * 12;
 
   Doubling<Index>(ctx, sa, rank, text).Run();
   SFK_HIP_CHECK(hipEventRecord(b->phase[1], b->stream));
 
   // K9: permutation + inverse check on the device (fused into the PLCP pass
-  // when the LCP is built); ISA in its stored width.
+  // when the LCP is built).
   const bool want_lcp = d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
   if (!want_lcp) {
     ValidateKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
         sa, rank, n, b->counters.As<DeviceCounters>());
     SFK_HIP_CHECK(hipGetLastError());
     ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
-  }
-  if (d_isa != nullptr && !isa_direct) {
-    ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
-    ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + isa_bits / 8);
   }
   SFK_HIP_CHECK(hipEventRecord(b->phase[2], b->stream));
   if (want_lcp) {
@@ -450,9 +464,42 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     }
   }
   SFK_HIP_CHECK(hipEventRecord(b->phase[3], b->stream));
-  if (!sa_direct) {
-    ConvertTo<Index>(b, sa, n, d_sa, sa_bits);
-    ctx.stats.algorithmic_bytes_sa += n * (sizeof(Index) + sa_bits / 8);
+  // SA and ISA in their stored widths (and, for a batch, per-reference
+  // coordinates).
+  if (ctx.refs > 0) {
+    const unsigned grid = static_cast<unsigned>(CeilDiv(n, 256));
+    auto localize = [&](auto* out_sa) {
+      using OutSa = std::remove_pointer_t<decltype(out_sa)>;
+      if (d_isa == nullptr) {
+        LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
+            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(nullptr),
+            b->counters.As<DeviceCounters>());
+      } else if (isa_bits == 32) {
+        LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
+            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(d_isa),
+            b->counters.As<DeviceCounters>());
+      } else {
+        LocalizeKernel<Index, OutSa, std::uint64_t><<<grid, 256, 0, b->stream>>>(
+            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint64_t*>(d_isa),
+            b->counters.As<DeviceCounters>());
+      }
+    };
+    if (sa_bits == 32) {
+      localize(static_cast<std::uint32_t*>(d_sa));
+    } else {
+      localize(static_cast<std::uint64_t*>(d_sa));
+    }
+    SFK_HIP_CHECK(hipGetLastError());
+    ctx.stats.algorithmic_bytes_isa += n * (2 * sizeof(Index) + sa_bits / 8 + isa_bits / 8);
+  } else {
+    if (d_isa != nullptr && !isa_direct) {
+      ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
+      ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + isa_bits / 8);
+    }
+    if (!sa_direct) {
+      ConvertTo<Index>(b, sa, n, d_sa, sa_bits);
+      ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + sa_bits / 8);
+    }
   }
   SFK_HIP_CHECK(hipEventRecord(b->phase[4], b->stream));
   const DeviceCounters& c = ReadCounters(b);
@@ -467,10 +514,12 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   float ms = 0.f;
   SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[0], b->phase[1]));
   ctx.stats.sa_seconds = ms * 1e-3;
-  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[1], b->phase[2]));
-  ctx.stats.isa_seconds = ms * 1e-3;
   SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[2], b->phase[3]));
   ctx.stats.lcp_seconds = ms * 1e-3;
+  float ms_isa = 0.f;
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[1], b->phase[2]));
+  SFK_HIP_CHECK(hipEventElapsedTime(&ms_isa, b->phase[3], b->phase[4]));
+  ctx.stats.isa_seconds = (ms + ms_isa) * 1e-3;
   SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[0], b->phase[4]));
   ctx.stats.device_seconds = ms * 1e-3;
   if (ctx.profile) {
@@ -534,10 +583,19 @@ int ValidateArgs(std::uint64_t n, std::uint32_t construction_bits, std::uint32_t
 int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t n,
                    std::uint32_t construction_bits, std::uint32_t flags, void* d_sa,
                    std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits, void* d_lcp,
-                   std::uint32_t lcp_bits, sfk_hip_stats* stats, std::string& message) {
+                   std::uint32_t lcp_bits, sfk_hip_stats* stats, std::string& message,
+                   const std::uint64_t* offsets = nullptr, std::uint32_t refs = 0) {
   BuildContext ctx{b, n, (flags & SFK_HIP_PROFILE_KERNELS) != 0, sfk_hip_stats{}};
   b->events_used = 0;
   SFK_HIP_CHECK(hipSetDevice(b->device));
+  if (refs > 0) {
+    b->ref_offsets.Reserve((static_cast<std::uint64_t>(refs) + 1) * 8);
+    SFK_HIP_CHECK(hipMemcpyAsync(b->ref_offsets.As<void>(), offsets,
+                                 (static_cast<std::uint64_t>(refs) + 1) * 8, hipMemcpyHostToDevice,
+                                 b->stream));
+    ctx.d_offsets = b->ref_offsets.As<std::uint64_t>();
+    ctx.refs = refs;
+  }
   if (construction_bits == 32) {
     BuildTyped<std::uint32_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
   } else {
@@ -646,6 +704,47 @@ int sfk_hip_build_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,
   try {
     return sfk::RunDeviceBuild(b, d_text, n, construction_bits, flags, d_sa, sa_bits, d_isa,
                                isa_bits, d_lcp, lcp_bits, stats, message);
+  } catch (const sfk::Failure& f) {
+    (void)hipStreamSynchronize(b->stream);
+    sfk::CopyError(f.message, error, error_capacity);
+    return f.status;
+  } catch (const std::bad_alloc&) {
+    sfk::CopyError("host allocation failed", error, error_capacity);
+    return SFK_HIP_BUILD_FAILURE;
+  }
+}
+
+int sfk_hip_build_batch_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,
+                               const uint64_t* offsets, uint32_t refs, uint32_t construction_bits,
+                               uint32_t flags, void* d_sa, uint32_t sa_bits, void* d_isa,
+                               uint32_t isa_bits, void* d_lcp, uint32_t lcp_bits,
+                               sfk_hip_stats* stats, char* error, size_t error_capacity) {
+  std::string message;
+  if (b == nullptr || d_text == nullptr || d_sa == nullptr || offsets == nullptr || refs == 0) {
+    sfk::CopyError("builder, text, SA buffer and at least one reference are required", error,
+                   error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (offsets[0] != 0 || offsets[refs] != n) {
+    sfk::CopyError("reference offsets must start at 0 and end at n", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  for (uint32_t r = 0; r < refs; ++r) {
+    if (offsets[r + 1] < offsets[r] + 2) {
+      sfk::CopyError("every reference needs at least one symbol and its sentinel", error,
+                     error_capacity);
+      return SFK_HIP_INVALID_INPUT;
+    }
+  }
+  int status = sfk::ValidateArgs(n, construction_bits, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits,
+                                 flags, message);
+  if (status != SFK_HIP_OK) {
+    sfk::CopyError(message, error, error_capacity);
+    return status;
+  }
+  try {
+    return sfk::RunDeviceBuild(b, d_text, n, construction_bits, flags, d_sa, sa_bits, d_isa,
+                               isa_bits, d_lcp, lcp_bits, stats, message, offsets, refs);
   } catch (const sfk::Failure& f) {
     (void)hipStreamSynchronize(b->stream);
     sfk::CopyError(f.message, error, error_capacity);

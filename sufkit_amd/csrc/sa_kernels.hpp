@@ -50,6 +50,7 @@ struct DeviceCounters {
   unsigned long long special_active;
   unsigned long long long_items;
   unsigned long long long_words;
+  unsigned long long zero_count;  // sentinels seen by the batch packer
   unsigned int error_flags;
   unsigned int pad;
 };
@@ -60,12 +61,13 @@ struct DeviceCounters {
 __global__ __launch_bounds__(256) void PackTextKernel(
     const std::uint8_t* __restrict__ text, std::uint64_t n, std::uint64_t words,
     std::uint64_t* __restrict__ codes, std::uint32_t* __restrict__ spec,
-    bool aligned, DeviceCounters* counters) {
+    bool aligned, bool batch, DeviceCounters* counters) {
   const std::uint64_t w = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w >= words) return;
   std::uint64_t c = 0;
   std::uint32_t s = 0;
   std::uint32_t flags = 0;
+  std::uint32_t zeros = 0;
   const std::uint64_t p0 = w * 32;
   std::uint8_t bytes[32];
   if (aligned && p0 + 32 <= n) {
@@ -101,14 +103,74 @@ __global__ __launch_bounds__(256) void PackTextKernel(
       code = 3;
       sp = 1;
     }
-    const bool last = (p0 + j == n - 1);
-    if ((v == 0) != last) flags |= kFlagBadSentinel;
+    if (batch) {
+      zeros += v == 0;
+    } else if ((v == 0) != (p0 + j == n - 1)) {
+      flags |= kFlagBadSentinel;
+    }
     c |= static_cast<std::uint64_t>(code) << (62 - 2 * j);
     s |= sp << (31 - j);
   }
   codes[w] = c;
   spec[w] = s;
   if (flags) atomicOr(&counters->error_flags, flags);
+  if (zeros) atomicAdd(&counters->zero_count, static_cast<unsigned long long>(zeros));
+}
+
+// Batch layout: reference r owns positions (and SA rows) [offsets[r],
+// offsets[r+1]); its last symbol must be its sentinel.
+__device__ inline std::uint32_t ReferenceOf(const std::uint64_t* offsets, std::uint32_t refs,
+                                            std::uint64_t k) {
+  std::uint32_t lo = 0, hi = refs;  // offsets[lo] <= k < offsets[hi]
+  while (hi - lo > 1) {
+    const std::uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= k) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void CheckReferenceEndsKernel(
+    const std::uint8_t* __restrict__ text, const std::uint64_t* __restrict__ offsets,
+    std::uint32_t refs, DeviceCounters* counters) {
+  const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= refs) return;
+  if (offsets[r + 1] <= offsets[r] + 1 || text[offsets[r + 1] - 1] != 0) {
+    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagBadSentinel));
+  }
+}
+
+// Initial segments of a batch: one per reference.
+__global__ __launch_bounds__(256) void InitSegmentsKernel(
+    const std::uint64_t* __restrict__ offsets, std::uint32_t refs, std::uint64_t n,
+    std::uint32_t* __restrict__ act_seg) {
+  const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < n) act_seg[k] = ReferenceOf(offsets, refs, k);
+}
+
+// Per-reference coordinates: SA values and ISA rows minus the reference's
+// first position, each checked to stay inside its own reference.
+template <typename Index, typename OutSa, typename OutIsa>
+__global__ __launch_bounds__(256) void LocalizeKernel(
+    const Index* sa, const Index* rank, std::uint64_t n, const std::uint64_t* __restrict__ offsets,
+    std::uint32_t refs, OutSa* out_sa, OutIsa* out_isa, DeviceCounters* counters) {
+  const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const std::uint32_t ref = ReferenceOf(offsets, refs, k);
+  const std::uint64_t begin = offsets[ref], end = offsets[ref + 1];
+  const std::uint64_t v = sa[k];
+  std::uint32_t flags = 0;
+  if (v < begin || v >= end) flags |= kFlagNotPermutation;
+  std::uint64_t isa_v = 0;
+  if (out_isa != nullptr) {
+    isa_v = rank[k];
+    if (isa_v < begin || isa_v >= end) flags |= kFlagNotInverse;
+  }
+  if (flags) {
+    atomicOr(&counters->error_flags, flags);
+    return;
+  }
+  out_sa[k] = static_cast<OutSa>(v - begin);
+  if (out_isa != nullptr) out_isa[k] = static_cast<OutIsa>(isa_v - begin);
 }
 
 // ------------------------------------------------------------ keys --------
@@ -366,7 +428,7 @@ __global__ __launch_bounds__(256) void PlcpIrreducibleKernel(
     val[i] = 0;
     return;
   }
-  if (r == 0) {
+  if (SymbolAt(t, i) == 4u) {  // the sentinel: first row of its reference, LCP 0
     val[i] = static_cast<Index>(i);
     return;
   }
