@@ -42,6 +42,9 @@ WORKLOADS = {
         "3% satellites, N gaps up to 10 Mbp"),
     4: ("batched build: 512 bacterial-like references, log-normal 0.5-12 Mbp (median 4 Mbp), "
         "GC 30-70%, 5% shared mobile elements; one batched device build vs one build per reference"),
+    5: ("served build loop: Poisson arrivals of references log-uniform 1-200 Mbp (64 Mbp-workload "
+        "generator), admission by HBM footprint, indexes resident, 1e6 exact count queries per "
+        "index (length 20-32, 50% sampled from the text, 50% random)"),
 }
 
 
@@ -55,6 +58,10 @@ def parse_args():
     ap.add_argument("--construction-bits", type=int, default=None, choices=(32, 64))
     ap.add_argument("--seed", type=int, default=20260822)
     ap.add_argument("--refs", type=int, default=512, help="references in the batched workload")
+    ap.add_argument("--requests", type=int, default=16, help="requests in the served workload")
+    ap.add_argument("--arrival-ms", type=float, default=20.0, help="mean inter-arrival time (served)")
+    ap.add_argument("--slots", type=int, default=4, help="concurrent builds (served)")
+    ap.add_argument("--queries", type=int, default=1_000_000, help="count queries per index (served)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-symbols", type=int, default=0,
                     help="symbols of the text timed on the CPU (0 = whole text)")
@@ -198,6 +205,60 @@ def run_batched(args) -> int:
     return 0 if identical else 1
 
 
+def run_served(args) -> int:
+    """Workload 5: the serving loop (sufkit_amd/host/serve.cpp)."""
+    import sufkit_amd
+
+    rng = np.random.default_rng(args.seed)
+    texts, bases = [], 0
+    for r in range(args.requests):
+        t, lens = sufkit_amd.synthetic_text(5, args.seed, r, args.scale)
+        texts.append(t)
+        bases += int(lens.sum())
+    arrivals = np.cumsum(rng.exponential(args.arrival_ms / 1e3, size=args.requests)).tolist()
+    # warm-up: one small request through the same loop (first-use costs)
+    sufkit_amd.serve([sufkit_amd.synthetic_text(1, 1, 0, 0.01)[0]], [0.0],
+                     slots=1, queries_per_index=1000)
+    t0 = time.perf_counter()
+    res, summary, samples = sufkit_amd.serve(texts, arrivals, slots=args.slots,
+                                             queries_per_index=args.queries, sample_queries=100)
+    wall = time.perf_counter() - t0
+    # Check sampled counts of the smallest requests against a scan of the text.
+    order = np.argsort([t.size for t in texts])[:3]
+    bad = 0
+    for i in order.tolist():
+        s = "".join("$|ACGTN"[c] for c in texts[i].tolist())
+        pats, lens_, cnts = samples[0][i], samples[1][i], samples[2][i]
+        for code, length, got in zip(pats, lens_, cnts):
+            p = sufkit_amd.decode_pattern(code, length)
+            want, start = 0, s.find(p)
+            while start != -1:
+                want += 1
+                start = s.find(p, start + 1)
+            bad += int(want != got)
+    lat = [r["built"] - r["arrival"] for r in res]
+    result = {
+        "metric": "SA+ISA+LCP build Mbp/s + HBM roofline frac, 1 MI355X vs divsufsort64 on host",
+        "value": round(summary["build_mbps"], 2), "unit": "Mbp/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(summary["makespan_seconds"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (deterministic splitmix64 generator, seed %d)" % args.seed,
+        "config": {"workload": WORKLOADS[5], "requests": args.requests, "bases": bases,
+                   "mean_interarrival_ms": args.arrival_ms, "slots": args.slots,
+                   "queries_per_index": args.queries},
+        "latency_p50_s": round(summary["latency_p50"], 4), "latency_p99_s": round(summary["latency_p99"], 4),
+        "end_to_end_p50_s": round(summary["end_to_end_p50"], 4),
+        "end_to_end_p99_s": round(summary["end_to_end_p99"], 4),
+        "queries_per_second": round(summary["queries_per_second"], 1),
+        "max_concurrent_builds": summary["max_concurrent"],
+        "peak_reserved_GB": round(summary["peak_reserved_bytes"] / 1e9, 2),
+        "requests_failed": summary["requests_failed"], "sampled_count_mismatches": bad,
+        "wall_seconds": round(wall, 3), "latency_max_s": round(max(lat), 4),
+    }
+    print(json.dumps(result), flush=True)
+    return 0 if bad == 0 and summary["requests_failed"] == 0 else 1
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -206,6 +267,8 @@ def main() -> int:
         return 2
     if args.workload == 4:
         return run_batched(args)
+    if args.workload == 5:
+        return run_served(args)
     import sufkit_amd
 
     text, contigs = sufkit_amd.synthetic_text(args.workload, args.seed, 0, args.scale)

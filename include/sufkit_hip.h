@@ -138,6 +138,35 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits,
                   uint32_t isa_bits, void* lcp, uint32_t lcp_bits,
                   sfk_hip_stats* stats, char* error, size_t error_capacity);
 
+/*
+ * Resident index for exact count queries (serving).  The index borrows the
+ * caller's device text (n symbols, sentinel last) and SA (sa_bits 32 or 64)
+ * and packs the text into its own 2-bit planes; the caller keeps d_text and
+ * d_sa alive until sfk_hip_index_destroy.  Patterns are up to 32 A/C/G/T
+ * symbols as 2-bit codes (A=0 .. T=3), the first symbol in the top bits of a
+ * uint64, with one length byte each.  Counting follows SuffixArray::Count
+ * (suffix_array.cpp:4413-4430): the number of suffixes that start with the
+ * pattern; N, separators and the sentinel never match.  `stream` is a
+ * hipStream_t to enqueue on, or NULL for the index's own stream with a
+ * synchronous return.  No counterpart in the reference (its queries run on
+ * the host).
+ */
+typedef struct sfk_hip_index sfk_hip_index;
+int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n,
+                         const void* d_sa, uint32_t sa_bits, sfk_hip_index** out,
+                         char* error, size_t error_capacity);
+void sfk_hip_index_destroy(sfk_hip_index* index);
+uint64_t sfk_hip_index_device_bytes(const sfk_hip_index* index);
+/* Deterministic query set: even queries sampled from the text, odd ones
+ * uniformly random; lengths uniform in [min_len, max_len] (1..32). */
+int sfk_hip_index_generate_queries(sfk_hip_index* index, uint64_t seed,
+                                   uint64_t count, uint32_t min_len,
+                                   uint32_t max_len, uint64_t* d_patterns,
+                                   uint8_t* d_lengths, void* stream);
+int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns,
+                        const uint8_t* d_lengths, uint64_t count,
+                        uint32_t* d_counts, void* stream);
+
 /* Device memory helpers so callers without a device allocator of their own
  * can stage buffers (tests, benchmarks). */
 int sfk_hip_malloc(int device, uint64_t bytes, void** out);

@@ -326,3 +326,71 @@ def hip_build_batch(texts: list[np.ndarray], construction_bits: int = 32, out_bi
             builder.close()
     parts = [(sa[a:b], isa[a:b], lcp[a:b]) for a, b in zip(offsets[:-1], offsets[1:])]
     return parts, stats
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("arrival_seconds", ctypes.c_double), ("text", ctypes.c_void_p), ("n", ctypes.c_uint64)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("queries_per_index", ctypes.c_uint64), ("min_query_len", ctypes.c_uint32),
+                ("max_query_len", ctypes.c_uint32), ("query_seed", ctypes.c_uint64),
+                ("keep_resident", ctypes.c_uint32), ("sample_queries", ctypes.c_uint32),
+                ("digest_outputs", ctypes.c_uint32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("arrival", ctypes.c_double), ("admitted", ctypes.c_double), ("built", ctypes.c_double),
+                ("answered", ctypes.c_double), ("device_build_seconds", ctypes.c_double),
+                ("n", ctypes.c_uint64), ("footprint_bytes", ctypes.c_uint64), ("count_sum", ctypes.c_uint64),
+                ("count_hash", ctypes.c_uint64), ("output_hash", ctypes.c_uint64 * 3),
+                ("status", ctypes.c_int), ("slot", ctypes.c_int)]
+
+
+class ServeSummary(ctypes.Structure):
+    _fields_ = [("makespan_seconds", ctypes.c_double), ("latency_p50", ctypes.c_double),
+                ("latency_p99", ctypes.c_double), ("end_to_end_p50", ctypes.c_double),
+                ("end_to_end_p99", ctypes.c_double), ("build_mbps", ctypes.c_double),
+                ("queries_per_second", ctypes.c_double), ("peak_reserved_bytes", ctypes.c_uint64),
+                ("requests_failed", ctypes.c_uint64), ("max_concurrent", ctypes.c_uint64),
+                ("evicted", ctypes.c_uint64)]
+
+
+def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
+          hbm_budget_bytes: int = 200 << 30, queries_per_index: int = 1_000_000,
+          min_query_len: int = 20, max_query_len: int = 32, query_seed: int = 7,
+          keep_resident: bool = True, sample_queries: int = 0, digest_outputs: bool = False):
+    """Run the serving loop (sufkit_amd/host/serve.cpp) over host texts with
+    the given arrival times.  Returns (results, summary, samples)."""
+    lib = load_host()
+    lib.sfk_serve_run.argtypes = [ctypes.POINTER(ServeRequest), ctypes.c_uint64, ctypes.POINTER(ServeConfig),
+                                  ctypes.POINTER(ServeResult), ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.POINTER(ServeSummary), ctypes.c_char_p,
+                                  ctypes.c_size_t]
+    lib.sfk_serve_run.restype = ctypes.c_int
+    texts = [np.ascontiguousarray(t, dtype=np.uint8) for t in texts]
+    count = len(texts)
+    reqs = (ServeRequest * count)(*[ServeRequest(a, t.ctypes.data, t.size) for a, t in zip(arrivals, texts)])
+    cfg = ServeConfig(hbm_budget_bytes, slots, queries_per_index, min_query_len, max_query_len,
+                      query_seed, int(keep_resident), sample_queries, int(digest_outputs))
+    results = (ServeResult * count)()
+    summary = ServeSummary()
+    sp = np.zeros(max(1, count * sample_queries), np.uint64)
+    sl = np.zeros(max(1, count * sample_queries), np.uint8)
+    sc = np.zeros(max(1, count * sample_queries), np.uint32)
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_serve_run(reqs, count, ctypes.byref(cfg), results, sp.ctypes.data, sl.ctypes.data,
+                               sc.ctypes.data, ctypes.byref(summary), err, len(err))
+    out = [{name: (list(getattr(r, name)) if name == "output_hash" else getattr(r, name))
+            for name, _ in ServeResult._fields_} for r in results]
+    summ = {name: getattr(summary, name) for name, _ in ServeSummary._fields_}
+    if status != 0 and not any(r["status"] == 0 for r in out):
+        raise SufkitError(status, err.value.decode(errors="replace") or "serving failed")
+    samples = (sp.reshape(count, -1), sl.reshape(count, -1), sc.reshape(count, -1)) if sample_queries else None
+    return out, summ, samples
+
+
+def decode_pattern(code: int, length: int) -> str:
+    """2-bit packed pattern (first symbol in the top bits) to an A/C/G/T string."""
+    return "".join("ACGT"[(int(code) >> (62 - 2 * j)) & 3] for j in range(int(length)))

@@ -27,6 +27,7 @@
 #include <utility>
 #include <vector>
 
+#include "query_kernels.hpp"
 #include "radix_sort.hpp"
 #include "sa_kernels.hpp"
 #include "sufkit_hip.h"
@@ -52,13 +53,15 @@ struct Failure {
     }                                                                        \
   } while (0)
 
+inline std::uint64_t CeilDiv(std::uint64_t a, std::uint64_t b) { return (a + b - 1) / b; }
+
 int BitsFor(std::uint64_t max_value) {
   int bits = 0;
   while (bits < 64 && (max_value >> bits) != 0) ++bits;
   return bits;
 }
 
-std::uint64_t CeilDiv(std::uint64_t a, std::uint64_t b) { return (a + b - 1) / b; }
+
 
 class DeviceBuffer {
  public:
@@ -806,6 +809,120 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, u
   d_lcp.Release();
   sfk_hip_builder_destroy(b);
   return status;
+}
+
+// ------------------------------------------------------ resident index ---
+
+struct sfk_hip_index {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  const uint8_t* d_text = nullptr;
+  uint64_t n = 0;
+  const void* d_sa = nullptr;
+  uint32_t sa_bits = 32;
+  sfk::DeviceBuffer codes, spec, counters;
+  sfk::DeviceCounters* host_counters = nullptr;
+};
+
+void sfk_hip_index_destroy(sfk_hip_index* index) {
+  if (index == nullptr) return;
+  (void)hipSetDevice(index->device);
+  if (index->stream != nullptr) (void)hipStreamSynchronize(index->stream);
+  if (index->host_counters != nullptr) (void)hipHostFree(index->host_counters);
+  if (index->stream != nullptr) (void)hipStreamDestroy(index->stream);
+  delete index;
+}
+
+int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const void* d_sa,
+                         uint32_t sa_bits, sfk_hip_index** out, char* error,
+                         size_t error_capacity) {
+  if (out == nullptr || d_text == nullptr || d_sa == nullptr || n < 1 ||
+      (sa_bits != 32 && sa_bits != 64)) {
+    sfk::CopyError("index needs a text, its SA and a 32/64-bit SA width", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  *out = nullptr;
+  sfk_hip_index* index = nullptr;
+  try {
+    index = new sfk_hip_index();
+    index->device = device;
+    index->d_text = d_text;
+    index->n = n;
+    index->d_sa = d_sa;
+    index->sa_bits = sa_bits;
+    SFK_HIP_CHECK(hipSetDevice(device));
+    SFK_HIP_CHECK(hipStreamCreateWithFlags(&index->stream, hipStreamNonBlocking));
+    SFK_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&index->host_counters),
+                                sizeof(sfk::DeviceCounters), hipHostMallocDefault));
+    const uint64_t words = sfk::PackedWords(n);
+    index->codes.Reserve(words * 8);
+    index->spec.Reserve(words * 4);
+    index->counters.Reserve(sizeof(sfk::DeviceCounters));
+    *index->host_counters = sfk::DeviceCounters{};
+    SFK_HIP_CHECK(hipMemcpyAsync(index->counters.As<void>(), index->host_counters,
+                                 sizeof(sfk::DeviceCounters), hipMemcpyHostToDevice, index->stream));
+    sfk::PackTextKernel<<<static_cast<unsigned>(sfk::CeilDiv(words, 256)), 256, 0, index->stream>>>(
+        d_text, n, words, index->codes.As<uint64_t>(), index->spec.As<uint32_t>(),
+        (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, false,
+        index->counters.As<sfk::DeviceCounters>());
+    SFK_HIP_CHECK(hipGetLastError());
+    SFK_HIP_CHECK(hipMemcpyAsync(index->host_counters, index->counters.As<void>(),
+                                 sizeof(sfk::DeviceCounters), hipMemcpyDeviceToHost, index->stream));
+    SFK_HIP_CHECK(hipStreamSynchronize(index->stream));
+    if (index->host_counters->error_flags != 0) {
+      sfk::Fail(SFK_HIP_INVALID_INPUT, "index text must hold symbols 0..6 with its only 0 last");
+    }
+    *out = index;
+    return SFK_HIP_OK;
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    sfk_hip_index_destroy(index);
+    return f.status;
+  }
+}
+
+int sfk_hip_index_generate_queries(sfk_hip_index* index, uint64_t seed, uint64_t count,
+                                   uint32_t min_len, uint32_t max_len, uint64_t* d_patterns,
+                                   uint8_t* d_lengths, void* stream) {
+  if (index == nullptr || min_len < 1 || max_len > 32 || min_len > max_len) {
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (count == 0) return SFK_HIP_OK;
+  hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  sfk::PackedText t{index->codes.As<uint64_t>(), index->spec.As<uint32_t>(), index->n};
+  sfk::GenerateQueriesKernel<<<static_cast<unsigned>(sfk::CeilDiv(count, 256)), 256, 0, s>>>(
+      t, seed, count, min_len, max_len, d_patterns, d_lengths);
+  if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  if (stream == nullptr && hipStreamSynchronize(s) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const uint8_t* d_lengths,
+                        uint64_t count, uint32_t* d_counts, void* stream) {
+  if (index == nullptr || (count > 0 && (d_patterns == nullptr || d_lengths == nullptr ||
+                                         d_counts == nullptr))) {
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (count == 0) return SFK_HIP_OK;
+  hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  sfk::PackedText t{index->codes.As<uint64_t>(), index->spec.As<uint32_t>(), index->n};
+  const unsigned grid = static_cast<unsigned>(sfk::CeilDiv(count, 256));
+  if (index->sa_bits == 32) {
+    sfk::CountPatternsKernel<uint32_t><<<grid, 256, 0, s>>>(
+        t, static_cast<const uint32_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts);
+  } else {
+    sfk::CountPatternsKernel<uint64_t><<<grid, 256, 0, s>>>(
+        t, static_cast<const uint64_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts);
+  }
+  if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  if (stream == nullptr && hipStreamSynchronize(s) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return SFK_HIP_OK;
+}
+
+uint64_t sfk_hip_index_device_bytes(const sfk_hip_index* index) {
+  return index == nullptr ? 0 : index->codes.Bytes() + index->spec.Bytes();
 }
 
 int sfk_hip_malloc(int device, uint64_t bytes, void** out) {
