@@ -67,6 +67,8 @@ def parse_args():
                     help="symbols of the text timed on the CPU (0 = whole text)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel HIP events (no roofline)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -292,6 +294,13 @@ def main() -> int:
     hip.sfk_hip_memcpy_h2d(d_text, text.ctypes.data, n)
     h2d = time.perf_counter() - t0
 
+    if args.calibrate_copy:
+        hip.sfk_hip_memcpy_d2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        size = args.calibrate_copy << 20
+        src, dst = dmalloc(size), dmalloc(size)
+        hip.sfk_hip_memcpy_d2d(dst, src, size)
+        hip.sfk_hip_free(src)
+        hip.sfk_hip_free(dst)
     builder = sufkit_amd.DeviceBuilder(0)
     flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
     if not args.no_profile:

@@ -173,6 +173,10 @@ int sfk_hip_malloc(int device, uint64_t bytes, void** out);
 int sfk_hip_free(void* ptr);
 int sfk_hip_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int sfk_hip_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
+/* out: LDS bytes per block, LDS bytes per CU, CUs, L2 bytes, HBM bytes,
+ * core clock kHz, memory clock kHz, memory bus width bits. */
+int sfk_hip_device_info(int device, uint64_t out[8]);
 
 #ifdef __cplusplus
 }

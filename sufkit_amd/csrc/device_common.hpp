@@ -1,15 +1,18 @@
 // Device-side building blocks shared by every suffix-index kernel (gfx950).
 //
-// Text layout in HBM (one copy per build):
-//   codes[w]  uint64, symbols 32w..32w+31, symbol 32w+j in bits [63-2j, 62-2j]
-//             (most significant first, so integer order = lexicographic order)
-//   spec[w]   uint32, bit (31-j) set when symbol 32w+j is not a base
-// A base A/C/G/T is code 0..3 with its spec bit clear.  A special symbol has
-// its spec bit set and its kind in the code slot: 0 sentinel, 1 separator,
-// 3 N.  (code, spec) is therefore a lossless encoding of the 7-symbol
-// alphabet of the encoded text: 0 sentinel < 1 separator < A < C < G < T < N.
-// Two zero padding words follow the last symbol so that any 32-symbol window
-// starting at a valid position can be read with two word loads.
+// Text layout in HBM (one copy per build): one 16-byte entry per 32 symbols,
+//   entry[w].x,y  uint64 codes of symbols 32w..32w+31, symbol 32w+j in bits
+//                 [63-2j, 62-2j] (most significant first, so integer order =
+//                 lexicographic order); .x holds the low 32 bits
+//   entry[w].z    uint32 special mask, bit (31-j) set when symbol 32w+j is
+//                 not a base
+//   entry[w].w    unused
+// A base A/C/G/T is code 0..3 with its special bit clear.  A special symbol
+// has its bit set and its kind in the code slot: 0 sentinel, 1 separator,
+// 3 N.  (code, special) is a lossless encoding of the 7-symbol alphabet of the
+// encoded text: 0 sentinel < 1 separator < A < C < G < T < N.  Codes and
+// mask share an entry so that a 32-symbol window starting anywhere is two
+// adjacent 16-byte loads.  Two zero entries follow the last symbol.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,14 +24,17 @@ namespace sfk {
 constexpr int kWave = 64;
 
 struct PackedText {
-  const std::uint64_t* codes;
-  const std::uint32_t* spec;
+  const uint4* words;
   std::uint64_t n;
 };
 
-// Number of 32-symbol words for a text of n symbols, padding included.
+// Number of 32-symbol entries for a text of n symbols, padding included.
 __host__ __device__ inline std::uint64_t PackedWords(std::uint64_t n) {
   return (n + 31) / 32 + 2;
+}
+
+__device__ inline std::uint64_t EntryCodes(const uint4& e) {
+  return (static_cast<std::uint64_t>(e.y) << 32) | e.x;
 }
 
 // The 32-symbol window starting at position i: 2-bit codes and spec mask.
@@ -36,11 +42,12 @@ __device__ inline void LoadWindow(const PackedText& t, std::uint64_t i,
                                   std::uint64_t& codes, std::uint32_t& spec) {
   const std::uint64_t w = i >> 5;
   const std::uint32_t o = static_cast<std::uint32_t>(i & 31);
-  const std::uint64_t c0 = t.codes[w];
-  const std::uint64_t c1 = t.codes[w + 1];
+  const uint4 e0 = t.words[w];
+  const uint4 e1 = t.words[w + 1];
+  const std::uint64_t c0 = EntryCodes(e0);
+  const std::uint64_t c1 = EntryCodes(e1);
   codes = o ? (c0 << (2 * o)) | (c1 >> (64 - 2 * o)) : c0;
-  const std::uint64_t s01 =
-      (static_cast<std::uint64_t>(t.spec[w]) << 32) | t.spec[w + 1];
+  const std::uint64_t s01 = (static_cast<std::uint64_t>(e0.z) << 32) | e1.z;
   spec = static_cast<std::uint32_t>((s01 << o) >> 32);
 }
 
@@ -48,9 +55,10 @@ __device__ inline void LoadWindow(const PackedText& t, std::uint64_t i,
 __device__ inline std::uint32_t SymbolAt(const PackedText& t, std::uint64_t p) {
   const std::uint64_t w = p >> 5;
   const std::uint32_t o = static_cast<std::uint32_t>(p & 31);
+  const uint4 e = t.words[w];
   const std::uint32_t code =
-      static_cast<std::uint32_t>(t.codes[w] >> (62 - 2 * o)) & 3u;
-  const std::uint32_t sp = (t.spec[w] >> (31 - o)) & 1u;
+      static_cast<std::uint32_t>(EntryCodes(e) >> (62 - 2 * o)) & 3u;
+  const std::uint32_t sp = (e.z >> (31 - o)) & 1u;
   return code | (sp << 2);
 }
 

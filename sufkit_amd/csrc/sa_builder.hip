@@ -111,7 +111,7 @@ struct EventPair {
 struct sfk_hip_builder {
   int device = 0;
   hipStream_t stream = nullptr;
-  sfk::DeviceBuffer codes, spec, rank, sa, act_idx, act_seg, keys0, keys1,
+  sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
       vals0, vals1, seg_delta0, seg_delta1, radix_counts,
       partial, counters, narrow, ref_offsets;
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
@@ -410,8 +410,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   const std::uint64_t words = PackedWords(n);
 
   // Workspace (grow-only).
-  b->codes.Reserve(words * 8);
-  b->spec.Reserve(words * 4);
+  b->packed.Reserve(words * 16);
   const bool sa_direct = sa_bits == kIndexBits;
   const bool isa_direct = d_isa != nullptr && isa_bits == kIndexBits;
   if (!sa_direct) b->sa.Reserve(n * sizeof(Index));
@@ -430,12 +429,12 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
 
   Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
   Index* rank = isa_direct ? static_cast<Index*>(d_isa) : b->rank.As<Index>();
-  PackedText text{b->codes.As<std::uint64_t>(), b->spec.As<std::uint32_t>(), n};
+  PackedText text{b->packed.As<uint4>(), n};
 
   ResetCounters(b);
   SFK_HIP_CHECK(hipEventRecord(b->phase[0], b->stream));
   PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
-      d_text, n, words, b->codes.As<std::uint64_t>(), b->spec.As<std::uint32_t>(),
+      d_text, n, words, b->packed.As<uint4>(),
       (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, ctx.refs > 0,
       b->counters.As<DeviceCounters>());
   SFK_HIP_CHECK(hipGetLastError());
@@ -444,7 +443,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
         d_text, ctx.d_offsets, ctx.refs, b->counters.As<DeviceCounters>());
     SFK_HIP_CHECK(hipGetLastError());
   }
-  ctx.stats.algorithmic_bytes_sa += n + words * 12;
+  ctx.stats.algorithmic_bytes_sa += n + words * 16;
 
   Doubling<Index>(ctx, sa, rank, text).Run();
   SFK_HIP_CHECK(hipEventRecord(b->phase[1], b->stream));
@@ -683,7 +682,7 @@ void* sfk_hip_builder_stream(sfk_hip_builder* b) {
 uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
   const uint64_t w = construction_bits == 64 ? 8 : 4;
   const uint64_t words = sfk::PackedWords(n);
-  return words * 12 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
+  return words * 16 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
          (n / 2 + 2) * w * 2 /*segment deltas*/ + n * w * 2 /*sa, rank when not direct*/ +
          sfk::RadixTile::kRadix * 1024 * 8 + 1024 * sizeof(sfk::RefinePartial);
 }
@@ -820,7 +819,7 @@ struct sfk_hip_index {
   uint64_t n = 0;
   const void* d_sa = nullptr;
   uint32_t sa_bits = 32;
-  sfk::DeviceBuffer codes, spec, counters;
+  sfk::DeviceBuffer packed, counters;
   sfk::DeviceCounters* host_counters = nullptr;
 };
 
@@ -855,14 +854,13 @@ int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const vo
     SFK_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&index->host_counters),
                                 sizeof(sfk::DeviceCounters), hipHostMallocDefault));
     const uint64_t words = sfk::PackedWords(n);
-    index->codes.Reserve(words * 8);
-    index->spec.Reserve(words * 4);
+    index->packed.Reserve(words * 16);
     index->counters.Reserve(sizeof(sfk::DeviceCounters));
     *index->host_counters = sfk::DeviceCounters{};
     SFK_HIP_CHECK(hipMemcpyAsync(index->counters.As<void>(), index->host_counters,
                                  sizeof(sfk::DeviceCounters), hipMemcpyHostToDevice, index->stream));
     sfk::PackTextKernel<<<static_cast<unsigned>(sfk::CeilDiv(words, 256)), 256, 0, index->stream>>>(
-        d_text, n, words, index->codes.As<uint64_t>(), index->spec.As<uint32_t>(),
+        d_text, n, words, index->packed.As<uint4>(),
         (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, false,
         index->counters.As<sfk::DeviceCounters>());
     SFK_HIP_CHECK(hipGetLastError());
@@ -890,7 +888,7 @@ int sfk_hip_index_generate_queries(sfk_hip_index* index, uint64_t seed, uint64_t
   if (count == 0) return SFK_HIP_OK;
   hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
   if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
-  sfk::PackedText t{index->codes.As<uint64_t>(), index->spec.As<uint32_t>(), index->n};
+  sfk::PackedText t{index->packed.As<uint4>(), index->n};
   sfk::GenerateQueriesKernel<<<static_cast<unsigned>(sfk::CeilDiv(count, 256)), 256, 0, s>>>(
       t, seed, count, min_len, max_len, d_patterns, d_lengths);
   if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
@@ -907,7 +905,7 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const 
   if (count == 0) return SFK_HIP_OK;
   hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
   if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
-  sfk::PackedText t{index->codes.As<uint64_t>(), index->spec.As<uint32_t>(), index->n};
+  sfk::PackedText t{index->packed.As<uint4>(), index->n};
   const unsigned grid = static_cast<unsigned>(sfk::CeilDiv(count, 256));
   if (index->sa_bits == 32) {
     sfk::CountPatternsKernel<uint32_t><<<grid, 256, 0, s>>>(
@@ -922,7 +920,7 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const 
 }
 
 uint64_t sfk_hip_index_device_bytes(const sfk_hip_index* index) {
-  return index == nullptr ? 0 : index->codes.Bytes() + index->spec.Bytes();
+  return index == nullptr ? 0 : index->packed.Bytes();
 }
 
 int sfk_hip_malloc(int device, uint64_t bytes, void** out) {
@@ -933,6 +931,28 @@ int sfk_hip_malloc(int device, uint64_t bytes, void** out) {
     *out = nullptr;
     return SFK_HIP_BUILD_FAILURE;
   }
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_memcpy_d2d(void* dst, const void* src, uint64_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? SFK_HIP_OK
+                                                                           : SFK_HIP_BUILD_FAILURE;
+}
+
+int sfk_hip_device_info(int device, uint64_t out[8]) {
+  hipDeviceProp_t prop;
+  if (out == nullptr || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    (void)hipGetLastError();
+    return SFK_HIP_UNSUPPORTED_BACKEND;
+  }
+  out[0] = prop.sharedMemPerBlock;
+  out[1] = prop.maxSharedMemoryPerMultiProcessor;
+  out[2] = static_cast<uint64_t>(prop.multiProcessorCount);
+  out[3] = static_cast<uint64_t>(prop.l2CacheSize);
+  out[4] = prop.totalGlobalMem;
+  out[5] = static_cast<uint64_t>(prop.clockRate);
+  out[6] = static_cast<uint64_t>(prop.memoryClockRate);
+  out[7] = static_cast<uint64_t>(prop.memoryBusWidth);
   return SFK_HIP_OK;
 }
 

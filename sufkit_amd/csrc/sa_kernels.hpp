@@ -57,11 +57,10 @@ struct DeviceCounters {
 
 // ------------------------------------------------------------ K0 pack -----
 
-// One thread per 32-symbol word.  Encoded bytes 0..6 -> (code, spec).
+// One thread per 32-symbol entry.  Encoded bytes 0..6 -> (code, spec).
 __global__ __launch_bounds__(256) void PackTextKernel(
     const std::uint8_t* __restrict__ text, std::uint64_t n, std::uint64_t words,
-    std::uint64_t* __restrict__ codes, std::uint32_t* __restrict__ spec,
-    bool aligned, bool batch, DeviceCounters* counters) {
+    uint4* __restrict__ packed, bool aligned, bool batch, DeviceCounters* counters) {
   const std::uint64_t w = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w >= words) return;
   std::uint64_t c = 0;
@@ -111,8 +110,7 @@ __global__ __launch_bounds__(256) void PackTextKernel(
     c |= static_cast<std::uint64_t>(code) << (62 - 2 * j);
     s |= sp << (31 - j);
   }
-  codes[w] = c;
-  spec[w] = s;
+  packed[w] = make_uint4(static_cast<std::uint32_t>(c), static_cast<std::uint32_t>(c >> 32), s, 0u);
   if (flags) atomicOr(&counters->error_flags, flags);
   if (zeros) atomicAdd(&counters->zero_count, static_cast<unsigned long long>(zeros));
 }
