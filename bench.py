@@ -358,7 +358,7 @@ def main() -> int:
     if scatter_n:
         achieved = scatter_b / scatter_s / 1e9
         roofline = {
-            "bound": "hbm", "kernel": "RadixDownsweep (LSD scatter pass)",
+            "bound": "hbm", "kernel": "RadixOnesweep (LSD scatter pass, decoupled look-back)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
             "launches_per_step": scatter_n // len(stats),

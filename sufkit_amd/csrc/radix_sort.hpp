@@ -1,17 +1,14 @@
-// LSD radix sort of (key, value) pairs, one digit per pass, for gfx950.
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, for gfx950.
 //
-// Each pass is reduce-then-scan over a fixed grid of blocks, each block
-// owning one contiguous chunk of whole tiles:
-//   upsweep    per-block digit histogram of its chunk (wave ballots collapse
-//              equal digits so one LDS atomic serves a whole peer group)
-//   spine      one block scans the [digit][block] count matrix into global
-//              output offsets
-//   downsweep  each block walks its chunk tile by tile: ranks every key of
-//              the tile stably with wave ballots (64-lane peer masks), stages
-//              the tile in LDS in digit order, and writes it out so that
-//              consecutive lanes store consecutive addresses of one bucket.
-// Per pass a block reads keys twice and values once, and writes both once:
-// HBM traffic per element = key + 2 * (key + value).
+// A sort is one histogram kernel that counts every pass's digits in a single
+// read of the keys, one tiny scan into bucket starts, and then one onesweep
+// kernel per digit pass: each block takes the next tile id from an atomic
+// counter, ranks its tile stably in LDS with wave ballots (64-lane peer
+// masks), publishes its per-digit counts, finds the counts of all earlier
+// tiles by looking back over their published status words, and writes the
+// tile out in digit order so that consecutive lanes store consecutive
+// addresses of one bucket.  HBM traffic per element: key once for the
+// histogram, then per pass one read and one write of key and value.
 // Digits whose bits are constant over all keys are skipped by the caller.
 #pragma once
 
@@ -24,183 +21,281 @@ struct RadixTile {
   static constexpr int kBlock = 256;
   static constexpr int kDigitBits = 8;
   static constexpr int kRadix = 1 << kDigitBits;
-  static constexpr int kMaxGrid = 1024;
 };
 
-// Items per thread so that a tile's keys and values fit in ~48 KiB of LDS.
-template <typename K, typename V>
-struct RadixItems {
-  static constexpr int kValue = (sizeof(K) + sizeof(V)) <= 8 ? 16 : ((sizeof(K) + sizeof(V)) <= 12 ? 16 : 12);
+// ------------------------------------------------------------- onesweep ---
+//
+// A tile waits only on tiles with smaller ids, which took their ids (and so
+// were running) before it: the look-back always ends.
+//
+// Status word per (tile, digit), 64 bits: [63:62] flag (1 = this tile's count,
+// 2 = inclusive prefix through this tile), [61:48] epoch of the pass, [47:0]
+// count.  Words of other epochs read as "not yet published", so the array is
+// cleared only when the 14-bit epoch wraps.  A word is self-contained (no
+// other memory is published with it), so its agent-scope atomics are relaxed:
+// release/acquire would write back or invalidate the XCD's L2 on every access.
+
+namespace onesweep {
+constexpr std::uint64_t kFlagAggregate = 1ull << 62;
+constexpr std::uint64_t kFlagInclusive = 2ull << 62;
+constexpr std::uint64_t kCountMask = (1ull << 48) - 1;
+constexpr std::uint32_t kEpochMask = (1u << 14) - 1;
+}  // namespace onesweep
+
+// Digit passes of one sort: the bit offset of each 8-bit digit that varies.
+struct RadixPasses {
+  static constexpr int kMax = 8;
+  int count;
+  int shift[kMax];
 };
 
-template <typename K, int kItems>
-__global__ __launch_bounds__(RadixTile::kBlock) void RadixUpsweep(
-    const K* __restrict__ keys, std::uint64_t m, std::uint64_t chunk,
-    int shift, std::uint64_t* __restrict__ counts, int grid) {
-  constexpr int kBlock = RadixTile::kBlock;
+// Clears the digit totals and the tile counters of one sort.
+__global__ __launch_bounds__(256) void RadixSortInit(unsigned long long* __restrict__ hist,
+                                                     unsigned int* __restrict__ tile_counter) {
+  for (int p = 0; p < RadixPasses::kMax; ++p) hist[p * RadixTile::kRadix + threadIdx.x] = 0;
+  if (threadIdx.x < RadixPasses::kMax) tile_counter[threadIdx.x] = 0;
+}
+
+// Digit totals of every pass in one read of the keys: hist[pass][digit]
+// (accumulated with atomics; cleared by RadixSortInit).  Each wave owns an
+// LDS sub-histogram and reads 16 bytes per lane per step.  Keys are often
+// sorted by their high bits (segment ids), so for one key slot the 64 lanes
+// of a wave frequently share a digit: that case is one atomic of 64 instead
+// of 64 atomics on one LDS address.
+template <typename K, int kBlock>
+__global__ __launch_bounds__(kBlock) void RadixHistogramAll(
+    const K* __restrict__ keys, std::uint64_t m, RadixPasses passes,
+    unsigned long long* __restrict__ hist) {
   constexpr int kRadix = RadixTile::kRadix;
-  __shared__ std::uint32_t hist[kRadix];
-  for (int d = threadIdx.x; d < kRadix; d += kBlock) hist[d] = 0;
+  constexpr int kMaxPasses = RadixPasses::kMax;
+  constexpr int kWaves = kBlock / kWave;
+  constexpr int kVec = 16 / sizeof(K);
+  __shared__ std::uint32_t local[kWaves][kMaxPasses][kRadix];
+  const int count = passes.count;
+  for (int i = threadIdx.x; i < kWaves * kMaxPasses * kRadix; i += kBlock) {
+    if ((i / kRadix) % kMaxPasses < count) (&local[0][0][0])[i] = 0;
+  }
   __syncthreads();
-  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
-  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
+  const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  for (std::uint64_t base = begin; base < end; base += kBlock * kItems) {
+  const std::uint64_t nvec = m / kVec;
+  const uint4* vkeys = reinterpret_cast<const uint4*>(keys);
+  // A wave takes kUnroll consecutive 64-vector groups per step, loading all
+  // of them before counting so the loads overlap.
+  constexpr int kUnroll = 4;
+  constexpr std::uint64_t kWaveSpan = static_cast<std::uint64_t>(kWave) * kUnroll;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kBlock * kUnroll;
+  for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWaveSpan;
+       wbase < nvec; wbase += stride) {
+    if (wbase + kWaveSpan <= nvec) {
+      // Whole span in range: wave-wide predicates are well defined.
+      uint4 q[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const std::uint64_t idx = base + static_cast<std::uint64_t>(j) * kBlock + threadIdx.x;
-      const bool valid = idx < end;
-      const std::uint32_t d =
-          valid ? static_cast<std::uint32_t>(keys[idx] >> shift) & (kRadix - 1) : 0u;
-      const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
-      if (valid && lane == static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(peers)) - 1)) {
-        atomicAdd(&hist[d], static_cast<std::uint32_t>(__popcll(peers)));
+      for (int u = 0; u < kUnroll; ++u) q[u] = vkeys[wbase + u * kWave + lane];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        K ks[kVec];
+        __builtin_memcpy(ks, &q[u], sizeof(uint4));
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) {
+          for (int p = 0; p < count; ++p) {
+            const std::uint32_t d = static_cast<std::uint32_t>(ks[j] >> passes.shift[p]) & (kRadix - 1);
+            const std::uint32_t first = __builtin_amdgcn_readfirstlane(d);
+            if (__all(d == first)) {
+              if (lane == 0) atomicAdd(&local[wave][p][first], static_cast<std::uint32_t>(kWave));
+            } else {
+              atomicAdd(&local[wave][p][d], 1u);
+            }
+          }
+        }
+      }
+    } else {
+      for (int u = 0; u < kUnroll; ++u) {
+        const std::uint64_t v = wbase + u * kWave + lane;
+        if (v >= nvec) break;
+        const uint4 q = vkeys[v];
+        K ks[kVec];
+        __builtin_memcpy(ks, &q, sizeof(q));
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) {
+          for (int p = 0; p < count; ++p) {
+            atomicAdd(&local[wave][p][static_cast<std::uint32_t>(ks[j] >> passes.shift[p]) & (kRadix - 1)], 1u);
+          }
+        }
+      }
+    }
+  }
+  // Keys past the last whole 16-byte vector.
+  if (blockIdx.x == 0) {
+    const std::uint64_t i = nvec * kVec + threadIdx.x;
+    if (threadIdx.x < kVec && i < m) {
+      const K k = keys[i];
+      for (int p = 0; p < count; ++p) {
+        atomicAdd(&local[wave][p][static_cast<std::uint32_t>(k >> passes.shift[p]) & (kRadix - 1)], 1u);
       }
     }
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kBlock) {
-    counts[static_cast<std::uint64_t>(d) * grid + blockIdx.x] = hist[d];
+  for (int i = threadIdx.x; i < count * kRadix; i += kBlock) {
+    const int p = i / kRadix, d = i % kRadix;
+    std::uint32_t sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) sum += local[w][p][d];
+    if (sum) atomicAdd(&hist[p * kRadix + d], static_cast<unsigned long long>(sum));
   }
 }
 
-// Spine, one block per digit: exclusive scan (in place) of that digit's row
-// of per-block counts, and the digit's total in digit_total[digit].  The
-// downsweep adds the exclusive prefix of digit totals itself.
-__global__ __launch_bounds__(256) void RadixSpine(std::uint64_t* __restrict__ counts,
-                                                  int grid,
-                                                  std::uint64_t* __restrict__ digit_total) {
+// Exclusive scan of each pass's digit totals into bucket starts (in place).
+__global__ __launch_bounds__(256) void RadixBucketStarts(unsigned long long* hist, int passes) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  std::uint64_t* row = counts + static_cast<std::uint64_t>(blockIdx.x) * grid;
-  const int per = (grid + kBlock - 1) / kBlock;  // grid <= kMaxGrid = 1024 -> per <= 4
-  const int begin = threadIdx.x * per;
-  std::uint64_t vals[4];
-  std::uint64_t local = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    vals[i] = (i < per && begin + i < grid) ? row[begin + i] : 0ull;
-    local += vals[i];
+  for (int p = 0; p < passes; ++p) {
+    std::uint64_t total;
+    const std::uint64_t v = hist[p * kBlock + threadIdx.x];
+    const std::uint64_t ex = BlockExclusiveSum<kBlock>(v, scratch, total);
+    hist[p * kBlock + threadIdx.x] = ex;
+    __syncthreads();
   }
-  std::uint64_t total;
-  std::uint64_t running = BlockExclusiveSum<kBlock>(local, scratch, total);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if (i < per && begin + i < grid) {
-      row[begin + i] = running;
-      running += vals[i];
-    }
-  }
-  if (threadIdx.x == 0) digit_total[blockIdx.x] = total;
 }
 
-template <typename K, typename V, int kItems>
-__global__ __launch_bounds__(RadixTile::kBlock) void RadixDownsweep(
-    const K* __restrict__ keys_in, const V* __restrict__ vals_in,
-    K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m,
-    std::uint64_t chunk, int shift, const std::uint64_t* __restrict__ offsets,
-    const std::uint64_t* __restrict__ digit_total, int grid) {
-  constexpr int kBlock = RadixTile::kBlock;
+template <typename K, typename V, int kBlock, int kItems>
+__global__ __launch_bounds__(kBlock) void RadixOnesweep(
+    const K* __restrict__ keys_in, const V* __restrict__ vals_in, K* __restrict__ keys_out,
+    V* __restrict__ vals_out, std::uint64_t m, int shift,
+    const unsigned long long* __restrict__ bucket_start, unsigned long long* status,
+    unsigned int* tile_counter, std::uint32_t epoch, unsigned int* error_flags,
+    std::uint32_t error_bit) {
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
   constexpr int kTile = kBlock * kItems;
-  __shared__ std::uint64_t run_off[kRadix];
+  static_assert(kBlock >= kRadix, "one digit per thread for the look-back");
   __shared__ std::uint32_t wave_cnt[kWaves][kRadix];
   __shared__ std::uint32_t local_start[kRadix];
-  __shared__ std::uint32_t tile_count[kRadix];
+  __shared__ std::uint64_t global_base[kRadix];
   __shared__ std::uint32_t scan_scratch[kBlock / kWave];
+  __shared__ unsigned int s_tile;
   __shared__ K s_keys[kTile];
   __shared__ V s_vals[kTile];
 
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  {
-    static_assert(kRadix == kBlock, "one digit per thread");
-    __shared__ std::uint64_t base_scratch[kBlock / kWave];
-    std::uint64_t total;
-    const std::uint64_t digit_base =
-        BlockExclusiveSum<kBlock>(digit_total[threadIdx.x], base_scratch, total);
-    run_off[threadIdx.x] =
-        digit_base + offsets[static_cast<std::uint64_t>(threadIdx.x) * grid + blockIdx.x];
+  if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;
+  __syncthreads();
+  const std::uint64_t tile = s_tile;
+  const std::uint64_t base = tile * kTile;
+  const std::uint64_t end = base + kTile < m ? base + kTile : m;
+
+  K key[kItems];
+  V val[kItems];
+  std::uint32_t rank[kItems];
+  std::uint32_t digit[kItems];
+  const std::uint64_t wave_base = base + static_cast<std::uint64_t>(wave) * kItems * kWave;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
+    const bool valid = idx < end;
+    key[j] = valid ? keys_in[idx] : K(0);
+    val[j] = valid ? vals_in[idx] : V(0);
+    const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
+    digit[j] = d;
+    const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
+    const int leader = peers ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : static_cast<int>(lane);
+    std::uint32_t old = 0;
+    if (valid && static_cast<int>(lane) == leader) {
+      old = wave_cnt[wave][d];
+      wave_cnt[wave][d] = old + static_cast<std::uint32_t>(__popcll(peers));
+    }
+    __builtin_amdgcn_wave_barrier();
+    old = __shfl(old, leader, kWave);
+    rank[j] = old + static_cast<std::uint32_t>(__popcll(peers & LaneMaskLt()));
   }
-  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
-  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
-  for (std::uint64_t base = begin; base < end; base += kTile) {
-    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) {
-      (&wave_cnt[0][0])[i] = 0;
-    }
-    __syncthreads();
-    K key[kItems];
-    V val[kItems];
-    std::uint32_t rank[kItems];
-    std::uint32_t digit[kItems];
-    const std::uint64_t wave_base =
-        base + static_cast<std::uint64_t>(wave) * kItems * kWave;
+  __syncthreads();
+  std::uint32_t my_count = 0;
+  if (threadIdx.x < kRadix) {
+    const int d = threadIdx.x;
+    std::uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
-      const bool valid = idx < end;
-      key[j] = valid ? keys_in[idx] : K(0);
-      val[j] = valid ? vals_in[idx] : V(0);
-      const std::uint32_t d =
-          static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
-      digit[j] = d;
-      const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
-      const int leader = peers ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : static_cast<int>(lane);
-      std::uint32_t old = 0;
-      if (valid && static_cast<int>(lane) == leader) {
-        old = wave_cnt[wave][d];
-        wave_cnt[wave][d] = old + static_cast<std::uint32_t>(__popcll(peers));
-      }
-      __builtin_amdgcn_wave_barrier();
-      old = __shfl(old, leader, kWave);
-      rank[j] = old + static_cast<std::uint32_t>(__popcll(peers & LaneMaskLt()));
+    for (int w = 0; w < kWaves; ++w) {
+      const std::uint32_t c = wave_cnt[w][d];
+      wave_cnt[w][d] = s;
+      s += c;
     }
-    __syncthreads();
-    // Per digit: prefix over waves (in place) and the tile total.
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) {
-      std::uint32_t s = 0;
+    my_count = s;
+    // Publish this tile's count, then look back for the exclusive prefix.
+    const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
+    unsigned long long* slot = status + tile * kRadix + d;
+    if (tile == 0) {
+      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      global_base[d] = bucket_start[d];
+    } else {
+      __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      // The wait is bounded: past kSpinLimit polls of one word the tile
+      // raises `error_bit` and stops waiting, and every waiting tile stops
+      // once the bit is up, so a broken protocol ends the kernel with an
+      // error instead of hanging it.
+      constexpr std::uint32_t kSpinLimit = 1u << 20;
+      std::uint64_t prefix = 0;
+      std::int64_t t = static_cast<std::int64_t>(tile) - 1;
+      std::uint32_t spins = 0;
+      while (true) {
+        const std::uint64_t word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * kRadix + d,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const std::uint64_t flag = word & (3ull << 62);
+        if (flag == 0 || ((word >> 48) & onesweep::kEpochMask) != (epoch & onesweep::kEpochMask)) {
+          // tile t has not published yet in this pass
+          if (++spins >= kSpinLimit) {
+            atomicOr(error_flags, error_bit);
+            break;
+          }
+          if ((spins & 63u) == 0 &&
+              (__hip_atomic_load(error_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & error_bit)) {
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        spins = 0;
+        prefix += word & onesweep::kCountMask;
+        if (flag == onesweep::kFlagInclusive) break;
+        --t;
+      }
+      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (prefix + s), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      global_base[d] = bucket_start[d] + prefix;
+    }
+  }
+  __syncthreads();
+  {
+    // Exclusive scan of the tile's digit counts (threads >= kRadix add 0).
+    std::uint32_t total;
+    const std::uint32_t v = threadIdx.x < kRadix ? my_count : 0u;
+    const std::uint32_t ex = BlockExclusiveSum<kBlock>(v, scan_scratch, total);
+    if (threadIdx.x < kRadix) local_start[threadIdx.x] = ex;
+  }
+  __syncthreads();
 #pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        const std::uint32_t c = wave_cnt[w][d];
-        wave_cnt[w][d] = s;
-        s += c;
-      }
-      tile_count[d] = s;
+  for (int j = 0; j < kItems; ++j) {
+    const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
+    if (idx < end) {
+      const std::uint32_t d = digit[j];
+      const std::uint32_t pos = local_start[d] + wave_cnt[wave][d] + rank[j];
+      s_keys[pos] = key[j];
+      s_vals[pos] = val[j];
     }
-    __syncthreads();
-    {
-      static_assert(kRadix == kBlock, "one digit per thread");
-      std::uint32_t total;
-      local_start[threadIdx.x] = BlockExclusiveSum<kBlock>(
-          tile_count[threadIdx.x], scan_scratch, total);
+  }
+  __syncthreads();
+  const std::uint32_t tile_n = static_cast<std::uint32_t>(end > base ? end - base : 0);
+  for (std::uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
+    const K k = s_keys[p];
+    const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
+    const std::uint64_t out = global_base[d] + (p - local_start[d]);
+    if (out < m) {
+      keys_out[out] = k;
+      vals_out[out] = s_vals[p];
     }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
-      if (idx < end) {
-        const std::uint32_t d = digit[j];
-        const std::uint32_t pos = local_start[d] + wave_cnt[wave][d] + rank[j];
-        s_keys[pos] = key[j];
-        s_vals[pos] = val[j];
-      }
-    }
-    __syncthreads();
-    const std::uint32_t tile_n = static_cast<std::uint32_t>(
-        end - base < static_cast<std::uint64_t>(kTile) ? end - base : kTile);
-    for (std::uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
-      const K k = s_keys[p];
-      const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
-      const std::uint64_t out = run_off[d] + (p - local_start[d]);
-      if (out < m) {
-        keys_out[out] = k;
-        vals_out[out] = s_vals[p];
-      }
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) run_off[d] += tile_count[d];
-    __syncthreads();
   }
 }
 

@@ -70,8 +70,9 @@ class DeviceBuffer {
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
   ~DeviceBuffer() { Release(); }
 
-  void Reserve(std::uint64_t bytes) {
-    if (bytes <= bytes_) return;
+  // Returns true when the buffer was (re)allocated, i.e. its contents are new.
+  bool Reserve(std::uint64_t bytes) {
+    if (bytes <= bytes_) return false;
     Release();
     void* ptr = nullptr;
     const hipError_t err = hipMalloc(&ptr, bytes);
@@ -82,6 +83,7 @@ class DeviceBuffer {
     }
     ptr_ = ptr;
     bytes_ = bytes;
+    return true;
   }
   void Release() {
     if (ptr_ != nullptr) (void)hipFree(ptr_);
@@ -112,13 +114,13 @@ struct sfk_hip_builder {
   int device = 0;
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
-      vals0, vals1, seg_delta0, seg_delta1, radix_counts,
+      vals0, vals1, seg_delta0, seg_delta1, radix_counts, radix_status,
       partial, counters, narrow, ref_offsets;
+  std::uint32_t sort_epoch = 0;  // epoch of the last onesweep pass
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
   std::size_t events_used = 0;
   hipEvent_t phase[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int radix_grid = 512;
 };
 
 namespace sfk {
@@ -192,31 +194,110 @@ SortGeometry Geometry(std::uint64_t m, int tile, int max_grid) {
   return SortGeometry{static_cast<int>(grid), per * static_cast<std::uint64_t>(tile)};
 }
 
-// Stable LSD sort of m (key, value) pairs over the digits in which `varying`
-// has a set bit.  On return keys/vals point at the sorted buffers.
+// Onesweep tile shapes, chosen per pass by key/value width and element
+// count (measured on MI355X: large tiles write longer runs per bucket and
+// look back over fewer tiles; below ~2M elements smaller tiles keep more
+// blocks in flight).
+//   u32 key + u32 value, m >= 4M   1024 threads x 16 items  (128 KiB tile)
+//   other widths,        m >= 2M   1024 threads x  8 items  (96-128 KiB)
+//   otherwise                       512 threads x  8 items
+constexpr std::uint64_t kMinOnesweepTile = 512 * 8;
+
 template <typename K, typename V>
-void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
-               std::uint64_t m, std::uint64_t varying, int total_bits) {
+struct OnesweepShape {
+  static constexpr bool kNarrow = sizeof(K) + sizeof(V) == 8;
+  static constexpr int kBigItems = kNarrow ? 16 : 8;
+  static constexpr std::uint64_t kBigFrom = kNarrow ? (4ull << 20) : (2ull << 20);
+};
+
+std::uint64_t OnesweepStatusBytes(std::uint64_t n) {
+  return (CeilDiv(std::max<std::uint64_t>(n, 1), kMinOnesweepTile) + 1) * RadixTile::kRadix * 8;
+}
+
+// Reserves the look-back status array; a new array is cleared (its words
+// then read as "not published" in every epoch).
+void ReserveOnesweep(sfk_hip_builder* b, std::uint64_t n) {
+  if (b->radix_status.Reserve(OnesweepStatusBytes(n))) {
+    SFK_HIP_CHECK(hipMemsetAsync(b->radix_status.As<void>(), 0, b->radix_status.Bytes(), b->stream));
+    b->sort_epoch = 0;
+  }
+}
+
+// Epoch of the next onesweep pass.  The status words carry 14 epoch bits;
+// when they wrap, the array is cleared so no stale word can match.
+std::uint32_t NextEpoch(sfk_hip_builder* b) {
+  b->sort_epoch += 1;
+  if (b->sort_epoch > onesweep::kEpochMask) {
+    SFK_HIP_CHECK(hipMemsetAsync(b->radix_status.As<void>(), 0, b->radix_status.Bytes(), b->stream));
+    b->sort_epoch = 1;
+  }
+  return b->sort_epoch;
+}
+
+struct SortScratch {
+  unsigned long long* hist;
+  unsigned int* tile_counter;
+  unsigned long long* status;
+  unsigned int* error_flags;
+};
+
+template <typename K, typename V, int kBlock, int kItems>
+void OnesweepPasses(BuildContext& ctx, K*& keys, V*& vals, K*& keys_alt, V*& vals_alt,
+                    std::uint64_t m, const RadixPasses& passes, const SortScratch& sc) {
   sfk_hip_builder* b = ctx.b;
-  constexpr int kItems = RadixItems<K, V>::kValue;
-  constexpr int kTile = RadixTile::kBlock * kItems;
-  const SortGeometry g = Geometry(m, kTile, b->radix_grid);
-  std::uint64_t* counts = b->radix_counts.As<std::uint64_t>();
-  std::uint64_t* digit_total = counts + static_cast<std::uint64_t>(RadixTile::kRadix) * RadixTile::kMaxGrid;
-  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) {
-    if (((varying >> shift) & (RadixTile::kRadix - 1)) == 0) continue;
-    RadixUpsweep<K, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
-        keys, m, g.chunk, shift, counts, g.grid);
-    RadixSpine<<<RadixTile::kRadix, 256, 0, b->stream>>>(counts, g.grid, digit_total);
+  constexpr std::uint64_t kTile = static_cast<std::uint64_t>(kBlock) * kItems;
+  static_assert(kTile >= kMinOnesweepTile, "status array sized for the smallest tile");
+  const std::uint64_t tiles = CeilDiv(m, kTile);
+  if ((tiles + 1) * RadixTile::kRadix * 8 > b->radix_status.Bytes()) {
+    Fail(SFK_HIP_BUILD_FAILURE, "radix status array smaller than the sort");
+  }
+  for (int p = 0; p < passes.count; ++p) {
+    const std::uint32_t epoch = NextEpoch(b);
     EventPair* ev = BeginProfiled(ctx, 2ull * m * (sizeof(K) + sizeof(V)));
-    RadixDownsweep<K, V, kItems><<<g.grid, RadixTile::kBlock, 0, b->stream>>>(
-        keys, vals, keys_alt, vals_alt, m, g.chunk, shift, counts, digit_total, g.grid);
+    RadixOnesweep<K, V, kBlock, kItems><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
+        keys, vals, keys_alt, vals_alt, m, passes.shift[p], sc.hist + p * RadixTile::kRadix, sc.status,
+        sc.tile_counter + p, epoch, sc.error_flags, kFlagLookback);
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
     std::swap(keys, keys_alt);
     std::swap(vals, vals_alt);
     ctx.stats.sort_passes += 1;
-    ctx.stats.algorithmic_bytes_sa += m * sizeof(K) + 2ull * m * (sizeof(K) + sizeof(V));
+    ctx.stats.algorithmic_bytes_sa += 2ull * m * (sizeof(K) + sizeof(V));
+  }
+}
+
+// Stable LSD sort of m (key, value) pairs over the digits in which `varying`
+// has a set bit.  On return keys/vals point at the sorted buffers.  One read
+// of the keys yields every pass's digit totals; each pass is then one
+// onesweep kernel (rank in LDS, look-back for the global offsets, scatter).
+template <typename K, typename V>
+void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
+               std::uint64_t m, std::uint64_t varying, int total_bits) {
+  sfk_hip_builder* b = ctx.b;
+  RadixPasses passes{};
+  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) {
+    if (((varying >> shift) & (RadixTile::kRadix - 1)) != 0) passes.shift[passes.count++] = shift;
+  }
+  if (passes.count == 0 || m == 0) return;
+  SortScratch sc;
+  sc.hist = b->radix_counts.As<unsigned long long>();
+  sc.tile_counter = reinterpret_cast<unsigned int*>(sc.hist + RadixPasses::kMax * RadixTile::kRadix);
+  sc.status = b->radix_status.As<unsigned long long>();
+  sc.error_flags = &b->counters.As<DeviceCounters>()->error_flags;
+  RadixSortInit<<<1, RadixTile::kRadix, 0, b->stream>>>(sc.hist, sc.tile_counter);
+  constexpr int kHistBlock = 512;
+  const std::uint64_t nvec = m / (16 / sizeof(K));
+  const unsigned hist_grid = static_cast<unsigned>(
+      std::max<std::uint64_t>(1, std::min<std::uint64_t>(512, nvec / (kHistBlock * 8))));
+  RadixHistogramAll<K, kHistBlock><<<hist_grid, kHistBlock, 0, b->stream>>>(keys, m, passes, sc.hist);
+  RadixBucketStarts<<<1, RadixTile::kRadix, 0, b->stream>>>(sc.hist, passes.count);
+  SFK_HIP_CHECK(hipGetLastError());
+  ctx.stats.algorithmic_bytes_sa += m * sizeof(K);
+  using Shape = OnesweepShape<K, V>;
+  if (m >= Shape::kBigFrom) {
+    OnesweepPasses<K, V, 1024, Shape::kBigItems>(ctx, keys, vals, keys_alt, vals_alt, m, passes, sc);
+  } else {
+    OnesweepPasses<K, V, 512, 8>(ctx, keys, vals, keys_alt, vals_alt, m, passes, sc);
   }
 }
 
@@ -324,6 +405,9 @@ class Doubling {
         b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next);
     SFK_HIP_CHECK(hipGetLastError());
     const DeviceCounters& after = ReadCounters(b_);
+    if (after.error_flags & kFlagLookback) {
+      Fail(SFK_HIP_BUILD_FAILURE, "radix look-back wait passed its bound");
+    }
     const std::uint64_t survivors = after.survivors;
     ctx_.stats.algorithmic_bytes_sa +=
         m * (2 * sizeof(K) + w) + (m - survivors) * 2 * w + survivors * 3 * w;
@@ -424,7 +508,8 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   const std::uint64_t seg_cap = n / 2 + 2;
   b->seg_delta0.Reserve(seg_cap * sizeof(Index));
   b->seg_delta1.Reserve(seg_cap * sizeof(Index));
-  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * (RadixTile::kMaxGrid + 1) * 8);
+  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * RadixPasses::kMax * 8 + 64);
+  ReserveOnesweep(b, n);
   b->partial.Reserve(1024 * sizeof(RefinePartial));
 
   Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
@@ -684,7 +769,8 @@ uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
   const uint64_t words = sfk::PackedWords(n);
   return words * 16 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
          (n / 2 + 2) * w * 2 /*segment deltas*/ + n * w * 2 /*sa, rank when not direct*/ +
-         sfk::RadixTile::kRadix * 1024 * 8 + 1024 * sizeof(sfk::RefinePartial);
+         sfk::RadixTile::kRadix * sfk::RadixPasses::kMax * 8 + 64 + 1024 * sizeof(sfk::RefinePartial) +
+         sfk::OnesweepStatusBytes(n);
 }
 
 int sfk_hip_build_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,

@@ -39,6 +39,7 @@ enum ErrorFlag : std::uint32_t {
   kFlagNotPermutation = 4u,
   kFlagNotInverse = 8u,
   kFlagRange = 16u,
+  kFlagLookback = 32u,  // a radix look-back wait passed its bound
 };
 
 // Device counters read back by the host after each step.
