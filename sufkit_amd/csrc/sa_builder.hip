@@ -114,8 +114,8 @@ struct sfk_hip_builder {
   int device = 0;
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
-      vals0, vals1, seg_delta0, seg_delta1, radix_counts, radix_status,
-      partial, counters, narrow, ref_offsets;
+      vals0, vals1, seg_delta0, seg_delta1, seg_start, radix_counts, radix_status,
+      partial, counters, narrow, ref_offsets, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
   std::uint32_t sort_epoch = 0;  // epoch of the last onesweep pass
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
@@ -141,8 +141,10 @@ struct BuildContext {
 
 void ResetCounters(sfk_hip_builder* b) {
   DeviceCounters init{};
-  init.key_or = 0;
-  init.key_and = ~0ull;
+  for (int i = 0; i < kKeySlots; ++i) {
+    init.key_or[i] = 0;
+    init.key_and[i] = ~0ull;
+  }
   *b->host_counters = init;
   SFK_HIP_CHECK(hipMemcpyAsync(b->counters.As<DeviceCounters>(), b->host_counters,
                                sizeof(DeviceCounters), hipMemcpyHostToDevice, b->stream));
@@ -150,8 +152,10 @@ void ResetCounters(sfk_hip_builder* b) {
 
 // Reset only the per-step key reduction fields, keeping error flags.
 __global__ void ResetKeyCountersKernel(DeviceCounters* c) {
-  c->key_or = 0;
-  c->key_and = ~0ull;
+  for (int i = 0; i < kKeySlots; ++i) {
+    c->key_or[i] = 0;
+    c->key_and[i] = ~0ull;
+  }
   c->special_active = 0;
 }
 
@@ -301,6 +305,13 @@ void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
   }
 }
 
+// Reset of the per-round list counters (small-list appends, large-list
+// allocation).
+__global__ void ResetRoundCountersKernel(DeviceCounters* c) {
+  c->q_next = 0;
+  c->l_alloc = 0;
+}
+
 template <typename Index>
 class Doubling {
  public:
@@ -308,59 +319,186 @@ class Doubling {
       : ctx_(ctx), b_(ctx.b), n_(ctx.n), sa_(sa), rank_(rank), text_(text) {}
 
   void Run() {
-    m_ = n_;
-    segments_ = ctx_.refs > 0 ? ctx_.refs : 1;
-    SFK_HIP_CHECK(hipMemsetAsync(b_->seg_delta0.As<Index>(), 0, segments_ * sizeof(Index), b_->stream));
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    if (n_ >= (1ull << kLargeAllocShift)) {
+      Fail(SFK_HIP_UNSUPPORTED_BACKEND, "texts of 2^38 symbols or more are not supported");
+    }
+    lm_ = n_;
+    l_segs_ = ctx_.refs > 0 ? ctx_.refs : 1;
+    l_idx_ = b_->act_idx.As<Index>();
+    l_seg_ = b_->act_seg.As<std::uint32_t>();
+    l_delta_ = b_->seg_delta0.As<Index>();
+    SFK_HIP_CHECK(hipMemsetAsync(l_delta_, 0, l_segs_ * sizeof(Index), b_->stream));
     if (ctx_.refs > 0) {
       // One initial segment per reference: its rows are its own positions.
       InitSegmentsKernel<<<static_cast<unsigned>(CeilDiv(n_, 256)), 256, 0, b_->stream>>>(
           ctx_.d_offsets, ctx_.refs, n_, b_->act_seg.As<std::uint32_t>());
       SFK_HIP_CHECK(hipGetLastError());
     }
-    Step(kSecFillHi, 32, 0, /*iota=*/true);
+    // First 16 symbols over every suffix (one segment per reference).
+    GlobalStep(kSecFillHi, 32, 0, /*iota=*/true, /*local_between=*/false);
     if (ctx_.refs > 0 && b_->host_counters->zero_count != ctx_.refs) {
       Fail(SFK_HIP_INVALID_INPUT, "every reference of a batch must end with its only 0 (sentinel)");
     }
-    if (m_ > 0) Step(kSecFillLo, 32, 0, false);
-    if (m_ > 0) {
-      ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(b_->counters.As<DeviceCounters>());
-      CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(m_, 256)), 256, 0, b_->stream>>>(
-          b_->act_idx.As<Index>(), m_, text_, b_->counters.As<DeviceCounters>());
-      const DeviceCounters& c = ReadCounters(b_);
-      if (c.special_active > 0) {
-        Step(kSecExact0, 32, 0, false);
-        if (m_ > 0) Step(kSecExact1, 32, 0, false);
-        if (m_ > 0) Step(kSecExact2, 32, 0, false);
+    if (lm_ == 0) return;
+    ReserveSmallLists(lm_);
+    ResetRoundCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+    Split();
+    EndRound(true);
+
+    // Exact 32-symbol groups: small segments in one local step on the exact
+    // 96-bit window key; large segments by the low 16 symbols, then (when a
+    // window holds a separator, N or the sentinel) by the exact key.
+    ResetRoundCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+    if (qm_ > 0) LocalRefine<true>();
+    bool split = false;
+    if (lm_ > 0) {
+      GlobalStep(kSecFillLo, 32, 0, false, false);
+      if (lm_ > 0) {
+        ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+        CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_, 256)), 256, 0, b_->stream>>>(
+            b_->act_idx.As<Index>(), lm_, text_, dc);
+        const DeviceCounters& c = ReadCounters(b_);
+        if (c.special_active > 0) {
+          GlobalStep(kSecExact0, 32, 0, false, false);
+          if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false);
+          if (lm_ > 0) GlobalStep(kSecExact2, 32, 0, false, false);
+        }
+      }
+      if (lm_ > 0) {
+        Split();
+        split = true;
       }
     }
+    EndRound(split);
+
     const int rank_bits = std::max(1, BitsFor(n_ - 1));
-    for (std::uint64_t h = 32; m_ > 0; h <<= 1) {
+    for (std::uint64_t h = 32; qm_ + lm_ > 0; h <<= 1) {
       if (h >= n_) Fail(SFK_HIP_BUILD_FAILURE, "prefix doubling did not converge");
-      Step(kSecRank, rank_bits, h, false);
+      ResetRoundCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
+      if (qm_ > 0) {
+        QKeygenKernel<Index><<<static_cast<unsigned>(CeilDiv(qm_, 256)), 256, 0, b_->stream>>>(
+            q_idx_cur_, qm_, rank_, n_, h, b_->q_key.As<Index>(), dc);
+        SFK_HIP_CHECK(hipGetLastError());
+        ctx_.stats.algorithmic_bytes_sa += qm_ * 3 * sizeof(Index);
+      }
+      split = false;
+      if (lm_ > 0) {
+        GlobalStep(kSecRank, rank_bits, h, false, /*local_between=*/qm_ > 0);
+        if (lm_ > 0) {
+          Split();
+          split = true;
+        }
+      } else {
+        LocalRefine<false>();
+      }
+      EndRound(split);
       ctx_.stats.doubling_rounds += 1;
     }
   }
 
  private:
-  void Step(int sec_kind, int sec_bits, std::uint64_t h, bool iota) {
-    const int seg_bits = segments_ > 1 ? BitsFor(segments_ - 1) : 0;
+  // Small lists hold at most the survivors of the first step.
+  void ReserveSmallLists(std::uint64_t cap) {
+    const std::uint64_t bytes = std::max<std::uint64_t>(cap, 1) * sizeof(Index);
+    b_->q_idx0.Reserve(bytes);
+    b_->q_idx1.Reserve(bytes);
+    b_->q_row0_0.Reserve(bytes);
+    b_->q_row0_1.Reserve(bytes);
+    b_->q_key.Reserve(bytes);
+    q_idx_cur_ = b_->q_idx0.As<Index>();
+    q_row0_cur_ = b_->q_row0_0.As<Index>();
+    q_idx_next_ = b_->q_idx1.As<Index>();
+    q_row0_next_ = b_->q_row0_1.As<Index>();
+    q_cap_ = cap;
+  }
+
+  Index* OtherDelta(Index* d) const {
+    return d == b_->seg_delta0.As<Index>() ? b_->seg_delta1.As<Index>() : b_->seg_delta0.As<Index>();
+  }
+
+  // Reads the round's list sizes back, checks the error flags and makes the
+  // next small list current.
+  void EndRound(bool split) {
+    const DeviceCounters& c = ReadCounters(b_);
+    if (c.error_flags & (kFlagLocalSegment | kFlagRange)) {
+      Fail(SFK_HIP_BUILD_FAILURE, c.error_flags & kFlagLocalSegment
+                                      ? "small-list segment larger than its bound"
+                                      : "rank gather past the end of the text");
+    }
+    qm_ = c.q_next;
+    if (split) {
+      lm_ = c.l_alloc & ((1ull << kLargeAllocShift) - 1);
+      l_segs_ = c.l_alloc >> kLargeAllocShift;
+    } else {
+      lm_ = 0;
+      l_segs_ = 0;
+    }
+    if (qm_ > q_cap_) Fail(SFK_HIP_BUILD_FAILURE, "small list overflow");
+    std::swap(q_idx_cur_, q_idx_next_);
+    std::swap(q_row0_cur_, q_row0_next_);
+  }
+
+  // Routes the segments of the list RefineApply just wrote: small ones to
+  // the next small list, large ones (renumbered) to the next large list in
+  // vals0 (positions) and vals1 (segment ids).
+  void Split() {
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    Index* dest = b_->keys0.As<Index>();
+    std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
+    Index* large_delta = OtherDelta(l_delta_);
+    SplitSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(l_segs_, 256 * kSplitItems)), 256, 0, b_->stream>>>(
+        b_->seg_start.As<Index>(), l_delta_, l_segs_, lm_, kLocalMaxSegment, dest, new_id, large_delta, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    Index* l_idx_next = b_->vals0.As<Index>();
+    std::uint32_t* l_seg_next = b_->vals1.As<std::uint32_t>();
+    SplitMoveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_, 256)), 256, 0, b_->stream>>>(
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), lm_, b_->seg_start.As<Index>(), l_delta_,
+        dest, new_id, q_idx_next_, q_row0_next_, l_idx_next, l_seg_next);
+    SFK_HIP_CHECK(hipGetLastError());
+    constexpr std::uint64_t w = sizeof(Index);
+    ctx_.stats.algorithmic_bytes_sa += l_segs_ * (4 * w + 4) + lm_ * (2 * w + 8 + 2 * w);
+    l_idx_ = l_idx_next;
+    l_seg_ = l_seg_next;
+    l_delta_ = large_delta;
+  }
+
+  template <bool kExact>
+  void LocalRefine() {
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    const unsigned grid = static_cast<unsigned>(CeilDiv(qm_, kLocalOwn));
+    LocalRefineKernel<Index, kExact><<<grid, kLocalThreads, 0, b_->stream>>>(
+        text_, q_idx_cur_, q_row0_cur_, b_->q_key.As<Index>(), qm_, n_, sa_, rank_, q_idx_next_,
+        q_row0_next_, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    constexpr std::uint64_t w = sizeof(Index);
+    ctx_.stats.refine_steps += 1;
+    ctx_.stats.active_total += qm_;
+    ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * w + (kExact ? 32 : w) + 2 * w + 2 * w);
+  }
+
+  void GlobalStep(int sec_kind, int sec_bits, std::uint64_t h, bool iota, bool local_between) {
+    const int seg_bits = l_segs_ > 1 ? BitsFor(l_segs_ - 1) : 0;
     const int total_bits = seg_bits + sec_bits;
     if (total_bits > 64) {
       Fail(SFK_HIP_UNSUPPORTED_BACKEND,
            "segment and rank bits exceed one 64-bit sort key");
     }
     ctx_.stats.refine_steps += 1;
-    ctx_.stats.active_total += m_;
+    ctx_.stats.active_total += lm_;
     if (total_bits <= 32) {
-      StepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota);
+      GlobalStepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between);
     } else {
-      StepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota);
+      GlobalStepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between);
     }
   }
 
+  // One refine step of the large list: keygen, (the small list's step, whose
+  // rank gathers are done), radix sort, boundaries/singletons/compaction.
   template <typename K>
-  void StepTyped(int sec_kind, int sec_bits, int seg_bits, std::uint64_t h, bool iota) {
-    const std::uint64_t m = m_;
+  void GlobalStepTyped(int sec_kind, int sec_bits, int seg_bits, std::uint64_t h, bool iota,
+                       bool local_between) {
+    const std::uint64_t m = lm_;
     const int total_bits = seg_bits + sec_bits;
     K* keys = b_->keys0.As<K>();
     K* keys_alt = b_->keys1.As<K>();
@@ -374,9 +512,10 @@ class Doubling {
     const bool rank_step = sec_kind == kSecRank;
     if (!rank_step) ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
     const unsigned keygen_grid = static_cast<unsigned>(std::min<std::uint64_t>(CeilDiv(m, 256), 2048));
+    // The large list may sit in vals0/vals1 (after a split): Keygen reads
+    // slot k of it before writing slot k of vals0, in the same thread.
     KeygenKernel<Index, K><<<keygen_grid, 256, 0, b_->stream>>>(
-        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), m, sec_kind, sec_bits, seg_bits,
-        text_, rank_, h, iota, keys, vals, dc);
+        l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, !rank_step, keys, vals, dc);
     SFK_HIP_CHECK(hipGetLastError());
     ctx_.stats.algorithmic_bytes_sa +=
         m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
@@ -384,25 +523,34 @@ class Doubling {
       // Text-derived keys: skip digits that are constant over the active set
       // (and surface the packing errors before any rank gather).
       const DeviceCounters& c = ReadCounters(b_);
-      if (c.error_flags != 0) {
-        Fail(SFK_HIP_INVALID_INPUT, c.error_flags & kFlagBadSymbol
-                                        ? "text holds a symbol outside 0..6"
-                                        : "text must end with its only 0 (sentinel)");
+      if (c.error_flags & kFlagBadSymbol) Fail(SFK_HIP_INVALID_INPUT, "text holds a symbol outside 0..6");
+      if (c.error_flags & kFlagBadSentinel) {
+        Fail(SFK_HIP_INVALID_INPUT, "text must end with its only 0 (sentinel)");
       }
-      varying = (c.key_or ^ c.key_and) & all_bits;
+      if (c.error_flags != 0) {
+        Fail(SFK_HIP_BUILD_FAILURE, "device consistency check failed (flags " +
+                                        std::to_string(c.error_flags) + ")");
+      }
+      std::uint64_t key_or = 0, key_and = ~0ull;
+      for (int i = 0; i < kKeySlots; ++i) {
+        key_or |= c.key_or[i];
+        key_and &= c.key_and[i];
+      }
+      varying = (key_or ^ key_and) & all_bits;
     }
+    if (local_between) LocalRefine<false>();
     SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, varying, total_bits);
 
     // Boundaries, singletons, compaction.
     const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
     RefinePartial* partial = b_->partial.As<RefinePartial>();
-    Index* delta_cur = (flip_ ? b_->seg_delta1 : b_->seg_delta0).template As<Index>();
-    Index* delta_next = (flip_ ? b_->seg_delta0 : b_->seg_delta1).template As<Index>();
+    Index* delta_next = OtherDelta(l_delta_);
     RefineReduceKernel<K><<<g.grid, 256, 0, b_->stream>>>(keys, m, g.chunk, partial);
     RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
     RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
-        keys, vals, m, g.chunk, sec_bits, delta_cur, partial, n_, sa_, rank_,
-        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next);
+        keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_,
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next,
+        b_->seg_start.As<Index>());
     SFK_HIP_CHECK(hipGetLastError());
     const DeviceCounters& after = ReadCounters(b_);
     if (after.error_flags & kFlagLookback) {
@@ -410,10 +558,12 @@ class Doubling {
     }
     const std::uint64_t survivors = after.survivors;
     ctx_.stats.algorithmic_bytes_sa +=
-        m * (2 * sizeof(K) + w) + (m - survivors) * 2 * w + survivors * 3 * w;
-    m_ = survivors;
-    segments_ = after.segments;
-    flip_ = !flip_;
+        m * (2 * sizeof(K) + w) + (m - survivors) * 2 * w + survivors * (3 * w + 4);
+    lm_ = survivors;
+    l_segs_ = after.segments;
+    l_idx_ = b_->act_idx.As<Index>();
+    l_seg_ = b_->act_seg.As<std::uint32_t>();
+    l_delta_ = delta_next;
   }
 
   BuildContext& ctx_;
@@ -422,9 +572,19 @@ class Doubling {
   Index* sa_;
   Index* rank_;
   PackedText text_;
-  std::uint64_t m_ = 0;
-  std::uint64_t segments_ = 1;
-  bool flip_ = false;
+  // Large list: suffixes, segment ids, per-segment deltas.
+  std::uint64_t lm_ = 0;
+  std::uint64_t l_segs_ = 0;
+  const Index* l_idx_ = nullptr;
+  const std::uint32_t* l_seg_ = nullptr;
+  Index* l_delta_ = nullptr;
+  // Small lists (current and next): suffixes and their segments' first rows.
+  std::uint64_t qm_ = 0;
+  std::uint64_t q_cap_ = 0;
+  Index* q_idx_cur_ = nullptr;
+  Index* q_row0_cur_ = nullptr;
+  Index* q_idx_next_ = nullptr;
+  Index* q_row0_next_ = nullptr;
 };
 
 template <typename Index, typename Out>
@@ -508,6 +668,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   const std::uint64_t seg_cap = n / 2 + 2;
   b->seg_delta0.Reserve(seg_cap * sizeof(Index));
   b->seg_delta1.Reserve(seg_cap * sizeof(Index));
+  b->seg_start.Reserve(seg_cap * sizeof(Index));
   b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * RadixPasses::kMax * 8 + 64);
   ReserveOnesweep(b, n);
   b->partial.Reserve(1024 * sizeof(RefinePartial));
@@ -768,7 +929,8 @@ uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
   const uint64_t w = construction_bits == 64 ? 8 : 4;
   const uint64_t words = sfk::PackedWords(n);
   return words * 16 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
-         (n / 2 + 2) * w * 2 /*segment deltas*/ + n * w * 2 /*sa, rank when not direct*/ +
+         (n / 2 + 2) * w * 3 /*segment deltas, starts*/ + n * w * 2 /*sa, rank when not direct*/ +
+         n * w * 5 /*small lists (at most the first step's survivors)*/ +
          sfk::RadixTile::kRadix * sfk::RadixPasses::kMax * 8 + 64 + 1024 * sizeof(sfk::RefinePartial) +
          sfk::OnesweepStatusBytes(n);
 }

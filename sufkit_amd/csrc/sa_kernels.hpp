@@ -2,22 +2,37 @@
 //
 // Prefix doubling over "segments": a segment is a run of SA rows whose
 // suffixes share their first h symbols (a group) and still hold more than
-// one suffix.  The active list keeps, per still-unsorted suffix, its text
-// position (act_idx) and its dense 32-bit segment id (act_seg); the segment
-// table keeps one value per segment, seg_delta = (first SA row of the
-// segment) - (its first active slot), so the SA row of active slot k is
-// seg_delta[seg] + k (slots only ever move left when survivors are
-// compacted, rows never do, so the delta is never negative).  One step:
-//   Keygen        key = (segment id << sec_bits) | secondary(suffix)
-//   radix sort    stable LSD over the varying digits of the key
-//   RefineReduce/RefineSpine/RefineApply
-//                 group boundaries (key change), singletons written to SA and
-//                 to rank (= their final ISA entry), survivors compacted into
-//                 a new active list with renumbered segments, ranks of
-//                 survivors set to the SA row where their group starts.
+// one suffix.  Active suffixes live in one of two lists:
+//
+//   large list  segments of more than kLocalMaxSegment suffixes.  Per suffix
+//               its text position (act_idx) and dense 32-bit segment id
+//               (act_seg); per segment seg_delta = (first SA row) - (first
+//               slot), so the row of slot k is seg_delta[seg] + k, computed
+//               modulo the index width (a segment placed after segments with
+//               larger rows has a "negative" delta).  One step:
+//     Keygen        key = (segment id << sec_bits) | secondary(suffix)
+//     radix sort    stable LSD over the varying digits of the key
+//     RefineReduce/RefineSpine/RefineApply
+//                   group boundaries (key change), singletons written to SA
+//                   and rank (= their final ISA entry), survivors compacted
+//                   into a new list with renumbered segments, ranks of
+//                   survivors set to the SA row where their group starts
+//     SplitSegments/SplitMove
+//                   segments of at most kLocalMaxSegment suffixes move to the
+//                   small list, the others are renumbered in slot order
+//   small list  segments of at most kLocalMaxSegment suffixes; per suffix its
+//               text position and the first SA row of its segment.  One
+//               LocalRefine kernel ranks every suffix inside its segment in
+//               LDS (count of smaller and equal keys), writes singletons,
+//               and appends surviving groups to the next small list.
+//
+// Within a doubling round every rank[suffix + h] is gathered (Keygen,
+// QKeygen) before any rank is rewritten (LocalRefine, RefineApply).
 // The rank array is indexed by text position; when no suffix is left active
 // it equals the ISA.
 #pragma once
+
+#include <type_traits>
 
 #include "block_scan.hpp"
 #include "device_common.hpp"
@@ -40,21 +55,41 @@ enum ErrorFlag : std::uint32_t {
   kFlagNotInverse = 8u,
   kFlagRange = 16u,
   kFlagLookback = 32u,  // a radix look-back wait passed its bound
+  kFlagLocalSegment = 64u,  // a small-list segment exceeded kLocalMaxSegment
 };
 
 // Device counters read back by the host after each step.
+// Key OR/AND reductions are spread over kKeySlots slots (block b uses slot
+// b % kKeySlots) so that blocks do not queue on one address; the host folds
+// the slots.
+constexpr int kKeySlots = 32;
+
 struct DeviceCounters {
-  unsigned long long key_or;
-  unsigned long long key_and;
+  unsigned long long key_or[kKeySlots];
+  unsigned long long key_and[kKeySlots];
   unsigned long long survivors;
   unsigned long long segments;
   unsigned long long special_active;
   unsigned long long long_items;
   unsigned long long long_words;
   unsigned long long zero_count;  // sentinels seen by the batch packer
+  unsigned long long q_next;      // suffixes appended to the next small list
+  unsigned long long l_alloc;     // next large list: (segments << 38) | suffixes
   unsigned int error_flags;
   unsigned int pad;
 };
+
+// Largest segment the small list holds, and the span/ownership geometry of
+// one LocalRefine block: a block reads kLocalSpan consecutive slots and owns
+// the segments that start in its first kLocalOwn slots (which therefore end
+// inside the span).
+constexpr int kLocalThreads = 1024;
+constexpr int kLocalItems = 2;
+constexpr int kLocalSpan = kLocalThreads * kLocalItems;
+constexpr std::uint32_t kLocalMaxSegment = 256;
+constexpr int kLocalOwn = kLocalSpan - static_cast<int>(kLocalMaxSegment);
+constexpr int kLargeAllocShift = 38;  // l_alloc packing: suffixes < 2^38
+constexpr std::uint32_t kSmallSegment = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------ K0 pack -----
 
@@ -174,12 +209,14 @@ __global__ __launch_bounds__(256) void LocalizeKernel(
 
 // ------------------------------------------------------------ keys --------
 
+// act_idx may be the same buffer as vals: slot k is read before slot k is
+// written, by the same thread (hence no __restrict__ on either).
 template <typename Index, typename K>
 __global__ __launch_bounds__(256) void KeygenKernel(
-    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg,
+    const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
-    const Index* __restrict__ rank, std::uint64_t h, bool iota,
-    K* __restrict__ keys, Index* __restrict__ vals, DeviceCounters* counters) {
+    const Index* __restrict__ rank, std::uint64_t h, bool iota, bool reduce,
+    K* __restrict__ keys, Index* vals, DeviceCounters* counters) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t s_or[kBlock / kWave];
   __shared__ std::uint64_t s_and[kBlock / kWave];
@@ -218,6 +255,7 @@ __global__ __launch_bounds__(256) void KeygenKernel(
     kor |= key;
     kand &= key;
   }
+  if (!reduce) return;  // rank steps sort every digit: no OR/AND needed
   // One atomic pair per block: wave butterfly, then LDS across waves.
   for (int d = 32; d >= 1; d >>= 1) {
     kor |= WaveShfl(kor, static_cast<int>(LaneId() ^ d));
@@ -235,8 +273,8 @@ __global__ __launch_bounds__(256) void KeygenKernel(
       o |= s_or[w];
       a &= s_and[w];
     }
-    atomicOr(&counters->key_or, static_cast<unsigned long long>(o));
-    atomicAnd(&counters->key_and, static_cast<unsigned long long>(a));
+    atomicOr(&counters->key_or[blockIdx.x % kKeySlots], static_cast<unsigned long long>(o));
+    atomicAnd(&counters->key_and[blockIdx.x % kKeySlots], static_cast<unsigned long long>(a));
   }
 }
 
@@ -333,7 +371,8 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
     std::uint64_t chunk, int sec_bits, const Index* __restrict__ seg_delta,
     const RefinePartial* __restrict__ partial,
     std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
-    std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta) {
+    std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
+    Index* __restrict__ out_start) {
   constexpr int kBlock = 256;
   constexpr int kTile = kBlock * kItems;
   __shared__ std::uint64_t scratch[kBlock / kWave];
@@ -370,20 +409,21 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
       const K key = keys[k];
       const Index v = vals[k];
       const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
-      const std::uint64_t delta = static_cast<std::uint64_t>(seg_delta[seg]);
-      const std::uint64_t row = delta + k;
+      const Index delta = seg_delta[seg];
+      const Index row = static_cast<Index>(delta + static_cast<Index>(k));
       if (starts[i]) cur_last = k + 1;
-      if (row >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
+      if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
       if (single[i]) {
         sa[row] = v;
-        rank[v] = static_cast<Index>(row);
+        rank[v] = row;
       } else {
         // The group start lies in the same segment: every segment start is a
         // key change, hence a group start.
-        const std::uint64_t head_row = delta + (cur_last - 1);
-        rank[v] = static_cast<Index>(head_row);
+        const Index head_row = static_cast<Index>(delta + static_cast<Index>(cur_last - 1));
+        rank[v] = head_row;
         if (starts[i]) {
-          out_delta[segc] = static_cast<Index>(row - pos);
+          out_delta[segc] = static_cast<Index>(row - static_cast<Index>(pos));
+          out_start[segc] = static_cast<Index>(pos);
           ++segc;
         }
         out_idx[pos] = v;
@@ -394,6 +434,301 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
     run_surv += tot_surv;
     run_segs += tot_segs;
     if (tot_last > run_last) run_last = tot_last;
+  }
+}
+
+// ------------------------------------------------------------ split -----
+
+// kSplitItems consecutive segments per thread of the list RefineApply just
+// wrote (segs segments, m suffixes, first slots in seg_start).  A segment of
+// at most small_max suffixes reserves its slots in the next small list (one
+// atomic per block); a larger one takes its new id and first slot in the
+// next large list from one 64-bit atomic, so ids grow with slots, and gets
+// its new delta.
+constexpr int kSplitItems = 8;
+
+template <typename Index>
+__global__ __launch_bounds__(256) void SplitSegmentsKernel(
+    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta, std::uint64_t segs,
+    std::uint64_t m, std::uint32_t small_max, Index* __restrict__ dest,
+    std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, DeviceCounters* counters) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  __shared__ std::uint64_t s_base;
+  const std::uint64_t first = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) * kSplitItems;
+  std::uint64_t small_total = 0;
+#pragma unroll
+  for (int i = 0; i < kSplitItems; ++i) {
+    const std::uint64_t s = first + i;
+    if (s < segs) {
+      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
+      const std::uint64_t end = s + 1 < segs ? static_cast<std::uint64_t>(seg_start[s + 1]) : m;
+      if (end - start <= small_max) small_total += end - start;
+    }
+  }
+  std::uint64_t total;
+  std::uint64_t off = BlockExclusiveSum<kBlock>(small_total, scratch, total);
+  if (threadIdx.x == 0) {
+    s_base = total ? static_cast<std::uint64_t>(atomicAdd(&counters->q_next, static_cast<unsigned long long>(total))) : 0ull;
+  }
+  __syncthreads();
+  off += s_base;
+#pragma unroll
+  for (int i = 0; i < kSplitItems; ++i) {
+    const std::uint64_t s = first + i;
+    if (s >= segs) break;
+    const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
+    const std::uint64_t end = s + 1 < segs ? static_cast<std::uint64_t>(seg_start[s + 1]) : m;
+    const std::uint64_t size = end - start;
+    if (size <= small_max) {
+      dest[s] = static_cast<Index>(off);
+      new_id[s] = kSmallSegment;
+      off += size;
+    } else {
+      const Index row0 = static_cast<Index>(seg_delta[s] + static_cast<Index>(start));
+      const unsigned long long got = atomicAdd(
+          &counters->l_alloc, (1ull << kLargeAllocShift) | static_cast<unsigned long long>(size));
+      const std::uint64_t id = got >> kLargeAllocShift;
+      const std::uint64_t pos = got & ((1ull << kLargeAllocShift) - 1);
+      dest[s] = static_cast<Index>(pos);
+      new_id[s] = static_cast<std::uint32_t>(id);
+      large_delta[id] = static_cast<Index>(row0 - static_cast<Index>(pos));
+    }
+  }
+}
+
+// One thread per suffix of that list: copy it to its segment's destination.
+template <typename Index>
+__global__ __launch_bounds__(256) void SplitMoveKernel(
+    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg, std::uint64_t m,
+    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
+    const Index* __restrict__ dest, const std::uint32_t* __restrict__ new_id,
+    Index* __restrict__ q_idx, Index* __restrict__ q_row0, Index* __restrict__ l_idx,
+    std::uint32_t* __restrict__ l_seg) {
+  const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const std::uint32_t s = act_seg[p];
+  const Index start = seg_start[s];
+  const std::uint64_t at = static_cast<std::uint64_t>(dest[s]) + (p - static_cast<std::uint64_t>(start));
+  const Index v = act_idx[p];
+  const std::uint32_t id = new_id[s];
+  if (id == kSmallSegment) {
+    q_idx[at] = v;
+    q_row0[at] = static_cast<Index>(seg_delta[s] + start);
+  } else {
+    l_idx[at] = v;
+    l_seg[at] = id;
+  }
+}
+
+// ------------------------------------------------------------ small list --
+
+// rank[suffix + h] of every small-list suffix, gathered before any rank of
+// the round is rewritten.
+template <typename Index>
+__global__ __launch_bounds__(256) void QKeygenKernel(
+    const Index* __restrict__ q_idx, std::uint64_t qm, const Index* __restrict__ rank,
+    std::uint64_t n, std::uint64_t h, Index* __restrict__ q_key, DeviceCounters* counters) {
+  const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= qm) return;
+  const std::uint64_t at = static_cast<std::uint64_t>(q_idx[p]) + h;
+  Index key = 0;
+  if (at < n) {
+    key = rank[at];
+  } else {
+    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
+  }
+  q_key[p] = key;
+}
+
+struct ExactKey {
+  std::uint32_t w0, w1, w2;
+};
+
+__device__ inline bool KeyLess(const ExactKey& a, const ExactKey& b) {
+  if (a.w0 != b.w0) return a.w0 < b.w0;
+  if (a.w1 != b.w1) return a.w1 < b.w1;
+  return a.w2 < b.w2;
+}
+__device__ inline bool KeyEqual(const ExactKey& a, const ExactKey& b) {
+  return a.w0 == b.w0 && a.w1 == b.w1 && a.w2 == b.w2;
+}
+template <typename T>
+__device__ inline bool KeyLess(T a, T b) {
+  return a < b;
+}
+template <typename T>
+__device__ inline bool KeyEqual(T a, T b) {
+  return a == b;
+}
+
+// All three words of the exact 96-bit key of a 32-symbol window.
+__device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
+  std::uint64_t hi = 0;
+  std::uint32_t lo = 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const std::uint32_t c = static_cast<std::uint32_t>(codes >> (62 - 2 * j)) & 3u;
+    const bool special = (spec >> (31 - j)) & 1u;
+    const std::uint32_t v = special ? (c == 3u ? 6u : c) : c + 2u;
+    hi = (hi << 3) | (lo >> 29);
+    lo = (lo << 3) | v;
+  }
+  return ExactKey{static_cast<std::uint32_t>(hi >> 32), static_cast<std::uint32_t>(hi), lo};
+}
+
+// One refine step of the small list.  kExact: the key is the exact 96-bit
+// key of the suffix's 32-symbol window (the small list's first step after the
+// 16-symbol split); otherwise it is q_key (rank[suffix + h]).  A block reads
+// kLocalSpan slots, kLocalItems consecutive slots per thread, and owns the
+// segments that start in its first kLocalOwn slots.  Segment starts come
+// from a block max-scan of "row0 changes here" flags, sizes from LDS
+// counters.  An owned suffix counts, inside its segment, the keys below its
+// own (its group's first row is row0 + that count) and the keys equal to it
+// (its group's size).  The first suffix of each surviving group reserves the
+// group's slots in the next small list (one atomic per block).
+template <typename Index, bool kExact>
+__global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
+    PackedText t, const Index* __restrict__ q_idx, const Index* __restrict__ q_row0,
+    const Index* __restrict__ q_key, std::uint64_t qm, std::uint64_t n,
+    Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
+    Index* __restrict__ out_row0, DeviceCounters* counters) {
+  using KeyT = typename std::conditional<kExact, ExactKey, Index>::type;
+  constexpr int kItems = kLocalItems;
+  __shared__ Index s_row0[kLocalSpan + 1];  // s_row0[k + 1] = slot k, s_row0[0] = slot -1
+  __shared__ KeyT s_key[kLocalSpan];
+  __shared__ std::uint32_t s_count[kLocalSpan];
+  __shared__ std::uint32_t s_off[kLocalSpan];
+  __shared__ std::uint64_t scratch[kLocalThreads / kWave];
+  __shared__ std::uint32_t scratch32[kLocalThreads / kWave];
+  __shared__ std::uint64_t s_base;
+  const int tid = threadIdx.x;
+  const std::uint64_t s0 = static_cast<std::uint64_t>(blockIdx.x) * kLocalOwn;
+  const int valid_len =
+      static_cast<int>(qm - s0 < static_cast<std::uint64_t>(kLocalSpan) ? qm - s0 : kLocalSpan);
+  Index v[kItems], r0[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = tid * kItems + i;
+    v[i] = 0;
+    r0[i] = 0;
+    KeyT key{};
+    if (k < valid_len) {
+      v[i] = q_idx[s0 + k];
+      r0[i] = q_row0[s0 + k];
+      if constexpr (kExact) {
+        std::uint64_t c;
+        std::uint32_t sp;
+        LoadWindow(t, static_cast<std::uint64_t>(v[i]), c, sp);
+        key = ExactKeyOf(c, sp);
+      } else {
+        key = q_key[s0 + k];
+      }
+    }
+    s_row0[k + 1] = r0[i];
+    s_key[k] = key;
+    s_count[k] = 0;
+  }
+  if (tid == 0) s_row0[0] = s0 > 0 ? q_row0[s0 - 1] : Index(0);
+  __syncthreads();
+
+  // st + 1 of every slot (0: its segment starts before the span).
+  std::uint64_t mark[kItems];
+  std::uint64_t run = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    // Branch-free on purpose (both LDS reads are in bounds): hipcc for
+    // gfx950 dropped the update of `run` when this was written with
+    // short-circuit && / || around the LDS reads.
+    const int k = tid * kItems + i;
+    const Index here = s_row0[k + 1];
+    const Index before = s_row0[k];
+    const bool first_slot = (k == 0) & (s0 == 0);
+    const bool starts = (k < valid_len) & (first_slot | (here != before));
+    run = starts ? static_cast<std::uint64_t>(k) + 1 : run;
+    mark[i] = run;
+  }
+  // Largest mark of the earlier threads (their last slot's mark).
+  std::uint64_t all_max;
+  const std::uint64_t prev = BlockExclusiveMax<kLocalThreads>(run, scratch, all_max);
+  int st[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = tid * kItems + i;
+    st[i] = -1;
+    const std::uint64_t mk = mark[i] > prev ? mark[i] : prev;
+    if (k < valid_len && mk > 0) {
+      st[i] = static_cast<int>(mk) - 1;
+      atomicAdd(&s_count[st[i]], 1u);
+    }
+  }
+  __syncthreads();
+
+  bool survivor[kItems];
+  std::uint32_t equal[kItems], tie[kItems];
+  int first_equal[kItems];
+  Index head[kItems];
+  std::uint32_t leader_total = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = tid * kItems + i;
+    survivor[i] = false;
+    equal[i] = 0;
+    tie[i] = 0;
+    first_equal[i] = k;
+    head[i] = r0[i];
+    if (st[i] < 0 || st[i] >= kLocalOwn) continue;  // not owned by this block
+    const std::uint32_t size = s_count[st[i]];
+    if (size > kLocalMaxSegment) {
+      atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagLocalSegment));
+      continue;
+    }
+    const int en = st[i] + static_cast<int>(size);
+    const KeyT key = s_key[k];
+    std::uint32_t less = 0;
+    for (int j = st[i]; j < en; ++j) {
+      const KeyT kj = s_key[j];
+      if (KeyLess(kj, key)) {
+        ++less;
+      } else if (KeyEqual(kj, key)) {
+        if (equal[i] == 0) first_equal[i] = j;
+        ++equal[i];
+        tie[i] += j < k;
+      }
+    }
+    head[i] = static_cast<Index>(r0[i] + static_cast<Index>(less));
+    if (static_cast<std::uint64_t>(v[i]) >= n || static_cast<std::uint64_t>(head[i]) >= n) {
+      atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
+    } else if (equal[i] == 1) {
+      sa[head[i]] = v[i];
+      rank[v[i]] = head[i];
+    } else {
+      rank[v[i]] = head[i];
+      survivor[i] = true;
+      if (tie[i] == 0) leader_total += equal[i];
+    }
+  }
+  std::uint32_t total;
+  std::uint32_t off = BlockExclusiveSum<kLocalThreads>(leader_total, scratch32, total);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = tid * kItems + i;
+    if (survivor[i] && tie[i] == 0) {
+      s_off[k] = off;
+      off += equal[i];
+    }
+  }
+  if (tid == 0) {
+    s_base = total ? static_cast<std::uint64_t>(atomicAdd(&counters->q_next, static_cast<unsigned long long>(total))) : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if (survivor[i]) {
+      const std::uint64_t at = s_base + s_off[first_equal[i]] + tie[i];
+      out_idx[at] = v[i];
+      out_row0[at] = head[i];
+    }
   }
 }
 
