@@ -141,23 +141,13 @@ struct BuildContext {
 
 void ResetCounters(sfk_hip_builder* b) {
   DeviceCounters init{};
-  for (int i = 0; i < kKeySlots; ++i) {
-    init.key_or[i] = 0;
-    init.key_and[i] = ~0ull;
-  }
   *b->host_counters = init;
   SFK_HIP_CHECK(hipMemcpyAsync(b->counters.As<DeviceCounters>(), b->host_counters,
                                sizeof(DeviceCounters), hipMemcpyHostToDevice, b->stream));
 }
 
-// Reset only the per-step key reduction fields, keeping error flags.
-__global__ void ResetKeyCountersKernel(DeviceCounters* c) {
-  for (int i = 0; i < kKeySlots; ++i) {
-    c->key_or[i] = 0;
-    c->key_and[i] = ~0ull;
-  }
-  c->special_active = 0;
-}
+// Reset of the special-window count (error flags are kept).
+__global__ void ResetSpecialCountKernel(DeviceCounters* c) { c->special_active = 0; }
 
 const DeviceCounters& ReadCounters(sfk_hip_builder* b) {
   SFK_HIP_CHECK(hipMemcpyAsync(b->host_counters, b->counters.As<DeviceCounters>(),
@@ -336,7 +326,7 @@ class Doubling {
       SFK_HIP_CHECK(hipGetLastError());
     }
     // First 16 symbols over every suffix (one segment per reference).
-    GlobalStep(kSecFillHi, 32, 0, /*iota=*/true, /*local_between=*/false);
+    GlobalStep(kSecFillHi, 32, 0, /*iota=*/true, /*local_between=*/false, /*readback=*/true);
     if (ctx_.refs > 0 && b_->host_counters->zero_count != ctx_.refs) {
       Fail(SFK_HIP_INVALID_INPUT, "every reference of a batch must end with its only 0 (sentinel)");
     }
@@ -353,17 +343,16 @@ class Doubling {
     if (qm_ > 0) LocalRefine<true>();
     bool split = false;
     if (lm_ > 0) {
-      GlobalStep(kSecFillLo, 32, 0, false, false);
-      if (lm_ > 0) {
-        ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
-        CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_, 256)), 256, 0, b_->stream>>>(
-            b_->act_idx.As<Index>(), lm_, text_, dc);
-        const DeviceCounters& c = ReadCounters(b_);
-        if (c.special_active > 0) {
-          GlobalStep(kSecExact0, 32, 0, false, false);
-          if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false);
-          if (lm_ > 0) GlobalStep(kSecExact2, 32, 0, false, false);
-        }
+      GlobalStep(kSecFillLo, 32, 0, false, false, /*readback=*/false);
+      ResetSpecialCountKernel<<<1, 1, 0, b_->stream>>>(dc);
+      CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256)), 256, 0, b_->stream>>>(
+          b_->act_idx.As<Index>(), text_, dc);
+      SFK_HIP_CHECK(hipGetLastError());
+      const bool special = ApplySizes(ReadCounters(b_)).special_active > 0;
+      if (special && lm_ > 0) {
+        GlobalStep(kSecExact0, 32, 0, false, false, true);
+        if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false, true);
+        if (lm_ > 0) GlobalStep(kSecExact2, 32, 0, false, false, true);
       }
       if (lm_ > 0) {
         Split();
@@ -384,11 +373,9 @@ class Doubling {
       }
       split = false;
       if (lm_ > 0) {
-        GlobalStep(kSecRank, rank_bits, h, false, /*local_between=*/qm_ > 0);
-        if (lm_ > 0) {
-          Split();
-          split = true;
-        }
+        GlobalStep(kSecRank, rank_bits, h, false, /*local_between=*/qm_ > 0, /*readback=*/false);
+        Split();
+        split = true;
       } else {
         LocalRefine<false>();
       }
@@ -417,15 +404,42 @@ class Doubling {
     return d == b_->seg_delta0.As<Index>() ? b_->seg_delta1.As<Index>() : b_->seg_delta0.As<Index>();
   }
 
+  // Fails on any error flag raised so far.
+  void CheckFlags(const DeviceCounters& c) {
+    const unsigned f = c.error_flags;
+    if (f & kFlagBadSymbol) Fail(SFK_HIP_INVALID_INPUT, "text holds a symbol outside 0..6");
+    if (f & kFlagBadSentinel) Fail(SFK_HIP_INVALID_INPUT, "text must end with its only 0 (sentinel)");
+    if (f & kFlagLookback) Fail(SFK_HIP_BUILD_FAILURE, "radix look-back wait passed its bound");
+    if (f & kFlagLocalSegment) Fail(SFK_HIP_BUILD_FAILURE, "small-list segment larger than its bound");
+    if (f & kFlagRange) Fail(SFK_HIP_BUILD_FAILURE, "rank gather past the end of the text");
+    if (f != 0) {
+      Fail(SFK_HIP_BUILD_FAILURE, "device consistency check failed (flags " + std::to_string(f) + ")");
+    }
+  }
+
+  // Takes the sizes of the list the last RefineApply wrote from counters
+  // read back after it, and books its traffic.
+  const DeviceCounters& ApplySizes(const DeviceCounters& c) {
+    CheckFlags(c);
+    if (pending_apply_) {
+      constexpr std::uint64_t w = sizeof(Index);
+      const std::uint64_t m = pending_m_, survivors = c.survivors;
+      ctx_.stats.algorithmic_bytes_sa +=
+          m * (2 * pending_key_bytes_ + w) + (m - survivors) * 2 * w + survivors * (3 * w + 4);
+      pending_apply_ = false;
+    }
+    lm_ = c.survivors;
+    l_segs_ = c.segments;
+    lm_bound_ = lm_;
+    return c;
+  }
+
   // Reads the round's list sizes back, checks the error flags and makes the
   // next small list current.
   void EndRound(bool split) {
     const DeviceCounters& c = ReadCounters(b_);
-    if (c.error_flags & (kFlagLocalSegment | kFlagRange)) {
-      Fail(SFK_HIP_BUILD_FAILURE, c.error_flags & kFlagLocalSegment
-                                      ? "small-list segment larger than its bound"
-                                      : "rank gather past the end of the text");
-    }
+    CheckFlags(c);
+    if (pending_apply_) ApplySizes(c);
     qm_ = c.q_next;
     if (split) {
       lm_ = c.l_alloc & ((1ull << kLargeAllocShift) - 1);
@@ -434,6 +448,7 @@ class Doubling {
       lm_ = 0;
       l_segs_ = 0;
     }
+    lm_bound_ = lm_;
     if (qm_ > q_cap_) Fail(SFK_HIP_BUILD_FAILURE, "small list overflow");
     std::swap(q_idx_cur_, q_idx_next_);
     std::swap(q_row0_cur_, q_row0_next_);
@@ -447,17 +462,21 @@ class Doubling {
     Index* dest = b_->keys0.As<Index>();
     std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
     Index* large_delta = OtherDelta(l_delta_);
-    SplitSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(l_segs_, 256 * kSplitItems)), 256, 0, b_->stream>>>(
-        b_->seg_start.As<Index>(), l_delta_, l_segs_, lm_, kLocalMaxSegment, dest, new_id, large_delta, dc);
+    // Sizes come from the counters RefineSpine wrote; grids cover bounds
+    // (a list of m suffixes has at most m / 2 segments).
+    const std::uint64_t m_bound = std::max<std::uint64_t>(lm_bound_, 1);
+    const std::uint64_t seg_bound = pending_apply_ ? m_bound / 2 + 1 : std::max<std::uint64_t>(l_segs_, 1);
+    SplitSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(seg_bound, 256 * kSplitItems)), 256, 0, b_->stream>>>(
+        b_->seg_start.As<Index>(), l_delta_, kLocalMaxSegment, dest, new_id, large_delta, dc);
     SFK_HIP_CHECK(hipGetLastError());
     Index* l_idx_next = b_->vals0.As<Index>();
     std::uint32_t* l_seg_next = b_->vals1.As<std::uint32_t>();
-    SplitMoveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_, 256)), 256, 0, b_->stream>>>(
-        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), lm_, b_->seg_start.As<Index>(), l_delta_,
-        dest, new_id, q_idx_next_, q_row0_next_, l_idx_next, l_seg_next);
+    SplitMoveKernel<Index><<<static_cast<unsigned>(CeilDiv(m_bound, 256)), 256, 0, b_->stream>>>(
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), b_->seg_start.As<Index>(), l_delta_,
+        dest, new_id, q_idx_next_, q_row0_next_, l_idx_next, l_seg_next, dc);
     SFK_HIP_CHECK(hipGetLastError());
     constexpr std::uint64_t w = sizeof(Index);
-    ctx_.stats.algorithmic_bytes_sa += l_segs_ * (4 * w + 4) + lm_ * (2 * w + 8 + 2 * w);
+    ctx_.stats.algorithmic_bytes_sa += (m_bound / 2) * (4 * w + 4) + m_bound * (2 * w + 8 + 2 * w);
     l_idx_ = l_idx_next;
     l_seg_ = l_seg_next;
     l_delta_ = large_delta;
@@ -477,7 +496,11 @@ class Doubling {
     ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * w + (kExact ? 32 : w) + 2 * w + 2 * w);
   }
 
-  void GlobalStep(int sec_kind, int sec_bits, std::uint64_t h, bool iota, bool local_between) {
+  // readback: read the new list's sizes back right after RefineApply (the
+  // host needs them before the next step); otherwise only lm_bound_ (the
+  // input size) is known until the next read.
+  void GlobalStep(int sec_kind, int sec_bits, std::uint64_t h, bool iota, bool local_between,
+                  bool readback) {
     const int seg_bits = l_segs_ > 1 ? BitsFor(l_segs_ - 1) : 0;
     const int total_bits = seg_bits + sec_bits;
     if (total_bits > 64) {
@@ -491,6 +514,7 @@ class Doubling {
     } else {
       GlobalStepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between);
     }
+    if (readback) ApplySizes(ReadCounters(b_));
   }
 
   // One refine step of the large list: keygen, (the small list's step, whose
@@ -508,38 +532,17 @@ class Doubling {
     constexpr std::uint64_t w = sizeof(Index);
 
     const std::uint64_t all_bits = total_bits >= 64 ? ~0ull : ((1ull << total_bits) - 1);
-    std::uint64_t varying = all_bits;
     const bool rank_step = sec_kind == kSecRank;
-    if (!rank_step) ResetKeyCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
     const unsigned keygen_grid = static_cast<unsigned>(std::min<std::uint64_t>(CeilDiv(m, 256), 2048));
     // The large list may sit in vals0/vals1 (after a split): Keygen reads
     // slot k of it before writing slot k of vals0, in the same thread.
     KeygenKernel<Index, K><<<keygen_grid, 256, 0, b_->stream>>>(
-        l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, !rank_step, keys, vals, dc);
+        l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, dc);
     SFK_HIP_CHECK(hipGetLastError());
     ctx_.stats.algorithmic_bytes_sa +=
         m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
-    if (!rank_step) {
-      // Text-derived keys: skip digits that are constant over the active set
-      // (and surface the packing errors before any rank gather).
-      const DeviceCounters& c = ReadCounters(b_);
-      if (c.error_flags & kFlagBadSymbol) Fail(SFK_HIP_INVALID_INPUT, "text holds a symbol outside 0..6");
-      if (c.error_flags & kFlagBadSentinel) {
-        Fail(SFK_HIP_INVALID_INPUT, "text must end with its only 0 (sentinel)");
-      }
-      if (c.error_flags != 0) {
-        Fail(SFK_HIP_BUILD_FAILURE, "device consistency check failed (flags " +
-                                        std::to_string(c.error_flags) + ")");
-      }
-      std::uint64_t key_or = 0, key_and = ~0ull;
-      for (int i = 0; i < kKeySlots; ++i) {
-        key_or |= c.key_or[i];
-        key_and &= c.key_and[i];
-      }
-      varying = (key_or ^ key_and) & all_bits;
-    }
     if (local_between) LocalRefine<false>();
-    SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, varying, total_bits);
+    SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, all_bits, total_bits);
 
     // Boundaries, singletons, compaction.
     const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
@@ -552,15 +555,10 @@ class Doubling {
         b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next,
         b_->seg_start.As<Index>());
     SFK_HIP_CHECK(hipGetLastError());
-    const DeviceCounters& after = ReadCounters(b_);
-    if (after.error_flags & kFlagLookback) {
-      Fail(SFK_HIP_BUILD_FAILURE, "radix look-back wait passed its bound");
-    }
-    const std::uint64_t survivors = after.survivors;
-    ctx_.stats.algorithmic_bytes_sa +=
-        m * (2 * sizeof(K) + w) + (m - survivors) * 2 * w + survivors * (3 * w + 4);
-    lm_ = survivors;
-    l_segs_ = after.segments;
+    pending_apply_ = true;
+    pending_m_ = m;
+    pending_key_bytes_ = sizeof(K);
+    lm_bound_ = m;
     l_idx_ = b_->act_idx.As<Index>();
     l_seg_ = b_->act_seg.As<std::uint32_t>();
     l_delta_ = delta_next;
@@ -572,8 +570,13 @@ class Doubling {
   Index* sa_;
   Index* rank_;
   PackedText text_;
-  // Large list: suffixes, segment ids, per-segment deltas.
+  // Large list: suffixes, segment ids, per-segment deltas.  After a
+  // RefineApply whose sizes are not read back yet, lm_bound_ bounds lm_.
   std::uint64_t lm_ = 0;
+  std::uint64_t lm_bound_ = 0;
+  bool pending_apply_ = false;
+  std::uint64_t pending_m_ = 0;
+  std::uint64_t pending_key_bytes_ = 0;
   std::uint64_t l_segs_ = 0;
   const Index* l_idx_ = nullptr;
   const std::uint32_t* l_seg_ = nullptr;

@@ -11,7 +11,7 @@
 //               modulo the index width (a segment placed after segments with
 //               larger rows has a "negative" delta).  One step:
 //     Keygen        key = (segment id << sec_bits) | secondary(suffix)
-//     radix sort    stable LSD over the varying digits of the key
+//     radix sort    stable LSD over the digits of the key
 //     RefineReduce/RefineSpine/RefineApply
 //                   group boundaries (key change), singletons written to SA
 //                   and rank (= their final ISA entry), survivors compacted
@@ -59,14 +59,7 @@ enum ErrorFlag : std::uint32_t {
 };
 
 // Device counters read back by the host after each step.
-// Key OR/AND reductions are spread over kKeySlots slots (block b uses slot
-// b % kKeySlots) so that blocks do not queue on one address; the host folds
-// the slots.
-constexpr int kKeySlots = 32;
-
 struct DeviceCounters {
-  unsigned long long key_or[kKeySlots];
-  unsigned long long key_and[kKeySlots];
   unsigned long long survivors;
   unsigned long long segments;
   unsigned long long special_active;
@@ -215,14 +208,10 @@ template <typename Index, typename K>
 __global__ __launch_bounds__(256) void KeygenKernel(
     const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
-    const Index* __restrict__ rank, std::uint64_t h, bool iota, bool reduce,
+    const Index* __restrict__ rank, std::uint64_t h, bool iota,
     K* __restrict__ keys, Index* vals, DeviceCounters* counters) {
-  constexpr int kBlock = 256;
-  __shared__ std::uint64_t s_or[kBlock / kWave];
-  __shared__ std::uint64_t s_and[kBlock / kWave];
-  std::uint64_t kor = 0, kand = ~0ull;
-  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kBlock;
-  for (std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x; k < m;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+  for (std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < m;
        k += stride) {
     const Index v = iota ? static_cast<Index>(k) : act_idx[k];
     std::uint64_t sec;
@@ -249,40 +238,18 @@ __global__ __launch_bounds__(256) void KeygenKernel(
       }
     }
     const std::uint64_t seg = seg_bits ? static_cast<std::uint64_t>(act_seg[k]) : 0ull;
-    const std::uint64_t key = (seg << sec_bits) | sec;
-    keys[k] = static_cast<K>(key);
+    keys[k] = static_cast<K>((seg << sec_bits) | sec);
     vals[k] = v;
-    kor |= key;
-    kand &= key;
-  }
-  if (!reduce) return;  // rank steps sort every digit: no OR/AND needed
-  // One atomic pair per block: wave butterfly, then LDS across waves.
-  for (int d = 32; d >= 1; d >>= 1) {
-    kor |= WaveShfl(kor, static_cast<int>(LaneId() ^ d));
-    kand &= WaveShfl(kand, static_cast<int>(LaneId() ^ d));
-  }
-  const int wave = threadIdx.x / kWave;
-  if (LaneId() == 0) {
-    s_or[wave] = kor;
-    s_and[wave] = kand;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    std::uint64_t o = 0, a = ~0ull;
-    for (int w = 0; w < kBlock / kWave; ++w) {
-      o |= s_or[w];
-      a &= s_and[w];
-    }
-    atomicOr(&counters->key_or[blockIdx.x % kKeySlots], static_cast<unsigned long long>(o));
-    atomicAnd(&counters->key_and[blockIdx.x % kKeySlots], static_cast<unsigned long long>(a));
   }
 }
 
-// Counts active suffixes whose 32-symbol window holds a special symbol.
+// Counts active suffixes whose 32-symbol window holds a special symbol, over
+// the list RefineApply just wrote (its size is read from the counters; the
+// grid covers an upper bound).
 template <typename Index>
 __global__ __launch_bounds__(256) void CountSpecialActiveKernel(
-    const Index* __restrict__ act_idx, std::uint64_t m, PackedText t,
-    DeviceCounters* counters) {
+    const Index* __restrict__ act_idx, PackedText t, DeviceCounters* counters) {
+  const std::uint64_t m = counters->survivors;
   const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool special = false;
   if (k < m) {
@@ -440,7 +407,8 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
 // ------------------------------------------------------------ split -----
 
 // kSplitItems consecutive segments per thread of the list RefineApply just
-// wrote (segs segments, m suffixes, first slots in seg_start).  A segment of
+// wrote (its segment and suffix counts are read from the counters; the grid
+// covers an upper bound; first slots in seg_start).  A segment of
 // at most small_max suffixes reserves its slots in the next small list (one
 // atomic per block); a larger one takes its new id and first slot in the
 // next large list from one 64-bit atomic, so ids grow with slots, and gets
@@ -449,10 +417,12 @@ constexpr int kSplitItems = 8;
 
 template <typename Index>
 __global__ __launch_bounds__(256) void SplitSegmentsKernel(
-    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta, std::uint64_t segs,
-    std::uint64_t m, std::uint32_t small_max, Index* __restrict__ dest,
+    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
+    std::uint32_t small_max, Index* __restrict__ dest,
     std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, DeviceCounters* counters) {
   constexpr int kBlock = 256;
+  const std::uint64_t segs = counters->segments;
+  const std::uint64_t m = counters->survivors;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   __shared__ std::uint64_t s_base;
   const std::uint64_t first = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) * kSplitItems;
@@ -497,16 +467,17 @@ __global__ __launch_bounds__(256) void SplitSegmentsKernel(
   }
 }
 
-// One thread per suffix of that list: copy it to its segment's destination.
+// One thread per suffix of that list (grid: an upper bound): copy it to its
+// segment's destination.
 template <typename Index>
 __global__ __launch_bounds__(256) void SplitMoveKernel(
-    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg, std::uint64_t m,
+    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg,
     const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
     const Index* __restrict__ dest, const std::uint32_t* __restrict__ new_id,
     Index* __restrict__ q_idx, Index* __restrict__ q_row0, Index* __restrict__ l_idx,
-    std::uint32_t* __restrict__ l_seg) {
+    std::uint32_t* __restrict__ l_seg, const DeviceCounters* counters) {
   const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= m) return;
+  if (p >= counters->survivors) return;
   const std::uint32_t s = act_seg[p];
   const Index start = seg_start[s];
   const std::uint64_t at = static_cast<std::uint64_t>(dest[s]) + (p - static_cast<std::uint64_t>(start));
@@ -577,26 +548,31 @@ __device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
   return ExactKey{static_cast<std::uint32_t>(hi >> 32), static_cast<std::uint32_t>(hi), lo};
 }
 
-// One refine step of the small list.  kExact: the key is the exact 96-bit
-// key of the suffix's 32-symbol window (the small list's first step after the
-// 16-symbol split); otherwise it is q_key (rank[suffix + h]).  A block reads
-// kLocalSpan slots, kLocalItems consecutive slots per thread, and owns the
-// segments that start in its first kLocalOwn slots.  Segment starts come
-// from a block max-scan of "row0 changes here" flags, sizes from LDS
-// counters.  An owned suffix counts, inside its segment, the keys below its
-// own (its group's first row is row0 + that count) and the keys equal to it
-// (its group's size).  The first suffix of each surviving group reserves the
-// group's slots in the next small list (one atomic per block).
+// One refine step of the small list.  A block reads kLocalSpan slots,
+// kLocalItems consecutive slots per thread, and owns the segments that start
+// in its first kLocalOwn slots.  Segment starts come from a block max-scan of
+// "row0 changes here" flags, sizes from LDS counters.  An owned suffix counts,
+// inside its segment, the keys below its own (its group's first row is row0 +
+// that count) and the keys equal to it (its group's size); the first suffix
+// of each surviving group reserves the group's slots in the next small list
+// (one atomic per block).
+//
+// Keys.  kExact (the small list's first step, on segments of equal 16-symbol
+// FillKey high halves): the low half of FillKey, which orders windows without
+// a separator, N or sentinel exactly; a segment holding such a window compares
+// the exact 96-bit keys instead.  Otherwise q_key (rank[suffix + h]).
 template <typename Index, bool kExact>
 __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     PackedText t, const Index* __restrict__ q_idx, const Index* __restrict__ q_row0,
     const Index* __restrict__ q_key, std::uint64_t qm, std::uint64_t n,
     Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
     Index* __restrict__ out_row0, DeviceCounters* counters) {
-  using KeyT = typename std::conditional<kExact, ExactKey, Index>::type;
+  using KeyT = typename std::conditional<kExact, std::uint32_t, Index>::type;
   constexpr int kItems = kLocalItems;
+  constexpr std::uint32_t kSizeMask = 0xFFFFu;  // s_count: size | (specials << 16)
   __shared__ Index s_row0[kLocalSpan + 1];  // s_row0[k + 1] = slot k, s_row0[0] = slot -1
   __shared__ KeyT s_key[kLocalSpan];
+  __shared__ ExactKey s_exact[kExact ? kLocalSpan : 1];
   __shared__ std::uint32_t s_count[kLocalSpan];
   __shared__ std::uint32_t s_off[kLocalSpan];
   __shared__ std::uint64_t scratch[kLocalThreads / kWave];
@@ -607,20 +583,22 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
   const int valid_len =
       static_cast<int>(qm - s0 < static_cast<std::uint64_t>(kLocalSpan) ? qm - s0 : kLocalSpan);
   Index v[kItems], r0[kItems];
+  std::uint64_t codes[kItems];
+  std::uint32_t spec[kItems];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const int k = tid * kItems + i;
     v[i] = 0;
     r0[i] = 0;
-    KeyT key{};
+    codes[i] = 0;
+    spec[i] = 0;
+    KeyT key = 0;
     if (k < valid_len) {
       v[i] = q_idx[s0 + k];
       r0[i] = q_row0[s0 + k];
       if constexpr (kExact) {
-        std::uint64_t c;
-        std::uint32_t sp;
-        LoadWindow(t, static_cast<std::uint64_t>(v[i]), c, sp);
-        key = ExactKeyOf(c, sp);
+        LoadWindow(t, static_cast<std::uint64_t>(v[i]), codes[i], spec[i]);
+        key = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]));
       } else {
         key = q_key[s0 + k];
       }
@@ -659,10 +637,20 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     const std::uint64_t mk = mark[i] > prev ? mark[i] : prev;
     if (k < valid_len && mk > 0) {
       st[i] = static_cast<int>(mk) - 1;
-      atomicAdd(&s_count[st[i]], 1u);
+      atomicAdd(&s_count[st[i]], 1u + ((spec[i] != 0) ? (1u << 16) : 0u));
     }
   }
   __syncthreads();
+  if constexpr (kExact) {
+    // Exact keys only where a segment holds a special window.
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (st[i] >= 0 && (s_count[st[i]] >> 16) != 0) {
+        s_exact[tid * kItems + i] = ExactKeyOf(codes[i], spec[i]);
+      }
+    }
+    __syncthreads();
+  }
 
   bool survivor[kItems];
   std::uint32_t equal[kItems], tie[kItems];
@@ -678,22 +666,37 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     first_equal[i] = k;
     head[i] = r0[i];
     if (st[i] < 0 || st[i] >= kLocalOwn) continue;  // not owned by this block
-    const std::uint32_t size = s_count[st[i]];
+    const std::uint32_t packed = s_count[st[i]];
+    const std::uint32_t size = packed & kSizeMask;
     if (size > kLocalMaxSegment) {
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagLocalSegment));
       continue;
     }
     const int en = st[i] + static_cast<int>(size);
-    const KeyT key = s_key[k];
     std::uint32_t less = 0;
-    for (int j = st[i]; j < en; ++j) {
-      const KeyT kj = s_key[j];
-      if (KeyLess(kj, key)) {
-        ++less;
-      } else if (KeyEqual(kj, key)) {
-        if (equal[i] == 0) first_equal[i] = j;
-        ++equal[i];
-        tie[i] += j < k;
+    if (kExact && (packed >> 16) != 0) {
+      const ExactKey key = s_exact[k];
+      for (int j = st[i]; j < en; ++j) {
+        const ExactKey kj = s_exact[j];
+        if (KeyLess(kj, key)) {
+          ++less;
+        } else if (KeyEqual(kj, key)) {
+          if (equal[i] == 0) first_equal[i] = j;
+          ++equal[i];
+          tie[i] += j < k;
+        }
+      }
+    } else {
+      const KeyT key = s_key[k];
+      for (int j = st[i]; j < en; ++j) {
+        const KeyT kj = s_key[j];
+        if (kj < key) {
+          ++less;
+        } else if (kj == key) {
+          if (equal[i] == 0) first_equal[i] = j;
+          ++equal[i];
+          tie[i] += j < k;
+        }
       }
     }
     head[i] = static_cast<Index>(r0[i] + static_cast<Index>(less));
