@@ -326,7 +326,10 @@ class Doubling {
       SFK_HIP_CHECK(hipGetLastError());
     }
     // First 16 symbols over every suffix (one segment per reference).
-    GlobalStep(kSecFillHi, 32, 0, /*iota=*/true, /*local_between=*/false, /*readback=*/true);
+    // Its survivors' ranks are not written: the next step writes the rank of
+    // every survivor (small list and large list alike) before any gather.
+    GlobalStep(kSecFillHi, 32, 0, /*iota=*/true, /*local_between=*/false, /*readback=*/true,
+               /*survivor_rank=*/false);
     if (ctx_.refs > 0 && b_->host_counters->zero_count != ctx_.refs) {
       Fail(SFK_HIP_INVALID_INPUT, "every reference of a batch must end with its only 0 (sentinel)");
     }
@@ -350,8 +353,8 @@ class Doubling {
       SFK_HIP_CHECK(hipGetLastError());
       const bool special = ApplySizes(ReadCounters(b_)).special_active > 0;
       if (special && lm_ > 0) {
-        GlobalStep(kSecExact0, 32, 0, false, false, true);
-        if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false, true);
+        GlobalStep(kSecExact0, 32, 0, false, false, true, /*survivor_rank=*/false);
+        if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false, true, /*survivor_rank=*/false);
         if (lm_ > 0) GlobalStep(kSecExact2, 32, 0, false, false, true);
       }
       if (lm_ > 0) {
@@ -500,7 +503,7 @@ class Doubling {
   // host needs them before the next step); otherwise only lm_bound_ (the
   // input size) is known until the next read.
   void GlobalStep(int sec_kind, int sec_bits, std::uint64_t h, bool iota, bool local_between,
-                  bool readback) {
+                  bool readback, bool survivor_rank = true) {
     const int seg_bits = l_segs_ > 1 ? BitsFor(l_segs_ - 1) : 0;
     const int total_bits = seg_bits + sec_bits;
     if (total_bits > 64) {
@@ -510,9 +513,9 @@ class Doubling {
     ctx_.stats.refine_steps += 1;
     ctx_.stats.active_total += lm_;
     if (total_bits <= 32) {
-      GlobalStepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between);
+      GlobalStepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between, survivor_rank);
     } else {
-      GlobalStepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between);
+      GlobalStepTyped<std::uint64_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between, survivor_rank);
     }
     if (readback) ApplySizes(ReadCounters(b_));
   }
@@ -521,7 +524,7 @@ class Doubling {
   // rank gathers are done), radix sort, boundaries/singletons/compaction.
   template <typename K>
   void GlobalStepTyped(int sec_kind, int sec_bits, int seg_bits, std::uint64_t h, bool iota,
-                       bool local_between) {
+                       bool local_between, bool survivor_rank) {
     const std::uint64_t m = lm_;
     const int total_bits = seg_bits + sec_bits;
     K* keys = b_->keys0.As<K>();
@@ -553,7 +556,7 @@ class Doubling {
     RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
         keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_,
         b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next,
-        b_->seg_start.As<Index>());
+        b_->seg_start.As<Index>(), survivor_rank);
     SFK_HIP_CHECK(hipGetLastError());
     pending_apply_ = true;
     pending_m_ = m;

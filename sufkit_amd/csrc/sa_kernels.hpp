@@ -331,7 +331,14 @@ __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial
   }
 }
 
-// Each thread owns kItems consecutive sorted slots of one tile.
+// One block per chunk of whole kTile-slot tiles (offsets from the spine).
+// A tile is staged in LDS with coalesced loads (plus the key before and after
+// it); flags and per-slot offsets come from a blocked scan (thread t owns
+// slots 8t..8t+7); the writes then go out striped, so that consecutive lanes
+// write consecutive SA rows and consecutive compacted slots.  survivor_rank:
+// also set rank[v] of surviving suffixes to their group's first row (a step
+// whose successor rewrites every survivor's rank before any rank is
+// gathered skips it).
 template <typename Index, typename K, int kItems>
 __global__ __launch_bounds__(256) void RefineApplyKernel(
     const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m,
@@ -339,9 +346,16 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
     const RefinePartial* __restrict__ partial,
     std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
     std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
-    Index* __restrict__ out_start) {
+    Index* __restrict__ out_start, bool survivor_rank) {
   constexpr int kBlock = 256;
   constexpr int kTile = kBlock * kItems;
+  __shared__ K s_key[kTile + 2];  // s_key[i + 1] = tile slot i
+  __shared__ Index s_val[kTile];
+  static_assert(kTile <= 32767, "per-slot offsets are 16-bit");
+  __shared__ std::uint16_t s_pos[kTile];    // survivor slot, relative to the tile's first
+  __shared__ std::uint16_t s_segc[kTile];   // segments opened through this slot (relative)
+  __shared__ std::int16_t s_group[kTile];   // slot of the group start, -1: before the tile
+  __shared__ std::uint8_t s_flags[kTile];   // bit0 starts, bit1 single
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
@@ -349,58 +363,99 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
   std::uint64_t run_segs = partial[blockIdx.x].newsegs;
   std::uint64_t run_last = partial[blockIdx.x].last_boundary;
   for (std::uint64_t base = begin; base < end; base += kTile) {
-    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
-    bool starts[kItems], single[kItems];
-    std::uint64_t surv = 0, segs = 0, last = 0;
+    const int len = static_cast<int>(end - base < static_cast<std::uint64_t>(kTile) ? end - base : kTile);
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      starts[i] = false;
-      single[i] = true;
-      if (k < end) {
-        BoundaryFlags(keys, k, m, starts[i], single[i]);
-        surv += !single[i];
-        segs += starts[i] && !single[i];
-        if (starts[i]) last = k + 1;
+    for (int j = 0; j < kItems; ++j) {
+      const int i = j * kBlock + static_cast<int>(threadIdx.x);
+      if (i < len) {
+        s_key[i + 1] = keys[base + i];
+        s_val[i] = vals[base + i];
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_key[0] = base > 0 ? keys[base - 1] : K(0);
+      s_key[len + 1] = base + len < m ? keys[base + len] : K(0);
+    }
+    __syncthreads();
+    // Blocked pass: flags, counts, then per-slot offsets.
+    const int first = static_cast<int>(threadIdx.x) * kItems;
+    std::uint64_t surv = 0, segs = 0, last = 0;
+    std::uint8_t fl[kItems];
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+      const int i = first + q;
+      fl[q] = 0;
+      if (i < len) {
+        const std::uint64_t k = base + i;
+        // Branch-free (all three LDS reads are in bounds), like the segment
+        // starts of LocalRefineKernel.
+        const K mine = s_key[i + 1];
+        const K before = s_key[i];
+        const K after = s_key[i + 2];
+        const bool starts = (k == 0) | (before != mine);
+        const bool ends = (k + 1 == m) | (after != mine);
+        const bool single = starts & ends;
+        fl[q] = static_cast<std::uint8_t>((starts ? 1 : 0) | (single ? 2 : 0));
+        surv += !single;
+        segs += starts && !single;
+        if (starts) last = static_cast<std::uint64_t>(i) + 1;
       }
     }
     std::uint64_t tot_surv, tot_segs, tot_last;
-    std::uint64_t pos = run_surv + BlockExclusiveSum<kBlock>(surv, scratch, tot_surv);
-    std::uint64_t segc = run_segs + BlockExclusiveSum<kBlock>(segs, scratch, tot_segs);
-    std::uint64_t cur_last = BlockExclusiveMax<kBlock>(last, scratch, tot_last);
-    if (run_last > cur_last) cur_last = run_last;
+    std::uint64_t pos = BlockExclusiveSum<kBlock>(surv, scratch, tot_surv);
+    std::uint64_t segc = BlockExclusiveSum<kBlock>(segs, scratch, tot_segs);
+    std::uint64_t cur_last = BlockExclusiveMax<kBlock>(last, scratch, tot_last);  // 0: none in tile yet
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      if (k >= end) break;
-      const K key = keys[k];
-      const Index v = vals[k];
+    for (int q = 0; q < kItems; ++q) {
+      const int i = first + q;
+      if (i < len) {
+        const bool starts = fl[q] & 1, single = fl[q] & 2;
+        if (starts) cur_last = static_cast<std::uint64_t>(i) + 1;
+        if (!single && starts) ++segc;
+        s_pos[i] = static_cast<std::uint16_t>(pos);
+        s_segc[i] = static_cast<std::uint16_t>(segc);
+        s_group[i] = static_cast<std::int16_t>(static_cast<int>(cur_last) - 1);
+        s_flags[i] = fl[q];
+        if (!single) ++pos;
+      }
+    }
+    __syncthreads();
+    // Striped pass: writes.
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int i = j * kBlock + static_cast<int>(threadIdx.x);
+      if (i >= len) break;
+      const std::uint64_t k = base + i;
+      const K key = s_key[i + 1];
+      const Index v = s_val[i];
+      const std::uint8_t f = s_flags[i];
       const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
       const Index delta = seg_delta[seg];
       const Index row = static_cast<Index>(delta + static_cast<Index>(k));
-      if (starts[i]) cur_last = k + 1;
       if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
-      if (single[i]) {
+      if (f & 2) {
         sa[row] = v;
         rank[v] = row;
       } else {
         // The group start lies in the same segment: every segment start is a
         // key change, hence a group start.
-        const Index head_row = static_cast<Index>(delta + static_cast<Index>(cur_last - 1));
-        rank[v] = head_row;
-        if (starts[i]) {
-          out_delta[segc] = static_cast<Index>(row - static_cast<Index>(pos));
-          out_start[segc] = static_cast<Index>(pos);
-          ++segc;
+        const int g = s_group[i];
+        const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
+        if (survivor_rank) rank[v] = static_cast<Index>(delta + static_cast<Index>(group_slot));
+        const std::uint64_t at = run_surv + s_pos[i];
+        const std::uint64_t sc = run_segs + s_segc[i];  // segments opened through this slot
+        if (f & 1) {
+          out_delta[sc - 1] = static_cast<Index>(row - static_cast<Index>(at));
+          out_start[sc - 1] = static_cast<Index>(at);
         }
-        out_idx[pos] = v;
-        out_seg[pos] = static_cast<std::uint32_t>(segc - 1);
-        ++pos;
+        out_idx[at] = v;
+        out_seg[at] = static_cast<std::uint32_t>(sc - 1);
       }
     }
     run_surv += tot_surv;
     run_segs += tot_segs;
-    if (tot_last > run_last) run_last = tot_last;
+    if (tot_last > 0) run_last = base + tot_last;
+    __syncthreads();
   }
 }
 
