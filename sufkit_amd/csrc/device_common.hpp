@@ -37,18 +37,22 @@ __device__ inline std::uint64_t EntryCodes(const uint4& e) {
   return (static_cast<std::uint64_t>(e.y) << 32) | e.x;
 }
 
-// The 32-symbol window starting at position i: 2-bit codes and spec mask.
-__device__ inline void LoadWindow(const PackedText& t, std::uint64_t i,
-                                  std::uint64_t& codes, std::uint32_t& spec) {
-  const std::uint64_t w = i >> 5;
-  const std::uint32_t o = static_cast<std::uint32_t>(i & 31);
-  const uint4 e0 = t.words[w];
-  const uint4 e1 = t.words[w + 1];
+// The 32-symbol window starting at offset o (0..31) of entry e0, continued
+// in entry e1: 2-bit codes and spec mask.
+__device__ inline void WindowOf(const uint4& e0, const uint4& e1, std::uint32_t o,
+                                std::uint64_t& codes, std::uint32_t& spec) {
   const std::uint64_t c0 = EntryCodes(e0);
   const std::uint64_t c1 = EntryCodes(e1);
   codes = o ? (c0 << (2 * o)) | (c1 >> (64 - 2 * o)) : c0;
   const std::uint64_t s01 = (static_cast<std::uint64_t>(e0.z) << 32) | e1.z;
   spec = static_cast<std::uint32_t>((s01 << o) >> 32);
+}
+
+// The 32-symbol window starting at position i: 2-bit codes and spec mask.
+__device__ inline void LoadWindow(const PackedText& t, std::uint64_t i,
+                                  std::uint64_t& codes, std::uint32_t& spec) {
+  const std::uint64_t w = i >> 5;
+  WindowOf(t.words[w], t.words[w + 1], static_cast<std::uint32_t>(i & 31), codes, spec);
 }
 
 // Code and spec bit of the single symbol at position p.

@@ -1,14 +1,16 @@
 // Stable LSD radix sort of (key, value) pairs, 8-bit digits, for gfx950.
 //
-// A sort is one histogram kernel that counts every pass's digits in a single
-// read of the keys, one tiny scan into bucket starts, and then one onesweep
-// kernel per digit pass: each block takes the next tile id from an atomic
+// The kernel that writes the keys also counts every pass's digits
+// (CountDigits / FlushDigits); then one onesweep kernel per digit pass: each
+// block turns the pass's digit counts into bucket starts, takes the next tile
+// id from an atomic
 // counter, ranks its tile stably in LDS with wave ballots (64-lane peer
 // masks), publishes its per-digit counts, finds the counts of all earlier
 // tiles by looking back over their published status words, and writes the
 // tile out in digit order so that consecutive lanes store consecutive
-// addresses of one bucket.  HBM traffic per element: key once for the
-// histogram, then per pass one read and one write of key and value.
+// addresses of one bucket.  HBM traffic per element and pass: one read and
+// one write of key and value (the first pass may generate its input instead
+// of reading it, see Source).
 // Digits whose bits are constant over all keys are skipped by the caller.
 #pragma once
 
@@ -49,94 +51,57 @@ struct RadixPasses {
   int shift[kMax];
 };
 
-// Clears the digit totals and the tile counters of one sort.
-__global__ __launch_bounds__(256) void RadixSortInit(unsigned long long* __restrict__ hist,
-                                                     unsigned int* __restrict__ tile_counter) {
-  for (int p = 0; p < RadixPasses::kMax; ++p) hist[p * RadixTile::kRadix + threadIdx.x] = 0;
-  if (threadIdx.x < RadixPasses::kMax) tile_counter[threadIdx.x] = 0;
+// Digit counts of one sort, hist[pass][digit], and the tile counters of its
+// passes live in one of two alternating sets; the kernel that counts a sort's
+// digits clears the other set for the next sort (which runs after this one on
+// the same stream).
+struct RadixSet {
+  unsigned long long* hist;   // [RadixPasses::kMax][kRadix]
+  unsigned int* tiles;        // [RadixPasses::kMax]
+  unsigned long long* next_hist;
+  unsigned int* next_tiles;
+};
+
+constexpr std::uint64_t kRadixSetBytes =
+    static_cast<std::uint64_t>(RadixPasses::kMax) * RadixTile::kRadix * 8 + RadixPasses::kMax * 4;
+
+// Clears the other set (block 0 only; call before any block's __syncthreads).
+__device__ inline void ClearNextRadixSet(const RadixSet& set) {
+  if (blockIdx.x != 0) return;
+  for (int i = threadIdx.x; i < RadixPasses::kMax * RadixTile::kRadix; i += blockDim.x) set.next_hist[i] = 0;
+  if (threadIdx.x < RadixPasses::kMax) set.next_tiles[threadIdx.x] = 0;
 }
 
-// Digit totals of every pass in one read of the keys: hist[pass][digit]
-// (accumulated with atomics; cleared by RadixSortInit).  Each wave owns an
-// LDS sub-histogram and reads 16 bytes per lane per step.  Keys are often
-// sorted by their high bits (segment ids), so for one key slot the 64 lanes
-// of a wave frequently share a digit: that case is one atomic of 64 instead
-// of 64 atomics on one LDS address.
-template <typename K, int kBlock>
-__global__ __launch_bounds__(kBlock) void RadixHistogramAll(
-    const K* __restrict__ keys, std::uint64_t m, RadixPasses passes,
-    unsigned long long* __restrict__ hist) {
+// Adds one key per lane to this wave's LDS digit counts (local[pass][digit]).
+// full: every lane of the wave holds a key (wave-uniform), which allows one
+// atomic of 64 when all lanes share a digit (keys sorted by their high bits
+// often do).
+template <typename K>
+__device__ inline void CountDigits(std::uint32_t (*local)[RadixTile::kRadix], const RadixPasses& passes,
+                                   K key, bool valid, bool full) {
   constexpr int kRadix = RadixTile::kRadix;
-  constexpr int kMaxPasses = RadixPasses::kMax;
-  constexpr int kWaves = kBlock / kWave;
-  constexpr int kVec = 16 / sizeof(K);
-  __shared__ std::uint32_t local[kWaves][kMaxPasses][kRadix];
-  const int count = passes.count;
-  for (int i = threadIdx.x; i < kWaves * kMaxPasses * kRadix; i += kBlock) {
-    if ((i / kRadix) % kMaxPasses < count) (&local[0][0][0])[i] = 0;
-  }
-  __syncthreads();
-  const int wave = threadIdx.x / kWave;
-  const std::uint32_t lane = LaneId();
-  const std::uint64_t nvec = m / kVec;
-  const uint4* vkeys = reinterpret_cast<const uint4*>(keys);
-  // A wave takes kUnroll consecutive 64-vector groups per step, loading all
-  // of them before counting so the loads overlap.
-  constexpr int kUnroll = 4;
-  constexpr std::uint64_t kWaveSpan = static_cast<std::uint64_t>(kWave) * kUnroll;
-  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kBlock * kUnroll;
-  for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWaveSpan;
-       wbase < nvec; wbase += stride) {
-    if (wbase + kWaveSpan <= nvec) {
-      // Whole span in range: wave-wide predicates are well defined.
-      uint4 q[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) q[u] = vkeys[wbase + u * kWave + lane];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        K ks[kVec];
-        __builtin_memcpy(ks, &q[u], sizeof(uint4));
-#pragma unroll
-        for (int j = 0; j < kVec; ++j) {
-          for (int p = 0; p < count; ++p) {
-            const std::uint32_t d = static_cast<std::uint32_t>(ks[j] >> passes.shift[p]) & (kRadix - 1);
-            const std::uint32_t first = __builtin_amdgcn_readfirstlane(d);
-            if (__all(d == first)) {
-              if (lane == 0) atomicAdd(&local[wave][p][first], static_cast<std::uint32_t>(kWave));
-            } else {
-              atomicAdd(&local[wave][p][d], 1u);
-            }
-          }
-        }
+  for (int p = 0; p < passes.count; ++p) {
+    const std::uint32_t d = static_cast<std::uint32_t>(key >> passes.shift[p]) & (kRadix - 1);
+    if (full) {
+      const std::uint32_t first = __builtin_amdgcn_readfirstlane(d);
+      if (__all(d == first)) {
+        if (LaneId() == 0) atomicAdd(&local[p][first], static_cast<std::uint32_t>(kWave));
+      } else {
+        atomicAdd(&local[p][d], 1u);
       }
-    } else {
-      for (int u = 0; u < kUnroll; ++u) {
-        const std::uint64_t v = wbase + u * kWave + lane;
-        if (v >= nvec) break;
-        const uint4 q = vkeys[v];
-        K ks[kVec];
-        __builtin_memcpy(ks, &q, sizeof(q));
-#pragma unroll
-        for (int j = 0; j < kVec; ++j) {
-          for (int p = 0; p < count; ++p) {
-            atomicAdd(&local[wave][p][static_cast<std::uint32_t>(ks[j] >> passes.shift[p]) & (kRadix - 1)], 1u);
-          }
-        }
-      }
+    } else if (valid) {
+      atomicAdd(&local[p][d], 1u);
     }
   }
-  // Keys past the last whole 16-byte vector.
-  if (blockIdx.x == 0) {
-    const std::uint64_t i = nvec * kVec + threadIdx.x;
-    if (threadIdx.x < kVec && i < m) {
-      const K k = keys[i];
-      for (int p = 0; p < count; ++p) {
-        atomicAdd(&local[wave][p][static_cast<std::uint32_t>(k >> passes.shift[p]) & (kRadix - 1)], 1u);
-      }
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < count * kRadix; i += kBlock) {
+}
+
+// Sums the block's wave counts into the sort's digit counts (call after a
+// __syncthreads that follows the last CountDigits).
+template <int kWaves>
+__device__ inline void FlushDigits(std::uint32_t (*local)[RadixPasses::kMax][RadixTile::kRadix],
+                                   const RadixPasses& passes, unsigned long long* hist) {
+  constexpr int kRadix = RadixTile::kRadix;
+  for (int i = threadIdx.x; i < passes.count * kRadix; i += blockDim.x) {
     const int p = i / kRadix, d = i % kRadix;
     std::uint32_t sum = 0;
 #pragma unroll
@@ -145,24 +110,22 @@ __global__ __launch_bounds__(kBlock) void RadixHistogramAll(
   }
 }
 
-// Exclusive scan of each pass's digit totals into bucket starts (in place).
-__global__ __launch_bounds__(256) void RadixBucketStarts(unsigned long long* hist, int passes) {
-  constexpr int kBlock = 256;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  for (int p = 0; p < passes; ++p) {
-    std::uint64_t total;
-    const std::uint64_t v = hist[p * kBlock + threadIdx.x];
-    const std::uint64_t ex = BlockExclusiveSum<kBlock>(v, scratch, total);
-    hist[p * kBlock + threadIdx.x] = ex;
-    __syncthreads();
+// Input of a pass read from arrays.
+template <typename K, typename V>
+struct ArraySource {
+  const K* __restrict__ keys;
+  const V* __restrict__ vals;
+  __device__ void Load(std::uint64_t i, K& key, V& val) const {
+    key = keys[i];
+    val = vals[i];
   }
-}
+};
 
-template <typename K, typename V, int kBlock, int kItems>
+// digit_count: this pass's counts per digit (from CountDigits).
+template <typename K, typename V, int kBlock, int kItems, typename Source>
 __global__ __launch_bounds__(kBlock) void RadixOnesweep(
-    const K* __restrict__ keys_in, const V* __restrict__ vals_in, K* __restrict__ keys_out,
-    V* __restrict__ vals_out, std::uint64_t m, int shift,
-    const unsigned long long* __restrict__ bucket_start, unsigned long long* status,
+    Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
+    const unsigned long long* __restrict__ digit_count, unsigned long long* status,
     unsigned int* tile_counter, std::uint32_t epoch, unsigned int* error_flags,
     std::uint32_t error_bit) {
   constexpr int kRadix = RadixTile::kRadix;
@@ -172,7 +135,9 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint32_t wave_cnt[kWaves][kRadix];
   __shared__ std::uint32_t local_start[kRadix];
   __shared__ std::uint64_t global_base[kRadix];
+  __shared__ std::uint64_t bucket_start[kRadix];
   __shared__ std::uint32_t scan_scratch[kBlock / kWave];
+  __shared__ std::uint64_t scan64[kBlock / kWave];
   __shared__ unsigned int s_tile;
   __shared__ K s_keys[kTile];
   __shared__ V s_vals[kTile];
@@ -183,6 +148,14 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;
   __syncthreads();
   const std::uint64_t tile = s_tile;
+  if (tile == 0) {
+    // Bucket starts of this pass (exclusive scan of its digit counts) enter
+    // the look-back through tile 0's inclusive values.
+    std::uint64_t total;
+    const std::uint64_t cnt = threadIdx.x < kRadix ? digit_count[threadIdx.x] : 0ull;
+    const std::uint64_t ex = BlockExclusiveSum<kBlock>(cnt, scan64, total);
+    if (threadIdx.x < kRadix) bucket_start[threadIdx.x] = ex;
+  }
   const std::uint64_t base = tile * kTile;
   const std::uint64_t end = base + kTile < m ? base + kTile : m;
 
@@ -195,8 +168,13 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   for (int j = 0; j < kItems; ++j) {
     const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
     const bool valid = idx < end;
-    key[j] = valid ? keys_in[idx] : K(0);
-    val[j] = valid ? vals_in[idx] : V(0);
+    // Every lane loads (slot 0 when out of range) and then selects: the
+    // branchy form made hipcc spill the 1024 x 16 tile's registers.
+    K kk;
+    V vv;
+    src.Load(valid ? idx : 0, kk, vv);
+    key[j] = valid ? kk : K(0);
+    val[j] = valid ? vv : V(0);
     const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
     digit[j] = d;
     const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
@@ -226,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
     unsigned long long* slot = status + tile * kRadix + d;
     if (tile == 0) {
-      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | s, __ATOMIC_RELAXED,
+      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (bucket_start[d] + s), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       global_base[d] = bucket_start[d];
     } else {
@@ -264,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       }
       __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (prefix + s), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-      global_base[d] = bucket_start[d] + prefix;
+      global_base[d] = prefix;
     }
   }
   __syncthreads();

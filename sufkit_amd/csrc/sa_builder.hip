@@ -117,6 +117,7 @@ struct sfk_hip_builder {
       vals0, vals1, seg_delta0, seg_delta1, seg_start, radix_counts, radix_status,
       partial, counters, narrow, ref_offsets, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
   std::uint32_t sort_epoch = 0;  // epoch of the last onesweep pass
+  int radix_set = 0;             // radix count set of the next sort
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
   std::size_t events_used = 0;
@@ -228,16 +229,31 @@ std::uint32_t NextEpoch(sfk_hip_builder* b) {
   return b->sort_epoch;
 }
 
-struct SortScratch {
-  unsigned long long* hist;
-  unsigned int* tile_counter;
-  unsigned long long* status;
-  unsigned int* error_flags;
-};
+// The radix count sets of the builder alternate between consecutive sorts.
+constexpr std::uint64_t kRadixSetStride = (kRadixSetBytes + 255) / 256 * 256;
 
-template <typename K, typename V, int kBlock, int kItems>
-void OnesweepPasses(BuildContext& ctx, K*& keys, V*& vals, K*& keys_alt, V*& vals_alt,
-                    std::uint64_t m, const RadixPasses& passes, const SortScratch& sc) {
+RadixSet TakeRadixSet(sfk_hip_builder* b) {
+  char* base = b->radix_counts.As<char>();
+  char* cur = base + b->radix_set * kRadixSetStride;
+  char* next = base + (1 - b->radix_set) * kRadixSetStride;
+  b->radix_set ^= 1;
+  constexpr std::uint64_t hist_bytes = static_cast<std::uint64_t>(RadixPasses::kMax) * RadixTile::kRadix * 8;
+  return RadixSet{reinterpret_cast<unsigned long long*>(cur), reinterpret_cast<unsigned int*>(cur + hist_bytes),
+                  reinterpret_cast<unsigned long long*>(next), reinterpret_cast<unsigned int*>(next + hist_bytes)};
+}
+
+// Every 8-bit digit below total_bits (keys are not inspected for constant
+// digits: that would take a host round trip per step).
+RadixPasses AllDigits(int total_bits) {
+  RadixPasses passes{};
+  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) passes.shift[passes.count++] = shift;
+  return passes;
+}
+
+template <typename K, typename V, int kBlock, int kItems, typename Source>
+void OnesweepPasses(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys,
+                    V*& vals, K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes,
+                    const RadixSet& set) {
   sfk_hip_builder* b = ctx.b;
   constexpr std::uint64_t kTile = static_cast<std::uint64_t>(kBlock) * kItems;
   static_assert(kTile >= kMinOnesweepTile, "status array sized for the smallest tile");
@@ -245,53 +261,46 @@ void OnesweepPasses(BuildContext& ctx, K*& keys, V*& vals, K*& keys_alt, V*& val
   if ((tiles + 1) * RadixTile::kRadix * 8 > b->radix_status.Bytes()) {
     Fail(SFK_HIP_BUILD_FAILURE, "radix status array smaller than the sort");
   }
+  unsigned long long* status = b->radix_status.As<unsigned long long>();
+  unsigned int* error_flags = &b->counters.As<DeviceCounters>()->error_flags;
   for (int p = 0; p < passes.count; ++p) {
     const std::uint32_t epoch = NextEpoch(b);
-    EventPair* ev = BeginProfiled(ctx, 2ull * m * (sizeof(K) + sizeof(V)));
-    RadixOnesweep<K, V, kBlock, kItems><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
-        keys, vals, keys_alt, vals_alt, m, passes.shift[p], sc.hist + p * RadixTile::kRadix, sc.status,
-        sc.tile_counter + p, epoch, sc.error_flags, kFlagLookback);
+    const std::uint64_t read = p == 0 ? first_read_bytes : m * (sizeof(K) + sizeof(V));
+    const std::uint64_t bytes = read + m * (sizeof(K) + sizeof(V));
+    EventPair* ev = BeginProfiled(ctx, bytes);
+    if (p == 0) {
+      RadixOnesweep<K, V, kBlock, kItems, Source><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
+          first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status,
+          set.tiles + p, epoch, error_flags, kFlagLookback);
+    } else {
+      RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>>
+          <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
+              ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
+              set.hist + p * RadixTile::kRadix, status, set.tiles + p, epoch, error_flags, kFlagLookback);
+    }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
     std::swap(keys, keys_alt);
     std::swap(vals, vals_alt);
     ctx.stats.sort_passes += 1;
-    ctx.stats.algorithmic_bytes_sa += 2ull * m * (sizeof(K) + sizeof(V));
+    ctx.stats.algorithmic_bytes_sa += bytes;
   }
 }
 
-// Stable LSD sort of m (key, value) pairs over the digits in which `varying`
-// has a set bit.  On return keys/vals point at the sorted buffers.  One read
-// of the keys yields every pass's digit totals; each pass is then one
-// onesweep kernel (rank in LDS, look-back for the global offsets, scatter).
-template <typename K, typename V>
-void SortPairs(BuildContext& ctx, K*& keys, V*& vals, K* keys_alt, V* vals_alt,
-               std::uint64_t m, std::uint64_t varying, int total_bits) {
-  sfk_hip_builder* b = ctx.b;
-  RadixPasses passes{};
-  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) {
-    if (((varying >> shift) & (RadixTile::kRadix - 1)) != 0) passes.shift[passes.count++] = shift;
-  }
+// Stable LSD sort of m (key, value) pairs over `passes`, whose digit counts
+// are already in set.hist.  The first pass reads `first` (the arrays keys/
+// vals, or input generated on the fly); on return keys/vals point at the
+// sorted buffers.  Each pass is one onesweep kernel.
+template <typename K, typename V, typename Source>
+void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys, V*& vals,
+               K* keys_alt, V* vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set) {
   if (passes.count == 0 || m == 0) return;
-  SortScratch sc;
-  sc.hist = b->radix_counts.As<unsigned long long>();
-  sc.tile_counter = reinterpret_cast<unsigned int*>(sc.hist + RadixPasses::kMax * RadixTile::kRadix);
-  sc.status = b->radix_status.As<unsigned long long>();
-  sc.error_flags = &b->counters.As<DeviceCounters>()->error_flags;
-  RadixSortInit<<<1, RadixTile::kRadix, 0, b->stream>>>(sc.hist, sc.tile_counter);
-  constexpr int kHistBlock = 512;
-  const std::uint64_t nvec = m / (16 / sizeof(K));
-  const unsigned hist_grid = static_cast<unsigned>(
-      std::max<std::uint64_t>(1, std::min<std::uint64_t>(512, nvec / (kHistBlock * 8))));
-  RadixHistogramAll<K, kHistBlock><<<hist_grid, kHistBlock, 0, b->stream>>>(keys, m, passes, sc.hist);
-  RadixBucketStarts<<<1, RadixTile::kRadix, 0, b->stream>>>(sc.hist, passes.count);
-  SFK_HIP_CHECK(hipGetLastError());
-  ctx.stats.algorithmic_bytes_sa += m * sizeof(K);
   using Shape = OnesweepShape<K, V>;
   if (m >= Shape::kBigFrom) {
-    OnesweepPasses<K, V, 1024, Shape::kBigItems>(ctx, keys, vals, keys_alt, vals_alt, m, passes, sc);
+    OnesweepPasses<K, V, 1024, Shape::kBigItems>(ctx, first, first_read_bytes, keys, vals, keys_alt,
+                                                 vals_alt, m, passes, set);
   } else {
-    OnesweepPasses<K, V, 512, 8>(ctx, keys, vals, keys_alt, vals_alt, m, passes, sc);
+    OnesweepPasses<K, V, 512, 8>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set);
   }
 }
 
@@ -534,18 +543,38 @@ class Doubling {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
     constexpr std::uint64_t w = sizeof(Index);
 
-    const std::uint64_t all_bits = total_bits >= 64 ? ~0ull : ((1ull << total_bits) - 1);
     const bool rank_step = sec_kind == kSecRank;
-    const unsigned keygen_grid = static_cast<unsigned>(std::min<std::uint64_t>(CeilDiv(m, 256), 2048));
-    // The large list may sit in vals0/vals1 (after a split): Keygen reads
-    // slot k of it before writing slot k of vals0, in the same thread.
-    KeygenKernel<Index, K><<<keygen_grid, 256, 0, b_->stream>>>(
-        l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, dc);
-    SFK_HIP_CHECK(hipGetLastError());
-    ctx_.stats.algorithmic_bytes_sa +=
-        m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
-    if (local_between) LocalRefine<false>();
-    SortPairs<K, Index>(ctx_, keys, vals, keys_alt, vals_alt, m, all_bits, total_bits);
+    const RadixSet set = TakeRadixSet(b_);
+    const RadixPasses passes = AllDigits(total_bits);
+    const unsigned keygen_grid = static_cast<unsigned>(
+        std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 4), 1024)));
+    // The first step of a single text generates its first pass's input from
+    // the text (keys of 16 symbols, values = positions); a counting kernel reads
+    // the text for the digit counts.
+    constexpr bool kNarrowKey = sizeof(K) == 4;
+    if (kNarrowKey && iota && seg_bits == 0 && sec_kind == kSecFillHi) {
+      const unsigned count_grid = static_cast<unsigned>(
+          std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 16), 1024)));
+      TextFillHiCountKernel<<<count_grid, kKeygenBlock, 0, b_->stream>>>(text_, passes, set);
+      SFK_HIP_CHECK(hipGetLastError());
+      const std::uint64_t text_bytes = PackedWords(n_) * 16;
+      ctx_.stats.algorithmic_bytes_sa += text_bytes;
+      if constexpr (kNarrowKey) {
+        SortPairs<K, Index>(ctx_, TextFillHiSource<Index>{text_}, text_bytes, keys, vals, keys_alt, vals_alt, m,
+                            passes, set);
+      }
+    } else {
+      // The large list may sit in vals0/vals1 (after a split): Keygen reads
+      // slot k of it before writing slot k of vals0, in the same thread.
+      KeygenKernel<Index, K><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
+          l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+      SFK_HIP_CHECK(hipGetLastError());
+      ctx_.stats.algorithmic_bytes_sa +=
+          m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
+      if (local_between) LocalRefine<false>();
+      SortPairs<K, Index>(ctx_, ArraySource<K, Index>{keys, vals}, m * (sizeof(K) + w), keys, vals, keys_alt,
+                          vals_alt, m, passes, set);
+    }
 
     // Boundaries, singletons, compaction.
     const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
@@ -675,7 +704,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   b->seg_delta0.Reserve(seg_cap * sizeof(Index));
   b->seg_delta1.Reserve(seg_cap * sizeof(Index));
   b->seg_start.Reserve(seg_cap * sizeof(Index));
-  b->radix_counts.Reserve(static_cast<std::uint64_t>(RadixTile::kRadix) * RadixPasses::kMax * 8 + 64);
+  b->radix_counts.Reserve(2 * kRadixSetStride);
   ReserveOnesweep(b, n);
   b->partial.Reserve(1024 * sizeof(RefinePartial));
 
@@ -684,6 +713,9 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   PackedText text{b->packed.As<uint4>(), n};
 
   ResetCounters(b);
+  // Both radix count sets start cleared (each sort then clears the other).
+  SFK_HIP_CHECK(hipMemsetAsync(b->radix_counts.As<void>(), 0, 2 * kRadixSetStride, b->stream));
+  b->radix_set = 0;
   SFK_HIP_CHECK(hipEventRecord(b->phase[0], b->stream));
   PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
       d_text, n, words, b->packed.As<uint4>(),

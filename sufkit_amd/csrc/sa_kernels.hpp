@@ -36,6 +36,7 @@
 
 #include "block_scan.hpp"
 #include "device_common.hpp"
+#include "radix_sort.hpp"
 
 namespace sfk {
 
@@ -82,6 +83,7 @@ constexpr int kLocalSpan = kLocalThreads * kLocalItems;
 constexpr std::uint32_t kLocalMaxSegment = 256;
 constexpr int kLocalOwn = kLocalSpan - static_cast<int>(kLocalMaxSegment);
 constexpr int kLargeAllocShift = 38;  // l_alloc packing: suffixes < 2^38
+constexpr int kKeygenBlock = 512;
 constexpr std::uint32_t kSmallSegment = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------ K0 pack -----
@@ -202,46 +204,116 @@ __global__ __launch_bounds__(256) void LocalizeKernel(
 
 // ------------------------------------------------------------ keys --------
 
+// Secondary key of suffix v for a refine step (see SecondaryKind).
+template <typename Index>
+__device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const Index* __restrict__ rank,
+                                             std::uint64_t v, std::uint64_t h, DeviceCounters* counters) {
+  if (sec_kind == kSecRank) {
+    // An unsorted suffix never reaches the sentinel within h symbols, so
+    // v + h < n; the guard only protects against a corrupted state.
+    const std::uint64_t p = v + h;
+    if (p < t.n) return static_cast<std::uint64_t>(rank[p]);
+    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
+    return 0;
+  }
+  std::uint64_t c;
+  std::uint32_t s;
+  LoadWindow(t, v, c, s);
+  if (sec_kind == kSecFillHi) return FillKey(c, s) >> 32;
+  if (sec_kind == kSecFillLo) return FillKey(c, s) & 0xFFFFFFFFull;
+  return ExactKeyWord(c, s, sec_kind - kSecExact0);
+}
+
+// Sort keys of the large list, key = (segment id << sec_bits) | secondary,
+// and the digit counts of every radix pass over them.
 // act_idx may be the same buffer as vals: slot k is read before slot k is
 // written, by the same thread (hence no __restrict__ on either).
 template <typename Index, typename K>
-__global__ __launch_bounds__(256) void KeygenKernel(
+__global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
     const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
     const Index* __restrict__ rank, std::uint64_t h, bool iota,
-    K* __restrict__ keys, Index* vals, DeviceCounters* counters) {
-  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
-  for (std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < m;
-       k += stride) {
-    const Index v = iota ? static_cast<Index>(k) : act_idx[k];
-    std::uint64_t sec;
-    if (sec_kind == kSecRank) {
-      // An unsorted suffix never reaches the sentinel within h symbols, so
-      // v + h < n; the guard only protects against a corrupted state.
-      const std::uint64_t p = static_cast<std::uint64_t>(v) + h;
-      if (p < t.n) {
-        sec = static_cast<std::uint64_t>(rank[p]);
-      } else {
-        sec = 0;
-        atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
-      }
-    } else {
-      std::uint64_t c;
-      std::uint32_t s;
-      LoadWindow(t, v, c, s);
-      if (sec_kind == kSecFillHi) {
-        sec = FillKey(c, s) >> 32;
-      } else if (sec_kind == kSecFillLo) {
-        sec = FillKey(c, s) & 0xFFFFFFFFull;
-      } else {
-        sec = ExactKeyWord(c, s, sec_kind - kSecExact0);
-      }
-    }
-    const std::uint64_t seg = seg_bits ? static_cast<std::uint64_t>(act_seg[k]) : 0ull;
-    keys[k] = static_cast<K>((seg << sec_bits) | sec);
-    vals[k] = v;
+    K* __restrict__ keys, Index* vals, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
+  constexpr int kWaves = kKeygenBlock / kWave;
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  ClearNextRadixSet(set);
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+    (&local[0][0][0])[i] = 0;
   }
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kKeygenBlock;
+  for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWave; wbase < m;
+       wbase += stride) {
+    const std::uint64_t k = wbase + LaneId();
+    const bool valid = k < m;
+    K key = 0;
+    if (valid) {
+      const Index v = iota ? static_cast<Index>(k) : act_idx[k];
+      const std::uint64_t sec = SecondaryKey<Index>(sec_kind, t, rank, static_cast<std::uint64_t>(v), h, counters);
+      const std::uint64_t seg = seg_bits ? static_cast<std::uint64_t>(act_seg[k]) : 0ull;
+      key = static_cast<K>((seg << sec_bits) | sec);
+      keys[k] = key;
+      vals[k] = v;
+    }
+    CountDigits<K>(local[wave], passes, key, valid, wbase + kWave <= m);
+  }
+  __syncthreads();
+  FlushDigits<kWaves>(local, passes, set.hist);
 }
+
+// Digit counts of the first step's keys (high half of FillKey of every
+// suffix of a single text), four consecutive positions per lane from one pair
+// of 16-byte entries (four positions never straddle a 32-symbol entry).
+__global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText t, RadixPasses passes,
+                                                                      RadixSet set) {
+  constexpr int kWaves = kKeygenBlock / kWave;
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  ClearNextRadixSet(set);
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+    (&local[0][0][0])[i] = 0;
+  }
+  __syncthreads();
+  const std::uint64_t m = t.n;
+  const std::uint64_t groups = (m + 3) / 4;
+  const int wave = threadIdx.x / kWave;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kKeygenBlock;
+  for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWave; wbase < groups;
+       wbase += stride) {
+    const std::uint64_t g = wbase + LaneId();
+    const bool full = (wbase + kWave) * 4 <= m;
+    uint4 e0 = make_uint4(0, 0, 0, 0), e1 = e0;
+    if (g < groups) {
+      e0 = t.words[g >> 3];
+      e1 = t.words[(g >> 3) + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const std::uint64_t p = g * 4 + j;
+      std::uint64_t c;
+      std::uint32_t sp;
+      WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
+      const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
+      CountDigits<std::uint32_t>(local[wave], passes, key, p < m, full);
+    }
+  }
+  __syncthreads();
+  FlushDigits<kWaves>(local, passes, set.hist);
+}
+
+// First pass of the first step: key = high half of FillKey of suffix i,
+// value = i, generated from the packed text instead of read.
+template <typename Index>
+struct TextFillHiSource {
+  PackedText t;
+  __device__ void Load(std::uint64_t i, std::uint32_t& key, Index& val) const {
+    std::uint64_t c;
+    std::uint32_t s;
+    LoadWindow(t, i, c, s);
+    key = static_cast<std::uint32_t>(FillKey(c, s) >> 32);
+    val = static_cast<Index>(i);
+  }
+};
 
 // Counts active suffixes whose 32-symbol window holds a special symbol, over
 // the list RefineApply just wrote (its size is read from the counters; the
