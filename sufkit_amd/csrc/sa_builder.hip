@@ -308,7 +308,8 @@ void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_
 // allocation).
 __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
   c->q_next = 0;
-  c->l_alloc = 0;
+  c->l_next = 0;
+  c->l_next_segs = 0;
 }
 
 template <typename Index>
@@ -319,9 +320,6 @@ class Doubling {
 
   void Run() {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
-    if (n_ >= (1ull << kLargeAllocShift)) {
-      Fail(SFK_HIP_UNSUPPORTED_BACKEND, "texts of 2^38 symbols or more are not supported");
-    }
     lm_ = n_;
     l_segs_ = ctx_.refs > 0 ? ctx_.refs : 1;
     l_idx_ = b_->act_idx.As<Index>();
@@ -397,9 +395,28 @@ class Doubling {
   }
 
  private:
-  // Small lists hold at most the survivors of the first step.
-  void ReserveSmallLists(std::uint64_t cap) {
-    const std::uint64_t bytes = std::max<std::uint64_t>(cap, 1) * sizeof(Index);
+  // Small lists hold at most the survivors of the first step, and no more
+  // than free device memory allows (segments that do not fit stay in the
+  // large list).
+  void ReserveSmallLists(std::uint64_t want) {
+    constexpr std::uint64_t kArrays = 5;
+    const std::uint64_t held = b_->q_idx0.Bytes() / sizeof(Index);
+    std::uint64_t cap = std::max<std::uint64_t>(want, 1);
+    if (cap > held) {
+      std::size_t free_bytes = 0, total_bytes = 0;
+      SFK_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
+      const std::uint64_t margin = 2ull << 30;
+      const std::uint64_t spare = free_bytes > margin ? free_bytes - margin : 0;
+      const std::uint64_t affordable = held + spare / (kArrays * sizeof(Index));
+      // Reallocating gigabytes costs more than it saves unless the lists then
+      // hold everything: a partial growth keeps what is held.
+      if (affordable < cap && held > 0) {
+        cap = held;
+      } else {
+        cap = std::max<std::uint64_t>(std::min(cap, affordable), 1);
+      }
+    }
+    const std::uint64_t bytes = cap * sizeof(Index);
     b_->q_idx0.Reserve(bytes);
     b_->q_idx1.Reserve(bytes);
     b_->q_row0_0.Reserve(bytes);
@@ -454,8 +471,8 @@ class Doubling {
     if (pending_apply_) ApplySizes(c);
     qm_ = c.q_next;
     if (split) {
-      lm_ = c.l_alloc & ((1ull << kLargeAllocShift) - 1);
-      l_segs_ = c.l_alloc >> kLargeAllocShift;
+      lm_ = c.l_next;
+      l_segs_ = c.l_next_segs;
     } else {
       lm_ = 0;
       l_segs_ = 0;
@@ -478,8 +495,13 @@ class Doubling {
     // (a list of m suffixes has at most m / 2 segments).
     const std::uint64_t m_bound = std::max<std::uint64_t>(lm_bound_, 1);
     const std::uint64_t seg_bound = pending_apply_ ? m_bound / 2 + 1 : std::max<std::uint64_t>(l_segs_, 1);
-    SplitSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(seg_bound, 256 * kSplitItems)), 256, 0, b_->stream>>>(
-        b_->seg_start.As<Index>(), l_delta_, kLocalMaxSegment, dest, new_id, large_delta, dc);
+    const SortGeometry g = Geometry(seg_bound, kSplitTile, 1024);
+    SplitPartial* partial = b_->partial.As<SplitPartial>();
+    SplitCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), kLocalMaxSegment, q_cap_,
+                                                            g.chunk, partial, dc);
+    SplitSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
+    SplitAssignKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), l_delta_, kLocalMaxSegment,
+                                                             g.chunk, partial, dest, new_id, large_delta, dc);
     SFK_HIP_CHECK(hipGetLastError());
     Index* l_idx_next = b_->vals0.As<Index>();
     std::uint32_t* l_seg_next = b_->vals1.As<std::uint32_t>();
@@ -706,7 +728,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   b->seg_start.Reserve(seg_cap * sizeof(Index));
   b->radix_counts.Reserve(2 * kRadixSetStride);
   ReserveOnesweep(b, n);
-  b->partial.Reserve(1024 * sizeof(RefinePartial));
+  b->partial.Reserve(1024 * std::max(sizeof(RefinePartial), sizeof(SplitPartial)));
 
   Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
   Index* rank = isa_direct ? static_cast<Index*>(d_isa) : b->rank.As<Index>();

@@ -17,9 +17,10 @@
 //                   and rank (= their final ISA entry), survivors compacted
 //                   into a new list with renumbered segments, ranks of
 //                   survivors set to the SA row where their group starts
-//     SplitSegments/SplitMove
+//     SplitCount/SplitSpine/SplitAssign/SplitMove
 //                   segments of at most kLocalMaxSegment suffixes move to the
-//                   small list, the others are renumbered in slot order
+//                   small list (while it has room), the others are renumbered
+//                   in slot order
 //   small list  segments of at most kLocalMaxSegment suffixes; per suffix its
 //               text position and the first SA row of its segment.  One
 //               LocalRefine kernel ranks every suffix inside its segment in
@@ -68,7 +69,8 @@ struct DeviceCounters {
   unsigned long long long_words;
   unsigned long long zero_count;  // sentinels seen by the batch packer
   unsigned long long q_next;      // suffixes appended to the next small list
-  unsigned long long l_alloc;     // next large list: (segments << 38) | suffixes
+  unsigned long long l_next;      // suffixes in the next large list
+  unsigned long long l_next_segs; // segments in the next large list
   unsigned int error_flags;
   unsigned int pad;
 };
@@ -82,7 +84,6 @@ constexpr int kLocalItems = 2;
 constexpr int kLocalSpan = kLocalThreads * kLocalItems;
 constexpr std::uint32_t kLocalMaxSegment = 256;
 constexpr int kLocalOwn = kLocalSpan - static_cast<int>(kLocalMaxSegment);
-constexpr int kLargeAllocShift = 38;  // l_alloc packing: suffixes < 2^38
 constexpr int kKeygenBlock = 512;
 constexpr std::uint32_t kSmallSegment = 0xFFFFFFFFu;
 
@@ -533,64 +534,158 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
 
 // ------------------------------------------------------------ split -----
 
-// kSplitItems consecutive segments per thread of the list RefineApply just
-// wrote (its segment and suffix counts are read from the counters; the grid
-// covers an upper bound; first slots in seg_start).  A segment of
-// at most small_max suffixes reserves its slots in the next small list (one
-// atomic per block); a larger one takes its new id and first slot in the
-// next large list from one 64-bit atomic, so ids grow with slots, and gets
-// its new delta.
+// Split of the list RefineApply just wrote (its segment and suffix counts are
+// read from the counters; first slots in seg_start) into the next small list
+// and the next large list.  Block b handles segments [b * chunk, (b + 1) *
+// chunk), kSplitItems consecutive segments per thread per tile.  The small
+// list takes a block's segments of at most small_max suffixes when all of
+// them fit under q_cap (one bounded reservation per block, so the small list
+// has no holes); every other segment goes to the large list, whose ids and
+// slots follow segment order (prefix sums over blocks).
 constexpr int kSplitItems = 8;
+constexpr int kSplitTile = 256 * kSplitItems;
+
+struct SplitPartial {
+  unsigned long long q_base;       // small-list slot of the block's first small segment
+  unsigned long long fits;         // the block's small segments went to the small list
+  unsigned long long large_segs;   // large-list segments (exclusive prefix after the spine)
+  unsigned long long large_elems;  // large-list suffixes (exclusive prefix after the spine)
+};
 
 template <typename Index>
-__global__ __launch_bounds__(256) void SplitSegmentsKernel(
-    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
-    std::uint32_t small_max, Index* __restrict__ dest,
-    std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, DeviceCounters* counters) {
+__device__ inline std::uint64_t SegmentSize(const Index* __restrict__ seg_start, std::uint64_t s,
+                                            std::uint64_t segs, std::uint64_t m) {
+  const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
+  const std::uint64_t end = s + 1 < segs ? static_cast<std::uint64_t>(seg_start[s + 1]) : m;
+  return end - start;
+}
+
+template <typename Index>
+__global__ __launch_bounds__(256) void SplitCountKernel(
+    const Index* __restrict__ seg_start, std::uint32_t small_max, std::uint64_t q_cap,
+    std::uint64_t chunk, SplitPartial* __restrict__ partial, DeviceCounters* counters) {
   constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t segs = counters->segments;
   const std::uint64_t m = counters->survivors;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  __shared__ std::uint64_t s_base;
-  const std::uint64_t first = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) * kSplitItems;
-  std::uint64_t small_total = 0;
-#pragma unroll
-  for (int i = 0; i < kSplitItems; ++i) {
-    const std::uint64_t s = first + i;
-    if (s < segs) {
-      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
-      const std::uint64_t end = s + 1 < segs ? static_cast<std::uint64_t>(seg_start[s + 1]) : m;
-      if (end - start <= small_max) small_total += end - start;
-    }
-  }
-  std::uint64_t total;
-  std::uint64_t off = BlockExclusiveSum<kBlock>(small_total, scratch, total);
-  if (threadIdx.x == 0) {
-    s_base = total ? static_cast<std::uint64_t>(atomicAdd(&counters->q_next, static_cast<unsigned long long>(total))) : 0ull;
-  }
-  __syncthreads();
-  off += s_base;
-#pragma unroll
-  for (int i = 0; i < kSplitItems; ++i) {
-    const std::uint64_t s = first + i;
-    if (s >= segs) break;
-    const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
-    const std::uint64_t end = s + 1 < segs ? static_cast<std::uint64_t>(seg_start[s + 1]) : m;
-    const std::uint64_t size = end - start;
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
+  std::uint64_t small_elems = 0, small_segs = 0, large_segs = 0, large_elems = 0;
+  for (std::uint64_t s = begin + threadIdx.x; s < end; s += kBlock) {
+    const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
     if (size <= small_max) {
-      dest[s] = static_cast<Index>(off);
-      new_id[s] = kSmallSegment;
-      off += size;
+      small_elems += size;
+      ++small_segs;
     } else {
-      const Index row0 = static_cast<Index>(seg_delta[s] + static_cast<Index>(start));
-      const unsigned long long got = atomicAdd(
-          &counters->l_alloc, (1ull << kLargeAllocShift) | static_cast<unsigned long long>(size));
-      const std::uint64_t id = got >> kLargeAllocShift;
-      const std::uint64_t pos = got & ((1ull << kLargeAllocShift) - 1);
-      dest[s] = static_cast<Index>(pos);
-      new_id[s] = static_cast<std::uint32_t>(id);
-      large_delta[id] = static_cast<Index>(row0 - static_cast<Index>(pos));
+      large_elems += size;
+      ++large_segs;
     }
+  }
+  std::uint64_t t_small, t_small_segs, t_large_segs, t_large_elems;
+  BlockExclusiveSum<kBlock>(small_elems, scratch, t_small);
+  BlockExclusiveSum<kBlock>(small_segs, scratch, t_small_segs);
+  BlockExclusiveSum<kBlock>(large_segs, scratch, t_large_segs);
+  BlockExclusiveSum<kBlock>(large_elems, scratch, t_large_elems);
+  if (threadIdx.x == 0) {
+    // Bounded reservation: every failed exchange means another block's
+    // reservation succeeded (q_next only grows, up to q_cap), so this ends.
+    int fits = 0;
+    unsigned long long cur = t_small ? atomicAdd(&counters->q_next, 0ull) : 0ull;
+    while (t_small) {
+      if (cur + t_small > q_cap) break;
+      const unsigned long long prev = atomicCAS(&counters->q_next, cur, cur + t_small);
+      if (prev == cur) {
+        fits = 1;
+        break;
+      }
+      cur = prev;
+    }
+    SplitPartial out;
+    out.q_base = cur;
+    out.fits = static_cast<unsigned long long>(fits);
+    out.large_segs = t_large_segs + (fits ? 0 : t_small_segs);
+    out.large_elems = t_large_elems + (fits ? 0 : t_small);
+    partial[blockIdx.x] = out;
+  }
+}
+
+// Exclusive scans of the large-list counts over at most 1024 blocks.
+__global__ __launch_bounds__(1024) void SplitSpineKernel(SplitPartial* partial, int grid,
+                                                         DeviceCounters* counters) {
+  constexpr int kBlock = 1024;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  const std::uint64_t segs_in = valid ? partial[threadIdx.x].large_segs : 0ull;
+  const std::uint64_t elems_in = valid ? partial[threadIdx.x].large_elems : 0ull;
+  std::uint64_t tot_segs, tot_elems;
+  const std::uint64_t segs = BlockExclusiveSum<kBlock>(segs_in, scratch, tot_segs);
+  const std::uint64_t elems = BlockExclusiveSum<kBlock>(elems_in, scratch, tot_elems);
+  if (valid) {
+    partial[threadIdx.x].large_segs = segs;
+    partial[threadIdx.x].large_elems = elems;
+  }
+  if (threadIdx.x == 0) {
+    counters->l_next = tot_elems;
+    counters->l_next_segs = tot_segs;
+  }
+}
+
+template <typename Index>
+__global__ __launch_bounds__(256) void SplitAssignKernel(
+    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta, std::uint32_t small_max,
+    std::uint64_t chunk, const SplitPartial* __restrict__ partial, Index* __restrict__ dest,
+    std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, const DeviceCounters* counters) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t segs = counters->segments;
+  const std::uint64_t m = counters->survivors;
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
+  const SplitPartial part = partial[blockIdx.x];
+  const bool fits = part.fits != 0;
+  std::uint64_t run_q = part.q_base, run_lsegs = part.large_segs, run_lelems = part.large_elems;
+  for (std::uint64_t base = begin; base < end; base += kSplitTile) {
+    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kSplitItems;
+    std::uint64_t q_elems = 0, l_segs = 0, l_elems = 0;
+#pragma unroll
+    for (int i = 0; i < kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      if (s < end) {
+        const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+        if (fits && size <= small_max) {
+          q_elems += size;
+        } else {
+          l_elems += size;
+          ++l_segs;
+        }
+      }
+    }
+    std::uint64_t t_q, t_lsegs, t_lelems;
+    std::uint64_t q_off = run_q + BlockExclusiveSum<kBlock>(q_elems, scratch, t_q);
+    std::uint64_t lseg_off = run_lsegs + BlockExclusiveSum<kBlock>(l_segs, scratch, t_lsegs);
+    std::uint64_t lelem_off = run_lelems + BlockExclusiveSum<kBlock>(l_elems, scratch, t_lelems);
+#pragma unroll
+    for (int i = 0; i < kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      if (s >= end) break;
+      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
+      const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+      if (fits && size <= small_max) {
+        dest[s] = static_cast<Index>(q_off);
+        new_id[s] = kSmallSegment;
+        q_off += size;
+      } else {
+        const Index row0 = static_cast<Index>(seg_delta[s] + static_cast<Index>(start));
+        dest[s] = static_cast<Index>(lelem_off);
+        new_id[s] = static_cast<std::uint32_t>(lseg_off);
+        large_delta[lseg_off] = static_cast<Index>(row0 - static_cast<Index>(lelem_off));
+        lelem_off += size;
+        ++lseg_off;
+      }
+    }
+    run_q += t_q;
+    run_lsegs += t_lsegs;
+    run_lelems += t_lelems;
   }
 }
 
