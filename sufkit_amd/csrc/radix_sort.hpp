@@ -2,16 +2,14 @@
 //
 // The kernel that writes the keys also counts every pass's digits
 // (CountDigits / FlushDigits); then one onesweep kernel per digit pass: each
-// block turns the pass's digit counts into bucket starts, takes the next tile
-// id from an atomic
-// counter, ranks its tile stably in LDS with wave ballots (64-lane peer
-// masks), publishes its per-digit counts, finds the counts of all earlier
-// tiles by looking back over their published status words, and writes the
-// tile out in digit order so that consecutive lanes store consecutive
-// addresses of one bucket.  HBM traffic per element and pass: one read and
-// one write of key and value (the first pass may generate its input instead
-// of reading it, see Source).
-// Digits whose bits are constant over all keys are skipped by the caller.
+// block ranks its tile (tile = block index) stably in LDS with wave ballots
+// (64-lane peer masks), publishes its per-digit counts, finds the counts of
+// all earlier tiles by looking back over their published status words (tile
+// 0 adds the pass's bucket starts, the exclusive scan of its digit counts),
+// and writes the tile out in digit order so that consecutive lanes store
+// consecutive addresses of one bucket.  HBM traffic per element and pass: one
+// read and one write of key and value (the first pass may generate its input
+// instead of reading it, see Source).
 #pragma once
 
 #include "block_scan.hpp"
@@ -27,8 +25,8 @@ struct RadixTile {
 
 // ------------------------------------------------------------- onesweep ---
 //
-// A tile waits only on tiles with smaller ids, which took their ids (and so
-// were running) before it: the look-back always ends.
+// A tile waits only on tiles with smaller block indices, which were
+// dispatched before it: the look-back always ends.
 //
 // Status word per (tile, digit), 64 bits: [63:62] flag (1 = this tile's count,
 // 2 = inclusive prefix through this tile), [61:48] epoch of the pass, [47:0]
@@ -44,6 +42,129 @@ constexpr std::uint64_t kCountMask = (1ull << 48) - 1;
 constexpr std::uint32_t kEpochMask = (1u << 14) - 1;
 }  // namespace onesweep
 
+// Decoupled look-back of one value over tiles in block order (tile =
+// blockIdx.x; workgroups are dispatched in increasing order, so every tile a
+// tile waits on is resident or finished): the status word of tile t is
+// status[t * stride].  Publishes this tile's value,
+// walks back to the nearest tile whose inclusive value is published, then
+// publishes its own inclusive value; returns the exclusive prefix.  Tile 0's
+// prefix is `first`.  kMax: values combine by max, else by sum.  The wait is
+// bounded: past kSpinLimit polls of one word the tile raises error_bit and
+// stops waiting, and every waiting tile stops once the bit is up, so a broken
+// protocol ends the kernel with an error instead of hanging it.  Status words
+// are self-contained, so their agent-scope atomics are relaxed.
+template <bool kMax>
+__device__ inline std::uint64_t LookBack(unsigned long long* status, std::uint64_t stride, std::uint64_t tile,
+                                         std::uint64_t value, std::uint64_t first, std::uint32_t epoch,
+                                         unsigned int* error_flags, std::uint32_t error_bit) {
+  constexpr std::uint32_t kSpinLimit = 1u << 20;
+  const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
+  auto combine = [](std::uint64_t a, std::uint64_t b) { return kMax ? (a > b ? a : b) : a + b; };
+  unsigned long long* slot = status + tile * stride;
+  if (tile == 0) {
+    __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | combine(first, value), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return first;
+  }
+  __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  std::uint64_t prefix = 0;
+  std::int64_t t = static_cast<std::int64_t>(tile) - 1;
+  std::uint32_t spins = 0;
+  while (true) {
+    const std::uint64_t word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * stride,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const std::uint64_t flag = word & (3ull << 62);
+    if (flag == 0 || ((word >> 48) & onesweep::kEpochMask) != (epoch & onesweep::kEpochMask)) {
+      // tile t has not published yet in this epoch
+      if (++spins >= kSpinLimit) {
+        atomicOr(error_flags, error_bit);
+        break;
+      }
+      if ((spins & 63u) == 0 &&
+          (__hip_atomic_load(error_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & error_bit)) {
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    spins = 0;
+    prefix = combine(prefix, word & onesweep::kCountMask);
+    if (flag == onesweep::kFlagInclusive) break;
+    --t;
+  }
+  __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | combine(prefix, value), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
+// The same look-back run by a whole wave (every lane calls it with the same
+// arguments): each step reads the status words of 64 predecessors at once,
+// finds the nearest inclusive one with a ballot and reduces the values up to
+// it across the wave, so a tile behind many unfinished tiles moves back 64
+// tiles per round trip instead of one.
+template <bool kMax>
+__device__ __forceinline__ std::uint64_t LookBackWave(unsigned long long* status, std::uint64_t stride, std::uint64_t tile,
+                                             std::uint64_t value, std::uint64_t first, std::uint32_t epoch,
+                                             unsigned int* error_flags, std::uint32_t error_bit) {
+  constexpr std::uint32_t kSpinLimit = 1u << 20;
+  const std::uint32_t lane = LaneId();
+  const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
+  auto combine = [](std::uint64_t a, std::uint64_t b) { return kMax ? (a > b ? a : b) : a + b; };
+  unsigned long long* slot = status + tile * stride;
+  if (tile == 0) {
+    if (lane == 0) {
+      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | combine(first, value), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return first;
+  }
+  if (lane == 0) {
+    __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  std::uint64_t prefix = 0;
+  std::int64_t hi = static_cast<std::int64_t>(tile) - 1;  // window: tiles hi, hi - 1, ..., hi - 63
+  std::uint32_t spins = 0;
+  while (true) {
+    const std::int64_t t = hi - static_cast<std::int64_t>(lane);
+    std::uint64_t word = 0;
+    if (t >= 0) {
+      word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * stride, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const std::uint64_t flag = word & (3ull << 62);
+    const bool ready = t >= 0 && flag != 0 &&
+                       ((word >> 48) & onesweep::kEpochMask) == (epoch & onesweep::kEpochMask);
+    const std::uint64_t inclusive = __ballot(ready && flag == onesweep::kFlagInclusive);
+    const std::uint64_t not_ready = __ballot(t >= 0 && !ready);
+    // Lanes up to the nearest inclusive tile (or the whole window) must be ready.
+    const int last = inclusive ? __ffsll(static_cast<unsigned long long>(inclusive)) - 1 : kWave - 1;
+    const std::uint64_t needed = last == kWave - 1 ? ~0ull : ((2ull << last) - 1);
+    if (not_ready & needed) {  // wave-uniform, as are all the exits below
+      if (++spins >= kSpinLimit) {
+        if (lane == 0) atomicOr(error_flags, error_bit);
+        break;
+      }
+      const bool given_up = (spins & 63u) == 0 &&
+          (__hip_atomic_load(error_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & error_bit) != 0;
+      if (__ballot(given_up)) break;
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    spins = 0;
+    std::uint64_t v = (t >= 0 && static_cast<int>(lane) <= last) ? (word & onesweep::kCountMask) : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = combine(v, WaveShfl(v, static_cast<int>(lane ^ d)));
+    prefix = combine(prefix, v);
+    if (inclusive) break;
+    hi -= kWave;
+  }
+  if (lane == 0) {
+    __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | combine(prefix, value), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return prefix;
+}
+
 // Digit passes of one sort: the bit offset of each 8-bit digit that varies.
 struct RadixPasses {
   static constexpr int kMax = 8;
@@ -57,19 +178,20 @@ struct RadixPasses {
 // the same stream).
 struct RadixSet {
   unsigned long long* hist;   // [RadixPasses::kMax][kRadix]
-  unsigned int* tiles;        // [RadixPasses::kMax]
+  unsigned int* tiles;        // [RadixPasses::kMax + 1]
   unsigned long long* next_hist;
   unsigned int* next_tiles;
 };
 
+constexpr int kRadixSetTickets = RadixPasses::kMax + 1;
 constexpr std::uint64_t kRadixSetBytes =
-    static_cast<std::uint64_t>(RadixPasses::kMax) * RadixTile::kRadix * 8 + RadixPasses::kMax * 4;
+    static_cast<std::uint64_t>(RadixPasses::kMax) * RadixTile::kRadix * 8 + kRadixSetTickets * 4;
 
 // Clears the other set (block 0 only; call before any block's __syncthreads).
 __device__ inline void ClearNextRadixSet(const RadixSet& set) {
   if (blockIdx.x != 0) return;
   for (int i = threadIdx.x; i < RadixPasses::kMax * RadixTile::kRadix; i += blockDim.x) set.next_hist[i] = 0;
-  if (threadIdx.x < RadixPasses::kMax) set.next_tiles[threadIdx.x] = 0;
+  if (threadIdx.x < kRadixSetTickets) set.next_tiles[threadIdx.x] = 0;
 }
 
 // Adds one key per lane to this wave's LDS digit counts (local[pass][digit]).
@@ -126,7 +248,7 @@ template <typename K, typename V, int kBlock, int kItems, typename Source>
 __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
     const unsigned long long* __restrict__ digit_count, unsigned long long* status,
-    unsigned int* tile_counter, std::uint32_t epoch, unsigned int* error_flags,
+    std::uint32_t epoch, unsigned int* error_flags,
     std::uint32_t error_bit) {
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
@@ -138,16 +260,14 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint64_t bucket_start[kRadix];
   __shared__ std::uint32_t scan_scratch[kBlock / kWave];
   __shared__ std::uint64_t scan64[kBlock / kWave];
-  __shared__ unsigned int s_tile;
   __shared__ K s_keys[kTile];
   __shared__ V s_vals[kTile];
 
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;
   __syncthreads();
-  const std::uint64_t tile = s_tile;
+  const std::uint64_t tile = blockIdx.x;
   if (tile == 0) {
     // Bucket starts of this pass (exclusive scan of its digit counts) enter
     // the look-back through tile 0's inclusive values.
@@ -201,49 +321,8 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     }
     my_count = s;
     // Publish this tile's count, then look back for the exclusive prefix.
-    const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
-    unsigned long long* slot = status + tile * kRadix + d;
-    if (tile == 0) {
-      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (bucket_start[d] + s), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      global_base[d] = bucket_start[d];
-    } else {
-      __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | s, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      // The wait is bounded: past kSpinLimit polls of one word the tile
-      // raises `error_bit` and stops waiting, and every waiting tile stops
-      // once the bit is up, so a broken protocol ends the kernel with an
-      // error instead of hanging it.
-      constexpr std::uint32_t kSpinLimit = 1u << 20;
-      std::uint64_t prefix = 0;
-      std::int64_t t = static_cast<std::int64_t>(tile) - 1;
-      std::uint32_t spins = 0;
-      while (true) {
-        const std::uint64_t word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * kRadix + d,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const std::uint64_t flag = word & (3ull << 62);
-        if (flag == 0 || ((word >> 48) & onesweep::kEpochMask) != (epoch & onesweep::kEpochMask)) {
-          // tile t has not published yet in this pass
-          if (++spins >= kSpinLimit) {
-            atomicOr(error_flags, error_bit);
-            break;
-          }
-          if ((spins & 63u) == 0 &&
-              (__hip_atomic_load(error_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & error_bit)) {
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        spins = 0;
-        prefix += word & onesweep::kCountMask;
-        if (flag == onesweep::kFlagInclusive) break;
-        --t;
-      }
-      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (prefix + s), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      global_base[d] = prefix;
-    }
+    global_base[d] = LookBack<false>(status + d, kRadix, tile, s, tile == 0 ? bucket_start[d] : 0, epoch,
+                                     error_flags, error_bit);
   }
   __syncthreads();
   {

@@ -127,7 +127,6 @@ struct sfk_hip_builder {
 namespace sfk {
 namespace {
 
-constexpr int kRefineItems = 8;
 constexpr int kScanItems = 8;
 
 struct BuildContext {
@@ -271,12 +270,12 @@ void OnesweepPasses(BuildContext& ctx, const Source& first, std::uint64_t first_
     if (p == 0) {
       RadixOnesweep<K, V, kBlock, kItems, Source><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
           first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status,
-          set.tiles + p, epoch, error_flags, kFlagLookback);
+          epoch, error_flags, kFlagLookback);
     } else {
       RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
-              set.hist + p * RadixTile::kRadix, status, set.tiles + p, epoch, error_flags, kFlagLookback);
+              set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback);
     }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
@@ -491,15 +490,15 @@ class Doubling {
     Index* dest = b_->keys0.As<Index>();
     std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
     Index* large_delta = OtherDelta(l_delta_);
-    // Sizes come from the counters RefineSpine wrote; grids cover bounds
+    // Sizes come from the counters RefineApply wrote; grids cover bounds
     // (a list of m suffixes has at most m / 2 segments).
     const std::uint64_t m_bound = std::max<std::uint64_t>(lm_bound_, 1);
     const std::uint64_t seg_bound = pending_apply_ ? m_bound / 2 + 1 : std::max<std::uint64_t>(l_segs_, 1);
     const SortGeometry g = Geometry(seg_bound, kSplitTile, 1024);
     SplitPartial* partial = b_->partial.As<SplitPartial>();
-    SplitCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), kLocalMaxSegment, q_cap_,
+    SplitCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), kLocalMaxSegment,
                                                             g.chunk, partial, dc);
-    SplitSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
+    SplitSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, q_cap_, dc);
     SplitAssignKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), l_delta_, kLocalMaxSegment,
                                                              g.chunk, partial, dest, new_id, large_delta, dc);
     SFK_HIP_CHECK(hipGetLastError());
@@ -598,16 +597,27 @@ class Doubling {
                           vals_alt, m, passes, set);
     }
 
-    // Boundaries, singletons, compaction.
-    const SortGeometry g = Geometry(m, 256 * kRefineItems, 1024);
-    RefinePartial* partial = b_->partial.As<RefinePartial>();
+    // Boundaries, singletons, compaction: reduce / spine / apply for a long
+    // one-segment list, one look-back kernel otherwise.
     Index* delta_next = OtherDelta(l_delta_);
-    RefineReduceKernel<K><<<g.grid, 256, 0, b_->stream>>>(keys, m, g.chunk, partial);
-    RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
-    RefineApplyKernel<Index, K, kRefineItems><<<g.grid, 256, 0, b_->stream>>>(
-        keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_,
-        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), delta_next,
-        b_->seg_start.As<Index>(), survivor_rank);
+    const std::uint64_t tiles = CeilDiv(m, kRefineTile);
+    if (seg_bits > 0 || tiles <= static_cast<std::uint64_t>(kRefineResidentTiles)) {
+      if ((tiles + 1) * 3 * 8 > b_->radix_status.Bytes()) {
+        Fail(SFK_HIP_BUILD_FAILURE, "look-back status array smaller than the list");
+      }
+      RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
+          keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
+          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), dc);
+    } else {
+      const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
+      RefinePartial* partial = b_->partial.As<RefinePartial>();
+      RefineReduceKernel<K><<<g.grid, kRefineChunkThreads, 0, b_->stream>>>(keys, m, g.chunk, partial);
+      RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
+      RefineApplyChunkedKernel<Index, K><<<g.grid, kRefineChunkThreads, 0, b_->stream>>>(
+          keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_, b_->act_idx.As<Index>(),
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank);
+    }
     SFK_HIP_CHECK(hipGetLastError());
     pending_apply_ = true;
     pending_m_ = m;
@@ -728,7 +738,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   b->seg_start.Reserve(seg_cap * sizeof(Index));
   b->radix_counts.Reserve(2 * kRadixSetStride);
   ReserveOnesweep(b, n);
-  b->partial.Reserve(1024 * std::max(sizeof(RefinePartial), sizeof(SplitPartial)));
+  b->partial.Reserve(std::max(kRefineMaxChunks * sizeof(RefinePartial), 1024 * sizeof(SplitPartial)));
 
   Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
   Index* rank = isa_direct ? static_cast<Index*>(d_isa) : b->rank.As<Index>();
@@ -991,7 +1001,8 @@ uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
   return words * 16 + n * (w + 4) /*act*/ + n * 16 /*keys*/ + n * w * 2 /*vals*/ +
          (n / 2 + 2) * w * 3 /*segment deltas, starts*/ + n * w * 2 /*sa, rank when not direct*/ +
          n * w * 5 /*small lists (at most the first step's survivors)*/ +
-         sfk::RadixTile::kRadix * sfk::RadixPasses::kMax * 8 + 64 + 1024 * sizeof(sfk::RefinePartial) +
+         2 * sfk::kRadixSetStride +
+         std::max(sfk::kRefineMaxChunks * sizeof(sfk::RefinePartial), 1024 * sizeof(sfk::SplitPartial)) +
          sfk::OnesweepStatusBytes(n);
 }
 

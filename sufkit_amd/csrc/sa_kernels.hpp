@@ -12,8 +12,7 @@
 //               larger rows has a "negative" delta).  One step:
 //     Keygen        key = (segment id << sec_bits) | secondary(suffix)
 //     radix sort    stable LSD over the digits of the key
-//     RefineReduce/RefineSpine/RefineApply
-//                   group boundaries (key change), singletons written to SA
+//     RefineApply   group boundaries (key change), singletons written to SA
 //                   and rank (= their final ISA entry), survivors compacted
 //                   into a new list with renumbered segments, ranks of
 //                   survivors set to the SA row where their group starts
@@ -339,6 +338,33 @@ __global__ __launch_bounds__(256) void CountSpecialActiveKernel(
 
 // ------------------------------------------------------------ refine ------
 
+// Boundaries, singletons and compaction of a sorted large list.  Slot k of
+// the sorted list starts a group when its key differs from slot k-1's; a
+// group of one is a singleton (its SA row is final: sa[row] = v, rank[v] =
+// row), the others survive into the compacted list (suffix, segment id) with
+// one new segment per surviving group (delta = first row - first compacted
+// slot, start = first compacted slot).  Two schedules share the per-tile code:
+//   - single pass: one block of kRefineThreads per tile; the survivor count,
+//     new-segment count and last group start of the tiles before come from
+//     wave-wide decoupled look-backs (three values, status words interleaved
+//     by 3); the last tile writes the totals (survivors, segments) to the
+//     counters.  Tiles retire roughly in list order, so when sorted order
+//     follows text order (segments that are the references of a batch) the
+//     rank writes in flight stay within a few references;
+//   - reduce / spine / apply: per-chunk totals, their scan, then each block
+//     walks its chunk of tiles in order with no waits between blocks.  Used
+//     for a one-segment list (the first step of a single text) beyond
+//     kRefineResidentTiles tiles: its rank writes have no locality to keep,
+//     and a look-back over thousands of tiles, which the chip's dies take
+//     round-robin, waits on the slowest die.
+// Per tile: keys staged in LDS with coalesced loads (plus the key before and
+// after it); flags and per-slot offsets from a blocked scan (thread t owns
+// slots kItems*t ...); the writes go out striped (values read coalesced from
+// global), so that consecutive lanes write consecutive SA rows and
+// consecutive compacted slots.  survivor_rank: also set rank[v] of surviving
+// suffixes to their group's first row (a step whose successor rewrites every
+// survivor's rank before any rank is gathered skips it).
+
 struct RefinePartial {
   std::uint64_t survivors;
   std::uint64_t newsegs;
@@ -346,30 +372,215 @@ struct RefinePartial {
   std::uint64_t pad;
 };
 
-template <typename K>
-__device__ inline void BoundaryFlags(const K* keys, std::uint64_t k, std::uint64_t m,
-                                     bool& starts, bool& single) {
-  const K mine = keys[k];
-  starts = (k == 0) || keys[k - 1] != mine;
-  const bool ends = (k + 1 == m) || keys[k + 1] != mine;
-  single = starts && ends;
+constexpr int kRefineThreads = 512;  // single pass
+constexpr int kRefineItems = 8;
+constexpr int kRefineTile = kRefineThreads * kRefineItems;
+constexpr int kRefineResidentTiles = 512;
+constexpr int kRefineChunkThreads = 256;  // reduce / spine / apply
+constexpr int kRefineChunkTile = kRefineChunkThreads * kRefineItems;
+
+template <typename K, int kTile>
+struct RefineTileLds {
+  static_assert(kTile <= 32767, "per-slot offsets are 16-bit");
+  K key[kTile + 2];             // key[i + 1] = tile slot i
+  std::uint16_t pos[kTile];     // survivor slot, relative to the tile's first
+  std::uint16_t segc[kTile];    // segments opened through this slot (relative)
+  std::int16_t group[kTile];    // slot of the group start, -1: before the tile
+  std::uint8_t flags[kTile];    // bit0 starts, bit1 single
+};
+
+// Totals of one tile: survivors, new segments, 1 + last group start (0: none).
+struct RefineTotals {
+  std::uint64_t surv, segs, last;
+};
+
+// Stages tile [base, base + len): keys into LDS, values into registers
+// (v[j] = slot j * kBlock + thread, the striped order of the writes), and
+// fills the per-slot arrays.  The caller puts a barrier between this and
+// RefineTileWrite.
+template <int kBlock, int kItems, typename Index, typename K>
+__device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ keys, const Index* __restrict__ vals,
+                                                       std::uint64_t m, std::uint64_t base, int len,
+                                                       RefineTileLds<K, kBlock * kItems>& lds,
+                                                       std::uint64_t* scratch, Index (&v)[kItems]) {
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int i = j * kBlock + static_cast<int>(threadIdx.x);
+    if (i < len) {
+      lds.key[i + 1] = keys[base + i];
+      v[j] = vals[base + i];
+    }
+  }
+  if (threadIdx.x == 0) {
+    lds.key[0] = base > 0 ? keys[base - 1] : K(0);
+    lds.key[len + 1] = base + len < m ? keys[base + len] : K(0);
+  }
+  __syncthreads();
+  const int first = static_cast<int>(threadIdx.x) * kItems;
+  std::uint64_t surv = 0, segs = 0, last = 0;
+  std::uint8_t fl[kItems];
+#pragma unroll
+  for (int q = 0; q < kItems; ++q) {
+    const int i = first + q;
+    fl[q] = 0;
+    if (i < len) {
+      // Branch-free (all three LDS reads are in bounds), like the segment
+      // starts of LocalRefineKernel.
+      const std::uint64_t k = base + i;
+      const K mine = lds.key[i + 1];
+      const K before = lds.key[i];
+      const K after = lds.key[i + 2];
+      const bool starts = (k == 0) | (before != mine);
+      const bool ends = (k + 1 == m) | (after != mine);
+      const bool single = starts & ends;
+      fl[q] = static_cast<std::uint8_t>((starts ? 1 : 0) | (single ? 2 : 0));
+      surv += !single;
+      segs += starts && !single;
+      if (starts) last = static_cast<std::uint64_t>(i) + 1;
+    }
+  }
+  RefineTotals tot;
+  std::uint64_t pos = BlockExclusiveSum<kBlock>(surv, scratch, tot.surv);
+  std::uint64_t segc = BlockExclusiveSum<kBlock>(segs, scratch, tot.segs);
+  std::uint64_t cur_last = BlockExclusiveMax<kBlock>(last, scratch, tot.last);  // 0: none in tile yet
+#pragma unroll
+  for (int q = 0; q < kItems; ++q) {
+    const int i = first + q;
+    if (i < len) {
+      const bool starts = fl[q] & 1, single = fl[q] & 2;
+      if (starts) cur_last = static_cast<std::uint64_t>(i) + 1;
+      if (!single && starts) ++segc;
+      lds.pos[i] = static_cast<std::uint16_t>(pos);
+      lds.segc[i] = static_cast<std::uint16_t>(segc);
+      lds.group[i] = static_cast<std::int16_t>(static_cast<int>(cur_last) - 1);
+      lds.flags[i] = fl[q];
+      if (!single) ++pos;
+    }
+  }
+  return tot;
 }
 
+// Writes of one tile, given the survivors, segments and 1 + last group start
+// of every slot before the tile.
+template <int kBlock, int kItems, typename Index, typename K>
+__device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std::uint64_t base, int len, int sec_bits,
+                                       const Index* __restrict__ seg_delta, std::uint64_t n,
+                                       Index* __restrict__ sa, Index* __restrict__ rank,
+                                       Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg,
+                                       Index* __restrict__ out_delta, Index* __restrict__ out_start,
+                                       bool survivor_rank, std::uint64_t run_surv, std::uint64_t run_segs,
+                                       std::uint64_t run_last, const RefineTileLds<K, kBlock * kItems>& lds) {
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int i = j * kBlock + static_cast<int>(threadIdx.x);
+    if (i >= len) break;
+    const std::uint64_t k = base + i;
+    const K key = lds.key[i + 1];
+    const Index v = vals[j];
+    const std::uint8_t f = lds.flags[i];
+    const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
+    const Index delta = seg_delta[seg];
+    const Index row = static_cast<Index>(delta + static_cast<Index>(k));
+    if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
+    if (f & 2) {
+      sa[row] = v;
+      rank[v] = row;
+    } else {
+      // The group start lies in the same segment: every segment start is a
+      // key change, hence a group start.
+      const int g = lds.group[i];
+      const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
+      if (survivor_rank) rank[v] = static_cast<Index>(delta + static_cast<Index>(group_slot));
+      const std::uint64_t at = run_surv + lds.pos[i];
+      const std::uint64_t sc = run_segs + lds.segc[i];  // segments opened through this slot
+      if (f & 1) {
+        out_delta[sc - 1] = static_cast<Index>(row - static_cast<Index>(at));
+        out_start[sc - 1] = static_cast<Index>(at);
+      }
+      out_idx[at] = v;
+      out_seg[at] = static_cast<std::uint32_t>(sc - 1);
+    }
+  }
+}
+
+// Single pass: one block per tile (tile = block index).
+template <typename Index, typename K>
+__global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
+    const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m, int sec_bits,
+    const Index* __restrict__ seg_delta, std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank,
+    Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
+    Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
+    DeviceCounters* counters) {
+  constexpr int kBlock = kRefineThreads;
+  constexpr int kTile = kRefineTile;
+  __shared__ RefineTileLds<K, kTile> lds;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  __shared__ std::uint64_t s_prefix[3];
+  const std::uint64_t tile = blockIdx.x;
+  const std::uint64_t base = tile * kTile;
+  const int len = static_cast<int>(m - base < static_cast<std::uint64_t>(kTile) ? m - base : kTile);
+  Index v[kRefineItems];
+  const RefineTotals tot = RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, lds, scratch, v);
+  // Three wave-wide look-backs, one per wave, side by side.
+  unsigned int* error_flags = &counters->error_flags;
+  const int wave = static_cast<int>(threadIdx.x) / kWave;
+  if (wave == 0) {
+    const std::uint64_t p = LookBackWave<false>(status, 3, tile, tot.surv, 0, epoch, error_flags, kFlagLookback);
+    if (LaneId() == 0) s_prefix[0] = p;
+  } else if (wave == 1) {
+    const std::uint64_t p = LookBackWave<false>(status + 1, 3, tile, tot.segs, 0, epoch, error_flags, kFlagLookback);
+    if (LaneId() == 0) s_prefix[1] = p;
+  } else if (wave == 2) {
+    const std::uint64_t p = LookBackWave<true>(status + 2, 3, tile, tot.last ? base + tot.last : 0, 0, epoch,
+                                               error_flags, kFlagLookback);
+    if (LaneId() == 0) s_prefix[2] = p;
+  }
+  __syncthreads();
+  const std::uint64_t run_surv = s_prefix[0];
+  const std::uint64_t run_segs = s_prefix[1];
+  RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
+                                        out_delta, out_start, survivor_rank, run_surv, run_segs, s_prefix[2],
+                                        lds);
+  if (threadIdx.x == 0 && base + len == m) {
+    counters->survivors = run_surv + tot.surv;
+    counters->segments = run_segs + tot.segs;
+  }
+}
+
+// Reduce: totals of chunk b = [b * chunk, (b + 1) * chunk); chunk is a
+// multiple of the tile, so every thread's 16-byte group of keys is aligned.
 template <typename K>
-__global__ __launch_bounds__(256) void RefineReduceKernel(
-    const K* __restrict__ keys, std::uint64_t m, std::uint64_t chunk,
-    RefinePartial* __restrict__ partial) {
-  constexpr int kBlock = 256;
+__global__ __launch_bounds__(kRefineChunkThreads) void RefineReduceKernel(const K* __restrict__ keys,
+                                                                        std::uint64_t m, std::uint64_t chunk,
+                                                                        RefinePartial* __restrict__ partial) {
+  constexpr int kBlock = kRefineChunkThreads;
+  constexpr int kVec = 16 / static_cast<int>(sizeof(K));
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
   std::uint64_t surv = 0, segs = 0, last = 0;
-  for (std::uint64_t k = begin + threadIdx.x; k < end; k += kBlock) {
-    bool starts, single;
-    BoundaryFlags(keys, k, m, starts, single);
-    surv += !single;
-    segs += starts && !single;
-    if (starts) last = k + 1;
+  for (std::uint64_t k0 = begin + threadIdx.x * kVec; k0 < end; k0 += kBlock * kVec) {
+    K w[kVec + 2];  // w[q + 1] = key of slot k0 + q
+    if (k0 + kVec <= m) {
+      const uint4 raw = *reinterpret_cast<const uint4*>(keys + k0);
+      __builtin_memcpy(&w[1], &raw, sizeof(raw));
+    } else {
+#pragma unroll
+      for (int q = 0; q < kVec; ++q) w[q + 1] = k0 + q < m ? keys[k0 + q] : K(0);
+    }
+    w[0] = k0 > 0 ? keys[k0 - 1] : K(0);
+    w[kVec + 1] = k0 + kVec < m ? keys[k0 + kVec] : K(0);
+#pragma unroll
+    for (int q = 0; q < kVec; ++q) {
+      const std::uint64_t k = k0 + q;
+      const bool in = k < end;
+      const bool starts = (k == 0) | (w[q] != w[q + 1]);
+      const bool ends = (k + 1 == m) | (w[q + 2] != w[q + 1]);
+      const bool single = starts & ends;
+      surv += in & !single;
+      segs += in & starts & !single;
+      last = (in & starts) ? k + 1 : last;
+    }
   }
   std::uint64_t total_surv, total_segs, total_last;
   BlockExclusiveSum<kBlock>(surv, scratch, total_surv);
@@ -382,21 +593,43 @@ __global__ __launch_bounds__(256) void RefineReduceKernel(
   }
 }
 
-// Exclusive scan of at most 1024 block partials by one block of 1024.
+// Spine: exclusive scan of at most kRefineMaxChunks chunk totals by one
+// block of 1024 (thread t owns chunks kSpineItems*t ...); writes the list
+// totals to the counters.
+constexpr int kSpineItems = 8;
+constexpr int kRefineMaxChunks = 1024 * kSpineItems;
+
 __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial, int grid,
                                                           DeviceCounters* counters) {
   constexpr int kBlock = 1024;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  const bool valid = static_cast<int>(threadIdx.x) < grid;
-  const RefinePartial p = valid ? partial[threadIdx.x] : RefinePartial{0, 0, 0, 0};
+  const int first = static_cast<int>(threadIdx.x) * kSpineItems;
+  std::uint64_t ps[kSpineItems], pg[kSpineItems], pl[kSpineItems];
+  std::uint64_t surv = 0, segs = 0, last = 0;
+#pragma unroll
+  for (int q = 0; q < kSpineItems; ++q) {
+    const bool valid = first + q < grid;
+    ps[q] = valid ? partial[first + q].survivors : 0;
+    pg[q] = valid ? partial[first + q].newsegs : 0;
+    pl[q] = valid ? partial[first + q].last_boundary : 0;
+    surv += ps[q];
+    segs += pg[q];
+    last = pl[q] > last ? pl[q] : last;
+  }
   std::uint64_t tot_surv, tot_segs, tot_last;
-  const std::uint64_t surv = BlockExclusiveSum<kBlock>(p.survivors, scratch, tot_surv);
-  const std::uint64_t segs = BlockExclusiveSum<kBlock>(p.newsegs, scratch, tot_segs);
-  const std::uint64_t last = BlockExclusiveMax<kBlock>(p.last_boundary, scratch, tot_last);
-  if (valid) {
-    partial[threadIdx.x].survivors = surv;
-    partial[threadIdx.x].newsegs = segs;
-    partial[threadIdx.x].last_boundary = last;
+  surv = BlockExclusiveSum<kBlock>(surv, scratch, tot_surv);
+  segs = BlockExclusiveSum<kBlock>(segs, scratch, tot_segs);
+  last = BlockExclusiveMax<kBlock>(last, scratch, tot_last);
+#pragma unroll
+  for (int q = 0; q < kSpineItems; ++q) {
+    if (first + q < grid) {
+      partial[first + q].survivors = surv;
+      partial[first + q].newsegs = segs;
+      partial[first + q].last_boundary = last;
+    }
+    surv += ps[q];
+    segs += pg[q];
+    last = pl[q] > last ? pl[q] : last;
   }
   if (threadIdx.x == 0) {
     counters->survivors = tot_surv;
@@ -404,31 +637,18 @@ __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial
   }
 }
 
-// One block per chunk of whole kTile-slot tiles (offsets from the spine).
-// A tile is staged in LDS with coalesced loads (plus the key before and after
-// it); flags and per-slot offsets come from a blocked scan (thread t owns
-// slots 8t..8t+7); the writes then go out striped, so that consecutive lanes
-// write consecutive SA rows and consecutive compacted slots.  survivor_rank:
-// also set rank[v] of surviving suffixes to their group's first row (a step
-// whose successor rewrites every survivor's rank before any rank is
-// gathered skips it).
-template <typename Index, typename K, int kItems>
-__global__ __launch_bounds__(256) void RefineApplyKernel(
-    const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m,
-    std::uint64_t chunk, int sec_bits, const Index* __restrict__ seg_delta,
-    const RefinePartial* __restrict__ partial,
+// Apply: block b walks the tiles of its chunk (a multiple of the tile) in
+// order, starting from the spine's exclusive totals.
+template <typename Index, typename K>
+__global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
+    const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m, std::uint64_t chunk,
+    int sec_bits, const Index* __restrict__ seg_delta, const RefinePartial* __restrict__ partial,
     std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
-    std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
-    Index* __restrict__ out_start, bool survivor_rank) {
-  constexpr int kBlock = 256;
-  constexpr int kTile = kBlock * kItems;
-  __shared__ K s_key[kTile + 2];  // s_key[i + 1] = tile slot i
-  __shared__ Index s_val[kTile];
-  static_assert(kTile <= 32767, "per-slot offsets are 16-bit");
-  __shared__ std::uint16_t s_pos[kTile];    // survivor slot, relative to the tile's first
-  __shared__ std::uint16_t s_segc[kTile];   // segments opened through this slot (relative)
-  __shared__ std::int16_t s_group[kTile];   // slot of the group start, -1: before the tile
-  __shared__ std::uint8_t s_flags[kTile];   // bit0 starts, bit1 single
+    std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta, Index* __restrict__ out_start,
+    bool survivor_rank) {
+  constexpr int kBlock = kRefineChunkThreads;
+  constexpr int kTile = kRefineChunkTile;
+  __shared__ RefineTileLds<K, kTile> lds;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
@@ -437,97 +657,14 @@ __global__ __launch_bounds__(256) void RefineApplyKernel(
   std::uint64_t run_last = partial[blockIdx.x].last_boundary;
   for (std::uint64_t base = begin; base < end; base += kTile) {
     const int len = static_cast<int>(end - base < static_cast<std::uint64_t>(kTile) ? end - base : kTile);
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const int i = j * kBlock + static_cast<int>(threadIdx.x);
-      if (i < len) {
-        s_key[i + 1] = keys[base + i];
-        s_val[i] = vals[base + i];
-      }
-    }
-    if (threadIdx.x == 0) {
-      s_key[0] = base > 0 ? keys[base - 1] : K(0);
-      s_key[len + 1] = base + len < m ? keys[base + len] : K(0);
-    }
+    Index v[kRefineItems];
+    const RefineTotals tot = RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, lds, scratch, v);
     __syncthreads();
-    // Blocked pass: flags, counts, then per-slot offsets.
-    const int first = static_cast<int>(threadIdx.x) * kItems;
-    std::uint64_t surv = 0, segs = 0, last = 0;
-    std::uint8_t fl[kItems];
-#pragma unroll
-    for (int q = 0; q < kItems; ++q) {
-      const int i = first + q;
-      fl[q] = 0;
-      if (i < len) {
-        const std::uint64_t k = base + i;
-        // Branch-free (all three LDS reads are in bounds), like the segment
-        // starts of LocalRefineKernel.
-        const K mine = s_key[i + 1];
-        const K before = s_key[i];
-        const K after = s_key[i + 2];
-        const bool starts = (k == 0) | (before != mine);
-        const bool ends = (k + 1 == m) | (after != mine);
-        const bool single = starts & ends;
-        fl[q] = static_cast<std::uint8_t>((starts ? 1 : 0) | (single ? 2 : 0));
-        surv += !single;
-        segs += starts && !single;
-        if (starts) last = static_cast<std::uint64_t>(i) + 1;
-      }
-    }
-    std::uint64_t tot_surv, tot_segs, tot_last;
-    std::uint64_t pos = BlockExclusiveSum<kBlock>(surv, scratch, tot_surv);
-    std::uint64_t segc = BlockExclusiveSum<kBlock>(segs, scratch, tot_segs);
-    std::uint64_t cur_last = BlockExclusiveMax<kBlock>(last, scratch, tot_last);  // 0: none in tile yet
-#pragma unroll
-    for (int q = 0; q < kItems; ++q) {
-      const int i = first + q;
-      if (i < len) {
-        const bool starts = fl[q] & 1, single = fl[q] & 2;
-        if (starts) cur_last = static_cast<std::uint64_t>(i) + 1;
-        if (!single && starts) ++segc;
-        s_pos[i] = static_cast<std::uint16_t>(pos);
-        s_segc[i] = static_cast<std::uint16_t>(segc);
-        s_group[i] = static_cast<std::int16_t>(static_cast<int>(cur_last) - 1);
-        s_flags[i] = fl[q];
-        if (!single) ++pos;
-      }
-    }
-    __syncthreads();
-    // Striped pass: writes.
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const int i = j * kBlock + static_cast<int>(threadIdx.x);
-      if (i >= len) break;
-      const std::uint64_t k = base + i;
-      const K key = s_key[i + 1];
-      const Index v = s_val[i];
-      const std::uint8_t f = s_flags[i];
-      const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
-      const Index delta = seg_delta[seg];
-      const Index row = static_cast<Index>(delta + static_cast<Index>(k));
-      if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
-      if (f & 2) {
-        sa[row] = v;
-        rank[v] = row;
-      } else {
-        // The group start lies in the same segment: every segment start is a
-        // key change, hence a group start.
-        const int g = s_group[i];
-        const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
-        if (survivor_rank) rank[v] = static_cast<Index>(delta + static_cast<Index>(group_slot));
-        const std::uint64_t at = run_surv + s_pos[i];
-        const std::uint64_t sc = run_segs + s_segc[i];  // segments opened through this slot
-        if (f & 1) {
-          out_delta[sc - 1] = static_cast<Index>(row - static_cast<Index>(at));
-          out_start[sc - 1] = static_cast<Index>(at);
-        }
-        out_idx[at] = v;
-        out_seg[at] = static_cast<std::uint32_t>(sc - 1);
-      }
-    }
-    run_surv += tot_surv;
-    run_segs += tot_segs;
-    if (tot_last > 0) run_last = base + tot_last;
+    RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
+                                          out_delta, out_start, survivor_rank, run_surv, run_segs, run_last, lds);
+    run_surv += tot.surv;
+    run_segs += tot.segs;
+    if (tot.last > 0) run_last = base + tot.last;
     __syncthreads();
   }
 }
@@ -562,8 +699,8 @@ __device__ inline std::uint64_t SegmentSize(const Index* __restrict__ seg_start,
 
 template <typename Index>
 __global__ __launch_bounds__(256) void SplitCountKernel(
-    const Index* __restrict__ seg_start, std::uint32_t small_max, std::uint64_t q_cap,
-    std::uint64_t chunk, SplitPartial* __restrict__ partial, DeviceCounters* counters) {
+    const Index* __restrict__ seg_start, std::uint32_t small_max, std::uint64_t chunk,
+    SplitPartial* __restrict__ partial, const DeviceCounters* counters) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t segs = counters->segments;
@@ -587,44 +724,47 @@ __global__ __launch_bounds__(256) void SplitCountKernel(
   BlockExclusiveSum<kBlock>(large_segs, scratch, t_large_segs);
   BlockExclusiveSum<kBlock>(large_elems, scratch, t_large_elems);
   if (threadIdx.x == 0) {
-    // Bounded reservation: every failed exchange means another block's
-    // reservation succeeded (q_next only grows, up to q_cap), so this ends.
-    int fits = 0;
-    unsigned long long cur = t_small ? atomicAdd(&counters->q_next, 0ull) : 0ull;
-    while (t_small) {
-      if (cur + t_small > q_cap) break;
-      const unsigned long long prev = atomicCAS(&counters->q_next, cur, cur + t_small);
-      if (prev == cur) {
-        fits = 1;
-        break;
-      }
-      cur = prev;
-    }
     SplitPartial out;
-    out.q_base = cur;
-    out.fits = static_cast<unsigned long long>(fits);
-    out.large_segs = t_large_segs + (fits ? 0 : t_small_segs);
-    out.large_elems = t_large_elems + (fits ? 0 : t_small);
+    out.q_base = t_small;  // small suffixes and segments, until the spine
+    out.fits = t_small_segs;
+    out.large_segs = t_large_segs;
+    out.large_elems = t_large_elems;
     partial[blockIdx.x] = out;
   }
 }
 
-// Exclusive scans of the large-list counts over at most 1024 blocks.
-__global__ __launch_bounds__(1024) void SplitSpineKernel(SplitPartial* partial, int grid,
+// Over at most 1024 blocks: a block's small segments go to the small list
+// when the small suffixes of all blocks up to it fit under q_cap (the blocks
+// that fit are a prefix of the blocks, so the list stays contiguous); the
+// others join the large segments, whose counts are then scanned.
+__global__ __launch_bounds__(1024) void SplitSpineKernel(SplitPartial* partial, int grid, std::uint64_t q_cap,
                                                          DeviceCounters* counters) {
   constexpr int kBlock = 1024;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const bool valid = static_cast<int>(threadIdx.x) < grid;
-  const std::uint64_t segs_in = valid ? partial[threadIdx.x].large_segs : 0ull;
-  const std::uint64_t elems_in = valid ? partial[threadIdx.x].large_elems : 0ull;
+  SplitPartial p = valid ? partial[threadIdx.x] : SplitPartial{0, 0, 0, 0};
+  const std::uint64_t small = p.q_base, small_segs = p.fits;
+  const std::uint64_t q0 = counters->q_next;  // local survivors appended so far
+  std::uint64_t tot_small;
+  const std::uint64_t before = BlockExclusiveSum<kBlock>(small, scratch, tot_small);
+  const bool fits = q0 + before + small <= q_cap;
+  const std::uint64_t lsegs_in = p.large_segs + (fits ? 0 : small_segs);
+  const std::uint64_t lelems_in = p.large_elems + (fits ? 0 : small);
   std::uint64_t tot_segs, tot_elems;
-  const std::uint64_t segs = BlockExclusiveSum<kBlock>(segs_in, scratch, tot_segs);
-  const std::uint64_t elems = BlockExclusiveSum<kBlock>(elems_in, scratch, tot_elems);
+  const std::uint64_t segs = BlockExclusiveSum<kBlock>(lsegs_in, scratch, tot_segs);
+  const std::uint64_t elems = BlockExclusiveSum<kBlock>(lelems_in, scratch, tot_elems);
+  // Small suffixes taken = those of the blocks that fit.
+  std::uint64_t taken_total;
+  BlockExclusiveSum<kBlock>(fits ? small : std::uint64_t{0}, scratch, taken_total);
   if (valid) {
-    partial[threadIdx.x].large_segs = segs;
-    partial[threadIdx.x].large_elems = elems;
+    p.q_base = q0 + before;
+    p.fits = fits ? 1 : 0;
+    p.large_segs = segs;
+    p.large_elems = elems;
+    partial[threadIdx.x] = p;
   }
   if (threadIdx.x == 0) {
+    counters->q_next = q0 + taken_total;
     counters->l_next = tot_elems;
     counters->l_next_segs = tot_segs;
   }
