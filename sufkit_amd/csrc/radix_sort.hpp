@@ -243,17 +243,46 @@ struct ArraySource {
   }
 };
 
-// digit_count: this pass's counts per digit (from CountDigits).
-template <typename K, typename V, int kBlock, int kItems, typename Source>
+// Segment of slot k when segment s owns slots [offsets[s], offsets[s+1]),
+// searched among segments [lo, hi] (offsets[lo] <= k < offsets[hi + 1]).
+__device__ inline std::uint32_t SegmentOfSlot(const std::uint64_t* offsets, std::uint32_t lo, std::uint32_t hi,
+                                              std::uint64_t k) {
+  hi += 1;
+  while (hi - lo > 1) {
+    const std::uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= k) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Placement of a pass that sorts within segments of the slot range: segment
+// s owns slots [offsets[s], offsets[s+1]) before and after the pass (the
+// input is grouped by segment and each segment stays in place), and an
+// element of segment s with digit d goes to
+//   adj[s * kRadix + d] + (elements with digit d before it in the input)
+// where adj[s][d] = offsets[s] + (elements of s with a digit below d)
+//                 - (elements with digit d in segments before s).
+// No segment digits are sorted: a batch of references sorts by its 32-bit
+// keys in four passes, as one text does.
+struct SegmentedPass {
+  const std::uint64_t* offsets;  // segments + 1 entries, ending at m
+  std::uint32_t segments;
+  const long long* adj;          // [segments][kRadix] of this pass
+};
+
+// digit_count: this pass's counts per digit (from CountDigits); unused when
+// kSegmented (seg gives the placement instead).
+template <typename K, typename V, int kBlock, int kItems, typename Source, bool kSegmented = false>
 __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
     const unsigned long long* __restrict__ digit_count, unsigned long long* status,
     std::uint32_t epoch, unsigned int* error_flags,
-    std::uint32_t error_bit) {
+    std::uint32_t error_bit, SegmentedPass seg = SegmentedPass{nullptr, 0, nullptr}) {
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
   constexpr int kTile = kBlock * kItems;
   static_assert(kBlock >= kRadix, "one digit per thread for the look-back");
+  static_assert(!kSegmented || kTile <= 65536, "segment offsets within a tile are 16-bit");
   __shared__ std::uint32_t wave_cnt[kWaves][kRadix];
   __shared__ std::uint32_t local_start[kRadix];
   __shared__ std::uint64_t global_base[kRadix];
@@ -262,13 +291,23 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint64_t scan64[kBlock / kWave];
   __shared__ K s_keys[kTile];
   __shared__ V s_vals[kTile];
+  __shared__ std::uint16_t s_seg[kSegmented ? kTile : 1];  // segment - first segment of the tile
+  __shared__ std::uint32_t s_seg_range[2];
 
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;
-  __syncthreads();
   const std::uint64_t tile = blockIdx.x;
-  if (tile == 0) {
+  const std::uint64_t base = tile * kTile;
+  const std::uint64_t end = base + kTile < m ? base + kTile : m;
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;
+  if constexpr (kSegmented) {
+    if (threadIdx.x < 2 && base < end) {
+      s_seg_range[threadIdx.x] =
+          SegmentOfSlot(seg.offsets, 0, seg.segments - 1, threadIdx.x == 0 ? base : end - 1);
+    }
+  }
+  __syncthreads();
+  if (!kSegmented && tile == 0) {
     // Bucket starts of this pass (exclusive scan of its digit counts) enter
     // the look-back through tile 0's inclusive values.
     std::uint64_t total;
@@ -276,13 +315,17 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const std::uint64_t ex = BlockExclusiveSum<kBlock>(cnt, scan64, total);
     if (threadIdx.x < kRadix) bucket_start[threadIdx.x] = ex;
   }
-  const std::uint64_t base = tile * kTile;
-  const std::uint64_t end = base + kTile < m ? base + kTile : m;
-
   K key[kItems];
   V val[kItems];
   std::uint32_t rank[kItems];
   std::uint32_t digit[kItems];
+  std::uint32_t seg_first = 0, seg_last = 0;
+  if constexpr (kSegmented) {
+    if (base < end) {
+      seg_first = s_seg_range[0];
+      seg_last = s_seg_range[1];
+    }
+  }
   const std::uint64_t wave_base = base + static_cast<std::uint64_t>(wave) * kItems * kWave;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -321,8 +364,8 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     }
     my_count = s;
     // Publish this tile's count, then look back for the exclusive prefix.
-    global_base[d] = LookBack<false>(status + d, kRadix, tile, s, tile == 0 ? bucket_start[d] : 0, epoch,
-                                     error_flags, error_bit);
+    global_base[d] = LookBack<false>(status + d, kRadix, tile, s, (!kSegmented && tile == 0) ? bucket_start[d] : 0,
+                                     epoch, error_flags, error_bit);
   }
   __syncthreads();
   {
@@ -341,6 +384,11 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       const std::uint32_t pos = local_start[d] + wave_cnt[wave][d] + rank[j];
       s_keys[pos] = key[j];
       s_vals[pos] = val[j];
+      if constexpr (kSegmented) {
+        // Segment of the input slot (the tile usually lies in one segment).
+        const std::uint32_t sg = seg_first == seg_last ? seg_first : SegmentOfSlot(seg.offsets, seg_first, seg_last, idx);
+        s_seg[pos] = static_cast<std::uint16_t>(sg - seg_first);
+      }
     }
   }
   __syncthreads();
@@ -348,7 +396,11 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   for (std::uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
     const K k = s_keys[p];
     const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
-    const std::uint64_t out = global_base[d] + (p - local_start[d]);
+    std::uint64_t out = global_base[d] + (p - local_start[d]);
+    if constexpr (kSegmented) {
+      const std::uint64_t sg = seg_first + s_seg[p];
+      out += static_cast<std::uint64_t>(seg.adj[sg * kRadix + d]);
+    }
     if (out < m) {
       keys_out[out] = k;
       vals_out[out] = s_vals[p];

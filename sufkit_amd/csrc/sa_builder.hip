@@ -115,7 +115,7 @@ struct sfk_hip_builder {
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
       vals0, vals1, seg_delta0, seg_delta1, seg_start, radix_counts, radix_status,
-      partial, counters, narrow, ref_offsets, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
+      partial, counters, narrow, ref_offsets, ref_counts, ref_adj, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
   std::uint32_t sort_epoch = 0;  // epoch of the last onesweep pass
   int radix_set = 0;             // radix count set of the next sort
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
@@ -303,6 +303,75 @@ void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_
   }
 }
 
+// Clears the count set a sort did not count into (the next sort's set; the
+// counting kernels of the other sorts do this themselves).
+__global__ void ClearNextRadixSetKernel(RadixSet set) { ClearNextRadixSet(set); }
+
+// The first step of a batch of references whose layout fits 32-bit counts:
+// every reference's suffixes sorted by the high half of FillKey, in place in
+// the reference's own slot range, in four passes whose first generates its
+// input from the text.  Placement tables come from per-reference digit
+// counts (SegmentedPass).
+constexpr std::uint32_t kMaxSegmentedRefs = 1u << 16;
+
+template <typename Index, int kBlock, int kItems>
+void SortFillHiSegmented(BuildContext& ctx, PackedText text, std::uint32_t*& keys, Index*& vals,
+                         std::uint32_t* keys_alt, Index* vals_alt, std::uint64_t m, const RadixSet& set) {
+  sfk_hip_builder* b = ctx.b;
+  const std::uint32_t refs = ctx.refs;
+  constexpr int kRadix = RadixTile::kRadix;
+  const std::uint64_t table = static_cast<std::uint64_t>(refs) * kFillHiPasses * kRadix;
+  b->ref_counts.Reserve(table * sizeof(unsigned int));
+  b->ref_adj.Reserve(table * sizeof(long long));
+  unsigned int* counts = b->ref_counts.As<unsigned int>();
+  long long* adj = b->ref_adj.As<long long>();
+  SFK_HIP_CHECK(hipMemsetAsync(counts, 0, table * sizeof(unsigned int), b->stream));
+  ClearNextRadixSetKernel<<<1, 256, 0, b->stream>>>(set);
+  constexpr std::uint64_t kChunk = 1ull << 16;
+  TextFillHiRefCountKernel<<<static_cast<unsigned>(CeilDiv(m, kChunk)), kKeygenBlock, 0, b->stream>>>(
+      text, ctx.d_offsets, refs, kChunk, AllDigits(32), counts);
+  RefDigitStartsKernel<<<dim3(refs, kFillHiPasses), kRadix, 0, b->stream>>>(counts, ctx.d_offsets, refs, adj);
+  RefDigitBeforeKernel<<<kFillHiPasses * kRadix, kRadix, 0, b->stream>>>(counts, refs, adj);
+  SFK_HIP_CHECK(hipGetLastError());
+  const std::uint64_t text_bytes = PackedWords(m) * 16;
+  ctx.stats.algorithmic_bytes_sa += text_bytes;
+
+  constexpr std::uint64_t kTile = static_cast<std::uint64_t>(kBlock) * kItems;
+  static_assert(kTile >= kMinOnesweepTile, "status array sized for the smallest tile");
+  const std::uint64_t tiles = CeilDiv(m, kTile);
+  if ((tiles + 1) * RadixTile::kRadix * 8 > b->radix_status.Bytes()) {
+    Fail(SFK_HIP_BUILD_FAILURE, "radix status array smaller than the sort");
+  }
+  unsigned long long* status = b->radix_status.As<unsigned long long>();
+  unsigned int* error_flags = &b->counters.As<DeviceCounters>()->error_flags;
+  constexpr std::uint64_t w = sizeof(Index);
+  for (int p = 0; p < kFillHiPasses; ++p) {
+    const std::uint32_t epoch = NextEpoch(b);
+    const std::uint64_t read = p == 0 ? text_bytes : m * (4 + w);
+    const std::uint64_t bytes = read + m * (4 + w);
+    const SegmentedPass seg{ctx.d_offsets, refs, adj + static_cast<std::uint64_t>(p) * refs * kRadix};
+    const int shift = p * RadixTile::kDigitBits;
+    EventPair* ev = BeginProfiled(ctx, bytes);
+    if (p == 0) {
+      RadixOnesweep<std::uint32_t, Index, kBlock, kItems, TextFillHiSource<Index>, true>
+          <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(TextFillHiSource<Index>{text}, keys_alt,
+                                                                   vals_alt, m, shift, nullptr, status, epoch,
+                                                                   error_flags, kFlagLookback, seg);
+    } else {
+      RadixOnesweep<std::uint32_t, Index, kBlock, kItems, ArraySource<std::uint32_t, Index>, true>
+          <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(ArraySource<std::uint32_t, Index>{keys, vals},
+                                                                   keys_alt, vals_alt, m, shift, nullptr, status,
+                                                                   epoch, error_flags, kFlagLookback, seg);
+    }
+    EndProfiled(ctx, ev);
+    SFK_HIP_CHECK(hipGetLastError());
+    std::swap(keys, keys_alt);
+    std::swap(vals, vals_alt);
+    ctx.stats.sort_passes += 1;
+    ctx.stats.algorithmic_bytes_sa += bytes;
+  }
+}
+
 // Reset of the per-round list counters (small-list appends, large-list
 // allocation).
 __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
@@ -320,12 +389,16 @@ class Doubling {
   void Run() {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
     lm_ = n_;
-    l_segs_ = ctx_.refs > 0 ? ctx_.refs : 1;
+    // A batch sorts its first step within each reference's slot range
+    // (boundaries at reference starts) when its counts fit the placement
+    // tables; otherwise each reference starts as a segment of its own.
+    segmented_first_ = ctx_.refs > 0 && ctx_.refs <= kMaxSegmentedRefs && n_ <= 0xFFFFFFFFull;
+    l_segs_ = ctx_.refs > 0 && !segmented_first_ ? ctx_.refs : 1;
     l_idx_ = b_->act_idx.As<Index>();
     l_seg_ = b_->act_seg.As<std::uint32_t>();
     l_delta_ = b_->seg_delta0.As<Index>();
     SFK_HIP_CHECK(hipMemsetAsync(l_delta_, 0, l_segs_ * sizeof(Index), b_->stream));
-    if (ctx_.refs > 0) {
+    if (ctx_.refs > 0 && !segmented_first_) {
       // One initial segment per reference: its rows are its own positions.
       InitSegmentsKernel<<<static_cast<unsigned>(CeilDiv(n_, 256)), 256, 0, b_->stream>>>(
           ctx_.d_offsets, ctx_.refs, n_, b_->act_seg.As<std::uint32_t>());
@@ -573,7 +646,17 @@ class Doubling {
     // the text (keys of 16 symbols, values = positions); a counting kernel reads
     // the text for the digit counts.
     constexpr bool kNarrowKey = sizeof(K) == 4;
-    if (kNarrowKey && iota && seg_bits == 0 && sec_kind == kSecFillHi) {
+    const bool segmented = segmented_first_ && sec_kind == kSecFillHi;
+    if (kNarrowKey && segmented) {
+      if (m != n_ || !iota || seg_bits != 0) Fail(SFK_HIP_BUILD_FAILURE, "segmented first step out of place");
+      if constexpr (kNarrowKey) {
+        if (m >= (2ull << 20)) {
+          SortFillHiSegmented<Index, 1024, 8>(ctx_, text_, keys, vals, keys_alt, vals_alt, m, set);
+        } else {
+          SortFillHiSegmented<Index, 512, 8>(ctx_, text_, keys, vals, keys_alt, vals_alt, m, set);
+        }
+      }
+    } else if (kNarrowKey && iota && seg_bits == 0 && sec_kind == kSecFillHi) {
       const unsigned count_grid = static_cast<unsigned>(
           std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 16), 1024)));
       TextFillHiCountKernel<<<count_grid, kKeygenBlock, 0, b_->stream>>>(text_, passes, set);
@@ -601,14 +684,15 @@ class Doubling {
     // one-segment list, one look-back kernel otherwise.
     Index* delta_next = OtherDelta(l_delta_);
     const std::uint64_t tiles = CeilDiv(m, kRefineTile);
-    if (seg_bits > 0 || tiles <= static_cast<std::uint64_t>(kRefineResidentTiles)) {
+    if (seg_bits > 0 || segmented || tiles <= static_cast<std::uint64_t>(kRefineResidentTiles)) {
       if ((tiles + 1) * 3 * 8 > b_->radix_status.Bytes()) {
         Fail(SFK_HIP_BUILD_FAILURE, "look-back status array smaller than the list");
       }
       RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
           b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
-          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), dc);
+          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), dc, segmented ? ctx_.d_offsets : nullptr,
+          segmented ? ctx_.refs : 0u);
     } else {
       const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
       RefinePartial* partial = b_->partial.As<RefinePartial>();
@@ -616,7 +700,7 @@ class Doubling {
       RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
       RefineApplyChunkedKernel<Index, K><<<g.grid, kRefineChunkThreads, 0, b_->stream>>>(
           keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_, b_->act_idx.As<Index>(),
-          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank);
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, nullptr, 0u);
     }
     SFK_HIP_CHECK(hipGetLastError());
     pending_apply_ = true;
@@ -639,6 +723,7 @@ class Doubling {
   std::uint64_t lm_ = 0;
   std::uint64_t lm_bound_ = 0;
   bool pending_apply_ = false;
+  bool segmented_first_ = false;
   std::uint64_t pending_m_ = 0;
   std::uint64_t pending_key_bytes_ = 0;
   std::uint64_t l_segs_ = 0;

@@ -301,6 +301,106 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText
   FlushDigits<kWaves>(local, passes, set.hist);
 }
 
+// Digit counts of the first step's keys per reference of a batch:
+// counts[(r * kFillHiPasses + p) * kRadix + d].  Block b counts positions
+// [b * chunk, (b + 1) * chunk), one reference at a time (a chunk usually lies
+// in one reference), four consecutive positions per lane.  passes: the four
+// 8-bit digits of the 32-bit key.
+constexpr int kFillHiPasses = 4;
+
+__global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedText t,
+                                                                         const std::uint64_t* __restrict__ offsets,
+                                                                         std::uint32_t refs, std::uint64_t chunk,
+                                                                         RadixPasses passes,
+                                                                         unsigned int* __restrict__ counts) {
+  constexpr int kWaves = kKeygenBlock / kWave;
+  constexpr int kRadix = RadixTile::kRadix;
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][kRadix];
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * kRadix; i += kKeygenBlock) (&local[0][0][0])[i] = 0;
+  const std::uint64_t c0 = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t c1 = c0 + chunk < t.n ? c0 + chunk : t.n;
+  if (c0 >= c1) return;
+  const int wave = threadIdx.x / kWave;
+  std::uint32_t r = SegmentOfSlot(offsets, 0, refs - 1, c0);
+  __syncthreads();
+  for (;;) {
+    const std::uint64_t lo = offsets[r] > c0 ? offsets[r] : c0;
+    const std::uint64_t hi = offsets[r + 1] < c1 ? offsets[r + 1] : c1;
+    // Groups of four aligned positions overlapping [lo, hi).
+    const std::uint64_t g0 = lo / 4, g1 = (hi + 3) / 4;
+    for (std::uint64_t wbase = g0 + static_cast<std::uint64_t>(wave) * kWave; wbase < g1;
+         wbase += kKeygenBlock) {
+      const std::uint64_t g = wbase + LaneId();
+      uint4 e0 = make_uint4(0, 0, 0, 0), e1 = e0;
+      if (g < g1) {
+        e0 = t.words[g >> 3];
+        e1 = t.words[(g >> 3) + 1];
+      }
+      const bool full = wbase + kWave <= g1 && wbase * 4 >= lo && (wbase + kWave) * 4 <= hi;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const std::uint64_t p = g * 4 + j;
+        std::uint64_t c;
+        std::uint32_t sp;
+        WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
+        const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
+        CountDigits<std::uint32_t>(local[wave], passes, key, g < g1 && p >= lo && p < hi, full);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kFillHiPasses * kRadix; i += kKeygenBlock) {
+      const int p = i / kRadix, d = i % kRadix;
+      std::uint32_t sum = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        sum += local[w][p][d];
+        local[w][p][d] = 0;
+      }
+      if (sum) atomicAdd(&counts[(static_cast<std::uint64_t>(r) * kFillHiPasses + p) * kRadix + d], sum);
+    }
+    __syncthreads();
+    if (hi >= c1) break;
+    ++r;
+  }
+}
+
+// adj[(p * refs + r) * kRadix + d] = offsets[r] + (elements of r with digit
+// below d in pass p); one block per (reference, pass).
+__global__ __launch_bounds__(RadixTile::kRadix) void RefDigitStartsKernel(const unsigned int* __restrict__ counts,
+                                                                          const std::uint64_t* __restrict__ offsets,
+                                                                          std::uint32_t refs,
+                                                                          long long* __restrict__ adj) {
+  constexpr int kRadix = RadixTile::kRadix;
+  __shared__ std::uint64_t scratch[kRadix / kWave];
+  const std::uint32_t r = blockIdx.x;
+  const int p = static_cast<int>(blockIdx.y);
+  const std::uint64_t c = counts[(static_cast<std::uint64_t>(r) * kFillHiPasses + p) * kRadix + threadIdx.x];
+  std::uint64_t total;
+  const std::uint64_t ex = BlockExclusiveSum<kRadix>(c, scratch, total);
+  adj[(static_cast<std::uint64_t>(p) * refs + r) * kRadix + threadIdx.x] = static_cast<long long>(offsets[r] + ex);
+}
+
+// adj[p][r][d] -= (elements with digit d in references before r); one block
+// per (pass, digit), scanning the references 256 at a time.
+__global__ __launch_bounds__(RadixTile::kRadix) void RefDigitBeforeKernel(const unsigned int* __restrict__ counts,
+                                                                          std::uint32_t refs,
+                                                                          long long* __restrict__ adj) {
+  constexpr int kRadix = RadixTile::kRadix;
+  __shared__ std::uint64_t scratch[kRadix / kWave];
+  const int p = static_cast<int>(blockIdx.x) / kRadix;
+  const int d = static_cast<int>(blockIdx.x) % kRadix;
+  std::uint64_t before = 0;
+  for (std::uint32_t r0 = 0; r0 < refs; r0 += kRadix) {
+    const std::uint32_t r = r0 + threadIdx.x;
+    const std::uint64_t c =
+        r < refs ? counts[(static_cast<std::uint64_t>(r) * kFillHiPasses + p) * kRadix + d] : 0ull;
+    std::uint64_t total;
+    const std::uint64_t ex = BlockExclusiveSum<kRadix>(c, scratch, total);
+    if (r < refs) adj[(static_cast<std::uint64_t>(p) * refs + r) * kRadix + d] -= static_cast<long long>(before + ex);
+    before += total;
+  }
+}
+
 // First pass of the first step: key = high half of FillKey of suffix i,
 // value = i, generated from the packed text instead of read.
 template <typename Index>
@@ -387,6 +487,7 @@ struct RefineTileLds {
   std::uint16_t segc[kTile];    // segments opened through this slot (relative)
   std::int16_t group[kTile];    // slot of the group start, -1: before the tile
   std::uint8_t flags[kTile];    // bit0 starts, bit1 single
+  std::uint32_t bound[kTile / 32 + 1];  // bit i: slot i starts a reference (batch first step)
 };
 
 // Totals of one tile: survivors, new segments, 1 + last group start (0: none).
@@ -396,13 +497,17 @@ struct RefineTotals {
 
 // Stages tile [base, base + len): keys into LDS, values into registers
 // (v[j] = slot j * kBlock + thread, the striped order of the writes), and
-// fills the per-slot arrays.  The caller puts a barrier between this and
-// RefineTileWrite.
+// fills the per-slot arrays.  ref_offsets (or null): the slots are a batch's
+// references laid out in place (refs + 1 entries), and a reference start is
+// a group start whatever the keys.  The caller puts a barrier between this
+// and RefineTileWrite.
 template <int kBlock, int kItems, typename Index, typename K>
 __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ keys, const Index* __restrict__ vals,
                                                        std::uint64_t m, std::uint64_t base, int len,
-                                                       RefineTileLds<K, kBlock * kItems>& lds,
+                                                       const std::uint64_t* __restrict__ ref_offsets,
+                                                       std::uint32_t refs, RefineTileLds<K, kBlock * kItems>& lds,
                                                        std::uint64_t* scratch, Index (&v)[kItems]) {
+  constexpr int kTile = kBlock * kItems;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int i = j * kBlock + static_cast<int>(threadIdx.x);
@@ -414,6 +519,20 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
   if (threadIdx.x == 0) {
     lds.key[0] = base > 0 ? keys[base - 1] : K(0);
     lds.key[len + 1] = base + len < m ? keys[base + len] : K(0);
+  }
+  if (ref_offsets) {
+    for (int i = threadIdx.x; i < kTile / 32 + 1; i += kBlock) lds.bound[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      // From the reference holding the tile's first slot (a start only when
+      // it begins there) through the one starting right after the tile.
+      for (std::uint32_t r = SegmentOfSlot(ref_offsets, 0, refs - 1, base);
+           r < refs && ref_offsets[r] <= base + static_cast<std::uint64_t>(len); ++r) {
+        if (ref_offsets[r] < base) continue;
+        const std::uint32_t i = static_cast<std::uint32_t>(ref_offsets[r] - base);
+        lds.bound[i >> 5] |= 1u << (i & 31);
+      }
+    }
   }
   __syncthreads();
   const int first = static_cast<int>(threadIdx.x) * kItems;
@@ -430,8 +549,12 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
       const K mine = lds.key[i + 1];
       const K before = lds.key[i];
       const K after = lds.key[i + 2];
-      const bool starts = (k == 0) | (before != mine);
-      const bool ends = (k + 1 == m) | (after != mine);
+      bool starts = (k == 0) | (before != mine);
+      bool ends = (k + 1 == m) | (after != mine);
+      if (ref_offsets) {
+        starts |= (lds.bound[i >> 5] >> (i & 31)) & 1u;
+        ends |= (lds.bound[(i + 1) >> 5] >> ((i + 1) & 31)) & 1u;
+      }
       const bool single = starts & ends;
       fl[q] = static_cast<std::uint8_t>((starts ? 1 : 0) | (single ? 2 : 0));
       surv += !single;
@@ -510,7 +633,7 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     const Index* __restrict__ seg_delta, std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank,
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
     Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
-    DeviceCounters* counters) {
+    DeviceCounters* counters, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
   constexpr int kBlock = kRefineThreads;
   constexpr int kTile = kRefineTile;
   __shared__ RefineTileLds<K, kTile> lds;
@@ -520,7 +643,8 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
   const std::uint64_t base = tile * kTile;
   const int len = static_cast<int>(m - base < static_cast<std::uint64_t>(kTile) ? m - base : kTile);
   Index v[kRefineItems];
-  const RefineTotals tot = RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, lds, scratch, v);
+  const RefineTotals tot =
+      RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, ref_offsets, refs, lds, scratch, v);
   // Three wave-wide look-backs, one per wave, side by side.
   unsigned int* error_flags = &counters->error_flags;
   const int wave = static_cast<int>(threadIdx.x) / kWave;
@@ -599,38 +723,35 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineReduceKernel(const 
 constexpr int kSpineItems = 8;
 constexpr int kRefineMaxChunks = 1024 * kSpineItems;
 
+// One field of the chunk totals: sum (kMax false) or max, scanned in place.
+template <bool kMax>
+__device__ __forceinline__ std::uint64_t SpineField(RefinePartial* partial, int grid, std::uint64_t RefinePartial::*field,
+                                                    std::uint64_t* scratch) {
+  constexpr int kBlock = 1024;
+  const int first = static_cast<int>(threadIdx.x) * kSpineItems;
+  std::uint64_t v[kSpineItems];
+  std::uint64_t acc = 0;
+#pragma unroll
+  for (int q = 0; q < kSpineItems; ++q) {
+    v[q] = first + q < grid ? partial[first + q].*field : 0ull;
+    acc = kMax ? (v[q] > acc ? v[q] : acc) : acc + v[q];
+  }
+  std::uint64_t total;
+  acc = kMax ? BlockExclusiveMax<kBlock>(acc, scratch, total) : BlockExclusiveSum<kBlock>(acc, scratch, total);
+#pragma unroll
+  for (int q = 0; q < kSpineItems; ++q) {
+    if (first + q < grid) partial[first + q].*field = acc;
+    acc = kMax ? (v[q] > acc ? v[q] : acc) : acc + v[q];
+  }
+  return total;
+}
+
 __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial, int grid,
                                                           DeviceCounters* counters) {
-  constexpr int kBlock = 1024;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  const int first = static_cast<int>(threadIdx.x) * kSpineItems;
-  std::uint64_t ps[kSpineItems], pg[kSpineItems], pl[kSpineItems];
-  std::uint64_t surv = 0, segs = 0, last = 0;
-#pragma unroll
-  for (int q = 0; q < kSpineItems; ++q) {
-    const bool valid = first + q < grid;
-    ps[q] = valid ? partial[first + q].survivors : 0;
-    pg[q] = valid ? partial[first + q].newsegs : 0;
-    pl[q] = valid ? partial[first + q].last_boundary : 0;
-    surv += ps[q];
-    segs += pg[q];
-    last = pl[q] > last ? pl[q] : last;
-  }
-  std::uint64_t tot_surv, tot_segs, tot_last;
-  surv = BlockExclusiveSum<kBlock>(surv, scratch, tot_surv);
-  segs = BlockExclusiveSum<kBlock>(segs, scratch, tot_segs);
-  last = BlockExclusiveMax<kBlock>(last, scratch, tot_last);
-#pragma unroll
-  for (int q = 0; q < kSpineItems; ++q) {
-    if (first + q < grid) {
-      partial[first + q].survivors = surv;
-      partial[first + q].newsegs = segs;
-      partial[first + q].last_boundary = last;
-    }
-    surv += ps[q];
-    segs += pg[q];
-    last = pl[q] > last ? pl[q] : last;
-  }
+  __shared__ std::uint64_t scratch[1024 / kWave];
+  const std::uint64_t tot_surv = SpineField<false>(partial, grid, &RefinePartial::survivors, scratch);
+  const std::uint64_t tot_segs = SpineField<false>(partial, grid, &RefinePartial::newsegs, scratch);
+  SpineField<true>(partial, grid, &RefinePartial::last_boundary, scratch);
   if (threadIdx.x == 0) {
     counters->survivors = tot_surv;
     counters->segments = tot_segs;
@@ -645,7 +766,7 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
     int sec_bits, const Index* __restrict__ seg_delta, const RefinePartial* __restrict__ partial,
     std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
     std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta, Index* __restrict__ out_start,
-    bool survivor_rank) {
+    bool survivor_rank, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
   constexpr int kBlock = kRefineChunkThreads;
   constexpr int kTile = kRefineChunkTile;
   __shared__ RefineTileLds<K, kTile> lds;
@@ -658,7 +779,8 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
   for (std::uint64_t base = begin; base < end; base += kTile) {
     const int len = static_cast<int>(end - base < static_cast<std::uint64_t>(kTile) ? end - base : kTile);
     Index v[kRefineItems];
-    const RefineTotals tot = RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, lds, scratch, v);
+    const RefineTotals tot =
+        RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, ref_offsets, refs, lds, scratch, v);
     __syncthreads();
     RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
                                           out_delta, out_start, survivor_rank, run_surv, run_segs, run_last, lds);

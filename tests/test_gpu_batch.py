@@ -56,6 +56,31 @@ def test_single_reference_batch(hip):
     assert np.array_equal(parts[0][0], sa) and np.array_equal(parts[0][2], lcp)
 
 
+def check_against_oracle(texts, parts):
+    for text, (sa, isa, lcp) in zip(texts, parts):
+        osa, oisa, olcp = oracle.fast_build(text)
+        assert np.array_equal(sa.astype(np.uint64), osa)
+        assert np.array_equal(isa.astype(np.uint64), oisa)
+        assert np.array_equal(lcp.astype(np.uint64), olcp)
+
+
+def test_many_tiny_references(hip):
+    # Hundreds of references inside one radix tile and one refine tile.
+    rng = np.random.default_rng(11)
+    texts = [synth.random_text(9000 + r, int(rng.integers(1, 30))) for r in range(5000)]
+    parts, _ = hip.hip_build_batch(texts)
+    check_against_oracle(texts, parts)
+
+
+def test_more_references_than_the_placement_tables(hip):
+    # Beyond 65536 references the first step sorts (reference, key) pairs
+    # instead of sorting in place per reference.
+    rng = np.random.default_rng(12)
+    texts = [synth.random_text(20000 + r, int(rng.integers(1, 7))) for r in range(70000)]
+    parts, _ = hip.hip_build_batch(texts)
+    check_against_oracle(texts, parts)
+
+
 def test_batch_rejects_bad_layouts(hip):
     good = [synth.random_text(1, 50), synth.random_text(2, 60)]
     bad_sentinel = [good[0].copy(), good[1].copy()]
