@@ -69,7 +69,40 @@ def parse_args():
                     help="time without per-kernel HIP events (no roofline)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in radix shapes instead of tuning them on the device first")
+    ap.add_argument("--save-tuning", metavar="JSON", help="write the tuned radix shape table here")
+    ap.add_argument("--load-tuning", metavar="JSON",
+                    help="install a saved radix shape table instead of measuring one")
     return ap.parse_args()
+
+
+def radix_tuning(args, max_elements: int):
+    """Radix shapes for this run, outside the timed region: measured on the
+    device by sfk_hip_autotune (default), loaded from a saved table, or the
+    built-in ones.  Returns the summary reported in the JSON line."""
+    import sufkit_amd
+
+    sufkit_amd.tune_reset()
+    if args.load_tuning:
+        table = json.loads(Path(args.load_tuning).read_text())
+        sufkit_amd.tune_set(table)
+        source, seconds = "loaded", 0.0
+    elif args.no_autotune:
+        return {"source": "built-in"}
+    else:
+        t0 = time.perf_counter()
+        measured = sufkit_amd.autotune(max_elements)
+        seconds = time.perf_counter() - t0
+        source = "sfk_hip_autotune"
+        if args.save_tuning:
+            Path(args.save_tuning).write_text(json.dumps([e for e in measured if e["chosen"]]))
+    chosen = {}
+    for e in sufkit_amd.tune_get():
+        if e["tuned"]:
+            chosen.setdefault(sufkit_amd.TUNE_KINDS[e["kind"]], {})[f"class{e['size_class']}"] = \
+                f"{e['block']}x{e['items']}"
+    return {"source": source, "seconds": round(seconds, 3), "max_elements": max_elements, "chosen": chosen}
 
 
 def cpu_info() -> str:
@@ -157,6 +190,7 @@ def run_batched(args) -> int:
     d_text.upload(np.concatenate(texts))
     del texts
     outs = {k: [sufkit_amd.DeviceArray(n * 4) for _ in range(3)] for k in ("batch", "plain")}
+    tuning = radix_tuning(args, n)
     builder = sufkit_amd.DeviceBuilder(0)
     flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
 
@@ -201,6 +235,7 @@ def run_batched(args) -> int:
         "batched_over_plain": round(t_plain / t_batch, 3),
         "outputs_identical": identical,
         "phases": {k: last[k] for k in ("refine_steps", "doubling_rounds", "active_total", "sort_passes")},
+        "radix_tuning": tuning,
     }
     builder.close()
     print(json.dumps(result), flush=True)
@@ -218,6 +253,7 @@ def run_served(args) -> int:
         texts.append(t)
         bases += int(lens.sum())
     arrivals = np.cumsum(rng.exponential(args.arrival_ms / 1e3, size=args.requests)).tolist()
+    tuning = radix_tuning(args, max(t.size for t in texts))
     # warm-up: one small request through the same loop (first-use costs)
     sufkit_amd.serve([sufkit_amd.synthetic_text(1, 1, 0, 0.01)[0]], [0.0],
                      slots=1, queries_per_index=1000)
@@ -256,6 +292,7 @@ def run_served(args) -> int:
         "peak_reserved_GB": round(summary["peak_reserved_bytes"] / 1e9, 2),
         "requests_failed": summary["requests_failed"], "sampled_count_mismatches": bad,
         "wall_seconds": round(wall, 3), "latency_max_s": round(max(lat), 4),
+        "radix_tuning": tuning,
     }
     print(json.dumps(result), flush=True)
     return 0 if bad == 0 and summary["requests_failed"] == 0 else 1
@@ -301,6 +338,7 @@ def main() -> int:
         hip.sfk_hip_memcpy_d2d(dst, src, size)
         hip.sfk_hip_free(src)
         hip.sfk_hip_free(dst)
+    tuning = radix_tuning(args, n)
     builder = sufkit_amd.DeviceBuilder(0)
     flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
     if not args.no_profile:
@@ -402,6 +440,7 @@ def main() -> int:
                                       if full_copy else None),
         "pcie_seconds": {"h2d": round(h2d, 4), "d2h": round(d2h, 4) if full_copy else None},
         "sampled_rows_checked": int(rows.size), "sampled_rows_failed": sample_bad,
+        "radix_tuning": tuning,
     }
 
     if not args.no_cpu_baseline:

@@ -167,6 +167,46 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns,
                         const uint8_t* d_lengths, uint64_t count,
                         uint32_t* d_counts, void* stream);
 
+/*
+ * Radix shape autotuner.  Every onesweep pass of a build runs one of a set
+ * of compiled shapes (threads per block x keys per thread), chosen per kind
+ * of sort and size class of the list from a per-device table:
+ *   kind 0 u32 key/u32 value, 1 u64/u32, 2 u32/u64, 3 u64/u64,
+ *        4 u32/u32 and 5 u32/u64 sorted in place per reference (the first
+ *        step of a batch)
+ *   size_class c: lists of [2^(16+2c), 2^(18+2c)) elements (class 0 also
+ *        the smaller ones, class 6 the larger ones)
+ * The table starts from measured defaults.  sfk_hip_autotune times every
+ * shape that fits a workgroup's LDS for every kind and class up to
+ * max_elements (capped by free device memory) on random keys, installs the
+ * fastest, and reports each measurement (chosen = 1 for the installed one).
+ * *count receives the number of measurements even when capacity is
+ * smaller.  Builds on the device read the table at each sort, so a table
+ * change never affects the result, only the speed.  sfk_hip_tune_get lists
+ * the shape in use for every (kind, class) (tuned = 1 when it came from the
+ * tuner or sfk_hip_tune_set); sfk_hip_tune_set installs the entries with
+ * chosen = 1 (a saved table) and rejects shapes that are not compiled for
+ * their kind; sfk_hip_tune_reset restores the defaults.  No counterpart in
+ * the reference, whose CPU engines have no device shapes to pick.
+ */
+typedef struct sfk_hip_tune_entry {
+  uint32_t kind;
+  uint32_t size_class;
+  uint32_t block;
+  uint32_t items;
+  double us_per_pass;
+  uint32_t chosen;
+  uint32_t tuned;
+} sfk_hip_tune_entry;
+int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entries,
+                     uint32_t capacity, uint32_t* count, char* error,
+                     size_t error_capacity);
+int sfk_hip_tune_get(int device, sfk_hip_tune_entry* entries, uint32_t capacity,
+                     uint32_t* count);
+int sfk_hip_tune_set(int device, const sfk_hip_tune_entry* entries, uint32_t count,
+                     char* error, size_t error_capacity);
+int sfk_hip_tune_reset(int device);
+
 /* Device memory helpers so callers without a device allocator of their own
  * can stage buffers (tests, benchmarks). */
 int sfk_hip_malloc(int device, uint64_t bytes, void** out);

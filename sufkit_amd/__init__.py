@@ -391,6 +391,80 @@ def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
     return out, summ, samples
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("size_class", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("items", ctypes.c_uint32), ("us_per_pass", ctypes.c_double), ("chosen", ctypes.c_uint32),
+                ("tuned", ctypes.c_uint32)]
+
+
+TUNE_KINDS = ("u32/u32", "u64/u32", "u32/u64", "u64/u64", "u32/u32 per reference", "u32/u64 per reference")
+
+
+def _tune_lib():
+    lib = load_hip()
+    lib.sfk_hip_autotune.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(TuneEntry), ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_autotune.restype = ctypes.c_int
+    lib.sfk_hip_tune_get.argtypes = [ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_uint32)]
+    lib.sfk_hip_tune_get.restype = ctypes.c_int
+    lib.sfk_hip_tune_set.argtypes = [ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_uint32, ctypes.c_char_p,
+                                     ctypes.c_size_t]
+    lib.sfk_hip_tune_set.restype = ctypes.c_int
+    lib.sfk_hip_tune_reset.argtypes = [ctypes.c_int]
+    lib.sfk_hip_tune_reset.restype = ctypes.c_int
+    return lib
+
+
+def _entries(arr, count):
+    return [{name: getattr(arr[i], name) for name, _ in TuneEntry._fields_} for i in range(count)]
+
+
+def autotune(max_elements: int, device: int = 0) -> list[dict]:
+    """Time every compiled radix shape per sort kind and size class up to
+    max_elements on the device, install the fastest, and return every
+    measurement (chosen = 1 for the installed shapes)."""
+    lib = _tune_lib()
+    cap = 512
+    arr = (TuneEntry * cap)()
+    count = ctypes.c_uint32(0)
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_autotune(device, max_elements, arr, cap, ctypes.byref(count), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace") or "autotuning failed")
+    return _entries(arr, min(count.value, cap))
+
+
+def tune_get(device: int = 0) -> list[dict]:
+    """The radix shape in use for every (kind, size class) of the device."""
+    lib = _tune_lib()
+    arr = (TuneEntry * 64)()
+    count = ctypes.c_uint32(0)
+    status = lib.sfk_hip_tune_get(device, arr, 64, ctypes.byref(count))
+    if status != 0:
+        raise SufkitError(status, "tune_get failed")
+    return _entries(arr, min(count.value, 64))
+
+
+def tune_set(entries: list[dict], device: int = 0) -> None:
+    """Install radix shapes (entries with chosen = 1), e.g. a saved table."""
+    lib = _tune_lib()
+    arr = (TuneEntry * max(1, len(entries)))()
+    for i, e in enumerate(entries):
+        arr[i] = TuneEntry(e["kind"], e["size_class"], e["block"], e["items"], e.get("us_per_pass", 0.0),
+                           e.get("chosen", 1), e.get("tuned", 1))
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_tune_set(device, arr, len(entries), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+
+
+def tune_reset(device: int = 0) -> None:
+    """Back to the built-in radix shapes."""
+    if _tune_lib().sfk_hip_tune_reset(device) != 0:
+        raise SufkitError(2, "tune_reset failed")
+
+
 def decode_pattern(code: int, length: int) -> str:
     """2-bit packed pattern (first symbol in the top bits) to an A/C/G/T string."""
     return "".join("ACGT"[(int(code) >> (62 - 2 * j)) & 3] for j in range(int(length)))

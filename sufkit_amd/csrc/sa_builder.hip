@@ -17,10 +17,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -188,21 +190,92 @@ SortGeometry Geometry(std::uint64_t m, int tile, int max_grid) {
   return SortGeometry{static_cast<int>(grid), per * static_cast<std::uint64_t>(tile)};
 }
 
-// Onesweep tile shapes, chosen per pass by key/value width and element
-// count (measured on MI355X: large tiles write longer runs per bucket and
-// look back over fewer tiles; below ~2M elements smaller tiles keep more
-// blocks in flight).
-//   u32 key + u32 value, m >= 4M   1024 threads x 16 items  (128 KiB tile)
-//   other widths,        m >= 2M   1024 threads x  8 items  (96-128 KiB)
-//   otherwise                       512 threads x  8 items
-constexpr std::uint64_t kMinOnesweepTile = 512 * 8;
+// ------------------------------------------------------------ radix shapes
 
-template <typename K, typename V>
-struct OnesweepShape {
-  static constexpr bool kNarrow = sizeof(K) + sizeof(V) == 8;
-  static constexpr int kBigItems = kNarrow ? 16 : 8;
-  static constexpr std::uint64_t kBigFrom = kNarrow ? (4ull << 20) : (2ull << 20);
+// Onesweep shapes (threads x keys per thread).  A build picks one per kind of
+// sort (key width, value width, in place per reference or not) and size
+// class of the list from a per-device table.  The table starts from
+// measured defaults and sfk_hip_autotune replaces it with the fastest
+// compiled shape per (kind, class) measured on the device itself.
+struct ShapeDesc {
+  int block;
+  int items;
 };
+constexpr ShapeDesc kShapes[] = {{256, 16}, {512, 8}, {512, 16}, {1024, 8}, {1024, 12}, {1024, 16}};
+constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
+constexpr std::uint64_t kMinOnesweepTile = 256 * 16;
+
+// Kinds: 0 u32/u32, 1 u64/u32, 2 u32/u64, 3 u64/u64 (key/value), 4 and 5
+// the u32/u32 and u32/u64 sorts in place per reference (batch first step).
+constexpr int kSortKinds = 6;
+// Size classes: class c holds lists of [2^(16 + 2c), 2^(18 + 2c)) elements
+// (class 0 also the smaller ones, class 6 the larger ones).
+constexpr int kSizeClasses = 7;
+
+int SizeClass(std::uint64_t m) {
+  int c = 0;
+  while (c + 1 < kSizeClasses && m >= (1ull << (18 + 2 * c))) ++c;
+  return c;
+}
+
+template <typename K, typename V, bool kSeg>
+constexpr int SortKind() {
+  if constexpr (kSeg) return sizeof(V) == 4 ? 4 : 5;
+  if constexpr (sizeof(K) == 4) return sizeof(V) == 4 ? 0 : 2;
+  return sizeof(V) == 4 ? 1 : 3;
+}
+
+// Static LDS of RadixOnesweep (its __shared__ arrays) against the 160 KiB a
+// gfx950 workgroup may hold.
+template <typename K, typename V, int kBlock, int kItems, bool kSeg>
+constexpr bool ShapeFits() {
+  constexpr std::uint64_t tile = static_cast<std::uint64_t>(kBlock) * kItems;
+  constexpr std::uint64_t lds = tile * (sizeof(K) + sizeof(V) + (kSeg ? 2 : 0)) +
+                                static_cast<std::uint64_t>(kBlock / kWave) * RadixTile::kRadix * 4 +
+                                RadixTile::kRadix * (4 + 8 + 8) + (kBlock / kWave) * 12 + 64;
+  return lds <= 160 * 1024;
+}
+
+constexpr bool KindShapeFits(int kind, int shape) {
+  const ShapeDesc d = kShapes[shape];
+  const std::uint64_t key = kind == 1 || kind == 3 ? 8 : 4;
+  const std::uint64_t val = kind == 2 || kind == 3 || kind == 5 ? 8 : 4;
+  const std::uint64_t seg = kind >= 4 ? 2 : 0;
+  const std::uint64_t lds = static_cast<std::uint64_t>(d.block) * d.items * (key + val + seg) +
+                            static_cast<std::uint64_t>(d.block / kWave) * RadixTile::kRadix * 4 +
+                            RadixTile::kRadix * (4 + 8 + 8) + (d.block / kWave) * 12 + 64;
+  return lds <= 160 * 1024;
+}
+
+// Defaults (measured on MI355X): large tiles write longer runs per bucket and
+// look back over fewer tiles; below ~2M elements smaller tiles keep more
+// blocks in flight.
+int DefaultShape(int kind, std::uint64_t m) {
+  if (kind == 0 && m >= (4ull << 20)) return 5;  // 1024 x 16
+  if (m >= (2ull << 20)) return 3;               // 1024 x 8
+  return 1;                                      // 512 x 8
+}
+
+constexpr int kMaxTuneDevices = 64;
+// -1: the default rule; otherwise an index into kShapes.
+std::atomic<int> g_shape_table[kMaxTuneDevices][kSortKinds][kSizeClasses];
+std::once_flag g_shape_table_once;
+
+void InitShapeTable() {
+  std::call_once(g_shape_table_once, [] {
+    for (auto& dev : g_shape_table)
+      for (auto& kind : dev)
+        for (auto& c : kind) c.store(-1);
+  });
+}
+
+int ShapeFor(int device, int kind, std::uint64_t m) {
+  InitShapeTable();
+  const int tuned = device >= 0 && device < kMaxTuneDevices
+                        ? g_shape_table[device][kind][SizeClass(m)].load(std::memory_order_relaxed)
+                        : -1;
+  return tuned >= 0 ? tuned : DefaultShape(kind, m);
+}
 
 std::uint64_t OnesweepStatusBytes(std::uint64_t n) {
   return (CeilDiv(std::max<std::uint64_t>(n, 1), kMinOnesweepTile) + 1) * RadixTile::kRadix * 8;
@@ -249,10 +322,10 @@ RadixPasses AllDigits(int total_bits) {
   return passes;
 }
 
-template <typename K, typename V, int kBlock, int kItems, typename Source>
-void OnesweepPasses(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys,
-                    V*& vals, K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes,
-                    const RadixSet& set) {
+template <typename K, typename V, int kBlock, int kItems, bool kSeg, typename Source>
+void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys, V*& vals,
+                     K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set,
+                     const long long* adj) {
   sfk_hip_builder* b = ctx.b;
   constexpr std::uint64_t kTile = static_cast<std::uint64_t>(kBlock) * kItems;
   static_assert(kTile >= kMinOnesweepTile, "status array sized for the smallest tile");
@@ -266,16 +339,21 @@ void OnesweepPasses(BuildContext& ctx, const Source& first, std::uint64_t first_
     const std::uint32_t epoch = NextEpoch(b);
     const std::uint64_t read = p == 0 ? first_read_bytes : m * (sizeof(K) + sizeof(V));
     const std::uint64_t bytes = read + m * (sizeof(K) + sizeof(V));
+    SegmentedPass seg{nullptr, 0, nullptr};
+    if constexpr (kSeg) {
+      seg = SegmentedPass{ctx.d_offsets, ctx.refs,
+                          adj + static_cast<std::uint64_t>(p) * ctx.refs * RadixTile::kRadix};
+    }
     EventPair* ev = BeginProfiled(ctx, bytes);
     if (p == 0) {
-      RadixOnesweep<K, V, kBlock, kItems, Source><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
-          first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status,
-          epoch, error_flags, kFlagLookback);
+      RadixOnesweep<K, V, kBlock, kItems, Source, kSeg><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
+          first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status, epoch,
+          error_flags, kFlagLookback, seg);
     } else {
-      RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>>
+      RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
-              set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback);
+              set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback, seg);
     }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
@@ -286,20 +364,38 @@ void OnesweepPasses(BuildContext& ctx, const Source& first, std::uint64_t first_
   }
 }
 
-// Stable LSD sort of m (key, value) pairs over `passes`, whose digit counts
-// are already in set.hist.  The first pass reads `first` (the arrays keys/
-// vals, or input generated on the fly); on return keys/vals point at the
-// sorted buffers.  Each pass is one onesweep kernel.
-template <typename K, typename V, typename Source>
-void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys, V*& vals,
-               K* keys_alt, V* vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set) {
-  if (passes.count == 0 || m == 0) return;
-  using Shape = OnesweepShape<K, V>;
-  if (m >= Shape::kBigFrom) {
-    OnesweepPasses<K, V, 1024, Shape::kBigItems>(ctx, first, first_read_bytes, keys, vals, keys_alt,
-                                                 vals_alt, m, passes, set);
+template <typename K, typename V, int kShape, bool kSeg, typename Source>
+void OnesweepPassesShape(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys,
+                         V*& vals, K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes,
+                         const RadixSet& set, const long long* adj) {
+  constexpr ShapeDesc d = kShapes[kShape];
+  if constexpr (ShapeFits<K, V, d.block, d.items, kSeg>()) {
+    OnesweepPassesT<K, V, d.block, d.items, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m,
+                                                  passes, set, adj);
   } else {
-    OnesweepPasses<K, V, 512, 8>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set);
+    Fail(SFK_HIP_BUILD_FAILURE, "radix shape does not fit the LDS of a workgroup");
+  }
+}
+
+// Stable LSD sort of m (key, value) pairs over `passes`, whose digit counts
+// are already in set.hist (or, kSeg, whose placement tables are in adj).
+// The first pass reads `first` (the arrays keys/vals, or input generated on
+// the fly); on return keys/vals point at the sorted buffers.  Each pass is
+// one onesweep kernel of the shape the device's table gives.
+template <typename K, typename V, bool kSeg = false, typename Source>
+void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys, V*& vals,
+               K* keys_alt, V* vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set,
+               const long long* adj = nullptr, int shape = -1) {
+  if (passes.count == 0 || m == 0) return;
+  if (shape < 0) shape = ShapeFor(ctx.b->device, SortKind<K, V, kSeg>(), m);
+  switch (shape) {
+    case 0: OnesweepPassesShape<K, V, 0, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 1: OnesweepPassesShape<K, V, 1, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 2: OnesweepPassesShape<K, V, 2, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 3: OnesweepPassesShape<K, V, 3, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 4: OnesweepPassesShape<K, V, 4, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 5: OnesweepPassesShape<K, V, 5, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    default: Fail(SFK_HIP_BUILD_FAILURE, "unknown radix shape");
   }
 }
 
@@ -314,7 +410,7 @@ __global__ void ClearNextRadixSetKernel(RadixSet set) { ClearNextRadixSet(set); 
 // counts (SegmentedPass).
 constexpr std::uint32_t kMaxSegmentedRefs = 1u << 16;
 
-template <typename Index, int kBlock, int kItems>
+template <typename Index>
 void SortFillHiSegmented(BuildContext& ctx, PackedText text, std::uint32_t*& keys, Index*& vals,
                          std::uint32_t* keys_alt, Index* vals_alt, std::uint64_t m, const RadixSet& set) {
   sfk_hip_builder* b = ctx.b;
@@ -328,51 +424,87 @@ void SortFillHiSegmented(BuildContext& ctx, PackedText text, std::uint32_t*& key
   SFK_HIP_CHECK(hipMemsetAsync(counts, 0, table * sizeof(unsigned int), b->stream));
   ClearNextRadixSetKernel<<<1, 256, 0, b->stream>>>(set);
   constexpr std::uint64_t kChunk = 1ull << 16;
+  const RadixPasses passes = AllDigits(32);
   TextFillHiRefCountKernel<<<static_cast<unsigned>(CeilDiv(m, kChunk)), kKeygenBlock, 0, b->stream>>>(
-      text, ctx.d_offsets, refs, kChunk, AllDigits(32), counts);
+      text, ctx.d_offsets, refs, kChunk, passes, counts);
   RefDigitStartsKernel<<<dim3(refs, kFillHiPasses), kRadix, 0, b->stream>>>(counts, ctx.d_offsets, refs, adj);
   RefDigitBeforeKernel<<<kFillHiPasses * kRadix, kRadix, 0, b->stream>>>(counts, refs, adj);
   SFK_HIP_CHECK(hipGetLastError());
   const std::uint64_t text_bytes = PackedWords(m) * 16;
   ctx.stats.algorithmic_bytes_sa += text_bytes;
+  SortPairs<std::uint32_t, Index, true>(ctx, TextFillHiSource<Index>{text}, text_bytes, keys, vals, keys_alt,
+                                        vals_alt, m, passes, set, adj);
+}
 
-  constexpr std::uint64_t kTile = static_cast<std::uint64_t>(kBlock) * kItems;
-  static_assert(kTile >= kMinOnesweepTile, "status array sized for the smallest tile");
-  const std::uint64_t tiles = CeilDiv(m, kTile);
-  if ((tiles + 1) * RadixTile::kRadix * 8 > b->radix_status.Bytes()) {
-    Fail(SFK_HIP_BUILD_FAILURE, "radix status array smaller than the sort");
+// ------------------------------------------------------------ autotuner
+
+struct TuneResult {
+  int kind;
+  int size_class;
+  int shape;
+  double us_per_pass;
+};
+
+// Times every compiled shape of one sort kind on uniformly random keys, for
+// each size class up to max_elements: two passes per sort, the best of two
+// timed sorts after one warm-up, on the tuning builder's stream.
+template <typename K, typename V, bool kSeg>
+void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hipEvent_t e1,
+              std::vector<TuneResult>& out) {
+  constexpr int kind = SortKind<K, V, kSeg>();
+  BuildContext ctx{};
+  ctx.b = b;
+  ctx.profile = false;
+  RadixPasses passes{};
+  passes.count = 2;
+  passes.shift[0] = 0;
+  passes.shift[1] = RadixTile::kDigitBits;
+  long long* adj = b->ref_adj.As<long long>();
+  if constexpr (kSeg) {
+    ctx.d_offsets = b->ref_offsets.As<std::uint64_t>();
+    ctx.refs = 1;
   }
-  unsigned long long* status = b->radix_status.As<unsigned long long>();
-  unsigned int* error_flags = &b->counters.As<DeviceCounters>()->error_flags;
-  constexpr std::uint64_t w = sizeof(Index);
-  for (int p = 0; p < kFillHiPasses; ++p) {
-    const std::uint32_t epoch = NextEpoch(b);
-    const std::uint64_t read = p == 0 ? text_bytes : m * (4 + w);
-    const std::uint64_t bytes = read + m * (4 + w);
-    const SegmentedPass seg{ctx.d_offsets, refs, adj + static_cast<std::uint64_t>(p) * refs * kRadix};
-    const int shift = p * RadixTile::kDigitBits;
-    EventPair* ev = BeginProfiled(ctx, bytes);
-    if (p == 0) {
-      RadixOnesweep<std::uint32_t, Index, kBlock, kItems, TextFillHiSource<Index>, true>
-          <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(TextFillHiSource<Index>{text}, keys_alt,
-                                                                   vals_alt, m, shift, nullptr, status, epoch,
-                                                                   error_flags, kFlagLookback, seg);
-    } else {
-      RadixOnesweep<std::uint32_t, Index, kBlock, kItems, ArraySource<std::uint32_t, Index>, true>
-          <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(ArraySource<std::uint32_t, Index>{keys, vals},
-                                                                   keys_alt, vals_alt, m, shift, nullptr, status,
-                                                                   epoch, error_flags, kFlagLookback, seg);
+  for (int c = 0; c < kSizeClasses; ++c) {
+    const std::uint64_t lower = c == 0 ? 0 : (1ull << (16 + 2 * c));
+    if (lower > max_elements) break;
+    const std::uint64_t m = std::min<std::uint64_t>(1ull << (17 + 2 * c), max_elements);
+    if (m < kMinOnesweepTile) continue;
+    ctx.n = m;
+    if constexpr (kSeg) {
+      const std::uint64_t offsets[2] = {0, m};
+      SFK_HIP_CHECK(hipMemcpyAsync(b->ref_offsets.As<void>(), offsets, sizeof(offsets), hipMemcpyHostToDevice,
+                                   b->stream));
+      SFK_HIP_CHECK(hipStreamSynchronize(b->stream));
     }
-    EndProfiled(ctx, ev);
-    SFK_HIP_CHECK(hipGetLastError());
-    std::swap(keys, keys_alt);
-    std::swap(vals, vals_alt);
-    ctx.stats.sort_passes += 1;
-    ctx.stats.algorithmic_bytes_sa += bytes;
+    const unsigned grid = static_cast<unsigned>(
+        std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 4), 1024)));
+    for (int shape = 0; shape < kShapeCount; ++shape) {
+      if (!KindShapeFits(kind, shape)) continue;
+      float best = 0;
+      for (int rep = 0; rep < 3; ++rep) {
+        const RadixSet set = TakeRadixSet(b);
+        K* keys = b->keys0.As<K>();
+        V* vals = b->vals0.As<V>();
+        TuneKeysKernel<K, V><<<grid, kKeygenBlock, 0, b->stream>>>(keys, vals, m, 0x5eed0000ull * (rep + 1),
+                                                                    passes, set);
+        if constexpr (kSeg) TuneAdjKernel<<<passes.count, RadixTile::kRadix, 0, b->stream>>>(set.hist, adj);
+        SFK_HIP_CHECK(hipGetLastError());
+        SFK_HIP_CHECK(hipEventRecord(e0, b->stream));
+        SortPairs<K, V, kSeg>(ctx, ArraySource<K, V>{keys, vals}, m * (sizeof(K) + sizeof(V)), keys, vals,
+                              b->keys1.As<K>(), b->vals1.As<V>(), m, passes, set, adj, shape);
+        SFK_HIP_CHECK(hipEventRecord(e1, b->stream));
+        SFK_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        SFK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep == 1 || (rep > 1 && ms < best)) best = ms;
+      }
+      out.push_back(TuneResult{kind, c, shape, best * 1000.0 / passes.count});
+    }
   }
 }
 
 // Reset of the per-round list counters (small-list appends, large-list
+// allocation).// Reset of the per-round list counters (small-list appends, large-list
 // allocation).
 __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
   c->q_next = 0;
@@ -650,11 +782,7 @@ class Doubling {
     if (kNarrowKey && segmented) {
       if (m != n_ || !iota || seg_bits != 0) Fail(SFK_HIP_BUILD_FAILURE, "segmented first step out of place");
       if constexpr (kNarrowKey) {
-        if (m >= (2ull << 20)) {
-          SortFillHiSegmented<Index, 1024, 8>(ctx_, text_, keys, vals, keys_alt, vals_alt, m, set);
-        } else {
-          SortFillHiSegmented<Index, 512, 8>(ctx_, text_, keys, vals, keys_alt, vals_alt, m, set);
-        }
+        SortFillHiSegmented<Index>(ctx_, text_, keys, vals, keys_alt, vals_alt, m, set);
       }
     } else if (kNarrowKey && iota && seg_bits == 0 && sec_kind == kSecFillHi) {
       const unsigned count_grid = static_cast<unsigned>(
@@ -1375,3 +1503,157 @@ int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
 }
 
 }  // extern "C"
+
+namespace {
+
+void TuneEntryOf(int kind, int size_class, int shape, double us, bool chosen, bool tuned, sfk_hip_tune_entry* e) {
+  e->kind = static_cast<uint32_t>(kind);
+  e->size_class = static_cast<uint32_t>(size_class);
+  e->block = static_cast<uint32_t>(sfk::kShapes[shape].block);
+  e->items = static_cast<uint32_t>(sfk::kShapes[shape].items);
+  e->us_per_pass = us;
+  e->chosen = chosen ? 1u : 0u;
+  e->tuned = tuned ? 1u : 0u;
+}
+
+}  // namespace
+
+int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entries, uint32_t capacity,
+                     uint32_t* count, char* error, size_t error_capacity) {
+  if (count != nullptr) *count = 0;
+  if (device < 0 || device >= sfk::kMaxTuneDevices) {
+    sfk::CopyError("device index out of range", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  sfk_hip_builder* b = nullptr;
+  const int status = sfk_hip_builder_create(device, &b, error, error_capacity);
+  if (status != SFK_HIP_OK) return status;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int result = SFK_HIP_OK;
+  try {
+    std::size_t free_bytes = 0, total_bytes = 0;
+    SFK_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
+    const std::uint64_t spare = free_bytes > (4ull << 30) ? free_bytes - (4ull << 30) : 0;
+    max_elements = std::min<std::uint64_t>(max_elements, spare / 40);
+    max_elements = std::max<std::uint64_t>(max_elements, sfk::kMinOnesweepTile);
+    const std::uint64_t top = std::min<std::uint64_t>(max_elements, 1ull << 29);
+    b->keys0.Reserve(top * 8);
+    b->keys1.Reserve(top * 8);
+    b->vals0.Reserve(top * 8);
+    b->vals1.Reserve(top * 8);
+    b->ref_offsets.Reserve(2 * sizeof(std::uint64_t));
+    b->ref_adj.Reserve(sfk::RadixPasses::kMax * sfk::RadixTile::kRadix * sizeof(long long));
+    b->radix_counts.Reserve(2 * sfk::kRadixSetStride);
+    SFK_HIP_CHECK(hipMemsetAsync(b->radix_counts.As<void>(), 0, b->radix_counts.Bytes(), b->stream));
+    b->radix_set = 0;
+    sfk::ReserveOnesweep(b, top);
+    sfk::ResetCounters(b);
+    SFK_HIP_CHECK(hipEventCreate(&e0));
+    SFK_HIP_CHECK(hipEventCreate(&e1));
+    std::vector<sfk::TuneResult> results;
+    sfk::TuneKind<std::uint32_t, std::uint32_t, false>(b, top, e0, e1, results);
+    sfk::TuneKind<std::uint64_t, std::uint32_t, false>(b, top, e0, e1, results);
+    sfk::TuneKind<std::uint32_t, std::uint64_t, false>(b, top, e0, e1, results);
+    sfk::TuneKind<std::uint64_t, std::uint64_t, false>(b, top, e0, e1, results);
+    sfk::TuneKind<std::uint32_t, std::uint32_t, true>(b, top, e0, e1, results);
+    sfk::TuneKind<std::uint32_t, std::uint64_t, true>(b, top, e0, e1, results);
+    const sfk::DeviceCounters& c = sfk::ReadCounters(b);
+    if (c.error_flags != 0) sfk::Fail(SFK_HIP_BUILD_FAILURE, "radix autotuning failed (look-back did not converge)");
+    // Fastest shape per (kind, class); classes above max_elements keep the
+    // table's current entry.
+    int best[sfk::kSortKinds][sfk::kSizeClasses];
+    double best_us[sfk::kSortKinds][sfk::kSizeClasses];
+    for (int k = 0; k < sfk::kSortKinds; ++k)
+      for (int cl = 0; cl < sfk::kSizeClasses; ++cl) best[k][cl] = -1, best_us[k][cl] = 0;
+    for (const auto& r : results) {
+      if (best[r.kind][r.size_class] < 0 || r.us_per_pass < best_us[r.kind][r.size_class]) {
+        best[r.kind][r.size_class] = r.shape;
+        best_us[r.kind][r.size_class] = r.us_per_pass;
+      }
+    }
+    sfk::InitShapeTable();
+    for (int k = 0; k < sfk::kSortKinds; ++k)
+      for (int cl = 0; cl < sfk::kSizeClasses; ++cl)
+        if (best[k][cl] >= 0) sfk::g_shape_table[device][k][cl].store(best[k][cl]);
+    std::uint32_t written = 0;
+    for (const auto& r : results) {
+      if (entries != nullptr && written < capacity) {
+        TuneEntryOf(r.kind, r.size_class, r.shape, r.us_per_pass, best[r.kind][r.size_class] == r.shape, true,
+                    &entries[written]);
+        ++written;
+      }
+    }
+    if (count != nullptr) *count = static_cast<uint32_t>(results.size());
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    result = f.status;
+  } catch (const std::bad_alloc&) {
+    sfk::CopyError("host allocation failed", error, error_capacity);
+    result = SFK_HIP_BUILD_FAILURE;
+  }
+  if (e0 != nullptr) (void)hipEventDestroy(e0);
+  if (e1 != nullptr) (void)hipEventDestroy(e1);
+  sfk_hip_builder_destroy(b);
+  return result;
+}
+
+int sfk_hip_tune_get(int device, sfk_hip_tune_entry* entries, uint32_t capacity, uint32_t* count) {
+  if (count != nullptr) *count = 0;
+  if (device < 0 || device >= sfk::kMaxTuneDevices) return SFK_HIP_INVALID_INPUT;
+  sfk::InitShapeTable();
+  std::uint32_t written = 0;
+  for (int k = 0; k < sfk::kSortKinds; ++k) {
+    for (int cl = 0; cl < sfk::kSizeClasses; ++cl) {
+      const int tuned = sfk::g_shape_table[device][k][cl].load();
+      const std::uint64_t rep = 1ull << (17 + 2 * cl);
+      const int shape = tuned >= 0 ? tuned : sfk::DefaultShape(k, rep);
+      if (entries != nullptr && written < capacity) TuneEntryOf(k, cl, shape, 0.0, true, tuned >= 0, &entries[written]);
+      ++written;
+    }
+  }
+  if (count != nullptr) *count = written;
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_tune_set(int device, const sfk_hip_tune_entry* entries, uint32_t count, char* error,
+                     size_t error_capacity) {
+  if (device < 0 || device >= sfk::kMaxTuneDevices || (entries == nullptr && count > 0)) {
+    sfk::CopyError("device index out of range or no entries", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  int shapes[sfk::kSortKinds][sfk::kSizeClasses];
+  for (auto& k : shapes)
+    for (auto& c : k) c = -2;  // untouched
+  for (uint32_t i = 0; i < count; ++i) {
+    const sfk_hip_tune_entry& e = entries[i];
+    if (e.chosen == 0) continue;
+    int shape = -1;
+    for (int j = 0; j < sfk::kShapeCount; ++j) {
+      if (static_cast<int>(e.block) == sfk::kShapes[j].block && static_cast<int>(e.items) == sfk::kShapes[j].items) {
+        shape = j;
+      }
+    }
+    if (e.kind >= static_cast<uint32_t>(sfk::kSortKinds) || e.size_class >= static_cast<uint32_t>(sfk::kSizeClasses) ||
+        shape < 0 || !sfk::KindShapeFits(static_cast<int>(e.kind), shape)) {
+      sfk::CopyError("entry " + std::to_string(i) + ": no compiled radix shape " + std::to_string(e.block) + "x" +
+                         std::to_string(e.items) + " for kind " + std::to_string(e.kind) + ", class " +
+                         std::to_string(e.size_class),
+                     error, error_capacity);
+      return SFK_HIP_INVALID_INPUT;
+    }
+    shapes[e.kind][e.size_class] = shape;
+  }
+  sfk::InitShapeTable();
+  for (int k = 0; k < sfk::kSortKinds; ++k)
+    for (int cl = 0; cl < sfk::kSizeClasses; ++cl)
+      if (shapes[k][cl] != -2) sfk::g_shape_table[device][k][cl].store(shapes[k][cl]);
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_tune_reset(int device) {
+  if (device < 0 || device >= sfk::kMaxTuneDevices) return SFK_HIP_INVALID_INPUT;
+  sfk::InitShapeTable();
+  for (auto& k : sfk::g_shape_table[device])
+    for (auto& c : k) c.store(-1);
+  return SFK_HIP_OK;
+}

@@ -401,6 +401,58 @@ __global__ __launch_bounds__(RadixTile::kRadix) void RefDigitBeforeKernel(const 
   }
 }
 
+// ------------------------------------------------------------ tuning ----
+
+// Synthetic input of the radix autotuner: key i = splitmix64(seed + i)
+// (truncated to K), value i = i, and the digit counts of `passes` over them.
+__device__ inline std::uint64_t SplitMix64(std::uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+template <typename K, typename V>
+__global__ __launch_bounds__(kKeygenBlock) void TuneKeysKernel(K* __restrict__ keys, V* __restrict__ vals,
+                                                               std::uint64_t m, std::uint64_t seed,
+                                                               RadixPasses passes, RadixSet set) {
+  constexpr int kWaves = kKeygenBlock / kWave;
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  ClearNextRadixSet(set);
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+    (&local[0][0][0])[i] = 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kKeygenBlock;
+  for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWave; wbase < m;
+       wbase += stride) {
+    const std::uint64_t i = wbase + LaneId();
+    const bool valid = i < m;
+    const K key = static_cast<K>(SplitMix64(seed + i));
+    if (valid) {
+      keys[i] = key;
+      vals[i] = static_cast<V>(i);
+    }
+    CountDigits<K>(local[wave], passes, key, valid, wbase + kWave <= m);
+  }
+  __syncthreads();
+  FlushDigits<kWaves>(local, passes, set.hist);
+}
+
+// Placement tables of a one-segment list (bucket starts of each pass), for
+// timing the in-place sorts: adj[p][d] = digits below d in pass p.
+__global__ __launch_bounds__(RadixTile::kRadix) void TuneAdjKernel(const unsigned long long* __restrict__ hist,
+                                                                   long long* __restrict__ adj) {
+  constexpr int kRadix = RadixTile::kRadix;
+  __shared__ std::uint64_t scratch[kRadix / kWave];
+  const int p = static_cast<int>(blockIdx.x);
+  std::uint64_t total;
+  const std::uint64_t ex = BlockExclusiveSum<kRadix>(static_cast<std::uint64_t>(hist[p * kRadix + threadIdx.x]),
+                                                     scratch, total);
+  adj[p * kRadix + threadIdx.x] = static_cast<long long>(ex);
+}
+
 // First pass of the first step: key = high half of FillKey of suffix i,
 // value = i, generated from the packed text instead of read.
 template <typename Index>
