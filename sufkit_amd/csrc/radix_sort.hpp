@@ -327,17 +327,24 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     }
   }
   const std::uint64_t wave_base = base + static_cast<std::uint64_t>(wave) * kItems * kWave;
+  // All of the tile's loads first, so that they are in flight together (a
+  // load next to its item's LDS ranking waits out one memory round trip per
+  // item).  Every lane loads (slot 0 when out of range) and then selects:
+  // the branchy form made hipcc spill the 1024 x 16 tile's registers.
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
     const bool valid = idx < end;
-    // Every lane loads (slot 0 when out of range) and then selects: the
-    // branchy form made hipcc spill the 1024 x 16 tile's registers.
     K kk;
     V vv;
     src.Load(valid ? idx : 0, kk, vv);
     key[j] = valid ? kk : K(0);
     val[j] = valid ? vv : V(0);
+  }
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
+    const bool valid = idx < end;
     const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
     digit[j] = d;
     const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);

@@ -998,7 +998,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   // SA and ISA in their stored widths (and, for a batch, per-reference
   // coordinates).
   if (ctx.refs > 0) {
-    const unsigned grid = static_cast<unsigned>(CeilDiv(n, 256));
+    const unsigned grid = static_cast<unsigned>(CeilDiv(n, kLocalizeTile));
     auto localize = [&](auto* out_sa) {
       using OutSa = std::remove_pointer_t<decltype(out_sa)>;
       if (d_isa == nullptr) {

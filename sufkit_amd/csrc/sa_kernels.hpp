@@ -177,29 +177,42 @@ __global__ __launch_bounds__(256) void InitSegmentsKernel(
 }
 
 // Per-reference coordinates: SA values and ISA rows minus the reference's
-// first position, each checked to stay inside its own reference.
+// first position, each checked to stay inside its own reference.  Block b
+// handles kLocalizeTile consecutive rows; the reference of a row follows from
+// the block's first and last reference (usually one), and a value or ISA row
+// only has to be checked against that reference's bounds.
+constexpr int kLocalizeItems = 16;
+constexpr int kLocalizeTile = 256 * kLocalizeItems;
+
 template <typename Index, typename OutSa, typename OutIsa>
 __global__ __launch_bounds__(256) void LocalizeKernel(
     const Index* sa, const Index* rank, std::uint64_t n, const std::uint64_t* __restrict__ offsets,
     std::uint32_t refs, OutSa* out_sa, OutIsa* out_isa, DeviceCounters* counters) {
-  const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const std::uint32_t ref = ReferenceOf(offsets, refs, k);
-  const std::uint64_t begin = offsets[ref], end = offsets[ref + 1];
-  const std::uint64_t v = sa[k];
+  __shared__ std::uint32_t s_ref[2];
+  const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kLocalizeTile;
+  const std::uint64_t last = base + kLocalizeTile < n ? base + kLocalizeTile - 1 : n - 1;
+  if (threadIdx.x < 2) s_ref[threadIdx.x] = SegmentOfSlot(offsets, 0, refs - 1, threadIdx.x == 0 ? base : last);
+  __syncthreads();
+  const std::uint32_t r0 = s_ref[0], r1 = s_ref[1];
   std::uint32_t flags = 0;
-  if (v < begin || v >= end) flags |= kFlagNotPermutation;
-  std::uint64_t isa_v = 0;
-  if (out_isa != nullptr) {
-    isa_v = rank[k];
-    if (isa_v < begin || isa_v >= end) flags |= kFlagNotInverse;
+#pragma unroll 4
+  for (int j = 0; j < kLocalizeItems; ++j) {
+    const std::uint64_t k = base + static_cast<std::uint64_t>(j) * 256 + threadIdx.x;
+    if (k > last) break;
+    const std::uint32_t ref = r0 == r1 ? r0 : SegmentOfSlot(offsets, r0, r1, k);
+    const std::uint64_t begin = offsets[ref], end = offsets[ref + 1];
+    const std::uint64_t v = sa[k];
+    const bool sa_ok = v >= begin && v < end;
+    flags |= sa_ok ? 0u : static_cast<std::uint32_t>(kFlagNotPermutation);
+    if (sa_ok) out_sa[k] = static_cast<OutSa>(v - begin);
+    if (out_isa != nullptr) {
+      const std::uint64_t isa_v = rank[k];
+      const bool isa_ok = isa_v >= begin && isa_v < end;
+      flags |= isa_ok ? 0u : static_cast<std::uint32_t>(kFlagNotInverse);
+      if (isa_ok) out_isa[k] = static_cast<OutIsa>(isa_v - begin);
+    }
   }
-  if (flags) {
-    atomicOr(&counters->error_flags, flags);
-    return;
-  }
-  out_sa[k] = static_cast<OutSa>(v - begin);
-  if (out_isa != nullptr) out_isa[k] = static_cast<OutIsa>(isa_v - begin);
+  if (flags) atomicOr(&counters->error_flags, flags);
 }
 
 // ------------------------------------------------------------ keys --------
