@@ -165,6 +165,87 @@ __device__ __forceinline__ std::uint64_t LookBackWave(unsigned long long* status
   return prefix;
 }
 
+// The onesweep look-back of all kRadix digits of a tile by a whole block of
+// kW * kRadix threads: the kW consecutive lanes of digit d = threadIdx.x / kW
+// each read the status word of a different predecessor tile (lane w reads
+// tile hi - w), so one round trip covers kW tiles; a ballot finds the nearest
+// inclusive word within the group and the group sums the counts up to it.
+// Every thread calls it; returns the digit's exclusive prefix (in all kW
+// lanes).  Publication, epochs and the bounded wait are those of LookBack.
+template <int kW>
+__device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::uint64_t tile, int d,
+                                               std::uint64_t value, std::uint64_t first, std::uint32_t epoch,
+                                               unsigned int* error_flags, std::uint32_t error_bit) {
+  constexpr int kRadix = 256;
+  constexpr std::uint32_t kSpinLimit = 1u << 20;
+  static_assert(kW >= 1 && kW <= 64 && (kW & (kW - 1)) == 0, "digit groups within a wave");
+  const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
+  const int w = static_cast<int>(threadIdx.x) % kW;
+  const std::uint32_t lane = LaneId();
+  const std::uint32_t group_base = lane - static_cast<std::uint32_t>(w);
+  constexpr std::uint64_t kGroupMask = kW == 64 ? ~0ull : ((1ull << kW) - 1);
+  unsigned long long* slot = status + tile * kRadix + static_cast<std::uint64_t>(d);
+  if (tile == 0) {
+    if (w == 0) {
+      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (first + value), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return first;
+  }
+  if (w == 0) {
+    __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  std::uint64_t prefix = 0;
+  std::int64_t hi = static_cast<std::int64_t>(tile) - 1;
+  std::uint32_t spins = 0;
+  bool done = false;
+  while (true) {
+    if (!done) {
+      const std::int64_t t = hi - w;
+      std::uint64_t word = onesweep::kFlagInclusive | ep;  // before tile 0: an inclusive zero
+      if (t >= 0) {
+        word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * kRadix + static_cast<std::uint64_t>(d),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const std::uint64_t flag = word & (3ull << 62);
+      const bool ready = flag != 0 && ((word >> 48) & onesweep::kEpochMask) == (epoch & onesweep::kEpochMask);
+      const bool incl = ready && flag == onesweep::kFlagInclusive;
+      const std::uint64_t rmask = (__ballot(ready) >> group_base) & kGroupMask;
+      const std::uint64_t imask = (__ballot(incl) >> group_base) & kGroupMask;
+      const int first_incl = imask ? __ffsll(static_cast<unsigned long long>(imask)) - 1 : kW;
+      const std::uint64_t need = first_incl < kW ? ((2ull << first_incl) - 1) : kGroupMask;
+      if ((rmask & need) != need) {
+        if (++spins >= kSpinLimit) {
+          atomicOr(error_flags, error_bit);
+          done = true;
+        } else if ((spins & 63u) == 0 &&
+                   (__hip_atomic_load(error_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & error_bit)) {
+          done = true;
+        } else {
+          __builtin_amdgcn_s_sleep(1);
+        }
+      } else {
+        spins = 0;
+        std::uint64_t v = w <= first_incl ? (word & onesweep::kCountMask) : 0ull;
+#pragma unroll
+        for (int o = 1; o < kW; o <<= 1) v += WaveShfl(v, static_cast<int>(lane ^ static_cast<std::uint32_t>(o)));
+        prefix += v;
+        if (first_incl < kW) {
+          done = true;
+        } else {
+          hi -= kW;
+        }
+      }
+    }
+    if (__all(done)) break;
+  }
+  if (w == 0) {
+    __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (prefix + value), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return prefix;
+}
+
 // Digit passes of one sort: the bit offset of each 8-bit digit that varies.
 struct RadixPasses {
   static constexpr int kMax = 8;
@@ -285,6 +366,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   static_assert(!kSegmented || kTile <= 65536, "segment offsets within a tile are 16-bit");
   __shared__ std::uint32_t wave_cnt[kWaves][kRadix];
   __shared__ std::uint32_t local_start[kRadix];
+  __shared__ std::uint32_t tile_cnt[kRadix];
   __shared__ std::uint64_t global_base[kRadix];
   __shared__ std::uint64_t bucket_start[kRadix];
   __shared__ std::uint32_t scan_scratch[kBlock / kWave];
@@ -370,9 +452,17 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       s += c;
     }
     my_count = s;
-    // Publish this tile's count, then look back for the exclusive prefix.
-    global_base[d] = LookBack<false>(status + d, kRadix, tile, s, (!kSegmented && tile == 0) ? bucket_start[d] : 0,
-                                     epoch, error_flags, error_bit);
+    tile_cnt[d] = s;
+  }
+  __syncthreads();
+  {
+    // Publish this tile's counts, then look back for the exclusive prefixes
+    // (the whole block, kBlock / kRadix lanes per digit).
+    constexpr int kW = kBlock / kRadix;
+    const int d = static_cast<int>(threadIdx.x) / kW;
+    const std::uint64_t first = (!kSegmented && tile == 0) ? bucket_start[d] : 0;
+    const std::uint64_t p = LookBackDigits<kW>(status, tile, d, tile_cnt[d], first, epoch, error_flags, error_bit);
+    if (static_cast<int>(threadIdx.x) % kW == 0) global_base[d] = p;
   }
   __syncthreads();
   {

@@ -798,8 +798,13 @@ class Doubling {
     } else {
       // The large list may sit in vals0/vals1 (after a split): Keygen reads
       // slot k of it before writing slot k of vals0, in the same thread.
-      KeygenKernel<Index, K><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
-          l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+      if (rank_step) {
+        KeygenKernel<Index, K, true><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
+            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+      } else {
+        KeygenKernel<Index, K, false><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
+            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+      }
       SFK_HIP_CHECK(hipGetLastError());
       ctx_.stats.algorithmic_bytes_sa +=
           m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);

@@ -240,8 +240,10 @@ __device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const I
 // Sort keys of the large list, key = (segment id << sec_bits) | secondary,
 // and the digit counts of every radix pass over them.
 // act_idx may be the same buffer as vals: slot k is read before slot k is
-// written, by the same thread (hence no __restrict__ on either).
-template <typename Index, typename K>
+// written, by the same thread (hence no __restrict__ on either).  kRank: the
+// secondary key is rank[suffix + h] (sec_kind is kSecRank); otherwise it
+// comes from the suffix's 32-symbol window (FillHi, FillLo, Exact0..2).
+template <typename Index, typename K, bool kRank>
 __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
     const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
@@ -256,20 +258,65 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
   __syncthreads();
   const int wave = threadIdx.x / kWave;
   const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kKeygenBlock;
+  // kUnroll wave spans per step, every load of the step before its stores,
+  // so that the step's gathers are in flight together.  Each slot is still
+  // read before it is written, by the same thread.
+  constexpr int kUnroll = 4;
   for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWave; wbase < m;
-       wbase += stride) {
-    const std::uint64_t k = wbase + LaneId();
-    const bool valid = k < m;
-    K key = 0;
-    if (valid) {
-      const Index v = iota ? static_cast<Index>(k) : act_idx[k];
-      const std::uint64_t sec = SecondaryKey<Index>(sec_kind, t, rank, static_cast<std::uint64_t>(v), h, counters);
-      const std::uint64_t seg = seg_bits ? static_cast<std::uint64_t>(act_seg[k]) : 0ull;
-      key = static_cast<K>((seg << sec_bits) | sec);
-      keys[k] = key;
-      vals[k] = v;
+       wbase += stride * kUnroll) {
+    Index v[kUnroll];
+    std::uint64_t seg[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const std::uint64_t k = wbase + u * stride + LaneId();
+      const bool valid = k < m;
+      v[u] = valid ? (iota ? static_cast<Index>(k) : act_idx[k]) : Index(0);
+      seg[u] = valid && seg_bits ? static_cast<std::uint64_t>(act_seg[k]) : 0ull;
     }
-    CountDigits<K>(local[wave], passes, key, valid, wbase + kWave <= m);
+    std::uint64_t sec[kUnroll];
+    if constexpr (kRank) {
+      // An unsorted suffix never reaches the sentinel within h symbols, so
+      // v + h < n; the guard only protects against a corrupted state.
+      bool bad = false;
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const bool valid = wbase + u * stride + LaneId() < m;
+        const std::uint64_t p = static_cast<std::uint64_t>(v[u]) + h;
+        const bool ok = p < t.n;
+        bad |= valid & !ok;
+        sec[u] = (valid && ok) ? static_cast<std::uint64_t>(rank[ok ? p : 0]) : 0ull;
+      }
+      if (bad) atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
+    } else {
+      uint4 e0[kUnroll], e1[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const std::uint64_t w = static_cast<std::uint64_t>(v[u]) >> 5;
+        e0[u] = t.words[w];
+        e1[u] = t.words[w + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        std::uint64_t c;
+        std::uint32_t sp;
+        WindowOf(e0[u], e1[u], static_cast<std::uint32_t>(v[u] & 31), c, sp);
+        sec[u] = sec_kind == kSecFillHi   ? FillKey(c, sp) >> 32
+                 : sec_kind == kSecFillLo ? FillKey(c, sp) & 0xFFFFFFFFull
+                                          : ExactKeyWord(c, sp, sec_kind - kSecExact0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const std::uint64_t kbase = wbase + u * stride;
+      const std::uint64_t k = kbase + LaneId();
+      const bool valid = k < m;
+      const K key = valid ? static_cast<K>((seg[u] << sec_bits) | sec[u]) : K(0);
+      if (valid) {
+        keys[k] = key;
+        vals[k] = v[u];
+      }
+      CountDigits<K>(local[wave], passes, key, valid, kbase + kWave <= m);
+    }
   }
   __syncthreads();
   FlushDigits<kWaves>(local, passes, set.hist);
@@ -1134,26 +1181,42 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
   Index v[kItems], r0[kItems];
   std::uint64_t codes[kItems];
   std::uint32_t spec[kItems];
+  // Loads of every item first (then the dependent window loads of every
+  // item), so that one thread's loads are in flight together.
+  KeyT key[kItems];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const int k = tid * kItems + i;
-    v[i] = 0;
-    r0[i] = 0;
+    const bool in = k < valid_len;
+    const std::uint64_t at = s0 + (in ? k : 0);
+    v[i] = in ? q_idx[at] : Index(0);
+    r0[i] = in ? q_row0[at] : Index(0);
+    key[i] = 0;
     codes[i] = 0;
     spec[i] = 0;
-    KeyT key = 0;
-    if (k < valid_len) {
-      v[i] = q_idx[s0 + k];
-      r0[i] = q_row0[s0 + k];
-      if constexpr (kExact) {
-        LoadWindow(t, static_cast<std::uint64_t>(v[i]), codes[i], spec[i]);
-        key = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]));
-      } else {
-        key = q_key[s0 + k];
-      }
+    if constexpr (!kExact) key[i] = in ? q_key[at] : KeyT(0);
+  }
+  if constexpr (kExact) {
+    uint4 e0[kItems], e1[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const std::uint64_t w = static_cast<std::uint64_t>(v[i]) >> 5;
+      e0[i] = t.words[w];
+      e1[i] = t.words[w + 1];
     }
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int k = tid * kItems + i;
+      WindowOf(e0[i], e1[i], static_cast<std::uint32_t>(v[i] & 31), codes[i], spec[i]);
+      if (k >= valid_len) codes[i] = 0, spec[i] = 0;
+      key[i] = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int k = tid * kItems + i;
     s_row0[k + 1] = r0[i];
-    s_key[k] = key;
+    s_key[k] = key[i];
     s_count[k] = 0;
   }
   if (tid == 0) s_row0[0] = s0 > 0 ? q_row0[s0 - 1] : Index(0);
