@@ -387,9 +387,8 @@ const SectionDescriptor& ParsedContainer::Require(SectionType type) const {
 
 std::vector<std::uint8_t> BuildHeader(const ContainerSpec& spec,
                                       const std::vector<SectionDescriptor>& sections) {
-  std::vector<std::uint8_t> h;
+  std::vector<std::uint8_t> h(kMagic.begin(), kMagic.end());
   h.reserve(kBaseHeaderSize + sections.size() * kSectionEntrySize);
-  h.insert(h.end(), kMagic.begin(), kMagic.end());
   AppendU16(h, 1);
   AppendU16(h, spec.format_minor);
   h.push_back(1);  // little endian
