@@ -105,6 +105,23 @@ def radix_tuning(args, max_elements: int):
     return {"source": source, "seconds": round(seconds, 3), "max_elements": max_elements, "chosen": chosen}
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "sa_build_64mbp_pmc_traffic.md"
+
+
+def measured_traffic() -> dict:
+    """HBM bytes per RadixOnesweep launch of the default workload, as measured
+    by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; calibrated against a
+    device copy of known size) and committed with their commands."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("onesweep_traffic_bytes_per_launch:"):
+                return {"traffic": int(line.split(":", 1)[1]),
+                        "traffic_source": "profiles/sa_build_64mbp_pmc_traffic.md"}
+    except OSError:
+        pass
+    return {}
+
+
 def cpu_info() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -403,6 +420,8 @@ def main() -> int:
             "avg_launch_us": round(scatter_s / scatter_n * 1e6, 2),
             "bytes_per_launch": scatter_b // scatter_n,
         }
+        if args.workload == 2 and args.scale == 1.0 and bits == 32:
+            roofline.update(measured_traffic())
 
     result = {
         "metric": "SA+ISA+LCP build Mbp/s + HBM roofline frac, 1 MI355X vs divsufsort64 on host",
