@@ -164,7 +164,37 @@ def cpu_baseline(text: np.ndarray, sample_symbols: int, width_bits: int = 32):
                    "SA + ISA + Kasai LCP "
                    f"(detail::BuildDivsufsort, 1 thread) on {cpu_info()}"),
         "seconds": round(dt, 3),
-    }, (sa, isa, lcp, sample is text)
+    }, (sa, isa, lcp, sample is text, sample)
+
+
+def host_threads() -> int:
+    """Host cores this process may use, capped at the 16 a one-GPU box
+    allots."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline_caps(sample: np.ndarray, width_bits: int, whole: bool):
+    """The reference's CaPS-SA backend (SA + merge LCP) plus a threaded ISA
+    scatter, on all host threads, on the same sample as the divsufsort
+    baseline."""
+    import oracle
+
+    threads = host_threads()
+    bases = int(np.count_nonzero(sample > 1))
+    t0 = time.perf_counter()
+    sa, isa, lcp, (t_sa_lcp, t_isa) = oracle.ref_caps_fast(sample, width_bits, threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(bases / 1e6 / dt, 3), "unit": "Mbp/s", "cores": threads, "kind": "reference",
+        "sample": (f"{'whole' if whole else 'prefix of the'} workload text, {sample.size} symbols ({bases} bases); "
+                   f"CaPS-SA {width_bits}-bit SA + merge LCP (detail::BuildCaps{width_bits}) + ISA scatter, "
+                   f"{threads} threads on {cpu_info()}"),
+        "seconds": round(dt, 3), "sa_lcp_seconds": round(t_sa_lcp, 3), "isa_seconds": round(t_isa, 3),
+    }, (sa, isa, lcp)
 
 
 def sample_check(text: np.ndarray, sa_rows: np.ndarray, sa_prev: np.ndarray, lcp_rows: np.ndarray) -> int:
@@ -464,12 +494,21 @@ def main() -> int:
 
     if not args.no_cpu_baseline:
         sample_symbols = args.cpu_sample_symbols or (0 if n <= (1 << 27) else 64_000_000)
-        base, (rsa, risa, rlcp, whole) = cpu_baseline(text, sample_symbols, bits)
+        base, (rsa, risa, rlcp, whole, sample) = cpu_baseline(text, sample_symbols, bits)
         result["cpu_baseline"] = base
         if whole and full_copy:
             ok = (np.array_equal(sa.astype(np.uint64), rsa) and np.array_equal(isa.astype(np.uint64), risa)
                   and np.array_equal(lcp.astype(np.uint64), rlcp))
             result["parity_vs_cpu_baseline"] = "bit-exact" if ok else "MISMATCH"
+        del rsa, risa, rlcp
+        import oracle
+        if oracle.ref_caps_available() and sample.size >= 16:
+            caps, (csa, cisa, clcp) = cpu_baseline_caps(sample, bits, whole)
+            result["cpu_baseline_caps"] = caps
+            if whole and full_copy:
+                ok = (np.array_equal(sa.astype(np.uint64), csa) and np.array_equal(isa.astype(np.uint64), cisa)
+                      and np.array_equal(lcp.astype(np.uint64), clcp))
+                result["parity_vs_cpu_baseline_caps"] = "bit-exact" if ok else "MISMATCH"
     else:
         result["cpu_baseline"] = None
 
@@ -477,7 +516,8 @@ def main() -> int:
     for ptr in (d_text, d_sa, d_isa, d_lcp):
         hip.sfk_hip_free(ptr)
     print(json.dumps(result), flush=True)
-    ok = result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact" and sample_bad == 0
+    ok = (result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact"
+          and result.get("parity_vs_cpu_baseline_caps", "bit-exact") == "bit-exact" and sample_bad == 0)
     return 0 if ok else 1
 
 

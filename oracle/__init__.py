@@ -11,7 +11,10 @@ Two layers:
   * _ref/libsufkit_ref.so: the reference's own divsufsort backend
     (detail::BuildDivsufsort32/64 over libdivsufsort 2.0.2) compiled from
     the reference sources by oracle/Makefile, when those sources were
-    present at build time.
+    present at build time;
+  * _ref/libsufkit_refcaps.so: the reference's CaPS-SA backend
+    (detail::BuildCaps32/64, multi-threaded) compiled the same way, the
+    multi-core CPU baseline.
 """
 from __future__ import annotations
 
@@ -36,7 +39,7 @@ def build(ref: str | None = "/root/reference") -> None:
     args = ["make", "-s", "-C", str(HERE), "oracle"]
     subprocess.run(args, check=True, env=env)
     if ref and Path(ref, "src", "divsufsort_backend.cpp").exists():
-        subprocess.run(["make", "-s", "-j8", "-C", str(HERE), "ref", "ref-full", f"REF={ref}"],
+        subprocess.run(["make", "-s", "-j8", "-C", str(HERE), "ref", "ref-caps", "ref-full", f"REF={ref}"],
                        check=True, env=env)
 
 
@@ -89,6 +92,36 @@ def ref() -> ctypes.CDLL:
         R.sfk_ref_divsufsort.restype = ctypes.c_int
         _REF = R
     return _REF
+
+
+_CAPS = None
+
+
+def ref_caps_available() -> bool:
+    return (HERE / "_ref" / "libsufkit_refcaps.so").exists()
+
+
+def ref_caps_fast(text: np.ndarray, width_bits: int, threads: int):
+    """SA, ISA, LCP (uint64) from the reference's CaPS-SA backend with
+    `threads` threads, and its (SA+LCP, ISA) seconds."""
+    global _CAPS
+    if _CAPS is None:
+        path = HERE / "_ref" / "libsufkit_refcaps.so"
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (reference sources absent at build time)")
+        C = ctypes.CDLL(str(path))
+        C.sfk_ref_caps_fast.argtypes = [U8P, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, U64P, U64P, U64P,
+                                        ctypes.POINTER(ctypes.c_double)]
+        C.sfk_ref_caps_fast.restype = ctypes.c_longlong
+        _CAPS = C
+    text, p = _u8(text)
+    n = text.size
+    sa, isa_, lcp = np.empty(n, np.uint64), np.empty(n, np.uint64), np.empty(n, np.uint64)
+    secs = (ctypes.c_double * 2)()
+    rows = _CAPS.sfk_ref_caps_fast(p, n, width_bits, threads, _u64(sa), _u64(isa_), _u64(lcp), secs)
+    if rows != n:
+        raise RuntimeError("reference CaPS-SA build failed")
+    return sa, isa_, lcp, (secs[0], secs[1])
 
 
 def _u8(a):

@@ -130,3 +130,15 @@ def test_oracle_matches_reference_digests(reference_digests, name):
     assert f"0x{oracle.digest(sa):016x}" == want["sa"]
     assert f"0x{oracle.digest(isa):016x}" == want["isa"]
     assert f"0x{oracle.digest(lcp):016x}" == want["lcp"]
+
+
+@pytest.mark.skipif(not oracle.ref_caps_available(), reason="compiled reference CaPS-SA not built")
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reference_caps_matches_the_oracle(bits):
+    # The multi-core CPU baseline agrees with the restatement on ACGT,
+    # 7-symbol and run-heavy texts.
+    for text in (synth.random_text(41, 3000), synth.random_symbols(42, 2500),
+                 synth.run_text(43, [("rand", 40), ("A", 700), ("tandem7", 400), ("sep", 1), ("N", 90)])):
+        sa, isa_, lcp, _ = oracle.ref_caps_fast(text, bits, 4)
+        osa, oisa, olcp = oracle.fast_build(text, "doubling")
+        assert np.array_equal(sa, osa) and np.array_equal(isa_, oisa) and np.array_equal(lcp, olcp)
