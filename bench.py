@@ -478,6 +478,17 @@ def main() -> int:
             "sa_seconds": round(last["sa_seconds"], 6),
             "isa_seconds": round(last["isa_seconds"], 6),
             "lcp_seconds": round(last["lcp_seconds"], 6),
+            "per_phase": {
+                ph: {"seconds": round(last[f"{ph}_seconds"], 6),
+                     "algorithmic_bytes": last[f"algorithmic_bytes_{ph}"],
+                     "GBps": round(last[f"algorithmic_bytes_{ph}"] / max(last[f"{ph}_seconds"], 1e-12) / 1e9, 1),
+                     "frac": round(last[f"algorithmic_bytes_{ph}"] / max(last[f"{ph}_seconds"], 1e-12) / 1e9
+                                   / HBM_PEAK_GBPS, 4)}
+                for ph in ("sa", "isa", "lcp")
+            },
+            "per_phase_note": ("isa: the ISA is the SA phase's rank array, written in place by its scatter "
+                               "(those bytes count in sa); the isa phase holds only stored-width conversion "
+                               "or the permutation check when there is no LCP pass"),
             "algorithmic_bytes": algo_bytes,
             "whole_build_GBps": round(algo_bytes / device_s / 1e9, 1),
             "whole_build_frac": round(algo_bytes / device_s / 1e9 / HBM_PEAK_GBPS, 4),
