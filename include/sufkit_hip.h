@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SFK_HIP_ABI_VERSION 1
+#define SFK_HIP_ABI_VERSION 2
 
 enum sfk_hip_status {
   SFK_HIP_OK = 0,
@@ -67,6 +67,7 @@ typedef struct sfk_hip_stats {
   double sort_scatter_seconds;
   uint64_t sort_scatter_launches;
   uint64_t sort_scatter_bytes;
+  double prefix_directory_seconds; /* sfk_hip_build_fast's directory pass */
 } sfk_hip_stats;
 
 typedef struct sfk_hip_builder sfk_hip_builder;
@@ -139,6 +140,37 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits,
                   sfk_hip_stats* stats, char* error, size_t error_capacity);
 
 /*
+ * Fast prefix directory: the exact k-mer -> SA-interval table that
+ * SuffixArray::Build and Load derive for a Fast index (BuildFastPrefix,
+ * suffix_array.cpp:2087-2107; FastPrefixIndex::Build/SelectK,
+ * fast_prefix_index.cpp:286-382), built on the device from a resident SA.
+ * Entry `key` (the k-mer's 2-bit codes A=0..T=3, first symbol most
+ * significant) holds the rows [begin, end) of the suffixes that start with
+ * that k-mer over A/C/G/T, or (n, n) when none does.  For n <= 2^32 - 1 an
+ * entry is one word (begin | end << 32); otherwise the table is two planes
+ * of 4^k words (all begins, then all ends).
+ *   sfk_prefix_directory_select_k: the k SelectK picks with the default
+ *     options (0: no directory) for suffix_count rows and the index's
+ *     resident core bytes (text + SA + ISA + LCP);
+ *   sfk_prefix_directory_words: table size in 64-bit words;
+ *   sfk_hip_prefix_directory: from a device text and SA into a device table
+ *     (counts[0] = indexed k-mers, counts[1] = non-empty entries);
+ *   sfk_hip_build_fast: sfk_hip_build plus the directory of its SA into a
+ *     host table, computed while the SA is still on the device
+ *     (prefix_k = 0: none; sfk_hip_build is this call with 0).
+ */
+uint32_t sfk_prefix_directory_select_k(uint64_t suffix_count, uint64_t resident_core_bytes);
+uint64_t sfk_prefix_directory_words(uint32_t k, uint64_t suffix_count);
+int sfk_hip_prefix_directory(sfk_hip_builder* builder, const uint8_t* d_text, uint64_t n,
+                             const void* d_sa, uint32_t sa_bits, uint32_t k, uint64_t* d_table,
+                             uint64_t counts[2], char* error, size_t error_capacity);
+int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bits,
+                       uint32_t flags, void* sa, uint32_t sa_bits, void* isa,
+                       uint32_t isa_bits, void* lcp, uint32_t lcp_bits, uint32_t prefix_k,
+                       uint64_t* prefix_table, uint64_t prefix_counts[2],
+                       sfk_hip_stats* stats, char* error, size_t error_capacity);
+
+/*
  * Resident index for exact count queries (serving).  The index borrows the
  * caller's device text (n symbols, sentinel last) and SA (sa_bits 32 or 64)
  * and packs the text into its own 2-bit planes; the caller keeps d_text and
@@ -163,6 +195,9 @@ int sfk_hip_index_generate_queries(sfk_hip_index* index, uint64_t seed,
                                    uint64_t count, uint32_t min_len,
                                    uint32_t max_len, uint64_t* d_patterns,
                                    uint8_t* d_lengths, void* stream);
+/* k of the index's prefix directory (0: none); count queries of at least
+ * k symbols search only the SA interval of their first k symbols. */
+uint32_t sfk_hip_index_prefix_k(const sfk_hip_index* index);
 int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns,
                         const uint8_t* d_lengths, uint64_t count,
                         uint32_t* d_counts, void* stream);

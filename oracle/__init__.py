@@ -69,6 +69,8 @@ def lib() -> ctypes.CDLL:
         L.sfk_oracle_crc32.restype = ctypes.c_uint32
         L.sfk_oracle_digest_u64.argtypes = [U64P, ctypes.c_uint64]
         L.sfk_oracle_digest_u64.restype = ctypes.c_uint64
+        L.sfk_oracle_prefix_directory.argtypes = [U8P, ctypes.c_uint64, U64P, ctypes.c_uint32, U64P, U64P, U64P]
+        L.sfk_oracle_prefix_directory.restype = ctypes.c_uint64
         _LIB = L
     return _LIB
 
@@ -92,6 +94,75 @@ def ref() -> ctypes.CDLL:
         R.sfk_ref_divsufsort.restype = ctypes.c_int
         _REF = R
     return _REF
+
+
+def prefix_select_k(suffix_count: int, resident_core_bytes: int) -> int:
+    """FastPrefixIndex::SelectK with default options, restated
+    (fast_prefix_index.cpp:286-307): none below 2^16 suffixes, else the
+    largest k in [8, 10] whose 4^k entries of two row-width coordinates fit
+    25% of the resident core bytes."""
+    if suffix_count < (1 << 16):
+        return 0
+    budget = (resident_core_bytes // 10000) * 2500 + ((resident_core_bytes % 10000) * 2500) // 10000
+    row_bytes = 4 if suffix_count <= 0xFFFFFFFF else 8
+    for k in (10, 9, 8):
+        if (1 << (2 * k)) * 2 * row_bytes <= budget:
+            return k
+    return 0
+
+
+def prefix_directory(text: np.ndarray, isa_: np.ndarray, k: int):
+    """(begin, end, indexed k-mers, non-empty entries) of the Fast prefix
+    directory, from the restatement in oracle.c."""
+    text, p = _u8(text)
+    isa_ = np.ascontiguousarray(isa_, dtype=np.uint64)
+    begin = np.empty(1 << (2 * k), np.uint64)
+    end = np.empty(1 << (2 * k), np.uint64)
+    nonempty = np.zeros(1, np.uint64)
+    indexed = lib().sfk_oracle_prefix_directory(p, text.size, _u64(isa_), k, _u64(begin), _u64(end), _u64(nonempty))
+    return begin, end, int(indexed), int(nonempty[0])
+
+
+def contigs_of(text: np.ndarray):
+    """Contig starts and lengths of a logical text (separators end contigs;
+    the last contig ends before the sentinel)."""
+    seps = np.flatnonzero(np.asarray(text) == 1)
+    ends = np.concatenate([seps, [text.size - 1]]) if (seps.size == 0 or seps[-1] != text.size - 2) else seps
+    starts = np.concatenate([[0], ends[:-1] + 1]).astype(np.uint64)
+    return starts, (ends - starts).astype(np.uint64)
+
+
+def ref_prefix_directory(text: np.ndarray, sa: np.ndarray, isa_: np.ndarray, resident_core_bytes: int):
+    """The reference's own FastPrefixIndex over the same arrays: (k, begin,
+    end, indexed, nonempty), or None when the reference writer library was
+    not built."""
+    if not ref_writer_available():
+        return None
+    W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
+    W.sfk_ref_prefix_directory.argtypes = [U8P, ctypes.c_uint64, U64P, U64P, U64P, U64P, ctypes.c_uint64,
+                                           ctypes.c_uint64, U64P, U64P, U64P]
+    W.sfk_ref_prefix_directory.restype = ctypes.c_int
+    text, p = _u8(text)
+    sa = np.ascontiguousarray(sa, np.uint64)
+    isa_ = np.ascontiguousarray(isa_, np.uint64)
+    starts, lengths = contigs_of(text)
+    begin = np.empty(1 << 20, np.uint64)
+    end = np.empty(1 << 20, np.uint64)
+    counts = np.zeros(2, np.uint64)
+    k = W.sfk_ref_prefix_directory(p, text.size, _u64(sa), _u64(isa_), _u64(starts), _u64(lengths), starts.size,
+                                   resident_core_bytes, _u64(begin), _u64(end), _u64(counts))
+    if k < 0:
+        raise RuntimeError("reference FastPrefixIndex::Build failed")
+    return k, begin[: 1 << (2 * k)], end[: 1 << (2 * k)], int(counts[0]), int(counts[1])
+
+
+def ref_prefix_select_k(suffix_count: int, resident_core_bytes: int):
+    if not ref_writer_available():
+        return None
+    W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
+    W.sfk_ref_prefix_select_k.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    W.sfk_ref_prefix_select_k.restype = ctypes.c_uint32
+    return int(W.sfk_ref_prefix_select_k(suffix_count, resident_core_bytes))
 
 
 _CAPS = None

@@ -242,3 +242,40 @@ uint64_t sfk_oracle_digest_u64(const uint64_t* values, uint64_t count) {
   }
   return h;
 }
+
+/* Fast prefix directory of a complete SA, restated from
+ * fast_prefix_index.cpp:137-176 (PopulatePartition) and :90-135
+ * (RecordRow): every position p whose k symbols are all A/C/G/T (codes
+ * 2..5) widens the interval of its k-mer key (2-bit codes, first symbol
+ * most significant) to cover row ISA[p]; keys never seen hold (n, n).
+ * Returns the number of indexed positions; *nonempty the number of keys
+ * seen. */
+uint64_t sfk_oracle_prefix_directory(const uint8_t* text, uint64_t n, const uint64_t* isa, uint32_t k,
+                                     uint64_t* begin, uint64_t* end, uint64_t* nonempty) {
+  const uint64_t entries = (uint64_t)1 << (2 * k);
+  uint64_t indexed = 0, seen = 0;
+  for (uint64_t key = 0; key < entries; ++key) begin[key] = end[key] = n;
+  for (uint64_t p = 0; p + k <= n; ++p) {
+    uint64_t key = 0;
+    int canonical = 1;
+    for (uint32_t i = 0; i < k && canonical; ++i) {
+      const uint8_t c = text[p + i];
+      canonical = c >= 2 && c <= 5;
+      key = (key << 2) | (uint64_t)(c - 2);
+    }
+    if (!canonical) continue;
+    const uint64_t row = isa[p];
+    if (begin[key] == n) {
+      begin[key] = row;
+      end[key] = row + 1;
+      ++seen;
+    } else {
+      if (row < begin[key]) begin[key] = row;
+      if (row + 1 > end[key]) end[key] = row + 1;
+    }
+    ++indexed;
+  }
+  if (nonempty) *nonempty = seen;
+  return indexed;
+}
+

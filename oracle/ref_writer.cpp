@@ -15,6 +15,9 @@
 
 #include <sufkit/sufkit.hpp>
 
+#include "coordinate_storage.hpp"
+#include "fast_prefix_index.hpp"
+
 extern "C" {
 
 // acceleration: SaAcceleration value; coordinate_width 0/32/64; storage 0/32/40/48/64.
@@ -46,6 +49,51 @@ int sfk_ref_write_sufidx(const char* const* names, const char* const* sequences,
     }
     return -1;
   }
+}
+
+// The reference's own FastPrefixIndex (fast_prefix_index.cpp:286-382) over
+// a complete SA/ISA with default options: its k (from resident_core_bytes),
+// and for every k-mer key its Lookup interval ((n, n) when absent), plus
+// the indexed k-mers and non-empty entries.  Returns k, or -1 on error.
+int sfk_ref_prefix_directory(const std::uint8_t* text, std::uint64_t n, const std::uint64_t* sa,
+                             const std::uint64_t* isa, const std::uint64_t* contig_starts,
+                             const std::uint64_t* contig_lengths, std::uint64_t contigs,
+                             std::uint64_t resident_core_bytes, std::uint64_t* begin, std::uint64_t* end,
+                             std::uint64_t* counts) {
+  try {
+    namespace d = sufkit::detail;
+    const std::vector<std::uint8_t> t(text, text + n);
+    auto sa_store = d::CoordinateStorage::FromUInt64(std::vector<std::uint64_t>(sa, sa + n),
+                                                     sufkit::CoordinateStorageWidth::kBits64, n);
+    auto isa_store = d::CoordinateStorage::FromUInt64(std::vector<std::uint64_t>(isa, isa + n),
+                                                      sufkit::CoordinateStorageWidth::kBits64, n);
+    const std::vector<sufkit::Position> starts(contig_starts, contig_starts + contigs);
+    const std::vector<sufkit::Position> lengths(contig_lengths, contig_lengths + contigs);
+    const auto index = d::FastPrefixIndex::Build(t, sa_store, isa_store, starts, lengths, resident_core_bytes);
+    const std::uint32_t k = index.K();
+    if (k > 0) {
+      std::vector<std::uint8_t> pattern(k);
+      for (std::uint64_t key = 0; key < (std::uint64_t{1} << (2 * k)); ++key) {
+        for (std::uint32_t i = 0; i < k; ++i) {
+          pattern[i] = static_cast<std::uint8_t>(2 + ((key >> (2 * (k - 1 - i))) & 3u));  // A=2 .. T=5
+        }
+        const auto interval = index.Lookup(pattern);
+        const bool present = interval && !interval->Empty();
+        begin[key] = present ? interval->begin : n;
+        end[key] = present ? interval->end : n;
+      }
+      counts[0] = index.IndexedKmers();
+      counts[1] = index.NonemptyEntries();
+    }
+    return static_cast<int>(k);
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// FastPrefixIndex::SelectK with default options.
+std::uint32_t sfk_ref_prefix_select_k(std::uint64_t suffix_count, std::uint64_t resident_core_bytes) {
+  return sufkit::detail::FastPrefixIndex::SelectK(suffix_count, resident_core_bytes);
 }
 
 }  // extern "C"

@@ -61,6 +61,7 @@ class HipStats(ctypes.Structure):
         ("sort_scatter_seconds", ctypes.c_double),
         ("sort_scatter_launches", ctypes.c_uint64),
         ("sort_scatter_bytes", ctypes.c_uint64),
+        ("prefix_directory_seconds", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -389,6 +390,89 @@ def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
         raise SufkitError(status, err.value.decode(errors="replace") or "serving failed")
     samples = (sp.reshape(count, -1), sl.reshape(count, -1), sc.reshape(count, -1)) if sample_queries else None
     return out, summ, samples
+
+
+def _prefix_lib():
+    lib = load_hip()
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib.sfk_prefix_directory_select_k.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    lib.sfk_prefix_directory_select_k.restype = ctypes.c_uint32
+    lib.sfk_prefix_directory_words.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
+    lib.sfk_prefix_directory_words.restype = ctypes.c_uint64
+    lib.sfk_hip_prefix_directory.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, u64p,
+                                             ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_prefix_directory.restype = ctypes.c_int
+    lib.sfk_hip_build_fast.argtypes = [
+        ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+        ctypes.c_uint32, u64p, u64p, ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_build_fast.restype = ctypes.c_int
+    lib.sfk_hip_index_prefix_k.argtypes = [ctypes.c_void_p]
+    lib.sfk_hip_index_prefix_k.restype = ctypes.c_uint32
+    return lib
+
+
+def prefix_select_k(suffix_count: int, resident_core_bytes: int) -> int:
+    """k of the Fast prefix directory for an index (0: none)."""
+    return int(_prefix_lib().sfk_prefix_directory_select_k(suffix_count, resident_core_bytes))
+
+
+def prefix_directory(text: np.ndarray, sa: np.ndarray, k: int):
+    """Device prefix directory of a text and its SA (uint32 or uint64 rows):
+    (begin, end, indexed k-mers, non-empty entries) as uint64 arrays of 4^k."""
+    lib = _prefix_lib()
+    text = np.ascontiguousarray(text, np.uint8)
+    sa = np.ascontiguousarray(sa)
+    n = text.size
+    bits = 8 * sa.dtype.itemsize
+    words = int(lib.sfk_prefix_directory_words(k, n))
+    d_text, d_sa, d_table = DeviceArray(n + 64), DeviceArray(sa.nbytes), DeviceArray(words * 8)
+    d_text.upload(text)
+    d_sa.upload(sa)
+    builder = DeviceBuilder(0)
+    counts = (ctypes.c_uint64 * 2)()
+    err = ctypes.create_string_buffer(512)
+    try:
+        status = lib.sfk_hip_prefix_directory(builder.handle, d_text.ptr, n, d_sa.ptr, bits, k, d_table.ptr,
+                                              counts, err, len(err))
+        if status != 0:
+            raise SufkitError(status, err.value.decode(errors="replace"))
+        table = d_table.download(np.uint64, words)
+    finally:
+        builder.close()
+    return (*_split_directory(table, k, n), int(counts[0]), int(counts[1]))
+
+
+def _split_directory(table: np.ndarray, k: int, n: int):
+    entries = 1 << (2 * k)
+    if n <= 0xFFFFFFFF:
+        return table & np.uint64(0xFFFFFFFF), table >> np.uint64(32)
+    return table[:entries], table[entries:]
+
+
+def hip_build_fast(text: np.ndarray, prefix_k: int, construction_bits: int = 32):
+    """sfk_hip_build_fast: SA/ISA/LCP (uint32) plus the prefix directory
+    computed from the device SA: (sa, isa, lcp, begin, end, indexed,
+    nonempty, stats)."""
+    lib = _prefix_lib()
+    text = np.ascontiguousarray(text, np.uint8)
+    n = text.size
+    sa, isa, lcp = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.uint32)
+    words = int(lib.sfk_prefix_directory_words(prefix_k, n))
+    table = np.zeros(max(words, 1), np.uint64)
+    counts = (ctypes.c_uint64 * 2)()
+    stats = HipStats()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_build_fast(text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits,
+                                    SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, sa.ctypes.data, 32, isa.ctypes.data,
+                                    32, lcp.ctypes.data, 32, prefix_k,
+                                    table.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), counts,
+                                    ctypes.byref(stats), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+    begin, end = _split_directory(table[:words], prefix_k, n) if words else (None, None)
+    return sa, isa, lcp, begin, end, int(counts[0]), int(counts[1]), stats.as_dict()
 
 
 class TuneEntry(ctypes.Structure):

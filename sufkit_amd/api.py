@@ -31,7 +31,7 @@ class _BuildStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in (
         "sa_seconds", "storage_compaction_seconds", "isa_seconds", "lcp_seconds",
         "text_prepare_seconds", "lcp_finalize_seconds", "total_seconds", "device_seconds",
-        "transfer_seconds")]
+        "transfer_seconds", "prefix_directory_seconds")]
 
 
 class _IndexInfo(ctypes.Structure):
@@ -42,7 +42,7 @@ class _IndexInfo(ctypes.Structure):
         (name, ctypes.c_uint64) for name in (
             "sequence_count", "total_bases", "text_symbols", "suffix_count", "ambiguous_bases",
             "fingerprint", "serialized_bytes", "text_bytes", "sa_bytes", "isa_bytes", "lcp_bytes",
-            "resident_core_bytes")]
+            "resident_core_bytes", "auxiliary_bytes")] + [("prefix_directory_k", ctypes.c_uint32)]
 
 
 _BOUND = False

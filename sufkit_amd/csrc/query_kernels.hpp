@@ -7,6 +7,7 @@
 // whose suffix is > it.  One 32-symbol window load answers each comparison.
 #pragma once
 
+#include "block_scan.hpp"
 #include "device_common.hpp"
 
 namespace sfk {
@@ -34,11 +35,111 @@ __device__ inline int ComparePattern(const PackedText& t, std::uint64_t s, std::
   return kind == 3u ? 1 : -1;
 }
 
+// ------------------------------------------------- fast prefix directory
+//
+// The exact k-mer directory of a Fast index (fast_prefix_index.cpp:322-382,
+// built by SuffixArray::Build and Load, suffix_array.cpp:2087-2107): for each
+// k-mer over A/C/G/T (key = 2-bit codes, first symbol most significant) the
+// SA interval [begin, end) of the suffixes that start with it; an absent
+// k-mer holds (n, n).  Suffixes starting with one k-mer are consecutive SA
+// rows, so one pass over the SA finds every interval: the row where a k-mer
+// starts writes begin, the row where it ends writes end (each entry is
+// written by exactly two rows, no atomics).  Rows up to 2^32 - 1 pack an
+// entry into one word (begin low, end high); larger ones use two planes of
+// 4^k words (begin, then end).
+constexpr std::uint64_t kNoDirectoryKey = ~0ull;
+constexpr int kDirectoryItems = 8;
+constexpr int kDirectoryTile = 256 * kDirectoryItems;
+
+__device__ inline std::uint64_t DirectoryKey(const PackedText& t, std::uint64_t p, int k) {
+  std::uint64_t c;
+  std::uint32_t sp;
+  LoadWindow(t, p, c, sp);
+  // The sentinel is special, so no canonical window reaches past the text.
+  const std::uint32_t smask = ~(~0u >> k);
+  return (sp & smask) ? kNoDirectoryKey : c >> (64 - 2 * k);
+}
+
+__global__ __launch_bounds__(256) void PrefixDirectoryFillKernel(std::uint64_t* __restrict__ table,
+                                                                 std::uint64_t words, std::uint64_t value) {
+  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < words) table[i] = value;
+}
+
+// counts[0] += rows with a canonical k-mer (indexed k-mers), counts[1] +=
+// distinct k-mers (non-empty entries).
+template <typename SaT>
+__global__ __launch_bounds__(256) void PrefixDirectoryKernel(PackedText t, const SaT* __restrict__ sa,
+                                                             std::uint64_t n, int k, bool wide,
+                                                             std::uint64_t* __restrict__ table,
+                                                             unsigned long long* __restrict__ counts) {
+  __shared__ std::uint64_t s_key[kDirectoryTile + 2];  // s_key[i + 1] = row base + i
+  __shared__ std::uint64_t scratch[256 / kWave];
+  const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kDirectoryTile;
+  const int len = static_cast<int>(n - base < static_cast<std::uint64_t>(kDirectoryTile) ? n - base
+                                                                                          : kDirectoryTile);
+  std::uint64_t p[kDirectoryItems];
+#pragma unroll
+  for (int j = 0; j < kDirectoryItems; ++j) {
+    const int i = j * 256 + static_cast<int>(threadIdx.x);
+    p[j] = i < len ? static_cast<std::uint64_t>(sa[base + i]) : 0ull;
+  }
+#pragma unroll
+  for (int j = 0; j < kDirectoryItems; ++j) {
+    const int i = j * 256 + static_cast<int>(threadIdx.x);
+    if (i < len) s_key[i + 1] = p[j] < n ? DirectoryKey(t, p[j], k) : kNoDirectoryKey;
+  }
+  if (threadIdx.x == 0) {
+    const std::uint64_t before = base > 0 ? static_cast<std::uint64_t>(sa[base - 1]) : n;
+    const std::uint64_t after = base + len < n ? static_cast<std::uint64_t>(sa[base + len]) : n;
+    s_key[0] = before < n ? DirectoryKey(t, before, k) : kNoDirectoryKey;
+    s_key[len + 1] = after < n ? DirectoryKey(t, after, k) : kNoDirectoryKey;
+  }
+  __syncthreads();
+  const std::uint64_t entries = 1ull << (2 * k);
+  std::uint64_t indexed = 0, distinct = 0;
+#pragma unroll
+  for (int j = 0; j < kDirectoryItems; ++j) {
+    const int i = j * 256 + static_cast<int>(threadIdx.x);
+    if (i >= len) break;
+    const std::uint64_t key = s_key[i + 1];
+    if (key == kNoDirectoryKey) continue;
+    const std::uint64_t row = base + static_cast<std::uint64_t>(i);
+    ++indexed;
+    if (s_key[i] != key) {
+      ++distinct;
+      if (wide) {
+        table[key] = row;
+      } else {
+        reinterpret_cast<std::uint32_t*>(table)[2 * key] = static_cast<std::uint32_t>(row);
+      }
+    }
+    if (s_key[i + 2] != key) {
+      if (wide) {
+        table[entries + key] = row + 1;
+      } else {
+        reinterpret_cast<std::uint32_t*>(table)[2 * key + 1] = static_cast<std::uint32_t>(row + 1);
+      }
+    }
+  }
+  std::uint64_t total_indexed, total_distinct;
+  BlockExclusiveSum<256>(indexed, scratch, total_indexed);
+  BlockExclusiveSum<256>(distinct, scratch, total_distinct);
+  if (threadIdx.x == 0) {
+    if (total_indexed) atomicAdd(&counts[0], static_cast<unsigned long long>(total_indexed));
+    if (total_distinct) atomicAdd(&counts[1], static_cast<unsigned long long>(total_distinct));
+  }
+}
+
+// Count queries; a directory (k > 0) narrows both binary searches to the
+// SA interval of the pattern's first k symbols when the pattern has that
+// many (an absent k-mer counts 0 without searching).
 template <typename SaT>
 __global__ __launch_bounds__(256) void CountPatternsKernel(
     PackedText t, const SaT* __restrict__ sa, std::uint64_t n,
     const std::uint64_t* __restrict__ patterns, const std::uint8_t* __restrict__ lengths,
-    std::uint64_t count, std::uint32_t* __restrict__ out) {
+    std::uint64_t count, std::uint32_t* __restrict__ out, const std::uint64_t* __restrict__ directory,
+    int dir_k, bool dir_wide) {
   const std::uint64_t q = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= count) return;
   const std::uint64_t p = patterns[q];
@@ -48,12 +149,28 @@ __global__ __launch_bounds__(256) void CountPatternsKernel(
     return;
   }
   std::uint64_t lo = 0, hi = n;
+  if (directory != nullptr && len >= static_cast<std::uint32_t>(dir_k)) {
+    const std::uint64_t key = p >> (64 - 2 * dir_k);
+    if (dir_wide) {
+      lo = directory[key];
+      hi = directory[(1ull << (2 * dir_k)) + key];
+    } else {
+      const std::uint64_t e = directory[key];
+      lo = e & 0xFFFFFFFFull;
+      hi = e >> 32;
+    }
+    if (lo >= n) {
+      out[q] = 0;
+      return;
+    }
+  }
+  const std::uint64_t range_end = hi;
   while (lo < hi) {
     const std::uint64_t mid = lo + ((hi - lo) >> 1);
     if (ComparePattern(t, sa[mid], p, len) < 0) lo = mid + 1; else hi = mid;
   }
   const std::uint64_t first = lo;
-  hi = n;
+  hi = range_end;
   while (lo < hi) {
     const std::uint64_t mid = lo + ((hi - lo) >> 1);
     if (ComparePattern(t, sa[mid], p, len) <= 0) lo = mid + 1; else hi = mid;

@@ -1294,12 +1294,81 @@ int sfk_hip_build_batch_device(sfk_hip_builder* b, const uint8_t* d_text, uint64
   }
 }
 
+// ------------------------------------------------- fast prefix directory
+//
+// Selection of k follows FastPrefixIndex::SelectK with its default options
+// (fast_prefix_index.cpp:286-307, .hpp FastPrefixIndexOptions): no directory
+// below 2^16 suffixes; otherwise the largest k in [8, 10] whose table
+// (4^k entries of two row-width coordinates) fits 25% of the resident core
+// bytes, else none.
+namespace sfk {
+namespace {
+
+constexpr std::uint32_t kDirectoryMinK = 8;
+constexpr std::uint32_t kDirectoryMaxK = 10;
+constexpr std::uint64_t kDirectoryMinSuffixes = 1ull << 16;
+constexpr std::uint64_t kDirectoryBudgetBasisPoints = 2500;
+
+bool DirectoryWide(std::uint64_t suffix_count) { return suffix_count > 0xFFFFFFFFull; }
+
+std::uint64_t DirectoryBytesFor(std::uint32_t k, std::uint64_t suffix_count) {
+  if (k == 0 || k > kDirectoryMaxK) return 0;
+  return (1ull << (2 * k)) * 2 * (DirectoryWide(suffix_count) ? 8 : 4);
+}
+
+std::uint32_t SelectDirectoryK(std::uint64_t suffix_count, std::uint64_t resident_core_bytes) {
+  if (suffix_count < kDirectoryMinSuffixes) return 0;
+  // 25% of the core bytes, split around the division as the policy does.
+  const std::uint64_t budget = (resident_core_bytes / 10000) * kDirectoryBudgetBasisPoints +
+                               ((resident_core_bytes % 10000) * kDirectoryBudgetBasisPoints) / 10000;
+  for (std::uint32_t k = kDirectoryMaxK; k >= kDirectoryMinK; --k) {
+    if (DirectoryBytesFor(k, suffix_count) <= budget) return k;
+  }
+  return 0;
+}
+
+// Fills the table and runs the directory pass on `stream`; counts (2 words
+// of device memory) receive the indexed k-mers and the non-empty entries.
+void LaunchPrefixDirectory(hipStream_t stream, PackedText t, const void* d_sa, std::uint32_t sa_bits,
+                           std::uint64_t n, std::uint32_t k, std::uint64_t* table, unsigned long long* counts) {
+  const bool wide = DirectoryWide(n);
+  const std::uint64_t entries = 1ull << (2 * k);
+  const std::uint64_t words = wide ? 2 * entries : entries;
+  const std::uint64_t empty = wide ? n : (n | (n << 32));
+  PrefixDirectoryFillKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, stream>>>(table, words, empty);
+  SFK_HIP_CHECK(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), stream));
+  const unsigned grid = static_cast<unsigned>(CeilDiv(n, kDirectoryTile));
+  if (sa_bits == 32) {
+    PrefixDirectoryKernel<std::uint32_t><<<grid, 256, 0, stream>>>(
+        t, static_cast<const std::uint32_t*>(d_sa), n, static_cast<int>(k), wide, table, counts);
+  } else {
+    PrefixDirectoryKernel<std::uint64_t><<<grid, 256, 0, stream>>>(
+        t, static_cast<const std::uint64_t*>(d_sa), n, static_cast<int>(k), wide, table, counts);
+  }
+  SFK_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace sfk
+
 int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t flags,
                   void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
                   uint32_t lcp_bits, sfk_hip_stats* stats, char* error, size_t error_capacity) {
+  return sfk_hip_build_fast(text, n, construction_bits, flags, sa, sa_bits, isa, isa_bits, lcp, lcp_bits, 0,
+                            nullptr, nullptr, stats, error, error_capacity);
+}
+
+int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t flags,
+                       void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
+                       uint32_t lcp_bits, uint32_t prefix_k, uint64_t* prefix_table, uint64_t prefix_counts[2],
+                       sfk_hip_stats* stats, char* error, size_t error_capacity) {
   std::string message;
   if (text == nullptr || sa == nullptr) {
     sfk::CopyError("text and SA buffers are required", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (prefix_k > sfk::kDirectoryMaxK || (prefix_k > 0 && prefix_table == nullptr)) {
+    sfk::CopyError("prefix directory k must be 0..10 with a table for k > 0", error, error_capacity);
     return SFK_HIP_INVALID_INPUT;
   }
   int status = sfk::ValidateArgs(n, construction_bits, sa_bits, isa, isa_bits, lcp, lcp_bits,
@@ -1328,6 +1397,27 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, u
                                  d_sa.As<void>(), sa_bits, want_isa ? d_isa.As<void>() : nullptr,
                                  isa_bits, want_lcp ? d_lcp.As<void>() : nullptr, lcp_bits,
                                  &local, message);
+    sfk::DeviceBuffer d_table;
+    if (status == SFK_HIP_OK && prefix_k > 0) {
+      // The directory from the device SA while it is resident (the packed
+      // text of the build is still in the builder).
+      const std::uint64_t words = sfk_prefix_directory_words(prefix_k, n);
+      d_table.Reserve(words * 8);
+      b->partial.Reserve(1024 * sizeof(sfk::SplitPartial));
+      auto* dcounts = b->partial.As<unsigned long long>();
+      const auto td = std::chrono::steady_clock::now();
+      sfk::LaunchPrefixDirectory(b->stream, sfk::PackedText{b->packed.As<uint4>(), n}, d_sa.As<void>(), sa_bits, n,
+                                 prefix_k, d_table.As<std::uint64_t>(), dcounts);
+      unsigned long long host[2] = {0, 0};
+      SFK_HIP_CHECK(hipMemcpyAsync(host, dcounts, sizeof(host), hipMemcpyDeviceToHost, b->stream));
+      SFK_HIP_CHECK(hipStreamSynchronize(b->stream));
+      local.prefix_directory_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
+      SFK_HIP_CHECK(hipMemcpy(prefix_table, d_table.As<void>(), words * 8, hipMemcpyDeviceToHost));
+      if (prefix_counts != nullptr) {
+        prefix_counts[0] = host[0];
+        prefix_counts[1] = host[1];
+      }
+    }
     const auto t2 = std::chrono::steady_clock::now();
     SFK_HIP_CHECK(hipMemcpy(sa, d_sa.As<void>(), n * (sa_bits / 8), hipMemcpyDeviceToHost));
     if (want_isa) SFK_HIP_CHECK(hipMemcpy(isa, d_isa.As<void>(), n * (isa_bits / 8), hipMemcpyDeviceToHost));
@@ -1347,6 +1437,53 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, u
   return status;
 }
 
+uint32_t sfk_prefix_directory_select_k(uint64_t suffix_count, uint64_t resident_core_bytes) {
+  return sfk::SelectDirectoryK(suffix_count, resident_core_bytes);
+}
+
+uint64_t sfk_prefix_directory_words(uint32_t k, uint64_t suffix_count) {
+  if (k == 0 || k > sfk::kDirectoryMaxK) return 0;
+  return (1ull << (2 * k)) * (sfk::DirectoryWide(suffix_count) ? 2 : 1);
+}
+
+int sfk_hip_prefix_directory(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n, const void* d_sa,
+                             uint32_t sa_bits, uint32_t k, uint64_t* d_table, uint64_t counts[2], char* error,
+                             size_t error_capacity) {
+  if (b == nullptr || d_text == nullptr || d_sa == nullptr || d_table == nullptr || n < 2 || k < 1 ||
+      k > sfk::kDirectoryMaxK || (sa_bits != 32 && sa_bits != 64)) {
+    sfk::CopyError("prefix directory needs a builder, a text, its SA (32/64-bit), a table and 1 <= k <= 10",
+                   error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  try {
+    SFK_HIP_CHECK(hipSetDevice(b->device));
+    const std::uint64_t words = sfk::PackedWords(n);
+    b->packed.Reserve(words * 16);
+    b->counters.Reserve(sizeof(sfk::DeviceCounters));
+    sfk::ResetCounters(b);
+    sfk::PackTextKernel<<<static_cast<unsigned>(sfk::CeilDiv(words, 256)), 256, 0, b->stream>>>(
+        d_text, n, words, b->packed.As<uint4>(), (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, false,
+        b->counters.As<sfk::DeviceCounters>());
+    SFK_HIP_CHECK(hipGetLastError());
+    b->partial.Reserve(1024 * sizeof(sfk::SplitPartial));
+    auto* dcounts = b->partial.As<unsigned long long>();
+    sfk::LaunchPrefixDirectory(b->stream, sfk::PackedText{b->packed.As<uint4>(), n}, d_sa, sa_bits, n, k, d_table,
+                               dcounts);
+    unsigned long long host[2] = {0, 0};
+    SFK_HIP_CHECK(hipMemcpyAsync(host, dcounts, sizeof(host), hipMemcpyDeviceToHost, b->stream));
+    const sfk::DeviceCounters& c = sfk::ReadCounters(b);
+    if (c.error_flags != 0) sfk::Fail(SFK_HIP_INVALID_INPUT, "text must hold symbols 0..6 with its only 0 last");
+    if (counts != nullptr) {
+      counts[0] = host[0];
+      counts[1] = host[1];
+    }
+    return SFK_HIP_OK;
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    return f.status;
+  }
+}
+
 // ------------------------------------------------------ resident index ---
 
 struct sfk_hip_index {
@@ -1356,7 +1493,8 @@ struct sfk_hip_index {
   uint64_t n = 0;
   const void* d_sa = nullptr;
   uint32_t sa_bits = 32;
-  sfk::DeviceBuffer packed, counters;
+  sfk::DeviceBuffer packed, counters, directory, directory_counts;
+  uint32_t directory_k = 0;  // 0: no directory
   sfk::DeviceCounters* host_counters = nullptr;
 };
 
@@ -1407,6 +1545,17 @@ int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const vo
     if (index->host_counters->error_flags != 0) {
       sfk::Fail(SFK_HIP_INVALID_INPUT, "index text must hold symbols 0..6 with its only 0 last");
     }
+    // The Fast profile's prefix directory, sized for the resident core
+    // arrays of a served index (text, SA, ISA and LCP of the SA's width).
+    index->directory_k = sfk::SelectDirectoryK(n, n * (1 + 3 * (sa_bits / 8)));
+    if (index->directory_k > 0) {
+      index->directory.Reserve(sfk_prefix_directory_words(index->directory_k, n) * 8);
+      index->directory_counts.Reserve(2 * sizeof(unsigned long long));
+      sfk::LaunchPrefixDirectory(index->stream, sfk::PackedText{index->packed.As<uint4>(), n}, d_sa, sa_bits, n,
+                                 index->directory_k, index->directory.As<std::uint64_t>(),
+                                 index->directory_counts.As<unsigned long long>());
+      SFK_HIP_CHECK(hipStreamSynchronize(index->stream));
+    }
     *out = index;
     return SFK_HIP_OK;
   } catch (const sfk::Failure& f) {
@@ -1444,12 +1593,17 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const 
   if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
   sfk::PackedText t{index->packed.As<uint4>(), index->n};
   const unsigned grid = static_cast<unsigned>(sfk::CeilDiv(count, 256));
+  const std::uint64_t* dir = index->directory_k > 0 ? index->directory.As<std::uint64_t>() : nullptr;
+  const int dir_k = static_cast<int>(index->directory_k);
+  const bool dir_wide = sfk::DirectoryWide(index->n);
   if (index->sa_bits == 32) {
     sfk::CountPatternsKernel<uint32_t><<<grid, 256, 0, s>>>(
-        t, static_cast<const uint32_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts);
+        t, static_cast<const uint32_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts, dir,
+        dir_k, dir_wide);
   } else {
     sfk::CountPatternsKernel<uint64_t><<<grid, 256, 0, s>>>(
-        t, static_cast<const uint64_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts);
+        t, static_cast<const uint64_t*>(index->d_sa), index->n, d_patterns, d_lengths, count, d_counts, dir,
+        dir_k, dir_wide);
   }
   if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
   if (stream == nullptr && hipStreamSynchronize(s) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
@@ -1457,7 +1611,11 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const 
 }
 
 uint64_t sfk_hip_index_device_bytes(const sfk_hip_index* index) {
-  return index == nullptr ? 0 : index->packed.Bytes();
+  return index == nullptr ? 0 : index->packed.Bytes() + index->directory.Bytes();
+}
+
+uint32_t sfk_hip_index_prefix_k(const sfk_hip_index* index) {
+  return index == nullptr ? 0 : index->directory_k;
 }
 
 int sfk_hip_malloc(int device, uint64_t bytes, void** out) {

@@ -56,7 +56,7 @@ struct sfk_build_options {
 
 struct sfk_build_stats {
   double sa_seconds, storage_compaction_seconds, isa_seconds, lcp_seconds, text_prepare_seconds,
-      lcp_finalize_seconds, total_seconds, device_seconds, transfer_seconds;
+      lcp_finalize_seconds, total_seconds, device_seconds, transfer_seconds, prefix_directory_seconds;
 };
 
 struct sfk_index_info {
@@ -66,7 +66,8 @@ struct sfk_index_info {
   uint8_t coordinate_width, stored_coordinate_width, acceleration, lcp_encoding;
   uint32_t sa_sampling_rate;
   uint64_t sequence_count, total_bases, text_symbols, suffix_count, ambiguous_bases, fingerprint,
-      serialized_bytes, text_bytes, sa_bytes, isa_bytes, lcp_bytes, resident_core_bytes;
+      serialized_bytes, text_bytes, sa_bytes, isa_bytes, lcp_bytes, resident_core_bytes, auxiliary_bytes;
+  uint32_t prefix_directory_k;
 };
 
 int sfk_reference_from_records(const char* const* names, const char* const* sequences,
@@ -140,7 +141,8 @@ int sfk_sa_build(const sfk_reference* r, const sfk_build_options* o, sfk_suffix_
     if (stats != nullptr) {
       *stats = sfk_build_stats{s.sa_seconds, s.storage_compaction_seconds, s.isa_seconds,
                                s.lcp_seconds, s.text_prepare_seconds, s.lcp_finalize_seconds,
-                               s.total_seconds, s.device_seconds, s.transfer_seconds};
+                               s.total_seconds, s.device_seconds, s.transfer_seconds,
+                               s.prefix_directory_seconds};
     }
   });
 }
@@ -183,6 +185,8 @@ void sfk_sa_info(const sfk_suffix_array* sa, sfk_index_info* out) {
   out->isa_bytes = i.isa_bytes;
   out->lcp_bytes = i.lcp_bytes;
   out->resident_core_bytes = i.resident_core_bytes;
+  out->auxiliary_bytes = i.auxiliary_bytes;
+  out->prefix_directory_k = sa->value.PrefixDirectoryK();
 }
 
 // Decoded arrays (uint64) of the resident index; return the element count.

@@ -60,6 +60,10 @@ struct SuffixArray::Impl {
   detail::CoordinateStorage sa;
   detail::CoordinateStorage isa;
   detail::LcpStorage lcp;
+  // Fast prefix directory (k-mer -> SA interval), derived after build and
+  // load, never serialized (fast_prefix_index.hpp): k = 0 when absent.
+  std::uint32_t prefix_k = 0;
+  std::vector<std::uint64_t> prefix_table;
   IndexInfo info;
 
   void RefreshInfo() {
@@ -82,8 +86,24 @@ struct SuffixArray::Impl {
     info.sa_bytes = sa.Bytes();
     info.isa_bytes = isa.Bytes();
     info.lcp_bytes = lcp.Bytes();
-    info.resident_core_bytes = info.text_bytes + info.sa_bytes + info.isa_bytes + info.lcp_bytes;
+    // The directory's k is chosen from the core arrays alone; its bytes
+    // then count as auxiliary and resident (suffix_array.cpp:2097-2106).
+    info.auxiliary_bytes = prefix_table.size() * sizeof(std::uint64_t);
+    info.resident_core_bytes = CoreBytes() + info.auxiliary_bytes;
   }
+
+  std::uint64_t CoreBytes() const noexcept { return text.size() + sa.Bytes() + isa.Bytes() + lcp.Bytes(); }
+
+  // BuildFastPrefix's conditions (suffix_array.cpp:2087-2091): the Fast
+  // profile over every row with an ISA (no learned model exists here).
+  bool WantsPrefixDirectory(SaResourceProfile profile) const noexcept {
+    return profile == SaResourceProfile::kFast && sampling_rate == 1 && !isa.Empty();
+  }
+
+  // The directory of a loaded index, computed on the device from the host
+  // text and SA (FastPrefixIndex::Build runs in Load too,
+  // suffix_array.cpp:5242); left absent when no device is visible.
+  void BuildPrefixDirectoryOnDevice();
 };
 
 namespace {
@@ -134,20 +154,26 @@ void SampleSaLcp(std::vector<S>& sa, std::vector<L>* lcp, std::uint32_t k) {
 
 template <class SaT, class IsaT, class LcpT>
 void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& options,
-                  CoordinateStorageWidth stored_width, bool retain_isa, bool retain_lcp,
+                  CoordinateStorageWidth stored_width, bool retain_isa, bool retain_lcp, std::uint32_t prefix_k,
                   SuffixArrayBuildStatistics& stats, const std::function<void(const char*)>& notify) {
   const std::uint64_t n = impl.text.size();
   std::vector<SaT> sa(n);
   std::vector<IsaT> isa(retain_isa || options.sampling_rate > 1 ? n : 0);
   std::vector<LcpT> lcp(retain_lcp ? n : 0);
+  std::vector<std::uint64_t> table(prefix_k > 0 ? sfk_prefix_directory_words(prefix_k, n) : 0);
   sfk_hip_stats hs{};
   char err[512] = {0};
   const std::uint32_t flags = (retain_isa ? SFK_HIP_RETAIN_ISA : 0u) | (retain_lcp ? SFK_HIP_RETAIN_LCP : 0u);
-  const int status = sfk_hip_build(impl.text.data(), n, impl.construction_width, flags, sa.data(),
-                                   8 * sizeof(SaT), isa.empty() ? nullptr : isa.data(), 8 * sizeof(IsaT),
-                                   lcp.empty() ? nullptr : lcp.data(), 8 * sizeof(LcpT), &hs, err,
-                                   sizeof(err));
+  std::uint64_t prefix_counts[2] = {0, 0};
+  const int status = sfk_hip_build_fast(impl.text.data(), n, impl.construction_width, flags, sa.data(),
+                                        8 * sizeof(SaT), isa.empty() ? nullptr : isa.data(), 8 * sizeof(IsaT),
+                                        lcp.empty() ? nullptr : lcp.data(), 8 * sizeof(LcpT), prefix_k,
+                                        table.empty() ? nullptr : table.data(), prefix_counts, &hs, err,
+                                        sizeof(err));
   if (status != SFK_HIP_OK) detail::Throw(FromStatus(status), std::string("HIP backend: ") + err);
+  impl.prefix_k = prefix_k;
+  impl.prefix_table = std::move(table);
+  stats.prefix_directory_seconds = hs.prefix_directory_seconds;
   stats.sa_seconds = hs.sa_seconds;
   stats.isa_seconds = hs.isa_seconds;
   stats.lcp_seconds = hs.lcp_seconds;
@@ -256,24 +282,40 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
   const std::uint64_t rows = 1 + (n - 1) / options.sampling_rate;
   const bool narrow_isa =
       detail::ResolveAuxiliaryStorageWidth(stored_width, rows) == CoordinateStorageWidth::kBits32;
+  // The Fast prefix directory is computed by the same device build while
+  // its SA is resident; its k comes from the core bytes this build will
+  // hold (text, SA, ISA, raw LCP in their resident widths), checked below.
+  std::uint32_t prefix_k = 0;
+  if (options.resource_profile == SaResourceProfile::kFast && options.sampling_rate == 1 && retain_isa) {
+    const std::uint64_t sa_row = stored_width == CoordinateStorageWidth::kBits32   ? 4
+                                 : stored_width == CoordinateStorageWidth::kBits40 ? 5
+                                 : stored_width == CoordinateStorageWidth::kBits48 ? 6
+                                                                                   : 8;
+    const std::uint64_t core = n + n * sa_row + n * (narrow_isa ? 4 : 8) +
+                               (retain_lcp ? n * (wide_lcp ? 8 : 4) : 0);
+    prefix_k = sfk_prefix_directory_select_k(n, core);
+  }
   // Device outputs already in the resident widths: SA 32 or 64 (40/48 are
   // split on the host from 64), ISA in the auxiliary width, LCP 32/64.
   if (narrow_sa && narrow_isa && !wide_lcp) {
     BuildWithHip<std::uint32_t, std::uint32_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
-                                                              retain_lcp, stats, notify);
+                                                              retain_lcp, prefix_k, stats, notify);
   } else if (narrow_sa && !wide_lcp) {
     BuildWithHip<std::uint32_t, std::uint64_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
-                                                              retain_lcp, stats, notify);
+                                                              retain_lcp, prefix_k, stats, notify);
   } else if (!wide_lcp) {
     BuildWithHip<std::uint64_t, std::uint64_t, std::uint32_t>(*impl, options, stored_width, retain_isa,
-                                                              retain_lcp, stats, notify);
+                                                              retain_lcp, prefix_k, stats, notify);
   } else {
     BuildWithHip<std::uint64_t, std::uint64_t, std::uint64_t>(*impl, options, stored_width, retain_isa,
-                                                              retain_lcp, stats, notify);
+                                                              retain_lcp, prefix_k, stats, notify);
+  }
+  notify("index-prefix-directory");
+  if (prefix_k != 0 && prefix_k != sfk_prefix_directory_select_k(n, impl->CoreBytes())) {
+    detail::Throw(ErrorCode::kBuildFailure, "Fast prefix directory sized for different resident bytes");
   }
   impl->RefreshInfo();
   impl->info.sa_resource_profile = options.resource_profile;
-  notify("index-prefix-directory");
   stats.total_seconds = Since(total_begin);
   notify("index-build-return");
   return SuffixArray(std::move(impl));
@@ -531,7 +573,62 @@ SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
                                std::to_string(container.spec.library_minor) + "." +
                                std::to_string(container.spec.library_patch);
   impl->info.serialized_bytes = container.file_size;
+  if (impl->WantsPrefixDirectory(container.spec.profile) && HipBackendAvailable()) {
+    impl->BuildPrefixDirectoryOnDevice();
+    impl->RefreshInfo();
+    impl->info.sa_resource_profile = container.spec.profile;
+    impl->info.format_version = "1." + std::to_string(container.spec.format_minor);
+    impl->info.library_version = std::to_string(container.spec.library_major) + "." +
+                                 std::to_string(container.spec.library_minor) + "." +
+                                 std::to_string(container.spec.library_patch);
+    impl->info.serialized_bytes = container.file_size;
+  }
   return SuffixArray(std::move(impl));
+}
+
+void SuffixArray::Impl::BuildPrefixDirectoryOnDevice() {
+  const std::uint64_t n = text.size();
+  const std::uint32_t k = sfk_prefix_directory_select_k(n, CoreBytes());
+  prefix_k = 0;
+  prefix_table.clear();
+  if (k == 0) return;
+  const std::uint32_t bits = n - 1 <= std::numeric_limits<std::uint32_t>::max() ? 32 : 64;
+  std::vector<std::uint64_t> sa64;
+  std::vector<std::uint32_t> sa32;
+  if (bits == 32) {
+    sa32.resize(n);
+    for (std::uint64_t r = 0; r < n; ++r) sa32[r] = static_cast<std::uint32_t>(sa.At(r));
+  } else {
+    sa64.resize(n);
+    for (std::uint64_t r = 0; r < n; ++r) sa64[r] = sa.At(r);
+  }
+  const std::uint64_t words = sfk_prefix_directory_words(k, n);
+  std::vector<std::uint64_t> table(words);
+  sfk_hip_builder* b = nullptr;
+  void *d_text = nullptr, *d_sa = nullptr, *d_table = nullptr;
+  char err[512] = {0};
+  int status = sfk_hip_builder_create(0, &b, err, sizeof(err));
+  if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, n + 64, &d_text);
+  if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, n * (bits / 8), &d_sa);
+  if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, words * 8, &d_table);
+  if (status == SFK_HIP_OK) status = sfk_hip_memcpy_h2d(d_text, text.data(), n);
+  if (status == SFK_HIP_OK) {
+    status = sfk_hip_memcpy_h2d(d_sa, bits == 32 ? static_cast<const void*>(sa32.data()) : sa64.data(),
+                                n * (bits / 8));
+  }
+  std::uint64_t counts[2] = {0, 0};
+  if (status == SFK_HIP_OK) {
+    status = sfk_hip_prefix_directory(b, static_cast<const std::uint8_t*>(d_text), n, d_sa, bits, k,
+                                      static_cast<std::uint64_t*>(d_table), counts, err, sizeof(err));
+  }
+  if (status == SFK_HIP_OK) status = sfk_hip_memcpy_d2h(table.data(), d_table, words * 8);
+  for (void* p : {d_text, d_sa, d_table}) {
+    if (p != nullptr) sfk_hip_free(p);
+  }
+  sfk_hip_builder_destroy(b);
+  if (status != SFK_HIP_OK) detail::Throw(FromStatus(status), std::string("Fast prefix directory: ") + err);
+  prefix_k = k;
+  prefix_table = std::move(table);
 }
 
 const IndexInfo& SuffixArray::Info() const noexcept { return impl_->info; }
@@ -552,6 +649,20 @@ std::uint64_t SuffixArray::LcpAt(std::uint64_t row) const {
   return impl_->lcp.At(row);
 }
 const std::vector<std::uint8_t>& SuffixArray::Text() const noexcept { return impl_->text; }
+
+std::uint32_t SuffixArray::PrefixDirectoryK() const noexcept { return impl_->prefix_k; }
+
+std::pair<std::uint64_t, std::uint64_t> SuffixArray::PrefixDirectoryInterval(std::uint64_t key) const {
+  const std::uint32_t k = impl_->prefix_k;
+  if (k == 0 || key >= (std::uint64_t{1} << (2 * k))) {
+    detail::Throw(ErrorCode::kInvalidInput, "no prefix directory entry for this key");
+  }
+  const auto& table = impl_->prefix_table;
+  if (impl_->text.size() - 1 <= std::numeric_limits<std::uint32_t>::max()) {
+    return {table[key] & 0xFFFFFFFFull, table[key] >> 32};
+  }
+  return {table[key], table[(std::uint64_t{1} << (2 * k)) + key]};
+}
 
 std::uint64_t SuffixArray::Count(std::string_view pattern) const {
   if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "pattern must not be empty");
@@ -579,12 +690,29 @@ std::uint64_t SuffixArray::Count(std::string_view pattern) const {
     return 0;
   };
   std::uint64_t lo = 0, hi = impl_->sa.Size();
+  // The Fast prefix directory gives the interval of the first k symbols
+  // (FastPrefixRange, suffix_array.cpp:2173-2190); an absent k-mer counts 0.
+  const std::uint32_t k = impl_->prefix_k;
+  if (k > 0 && p.size() >= k) {
+    std::uint64_t key = 0;
+    for (std::uint32_t i = 0; i < k; ++i) key = (key << 2) | static_cast<std::uint64_t>(p[i] - detail::kA);
+    const auto& table = impl_->prefix_table;
+    if (n - 1 <= std::numeric_limits<std::uint32_t>::max()) {
+      lo = table[key] & 0xFFFFFFFFull;
+      hi = table[key] >> 32;
+    } else {
+      lo = table[key];
+      hi = table[(std::uint64_t{1} << (2 * k)) + key];
+    }
+    if (lo >= impl_->sa.Size()) return 0;
+  }
+  const auto range_end = hi;
   while (lo < hi) {  // first row with suffix >= pattern
     const auto mid = lo + (hi - lo) / 2;
     if (cmp(impl_->sa.At(mid)) < 0) lo = mid + 1; else hi = mid;
   }
   const auto first = lo;
-  hi = impl_->sa.Size();
+  hi = range_end;
   while (lo < hi) {  // first row with suffix > pattern (prefix matches count as equal)
     const auto mid = lo + (hi - lo) / 2;
     if (cmp(impl_->sa.At(mid)) <= 0) lo = mid + 1; else hi = mid;

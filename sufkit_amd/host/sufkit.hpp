@@ -17,6 +17,7 @@
 #include <stdexcept>
 #include <string>
 #include <string_view>
+#include <utility>
 #include <vector>
 
 namespace sufkit {
@@ -80,6 +81,7 @@ struct SuffixArrayBuildStatistics {
   double text_prepare_seconds = 0.0;
   double lcp_finalize_seconds = 0.0;
   double total_seconds = 0.0;
+  double prefix_directory_seconds = 0.0;
   // Device-side detail of the HIP backend.
   double device_seconds = 0.0;
   double transfer_seconds = 0.0;
@@ -126,6 +128,7 @@ struct IndexInfo {
   std::uint64_t sa_bytes = 0;
   std::uint64_t isa_bytes = 0;
   std::uint64_t lcp_bytes = 0;
+  std::uint64_t auxiliary_bytes = 0;  // the Fast prefix directory
   std::uint64_t resident_core_bytes = 0;
 };
 
@@ -178,10 +181,15 @@ class SuffixArray {
   std::uint64_t IsaAt(std::uint64_t sample) const;
   std::uint64_t LcpAt(std::uint64_t row) const;
   const std::vector<std::uint8_t>& Text() const noexcept;  // incl. sentinel
-  // Exact occurrence count of an A/C/G/T pattern (two binary searches;
-  // suffix_array.cpp:4413-4430 semantics: patterns with other symbols
-  // match nothing).
+  // Exact occurrence count of an A/C/G/T pattern (two binary searches,
+  // inside the prefix directory's interval when the pattern is at least k
+  // long; suffix_array.cpp:4413-4430 semantics: patterns with other
+  // symbols match nothing).
   std::uint64_t Count(std::string_view pattern) const;
+  // k of the Fast prefix directory (0: none), and its SA interval [begin,
+  // end) for a k-mer key (2-bit codes, first symbol most significant).
+  std::uint32_t PrefixDirectoryK() const noexcept;
+  std::pair<std::uint64_t, std::uint64_t> PrefixDirectoryInterval(std::uint64_t key) const;
 
   struct Impl;
 

@@ -1,0 +1,166 @@
+"""Fast prefix directory (k-mer -> SA interval): the selection rule and the
+restatement pinned against the reference's own FastPrefixIndex (CPU), and
+the device directory, the host index and the served index against them
+(GPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import sufkit_amd
+from sufkit_amd import api
+from tests import synth
+
+
+def texts():
+    # Above ~161K symbols, since a table (k >= 8) needs 25% of 13 bytes per
+    # symbol of text, SA, ISA and LCP to reach 512 KiB.
+    return [synth.repeat_text(31, 250000, cover=0.5, n_runs=3, contigs=5),
+            synth.run_text(32, [("rand", 120000), ("A", 3000), ("sep", 1), ("N", 500), ("tandem7", 4000),
+                                ("sep", 1), ("rand", 90000), ("tandem2", 2000)])]
+
+
+@pytest.mark.parametrize("n,core", [(1000, 10**9), (65535, 10**9), (65536, 1), (65536, 13 * 65536),
+                                    (10**6, 13 * 10**6), (10**6, 10**12), (2**32 - 1, 10**10),
+                                    (2**32, 10**10), (2**32, 10**11), (123456789, 4 * 8 * 2**20 - 1)])
+def test_select_k_matches_the_reference(n, core):
+    k = oracle.prefix_select_k(n, core)
+    assert sufkit_amd.prefix_select_k(n, core) == k
+    ref = oracle.ref_prefix_select_k(n, core)
+    if ref is not None:
+        assert ref == k
+
+
+@pytest.mark.skipif(not oracle.ref_writer_available(), reason="compiled reference library not built")
+@pytest.mark.parametrize("core_per_symbol", [13, 100, 1000])
+def test_restatement_matches_the_reference_directory(core_per_symbol):
+    for text in texts():
+        sa, isa_, _ = oracle.fast_build(text, "doubling")
+        k, rb, re_, rix, rne = oracle.ref_prefix_directory(text, sa, isa_, core_per_symbol * text.size)
+        assert k == oracle.prefix_select_k(text.size, core_per_symbol * text.size) > 0
+        b, e, ix, ne = oracle.prefix_directory(text, isa_, k)
+        assert np.array_equal(b, rb) and np.array_equal(e, re_) and (ix, ne) == (rix, rne)
+
+
+def test_restatement_catches_a_shifted_row():
+    # A plausible mistake (rows off by one) fails the comparison.
+    text = texts()[1]
+    sa, isa_, _ = oracle.fast_build(text, "doubling")
+    b, e, _, _ = oracle.prefix_directory(text, isa_, 8)
+    b2, e2, _, _ = oracle.prefix_directory(text, isa_ + 1, 8)
+    assert not np.array_equal(b, b2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [8, 9, 10])
+@pytest.mark.parametrize("wide", [False, True])
+def test_device_directory_matches_the_oracle(hip, k, wide):
+    for text in texts():
+        sa, isa_, _, _ = hip.hip_build(text)
+        b, e, ix, ne = hip.prefix_directory(text, sa.astype(np.uint64) if wide else sa, k)
+        ob, oe, oix, one = oracle.prefix_directory(text, isa_.astype(np.uint64), k)
+        assert np.array_equal(b, ob) and np.array_equal(e, oe) and (ix, ne) == (oix, one)
+
+
+@pytest.mark.gpu
+def test_build_fast_returns_the_directory_of_its_sa(hip):
+    text = texts()[0]
+    k = hip.prefix_select_k(text.size, 13 * text.size)
+    sa, isa_, lcp, b, e, ix, ne, stats = hip.hip_build_fast(text, k)
+    osa, oisa, olcp = oracle.fast_build(text)
+    assert np.array_equal(sa.astype(np.uint64), osa) and np.array_equal(lcp.astype(np.uint64), olcp)
+    ob, oe, oix, one = oracle.prefix_directory(text, oisa, k)
+    assert np.array_equal(b, ob) and np.array_equal(e, oe) and (ix, ne) == (oix, one)
+    assert stats["prefix_directory_seconds"] > 0
+
+
+@pytest.mark.gpu
+def test_host_index_counts_through_the_directory(hip):
+    text = texts()[0]
+    ref = api.GenomeReference.from_encoded(text, synth.contig_lengths(text))
+    index = api.SuffixArray.build(ref)
+    info = index.info()
+    k = info["prefix_directory_k"]
+    core = info["text_bytes"] + info["sa_bytes"] + info["isa_bytes"] + info["lcp_bytes"]
+    assert k == oracle.prefix_select_k(text.size, core) > 0
+    assert info["auxiliary_bytes"] == 8 * (1 << (2 * k))
+    assert info["resident_core_bytes"] == core + info["auxiliary_bytes"]
+    raw = "".join("ACGTN"[c - 2] if c >= 2 else "|" for c in text[:-1])
+    rng = np.random.default_rng(3)
+    for length in (1, k - 1, k, k + 1, 20):
+        for _ in range(12):
+            if rng.random() < 0.6:
+                start = int(rng.integers(0, text.size - 40))
+                p = "".join(ch if ch in "ACGT" else "A" for ch in raw[start:start + length])
+            else:
+                p = "".join("ACGT"[i] for i in rng.integers(0, 4, size=length))
+            want, at = 0, raw.find(p)
+            while at != -1:
+                want += 1
+                at = raw.find(p, at + 1)
+            assert index.count(p) == want, p
+    # a saved and loaded index rebuilds the same directory
+    loaded_k = None
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "x.sufidx")
+        index.save(path)
+        loaded = api.SuffixArray.load(path)
+        loaded_k = loaded.info()["prefix_directory_k"]
+        assert loaded.count("ACGTACGTAC") == index.count("ACGTACGTAC")
+    assert loaded_k == k
+
+
+@pytest.mark.gpu
+def test_served_index_counts_through_the_directory(hip):
+    text = texts()[1]
+    n = text.size
+    d_text = hip.DeviceArray(n + 64)
+    d_text.upload(text)
+    sa, _, _, _ = hip.hip_build(text)
+    d_sa = hip.DeviceArray(4 * n)
+    d_sa.upload(sa)
+    lib = hip._prefix_lib()
+    lib.sfk_hip_index_create.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p,
+                                         ctypes.c_size_t]
+    lib.sfk_hip_index_count.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+    lib.sfk_hip_index_destroy.argtypes = [ctypes.c_void_p]
+    index = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(256)
+    assert lib.sfk_hip_index_create(0, d_text.ptr, n, d_sa.ptr, 32, ctypes.byref(index), err, 256) == 0
+    try:
+        assert lib.sfk_hip_index_prefix_k(index) == oracle.prefix_select_k(n, n * 13) > 0
+        raw = "".join("ACGTN"[c - 2] if c >= 2 else "|" for c in text[:-1])
+        rng = np.random.default_rng(4)
+        pats, codes, lens = [], [], []
+        for q in range(300):
+            length = int(rng.integers(1, 33))
+            if q % 2 == 0:
+                start = int(rng.integers(0, n - length - 1))
+                p = "".join(ch if ch in "ACGT" else "A" for ch in raw[start:start + length])
+            else:
+                p = "".join("ACGT"[i] for i in rng.integers(0, 4, size=length))
+            code = 0
+            for j, ch in enumerate(p):
+                code |= "ACGT".index(ch) << (62 - 2 * j)
+            pats.append(p)
+            codes.append(code)
+            lens.append(len(p))
+        d_p = hip.DeviceArray(8 * len(pats))
+        d_p.upload(np.array(codes, np.uint64))
+        d_l = hip.DeviceArray(len(pats))
+        d_l.upload(np.array(lens, np.uint8))
+        d_c = hip.DeviceArray(4 * len(pats))
+        assert lib.sfk_hip_index_count(index, d_p.ptr, d_l.ptr, len(pats), d_c.ptr, None) == 0
+        got = d_c.download(np.uint32, len(pats))
+    finally:
+        lib.sfk_hip_index_destroy(index)
+    for p, g in zip(pats, got):
+        want, at = 0, raw.find(p)
+        while at != -1:
+            want += 1
+            at = raw.find(p, at + 1)
+        assert g == want, p
