@@ -1369,7 +1369,9 @@ __global__ __launch_bounds__(256) void PlcpIrreducibleKernel(
     DeviceCounters* counters) {
   const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const std::uint64_t r = isa[i];
+  // ISA streams through once (non-temporal: it should not evict the SA and
+  // text lines read at random).
+  const std::uint64_t r = static_cast<std::uint64_t>(__builtin_nontemporal_load(&isa[i]));
   // Validation fused here: ISA[i] < n and SA[ISA[i]] == i for every i makes
   // ISA a bijection and SA its inverse (the permutation check of the SA).
   if (r >= n || static_cast<std::uint64_t>(sa[r]) != i) {
@@ -1525,7 +1527,12 @@ __global__ __launch_bounds__(256) void LcpPermuteKernel(
     Out* __restrict__ lcp) {
   const std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  lcp[r] = r == 0 ? Out(0) : static_cast<Out>(plcp[sa[r]]);
+  // SA and LCP stream through once: non-temporal, so that they do not evict
+  // the randomly read PLCP lines from the caches.  A row whose SA value is
+  // out of range (the validation has flagged it) reads nothing.
+  const std::uint64_t v = static_cast<std::uint64_t>(__builtin_nontemporal_load(&sa[r]));
+  const Out out = (r == 0 || v >= n) ? Out(0) : static_cast<Out>(plcp[v]);
+  __builtin_nontemporal_store(out, &lcp[r]);
 }
 
 // ------------------------------------------------------------ K9/K10 ------
