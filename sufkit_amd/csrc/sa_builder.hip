@@ -547,29 +547,40 @@ class Doubling {
     if (lm_ == 0) return;
     ReserveSmallLists(lm_);
     ResetRoundCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
-    Split();
+    Split(OtherDelta(l_delta_));
     EndRound(true);
 
     // Exact 32-symbol groups: small segments in one local step on the exact
-    // 96-bit window key; large segments by the low 16 symbols, then (when a
-    // window holds a separator, N or the sentinel) by the exact key.
+    // 96-bit window key; large segments by the low 16 symbols, then the
+    // segments holding a window with a separator, N or the sentinel by the
+    // exact key (the others are exact groups already and wait aside).
     ResetRoundCountersKernel<<<1, 1, 0, b_->stream>>>(dc);
     if (qm_ > 0) LocalRefine<true>();
     bool split = false;
     if (lm_ > 0) {
       GlobalStep(kSecFillLo, 32, 0, false, false, /*readback=*/false);
+      std::uint32_t* flags = SpecialFlags();
+      SFK_HIP_CHECK(hipMemsetAsync(flags, 0, (lm_bound_ / 2 + 1) * sizeof(std::uint32_t), b_->stream));
       ResetSpecialCountKernel<<<1, 1, 0, b_->stream>>>(dc);
-      CountSpecialActiveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256)), 256, 0, b_->stream>>>(
-          b_->act_idx.As<Index>(), text_, dc);
+      MarkSpecialSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256)), 256, 0, b_->stream>>>(
+          b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), text_, flags, dc);
       SFK_HIP_CHECK(hipGetLastError());
+      ctx_.stats.algorithmic_bytes_sa += lm_bound_ * (sizeof(Index) + 4 + 16);
       const bool special = ApplySizes(ReadCounters(b_)).special_active > 0;
+      bool stashed = false;
       if (special && lm_ > 0) {
+        stashed = Partition(flags);
         GlobalStep(kSecExact0, 32, 0, false, false, true, /*survivor_rank=*/false);
         if (lm_ > 0) GlobalStep(kSecExact1, 32, 0, false, false, true, /*survivor_rank=*/false);
         if (lm_ > 0) GlobalStep(kSecExact2, 32, 0, false, false, true);
       }
+      Index* large_delta = OtherDelta(l_delta_);
       if (lm_ > 0) {
-        Split();
+        Split(large_delta);
+        split = true;
+      }
+      if (stashed) {
+        SplitStash(large_delta);
         split = true;
       }
     }
@@ -588,7 +599,7 @@ class Doubling {
       split = false;
       if (lm_ > 0) {
         GlobalStep(kSecRank, rank_bits, h, false, /*local_between=*/qm_ > 0, /*readback=*/false);
-        Split();
+        Split(OtherDelta(l_delta_));
         split = true;
       } else {
         LocalRefine<false>();
@@ -656,13 +667,13 @@ class Doubling {
     CheckFlags(c);
     if (pending_apply_) {
       constexpr std::uint64_t w = sizeof(Index);
-      const std::uint64_t m = pending_m_, survivors = c.survivors;
+      const std::uint64_t m = pending_m_, survivors = c.list.survivors;
       ctx_.stats.algorithmic_bytes_sa +=
           m * (2 * pending_key_bytes_ + w) + (m - survivors) * 2 * w + survivors * (3 * w + 4);
       pending_apply_ = false;
     }
-    lm_ = c.survivors;
-    l_segs_ = c.segments;
+    lm_ = c.list.survivors;
+    l_segs_ = c.list.segments;
     lm_bound_ = lm_;
     return c;
   }
@@ -689,35 +700,94 @@ class Doubling {
 
   // Routes the segments of the list RefineApply just wrote: small ones to
   // the next small list, large ones (renumbered) to the next large list in
-  // vals0 (positions) and vals1 (segment ids).
-  void Split() {
+  // vals0 (positions) and vals1 (segment ids), with deltas in large_delta.
+  void Split(Index* large_delta) {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
-    Index* dest = b_->keys0.As<Index>();
-    std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
-    Index* large_delta = OtherDelta(l_delta_);
     // Sizes come from the counters RefineApply wrote; grids cover bounds
     // (a list of m suffixes has at most m / 2 segments).
     const std::uint64_t m_bound = std::max<std::uint64_t>(lm_bound_, 1);
     const std::uint64_t seg_bound = pending_apply_ ? m_bound / 2 + 1 : std::max<std::uint64_t>(l_segs_, 1);
+    SplitList(b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), b_->seg_start.As<Index>(), l_delta_,
+              &dc->list, m_bound, seg_bound, large_delta);
+  }
+
+  // Routes the stash a partition set aside, after the special list's split,
+  // into the same next lists.
+  void SplitStash(Index* large_delta) {
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    SplitList(q_idx_cur_, reinterpret_cast<const std::uint32_t*>(b_->q_key.As<Index>()), q_row0_cur_,
+              q_row0_cur_ + stash_seg_room_, &dc->stash, std::max<std::uint64_t>(stash_m_, 1),
+              std::max<std::uint64_t>(stash_segs_, 1), large_delta);
+  }
+
+  void SplitList(const Index* idx, const std::uint32_t* seg, const Index* start, const Index* delta,
+                 const ListSize* size, std::uint64_t m_bound, std::uint64_t seg_bound, Index* large_delta) {
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    Index* dest = b_->keys0.As<Index>();
+    std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
     const SortGeometry g = Geometry(seg_bound, kSplitTile, 1024);
     SplitPartial* partial = b_->partial.As<SplitPartial>();
-    SplitCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), kLocalMaxSegment,
-                                                            g.chunk, partial, dc);
+    SplitCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(start, kLocalMaxSegment, g.chunk, partial, size);
     SplitSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, q_cap_, dc);
-    SplitAssignKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), l_delta_, kLocalMaxSegment,
-                                                             g.chunk, partial, dest, new_id, large_delta, dc);
+    SplitAssignKernel<Index><<<g.grid, 256, 0, b_->stream>>>(start, delta, kLocalMaxSegment, g.chunk, partial,
+                                                             dest, new_id, large_delta, size);
     SFK_HIP_CHECK(hipGetLastError());
     Index* l_idx_next = b_->vals0.As<Index>();
     std::uint32_t* l_seg_next = b_->vals1.As<std::uint32_t>();
     SplitMoveKernel<Index><<<static_cast<unsigned>(CeilDiv(m_bound, 256)), 256, 0, b_->stream>>>(
-        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), b_->seg_start.As<Index>(), l_delta_,
-        dest, new_id, q_idx_next_, q_row0_next_, l_idx_next, l_seg_next, dc);
+        idx, seg, start, delta, dest, new_id, q_idx_next_, q_row0_next_, l_idx_next, l_seg_next, size);
     SFK_HIP_CHECK(hipGetLastError());
     constexpr std::uint64_t w = sizeof(Index);
     ctx_.stats.algorithmic_bytes_sa += (m_bound / 2) * (4 * w + 4) + m_bound * (2 * w + 8 + 2 * w);
     l_idx_ = l_idx_next;
     l_seg_ = l_seg_next;
     l_delta_ = large_delta;
+  }
+
+  // Per-segment flags of the partition: above the split's new ids in keys1.
+  std::uint32_t* SpecialFlags() const { return b_->keys1.As<std::uint32_t>() + n_; }
+
+  // Partition of the list RefineApply just wrote (sizes read back) by the
+  // segment flags: flagged segments become the large list (vals0 / vals1,
+  // deltas in the other delta array), the others wait in the current small
+  // list's buffers, which the small list's step has consumed (positions in
+  // q_idx_cur, segment ids in q_key, first slots and deltas in q_row0_cur).
+  // Returns false, leaving the list whole, when they do not fit there.
+  bool Partition(const std::uint32_t* flags) {
+    if (lm_ > q_cap_ || l_segs_ == 0) return false;
+    DeviceCounters* dc = b_->counters.As<DeviceCounters>();
+    const SortGeometry g = Geometry(l_segs_, kSplitTile, 1024);
+    PartitionPartial* partial = b_->partial.As<PartitionPartial>();
+    Index* dest = b_->keys0.As<Index>();
+    std::uint32_t* new_id = b_->keys1.As<std::uint32_t>();
+    Index* x_delta = OtherDelta(l_delta_);
+    stash_seg_room_ = l_segs_;
+    PartitionCountKernel<Index><<<g.grid, 256, 0, b_->stream>>>(b_->seg_start.As<Index>(), flags, g.chunk,
+                                                                partial, dc);
+    PartitionSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
+    PartitionAssignKernel<Index><<<g.grid, 256, 0, b_->stream>>>(
+        b_->seg_start.As<Index>(), l_delta_, flags, g.chunk, partial, dest, new_id, x_delta, q_row0_cur_,
+        q_row0_cur_ + stash_seg_room_, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    Index* x_idx = b_->vals0.As<Index>();
+    std::uint32_t* x_seg = b_->vals1.As<std::uint32_t>();
+    PartitionMoveKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_, 256)), 256, 0, b_->stream>>>(
+        b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), b_->seg_start.As<Index>(), flags, dest, new_id,
+        x_idx, x_seg, q_idx_cur_, reinterpret_cast<std::uint32_t*>(b_->q_key.As<Index>()), dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    constexpr std::uint64_t w = sizeof(Index);
+    ctx_.stats.algorithmic_bytes_sa += l_segs_ * (4 * w + 8) + lm_ * (2 * w + 8 + 2 * w);
+    const DeviceCounters& c = ReadCounters(b_);
+    CheckFlags(c);
+    lm_ = c.special.survivors;
+    l_segs_ = c.special.segments;
+    lm_bound_ = lm_;
+    stash_m_ = c.stash.survivors;
+    stash_segs_ = c.stash.segments;
+    l_idx_ = x_idx;
+    l_seg_ = x_seg;
+    l_delta_ = x_delta;
+    return true;
   }
 
   template <bool kExact>
@@ -860,6 +930,11 @@ class Doubling {
   std::uint64_t pending_m_ = 0;
   std::uint64_t pending_key_bytes_ = 0;
   std::uint64_t l_segs_ = 0;
+  // The stash of a partition: sizes, and where its deltas start in
+  // q_row0_cur (after room for the input list's segment count).
+  std::uint64_t stash_m_ = 0;
+  std::uint64_t stash_segs_ = 0;
+  std::uint64_t stash_seg_room_ = 0;
   const Index* l_idx_ = nullptr;
   const std::uint32_t* l_seg_ = nullptr;
   Index* l_delta_ = nullptr;

@@ -60,9 +60,16 @@ enum ErrorFlag : std::uint32_t {
 };
 
 // Device counters read back by the host after each step.
-struct DeviceCounters {
+// Sizes of one list of segments: suffixes and segments.
+struct ListSize {
   unsigned long long survivors;
   unsigned long long segments;
+};
+
+struct DeviceCounters {
+  ListSize list;      // the list RefineApply last wrote
+  ListSize special;   // partition: segments holding a special window
+  ListSize stash;     // partition: the other segments, set aside
   unsigned long long special_active;
   unsigned long long long_items;
   unsigned long long long_words;
@@ -527,23 +534,33 @@ struct TextFillHiSource {
   }
 };
 
-// Counts active suffixes whose 32-symbol window holds a special symbol, over
-// the list RefineApply just wrote (its size is read from the counters; the
-// grid covers an upper bound).
+// Over the list RefineApply just wrote (its size is read from the counters;
+// the grid covers an upper bound): counts the suffixes whose 32-symbol window
+// holds a special symbol and marks their segments (flags[segment] = 1, the
+// only value ever stored, so racing stores agree).  The list is in segment
+// order, so one lane per run of special suffixes of one segment in a wave
+// stores, and only while the flag still reads 0.
 template <typename Index>
-__global__ __launch_bounds__(256) void CountSpecialActiveKernel(
-    const Index* __restrict__ act_idx, PackedText t, DeviceCounters* counters) {
-  const std::uint64_t m = counters->survivors;
+__global__ __launch_bounds__(256) void MarkSpecialSegmentsKernel(
+    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg, PackedText t,
+    std::uint32_t* __restrict__ flags, DeviceCounters* counters) {
+  const std::uint64_t m = counters->list.survivors;
   const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool special = false;
+  std::uint32_t seg = 0;
   if (k < m) {
     std::uint64_t c;
     std::uint32_t s;
+    seg = act_seg[k];
     LoadWindow(t, act_idx[k], c, s);
     special = s != 0;
   }
   const std::uint64_t vote = __ballot(special);
-  if (LaneId() == 0 && vote) {
+  const std::uint32_t lane = LaneId();
+  const std::uint32_t prev_seg = __shfl_up(seg, 1, kWave);
+  const bool prev_special = lane > 0 && ((vote >> (lane - 1)) & 1ull) != 0;
+  if (special && !(prev_special && prev_seg == seg) && flags[seg] == 0) flags[seg] = 1;
+  if (lane == 0 && vote) {
     atomicAdd(&counters->special_active, static_cast<unsigned long long>(__popcll(vote)));
   }
 }
@@ -778,8 +795,8 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
                                         out_delta, out_start, survivor_rank, run_surv, run_segs, s_prefix[2],
                                         lds);
   if (threadIdx.x == 0 && base + len == m) {
-    counters->survivors = run_surv + tot.surv;
-    counters->segments = run_segs + tot.segs;
+    counters->list.survivors = run_surv + tot.surv;
+    counters->list.segments = run_segs + tot.segs;
   }
 }
 
@@ -865,8 +882,8 @@ __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial
   const std::uint64_t tot_segs = SpineField<false>(partial, grid, &RefinePartial::newsegs, scratch);
   SpineField<true>(partial, grid, &RefinePartial::last_boundary, scratch);
   if (threadIdx.x == 0) {
-    counters->survivors = tot_surv;
-    counters->segments = tot_segs;
+    counters->list.survivors = tot_surv;
+    counters->list.segments = tot_segs;
   }
 }
 
@@ -905,14 +922,16 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
 
 // ------------------------------------------------------------ split -----
 
-// Split of the list RefineApply just wrote (its segment and suffix counts are
-// read from the counters; first slots in seg_start) into the next small list
-// and the next large list.  Block b handles segments [b * chunk, (b + 1) *
-// chunk), kSplitItems consecutive segments per thread per tile.  The small
-// list takes a block's segments of at most small_max suffixes when all of
-// them fit under q_cap (one bounded reservation per block, so the small list
-// has no holes); every other segment goes to the large list, whose ids and
-// slots follow segment order (prefix sums over blocks).
+// Split of a list of segments (the one RefineApply just wrote, or one set
+// aside by a partition; its segment and suffix counts are read from `size`,
+// first slots from seg_start) into the next small list and the next large
+// list.  Block b handles segments [b * chunk, (b + 1) * chunk), kSplitItems
+// consecutive segments per thread per tile.  The small list takes a block's
+// segments of at most small_max suffixes when all of them fit under q_cap
+// (one bounded reservation per block, so the small list has no holes); every
+// other segment goes to the large list, whose ids and slots follow segment
+// order (prefix sums over blocks) after those of earlier splits of the round
+// (the round counters l_next, l_next_segs).
 constexpr int kSplitItems = 8;
 constexpr int kSplitTile = 256 * kSplitItems;
 
@@ -934,11 +953,11 @@ __device__ inline std::uint64_t SegmentSize(const Index* __restrict__ seg_start,
 template <typename Index>
 __global__ __launch_bounds__(256) void SplitCountKernel(
     const Index* __restrict__ seg_start, std::uint32_t small_max, std::uint64_t chunk,
-    SplitPartial* __restrict__ partial, const DeviceCounters* counters) {
+    SplitPartial* __restrict__ partial, const ListSize* size) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  const std::uint64_t segs = counters->segments;
-  const std::uint64_t m = counters->survivors;
+  const std::uint64_t segs = size->segments;
+  const std::uint64_t m = size->survivors;
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
   std::uint64_t small_elems = 0, small_segs = 0, large_segs = 0, large_elems = 0;
@@ -979,14 +998,15 @@ __global__ __launch_bounds__(1024) void SplitSpineKernel(SplitPartial* partial, 
   SplitPartial p = valid ? partial[threadIdx.x] : SplitPartial{0, 0, 0, 0};
   const std::uint64_t small = p.q_base, small_segs = p.fits;
   const std::uint64_t q0 = counters->q_next;  // local survivors appended so far
+  const std::uint64_t l0 = counters->l_next, ls0 = counters->l_next_segs;  // earlier splits of the round
   std::uint64_t tot_small;
   const std::uint64_t before = BlockExclusiveSum<kBlock>(small, scratch, tot_small);
   const bool fits = q0 + before + small <= q_cap;
   const std::uint64_t lsegs_in = p.large_segs + (fits ? 0 : small_segs);
   const std::uint64_t lelems_in = p.large_elems + (fits ? 0 : small);
   std::uint64_t tot_segs, tot_elems;
-  const std::uint64_t segs = BlockExclusiveSum<kBlock>(lsegs_in, scratch, tot_segs);
-  const std::uint64_t elems = BlockExclusiveSum<kBlock>(lelems_in, scratch, tot_elems);
+  const std::uint64_t segs = ls0 + BlockExclusiveSum<kBlock>(lsegs_in, scratch, tot_segs);
+  const std::uint64_t elems = l0 + BlockExclusiveSum<kBlock>(lelems_in, scratch, tot_elems);
   // Small suffixes taken = those of the blocks that fit.
   std::uint64_t taken_total;
   BlockExclusiveSum<kBlock>(fits ? small : std::uint64_t{0}, scratch, taken_total);
@@ -999,8 +1019,8 @@ __global__ __launch_bounds__(1024) void SplitSpineKernel(SplitPartial* partial, 
   }
   if (threadIdx.x == 0) {
     counters->q_next = q0 + taken_total;
-    counters->l_next = tot_elems;
-    counters->l_next_segs = tot_segs;
+    counters->l_next = l0 + tot_elems;
+    counters->l_next_segs = ls0 + tot_segs;
   }
 }
 
@@ -1008,11 +1028,11 @@ template <typename Index>
 __global__ __launch_bounds__(256) void SplitAssignKernel(
     const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta, std::uint32_t small_max,
     std::uint64_t chunk, const SplitPartial* __restrict__ partial, Index* __restrict__ dest,
-    std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, const DeviceCounters* counters) {
+    std::uint32_t* __restrict__ new_id, Index* __restrict__ large_delta, const ListSize* size) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  const std::uint64_t segs = counters->segments;
-  const std::uint64_t m = counters->survivors;
+  const std::uint64_t segs = size->segments;
+  const std::uint64_t m = size->survivors;
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
   const SplitPartial part = partial[blockIdx.x];
@@ -1071,9 +1091,9 @@ __global__ __launch_bounds__(256) void SplitMoveKernel(
     const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
     const Index* __restrict__ dest, const std::uint32_t* __restrict__ new_id,
     Index* __restrict__ q_idx, Index* __restrict__ q_row0, Index* __restrict__ l_idx,
-    std::uint32_t* __restrict__ l_seg, const DeviceCounters* counters) {
+    std::uint32_t* __restrict__ l_seg, const ListSize* size) {
   const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= counters->survivors) return;
+  if (p >= size->survivors) return;
   const std::uint32_t s = act_seg[p];
   const Index start = seg_start[s];
   const std::uint64_t at = static_cast<std::uint64_t>(dest[s]) + (p - static_cast<std::uint64_t>(start));
@@ -1085,6 +1105,158 @@ __global__ __launch_bounds__(256) void SplitMoveKernel(
   } else {
     l_idx[at] = v;
     l_seg[at] = id;
+  }
+}
+
+// ------------------------------------------------------------ partition -
+
+// Partition of the list RefineApply just wrote by a per-segment flag: flagged
+// segments form the "special" list (renumbered in segment order, to x_idx /
+// x_seg with deltas x_delta), the others the "stash" list (to y_idx / y_seg,
+// first slots y_start, deltas y_delta).  Rows stay those of the input list.
+// Same geometry as the split: block b handles segments [b * chunk, (b + 1) *
+// chunk).
+struct PartitionPartial {
+  unsigned long long x_segs, x_elems, y_segs, y_elems;  // exclusive prefixes after the spine
+};
+
+template <typename Index>
+__global__ __launch_bounds__(256) void PartitionCountKernel(
+    const Index* __restrict__ seg_start, const std::uint32_t* __restrict__ flags, std::uint64_t chunk,
+    PartitionPartial* __restrict__ partial, const DeviceCounters* counters) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t segs = counters->list.segments;
+  const std::uint64_t m = counters->list.survivors;
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
+  std::uint64_t x_segs = 0, x_elems = 0, y_segs = 0, y_elems = 0;
+  for (std::uint64_t s = begin + threadIdx.x; s < end; s += kBlock) {
+    const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+    if (flags[s] != 0) {
+      x_elems += size;
+      ++x_segs;
+    } else {
+      y_elems += size;
+      ++y_segs;
+    }
+  }
+  std::uint64_t t_xs, t_xe, t_ys, t_ye;
+  BlockExclusiveSum<kBlock>(x_segs, scratch, t_xs);
+  BlockExclusiveSum<kBlock>(x_elems, scratch, t_xe);
+  BlockExclusiveSum<kBlock>(y_segs, scratch, t_ys);
+  BlockExclusiveSum<kBlock>(y_elems, scratch, t_ye);
+  if (threadIdx.x == 0) partial[blockIdx.x] = PartitionPartial{t_xs, t_xe, t_ys, t_ye};
+}
+
+// Over at most 1024 blocks: exclusive prefixes of the four counts, totals to
+// counters->special and counters->stash.
+__global__ __launch_bounds__(1024) void PartitionSpineKernel(PartitionPartial* partial, int grid,
+                                                             DeviceCounters* counters) {
+  constexpr int kBlock = 1024;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  const PartitionPartial p = valid ? partial[threadIdx.x] : PartitionPartial{0, 0, 0, 0};
+  std::uint64_t t_xs, t_xe, t_ys, t_ye;
+  PartitionPartial out;
+  out.x_segs = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.x_segs), scratch, t_xs);
+  out.x_elems = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.x_elems), scratch, t_xe);
+  out.y_segs = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.y_segs), scratch, t_ys);
+  out.y_elems = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.y_elems), scratch, t_ye);
+  if (valid) partial[threadIdx.x] = out;
+  if (threadIdx.x == 0) {
+    counters->special = ListSize{t_xe, t_xs};
+    counters->stash = ListSize{t_ye, t_ys};
+  }
+}
+
+// Per segment: its destination slot and new id in its list, and that list's
+// delta (and, for the stash, first slot).
+template <typename Index>
+__global__ __launch_bounds__(256) void PartitionAssignKernel(
+    const Index* __restrict__ seg_start, const Index* __restrict__ seg_delta,
+    const std::uint32_t* __restrict__ flags, std::uint64_t chunk, const PartitionPartial* __restrict__ partial,
+    Index* __restrict__ dest, std::uint32_t* __restrict__ new_id, Index* __restrict__ x_delta,
+    Index* __restrict__ y_start, Index* __restrict__ y_delta, const DeviceCounters* counters) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t segs = counters->list.segments;
+  const std::uint64_t m = counters->list.survivors;
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < segs ? begin + chunk : segs;
+  const PartitionPartial part = partial[blockIdx.x];
+  std::uint64_t run_xs = part.x_segs, run_xe = part.x_elems, run_ys = part.y_segs, run_ye = part.y_elems;
+  for (std::uint64_t base = begin; base < end; base += kSplitTile) {
+    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kSplitItems;
+    std::uint64_t xs = 0, xe = 0, ys = 0, ye = 0;
+#pragma unroll
+    for (int i = 0; i < kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      if (s < end) {
+        const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+        if (flags[s] != 0) {
+          xe += size;
+          ++xs;
+        } else {
+          ye += size;
+          ++ys;
+        }
+      }
+    }
+    std::uint64_t t_xs, t_xe, t_ys, t_ye;
+    std::uint64_t xs_off = run_xs + BlockExclusiveSum<kBlock>(xs, scratch, t_xs);
+    std::uint64_t xe_off = run_xe + BlockExclusiveSum<kBlock>(xe, scratch, t_xe);
+    std::uint64_t ys_off = run_ys + BlockExclusiveSum<kBlock>(ys, scratch, t_ys);
+    std::uint64_t ye_off = run_ye + BlockExclusiveSum<kBlock>(ye, scratch, t_ye);
+#pragma unroll
+    for (int i = 0; i < kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      if (s >= end) break;
+      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
+      const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+      const Index row0 = static_cast<Index>(seg_delta[s] + static_cast<Index>(start));
+      if (flags[s] != 0) {
+        dest[s] = static_cast<Index>(xe_off);
+        new_id[s] = static_cast<std::uint32_t>(xs_off);
+        x_delta[xs_off] = static_cast<Index>(row0 - static_cast<Index>(xe_off));
+        xe_off += size;
+        ++xs_off;
+      } else {
+        dest[s] = static_cast<Index>(ye_off);
+        new_id[s] = static_cast<std::uint32_t>(ys_off);
+        y_start[ys_off] = static_cast<Index>(ye_off);
+        y_delta[ys_off] = static_cast<Index>(row0 - static_cast<Index>(ye_off));
+        ye_off += size;
+        ++ys_off;
+      }
+    }
+    run_xs += t_xs;
+    run_xe += t_xe;
+    run_ys += t_ys;
+    run_ye += t_ye;
+  }
+}
+
+// One thread per suffix of the input list (grid: an upper bound).
+template <typename Index>
+__global__ __launch_bounds__(256) void PartitionMoveKernel(
+    const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg,
+    const Index* __restrict__ seg_start, const std::uint32_t* __restrict__ flags,
+    const Index* __restrict__ dest, const std::uint32_t* __restrict__ new_id, Index* __restrict__ x_idx,
+    std::uint32_t* __restrict__ x_seg, Index* __restrict__ y_idx, std::uint32_t* __restrict__ y_seg,
+    const DeviceCounters* counters) {
+  const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= counters->list.survivors) return;
+  const std::uint32_t s = act_seg[p];
+  const std::uint64_t at =
+      static_cast<std::uint64_t>(dest[s]) + (p - static_cast<std::uint64_t>(seg_start[s]));
+  const Index v = act_idx[p];
+  if (flags[s] != 0) {
+    x_idx[at] = v;
+    x_seg[at] = new_id[s];
+  } else {
+    y_idx[at] = v;
+    y_seg[at] = new_id[s];
   }
 }
 
