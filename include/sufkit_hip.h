@@ -203,6 +203,41 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns,
                         uint32_t* d_counts, void* stream);
 
 /*
+ * FM-index of a resident index, built on the device from its SA: the BWT
+ * (the symbol before each suffix) as per-base membership masks in records
+ * of 64 rows, each with the base counts of all rows before it, and C (the
+ * first row of each base's suffixes).  Replaces the SDSL CSA that
+ * FmIndex::Build constructs on the host (fm_index.cpp:236-294,488-524);
+ * backward search gives the same half-open row ranges as the reference's
+ * EqualRange (fm_index.cpp:298-309), because the SA of text ++ sentinel is
+ * unique.  About n bytes of device memory.
+ */
+int sfk_hip_index_build_fm(sfk_hip_index* index, char* error,
+                           size_t error_capacity);
+uint64_t sfk_hip_index_fm_bytes(const sfk_hip_index* index);
+/* out = {C[A], C[C], C[G], C[T], end of T's rows, symbols below A}. */
+int sfk_hip_index_fm_header(sfk_hip_index* index, uint64_t out[6]);
+/* Backward search of `count` patterns of any length: pattern q is
+ * d_codes[d_offsets[q] .. d_offsets[q + 1]) (A=0 .. T=3, one byte each),
+ * searched as given (d_orientation[q] = 0) or as its reverse complement
+ * (1).  Its row range [begin, end) goes to d_ranges[2q], d_ranges[2q + 1],
+ * (0, 0) when it does not occur (FmIndex::EqualRange / EqualRangeBatch /
+ * CountBatch lanes, fm_index.cpp:298-309,526-610). */
+int sfk_hip_index_fm_ranges(sfk_hip_index* index, const uint8_t* d_codes,
+                            const uint64_t* d_offsets,
+                            const uint8_t* d_orientation, uint64_t count,
+                            uint64_t* d_ranges, void* stream);
+/* Count queries in sfk_hip_index_count's packed format, by backward search
+ * (the served loop's exact FM count queries). */
+int sfk_hip_index_fm_count(sfk_hip_index* index, const uint64_t* d_patterns,
+                           const uint8_t* d_lengths, uint64_t count,
+                           uint32_t* d_counts, void* stream);
+/* SA values of rows [begin, end) as uint64 into host memory (the positions
+ * FmIndex::Locate recovers for a row range, fm_index.cpp:470-480). */
+int sfk_hip_index_sa_rows(sfk_hip_index* index, uint64_t begin, uint64_t end,
+                          uint64_t* out);
+
+/*
  * Radix shape autotuner.  Every onesweep pass of a build runs one of a set
  * of compiled shapes (threads per block x keys per thread), chosen per kind
  * of sort and size class of the list from a per-device table:

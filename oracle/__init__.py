@@ -320,3 +320,141 @@ def ref_write_sufidx(records: list[tuple[str, bytes]], path: str, coordinate_wid
                                 coordinate_width, storage_width, acceleration, err, len(err))
     if rc != 0:
         raise RuntimeError(err.value.decode())
+
+
+# ------------------------------------------------------------ exact search
+
+_BASE_CODE = {ord(c): v for c, v in zip("ACGTacgt", (2, 3, 4, 5, 2, 3, 4, 5))}
+
+
+def encode_pattern(pattern: str) -> np.ndarray:
+    """A/C/G/T pattern as text codes 2..5 (query.cpp:47-65)."""
+    return np.array([_BASE_CODE[ord(ch)] for ch in pattern], dtype=np.uint8)
+
+
+def reverse_complement(pattern: str) -> str:
+    return pattern[::-1].translate(str.maketrans("ACGTacgt", "TGCAtgca"))
+
+
+def occurrences(text: np.ndarray, pattern: str) -> np.ndarray:
+    """Positions where the pattern occurs in the text, increasing."""
+    L = lib()
+    L.sfk_oracle_occurrences.argtypes = [U8P, ctypes.c_uint64, U8P, ctypes.c_uint64, U64P, ctypes.c_uint64]
+    L.sfk_oracle_occurrences.restype = ctypes.c_uint64
+    text, tp = _u8(text)
+    pat, pp = _u8(encode_pattern(pattern))
+    total = int(L.sfk_oracle_occurrences(tp, text.size, pp, pat.size, None, 0))
+    out = np.zeros(max(total, 1), np.uint64)
+    L.sfk_oracle_occurrences(tp, text.size, pp, pat.size, _u64(out), total)
+    return out[:total]
+
+
+def equal_range(text: np.ndarray, pattern: str) -> tuple[int, int]:
+    """The pattern's SA row range [begin, end) by definition, (0, 0) when absent."""
+    L = lib()
+    L.sfk_oracle_suffixes_below.argtypes = [U8P, ctypes.c_uint64, U8P, ctypes.c_uint64]
+    L.sfk_oracle_suffixes_below.restype = ctypes.c_uint64
+    count = occurrences(text, pattern).size
+    if count == 0:
+        return 0, 0
+    text, tp = _u8(text)
+    pat, pp = _u8(encode_pattern(pattern))
+    below = int(L.sfk_oracle_suffixes_below(tp, text.size, pp, pat.size))
+    return below, below + count
+
+
+def locate(text: np.ndarray, contig_starts: np.ndarray, pattern: str, strands: int = 0,
+           max_hits: int | None = None) -> dict:
+    """FmIndex::Locate by definition (fm_index.cpp:112-181,612-677):
+    (sequence id, local position, length, strand) sorted by global position,
+    a position found on both strands of a palindrome reported once as both
+    (strand 2), the max_hits smallest kept."""
+    rc = reverse_complement(pattern)
+    if strands == 0:
+        lanes = [(pattern, 0)]
+    elif strands == 1:
+        lanes = [(rc, 1)]
+    else:
+        lanes = [(pattern, 2)] if rc.upper() == pattern.upper() else [(pattern, 0), (rc, 1)]
+    hits = sorted((int(p), s) for lane, s in lanes for p in occurrences(text, lane))
+    total = len(hits)
+    if max_hits is not None:
+        hits = hits[:max_hits]
+    starts = np.asarray(contig_starts, np.uint64)
+    out = []
+    for pos, s in hits:
+        cid = int(np.searchsorted(starts, pos, side="right")) - 1
+        out.append((cid, pos - int(starts[cid]), len(pattern), s))
+    return {"total_hits": total, "hits": out, "truncated": len(out) < total}
+
+
+def ref_fm_available() -> bool:
+    return ref_writer_available()
+
+
+class RefFmIndex:
+    """The reference's own FmIndex (SDSL csa_wt on the host), compiled into
+    oracle/_ref; test infrastructure only (golden vectors, error codes)."""
+
+    def __init__(self, records: list[tuple[str, bytes]], backend: int = 1):
+        W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
+        vp, cp = ctypes.c_void_p, ctypes.c_char_p
+        W.sfk_ref_fm_build.argtypes = [ctypes.POINTER(cp), ctypes.POINTER(cp), U64P, ctypes.c_uint64, ctypes.c_int,
+                                       ctypes.POINTER(vp)]
+        W.sfk_ref_fm_free.argtypes = [vp]
+        W.sfk_ref_fm_free.restype = None
+        W.sfk_ref_fm_equal_range.argtypes = [vp, cp, ctypes.c_uint64, U64P]
+        W.sfk_ref_fm_count.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_int, U64P]
+        W.sfk_ref_fm_count_batch.argtypes = [vp, cp, U64P, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, U64P]
+        W.sfk_ref_fm_locate.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, U64P,
+                                        ctypes.POINTER(ctypes.c_uint32), U64P, U64P, U8P, ctypes.c_uint64]
+        self._W = W
+        names = (cp * len(records))(*[r[0].encode() for r in records])
+        seqs = (cp * len(records))(*[bytes(r[1]) for r in records])
+        lens = np.array([len(r[1]) for r in records], dtype=np.uint64)
+        self._h = vp()
+        self.build_status = W.sfk_ref_fm_build(names, seqs, _u64(lens), len(records), backend, ctypes.byref(self._h))
+
+    def equal_range(self, pattern: str):
+        out = (ctypes.c_uint64 * 2)()
+        p = pattern.encode()
+        rc = self._W.sfk_ref_fm_equal_range(self._h, p, len(p), out)
+        return (int(out[0]), int(out[1])) if rc == 0 else ("error", rc)
+
+    def count(self, pattern: str, strands: int = 0):
+        out = ctypes.c_uint64()
+        p = pattern.encode()
+        rc = self._W.sfk_ref_fm_count(self._h, p, len(p), strands, ctypes.byref(out))
+        return int(out.value) if rc == 0 else ("error", rc)
+
+    def count_batch(self, patterns: list[str], strands: int = 0, batch_width: int = 0):
+        data = [p.encode() for p in patterns]
+        offsets = np.zeros(len(data) + 1, np.uint64)
+        if data:
+            offsets[1:] = np.cumsum([len(d) for d in data])
+        out = np.zeros(max(len(data), 1), np.uint64)
+        rc = self._W.sfk_ref_fm_count_batch(self._h, b"".join(data) or b"\0", _u64(offsets), len(data), strands,
+                                            batch_width, _u64(out))
+        return [int(v) for v in out[:len(data)]] if rc == 0 else ("error", rc)
+
+    def locate(self, pattern: str, strands: int = 0, max_hits: int | None = None):
+        info = (ctypes.c_uint64 * 3)()
+        p = pattern.encode()
+        cap = 1 << 16
+        ids = np.zeros(cap, np.uint32)
+        pos = np.zeros(cap, np.uint64)
+        lens = np.zeros(cap, np.uint64)
+        strs = np.zeros(cap, np.uint8)
+        rc = self._W.sfk_ref_fm_locate(self._h, p, len(p), strands, int(max_hits is not None), max_hits or 0, info,
+                                       ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _u64(pos), _u64(lens),
+                                       strs.ctypes.data_as(U8P), cap)
+        if rc != 0:
+            return ("error", rc)
+        size = min(int(info[1]), cap)
+        hits = [(int(ids[i]), int(pos[i]), int(lens[i]), int(strs[i])) for i in range(size)]
+        return {"total_hits": int(info[0]), "hits": hits, "truncated": bool(info[2])}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._W.sfk_ref_fm_free(self._h)
+            self._h = None

@@ -22,6 +22,9 @@
  *                              LCP[0] = 0)
  *   sfk_oracle_crc32           zlib crc32 (IEEE 802.3, reflected 0xEDB88320)
  *                              as used by serialization.cpp:60-141
+ *   sfk_oracle_occurrences,    exact pattern occurrences and the first SA
+ *   sfk_oracle_suffixes_below  row of a pattern by definition (the FM-index
+ *                              queries, fm_index.cpp:298-309,547-677)
  *
  * Pinning (what makes each function trustworthy, checked in tests/):
  *   - naive SA, doubling SA: the reference's own golden vectors decoded
@@ -31,7 +34,10 @@
  *   - Kasai: brute-force adjacent compare on small texts + fixture LCPs;
  *   - CRC32: the standard check value crc32("123456789") = 0xCBF43926 and
  *     the header/section CRCs stored in the legacy fixtures;
- *   - FNV-1a-64: the fixture fingerprint 0x0e8d89af19be03e1.
+ *   - FNV-1a-64: the fixture fingerprint 0x0e8d89af19be03e1;
+ *   - occurrences / suffixes below: the reference's own FmIndex (SDSL,
+ *     compiled into oracle/_ref) on the same references and patterns, and
+ *     the SA interval of the pattern in the brute-force SA.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -279,3 +285,44 @@ uint64_t sfk_oracle_prefix_directory(const uint8_t* text, uint64_t n, const uint
   return indexed;
 }
 
+
+/*
+ * Exact pattern search by definition, for the FM-index queries
+ * (fm_index.cpp:298-309,547-677): a pattern of bases (codes 2..5) occurs at
+ * p when text[p .. p + len) equals it.  sfk_oracle_occurrences writes the
+ * occurrence positions in increasing order (up to capacity) and returns
+ * their number; sfk_oracle_suffixes_below counts the suffixes whose first
+ * len symbols compare below the pattern (the sentinel ends every suffix and
+ * is below every base), which is the first row of the pattern's SA range.
+ * Parity of these with the reference's FmIndex is checked in tests/.
+ */
+uint64_t sfk_oracle_occurrences(const uint8_t* text, uint64_t n, const uint8_t* pattern, uint64_t len,
+                                uint64_t* out, uint64_t capacity) {
+  uint64_t found = 0;
+  if (len == 0 || len > n) return 0;
+  for (uint64_t p = 0; p + len <= n; ++p) {
+    uint64_t i = 0;
+    while (i < len && text[p + i] == pattern[i]) ++i;
+    if (i == len) {
+      if (found < capacity) out[found] = p;
+      ++found;
+    }
+  }
+  return found;
+}
+
+uint64_t sfk_oracle_suffixes_below(const uint8_t* text, uint64_t n, const uint8_t* pattern, uint64_t len) {
+  uint64_t below = 0;
+  for (uint64_t s = 0; s < n; ++s) {
+    for (uint64_t i = 0; i < len; ++i) {
+      /* s + i < n always holds before a difference: text[n - 1] is the
+       * sentinel and no pattern symbol equals it. */
+      const uint8_t c = text[s + i];
+      if (c != pattern[i]) {
+        if (c < pattern[i]) ++below;
+        break;
+      }
+    }
+  }
+  return below;
+}

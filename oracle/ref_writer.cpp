@@ -1,6 +1,7 @@
 // C entry points over the compiled reference library: build with the
 // reference's own SuffixArray::Build (divsufsort backend) and Save a .sufidx
-// 1.4 file, or Load one and report its SA/ISA/LCP.
+// 1.4 file, or Load one and report its SA/ISA/LCP; run the reference's
+// FastPrefixIndex and FmIndex.
 //
 // TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile (target ref-full)
 // against the reference sources where they lie; nothing of the reference is
@@ -11,12 +12,31 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <string_view>
 #include <vector>
 
+#include <sufkit/fm_index.hpp>
 #include <sufkit/sufkit.hpp>
 
 #include "coordinate_storage.hpp"
 #include "fast_prefix_index.hpp"
+
+// Runs f: 0, or the reference's ErrorCode + 1 when it throws sufkit::Error.
+namespace {
+
+template <class F>
+int RefGuard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const sufkit::Error& e) {
+    return static_cast<int>(e.Code()) + 1;
+  } catch (const std::exception&) {
+    return 100;
+  }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -89,6 +109,80 @@ int sfk_ref_prefix_directory(const std::uint8_t* text, std::uint64_t n, const st
   } catch (const std::exception&) {
     return -1;
   }
+}
+
+// ------------------------------------------------------------ FM index
+//
+// The reference's own FmIndex (fm_index.cpp: SDSL csa_wt built on the host)
+// over a reference from records, for the device FM-index's golden vectors
+// and error behaviour (entry points return as RefGuard does).
+int sfk_ref_fm_build(const char* const* names, const char* const* sequences, const std::uint64_t* lengths,
+                     std::uint64_t count, int backend, void** out) {
+  return RefGuard([&] {
+    std::vector<sufkit::SequenceRecord> records(count);
+    for (std::uint64_t i = 0; i < count; ++i) {
+      records[i].name = names[i];
+      records[i].sequence.assign(sequences[i], lengths[i]);
+    }
+    const auto reference = sufkit::GenomeReference::FromRecords(std::move(records));
+    sufkit::FmIndexBuildOptions options;
+    options.backend = static_cast<sufkit::FmBackend>(backend);
+    *out = new sufkit::FmIndex(sufkit::FmIndex::Build(reference, options));
+  });
+}
+
+void sfk_ref_fm_free(void* fm) { delete static_cast<sufkit::FmIndex*>(fm); }
+
+int sfk_ref_fm_equal_range(const void* fm, const char* pattern, std::uint64_t length, std::uint64_t out[2]) {
+  return RefGuard([&] {
+    const auto r = static_cast<const sufkit::FmIndex*>(fm)->EqualRange(std::string_view(pattern, length));
+    out[0] = r.begin;
+    out[1] = r.end;
+  });
+}
+
+int sfk_ref_fm_count(const void* fm, const char* pattern, std::uint64_t length, int strands, std::uint64_t* out) {
+  return RefGuard([&] {
+    *out = static_cast<const sufkit::FmIndex*>(fm)->Count(std::string_view(pattern, length),
+                                                           static_cast<sufkit::StrandMode>(strands));
+  });
+}
+
+int sfk_ref_fm_count_batch(const void* fm, const char* patterns, const std::uint64_t* offsets, std::uint64_t count,
+                           int strands, std::uint32_t batch_width, std::uint64_t* out) {
+  return RefGuard([&] {
+    std::vector<std::string_view> list(count);
+    for (std::uint64_t i = 0; i < count; ++i) {
+      list[i] = std::string_view(patterns + offsets[i], offsets[i + 1] - offsets[i]);
+    }
+    sufkit::FmBatchOptions options;
+    options.strands = static_cast<sufkit::StrandMode>(strands);
+    options.batch_width = batch_width;
+    const auto counts = static_cast<const sufkit::FmIndex*>(fm)->CountBatch(list, options);
+    std::memcpy(out, counts.data(), count * sizeof(std::uint64_t));
+  });
+}
+
+// info = {total hits, retained hits, truncated}; the first `capacity` hits
+// go to ids / positions / lengths / strand codes.
+int sfk_ref_fm_locate(const void* fm, const char* pattern, std::uint64_t length, int strands, int has_max,
+                      std::uint64_t max_hits, std::uint64_t info[3], std::uint32_t* ids, std::uint64_t* positions,
+                      std::uint64_t* lengths, std::uint8_t* strand_codes, std::uint64_t capacity) {
+  return RefGuard([&] {
+    sufkit::LocateOptions options;
+    options.strands = static_cast<sufkit::StrandMode>(strands);
+    if (has_max != 0) options.max_hits = max_hits;
+    const auto result = static_cast<const sufkit::FmIndex*>(fm)->Locate(std::string_view(pattern, length), options);
+    info[0] = result.total_hits;
+    info[1] = result.hits.size();
+    info[2] = result.truncated ? 1 : 0;
+    for (std::uint64_t i = 0; i < result.hits.size() && i < capacity; ++i) {
+      ids[i] = result.hits[i].sequence_id;
+      positions[i] = result.hits[i].position;
+      lengths[i] = result.hits[i].length;
+      strand_codes[i] = static_cast<std::uint8_t>(result.hits[i].strand);
+    }
+  });
 }
 
 // FastPrefixIndex::SelectK with default options.

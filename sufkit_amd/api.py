@@ -81,6 +81,24 @@ def _lib():
     lib.sfk_sa_count.argtypes = [vp, cp, ctypes.c_uint64, u64p, cp, sz]
     lib.sfk_hip_backend_available.restype = ctypes.c_int
     lib.sfk_sufidx_verify.argtypes = [cp, cp, sz]
+    lib.sfk_fm_build.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), cp, sz]
+    lib.sfk_fm_free.argtypes = [vp]
+    lib.sfk_fm_free.restype = None
+    lib.sfk_fm_summary.argtypes = [vp, u64p]
+    lib.sfk_fm_summary.restype = None
+    lib.sfk_fm_equal_range.argtypes = [vp, cp, ctypes.c_uint64, u64p, cp, sz]
+    lib.sfk_fm_equal_range_batch.argtypes = [vp, cp, u64p, ctypes.c_uint64, ctypes.c_uint32, u64p, cp, sz]
+    lib.sfk_fm_count.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_int, u64p, cp, sz]
+    lib.sfk_fm_count_batch.argtypes = [vp, cp, u64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, u64p, cp, sz]
+    lib.sfk_fm_locate.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                  ctypes.POINTER(vp), cp, sz]
+    lib.sfk_fm_hits_summary.argtypes = [vp, u64p]
+    lib.sfk_fm_hits_summary.restype = None
+    lib.sfk_fm_hits_get.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), u64p, u64p, ctypes.POINTER(ctypes.c_uint8)]
+    lib.sfk_fm_hits_get.restype = None
+    lib.sfk_fm_hits_free.argtypes = [vp]
+    lib.sfk_fm_hits_free.restype = None
+    lib.sfk_fm_save.argtypes = [vp, cp, ctypes.c_int, cp, sz]
     _BOUND = True
     return lib
 
@@ -227,6 +245,107 @@ class SuffixArray:
     def __del__(self):
         if getattr(self, "_h", None):
             _lib().sfk_sa_free(self._h)
+            self._h = None
+
+
+# FmBackend / StrandMode / Strand values (types.hpp:200-229).
+FM_SDSL_CSA_WT_HUFF, FM_SDSL_CSA_WT_BALANCED, FM_SDSL_CSA_SADA, FM_SDSL_CSA_WT_EPR = 1, 2, 3, 4
+FORWARD, REVERSE_COMPLEMENT, BOTH = 0, 1, 2
+
+
+def _joined(patterns: list[str]):
+    data = [p.encode() for p in patterns]
+    offsets = np.zeros(len(data) + 1, dtype=np.uint64)
+    if data:
+        offsets[1:] = np.cumsum([len(d) for d in data])
+    return b"".join(data) or b"\0", offsets
+
+
+class FmIndex:
+    """FmIndex over the device FM-index built from the GPU SA (fm_index.hpp)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def build(cls, reference: GenomeReference, backend: int = FM_SDSL_CSA_WT_HUFF) -> "FmIndex":
+        out = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        _check(_lib().sfk_fm_build(reference._h, backend, ctypes.byref(out), err, len(err)), err)
+        return cls(out)
+
+    def summary(self) -> dict:
+        out = (ctypes.c_uint64 * 7)()
+        _lib().sfk_fm_summary(self._h, out)
+        keys = ("text_symbols", "sequence_count", "total_bases", "fingerprint", "fm_device_bytes",
+                "prefix_directory_k", "sa_bits")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def equal_range(self, pattern: str) -> tuple[int, int]:
+        out = (ctypes.c_uint64 * 2)()
+        err = ctypes.create_string_buffer(512)
+        p = pattern.encode()
+        _check(_lib().sfk_fm_equal_range(self._h, p, len(p), out, err, len(err)), err)
+        return int(out[0]), int(out[1])
+
+    def equal_range_batch(self, patterns: list[str], batch_width: int = 0) -> list[tuple[int, int]]:
+        data, offsets = _joined(patterns)
+        out = np.zeros(2 * max(len(patterns), 1), dtype=np.uint64)
+        err = ctypes.create_string_buffer(512)
+        _check(_lib().sfk_fm_equal_range_batch(self._h, data, offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                               len(patterns), batch_width,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), err, len(err)),
+               err)
+        return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(len(patterns))]
+
+    def count(self, pattern: str, strands: int = FORWARD) -> int:
+        out = ctypes.c_uint64()
+        err = ctypes.create_string_buffer(512)
+        p = pattern.encode()
+        _check(_lib().sfk_fm_count(self._h, p, len(p), strands, ctypes.byref(out), err, len(err)), err)
+        return int(out.value)
+
+    def count_batch(self, patterns: list[str], strands: int = FORWARD, batch_width: int = 0) -> list[int]:
+        data, offsets = _joined(patterns)
+        out = np.zeros(max(len(patterns), 1), dtype=np.uint64)
+        err = ctypes.create_string_buffer(512)
+        _check(_lib().sfk_fm_count_batch(self._h, data, offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                         len(patterns), strands, batch_width,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), err, len(err)), err)
+        return [int(v) for v in out[:len(patterns)]]
+
+    def locate(self, pattern: str, strands: int = FORWARD, max_hits: int | None = None) -> dict:
+        """{"total_hits", "hits": [(sequence_id, position, length, strand)], "truncated"}."""
+        lib = _lib()
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        p = pattern.encode()
+        _check(lib.sfk_fm_locate(self._h, p, len(p), strands, int(max_hits is not None), max_hits or 0,
+                                 ctypes.byref(h), err, len(err)), err)
+        try:
+            summary = (ctypes.c_uint64 * 3)()
+            lib.sfk_fm_hits_summary(h, summary)
+            size = int(summary[1])
+            ids = np.zeros(max(size, 1), dtype=np.uint32)
+            pos = np.zeros(max(size, 1), dtype=np.uint64)
+            lens = np.zeros(max(size, 1), dtype=np.uint64)
+            strs = np.zeros(max(size, 1), dtype=np.uint8)
+            lib.sfk_fm_hits_get(h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                strs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        finally:
+            lib.sfk_fm_hits_free(h)
+        hits = [(int(ids[i]), int(pos[i]), int(lens[i]), int(strs[i])) for i in range(size)]
+        return {"total_hits": int(summary[0]), "hits": hits, "truncated": bool(summary[2])}
+
+    def save(self, path: str | Path, overwrite: bool = False) -> None:
+        err = ctypes.create_string_buffer(512)
+        _check(_lib().sfk_fm_save(self._h, str(path).encode(), int(overwrite), err, len(err)), err)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().sfk_fm_free(self._h)
             self._h = None
 
 

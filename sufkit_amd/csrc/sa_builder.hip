@@ -32,6 +32,7 @@
 #include "query_kernels.hpp"
 #include "radix_sort.hpp"
 #include "sa_kernels.hpp"
+#include "fm_kernels.hpp"  // after sa_kernels.hpp: uses its error flags
 #include "sufkit_hip.h"
 
 namespace sfk {
@@ -1571,6 +1572,9 @@ struct sfk_hip_index {
   sfk::DeviceBuffer packed, counters, directory, directory_counts;
   uint32_t directory_k = 0;  // 0: no directory
   sfk::DeviceCounters* host_counters = nullptr;
+  // FM-index (sfk_hip_index_build_fm): occurrence records and C.
+  sfk::DeviceBuffer fm_blocks, fm_header, fm_partial;
+  bool fm_built = false;
 };
 
 void sfk_hip_index_destroy(sfk_hip_index* index) {
@@ -1686,7 +1690,130 @@ int sfk_hip_index_count(sfk_hip_index* index, const uint64_t* d_patterns, const 
 }
 
 uint64_t sfk_hip_index_device_bytes(const sfk_hip_index* index) {
-  return index == nullptr ? 0 : index->packed.Bytes() + index->directory.Bytes();
+  return index == nullptr ? 0 : index->packed.Bytes() + index->directory.Bytes() + index->fm_blocks.Bytes();
+}
+
+int sfk_hip_index_build_fm(sfk_hip_index* index, char* error, size_t error_capacity) {
+  if (index == nullptr) {
+    sfk::CopyError("no index", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  try {
+    SFK_HIP_CHECK(hipSetDevice(index->device));
+    const std::uint64_t n = index->n;
+    const std::uint64_t nblocks = n / 64 + 1;  // occ(c, n) reads record n / 64
+    index->fm_blocks.Reserve(nblocks * sizeof(sfk::FmBlock));
+    index->fm_header.Reserve(sizeof(sfk::FmHeader));
+    index->fm_partial.Reserve(1024 * sizeof(sfk::FmCounts));
+    hipStream_t s = index->stream;
+    sfk::FmBlock* blocks = index->fm_blocks.As<sfk::FmBlock>();
+    sfk::FmHeader* header = index->fm_header.As<sfk::FmHeader>();
+    sfk::FmCounts* partial = index->fm_partial.As<sfk::FmCounts>();
+    sfk::DeviceCounters* dc = index->counters.As<sfk::DeviceCounters>();
+    const sfk::PackedText t{index->packed.As<uint4>(), n};
+    index->fm_built = false;
+    SFK_HIP_CHECK(hipMemsetAsync(header, 0, sizeof(sfk::FmHeader), s));
+    SFK_HIP_CHECK(hipMemsetAsync(&dc->error_flags, 0, sizeof(unsigned int), s));
+    sfk::FmLowCountKernel<<<static_cast<unsigned>(sfk::CeilDiv(sfk::CeilDiv(n, 32), 256)), 256, 0, s>>>(t, header);
+    const unsigned grid = static_cast<unsigned>(sfk::CeilDiv(nblocks * sfk::kWave, 256));
+    if (index->sa_bits == 32) {
+      sfk::FmBlocksKernel<uint32_t><<<grid, 256, 0, s>>>(t, static_cast<const uint32_t*>(index->d_sa), n, blocks,
+                                                          nblocks, &dc->error_flags);
+    } else {
+      sfk::FmBlocksKernel<uint64_t><<<grid, 256, 0, s>>>(t, static_cast<const uint64_t*>(index->d_sa), n, blocks,
+                                                          nblocks, &dc->error_flags);
+    }
+    const sfk::SortGeometry g = sfk::Geometry(nblocks, sfk::kFmTile, 1024);
+    sfk::FmReduceKernel<<<g.grid, 256, 0, s>>>(blocks, nblocks, g.chunk, partial);
+    sfk::FmSpineKernel<<<1, 1024, 0, s>>>(partial, g.grid, header);
+    sfk::FmApplyKernel<<<g.grid, 256, 0, s>>>(blocks, nblocks, g.chunk, partial);
+    SFK_HIP_CHECK(hipGetLastError());
+    SFK_HIP_CHECK(hipMemcpyAsync(index->host_counters, dc, sizeof(sfk::DeviceCounters), hipMemcpyDeviceToHost, s));
+    SFK_HIP_CHECK(hipStreamSynchronize(s));
+    if (index->host_counters->error_flags != 0) {
+      sfk::Fail(SFK_HIP_INVALID_INPUT, "index SA holds a value outside the text");
+    }
+    index->fm_built = true;
+    return SFK_HIP_OK;
+  } catch (const sfk::Failure& f) {
+    sfk::CopyError(f.message, error, error_capacity);
+    return f.status;
+  }
+}
+
+uint64_t sfk_hip_index_fm_bytes(const sfk_hip_index* index) {
+  return index == nullptr || !index->fm_built ? 0 : index->fm_blocks.Bytes();
+}
+
+int sfk_hip_index_fm_header(sfk_hip_index* index, uint64_t out[6]) {
+  if (index == nullptr || !index->fm_built || out == nullptr) return SFK_HIP_INVALID_INPUT;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  sfk::FmHeader h{};
+  if (hipMemcpy(&h, index->fm_header.As<void>(), sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
+    return SFK_HIP_BUILD_FAILURE;
+  }
+  for (int c = 0; c < 5; ++c) out[c] = h.first[c];
+  out[5] = h.low;
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_index_fm_ranges(sfk_hip_index* index, const uint8_t* d_codes, const uint64_t* d_offsets,
+                            const uint8_t* d_orientation, uint64_t count, uint64_t* d_ranges, void* stream) {
+  if (index == nullptr || !index->fm_built ||
+      (count > 0 && (d_codes == nullptr || d_offsets == nullptr || d_orientation == nullptr || d_ranges == nullptr))) {
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (count == 0) return SFK_HIP_OK;
+  hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  const std::uint64_t* dir = index->directory_k > 0 ? index->directory.As<std::uint64_t>() : nullptr;
+  sfk::FmRangesKernel<<<static_cast<unsigned>(sfk::CeilDiv(count, 256)), 256, 0, s>>>(
+      index->fm_blocks.As<sfk::FmBlock>(), index->fm_header.As<sfk::FmHeader>(), index->n, dir,
+      static_cast<int>(index->directory_k), sfk::DirectoryWide(index->n), d_codes, d_offsets, d_orientation, count,
+      d_ranges);
+  if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  if (stream == nullptr && hipStreamSynchronize(s) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_index_fm_count(sfk_hip_index* index, const uint64_t* d_patterns, const uint8_t* d_lengths,
+                           uint64_t count, uint32_t* d_counts, void* stream) {
+  if (index == nullptr || !index->fm_built ||
+      (count > 0 && (d_patterns == nullptr || d_lengths == nullptr || d_counts == nullptr))) {
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (count == 0) return SFK_HIP_OK;
+  hipStream_t s = stream != nullptr ? static_cast<hipStream_t>(stream) : index->stream;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  const std::uint64_t* dir = index->directory_k > 0 ? index->directory.As<std::uint64_t>() : nullptr;
+  sfk::FmCountPackedKernel<<<static_cast<unsigned>(sfk::CeilDiv(count, 256)), 256, 0, s>>>(
+      index->fm_blocks.As<sfk::FmBlock>(), index->fm_header.As<sfk::FmHeader>(), index->n, dir,
+      static_cast<int>(index->directory_k), sfk::DirectoryWide(index->n), d_patterns, d_lengths, count, d_counts);
+  if (hipGetLastError() != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  if (stream == nullptr && hipStreamSynchronize(s) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return SFK_HIP_OK;
+}
+
+int sfk_hip_index_sa_rows(sfk_hip_index* index, uint64_t begin, uint64_t end, uint64_t* out) {
+  if (index == nullptr || begin > end || end > index->n || (end > begin && out == nullptr)) {
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (end == begin) return SFK_HIP_OK;
+  if (hipSetDevice(index->device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  if (hipStreamSynchronize(index->stream) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  const std::uint64_t rows = end - begin;
+  if (index->sa_bits == 64) {
+    const auto* src = static_cast<const std::uint64_t*>(index->d_sa) + begin;
+    if (hipMemcpy(out, src, rows * 8, hipMemcpyDeviceToHost) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+    return SFK_HIP_OK;
+  }
+  // 32-bit rows land in the upper half of the output, then widen in place
+  // from the front (element i is read before any write can reach it).
+  const auto* src = static_cast<const std::uint32_t*>(index->d_sa) + begin;
+  auto* staged = reinterpret_cast<std::uint32_t*>(out) + rows;
+  if (hipMemcpy(staged, src, rows * 4, hipMemcpyDeviceToHost) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  for (std::uint64_t i = 0; i < rows; ++i) out[i] = staged[i];
+  return SFK_HIP_OK;
 }
 
 uint32_t sfk_hip_index_prefix_k(const sfk_hip_index* index) {

@@ -216,6 +216,131 @@ int sfk_sa_count(const sfk_suffix_array* sa, const char* pattern, uint64_t lengt
                [&] { *count = sa->value.Count(std::string_view(pattern, length)); });
 }
 
+// ------------------------------------------------------------ FM index
+
+struct sfk_fm_index {
+  sufkit::FmIndex value;
+};
+struct sfk_fm_hits {
+  sufkit::QueryResult value;
+};
+
+namespace {
+
+std::vector<std::string_view> SplitPatterns(const char* patterns, const uint64_t* offsets, uint64_t count) {
+  std::vector<std::string_view> out(count);
+  for (uint64_t i = 0; i < count; ++i) out[i] = std::string_view(patterns + offsets[i], offsets[i + 1] - offsets[i]);
+  return out;
+}
+
+sufkit::StrandMode StrandOf(int strands) {
+  if (strands < 0 || strands > 2) {
+    sufkit::detail::Throw(sufkit::ErrorCode::kInvalidInput, "invalid strand mode for FM query");
+  }
+  return static_cast<sufkit::StrandMode>(strands);
+}
+
+}  // namespace
+
+int sfk_fm_build(const sfk_reference* r, int backend, sfk_fm_index** out, char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    sufkit::FmIndexBuildOptions options;
+    options.backend = static_cast<sufkit::FmBackend>(backend);
+    *out = new sfk_fm_index{sufkit::FmIndex::Build(r->value, options)};
+  });
+}
+
+void sfk_fm_free(sfk_fm_index* fm) { delete fm; }
+
+// {text symbols, sequences, total bases, fingerprint, FM device bytes, prefix k, SA width}.
+void sfk_fm_summary(const sfk_fm_index* fm, uint64_t out[7]) {
+  const auto& i = fm->value.GetInfo();
+  out[0] = i.text_symbols;
+  out[1] = i.sequence_count;
+  out[2] = i.total_bases;
+  out[3] = i.fingerprint;
+  out[4] = i.auxiliary_bytes;
+  out[5] = fm->value.PrefixDirectoryK();
+  out[6] = i.coordinate_width;
+}
+
+int sfk_fm_equal_range(const sfk_fm_index* fm, const char* pattern, uint64_t length, uint64_t out[2], char* error,
+                       size_t capacity) {
+  return Guard(error, capacity, [&] {
+    const auto r = fm->value.EqualRange(std::string_view(pattern, length));
+    out[0] = r.begin;
+    out[1] = r.end;
+  });
+}
+
+// Patterns concatenated; pattern i is patterns[offsets[i] .. offsets[i + 1]).
+int sfk_fm_equal_range_batch(const sfk_fm_index* fm, const char* patterns, const uint64_t* offsets, uint64_t count,
+                             uint32_t batch_width, uint64_t* out, char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    const auto ranges = fm->value.EqualRangeBatch(SplitPatterns(patterns, offsets, count), batch_width);
+    for (uint64_t i = 0; i < count; ++i) {
+      out[2 * i] = ranges[i].begin;
+      out[2 * i + 1] = ranges[i].end;
+    }
+  });
+}
+
+int sfk_fm_count(const sfk_fm_index* fm, const char* pattern, uint64_t length, int strands, uint64_t* out,
+                 char* error, size_t capacity) {
+  return Guard(error, capacity,
+               [&] { *out = fm->value.Count(std::string_view(pattern, length), StrandOf(strands)); });
+}
+
+int sfk_fm_count_batch(const sfk_fm_index* fm, const char* patterns, const uint64_t* offsets, uint64_t count,
+                       int strands, uint32_t batch_width, uint64_t* out, char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    sufkit::FmBatchOptions options;
+    options.strands = StrandOf(strands);
+    options.batch_width = batch_width;
+    const auto counts = fm->value.CountBatch(SplitPatterns(patterns, offsets, count), options);
+    std::memcpy(out, counts.data(), count * sizeof(uint64_t));
+  });
+}
+
+// has_max != 0: keep the max_hits smallest hits (LocateOptions::max_hits).
+int sfk_fm_locate(const sfk_fm_index* fm, const char* pattern, uint64_t length, int strands, int has_max,
+                  uint64_t max_hits, sfk_fm_hits** out, char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    sufkit::LocateOptions options;
+    options.strands = StrandOf(strands);
+    if (has_max != 0) options.max_hits = max_hits;
+    *out = new sfk_fm_hits{fm->value.Locate(std::string_view(pattern, length), options)};
+  });
+}
+
+// {total hits, retained hits, truncated}.
+void sfk_fm_hits_summary(const sfk_fm_hits* h, uint64_t out[3]) {
+  out[0] = h->value.total_hits;
+  out[1] = h->value.hits.size();
+  out[2] = h->value.truncated ? 1 : 0;
+}
+
+void sfk_fm_hits_get(const sfk_fm_hits* h, uint32_t* sequence_ids, uint64_t* positions, uint64_t* lengths,
+                     uint8_t* strands) {
+  const auto& hits = h->value.hits;
+  for (size_t i = 0; i < hits.size(); ++i) {
+    sequence_ids[i] = hits[i].sequence_id;
+    positions[i] = hits[i].position;
+    lengths[i] = hits[i].length;
+    strands[i] = static_cast<uint8_t>(hits[i].strand);
+  }
+}
+
+void sfk_fm_hits_free(sfk_fm_hits* h) { delete h; }
+
+int sfk_fm_save(const sfk_fm_index* fm, const char* path, int overwrite, char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    sufkit::SaveOptions opts;
+    opts.overwrite = overwrite != 0;
+    fm->value.Save(path, opts);
+  });
+}
+
 int sfk_hip_backend_available(void) { return sufkit::HipBackendAvailable() ? 1 : 0; }
 
 // The width/backend policy SuffixArray::Build applies, without building:

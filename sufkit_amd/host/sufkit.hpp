@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <filesystem>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -195,6 +196,91 @@ class SuffixArray {
 
  private:
   explicit SuffixArray(std::unique_ptr<Impl> impl);
+  std::unique_ptr<Impl> impl_;
+};
+
+// ------------------------------------------------------------ FM index
+//
+// The reference's FmIndex surface (include/sufkit/fm_index.hpp,
+// types.hpp:200-229,564-610) over an FM-index that this build constructs on
+// the device from the GPU-built SA (BWT membership masks with per-block base
+// counts, kept resident) instead of an SDSL CSA on the host.  Queries return
+// what the reference returns: the SA of text ++ sentinel is unique, so row
+// ranges, counts and located coordinates are those of the SDSL index.
+
+// types.hpp:200-209.  The three SDSL types select the same device index
+// here (their query results are identical); kSdslCsaSada stays unavailable.
+enum class FmBackend : std::uint8_t {
+  kSdslCsaWtHuff = 1,
+  kSdslCsaWtBalanced = 2,
+  kSdslCsaSada = 3,
+  kSdslCsaWtEpr = 4
+};
+enum class StrandMode : std::uint8_t { kForward = 0, kReverseComplement = 1, kBoth = 2 };
+enum class Strand : std::uint8_t { kForward = 0, kReverseComplement = 1, kBoth = 2 };
+
+struct FmIndexBuildOptions {
+  FmBackend backend = FmBackend::kSdslCsaWtHuff;
+};
+
+struct FmBatchOptions {
+  StrandMode strands = StrandMode::kForward;
+  std::uint32_t batch_width = 0;  // 0 selects 16; explicit 1..256
+};
+
+struct LocateOptions {
+  StrandMode strands = StrandMode::kForward;
+  std::optional<std::uint64_t> max_hits;  // absent: every hit
+};
+
+struct SuffixRange {
+  std::uint64_t begin = 0;
+  std::uint64_t end = 0;
+  std::uint64_t Size() const noexcept { return end - begin; }
+  bool Empty() const noexcept { return begin == end; }
+};
+
+struct Match {
+  SequenceId sequence_id = 0;
+  Position position = 0;
+  std::uint64_t length = 0;
+  Strand strand = Strand::kForward;
+};
+
+struct QueryResult {
+  std::uint64_t total_hits = 0;
+  std::vector<Match> hits;
+  bool truncated = false;
+};
+
+class FmIndex {
+ public:
+  static FmIndex Build(const GenomeReference& reference, const FmIndexBuildOptions& options = {});
+  // The SDSL payload of an FM .sufidx is not produced by this build.
+  static FmIndex Load(const std::filesystem::path& path);
+  void Save(const std::filesystem::path& path, const SaveOptions& options = {}) const;
+
+  FmIndex(FmIndex&&) noexcept;
+  FmIndex& operator=(FmIndex&&) noexcept;
+  ~FmIndex();
+
+  SuffixRange EqualRange(std::string_view pattern) const;
+  std::vector<SuffixRange> EqualRangeBatch(const std::vector<std::string_view>& patterns,
+                                           std::uint32_t batch_width = 0) const;
+  std::uint64_t Count(std::string_view pattern, StrandMode strands = StrandMode::kForward) const;
+  std::vector<std::uint64_t> CountBatch(const std::vector<std::string_view>& patterns,
+                                        const FmBatchOptions& options = {}) const;
+  QueryResult Locate(std::string_view pattern, const LocateOptions& options = {}) const;
+  SequenceInfo GetSequenceInfo(SequenceId id) const;
+  const IndexInfo& GetInfo() const noexcept;
+  // k of the device index's prefix directory (0: none) and its FM bytes.
+  std::uint32_t PrefixDirectoryK() const noexcept;
+  std::uint64_t DeviceBytes() const noexcept;
+
+  struct Impl;
+
+ private:
+  explicit FmIndex(std::unique_ptr<Impl> impl);
   std::unique_ptr<Impl> impl_;
 };
 
