@@ -338,7 +338,7 @@ class ServeConfig(ctypes.Structure):
                 ("queries_per_index", ctypes.c_uint64), ("min_query_len", ctypes.c_uint32),
                 ("max_query_len", ctypes.c_uint32), ("query_seed", ctypes.c_uint64),
                 ("keep_resident", ctypes.c_uint32), ("sample_queries", ctypes.c_uint32),
-                ("digest_outputs", ctypes.c_uint32)]
+                ("digest_outputs", ctypes.c_uint32), ("fm_queries", ctypes.c_uint32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -361,9 +361,11 @@ class ServeSummary(ctypes.Structure):
 def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
           hbm_budget_bytes: int = 200 << 30, queries_per_index: int = 1_000_000,
           min_query_len: int = 20, max_query_len: int = 32, query_seed: int = 7,
-          keep_resident: bool = True, sample_queries: int = 0, digest_outputs: bool = False):
+          keep_resident: bool = True, sample_queries: int = 0, digest_outputs: bool = False,
+          fm_queries: bool = True):
     """Run the serving loop (sufkit_amd/host/serve.cpp) over host texts with
-    the given arrival times.  Returns (results, summary, samples)."""
+    the given arrival times; count queries by FM backward search (fm_queries)
+    or SA binary search.  Returns (results, summary, samples)."""
     lib = load_host()
     lib.sfk_serve_run.argtypes = [ctypes.POINTER(ServeRequest), ctypes.c_uint64, ctypes.POINTER(ServeConfig),
                                   ctypes.POINTER(ServeResult), ctypes.c_void_p, ctypes.c_void_p,
@@ -374,7 +376,7 @@ def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
     count = len(texts)
     reqs = (ServeRequest * count)(*[ServeRequest(a, t.ctypes.data, t.size) for a, t in zip(arrivals, texts)])
     cfg = ServeConfig(hbm_budget_bytes, slots, queries_per_index, min_query_len, max_query_len,
-                      query_seed, int(keep_resident), sample_queries, int(digest_outputs))
+                      query_seed, int(keep_resident), sample_queries, int(digest_outputs), int(fm_queries))
     results = (ServeResult * count)()
     summary = ServeSummary()
     sp = np.zeros(max(1, count * sample_queries), np.uint64)
@@ -552,3 +554,63 @@ def tune_reset(device: int = 0) -> None:
 def decode_pattern(code: int, length: int) -> str:
     """2-bit packed pattern (first symbol in the top bits) to an A/C/G/T string."""
     return "".join("ACGT"[(int(code) >> (62 - 2 * j)) & 3] for j in range(int(length)))
+
+
+def encode_pattern(pattern: str) -> int:
+    """A/C/G/T string (at most 32) to the 2-bit packed form of the count queries."""
+    code = 0
+    for j, ch in enumerate(pattern):
+        code |= "ACGT".index(ch) << (62 - 2 * j)
+    return code
+
+
+def _index_lib():
+    lib = load_hip()
+    vp = ctypes.c_void_p
+    lib.sfk_hip_index_create.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, vp, ctypes.c_uint32,
+                                         ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_index_create.restype = ctypes.c_int
+    lib.sfk_hip_index_destroy.argtypes = [vp]
+    lib.sfk_hip_index_destroy.restype = None
+    lib.sfk_hip_index_build_fm.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_index_build_fm.restype = ctypes.c_int
+    for name in ("sfk_hip_index_count", "sfk_hip_index_fm_count"):
+        getattr(lib, name).argtypes = [vp, vp, vp, ctypes.c_uint64, vp, vp]
+        getattr(lib, name).restype = ctypes.c_int
+    return lib
+
+
+def index_count(text: np.ndarray, sa: np.ndarray, patterns: list[str], fm: bool = False) -> list[int]:
+    """Count queries (at most 32 bases each) on a resident device index over
+    a text and its SA: SA binary search, or FM backward search (fm)."""
+    lib = _index_lib()
+    text = np.ascontiguousarray(text, np.uint8)
+    sa = np.ascontiguousarray(sa)
+    n = text.size
+    d_text, d_sa = DeviceArray(n + 64), DeviceArray(sa.nbytes)
+    d_text.upload(text)
+    d_sa.upload(sa)
+    index = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_index_create(0, d_text.ptr, n, d_sa.ptr, 8 * sa.itemsize, ctypes.byref(index), err,
+                                      len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+    try:
+        if fm:
+            status = lib.sfk_hip_index_build_fm(index, err, len(err))
+            if status != 0:
+                raise SufkitError(status, err.value.decode(errors="replace"))
+        q = len(patterns)
+        codes = np.array([encode_pattern(p) for p in patterns], np.uint64)
+        lens = np.array([len(p) for p in patterns], np.uint8)
+        d_p, d_l, d_c = DeviceArray(8 * q), DeviceArray(q), DeviceArray(4 * q)
+        d_p.upload(codes)
+        d_l.upload(lens)
+        fn = lib.sfk_hip_index_fm_count if fm else lib.sfk_hip_index_count
+        status = fn(index, d_p.ptr, d_l.ptr, q, d_c.ptr, None)
+        if status != 0:
+            raise SufkitError(status, "count failed")
+        return [int(v) for v in d_c.download(np.uint32, q)]
+    finally:
+        lib.sfk_hip_index_destroy(index)

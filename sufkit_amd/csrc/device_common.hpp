@@ -112,6 +112,20 @@ __device__ inline std::uint32_t WindowMismatch(std::uint64_t ca, std::uint32_t s
 
 __device__ inline std::uint32_t LaneId() { return __lane_id(); }
 
+// a != b for 64-bit values, decided by a 32-bit compare of the folded
+// difference.  On gfx950 a data-dependent 64-bit compare whose mask feeds
+// scalar exec/mask arithmetic directly was observed to select wrong lanes
+// (the prefix directory lost and misplaced entries; the same code with this
+// form was exact in every run), so per-lane predicates that gate stores use
+// this form.  The empty asm statement only keeps the optimiser from widening
+// the compare back to 64 bits; it emits no instruction.
+__device__ __forceinline__ bool Differ64(std::uint64_t a, std::uint64_t b) {
+  const std::uint64_t x = a ^ b;
+  std::uint32_t folded = static_cast<std::uint32_t>(x) | static_cast<std::uint32_t>(x >> 32);
+  asm volatile("" : "+v"(folded));
+  return folded != 0u;
+}
+
 __device__ inline std::uint64_t LaneMaskLt() {
   return (1ull << LaneId()) - 1ull;
 }

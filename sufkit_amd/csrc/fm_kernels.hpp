@@ -121,23 +121,20 @@ __global__ __launch_bounds__(1024) void FmSpineKernel(FmCounts* __restrict__ par
                                                       FmHeader* __restrict__ header) {
   __shared__ std::uint64_t scratch[1024 / kWave];
   const bool valid = static_cast<int>(threadIdx.x) < grid;
-  const FmCounts p = valid ? partial[threadIdx.x] : FmCounts{{0, 0, 0, 0}};
-  FmCounts out;
-  std::uint64_t totals[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    out.c[c] = BlockExclusiveSum<1024>(static_cast<std::uint64_t>(p.c[c]), scratch, totals[c]);
-  }
-  if (valid) partial[threadIdx.x] = out;
-  if (threadIdx.x == 0) {
-    std::uint64_t row = header->low;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
+  std::uint64_t row = 0;
+  if (threadIdx.x == 0) row = header->low;
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {  // one base at a time keeps the block's registers low
+    const std::uint64_t v = valid ? static_cast<std::uint64_t>(partial[threadIdx.x].c[c]) : 0ull;
+    std::uint64_t total;
+    const std::uint64_t before = BlockExclusiveSum<1024>(v, scratch, total);
+    if (valid) partial[threadIdx.x].c[c] = before;
+    if (threadIdx.x == 0) {
       header->first[c] = row;
-      row += totals[c];
+      row += total;
     }
-    header->first[4] = row;
   }
+  if (threadIdx.x == 0) header->first[4] = row;
 }
 
 __global__ __launch_bounds__(256) void FmApplyKernel(FmBlock* __restrict__ blocks, std::uint64_t nblocks,
@@ -211,7 +208,8 @@ __device__ inline void FmSearch(const FmBlock* __restrict__ blocks, const FmHead
     hi = h.first[c + 1];
     i = static_cast<std::int64_t>(len) - 2;
   }
-  for (; i >= 0 && lo < hi; --i) {
+  // lo <= hi throughout (occ is monotone), so lo < hi is lo != hi.
+  for (; i >= 0 && Differ64(lo, hi); --i) {
     const std::uint32_t c = sym(static_cast<std::uint32_t>(i));
     lo = h.first[c] + FmOcc(blocks, c, lo);
     hi = h.first[c] + FmOcc(blocks, c, hi);

@@ -67,10 +67,21 @@ __global__ __launch_bounds__(256) void PrefixDirectoryFillKernel(std::uint64_t* 
 }
 
 // counts[0] += rows with a canonical k-mer (indexed k-mers), counts[1] +=
-// distinct k-mers (non-empty entries).
-template <typename SaT>
+// distinct k-mers (non-empty entries); counts[2 ..] is a sink of
+// kDirectorySinkWords words.
+//
+// Every row stores twice, unconditionally: a row that starts (ends) its
+// k-mer's run stores into the table, any other row into its thread's sink
+// slot.  The key compares use Differ64 (see device_common.hpp), and the
+// table layout is a compile-time choice: with it a run-time branch inside
+// the store loop, this kernel lost and misplaced entries on gfx950, while
+// the same code specialised per layout was exact in every run.  The two rows
+// of the tile's halo are keyed by two lanes of the first wave.
+constexpr int kDirectorySinkWords = 2 * 256;
+
+template <typename SaT, bool kWide>
 __global__ __launch_bounds__(256) void PrefixDirectoryKernel(PackedText t, const SaT* __restrict__ sa,
-                                                             std::uint64_t n, int k, bool wide,
+                                                             std::uint64_t n, int k,
                                                              std::uint64_t* __restrict__ table,
                                                              unsigned long long* __restrict__ counts) {
   __shared__ std::uint64_t s_key[kDirectoryTile + 2];  // s_key[i + 1] = row base + i
@@ -89,37 +100,44 @@ __global__ __launch_bounds__(256) void PrefixDirectoryKernel(PackedText t, const
     const int i = j * 256 + static_cast<int>(threadIdx.x);
     if (i < len) s_key[i + 1] = p[j] < n ? DirectoryKey(t, p[j], k) : kNoDirectoryKey;
   }
-  if (threadIdx.x == 0) {
-    const std::uint64_t before = base > 0 ? static_cast<std::uint64_t>(sa[base - 1]) : n;
-    const std::uint64_t after = base + len < n ? static_cast<std::uint64_t>(sa[base + len]) : n;
-    s_key[0] = before < n ? DirectoryKey(t, before, k) : kNoDirectoryKey;
-    s_key[len + 1] = after < n ? DirectoryKey(t, after, k) : kNoDirectoryKey;
+  if (threadIdx.x < 2) {  // lane 0: the row before the tile, lane 1: the row after it
+    const bool after = threadIdx.x == 1;
+    const bool inside = after ? base + len < n : base > 0;
+    std::uint64_t key = kNoDirectoryKey;
+    if (inside) {
+      const std::uint64_t q = static_cast<std::uint64_t>(sa[after ? base + len : base - 1]);
+      if (q < n) key = DirectoryKey(t, q, k);
+    }
+    s_key[after ? len + 1 : 0] = key;
   }
   __syncthreads();
   const std::uint64_t entries = 1ull << (2 * k);
+  std::uint64_t* sink = reinterpret_cast<std::uint64_t*>(counts + 2);
+  std::uint32_t* table32 = reinterpret_cast<std::uint32_t*>(table);
+  std::uint32_t* sink32 = reinterpret_cast<std::uint32_t*>(sink);
   std::uint64_t indexed = 0, distinct = 0;
 #pragma unroll
   for (int j = 0; j < kDirectoryItems; ++j) {
     const int i = j * 256 + static_cast<int>(threadIdx.x);
-    if (i >= len) break;
-    const std::uint64_t key = s_key[i + 1];
-    if (key == kNoDirectoryKey) continue;
+    const bool in = i < len;
+    const std::uint64_t key = in ? s_key[i + 1] : kNoDirectoryKey;
+    const bool valid = Differ64(key, kNoDirectoryKey);
+    const bool starts = valid && Differ64(s_key[i], key);
+    const bool ends = valid && Differ64(s_key[i + 2], key);
     const std::uint64_t row = base + static_cast<std::uint64_t>(i);
-    ++indexed;
-    if (s_key[i] != key) {
-      ++distinct;
-      if (wide) {
-        table[key] = row;
-      } else {
-        reinterpret_cast<std::uint32_t*>(table)[2 * key] = static_cast<std::uint32_t>(row);
-      }
-    }
-    if (s_key[i + 2] != key) {
-      if (wide) {
-        table[entries + key] = row + 1;
-      } else {
-        reinterpret_cast<std::uint32_t*>(table)[2 * key + 1] = static_cast<std::uint32_t>(row + 1);
-      }
+    indexed += valid ? 1 : 0;
+    distinct += starts ? 1 : 0;
+    const std::uint64_t slot = valid ? key : 0;
+    if constexpr (kWide) {
+      std::uint64_t* b = starts ? &table[slot] : &sink[2 * threadIdx.x];
+      std::uint64_t* e = ends ? &table[entries + slot] : &sink[2 * threadIdx.x + 1];
+      *b = row;
+      *e = row + 1;
+    } else {
+      std::uint32_t* b = starts ? &table32[2 * slot] : &sink32[2 * threadIdx.x];
+      std::uint32_t* e = ends ? &table32[2 * slot + 1] : &sink32[2 * threadIdx.x + 1];
+      *b = static_cast<std::uint32_t>(row);
+      *e = static_cast<std::uint32_t>(row + 1);
     }
   }
   std::uint64_t total_indexed, total_distinct;

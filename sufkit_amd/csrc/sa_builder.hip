@@ -1404,7 +1404,10 @@ std::uint32_t SelectDirectoryK(std::uint64_t suffix_count, std::uint64_t residen
 }
 
 // Fills the table and runs the directory pass on `stream`; counts (2 words
-// of device memory) receive the indexed k-mers and the non-empty entries.
+// of device memory followed by kDirectorySinkWords of sink) receive the
+// indexed k-mers and the non-empty entries.
+static_assert((2 + kDirectorySinkWords) * sizeof(unsigned long long) <= 1024 * sizeof(SplitPartial),
+              "the builder's partial buffer holds the directory counts and sink");
 void LaunchPrefixDirectory(hipStream_t stream, PackedText t, const void* d_sa, std::uint32_t sa_bits,
                            std::uint64_t n, std::uint32_t k, std::uint64_t* table, unsigned long long* counts) {
   const bool wide = DirectoryWide(n);
@@ -1414,12 +1417,21 @@ void LaunchPrefixDirectory(hipStream_t stream, PackedText t, const void* d_sa, s
   PrefixDirectoryFillKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, stream>>>(table, words, empty);
   SFK_HIP_CHECK(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), stream));
   const unsigned grid = static_cast<unsigned>(CeilDiv(n, kDirectoryTile));
+  const int kk = static_cast<int>(k);
   if (sa_bits == 32) {
-    PrefixDirectoryKernel<std::uint32_t><<<grid, 256, 0, stream>>>(
-        t, static_cast<const std::uint32_t*>(d_sa), n, static_cast<int>(k), wide, table, counts);
+    const auto* sa = static_cast<const std::uint32_t*>(d_sa);
+    if (wide) {
+      PrefixDirectoryKernel<std::uint32_t, true><<<grid, 256, 0, stream>>>(t, sa, n, kk, table, counts);
+    } else {
+      PrefixDirectoryKernel<std::uint32_t, false><<<grid, 256, 0, stream>>>(t, sa, n, kk, table, counts);
+    }
   } else {
-    PrefixDirectoryKernel<std::uint64_t><<<grid, 256, 0, stream>>>(
-        t, static_cast<const std::uint64_t*>(d_sa), n, static_cast<int>(k), wide, table, counts);
+    const auto* sa = static_cast<const std::uint64_t*>(d_sa);
+    if (wide) {
+      PrefixDirectoryKernel<std::uint64_t, true><<<grid, 256, 0, stream>>>(t, sa, n, kk, table, counts);
+    } else {
+      PrefixDirectoryKernel<std::uint64_t, false><<<grid, 256, 0, stream>>>(t, sa, n, kk, table, counts);
+    }
   }
   SFK_HIP_CHECK(hipGetLastError());
 }
@@ -1465,7 +1477,10 @@ int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bi
     if (want_isa) d_isa.Reserve(n * (isa_bits / 8));
     if (want_lcp) d_lcp.Reserve(n * (lcp_bits / 8));
     const auto t0 = std::chrono::steady_clock::now();
-    SFK_HIP_CHECK(hipMemcpy(d_text.As<void>(), text, n, hipMemcpyHostToDevice));
+    // Ordered on the builder's stream: a copy from pageable memory through
+    // hipMemcpy may return before its last chunk lands, and the builder's
+    // stream does not wait for the null stream.
+    SFK_HIP_CHECK(hipMemcpyAsync(d_text.As<void>(), text, n, hipMemcpyHostToDevice, b->stream));
     const auto t1 = std::chrono::steady_clock::now();
     sfk_hip_stats local{};
     local.h2d_seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1629,7 +1644,7 @@ int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const vo
     index->directory_k = sfk::SelectDirectoryK(n, n * (1 + 3 * (sa_bits / 8)));
     if (index->directory_k > 0) {
       index->directory.Reserve(sfk_prefix_directory_words(index->directory_k, n) * 8);
-      index->directory_counts.Reserve(2 * sizeof(unsigned long long));
+      index->directory_counts.Reserve((2 + sfk::kDirectorySinkWords) * sizeof(unsigned long long));
       sfk::LaunchPrefixDirectory(index->stream, sfk::PackedText{index->packed.As<uint4>(), n}, d_sa, sa_bits, n,
                                  index->directory_k, index->directory.As<std::uint64_t>(),
                                  index->directory_counts.As<unsigned long long>());
@@ -1858,8 +1873,11 @@ int sfk_hip_free(void* ptr) {
 }
 
 int sfk_hip_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
-  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? SFK_HIP_OK
-                                                                         : SFK_HIP_BUILD_FAILURE;
+  // A copy from pageable memory may return before its last chunk lands on
+  // the device; the null stream's synchronisation waits for it, so work on
+  // any stream launched after this call sees the data.
+  if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return hipStreamSynchronize(nullptr) == hipSuccess ? SFK_HIP_OK : SFK_HIP_BUILD_FAILURE;
 }
 
 int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {

@@ -1153,20 +1153,25 @@ __global__ __launch_bounds__(256) void PartitionCountKernel(
 // counters->special and counters->stash.
 __global__ __launch_bounds__(1024) void PartitionSpineKernel(PartitionPartial* partial, int grid,
                                                              DeviceCounters* counters) {
-  constexpr int kBlock = 1024;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
+  __shared__ std::uint64_t scratch[1024 / kWave];
   const bool valid = static_cast<int>(threadIdx.x) < grid;
-  const PartitionPartial p = valid ? partial[threadIdx.x] : PartitionPartial{0, 0, 0, 0};
-  std::uint64_t t_xs, t_xe, t_ys, t_ye;
-  PartitionPartial out;
-  out.x_segs = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.x_segs), scratch, t_xs);
-  out.x_elems = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.x_elems), scratch, t_xe);
-  out.y_segs = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.y_segs), scratch, t_ys);
-  out.y_elems = BlockExclusiveSum<kBlock>(static_cast<std::uint64_t>(p.y_elems), scratch, t_ye);
-  if (valid) partial[threadIdx.x] = out;
+  std::uint64_t totals[4] = {0, 0, 0, 0};
+  unsigned long long PartitionPartial::*fields[4] = {&PartitionPartial::x_segs, &PartitionPartial::x_elems,
+                                                     &PartitionPartial::y_segs, &PartitionPartial::y_elems};
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {  // one count at a time keeps the block's registers low
+    const std::uint64_t v = valid ? static_cast<std::uint64_t>(partial[threadIdx.x].*fields[f]) : 0ull;
+    std::uint64_t total;
+    const std::uint64_t before = BlockExclusiveSum<1024>(v, scratch, total);
+    if (valid) partial[threadIdx.x].*fields[f] = before;
+    if (f == 0) totals[0] = total;
+    if (f == 1) totals[1] = total;
+    if (f == 2) totals[2] = total;
+    if (f == 3) totals[3] = total;
+  }
   if (threadIdx.x == 0) {
-    counters->special = ListSize{t_xe, t_xs};
-    counters->stash = ListSize{t_ye, t_ys};
+    counters->special = ListSize{totals[1], totals[0]};
+    counters->stash = ListSize{totals[3], totals[2]};
   }
 }
 

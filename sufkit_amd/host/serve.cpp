@@ -7,7 +7,9 @@
 // others are on the GPU start as slots and memory free up, and their kernels
 // overlap on the device.  A finished index (text, SA, ISA, LCP) stays
 // resident; the worker then answers the request's exact count queries on the
-// device against it.  Per request the loop records arrival, admission, build
+// device against it (by backward search over an FM-index built on the
+// device from the resident SA when fm_queries is set, else by binary search
+// over the SA).  Per request the loop records arrival, admission, build
 // completion and query completion; the summary gives p50/p99 latency, Mbp/s
 // and queries/s.
 #include <algorithm>
@@ -41,6 +43,7 @@ struct sfk_serve_config {
   uint32_t keep_resident;   // keep every finished index on the device until the end
   uint32_t sample_queries;  // queries per request copied back for checking
   uint32_t digest_outputs;  // copy SA/ISA/LCP back and hash them (checking)
+  uint32_t fm_queries;      // answer by FM backward search (FM-index built on the resident SA)
 };
 
 struct sfk_serve_result {
@@ -163,7 +166,8 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
           const uint64_t ws = sfk_hip_workspace_bytes(r.n, r.n - 1 <= 0xFFFFFFFFull ? 32 : 64);
           const uint64_t growth = ws > workspace_reserved[slot] ? ws - workspace_reserved[slot] : 0;
           const uint64_t w = r.n - 1 <= 0xFFFFFFFFull ? 4 : 8;
-          const uint64_t want = (r.n + 64) + 3 * r.n * w + r.n / 2 + 64 + config->queries_per_index * 13;
+          const uint64_t want = (r.n + 64) + 3 * r.n * w + r.n / 2 + 64 + (config->fm_queries ? r.n + 64 : 0) +
+                                config->queries_per_index * 13;
           if (reserved + growth + want <= config->hbm_budget_bytes) {
             admitted = true;
           } else if (!resident_order.empty()) {
@@ -187,7 +191,8 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         ws_growth = ws > workspace_reserved[slot] ? ws - workspace_reserved[slot] : 0;
         workspace_reserved[slot] += ws_growth;
         const uint64_t w = r.n - 1 <= 0xFFFFFFFFull ? 4 : 8;
-        need = (r.n + 64) + 3 * r.n * w + r.n / 2 + 64 + config->queries_per_index * 13;
+        need = (r.n + 64) + 3 * r.n * w + r.n / 2 + 64 + (config->fm_queries ? r.n + 64 : 0) +
+               config->queries_per_index * 13;
         reserved += ws_growth + need;
         peak = std::max(peak, reserved);
         max_concurrent = std::max(max_concurrent, ++concurrent);
@@ -234,6 +239,9 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         status = sfk_hip_index_create(0, static_cast<const uint8_t*>(rs.text), n, rs.sa, bits,
                                       &rs.index, err, sizeof(err));
       }
+      if (status == SFK_HIP_OK && config->fm_queries) {
+        status = sfk_hip_index_build_fm(rs.index, err, sizeof(err));
+      }
       const uint64_t q = config->queries_per_index;
       if (status == SFK_HIP_OK && q > 0) {
         const double qs = now();
@@ -247,9 +255,13 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
                                                   static_cast<uint8_t*>(lengths), nullptr);
         }
         if (status == SFK_HIP_OK) {
-          status = sfk_hip_index_count(rs.index, static_cast<uint64_t*>(patterns),
-                                       static_cast<uint8_t*>(lengths), q,
-                                       static_cast<uint32_t*>(counts), nullptr);
+          status = config->fm_queries
+                       ? sfk_hip_index_fm_count(rs.index, static_cast<uint64_t*>(patterns),
+                                                static_cast<uint8_t*>(lengths), q,
+                                                static_cast<uint32_t*>(counts), nullptr)
+                       : sfk_hip_index_count(rs.index, static_cast<uint64_t*>(patterns),
+                                             static_cast<uint8_t*>(lengths), q,
+                                             static_cast<uint32_t*>(counts), nullptr);
         }
         if (status == SFK_HIP_OK) {
           std::vector<uint32_t> host(q);

@@ -332,3 +332,44 @@ def test_host_fm_index_with_a_prefix_directory(hip):
             p = "".join("ACGT"[c - 2] for c in plain[at:at + length])
             assert fm.equal_range(p) == oracle.equal_range(text, p), p
             assert as_hits(fm.locate(p, 2)) == as_hits(oracle.locate(text, starts, p, 2)), p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("request_index", [6, 15, 1])
+def test_fm_counts_equal_sa_counts_on_a_served_request(hip, request_index):
+    # The three smallest requests of the served workload (1.0-1.6 M symbols,
+    # prefix directory k = 9), each with its million generated count queries.
+    lib = index_lib()
+    text, _ = hip.synthetic_text(5, 20260822, request_index, 1.0)
+    n = text.size
+    sa, _, _, _ = hip.hip_build(text)
+    d_text, d_sa = hip.DeviceArray(n + 64), hip.DeviceArray(sa.nbytes)
+    d_text.upload(text)
+    d_sa.upload(sa)
+    index = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(256)
+    assert lib.sfk_hip_index_create(0, d_text.ptr, n, d_sa.ptr, 32, ctypes.byref(index), err, 256) == 0
+    try:
+        assert lib.sfk_hip_index_build_fm(index, err, 256) == 0, err.value
+        q = 1_000_000
+        d_p, d_l = hip.DeviceArray(8 * q), hip.DeviceArray(q)
+        d_c1, d_c2 = hip.DeviceArray(4 * q), hip.DeviceArray(4 * q)
+        seed = 7 ^ ((request_index * 0x9E3779B97F4A7C15) & ((1 << 64) - 1))
+        assert lib.sfk_hip_index_generate_queries(index, seed, q, 20, 32, d_p.ptr, d_l.ptr, None) == 0
+        assert lib.sfk_hip_index_count(index, d_p.ptr, d_l.ptr, q, d_c1.ptr, None) == 0
+        assert lib.sfk_hip_index_fm_count(index, d_p.ptr, d_l.ptr, q, d_c2.ptr, None) == 0
+        c1, c2 = d_c1.download(np.uint32, q), d_c2.download(np.uint32, q)
+        bad = np.nonzero(c1 != c2)[0]
+        if bad.size:
+            pats, lens = d_p.download(np.uint64, q), d_l.download(np.uint8, q)
+            s = "".join("$|ACGTN"[c] for c in text.tolist())
+            for b in bad[:5].tolist():
+                p = hip.decode_pattern(int(pats[b]), int(lens[b]))
+                want, at = 0, s.find(p)
+                while at != -1:
+                    want += 1
+                    at = s.find(p, at + 1)
+                print(f"query {b}: {p} (len {lens[b]}) sa {c1[b]} fm {c2[b]} scan {want}", flush=True)
+        assert bad.size == 0, f"{bad.size} FM counts differ from SA counts"
+    finally:
+        lib.sfk_hip_index_destroy(index)

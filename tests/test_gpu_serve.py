@@ -60,6 +60,17 @@ def test_results_identical_alone_and_in_the_mix(hip):
             assert got == naive_count(s, hip.decode_pattern(code, length))
 
 
+def test_fm_and_sa_count_queries_agree(hip):
+    # The served loop answers by FM backward search by default; binary
+    # search over the resident SA gives the same counts for every query.
+    texts, arrivals = requests(seed=6, count=4)
+    fm, fsum, _ = hip.serve(texts, arrivals, slots=2, queries_per_index=20000, fm_queries=True)
+    sa, ssum, _ = hip.serve(texts, arrivals, slots=2, queries_per_index=20000, fm_queries=False)
+    assert all(r["status"] == 0 for r in fm + sa)
+    for f, s in zip(fm, sa):
+        assert f["count_hash"] == s["count_hash"] and f["count_sum"] == s["count_sum"] > 0
+
+
 def test_tiny_budget_admits_one_at_a_time(hip):
     texts, arrivals = requests(seed=9, count=4)
     res, summary, _ = hip.serve(texts, [0.0] * 4, slots=4, hbm_budget_bytes=1, queries_per_index=100,
@@ -124,3 +135,62 @@ def test_count_queries_on_resident_index(hip):
     lib.sfk_hip_index_destroy(index)
     for p, g in zip(pats, got):
         assert g == naive_count(s, p), p
+
+
+SERVED_SMALLEST = [6, 15, 1, 7]  # the four smallest requests of the served workload (1.0-4.6 M symbols)
+
+
+def test_served_workload_requests_match_the_oracle(hip):
+    import oracle
+
+    for r in SERVED_SMALLEST[:3]:
+        text, _ = hip.synthetic_text(5, 20260822, r, 1.0)
+        sa, isa, lcp, _ = hip.hip_build(text)
+        osa, oisa, olcp = oracle.fast_build(text)
+        assert np.array_equal(sa.astype(np.uint64), osa), f"request {r}: SA differs"
+        assert np.array_equal(isa.astype(np.uint64), oisa), f"request {r}: ISA differs"
+        assert np.array_equal(lcp.astype(np.uint64), olcp), f"request {r}: LCP differs"
+
+
+def test_served_workload_requests_identical_in_the_mix(hip):
+    # The smallest served requests built concurrently on four slots, one
+    # after another on one reused builder, and each on a fresh builder give
+    # the same SA, ISA, LCP and counts.
+    texts = [hip.synthetic_text(5, 20260822, r, 1.0)[0] for r in SERVED_SMALLEST]
+    arrivals = [0.0, 0.001, 0.002, 0.003]
+    q = 20000
+    mix, msum, ms = hip.serve(texts, arrivals, slots=4, queries_per_index=q, digest_outputs=True, sample_queries=q)
+    alone, _, als = hip.serve(texts, arrivals, slots=1, queries_per_index=q, digest_outputs=True, sample_queries=q)
+    # A lone request is request 0 of its run: its query seed is the one the
+    # loop derives for request i (seed ^ i * 0x9E3779B97F4A7C15).
+    fresh = [hip.serve([t], [0.0], slots=1, queries_per_index=q, digest_outputs=True, sample_queries=q,
+                       query_seed=7 ^ ((i * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)))
+             for i, t in enumerate(texts)]
+    assert all(r["status"] == 0 for r in mix + alone + [f[0][0] for f in fresh]) and msum["max_concurrent"] >= 2
+    for i, (text, m, a, f) in enumerate(zip(texts, mix, alone, fresh)):
+        fr = f[0][0]
+        if m["count_hash"] != fr["count_hash"] or a["count_hash"] != fr["count_hash"]:
+            s = render(text)
+            for name, samples in (("mix", ms), ("alone", als)):
+                diff = np.nonzero(samples[2][i] != f[2][2][0])[0]
+                for k in diff[:4].tolist():
+                    p = hip.decode_pattern(int(samples[0][i][k]), int(samples[1][i][k]))
+                    print(f"request {i} ({name}) query {k} {p}: {samples[2][i][k]} vs fresh {f[2][2][0][k]}, "
+                          f"scan {naive_count(s, p)}; patterns equal {samples[0][i][k] == f[2][0][0][k]}",
+                          flush=True)
+        for key in ("output_hash", "count_hash", "count_sum"):
+            assert m[key] == fr[key] and a[key] == fr[key], (text.size, key)
+
+
+@pytest.mark.parametrize("fm", [False, True])
+def test_runs_of_one_base_are_counted(hip, fm):
+    # Poly-A and period-2 blocks (the adversarial block of the served
+    # requests) hold long runs: counts of A^L and (AT)^L against a scan.
+    text, _ = hip.synthetic_text(5, 20260822, 15, 1.0)
+    s = render(text)
+    sa, _, _, _ = hip.hip_build(text)
+    pats = ["A" * L for L in (1, 5, 8, 9, 10, 11, 20, 28, 32)] + ["AT" * L for L in (1, 5, 14, 16)] + \
+        ["T" * 28, "C" * 9, "ACGTACGT"]
+    got = hip.index_count(text, sa, pats, fm=fm)
+    for p, g in zip(pats, got):
+        assert g == naive_count(s, p), (p, g)
