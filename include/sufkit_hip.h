@@ -277,6 +277,54 @@ int sfk_hip_tune_set(int device, const sfk_hip_tune_entry* entries, uint32_t cou
                      char* error, size_t error_capacity);
 int sfk_hip_tune_reset(int device);
 
+/*
+ * Streaming .sufidx section payloads from device buffers, with CRC-32
+ * computed on the device.  Replaces, for data resident in HBM, the
+ * checksumming stream buffer the reference writes every section through
+ * (serialization.cpp:60-141, WriteContainer :529-619) and the plane writes
+ * of WriteCoordinateSectionV14 / WriteLcpSectionV14 (sa_codec.cpp:466-631).
+ *
+ * A plane kind names the stored form of each value:
+ *   BYTES     the bytes as they are (value_bits 8; the text section);
+ *   NATIVE32  32-bit values (native32 codec, raw32 LCP); values must fit;
+ *   LOW32     the low 32 bits (first plane of split40 / split48);
+ *   HIGH8     bits 32..39 (second plane of split40); values < 2^40;
+ *   HIGH16    bits 32..47 (second plane of split48); values < 2^48;
+ *   NATIVE64  64-bit values (native64 codec, raw64 LCP).
+ * Device values are 32- or 64-bit (value_bits) for every kind but BYTES.
+ * The plane's bytes are handed to `sink` in order, in chunks of at most the
+ * writer's chunk size (pinned host memory, valid during the call); a
+ * non-zero return from the sink aborts with SFK_HIP_IO_ERROR.  *crc_out is
+ * zlib's crc32(crc_in, plane bytes), so a section's checksum chains through
+ * its header bytes (checksummed by the caller) and its planes.  A value out
+ * of its plane's range gives SFK_HIP_INVALID_INPUT.  The call waits for all
+ * prior work on the device before reading the values.
+ */
+#define SFK_HIP_PLANE_BYTES 0u
+#define SFK_HIP_PLANE_NATIVE32 1u
+#define SFK_HIP_PLANE_LOW32 2u
+#define SFK_HIP_PLANE_HIGH8 3u
+#define SFK_HIP_PLANE_HIGH16 4u
+#define SFK_HIP_PLANE_NATIVE64 5u
+typedef struct sfk_hip_plane_writer sfk_hip_plane_writer;
+typedef int (*sfk_hip_sink)(void* context, const void* data, uint64_t bytes);
+/* chunk_bytes 0 selects 64 MiB; two pinned host and two device buffers of
+ * that size are held by the writer. */
+int sfk_hip_plane_writer_create(int device, uint64_t chunk_bytes,
+                                sfk_hip_plane_writer** out, char* error,
+                                size_t error_capacity);
+void sfk_hip_plane_writer_destroy(sfk_hip_plane_writer* writer);
+int sfk_hip_write_plane(sfk_hip_plane_writer* writer, const void* d_values,
+                        uint64_t count, uint32_t value_bits, uint32_t plane,
+                        uint32_t crc_in, sfk_hip_sink sink, void* context,
+                        uint32_t* crc_out, char* error, size_t error_capacity);
+/* zlib's crc32(crc_in, bytes at d_data) computed on the device (any
+ * alignment); the UpdateCrc / CrcFileRange checksums of
+ * serialization.cpp:47-58,200-240 for data in HBM. */
+int sfk_hip_crc32(int device, const void* d_data, uint64_t bytes,
+                  uint32_t crc_in, uint32_t* crc_out, char* error,
+                  size_t error_capacity);
+
 /* Device memory helpers so callers without a device allocator of their own
  * can stage buffers (tests, benchmarks). */
 int sfk_hip_malloc(int device, uint64_t bytes, void** out);

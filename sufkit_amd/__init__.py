@@ -614,3 +614,23 @@ def index_count(text: np.ndarray, sa: np.ndarray, patterns: list[str], fm: bool 
         return [int(v) for v in d_c.download(np.uint32, q)]
     finally:
         lib.sfk_hip_index_destroy(index)
+
+
+def device_crc32(data: np.ndarray, crc_in: int = 0, offset: int = 0) -> int:
+    """zlib's crc32(crc_in, data) computed on the device: `data` is uploaded
+    `offset` bytes into a fresh device buffer, so offset sets the alignment."""
+    lib = load_hip()
+    lib.sfk_hip_crc32.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_uint32), ctypes.c_char_p, ctypes.c_size_t]
+    lib.sfk_hip_crc32.restype = ctypes.c_int
+    data = np.ascontiguousarray(data, np.uint8)
+    buf = DeviceArray(offset + data.size + 16)
+    if data.size:
+        if lib.sfk_hip_memcpy_h2d(buf.ptr + offset, data.ctypes.data, data.size) != 0:
+            raise RuntimeError("host to device copy failed")
+    out = ctypes.c_uint32()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_crc32(0, buf.ptr + offset, data.size, crc_in, ctypes.byref(out), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+    return int(out.value)

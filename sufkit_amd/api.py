@@ -99,6 +99,13 @@ def _lib():
     lib.sfk_fm_hits_free.argtypes = [vp]
     lib.sfk_fm_hits_free.restype = None
     lib.sfk_fm_save.argtypes = [vp, cp, ctypes.c_int, cp, sz]
+    u32 = ctypes.c_uint32
+    lib.sfk_save_device_index.argtypes = [vp, ctypes.c_int, vp, vp, u32, vp, u32, vp, u32, u32,
+                                          ctypes.POINTER(_BuildOptions), cp, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double), cp, sz]
+    lib.sfk_save_device_index.restype = ctypes.c_int
+    lib.sfk_file_crc32.argtypes = [cp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(u32), cp, sz]
+    lib.sfk_file_crc32.restype = ctypes.c_int
     _BOUND = True
     return lib
 
@@ -369,3 +376,56 @@ def resolve_build_policy(backend: int, coordinate_width: int, storage_width: int
 
 def hip_backend_available() -> bool:
     return bool(_lib().sfk_hip_backend_available())
+
+
+def file_crc32(path: str | Path, offset: int = 0, size: int | None = None) -> int:
+    """zlib CRC32 of a byte range of a file, as the container reader checks
+    section checksums (several threads, joined)."""
+    if size is None:
+        size = Path(path).stat().st_size - offset
+    out = ctypes.c_uint32()
+    err = ctypes.create_string_buffer(512)
+    _check(_lib().sfk_file_crc32(str(path).encode(), offset, size, ctypes.byref(out), err, len(err)), err)
+    return int(out.value)
+
+
+def save_device_index(reference: GenomeReference, path: str | Path, options: BuildOptions | None = None, *,
+                      construction_bits: int = 32, sa_bits: int | None = None, isa_bits: int | None = None,
+                      lcp_bits: int = 32, overwrite: bool = False, device_sa: np.ndarray | None = None) -> dict:
+    """Build `reference` on the device (sfk_hip_build_device) and write its
+    .sufidx straight from the device arrays (SaveDeviceIndex).  The device
+    widths default to 64 bits when the stored width is wider than 32.
+    `device_sa` replaces the built SA by the given values (error-path tests).
+    Returns the save statistics."""
+    from . import DeviceArray, DeviceBuilder, SFK_HIP_RETAIN_ISA, SFK_HIP_RETAIN_LCP
+    o = options or BuildOptions()
+    wide = o.storage_width in (40, 48, 64)
+    sa_bits = sa_bits or (64 if wide else 32)
+    isa_bits = isa_bits or (64 if wide else 32)
+    text = reference.text()
+    if text.size == 0 or text[-1] != 0:
+        text = np.append(text, np.uint8(0))
+    n = int(text.size)
+    keep_isa = o.acceleration == ACCEL_LCP_SUFFIX_LINK
+    keep_lcp = o.acceleration != ACCEL_NONE
+    d_text = DeviceArray(n + 64)
+    d_text.upload(text)
+    d_sa = DeviceArray(n * sa_bits // 8)
+    d_isa = DeviceArray(n * isa_bits // 8)
+    d_lcp = DeviceArray(n * lcp_bits // 8)
+    flags = (SFK_HIP_RETAIN_ISA if keep_isa else 0) | (SFK_HIP_RETAIN_LCP if keep_lcp else 0)
+    with DeviceBuilder() as builder:
+        builder.build(d_text.ptr, n, construction_bits, d_sa.ptr, sa_bits, d_isa.ptr if keep_isa else None,
+                      isa_bits, d_lcp.ptr if keep_lcp else None, lcp_bits, flags=flags)
+    if device_sa is not None:
+        d_sa.upload(np.ascontiguousarray(device_sa, dtype=np.uint64 if sa_bits == 64 else np.uint32))
+    copts = _BuildOptions(o.backend, o.coordinate_width, o.storage_width, o.resource_profile,
+                          o.acceleration, o.threads, o.sampling_rate)
+    stats = (ctypes.c_double * 4)()
+    err = ctypes.create_string_buffer(512)
+    _check(_lib().sfk_save_device_index(reference._h, 0, d_text.ptr, d_sa.ptr, sa_bits,
+                                        d_isa.ptr if keep_isa else None, isa_bits,
+                                        d_lcp.ptr if keep_lcp else None, lcp_bits, construction_bits,
+                                        ctypes.byref(copts), str(path).encode(), int(overwrite), stats, err,
+                                        len(err)), err)
+    return dict(seconds=stats[0], write_seconds=stats[1], validate_seconds=stats[2], bytes=int(stats[3]))

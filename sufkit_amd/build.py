@@ -50,7 +50,7 @@ def build_hip() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libsufkit_hip.so"
     tmp = LIB / "libsufkit_hip.so.tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "sa_builder.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), *map(str, sorted(CSRC.glob("*.hip")))])
     os.replace(tmp, out)
     return out
 

@@ -156,6 +156,49 @@ int sfk_sa_save(const sfk_suffix_array* sa, const char* path, int overwrite, cha
   });
 }
 
+// SaveDeviceIndex: the file Build(options) + Save would write, streamed from
+// device arrays.  stats (optional): seconds, write seconds, validate
+// seconds, bytes.
+int sfk_save_device_index(const sfk_reference* r, int device, const void* d_text, const void* d_sa,
+                          uint32_t sa_bits, const void* d_isa, uint32_t isa_bits, const void* d_lcp,
+                          uint32_t lcp_bits, uint32_t construction_bits, const sfk_build_options* o,
+                          const char* path, int overwrite, double stats[4], char* error, size_t capacity) {
+  return Guard(error, capacity, [&] {
+    sufkit::SuffixArrayBuildOptions options;
+    if (o != nullptr) {
+      options.storage_width = static_cast<sufkit::CoordinateStorageWidth>(o->storage_width);
+      options.resource_profile = static_cast<sufkit::SaResourceProfile>(o->resource_profile);
+      options.acceleration = static_cast<sufkit::SaAcceleration>(o->acceleration);
+      options.sampling_rate = o->sampling_rate;
+    }
+    sufkit::DeviceIndexBuffers b;
+    b.device = device;
+    b.text = d_text;
+    b.sa = d_sa;
+    b.sa_bits = sa_bits;
+    b.isa = d_isa;
+    b.isa_bits = isa_bits;
+    b.lcp = d_lcp;
+    b.lcp_bits = lcp_bits;
+    b.construction_width = static_cast<sufkit::CoordinateWidth>(construction_bits);
+    sufkit::SaveOptions save;
+    save.overwrite = overwrite != 0;
+    sufkit::DeviceSaveStatistics st;
+    sufkit::SaveDeviceIndex(r->value, b, path, save, options, &st);
+    if (stats != nullptr) {
+      stats[0] = st.seconds;
+      stats[1] = st.write_seconds;
+      stats[2] = st.validate_seconds;
+      stats[3] = static_cast<double>(st.bytes);
+    }
+  });
+}
+
+int sfk_file_crc32(const char* path, uint64_t offset, uint64_t size, uint32_t* out, char* error,
+                   size_t capacity) {
+  return Guard(error, capacity, [&] { *out = sufkit::detail::CrcFileRange(path, offset, size); });
+}
+
 int sfk_sa_load(const char* path, sfk_suffix_array** out, char* error, size_t capacity) {
   return Guard(error, capacity, [&] { *out = new sfk_suffix_array{sufkit::SuffixArray::Load(path)}; });
 }

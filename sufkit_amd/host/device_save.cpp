@@ -1,0 +1,157 @@
+// SaveDeviceIndex: the .sufidx file of a device build, streamed from HBM.
+//
+// The sections, their order and bytes are those SuffixArray::Save writes
+// for the same build (metadata, text, SA, ISA, LCP; suffix_array.cpp:
+// 4853-4916): the stored widths follow the same policies Build applies
+// (ResolveProfileStorageWidth, ResolveAuxiliaryStorageWidth, raw LCP of 32
+// bits unless n - 1 needs 64), the codec headers are built on the host, and
+// every payload plane is streamed from device memory by
+// sfk_hip_write_plane, which checksums it on the device.
+#include <chrono>
+#include <limits>
+#include <sstream>
+#include <string>
+
+#include "internal.hpp"
+#include "sufkit_hip.h"
+
+namespace sufkit {
+
+namespace {
+
+using detail::StreamPart;
+using detail::StreamSection;
+
+StreamPart HostPart(std::vector<std::uint8_t> bytes) {
+  StreamPart p;
+  p.host = std::move(bytes);
+  return p;
+}
+
+StreamPart DevicePart(const void* values, std::uint64_t count, std::uint32_t bits, std::uint32_t plane) {
+  StreamPart p;
+  p.on_device = true;
+  p.device_values = values;
+  p.count = count;
+  p.value_bits = bits;
+  p.plane = plane;
+  return p;
+}
+
+// SKC1 section: codec header, then the planes of the stored width.
+StreamSection CoordinateSection(detail::SectionType type, CoordinateStorageWidth width, const void* values,
+                                std::uint32_t bits, std::uint64_t count, std::uint64_t symbol_count) {
+  StreamSection s{type, {}};
+  s.parts.push_back(HostPart(detail::CoordinateSectionHeader(width, symbol_count, count)));
+  switch (width) {
+    case CoordinateStorageWidth::kBits32:
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_NATIVE32));
+      break;
+    case CoordinateStorageWidth::kBits40:
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_LOW32));
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_HIGH8));
+      break;
+    case CoordinateStorageWidth::kBits48:
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_LOW32));
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_HIGH16));
+      break;
+    case CoordinateStorageWidth::kBits64:
+      s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_NATIVE64));
+      break;
+    default:
+      detail::Throw(ErrorCode::kBuildFailure, "coordinate storage width is not resolved");
+  }
+  return s;
+}
+
+void CheckBits(std::uint32_t bits, const char* label) {
+  if (bits != 32 && bits != 64) {
+    detail::Throw(ErrorCode::kInvalidInput, std::string(label) + " device values must be 32 or 64 bits wide");
+  }
+}
+
+}  // namespace
+
+void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers& buffers,
+                     const std::filesystem::path& path, const SaveOptions& save,
+                     const SuffixArrayBuildOptions& options, DeviceSaveStatistics* statistics) {
+  const auto begin = std::chrono::steady_clock::now();
+  const std::uint64_t n = reference.Encoded().size() + 1;
+  if (n < 2) detail::Throw(ErrorCode::kInvalidInput, "reference text is empty");
+  if (options.sampling_rate != 1) {
+    detail::Throw(ErrorCode::kInvalidInput, "saving from device memory needs sampling rate 1");
+  }
+  if (options.resource_profile == SaResourceProfile::kLowMemory) {
+    detail::Throw(ErrorCode::kUnsupportedBackend,
+                  "the Low-memory profile (byte-coded LCP) is not part of this build");
+  }
+  if (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull) {
+    detail::Throw(ErrorCode::kUnsupportedBackend, "CHILD arrays are not part of this build");
+  }
+  if (static_cast<std::uint8_t>(options.acceleration) > static_cast<std::uint8_t>(SaAcceleration::kFull)) {
+    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array acceleration");
+  }
+  const auto width = buffers.construction_width;
+  if (width != CoordinateWidth::kBits32 && width != CoordinateWidth::kBits64) {
+    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array coordinate width");
+  }
+  if (!detail::SaConstructionCanRepresent(SaBackend::kHip, width, n)) {
+    detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
+  }
+  const bool retain_lcp = options.acceleration != SaAcceleration::kNone;
+  const bool retain_isa = options.acceleration == SaAcceleration::kLcpSuffixLink;
+  if (buffers.text == nullptr || buffers.sa == nullptr || (retain_isa && buffers.isa == nullptr) ||
+      (retain_lcp && buffers.lcp == nullptr)) {
+    detail::Throw(ErrorCode::kInvalidInput, "device text, SA and the retained ISA / LCP are required");
+  }
+  CheckBits(buffers.sa_bits, "SA");
+  if (retain_isa) CheckBits(buffers.isa_bits, "ISA");
+  if (retain_lcp) CheckBits(buffers.lcp_bits, "LCP");
+  const auto stored = detail::ResolveProfileStorageWidth(options.storage_width, options.resource_profile, n);
+
+  detail::ContainerSpec spec;
+  spec.format_minor = 4;
+  spec.kind = 1;
+  spec.backend = width == CoordinateWidth::kBits32 ? detail::StoredBackend::kHip32 : detail::StoredBackend::kHip64;
+  spec.coordinate_width = static_cast<std::uint8_t>(width);
+  spec.profile = options.resource_profile;
+  spec.sequence_count = reference.Sequences().size();
+  spec.total_bases = reference.TotalBases();
+  spec.text_symbols = n;
+  spec.ambiguous_bases = reference.AmbiguousBases();
+  spec.fingerprint = reference.Fingerprint();
+
+  std::vector<StreamSection> sections;
+  {
+    std::ostringstream meta;
+    detail::WriteMetadataSection(meta, reference.Sequences());
+    const std::string bytes = meta.str();
+    sections.push_back({detail::SectionType::kMetadata, {HostPart({bytes.begin(), bytes.end()})}});
+  }
+  sections.push_back({detail::SectionType::kText, {DevicePart(buffers.text, n, 8, SFK_HIP_PLANE_BYTES)}});
+  sections.push_back(
+      CoordinateSection(detail::SectionType::kSuffixArray, stored, buffers.sa, buffers.sa_bits, n, n));
+  if (retain_isa) {
+    const auto aux = detail::ResolveAuxiliaryStorageWidth(stored, n);
+    sections.push_back(
+        CoordinateSection(detail::SectionType::kInverseSuffixArray, aux, buffers.isa, buffers.isa_bits, n, n));
+  }
+  if (retain_lcp) {
+    const std::uint32_t lcp_width = n - 1U > std::numeric_limits<std::uint32_t>::max() ? 64 : 32;
+    StreamSection s{detail::SectionType::kLcp, {}};
+    s.parts.push_back(HostPart(detail::LcpSectionHeader(lcp_width, 1, n, n)));
+    s.parts.push_back(DevicePart(buffers.lcp, n, buffers.lcp_bits,
+                                 lcp_width == 32 ? SFK_HIP_PLANE_NATIVE32 : SFK_HIP_PLANE_NATIVE64));
+    sections.push_back(std::move(s));
+  }
+  detail::StreamStatistics s;
+  detail::WriteContainerStreamed(path, save, spec, sections, buffers.device, &s);
+  if (statistics != nullptr) {
+    statistics->write_seconds = s.write_seconds;
+    statistics->validate_seconds = s.validate_seconds;
+    statistics->bytes = s.bytes;
+    statistics->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
+  }
+}
+
+}  // namespace sufkit

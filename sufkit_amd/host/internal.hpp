@@ -162,6 +162,13 @@ CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t secti
                                            std::uint64_t expected_elements,
                                            std::uint64_t expected_symbols, const char* label);
 void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t text_symbols);
+// The codec headers those sections start with (SKC1 56 bytes, SKL1 80 bytes).
+std::vector<std::uint8_t> CoordinateSectionHeader(CoordinateStorageWidth width, std::uint64_t symbol_count,
+                                                  std::uint64_t count);
+std::vector<std::uint8_t> LcpSectionHeader(std::uint32_t width, std::uint32_t sampling_rate,
+                                           std::uint64_t text_symbols, std::uint64_t rows);
+// Section 1 (serialization.cpp:785-799): the sequence records.
+void WriteMetadataSection(std::ostream& out, const std::vector<SequenceInfo>& sequences);
 LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size,
                              std::uint64_t expected_rows, std::uint32_t sampling_rate,
                              std::uint64_t text_symbols);
@@ -213,6 +220,35 @@ void WriteContainer(const std::filesystem::path& path, const SaveOptions& option
                     const ContainerSpec& spec, const std::vector<SectionWriter>& writers);
 ParsedContainer ReadContainer(const std::filesystem::path& path);
 std::vector<std::uint8_t> ReadSection(const ParsedContainer& container, SectionType type);
+// zlib CRC32 of bytes [offset, offset + size) of a file, read by several
+// threads in parallel and joined (the reference's CrcFileRange,
+// serialization.cpp:225-244).
+std::uint32_t CrcFileRange(const std::filesystem::path& path, std::uint64_t offset, std::uint64_t size);
+
+// A section whose payload is streamed: host bytes and device planes, in
+// order.  A device plane is handed to sfk_hip_write_plane.
+struct StreamPart {
+  std::vector<std::uint8_t> host;
+  const void* device_values = nullptr;
+  std::uint64_t count = 0;
+  std::uint32_t value_bits = 0;
+  std::uint32_t plane = 0;
+  bool on_device = false;
+};
+struct StreamSection {
+  SectionType type;
+  std::vector<StreamPart> parts;
+};
+struct StreamStatistics {
+  double write_seconds = 0.0;     // sections and header written
+  double validate_seconds = 0.0;  // reread with every CRC checked
+  std::uint64_t bytes = 0;
+};
+// WriteContainer for sections held in device memory: same file, same
+// publication, CRCs of device planes computed on the device.
+void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions& options,
+                            const ContainerSpec& spec, const std::vector<StreamSection>& sections, int device,
+                            StreamStatistics* stats);
 
 void WriteU32(std::ostream& out, std::uint32_t value);
 void WriteU64(std::ostream& out, std::uint64_t value);

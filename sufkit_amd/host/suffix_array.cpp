@@ -336,20 +336,7 @@ void SuffixArray::Save(const std::filesystem::path& path, const SaveOptions& opt
   const Impl& impl = *impl_;
   std::vector<detail::SectionWriter> writers{
       {detail::SectionType::kMetadata,
-       [&](std::ostream& out) {
-         // serialization.cpp:785-799
-         detail::WriteU32(out, static_cast<std::uint32_t>(impl.sequences.size()));
-         for (const auto& s : impl.sequences) {
-           detail::WriteU32(out, s.id);
-           detail::WriteU64(out, s.global_offset);
-           detail::WriteU64(out, s.length);
-           detail::WriteU64(out, s.ambiguous_bases);
-           detail::WriteU32(out, static_cast<std::uint32_t>(s.name.size()));
-           out.write(s.name.data(), static_cast<std::streamsize>(s.name.size()));
-           detail::WriteU32(out, static_cast<std::uint32_t>(s.description.size()));
-           out.write(s.description.data(), static_cast<std::streamsize>(s.description.size()));
-         }
-       }},
+       [&](std::ostream& out) { detail::WriteMetadataSection(out, impl.sequences); }},
       {detail::SectionType::kText,
        [&](std::ostream& out) {
          out.write(reinterpret_cast<const char*>(impl.text.data()), static_cast<std::streamsize>(impl.text.size()));

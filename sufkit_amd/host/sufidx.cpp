@@ -15,8 +15,10 @@
 #include <ostream>
 #include <set>
 #include <system_error>
+#include <thread>
 
 #include "internal.hpp"
+#include "sufkit_hip.h"
 
 namespace sufkit::detail {
 namespace {
@@ -219,28 +221,70 @@ void WriteU64(std::ostream& out, std::uint64_t v) {
 
 // ------------------------------------------------------------- codecs ---
 
-void WriteCoordinateSectionV14(std::ostream& out, const CoordinateStorage& storage,
-                               std::uint64_t symbol_count) {
-  (void)ResolveCoordinateStorageWidth(storage.Width(), symbol_count, ErrorCode::kBuildFailure);
-  const std::uint64_t count = storage.Size();
+std::vector<std::uint8_t> CoordinateSectionHeader(CoordinateStorageWidth width, std::uint64_t symbol_count,
+                                                  std::uint64_t count) {
+  (void)ResolveCoordinateStorageWidth(width, symbol_count, ErrorCode::kBuildFailure);
   CoordinateCodec codec = CoordinateCodec::kNative32;
   std::uint64_t low = 0, high = 0;
-  switch (storage.Width()) {
+  switch (width) {
     case CoordinateStorageWidth::kBits32: codec = CoordinateCodec::kNative32; low = count * 4; break;
     case CoordinateStorageWidth::kBits64: codec = CoordinateCodec::kNative64; low = count * 8; break;
     case CoordinateStorageWidth::kBits40: codec = CoordinateCodec::kSplit40; low = count * 4; high = count; break;
     case CoordinateStorageWidth::kBits48: codec = CoordinateCodec::kSplit48; low = count * 4; high = count * 2; break;
     default: Throw(ErrorCode::kBuildFailure, "coordinate storage width is not resolved");
   }
-  WriteU32(out, kCoordinateMagic);
-  WriteU32(out, kCodecVersion);
-  WriteU32(out, static_cast<std::uint32_t>(codec));
-  WriteU32(out, 0);
-  WriteU64(out, symbol_count);
-  WriteU64(out, count);
-  WriteU64(out, low);
-  WriteU64(out, high);
-  WriteU64(out, 0);
+  std::vector<std::uint8_t> h;
+  h.reserve(kCoordinateHeaderBytes);
+  AppendU32(h, kCoordinateMagic);
+  AppendU32(h, kCodecVersion);
+  AppendU32(h, static_cast<std::uint32_t>(codec));
+  AppendU32(h, 0);
+  AppendU64(h, symbol_count);
+  AppendU64(h, count);
+  AppendU64(h, low);
+  AppendU64(h, high);
+  AppendU64(h, 0);
+  return h;
+}
+
+std::vector<std::uint8_t> LcpSectionHeader(std::uint32_t width, std::uint32_t sampling_rate,
+                                           std::uint64_t text_symbols, std::uint64_t rows) {
+  std::vector<std::uint8_t> h;
+  h.reserve(kLcpHeaderBytes);
+  AppendU32(h, kLcpMagic);
+  AppendU32(h, kCodecVersion);
+  AppendU32(h, 0);  // raw
+  AppendU32(h, width);
+  AppendU32(h, sampling_rate);
+  AppendU32(h, kLcpGuideBlockShift);
+  AppendU64(h, text_symbols);
+  AppendU64(h, rows);
+  AppendU64(h, rows * (width / 8u));
+  AppendU64(h, 0);
+  AppendU64(h, 0);
+  AppendU64(h, 0);
+  AppendU64(h, 0);
+  return h;
+}
+
+void WriteMetadataSection(std::ostream& out, const std::vector<SequenceInfo>& sequences) {
+  WriteU32(out, static_cast<std::uint32_t>(sequences.size()));
+  for (const auto& s : sequences) {
+    WriteU32(out, s.id);
+    WriteU64(out, s.global_offset);
+    WriteU64(out, s.length);
+    WriteU64(out, s.ambiguous_bases);
+    WriteU32(out, static_cast<std::uint32_t>(s.name.size()));
+    out.write(s.name.data(), static_cast<std::streamsize>(s.name.size()));
+    WriteU32(out, static_cast<std::uint32_t>(s.description.size()));
+    out.write(s.description.data(), static_cast<std::streamsize>(s.description.size()));
+  }
+}
+
+void WriteCoordinateSectionV14(std::ostream& out, const CoordinateStorage& storage,
+                               std::uint64_t symbol_count) {
+  const auto header = CoordinateSectionHeader(storage.Width(), symbol_count, storage.Size());
+  WriteBytes(out, header.data(), header.size());
   const auto& raw = storage.Raw();
   switch (raw.index()) {
     case 1: WritePlane(out, std::get<1>(raw)); break;
@@ -313,19 +357,8 @@ void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t 
   if (text_symbols == 0 || lcp.Size() != 1 + (text_symbols - 1) / lcp.SamplingRate()) {
     Throw(ErrorCode::kBuildFailure, "SA section codec: LCP metadata is inconsistent before serialization");
   }
-  WriteU32(out, kLcpMagic);
-  WriteU32(out, kCodecVersion);
-  WriteU32(out, 0);  // raw
-  WriteU32(out, lcp.Width());
-  WriteU32(out, lcp.SamplingRate());
-  WriteU32(out, kLcpGuideBlockShift);
-  WriteU64(out, text_symbols);
-  WriteU64(out, lcp.Size());
-  WriteU64(out, lcp.Size() * (lcp.Width() / 8u));
-  WriteU64(out, 0);
-  WriteU64(out, 0);
-  WriteU64(out, 0);
-  WriteU64(out, 0);
+  const auto header = LcpSectionHeader(lcp.Width(), lcp.SamplingRate(), text_symbols, lcp.Size());
+  WriteBytes(out, header.data(), header.size());
   if (lcp.Width() == 32) {
     WritePlane(out, lcp.Raw32());
   } else {
@@ -548,21 +581,169 @@ ParsedContainer ReadContainer(const std::filesystem::path& path) {
       Throw(ErrorCode::kCorruptIndex, "index sections overlap");
     }
   }
-  // Every section's CRC32, streamed.
-  std::vector<char> buffer(1 << 20);
+  // Every section's CRC32.
   for (const auto& d : c.sections) {
-    in.seekg(static_cast<std::streamoff>(d.offset));
-    std::uint64_t remaining = d.size;
-    std::uint32_t crc = 0;
-    while (remaining > 0) {
-      const auto n = static_cast<std::size_t>(std::min<std::uint64_t>(remaining, buffer.size()));
-      ReadExact(in, buffer.data(), n, "index section");
-      crc = Crc32(crc, buffer.data(), n);
-      remaining -= n;
+    if (CrcFileRange(path, d.offset, d.size) != d.crc32) {
+      Throw(ErrorCode::kCorruptIndex, "index section checksum mismatch");
     }
-    if (crc != d.crc32) Throw(ErrorCode::kCorruptIndex, "index section checksum mismatch");
   }
   return c;
+}
+
+std::uint32_t CrcFileRange(const std::filesystem::path& path, std::uint64_t offset, std::uint64_t size) {
+  constexpr std::uint64_t kPiece = 64ull << 20;
+  constexpr std::size_t kBuffer = 4u << 20;
+  if (size == 0) return 0;
+  const std::uint64_t pieces = (size + kPiece - 1) / kPiece;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned threads = static_cast<unsigned>(std::min<std::uint64_t>({pieces, hw, 16}));
+  const int fd = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
+  if (fd < 0) Throw(ErrorCode::kIoError, "cannot open index: " + path.string());
+  std::vector<std::uint32_t> crcs(pieces, 0);
+  std::atomic<std::uint64_t> next{0};
+  std::atomic<bool> failed{false};
+  auto work = [&] {
+    std::vector<unsigned char> buffer(kBuffer);
+    for (std::uint64_t p = next.fetch_add(1); p < pieces && !failed.load(); p = next.fetch_add(1)) {
+      std::uint64_t at = offset + p * kPiece;
+      std::uint64_t left = std::min(kPiece, size - p * kPiece);
+      uLong crc = crc32(0L, Z_NULL, 0);
+      while (left > 0) {
+        const auto want = static_cast<std::size_t>(std::min<std::uint64_t>(left, kBuffer));
+        const ssize_t got = ::pread(fd, buffer.data(), want, static_cast<off_t>(at));
+        if (got <= 0) {
+          failed = true;
+          return;
+        }
+        crc = crc32(crc, buffer.data(), static_cast<uInt>(got));
+        at += static_cast<std::uint64_t>(got);
+        left -= static_cast<std::uint64_t>(got);
+      }
+      crcs[p] = static_cast<std::uint32_t>(crc);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  ::close(fd);
+  if (failed) Throw(ErrorCode::kCorruptIndex, "index is truncated: cannot read a section");
+  uLong crc = crcs[0];
+  for (std::uint64_t p = 1; p < pieces; ++p) {
+    crc = crc32_combine(crc, crcs[p], static_cast<z_off_t>(std::min(kPiece, size - p * kPiece)));
+  }
+  return static_cast<std::uint32_t>(crc);
+}
+
+namespace {
+
+struct FileSink {
+  int fd = -1;
+  std::uint64_t written = 0;
+  static int Write(void* context, const void* data, std::uint64_t bytes) {
+    auto* sink = static_cast<FileSink*>(context);
+    const auto* p = static_cast<const char*>(data);
+    while (bytes > 0) {
+      const ssize_t n = ::write(sink->fd, p, static_cast<std::size_t>(std::min<std::uint64_t>(bytes, 1u << 30)));
+      if (n <= 0) return 1;
+      p += n;
+      bytes -= static_cast<std::uint64_t>(n);
+      sink->written += static_cast<std::uint64_t>(n);
+    }
+    return 0;
+  }
+};
+
+class PlaneWriter {
+ public:
+  explicit PlaneWriter(int device) {
+    char err[512] = {0};
+    const int status = sfk_hip_plane_writer_create(device, 0, &writer_, err, sizeof(err));
+    if (status != SFK_HIP_OK) Throw(static_cast<ErrorCode>(status - 1), std::string("plane writer: ") + err);
+  }
+  ~PlaneWriter() { sfk_hip_plane_writer_destroy(writer_); }
+  PlaneWriter(const PlaneWriter&) = delete;
+  PlaneWriter& operator=(const PlaneWriter&) = delete;
+  sfk_hip_plane_writer* get() const { return writer_; }
+
+ private:
+  sfk_hip_plane_writer* writer_ = nullptr;
+};
+
+}  // namespace
+
+void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions& options,
+                            const ContainerSpec& spec, const std::vector<StreamSection>& sections, int device,
+                            StreamStatistics* stats) {
+  using Clock = std::chrono::steady_clock;
+  const auto begin = Clock::now();
+  if (sections.empty() || sections.size() > kMaxSections) {
+    Throw(ErrorCode::kBuildFailure, "invalid number of index sections");
+  }
+  if (path.empty()) Throw(ErrorCode::kInvalidInput, "index output path must not be empty");
+  if (!options.overwrite && std::filesystem::exists(path)) {
+    Throw(ErrorCode::kIoError, "index already exists: " + path.string());
+  }
+  PlaneWriter writer(device);
+  const auto temp = TemporaryPathFor(path);
+  FileSink sink;
+  try {
+    sink.fd = ::open(temp.c_str(), O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+    if (sink.fd < 0) Throw(ErrorCode::kIoError, "cannot create temporary index: " + temp.string());
+    const std::size_t header_size = kBaseHeaderSize + sections.size() * kSectionEntrySize;
+    const std::vector<std::uint8_t> placeholder(header_size, 0);
+    if (FileSink::Write(&sink, placeholder.data(), placeholder.size()) != 0) {
+      Throw(ErrorCode::kIoError, "failed while writing index header");
+    }
+    std::vector<SectionDescriptor> written;
+    std::set<SectionType> types;
+    for (const auto& section : sections) {
+      if (!types.insert(section.type).second) Throw(ErrorCode::kBuildFailure, "duplicate index section type");
+      SectionDescriptor d;
+      d.type = section.type;
+      d.flags = 1;
+      d.offset = sink.written;
+      std::uint32_t crc = 0;
+      for (const auto& part : section.parts) {
+        if (!part.on_device) {
+          if (FileSink::Write(&sink, part.host.data(), part.host.size()) != 0) {
+            Throw(ErrorCode::kIoError, "failed while writing index section");
+          }
+          crc = Crc32(crc, part.host.data(), part.host.size());
+          continue;
+        }
+        char err[512] = {0};
+        const int status = sfk_hip_write_plane(writer.get(), part.device_values, part.count, part.value_bits,
+                                               part.plane, crc, &FileSink::Write, &sink, &crc, err, sizeof(err));
+        if (status != SFK_HIP_OK) {
+          Throw(static_cast<ErrorCode>(status - 1), std::string("streaming a device plane: ") + err);
+        }
+      }
+      d.size = sink.written - d.offset;
+      d.crc32 = crc;
+      written.push_back(d);
+    }
+    const auto header = BuildHeader(spec, written);
+    if (::pwrite(sink.fd, header.data(), header.size(), 0) != static_cast<ssize_t>(header.size())) {
+      Throw(ErrorCode::kIoError, "failed to write the index header");
+    }
+    const int closed = ::close(sink.fd);
+    sink.fd = -1;
+    if (closed != 0) Throw(ErrorCode::kIoError, "failed to finalize temporary index");
+    const auto validate = Clock::now();
+    if (stats != nullptr) {
+      stats->write_seconds = std::chrono::duration<double>(validate - begin).count();
+      stats->bytes = sink.written;
+    }
+    (void)ReadContainer(temp);  // self-validate before publication
+    if (stats != nullptr) stats->validate_seconds = std::chrono::duration<double>(Clock::now() - validate).count();
+    Publish(temp, path, options.overwrite);
+  } catch (...) {
+    if (sink.fd >= 0) ::close(sink.fd);
+    std::error_code ignored;
+    std::filesystem::remove(temp, ignored);
+    throw;
+  }
 }
 
 std::vector<std::uint8_t> ReadSection(const ParsedContainer& container, SectionType type) {

@@ -199,6 +199,41 @@ class SuffixArray {
   std::unique_ptr<Impl> impl_;
 };
 
+// ----------------------------------------------- saving from device memory
+//
+// The arrays of a device build (sfk_hip_build_device) held in HIP device
+// memory, with the device text they were built from (encoded ++ sentinel).
+// Value widths are those of the device buffers (32 or 64 bits).
+struct DeviceIndexBuffers {
+  int device = 0;
+  const void* text = nullptr;
+  const void* sa = nullptr;
+  std::uint32_t sa_bits = 32;
+  const void* isa = nullptr;  // required when the options keep the ISA
+  std::uint32_t isa_bits = 32;
+  const void* lcp = nullptr;  // required when the options keep the LCP
+  std::uint32_t lcp_bits = 32;
+  CoordinateWidth construction_width = CoordinateWidth::kBits32;
+};
+
+struct DeviceSaveStatistics {
+  double seconds = 0.0;           // whole call
+  double write_seconds = 0.0;     // sections and header written
+  double validate_seconds = 0.0;  // the file reread with every CRC checked
+  std::uint64_t bytes = 0;
+};
+
+// Writes the .sufidx file that SuffixArray::Build(reference, options)
+// followed by Save(path, save) writes, byte for byte, from the device
+// arrays: each section is streamed from HBM through pinned buffers with its
+// CRC32 computed on the device, then the file is reread and checked and
+// published as Save does (suffix_array.cpp:4853-4916,
+// serialization.cpp:529-619).  Sampling rate 1 only.
+void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers& buffers,
+                     const std::filesystem::path& path, const SaveOptions& save = {},
+                     const SuffixArrayBuildOptions& options = FastSuffixArrayBuildOptions(),
+                     DeviceSaveStatistics* statistics = nullptr);
+
 // ------------------------------------------------------------ FM index
 //
 // The reference's FmIndex surface (include/sufkit/fm_index.hpp,
