@@ -73,6 +73,10 @@ def parse_args():
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in radix shapes instead of tuning them on the device first")
+    ap.add_argument("--save", action="store_true",
+                    help="also write the built index to a .sufidx file straight from the device arrays "
+                         "(SaveDeviceIndex) and time the reference's own Save of the same index")
+    ap.add_argument("--save-dir", default=None, help="directory for the --save files (default: a temporary one)")
     ap.add_argument("--save-tuning", metavar="JSON", help="write the tuned radix shape table here")
     ap.add_argument("--load-tuning", metavar="JSON",
                     help="install a saved radix shape table instead of measuring one")
@@ -352,6 +356,81 @@ def run_served(args) -> int:
     return 0 if bad == 0 and summary["requests_failed"] == 0 else 1
 
 
+def section_table(path: str) -> tuple[bytes, dict]:
+    """Header bytes and {section type: (offset, size, crc)} of a .sufidx file."""
+    import struct
+    with open(path, "rb") as f:
+        head = f.read(80)
+        count, header_size = struct.unpack_from("<II", head, 24)
+        head += f.read(header_size - 80)
+    table = {}
+    for i in range(count):
+        stype, _, offset, size, crc, _ = struct.unpack_from("<IIQQII", head, 80 + 32 * i)
+        table[stype] = (offset, size, crc)
+    return head, table
+
+
+def save_leg(args, text, contigs, d_text, d_sa, d_isa, d_lcp, bits, out_bits) -> dict:
+    """Write the built index from the device arrays (SaveDeviceIndex: planes
+    checksummed on the device, streamed through pinned buffers, the file
+    reread and checked before it is published), then the reference's own
+    SuffixArray::Build + Save of the same reference on the host: the saved
+    sections must be identical, and the reference's Save time is the CPU
+    baseline of this leg."""
+    import shutil
+    import tempfile
+    from sufkit_amd import api
+    import oracle
+    ref = api.GenomeReference.from_encoded(text, contigs)
+    if not np.array_equal(ref.text(), text):
+        raise RuntimeError("the reference's normalized text differs from the benchmark text")
+    opts = api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=bits)
+    tmp = tempfile.mkdtemp(prefix="sufkit_save_", dir=args.save_dir)
+    out: dict = {}
+    try:
+        # The first save on a device also sets up its plane writer (pinned
+        # buffers); it is reported, and the second save is the timed one.
+        ours = os.path.join(tmp, "device.sufidx")
+        first = api.save_device_arrays(ref, ours, d_text, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits,
+                                       bits, opts)
+        os.remove(ours)
+        st = api.save_device_arrays(ref, ours, d_text, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits, bits,
+                                    opts)
+        out = {
+            "what": ("SaveDeviceIndex: .sufidx 1.4 from the device arrays (sections 1/2/3/5/6, CRC32 on the "
+                     "device, pinned double buffering), reread with every CRC checked, then published"),
+            "bytes": st["bytes"], "seconds": round(st["seconds"], 4),
+            "write_seconds": round(st["write_seconds"], 4), "validate_seconds": round(st["validate_seconds"], 4),
+            "GBps": round(st["bytes"] / st["seconds"] / 1e9, 3),
+            "write_GBps": round(st["bytes"] / st["write_seconds"] / 1e9, 3),
+            "first_save_seconds": round(first["seconds"], 4),
+        }
+        if oracle.ref_writer_available() and not args.no_cpu_baseline:
+            letters = np.frombuffer(b"??ACGTN", np.uint8)
+            records, at = [], 0
+            for i, length in enumerate(contigs.tolist()):
+                records.append((f"contig{i}", letters[text[at:at + length]].tobytes()))
+                at += length + 1
+            theirs = os.path.join(tmp, "reference.sufidx")
+            t = oracle.ref_write_sufidx_timed(records, theirs, coordinate_width=bits)
+            out["cpu_baseline"] = {
+                "value": round(st["bytes"] / t["save_seconds"] / 1e9, 3), "unit": "GB/s", "cores": 1,
+                "kind": "reference", "seconds": round(t["save_seconds"], 3),
+                "sample": ("the same index: the reference's own SuffixArray::Save (serialization.cpp "
+                           "WriteContainer, zlib CRC32 while writing, reread for validation) after its "
+                           "divsufsort Build of the same records (not timed)"),
+            }
+            h_ours, s_ours = section_table(ours)
+            h_theirs, s_theirs = section_table(theirs)
+            same = (s_ours.keys() == s_theirs.keys() and all(s_ours[k] == s_theirs[k] for k in s_ours)
+                    and os.path.getsize(ours) == os.path.getsize(theirs)
+                    and {i for i in range(len(h_ours)) if h_ours[i] != h_theirs[i]} <= {14, 72, 73, 74, 75})
+            out["parity_vs_reference_save"] = "sections identical" if same else "MISMATCH"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -530,10 +609,14 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
 
+    if args.save:
+        result["save"] = save_leg(args, text, contigs, d_text, d_sa, d_isa, d_lcp, bits, out_bits)
     builder.close()
     for ptr in (d_text, d_sa, d_isa, d_lcp):
         hip.sfk_hip_free(ptr)
     print(json.dumps(result), flush=True)
+    if args.save and result["save"].get("parity_vs_reference_save", "sections identical") != "sections identical":
+        return 1
     ok = (result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact"
           and result.get("parity_vs_cpu_baseline_caps", "bit-exact") == "bit-exact" and sample_bad == 0)
     return 0 if ok else 1

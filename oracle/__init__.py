@@ -322,6 +322,28 @@ def ref_write_sufidx(records: list[tuple[str, bytes]], path: str, coordinate_wid
         raise RuntimeError(err.value.decode())
 
 
+def ref_write_sufidx_timed(records: list[tuple[str, bytes]], path: str, coordinate_width: int = 0,
+                           storage_width: int = 0, acceleration: int = 3) -> dict:
+    """ref_write_sufidx, returning the wall time of the reference's own
+    normalization, Build and Save (the CPU baseline of saving an index)."""
+    W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
+    W.sfk_ref_write_sufidx_timed.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                             U64P, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.c_char_p, ctypes.c_size_t]
+    W.sfk_ref_write_sufidx_timed.restype = ctypes.c_int
+    names = (ctypes.c_char_p * len(records))(*[r[0].encode() for r in records])
+    seqs = (ctypes.c_char_p * len(records))(*[r[1] for r in records])
+    lens = np.array([len(r[1]) for r in records], dtype=np.uint64)
+    seconds = (ctypes.c_double * 3)()
+    err = ctypes.create_string_buffer(512)
+    rc = W.sfk_ref_write_sufidx_timed(names, seqs, _u64(lens), len(records), str(path).encode(),
+                                      coordinate_width, storage_width, acceleration, seconds, err, len(err))
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return dict(normalize_seconds=seconds[0], build_seconds=seconds[1], save_seconds=seconds[2])
+
+
 # ------------------------------------------------------------ exact search
 
 _BASE_CODE = {ord(c): v for c, v in zip("ACGTacgt", (2, 3, 4, 5, 2, 3, 4, 5))}

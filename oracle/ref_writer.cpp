@@ -8,6 +8,7 @@
 // copied into this repository.  Used by tests/golden/make_golden.py to write
 // golden .sufidx files, so the writer of this repository can be compared
 // with the reference's byte for byte.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -41,6 +42,48 @@ int RefGuard(F&& f) {
 extern "C" {
 
 // acceleration: SaAcceleration value; coordinate_width 0/32/64; storage 0/32/40/48/64.
+// seconds (optional): normalization, Build, Save wall time of the reference's
+// own calls.
+int sfk_ref_write_sufidx_timed(const char* const* names, const char* const* sequences,
+                               const std::uint64_t* lengths, std::uint64_t count, const char* path,
+                               int coordinate_width, int storage_width, int acceleration, double* seconds,
+                               char* error, std::size_t capacity) {
+  try {
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
+    std::vector<sufkit::SequenceRecord> records(count);
+    for (std::uint64_t i = 0; i < count; ++i) {
+      records[i].name = names[i];
+      records[i].sequence.assign(sequences[i], lengths[i]);
+    }
+    const auto reference = sufkit::GenomeReference::FromRecords(std::move(records));
+    sufkit::SuffixArrayBuildOptions options;
+    options.backend = sufkit::SaBackend::kDivsufsort;
+    options.coordinate_width = static_cast<sufkit::CoordinateWidth>(coordinate_width);
+    options.storage_width = static_cast<sufkit::CoordinateStorageWidth>(storage_width);
+    options.acceleration = static_cast<sufkit::SaAcceleration>(acceleration);
+    const auto t1 = Clock::now();
+    const auto index = sufkit::SuffixArray::Build(reference, options);
+    const auto t2 = Clock::now();
+    sufkit::SaveOptions save;
+    save.overwrite = true;
+    index.Save(path, save);
+    const auto t3 = Clock::now();
+    if (seconds != nullptr) {
+      seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+      seconds[1] = std::chrono::duration<double>(t2 - t1).count();
+      seconds[2] = std::chrono::duration<double>(t3 - t2).count();
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    if (error != nullptr && capacity > 0) {
+      std::strncpy(error, e.what(), capacity - 1);
+      error[capacity - 1] = '\0';
+    }
+    return -1;
+  }
+}
+
 int sfk_ref_write_sufidx(const char* const* names, const char* const* sequences,
                          const std::uint64_t* lengths, std::uint64_t count, const char* path,
                          int coordinate_width, int storage_width, int acceleration,

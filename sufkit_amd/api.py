@@ -419,13 +419,22 @@ def save_device_index(reference: GenomeReference, path: str | Path, options: Bui
                       isa_bits, d_lcp.ptr if keep_lcp else None, lcp_bits, flags=flags)
     if device_sa is not None:
         d_sa.upload(np.ascontiguousarray(device_sa, dtype=np.uint64 if sa_bits == 64 else np.uint32))
+    return save_device_arrays(reference, path, d_text.ptr, d_sa.ptr, sa_bits, d_isa.ptr if keep_isa else None,
+                              isa_bits, d_lcp.ptr if keep_lcp else None, lcp_bits, construction_bits, o,
+                              overwrite=overwrite)
+
+
+def save_device_arrays(reference: GenomeReference, path: str | Path, d_text: int, d_sa: int, sa_bits: int,
+                       d_isa: int | None, isa_bits: int, d_lcp: int | None, lcp_bits: int,
+                       construction_bits: int, options: BuildOptions | None = None, *, overwrite: bool = False,
+                       device: int = 0) -> dict:
+    """SaveDeviceIndex over device arrays already built (device pointers)."""
+    o = options or BuildOptions()
     copts = _BuildOptions(o.backend, o.coordinate_width, o.storage_width, o.resource_profile,
                           o.acceleration, o.threads, o.sampling_rate)
     stats = (ctypes.c_double * 4)()
     err = ctypes.create_string_buffer(512)
-    _check(_lib().sfk_save_device_index(reference._h, 0, d_text.ptr, d_sa.ptr, sa_bits,
-                                        d_isa.ptr if keep_isa else None, isa_bits,
-                                        d_lcp.ptr if keep_lcp else None, lcp_bits, construction_bits,
-                                        ctypes.byref(copts), str(path).encode(), int(overwrite), stats, err,
-                                        len(err)), err)
+    _check(_lib().sfk_save_device_index(reference._h, device, d_text, d_sa, sa_bits, d_isa, isa_bits, d_lcp,
+                                        lcp_bits, construction_bits, ctypes.byref(copts), str(path).encode(),
+                                        int(overwrite), stats, err, len(err)), err)
     return dict(seconds=stats[0], write_seconds=stats[1], validate_seconds=stats[2], bytes=int(stats[3]))

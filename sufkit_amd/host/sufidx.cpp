@@ -12,6 +12,8 @@
 #include <cstring>
 #include <fstream>
 #include <istream>
+#include <map>
+#include <mutex>
 #include <ostream>
 #include <set>
 #include <system_error>
@@ -591,7 +593,7 @@ ParsedContainer ReadContainer(const std::filesystem::path& path) {
 }
 
 std::uint32_t CrcFileRange(const std::filesystem::path& path, std::uint64_t offset, std::uint64_t size) {
-  constexpr std::uint64_t kPiece = 64ull << 20;
+  constexpr std::uint64_t kPiece = 16ull << 20;
   constexpr std::size_t kBuffer = 4u << 20;
   if (size == 0) return 0;
   const std::uint64_t pieces = (size + kPiece - 1) / kPiece;
@@ -637,14 +639,18 @@ std::uint32_t CrcFileRange(const std::filesystem::path& path, std::uint64_t offs
 
 namespace {
 
+// Appends to a file with positioned writes (the header is patched at
+// offset 0 at the end).
 struct FileSink {
   int fd = -1;
   std::uint64_t written = 0;
+
   static int Write(void* context, const void* data, std::uint64_t bytes) {
     auto* sink = static_cast<FileSink*>(context);
     const auto* p = static_cast<const char*>(data);
     while (bytes > 0) {
-      const ssize_t n = ::write(sink->fd, p, static_cast<std::size_t>(std::min<std::uint64_t>(bytes, 1u << 30)));
+      const ssize_t n = ::pwrite(sink->fd, p, static_cast<std::size_t>(std::min<std::uint64_t>(bytes, 1u << 30)),
+                                 static_cast<off_t>(sink->written));
       if (n <= 0) return 1;
       p += n;
       bytes -= static_cast<std::uint64_t>(n);
@@ -654,19 +660,39 @@ struct FileSink {
   }
 };
 
+// One plane writer per device, created on the first save and kept for the
+// life of the process (its pinned buffers cost more to set up than a
+// chunk costs to stream); saves on one device take turns on it.
+constexpr std::uint64_t kPlaneChunkBytes = 16ull << 20;
+
 class PlaneWriter {
  public:
-  explicit PlaneWriter(int device) {
-    char err[512] = {0};
-    const int status = sfk_hip_plane_writer_create(device, 0, &writer_, err, sizeof(err));
-    if (status != SFK_HIP_OK) Throw(static_cast<ErrorCode>(status - 1), std::string("plane writer: ") + err);
+  explicit PlaneWriter(int device) : lock_(Pool().mutex) {
+    auto& slot = Pool().writers[device];
+    if (slot == nullptr) {
+      char err[512] = {0};
+      const int status = sfk_hip_plane_writer_create(device, kPlaneChunkBytes, &slot, err, sizeof(err));
+      if (status != SFK_HIP_OK) {
+        slot = nullptr;
+        Throw(static_cast<ErrorCode>(status - 1), std::string("plane writer: ") + err);
+      }
+    }
+    writer_ = slot;
   }
-  ~PlaneWriter() { sfk_hip_plane_writer_destroy(writer_); }
   PlaneWriter(const PlaneWriter&) = delete;
   PlaneWriter& operator=(const PlaneWriter&) = delete;
   sfk_hip_plane_writer* get() const { return writer_; }
 
  private:
+  struct Writers {
+    std::mutex mutex;
+    std::map<int, sfk_hip_plane_writer*> writers;
+  };
+  static Writers& Pool() {
+    static Writers* pool = new Writers();  // never destroyed: outlives every save
+    return *pool;
+  }
+  std::lock_guard<std::mutex> lock_;
   sfk_hip_plane_writer* writer_ = nullptr;
 };
 
