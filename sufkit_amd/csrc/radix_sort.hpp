@@ -313,14 +313,30 @@ __device__ inline void FlushDigits(std::uint32_t (*local)[RadixPasses::kMax][Rad
   }
 }
 
+// The largest divisor of `items` that is at most `limit`.
+constexpr int FetchGroup(int items, int limit) {
+  int g = items < limit ? items : limit;
+  while (items % g != 0) --g;
+  return g;
+}
+
 // Input of a pass read from arrays.
 template <typename K, typename V>
 struct ArraySource {
+  static constexpr int kFetchGroup = 64;  // every item of a tile at once
   const K* __restrict__ keys;
   const V* __restrict__ vals;
-  __device__ void Load(std::uint64_t i, K& key, V& val) const {
-    key = keys[i];
-    val = vals[i];
+  // A source is read in two phases: Fetch issues an item's loads, Make turns
+  // what they returned into the item's key and value.  A tile fetches all
+  // of its items before making any, so their loads are in flight together.
+  struct Raw {
+    K key;
+    V val;
+  };
+  __device__ Raw Fetch(std::uint64_t i) const { return Raw{keys[i], vals[i]}; }
+  __device__ void Make(const Raw& r, std::uint64_t, K& key, V& val) const {
+    key = r.key;
+    val = r.val;
   }
 };
 
@@ -410,18 +426,31 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   }
   const std::uint64_t wave_base = base + static_cast<std::uint64_t>(wave) * kItems * kWave;
   // All of the tile's loads first, so that they are in flight together (a
-  // load next to its item's LDS ranking waits out one memory round trip per
-  // item).  Every lane loads (slot 0 when out of range) and then selects:
-  // the branchy form made hipcc spill the 1024 x 16 tile's registers.
+  // load next to its item's LDS ranking, or next to the key computed from
+  // it, waits out one memory round trip per item).  Every lane loads (slot 0
+  // when out of range) and then selects: the branchy form made hipcc spill
+  // the 1024 x 16 tile's registers.
+  // Items are fetched in groups of at most Source::kFetchGroup (a source
+  // whose raw form is wide would otherwise run out of registers).
+  constexpr int kGroup = FetchGroup(kItems, Source::kFetchGroup);
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
-    const bool valid = idx < end;
-    K kk;
-    V vv;
-    src.Load(valid ? idx : 0, kk, vv);
-    key[j] = valid ? kk : K(0);
-    val[j] = valid ? vv : V(0);
+  for (int g = 0; g < kItems; g += kGroup) {
+    typename Source::Raw raw[kGroup];
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(g + j) * kWave + lane;
+      raw[j] = src.Fetch(idx < end ? idx : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(g + j) * kWave + lane;
+      const bool valid = idx < end;
+      K kk;
+      V vv;
+      src.Make(raw[j], valid ? idx : 0, kk, vv);
+      key[g + j] = valid ? kk : K(0);
+      val[g + j] = valid ? vv : V(0);
+    }
   }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {

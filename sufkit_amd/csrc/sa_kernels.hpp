@@ -524,11 +524,19 @@ __global__ __launch_bounds__(RadixTile::kRadix) void TuneAdjKernel(const unsigne
 // value = i, generated from the packed text instead of read.
 template <typename Index>
 struct TextFillHiSource {
+  static constexpr int kFetchGroup = 8;  // two 12-byte entry loads per item
   PackedText t;
-  __device__ void Load(std::uint64_t i, std::uint32_t& key, Index& val) const {
+  struct Raw {
+    uint4 e0, e1;  // the packed entries holding the window
+  };
+  __device__ Raw Fetch(std::uint64_t i) const {
+    const std::uint64_t w = i >> 5;
+    return Raw{t.words[w], t.words[w + 1]};
+  }
+  __device__ void Make(const Raw& r, std::uint64_t i, std::uint32_t& key, Index& val) const {
     std::uint64_t c;
     std::uint32_t s;
-    LoadWindow(t, i, c, s);
+    WindowOf(r.e0, r.e1, static_cast<std::uint32_t>(i & 31), c, s);
     key = static_cast<std::uint32_t>(FillKey(c, s) >> 32);
     val = static_cast<Index>(i);
   }
