@@ -325,6 +325,24 @@ int sfk_hip_crc32(int device, const void* d_data, uint64_t bytes,
                   uint32_t crc_in, uint32_t* crc_out, char* error,
                   size_t error_capacity);
 
+/*
+ * Byte-coded LCP of the Low-memory profile from a complete raw LCP (SA
+ * order) and its ISA in device memory: BuildByteData / BuildByteDataDirect
+ * of lcp_storage.cpp:233-420.  primary (host, n bytes; may be NULL) gets
+ * min(LCP[r], 255).  *anchor_count gets the number of overflow anchors: text
+ * positions p with PLCP[p] >= 255 that do not continue the run of p - 1
+ * (PLCP[p - 1] >= 255 and PLCP[p - 1] - 1 == PLCP[p]).  When
+ * anchor_positions and anchor_values are given (host, capacity entries),
+ * they get the anchors' positions and PLCP values in increasing position;
+ * call once with NULL arrays to learn the count.
+ */
+int sfk_hip_byte_coded_lcp(int device, const void* d_lcp, uint32_t lcp_bits,
+                           const void* d_isa, uint32_t isa_bits, uint64_t n,
+                           uint8_t* primary, uint64_t* anchor_count,
+                           uint64_t* anchor_positions, uint64_t* anchor_values,
+                           uint64_t capacity, char* error,
+                           size_t error_capacity);
+
 /* Device memory helpers so callers without a device allocator of their own
  * can stage buffers (tests, benchmarks). */
 int sfk_hip_malloc(int device, uint64_t bytes, void** out);

@@ -304,20 +304,22 @@ def ref_writer_available() -> bool:
 
 
 def ref_write_sufidx(records: list[tuple[str, bytes]], path: str, coordinate_width: int = 0,
-                     storage_width: int = 0, acceleration: int = 3) -> None:
+                     storage_width: int = 0, acceleration: int = 3, resource_profile: int = 0) -> None:
     """Build with the reference's own SuffixArray::Build (divsufsort) and Save
     a .sufidx 1.4 file (test infrastructure: golden files only)."""
     W = ctypes.CDLL(str(HERE / "_ref" / "libsufkit_refwriter.so"))
-    W.sfk_ref_write_sufidx.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
-                                       U64P, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
-    W.sfk_ref_write_sufidx.restype = ctypes.c_int
+    W.sfk_ref_write_sufidx_profile.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                               U64P, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
+    W.sfk_ref_write_sufidx_profile.restype = ctypes.c_int
     names = (ctypes.c_char_p * len(records))(*[r[0].encode() for r in records])
     seqs = (ctypes.c_char_p * len(records))(*[r[1] for r in records])
     lens = np.array([len(r[1]) for r in records], dtype=np.uint64)
     err = ctypes.create_string_buffer(512)
-    rc = W.sfk_ref_write_sufidx(names, seqs, _u64(lens), len(records), str(path).encode(),
-                                coordinate_width, storage_width, acceleration, err, len(err))
+    rc = W.sfk_ref_write_sufidx_profile(names, seqs, _u64(lens), len(records), str(path).encode(),
+                                        coordinate_width, storage_width, acceleration, resource_profile, None,
+                                        err, len(err))
     if rc != 0:
         raise RuntimeError(err.value.decode())
 

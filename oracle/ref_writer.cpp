@@ -41,13 +41,14 @@ int RefGuard(F&& f) {
 
 extern "C" {
 
-// acceleration: SaAcceleration value; coordinate_width 0/32/64; storage 0/32/40/48/64.
-// seconds (optional): normalization, Build, Save wall time of the reference's
-// own calls.
-int sfk_ref_write_sufidx_timed(const char* const* names, const char* const* sequences,
-                               const std::uint64_t* lengths, std::uint64_t count, const char* path,
-                               int coordinate_width, int storage_width, int acceleration, double* seconds,
-                               char* error, std::size_t capacity) {
+// acceleration: SaAcceleration value; coordinate_width 0/32/64; storage
+// 0/32/40/48/64; profile: SaResourceProfile (0 Fast, 1 Low-memory).
+// seconds (optional): normalization, Build, Save wall time of the
+// reference's own calls.
+int sfk_ref_write_sufidx_profile(const char* const* names, const char* const* sequences,
+                                 const std::uint64_t* lengths, std::uint64_t count, const char* path,
+                                 int coordinate_width, int storage_width, int acceleration, int profile,
+                                 double* seconds, char* error, std::size_t capacity) {
   try {
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
@@ -62,6 +63,7 @@ int sfk_ref_write_sufidx_timed(const char* const* names, const char* const* sequ
     options.coordinate_width = static_cast<sufkit::CoordinateWidth>(coordinate_width);
     options.storage_width = static_cast<sufkit::CoordinateStorageWidth>(storage_width);
     options.acceleration = static_cast<sufkit::SaAcceleration>(acceleration);
+    options.resource_profile = static_cast<sufkit::SaResourceProfile>(profile);
     const auto t1 = Clock::now();
     const auto index = sufkit::SuffixArray::Build(reference, options);
     const auto t2 = Clock::now();
@@ -84,34 +86,20 @@ int sfk_ref_write_sufidx_timed(const char* const* names, const char* const* sequ
   }
 }
 
+int sfk_ref_write_sufidx_timed(const char* const* names, const char* const* sequences,
+                               const std::uint64_t* lengths, std::uint64_t count, const char* path,
+                               int coordinate_width, int storage_width, int acceleration, double* seconds,
+                               char* error, std::size_t capacity) {
+  return sfk_ref_write_sufidx_profile(names, sequences, lengths, count, path, coordinate_width, storage_width,
+                                      acceleration, 0, seconds, error, capacity);
+}
+
 int sfk_ref_write_sufidx(const char* const* names, const char* const* sequences,
                          const std::uint64_t* lengths, std::uint64_t count, const char* path,
                          int coordinate_width, int storage_width, int acceleration,
                          char* error, std::size_t capacity) {
-  try {
-    std::vector<sufkit::SequenceRecord> records(count);
-    for (std::uint64_t i = 0; i < count; ++i) {
-      records[i].name = names[i];
-      records[i].sequence.assign(sequences[i], lengths[i]);
-    }
-    const auto reference = sufkit::GenomeReference::FromRecords(std::move(records));
-    sufkit::SuffixArrayBuildOptions options;
-    options.backend = sufkit::SaBackend::kDivsufsort;
-    options.coordinate_width = static_cast<sufkit::CoordinateWidth>(coordinate_width);
-    options.storage_width = static_cast<sufkit::CoordinateStorageWidth>(storage_width);
-    options.acceleration = static_cast<sufkit::SaAcceleration>(acceleration);
-    const auto index = sufkit::SuffixArray::Build(reference, options);
-    sufkit::SaveOptions save;
-    save.overwrite = true;
-    index.Save(path, save);
-    return 0;
-  } catch (const std::exception& e) {
-    if (error != nullptr && capacity > 0) {
-      std::strncpy(error, e.what(), capacity - 1);
-      error[capacity - 1] = '\0';
-    }
-    return -1;
-  }
+  return sfk_ref_write_sufidx_profile(names, sequences, lengths, count, path, coordinate_width, storage_width,
+                                      acceleration, 0, nullptr, error, capacity);
 }
 
 // The reference's own FastPrefixIndex (fast_prefix_index.cpp:286-382) over

@@ -128,29 +128,55 @@ class CoordinateStorage {
 };
 
 // Raw LCP (the Fast profile's layout): one 32- or 64-bit value per row,
-// LCP[0] == 0 (lcp_storage.cpp:508-528).
+// LCP[0] == 0 (lcp_storage.cpp:508-528).  Byte-coded LCP (the Low-memory
+// profile's layout, lcp_storage.cpp:55-230): one byte per row, min(LCP,
+// 255); a 255 is decoded from the overflow anchors (text position, PLCP) by
+// the suffix position of its row, through a guide of the first anchor of
+// every 4096-position block.  Width() is the value width of a raw LCP and
+// the anchor coordinate width of a byte-coded one.
 class LcpStorage {
  public:
+  static constexpr std::uint32_t kByteLimit = 255;
+  static constexpr std::uint32_t kGuideBlockShift = 12;
+
   LcpStorage() = default;
   static LcpStorage FromRaw32(std::vector<std::uint32_t>&& values, std::uint32_t sampling_rate,
                               ErrorCode error_code);
   static LcpStorage FromRaw64(std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
                               ErrorCode error_code);
+  // Validates as ValidateByteStructure (lcp_storage.cpp:97-150) and builds
+  // the guide (BuildGuide, :55-95).
+  static LcpStorage FromByteCoded(std::vector<std::uint8_t>&& primary, std::vector<std::uint64_t>&& positions,
+                                  std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
+                                  std::uint64_t text_symbols, std::uint8_t coordinate_width, ErrorCode error_code);
   bool Empty() const noexcept { return width_ == 0; }
+  bool ByteCoded() const noexcept { return byte_coded_; }
   std::uint64_t Size() const noexcept;
   std::uint8_t Width() const noexcept { return width_; }
   std::uint32_t SamplingRate() const noexcept { return sampling_rate_; }
+  // Raw LCP only.
   std::uint64_t At(std::uint64_t row) const noexcept;
-  std::uint64_t Bytes() const noexcept { return Size() * (width_ / 8u); }
+  // Either layout: the value of `row`, whose suffix starts at
+  // suffix_position (DecodeByte, lcp_storage.cpp:186-215).
+  std::uint64_t Decode(std::uint64_t row, std::uint64_t suffix_position, ErrorCode error_code) const;
+  std::uint64_t Bytes() const noexcept;
   const std::vector<std::uint32_t>& Raw32() const noexcept { return raw32_; }
   const std::vector<std::uint64_t>& Raw64() const noexcept { return raw64_; }
-  // lcp_storage.cpp:693-770 for raw LCP: row count, LCP[0]==0, bounds.
+  const std::vector<std::uint8_t>& Primary() const noexcept { return primary_; }
+  const std::vector<std::uint64_t>& AnchorPositions() const noexcept { return anchor_positions_; }
+  const std::vector<std::uint64_t>& AnchorValues() const noexcept { return anchor_values_; }
+  // lcp_storage.cpp:693-770: row count, LCP[0]==0, bounds.
   void Validate(const CoordinateStorage& sa, std::uint64_t text_symbols,
                 ErrorCode error_code) const;
 
  private:
   std::vector<std::uint32_t> raw32_;
   std::vector<std::uint64_t> raw64_;
+  std::vector<std::uint8_t> primary_;
+  std::vector<std::uint64_t> anchor_positions_;
+  std::vector<std::uint64_t> anchor_values_;
+  std::vector<std::uint64_t> guide_;
+  bool byte_coded_ = false;
   std::uint8_t width_ = 0;
   std::uint32_t sampling_rate_ = 1;
 };
@@ -166,7 +192,8 @@ void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t 
 std::vector<std::uint8_t> CoordinateSectionHeader(CoordinateStorageWidth width, std::uint64_t symbol_count,
                                                   std::uint64_t count);
 std::vector<std::uint8_t> LcpSectionHeader(std::uint32_t width, std::uint32_t sampling_rate,
-                                           std::uint64_t text_symbols, std::uint64_t rows);
+                                           std::uint64_t text_symbols, std::uint64_t rows,
+                                           bool byte_coded = false, std::uint64_t anchors = 0);
 // Section 1 (serialization.cpp:785-799): the sequence records.
 void WriteMetadataSection(std::ostream& out, const std::vector<SequenceInfo>& sequences);
 LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size,

@@ -297,12 +297,99 @@ LcpStorage LcpStorage::FromRaw64(std::vector<std::uint64_t>&& values, std::uint3
   return s;
 }
 
+LcpStorage LcpStorage::FromByteCoded(std::vector<std::uint8_t>&& primary, std::vector<std::uint64_t>&& positions,
+                                     std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
+                                     std::uint64_t text_symbols, std::uint8_t coordinate_width,
+                                     ErrorCode error_code) {
+  if (sampling_rate == 0) Throw(error_code, "LCP storage: sampling rate must be positive");
+  if (coordinate_width != 32 && coordinate_width != 64) Throw(error_code, "LCP storage: invalid coordinate width");
+  if (positions.size() != values.size()) {
+    Throw(error_code, "LCP storage: overflow anchor planes have different lengths");
+  }
+  const std::uint64_t max_coordinate =
+      coordinate_width == 32 ? std::numeric_limits<std::uint32_t>::max() : std::numeric_limits<std::uint64_t>::max();
+  if (text_symbols != 0 && text_symbols - 1 > max_coordinate) {
+    Throw(error_code, "LCP storage: text does not fit the overflow coordinate width");
+  }
+  const std::uint64_t rows = text_symbols == 0 ? 0 : 1 + (text_symbols - 1) / sampling_rate;
+  if (primary.size() != rows) Throw(error_code, "LCP storage: LCP row count disagrees with sampling metadata");
+  if (positions.size() > (primary.empty() ? 0 : primary.size() - 1)) {
+    Throw(error_code, "LCP storage: too many overflow anchors for the LCP rows");
+  }
+  if (!primary.empty() && primary.front() != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
+  for (std::size_t i = 0; i < positions.size(); ++i) {
+    const std::uint64_t p = positions[i], v = values[i];
+    if (p >= text_symbols) Throw(error_code, "LCP storage: overflow anchor position is out of range");
+    if (p % sampling_rate != 0) Throw(error_code, "LCP storage: overflow anchor is not sampling-rate aligned");
+    if (v < kByteLimit) Throw(error_code, "LCP storage: overflow anchor value is smaller than 255");
+    if (v > max_coordinate) Throw(error_code, "LCP storage: overflow anchor value exceeds its width");
+    if (i == 0) continue;
+    if (p <= positions[i - 1]) Throw(error_code, "LCP storage: overflow anchor positions are not increasing");
+    const std::uint64_t delta = p - positions[i - 1];
+    if (delta == sampling_rate && values[i - 1] >= delta && values[i - 1] - delta == v) {
+      Throw(error_code, "LCP storage: overflow anchor redundantly continues a run");
+    }
+  }
+  LcpStorage s;
+  s.byte_coded_ = true;
+  s.width_ = coordinate_width;
+  s.sampling_rate_ = sampling_rate;
+  s.primary_ = std::move(primary);
+  s.anchor_positions_ = std::move(positions);
+  s.anchor_values_ = std::move(values);
+  if (text_symbols != 0) {
+    // guide[b] = first anchor at or after position b * 4096, b = 0..blocks.
+    const std::uint64_t blocks = 1 + ((text_symbols - 1) >> kGuideBlockShift);
+    s.guide_.resize(blocks + 1);
+    std::size_t a = 0;
+    for (std::uint64_t b = 0; b <= blocks; ++b) {
+      const std::uint64_t boundary = b << kGuideBlockShift;
+      while (a < s.anchor_positions_.size() && s.anchor_positions_[a] < boundary) ++a;
+      s.guide_[b] = a;
+    }
+  }
+  return s;
+}
+
 std::uint64_t LcpStorage::Size() const noexcept {
+  if (byte_coded_) return primary_.size();
   return width_ == 32 ? raw32_.size() : (width_ == 64 ? raw64_.size() : 0);
+}
+
+std::uint64_t LcpStorage::Bytes() const noexcept {
+  if (!byte_coded_) return Size() * (width_ / 8u);
+  return primary_.size() + (2 * anchor_positions_.size() + guide_.size()) * (width_ / 8u);
 }
 
 std::uint64_t LcpStorage::At(std::uint64_t row) const noexcept {
   return width_ == 32 ? raw32_[row] : raw64_[row];
+}
+
+std::uint64_t LcpStorage::Decode(std::uint64_t row, std::uint64_t suffix_position, ErrorCode error_code) const {
+  if (!byte_coded_) return At(row);
+  if (row >= primary_.size()) Throw(error_code, "LCP storage: row is out of range");
+  const std::uint8_t primary = primary_[row];
+  if (primary < kByteLimit) return primary;
+  if (suffix_position % sampling_rate_ != 0) Throw(error_code, "LCP storage: suffix position is not aligned");
+  if (anchor_positions_.empty()) Throw(error_code, "LCP storage: byte overflow has no anchor");
+  const std::uint64_t block = suffix_position >> kGuideBlockShift;
+  if (block + 1 >= guide_.size()) Throw(error_code, "LCP storage: suffix position exceeds the overflow guide");
+  const auto first = anchor_positions_.begin() + static_cast<std::ptrdiff_t>(guide_[block]);
+  const auto last = anchor_positions_.begin() + static_cast<std::ptrdiff_t>(guide_[block + 1]);
+  const auto found = std::upper_bound(first, last, suffix_position);
+  std::size_t anchor;
+  if (found != first) {
+    anchor = static_cast<std::size_t>(found - anchor_positions_.begin()) - 1;
+  } else {
+    if (guide_[block] == 0) Throw(error_code, "LCP storage: byte overflow precedes its first anchor");
+    anchor = static_cast<std::size_t>(guide_[block]) - 1;
+  }
+  const std::uint64_t at = anchor_positions_[anchor], value = anchor_values_[anchor];
+  const std::uint64_t delta = suffix_position - at;
+  if (at > suffix_position || delta % sampling_rate_ != 0 || value < delta || value - delta < kByteLimit) {
+    Throw(error_code, "LCP storage: overflow anchor does not cover the suffix position");
+  }
+  return value - delta;
 }
 
 void LcpStorage::Validate(const CoordinateStorage& sa, std::uint64_t text_symbols,
@@ -312,13 +399,13 @@ void LcpStorage::Validate(const CoordinateStorage& sa, std::uint64_t text_symbol
   if (text_symbols == 0 || Size() != 1 + (text_symbols - 1) / sampling_rate_) {
     Throw(error_code, "LCP storage: row count disagrees with sampling metadata");
   }
-  if (At(0) != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
+  if (Decode(0, sa.At(0), error_code) != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
   for (std::uint64_t row = 1; row < Size(); ++row) {
     const std::uint64_t suffix = sa.At(row), previous = sa.At(row - 1);
     if (suffix >= text_symbols || previous >= text_symbols) {
       Throw(error_code, "LCP storage: suffix-array position is invalid");
     }
-    if (At(row) > std::min(text_symbols - suffix, text_symbols - previous)) {
+    if (Decode(row, suffix, error_code) > std::min(text_symbols - suffix, text_symbols - previous)) {
       Throw(error_code, "LCP storage: value exceeds adjacent suffix bounds");
     }
   }

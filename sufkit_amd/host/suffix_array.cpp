@@ -81,7 +81,8 @@ struct SuffixArray::Impl {
     info.ambiguous_bases = ambiguous_bases;
     info.fingerprint = fingerprint;
     info.sa_acceleration = acceleration;
-    info.lcp_encoding = lcp.Empty() ? SaLcpEncoding::kNone : SaLcpEncoding::kRaw;
+    info.lcp_encoding =
+        lcp.Empty() ? SaLcpEncoding::kNone : (lcp.ByteCoded() ? SaLcpEncoding::kByteCoded : SaLcpEncoding::kRaw);
     info.text_bytes = text.size();
     info.sa_bytes = sa.Bytes();
     info.isa_bytes = isa.Bytes();
@@ -209,6 +210,79 @@ void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& option
   }
 }
 
+// Device buffer of the Low-memory build, freed on every path.
+struct DeviceBuffer {
+  void* p = nullptr;
+  explicit DeviceBuffer(std::uint64_t bytes) {
+    if (sfk_hip_malloc(0, bytes == 0 ? 1 : bytes, &p) != SFK_HIP_OK) {
+      p = nullptr;
+      detail::Throw(ErrorCode::kBuildFailure, "HIP backend: device allocation failed");
+    }
+  }
+  ~DeviceBuffer() {
+    if (p != nullptr) sfk_hip_free(p);
+  }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
+
+// Low-memory profile: the device builds SA, ISA and raw LCP, keeps the ISA
+// and raw LCP only long enough to byte-code the LCP
+// (sfk_hip_byte_coded_lcp), and returns the SA and the byte-coded LCP
+// (BuildByteCodedLcpDirect, suffix_array.cpp:1103-1113, with the anchor
+// coordinate width of the raw LCP's, :4152-4156).
+void BuildLowMemoryWithHip(SuffixArray::Impl& impl, CoordinateStorageWidth stored_width,
+                           SuffixArrayBuildStatistics& stats) {
+  const std::uint64_t n = impl.text.size();
+  const std::uint32_t bits = impl.construction_width;
+  const std::uint32_t sa_bits = stored_width == CoordinateStorageWidth::kBits32 ? 32 : 64;
+  const std::uint32_t wide = n - 1U > std::numeric_limits<std::uint32_t>::max() ? 64 : 32;
+  char err[512] = {0};
+  auto check = [&](int status, const char* what) {
+    if (status != SFK_HIP_OK) detail::Throw(FromStatus(status), std::string("HIP backend: ") + what + ": " + err);
+  };
+  DeviceBuffer text(n + 64), sa(n * (sa_bits / 8)), isa(n * (wide / 8)), lcp(n * (wide / 8));
+  check(sfk_hip_memcpy_h2d(text.p, impl.text.data(), n), "text upload");
+  sfk_hip_builder* builder = nullptr;
+  check(sfk_hip_builder_create(0, &builder, err, sizeof(err)), "builder");
+  sfk_hip_stats hs{};
+  const int built = sfk_hip_build_device(builder, static_cast<const std::uint8_t*>(text.p), n, bits,
+                                         SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, sa.p, sa_bits, isa.p, wide, lcp.p,
+                                         wide, &hs, err, sizeof(err));
+  sfk_hip_builder_destroy(builder);
+  check(built, "build");
+  stats.sa_seconds = hs.sa_seconds;
+  stats.isa_seconds = hs.isa_seconds;
+  stats.lcp_seconds = hs.lcp_seconds;
+  stats.device_seconds = hs.device_seconds;
+  std::vector<std::uint8_t> primary(n);
+  std::uint64_t anchors = 0;
+  const auto fin = Clock::now();
+  check(sfk_hip_byte_coded_lcp(0, lcp.p, wide, isa.p, wide, n, primary.data(), &anchors, nullptr, nullptr, 0, err,
+                               sizeof(err)),
+        "byte-coded LCP");
+  std::vector<std::uint64_t> positions(anchors), values(anchors);
+  if (anchors != 0) {
+    check(sfk_hip_byte_coded_lcp(0, lcp.p, wide, isa.p, wide, n, nullptr, &anchors, positions.data(),
+                                 values.data(), positions.size(), err, sizeof(err)),
+          "byte-coded LCP anchors");
+  }
+  impl.lcp = detail::LcpStorage::FromByteCoded(std::move(primary), std::move(positions), std::move(values), 1, n,
+                                               static_cast<std::uint8_t>(wide), ErrorCode::kBuildFailure);
+  stats.lcp_finalize_seconds = Since(fin);
+  const auto compaction = Clock::now();
+  if (sa_bits == 32) {
+    std::vector<std::uint32_t> values32(n);
+    check(sfk_hip_memcpy_d2h(values32.data(), sa.p, n * 4), "SA download");
+    impl.sa = Store(std::move(values32), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
+  } else {
+    std::vector<std::uint64_t> values64(n);
+    check(sfk_hip_memcpy_d2h(values64.data(), sa.p, n * 8), "SA download");
+    impl.sa = Store(std::move(values64), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
+  }
+  stats.storage_compaction_seconds = Since(compaction);
+}
+
 }  // namespace
 
 SuffixArray::SuffixArray(std::unique_ptr<Impl> impl) : impl_(std::move(impl)) {}
@@ -261,11 +335,14 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
     detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
   }
   const auto stored_width = detail::ResolveProfileStorageWidth(options.storage_width, options.resource_profile, n);
-  if (options.resource_profile == SaResourceProfile::kLowMemory) {
-    detail::Throw(ErrorCode::kUnsupportedBackend,
-                  "the Low-memory profile (byte-coded LCP) is not part of this build");
+  const bool low_memory = options.resource_profile == SaResourceProfile::kLowMemory;
+  if (low_memory && options.sampling_rate != 1) {
+    detail::Throw(ErrorCode::kInvalidInput, "low-memory profile requires a complete suffix array");
   }
-  if (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull) {
+  // The Low-memory profile is authoritative (suffix_array.cpp:4142-4151):
+  // byte-coded LCP, and no ISA or CHILD stays resident.
+  if (!low_memory &&
+      (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull)) {
     detail::Throw(ErrorCode::kUnsupportedBackend, "CHILD arrays are not part of this build");
   }
   const bool wide_lcp = n - 1U > std::numeric_limits<std::uint32_t>::max();
@@ -274,8 +351,17 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
   impl->construction_width = static_cast<std::uint8_t>(width);
   impl->backend = width == CoordinateWidth::kBits32 ? detail::StoredBackend::kHip32 : detail::StoredBackend::kHip64;
   impl->sampling_rate = options.sampling_rate;
-  impl->acceleration = options.acceleration;
+  impl->acceleration = low_memory ? SaAcceleration::kLcp : options.acceleration;
   stats.text_prepare_seconds = Since(total_begin);
+  if (low_memory) {
+    notify("index-sa-lcp");
+    BuildLowMemoryWithHip(*impl, stored_width, stats);
+    impl->RefreshInfo();
+    impl->info.sa_resource_profile = options.resource_profile;
+    stats.total_seconds = Since(total_begin);
+    notify("index-build-return");
+    return SuffixArray(std::move(impl));
+  }
 
   notify("index-sa-lcp");
   const bool narrow_sa = stored_width == CoordinateStorageWidth::kBits32;
@@ -633,7 +719,7 @@ std::uint64_t SuffixArray::IsaAt(std::uint64_t sample) const {
 }
 std::uint64_t SuffixArray::LcpAt(std::uint64_t row) const {
   if (row >= impl_->lcp.Size()) detail::Throw(ErrorCode::kInvalidInput, "LCP row out of range");
-  return impl_->lcp.At(row);
+  return impl_->lcp.Decode(row, impl_->lcp.ByteCoded() ? impl_->sa.At(row) : 0, ErrorCode::kInvalidInput);
 }
 const std::vector<std::uint8_t>& SuffixArray::Text() const noexcept { return impl_->text; }
 

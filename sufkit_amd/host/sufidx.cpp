@@ -250,21 +250,22 @@ std::vector<std::uint8_t> CoordinateSectionHeader(CoordinateStorageWidth width, 
 }
 
 std::vector<std::uint8_t> LcpSectionHeader(std::uint32_t width, std::uint32_t sampling_rate,
-                                           std::uint64_t text_symbols, std::uint64_t rows) {
+                                           std::uint64_t text_symbols, std::uint64_t rows, bool byte_coded,
+                                           std::uint64_t anchors) {
   std::vector<std::uint8_t> h;
   h.reserve(kLcpHeaderBytes);
   AppendU32(h, kLcpMagic);
   AppendU32(h, kCodecVersion);
-  AppendU32(h, 0);  // raw
+  AppendU32(h, byte_coded ? 1 : 0);  // raw or byte-coded
   AppendU32(h, width);
   AppendU32(h, sampling_rate);
   AppendU32(h, kLcpGuideBlockShift);
   AppendU64(h, text_symbols);
   AppendU64(h, rows);
-  AppendU64(h, rows * (width / 8u));
-  AppendU64(h, 0);
-  AppendU64(h, 0);
-  AppendU64(h, 0);
+  AppendU64(h, byte_coded ? rows : rows * (width / 8u));
+  AppendU64(h, byte_coded ? anchors : 0);
+  AppendU64(h, byte_coded ? anchors * (width / 8u) : 0);
+  AppendU64(h, byte_coded ? anchors * (width / 8u) : 0);
   AppendU64(h, 0);
   return h;
 }
@@ -359,9 +360,20 @@ void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t 
   if (text_symbols == 0 || lcp.Size() != 1 + (text_symbols - 1) / lcp.SamplingRate()) {
     Throw(ErrorCode::kBuildFailure, "SA section codec: LCP metadata is inconsistent before serialization");
   }
-  const auto header = LcpSectionHeader(lcp.Width(), lcp.SamplingRate(), text_symbols, lcp.Size());
+  const auto header = LcpSectionHeader(lcp.Width(), lcp.SamplingRate(), text_symbols, lcp.Size(), lcp.ByteCoded(),
+                                       lcp.AnchorPositions().size());
   WriteBytes(out, header.data(), header.size());
-  if (lcp.Width() == 32) {
+  if (lcp.ByteCoded()) {
+    WriteBytes(out, lcp.Primary().data(), lcp.Primary().size());
+    for (const auto* plane : {&lcp.AnchorPositions(), &lcp.AnchorValues()}) {
+      if (lcp.Width() == 32) {
+        std::vector<std::uint32_t> narrow(plane->begin(), plane->end());
+        WritePlane(out, narrow);
+      } else {
+        WritePlane(out, *plane);
+      }
+    }
+  } else if (lcp.Width() == 32) {
     WritePlane(out, lcp.Raw32());
   } else {
     WritePlane(out, lcp.Raw64());
@@ -376,9 +388,6 @@ LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size, std::
   }
   const std::uint32_t codec = ReadU32(in, "LCP codec identifier");
   if (codec > 1) Throw(ErrorCode::kVersionMismatch, "SA section codec: unknown LCP codec identifier");
-  if (codec == 1) {
-    Throw(ErrorCode::kUnsupportedBackend, "byte-coded (Low-memory) LCP sections are not supported by this build");
-  }
   const std::uint32_t width = ReadU32(in, "LCP coordinate width");
   const std::uint32_t rate = ReadU32(in, "LCP sampling rate");
   if (ReadU32(in, "LCP guide block shift") != kLcpGuideBlockShift) {
@@ -394,12 +403,36 @@ LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size, std::
   if (rate == 0 || (width != 32 && width != 64) || symbols == 0) {
     Throw(ErrorCode::kCorruptIndex, "SA section codec: invalid LCP header");
   }
+  if (rows != expected_rows || rate != sampling_rate || symbols != text_symbols) {
+    Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP metadata disagrees with the container");
+  }
+  if (codec == 1) {  // byte-coded: primary bytes, anchor positions, anchor values (sa_codec.cpp:353-370, 655-680)
+    if (width == 32 && symbols - 1 > std::numeric_limits<std::uint32_t>::max()) {
+      Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP logical text exceeds its coordinate width");
+    }
+    if (rows != 1 + (symbols - 1) / rate || primary != rows || anchors >= rows ||
+        anchor_pos != CheckedMul(anchors, width / 8u, ErrorCode::kCorruptIndex) || anchor_val != anchor_pos ||
+        kLcpHeaderBytes + primary + anchor_pos + anchor_val != section_size) {
+      Throw(ErrorCode::kCorruptIndex, "SA section codec: byte-coded LCP payload lengths are inconsistent");
+    }
+    std::vector<std::uint8_t> bytes(static_cast<std::size_t>(rows));
+    ReadExact(in, bytes.data(), bytes.size(), "byte-coded LCP primary");
+    std::vector<std::uint64_t> positions, values;
+    if (width == 32) {
+      auto p32 = ReadPlane<std::uint32_t>(in, anchors, "LCP anchor positions");
+      auto v32 = ReadPlane<std::uint32_t>(in, anchors, "LCP anchor values");
+      positions.assign(p32.begin(), p32.end());
+      values.assign(v32.begin(), v32.end());
+    } else {
+      positions = ReadPlane<std::uint64_t>(in, anchors, "LCP anchor positions");
+      values = ReadPlane<std::uint64_t>(in, anchors, "LCP anchor values");
+    }
+    return LcpStorage::FromByteCoded(std::move(bytes), std::move(positions), std::move(values), rate, symbols,
+                                     static_cast<std::uint8_t>(width), ErrorCode::kCorruptIndex);
+  }
   if (rows != 1 + (symbols - 1) / rate || primary != CheckedMul(rows, width / 8u, ErrorCode::kCorruptIndex) ||
       anchors != 0 || anchor_pos != 0 || anchor_val != 0 || kLcpHeaderBytes + primary != section_size) {
     Throw(ErrorCode::kCorruptIndex, "SA section codec: raw LCP payload lengths are inconsistent");
-  }
-  if (rows != expected_rows || rate != sampling_rate || symbols != text_symbols) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP metadata disagrees with the container");
   }
   if (width == 32) return LcpStorage::FromRaw32(ReadPlane<std::uint32_t>(in, rows, "raw32 LCP"), rate, ErrorCode::kCorruptIndex);
   return LcpStorage::FromRaw64(ReadPlane<std::uint64_t>(in, rows, "raw64 LCP"), rate, ErrorCode::kCorruptIndex);

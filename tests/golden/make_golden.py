@@ -117,9 +117,21 @@ def golden_references() -> dict[str, list[tuple[str, bytes]]]:
         body[10:90] = b"A" * 80                              # poly-A
         body.extend(b"acgtRYKM" + rand(300))                 # lowercase + IUPAC -> N
         mixed.append((f"chr{c + 1}", bytes(body)))
+    # Long exact repeats, a long poly-A and a long N run: LCP values of 255
+    # and more, so a byte-coded LCP needs overflow anchors (runs that continue
+    # and runs that restart).
+    rep_rng = np.random.default_rng(20261203)
+    def rep_rand(k):
+        return alphabet[rep_rng.integers(0, 4, size=k)].tobytes()
+    unit = rep_rand(1500)
+    repeats = [
+        ("rep1", rep_rand(400) + unit + rep_rand(200) + unit + b"A" * 700 + rep_rand(300) + unit[:900]),
+        ("rep2", rep_rand(250) + b"N" * 600 + unit[300:] + rep_rand(500) + b"AC" * 400),
+    ]
     return {
         "fixture": [("chr1", b"ACGTNACGT"), ("chr2", b"TTTACGTAAA"), ("chr3", b"AAA")],
         "mixed": mixed,
+        "repeats": repeats,
     }
 
 
@@ -127,6 +139,11 @@ def golden_references() -> dict[str, list[tuple[str, bytes]]]:
 SUFIDX_VARIANTS = [
     (0, 0, 3), (64, 0, 3), (0, 40, 3), (0, 48, 3), (0, 64, 3), (0, 0, 1), (0, 0, 0), (64, 48, 1),
 ]
+
+
+# Low-memory profile variants (byte-coded LCP, no ISA; the profile sets the
+# acceleration to LCP whatever is asked).
+LOWMEM_VARIANTS = [(0, 0, 3), (64, 0, 3), (0, 40, 3), (0, 64, 0)]
 
 
 def sufidx_golden() -> dict:
@@ -140,7 +157,12 @@ def sufidx_golden() -> dict:
             fname = f"{name}_c{cw or 'auto'}_s{sw or 'auto'}_a{accel}.sufidx"
             oracle.ref_write_sufidx(records, str(out_dir / fname), cw, sw, accel)
             index[fname] = {"reference": name, "coordinate_width": cw, "storage_width": sw,
-                            "acceleration": accel}
+                            "acceleration": accel, "resource_profile": 0}
+        for cw, sw, accel in LOWMEM_VARIANTS:
+            fname = f"{name}_c{cw or 'auto'}_s{sw or 'auto'}_a{accel}_lowmem.sufidx"
+            oracle.ref_write_sufidx(records, str(out_dir / fname), cw, sw, accel, resource_profile=1)
+            index[fname] = {"reference": name, "coordinate_width": cw, "storage_width": sw,
+                            "acceleration": accel, "resource_profile": 1}
     return index
 
 

@@ -40,7 +40,9 @@ def contigs(rng, total: int, count: int) -> list[bytes]:
 
 
 def golden_items():
-    return sorted(json.loads((GOLDEN / "sufidx_index.json").read_text()).items())
+    """The Fast-profile golden files (the device writer saves Fast indexes)."""
+    items = json.loads((GOLDEN / "sufidx_index.json").read_text()).items()
+    return sorted((k, v) for k, v in items if v.get("resource_profile", 0) == 0)
 
 
 # ----------------------------------------------------------------- CPU ---
@@ -214,3 +216,12 @@ def test_a_corrupted_section_is_detected_on_load(tmp_path, hip):
     with pytest.raises(SufkitError) as e:
         api.SuffixArray.load(out)
     assert e.value.status == 4  # kCorruptIndex
+
+
+@pytest.mark.gpu
+def test_device_save_of_a_low_memory_index_is_unsupported(tmp_path, hip):
+    ref = api.GenomeReference.from_records([("a", "ACGT" * 40)])
+    with pytest.raises(SufkitError) as e:
+        api.save_device_index(ref, tmp_path / "l.sufidx", api.BuildOptions(backend=api.BACKEND_HIP,
+                                                                           resource_profile=1))
+    assert e.value.status == 3

@@ -154,6 +154,12 @@ def test_load_reference_written_v14_files(fname, meta):
     assert info["format_version"] == "1.4"
     assert info["coordinate_width"] == (meta["coordinate_width"] or 32)
     assert info["stored_coordinate_width"] == (meta["storage_width"] or 32)
+    if meta.get("resource_profile", 0) == 1:
+        # Low-memory: no ISA, byte-coded LCP decoded through its anchors
+        assert sa.isa() is None
+        assert info["lcp_encoding"] == 2
+        assert np.array_equal(sa.lcp(), olcp)
+        return
     if meta["acceleration"] == 3:
         assert np.array_equal(sa.isa(), oisa)
     if meta["acceleration"] in (1, 3):

@@ -39,7 +39,8 @@ def test_saved_file_matches_reference_save(tmp_path, hip, fname, meta):
     records = golden_references()[meta["reference"]]
     ref = api.GenomeReference.from_records(records)
     opts = api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=meta["coordinate_width"],
-                            storage_width=meta["storage_width"], acceleration=meta["acceleration"])
+                            storage_width=meta["storage_width"], acceleration=meta["acceleration"],
+                            resource_profile=meta.get("resource_profile", 0))
     sa = api.SuffixArray.build(ref, opts)
     out = tmp_path / "ours.sufidx"
     sa.save(out)
@@ -129,10 +130,45 @@ def test_stage_statistics_reported(hip):
 
 def test_unsupported_profiles(hip):
     ref = api.GenomeReference.from_records([("a", b"ACGTACGT")])
-    for opts in (api.BuildOptions(resource_profile=1), api.BuildOptions(acceleration=api.ACCEL_FULL)):
-        with pytest.raises(SufkitError) as e:
-            api.SuffixArray.build(ref, opts)
-        assert e.value.code == "unsupported_backend"
+    with pytest.raises(SufkitError) as e:
+        api.SuffixArray.build(ref, api.BuildOptions(acceleration=api.ACCEL_FULL))
+    assert e.value.code == "unsupported_backend"
+    # Low-memory needs the complete SA (suffix_array.cpp:4143-4146)
+    with pytest.raises(SufkitError) as e:
+        api.SuffixArray.build(ref, api.BuildOptions(resource_profile=1, sampling_rate=2))
+    assert e.value.code == "invalid_input"
+
+
+@pytest.mark.parametrize("construction", [32, 64])
+def test_low_memory_byte_coded_lcp_decodes_to_the_raw_lcp(tmp_path, hip, construction):
+    """Low-memory build of a repeat-heavy reference (thousands of LCP values
+    of 255 and more): no ISA, byte-coded LCP whose decoded rows equal the
+    Fast build's raw LCP, through Save and Load too."""
+    rng = np.random.default_rng(construction)
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    unit = letters[rng.integers(0, 4, 3000)].tobytes()
+    parts = []
+    for i in range(40):
+        parts.append(letters[rng.integers(0, 4, int(rng.integers(100, 5000)))].tobytes())
+        parts.append(unit[int(rng.integers(0, 1000)):])
+        if i % 7 == 0:
+            parts.append(b"A" * int(rng.integers(300, 2000)) + b"N" * int(rng.integers(300, 900)))
+    ref = api.GenomeReference.from_records([("c1", b"".join(parts[:40])), ("c2", b"".join(parts[40:]))])
+    fast = api.SuffixArray.build(ref, api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=construction))
+    low = api.SuffixArray.build(ref, api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=construction,
+                                                      resource_profile=1))
+    raw = fast.lcp()
+    assert int((raw >= 255).sum()) > 1000
+    info = low.info()
+    assert info["lcp_encoding"] == 2 and low.isa() is None
+    assert np.array_equal(low.sa(), fast.sa())
+    assert np.array_equal(low.lcp(), raw)
+    path = tmp_path / "low.sufidx"
+    low.save(path)
+    back = api.SuffixArray.load(path)
+    assert np.array_equal(back.lcp(), raw)
+    assert back.info()["lcp_bytes"] == info["lcp_bytes"]
+    assert info["lcp_bytes"] < raw.size * 2
 
 
 def test_save_refuses_to_clobber(hip, tmp_path):
