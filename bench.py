@@ -77,6 +77,9 @@ def parse_args():
                     help="also write the built index to a .sufidx file straight from the device arrays "
                          "(SaveDeviceIndex) and time the reference's own Save of the same index")
     ap.add_argument("--save-dir", default=None, help="directory for the --save files (default: a temporary one)")
+    ap.add_argument("--low-memory", action="store_true",
+                    help="also time the Low-memory profile's byte-coded LCP (sfk_hip_byte_coded_lcp) of the built "
+                         "index and check it decodes to the raw LCP")
     ap.add_argument("--save-tuning", metavar="JSON", help="write the tuned radix shape table here")
     ap.add_argument("--load-tuning", metavar="JSON",
                     help="install a saved radix shape table instead of measuring one")
@@ -431,6 +434,52 @@ def save_leg(args, text, contigs, d_text, d_sa, d_isa, d_lcp, bits, out_bits) ->
     return out
 
 
+def low_memory_leg(hip, d_sa, d_isa, d_lcp, n, out_bits, sa, lcp) -> dict:
+    """Byte-code the built LCP as the Low-memory profile stores it (primary
+    byte per row, overflow anchors in text order), timed end to end into
+    host memory, then decode every row back through the anchors and compare
+    with the raw LCP when the full arrays are on the host."""
+    hip.sfk_hip_byte_coded_lcp.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                           ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_uint64, ctypes.c_char_p, ctypes.c_size_t]
+    hip.sfk_hip_byte_coded_lcp.restype = ctypes.c_int
+    err = ctypes.create_string_buffer(512)
+    primary = np.empty(n, np.uint8)
+    count = ctypes.c_uint64()
+    def run():
+        if hip.sfk_hip_byte_coded_lcp(0, d_lcp, out_bits, d_isa, out_bits, n, primary.ctypes.data,
+                                      ctypes.byref(count), None, None, 0, err, len(err)) != 0:
+            raise RuntimeError(err.value.decode())
+        pos = np.empty(count.value, np.uint64)
+        val = np.empty(count.value, np.uint64)
+        if count.value and hip.sfk_hip_byte_coded_lcp(0, d_lcp, out_bits, d_isa, out_bits, n, None,
+                                                      ctypes.byref(count), pos.ctypes.data, val.ctypes.data,
+                                                      pos.size, err, len(err)) != 0:
+            raise RuntimeError(err.value.decode())
+        return pos, val
+    run()  # warm-up
+    t0 = time.perf_counter()
+    pos, val = run()
+    seconds = time.perf_counter() - t0
+    out = {"seconds": round(seconds, 4), "anchors": int(count.value),
+           "rows_255_or_more": int((primary == 255).sum()),
+           "bytes": int(n + 2 * pos.size * (4 if out_bits == 32 else 8)),
+           "raw_lcp_bytes": int(n * out_bits // 8)}
+    if sa is not None and lcp is not None:
+        # decode: rows below 255 as they are; the others from the last anchor
+        # at or before their suffix position
+        rows = np.nonzero(primary == 255)[0]
+        p = sa[rows].astype(np.uint64)
+        k = np.searchsorted(pos, p, side="right") - 1
+        decoded = primary.astype(np.uint64)
+        ok = bool((k >= 0).all())
+        if ok and rows.size:
+            decoded[rows] = val[k] - (p - pos[k])
+        out["decodes_to_raw_lcp"] = ok and bool(np.array_equal(decoded, lcp.astype(np.uint64)))
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -609,13 +658,17 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
 
+    builder.close()  # its workspace is not needed by the legs below
     if args.save:
         result["save"] = save_leg(args, text, contigs, d_text, d_sa, d_isa, d_lcp, bits, out_bits)
-    builder.close()
+    if args.low_memory:
+        result["low_memory"] = low_memory_leg(hip, d_sa, d_isa, d_lcp, n, out_bits, sa, lcp)
     for ptr in (d_text, d_sa, d_isa, d_lcp):
         hip.sfk_hip_free(ptr)
     print(json.dumps(result), flush=True)
     if args.save and result["save"].get("parity_vs_reference_save", "sections identical") != "sections identical":
+        return 1
+    if args.low_memory and result["low_memory"].get("decodes_to_raw_lcp", True) is not True:
         return 1
     ok = (result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact"
           and result.get("parity_vs_cpu_baseline_caps", "bit-exact") == "bit-exact" and sample_bad == 0)
