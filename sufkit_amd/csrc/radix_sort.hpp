@@ -367,6 +367,33 @@ struct SegmentedPass {
   const long long* adj;          // [segments][kRadix] of this pass
 };
 
+// LDS of RadixOnesweep besides its staging buffer, and whether a tile stages
+// keys and values one after the other through one buffer (when both at once
+// would not fit the 160 KiB a gfx950 workgroup may hold).  A split stage
+// costs one more LDS round but lets 12- and 16-byte pairs use tiles of up to
+// 16384 elements, whose bucket runs span whole cache lines.
+constexpr std::uint64_t kLdsLimit = 160 * 1024;
+constexpr std::uint64_t OnesweepFixedLds(int block, int items, bool segmented) {
+  return static_cast<std::uint64_t>(block / kWave) * RadixTile::kRadix * 4 + RadixTile::kRadix * (4 + 4 + 8 + 8) +
+         static_cast<std::uint64_t>(block / kWave) * (4 + 8) +
+         (segmented ? static_cast<std::uint64_t>(block) * items * 2 : 2) + 8;
+}
+constexpr bool OnesweepSplitStage(int block, int items, int key_bytes, int val_bytes, bool segmented) {
+  return static_cast<std::uint64_t>(block) * items * (key_bytes + val_bytes) +
+             OnesweepFixedLds(block, items, segmented) > kLdsLimit;
+}
+// A shape fits when its stage fits the LDS and its per-thread keys and
+// values (and segment ids) fit the registers a wave of that block size gets
+// without spilling (1024-thread blocks: 128 VGPRs, of which at most 36 for
+// the items; the compiler's resource report shows no scratch for any shape
+// this admits).
+constexpr bool OnesweepFits(int block, int items, int key_bytes, int val_bytes, bool segmented) {
+  const int widest = key_bytes > val_bytes ? key_bytes : val_bytes;
+  const int item_budget = block >= 1024 ? 144 : 384;
+  return static_cast<std::uint64_t>(block) * items * widest + OnesweepFixedLds(block, items, segmented) <= kLdsLimit &&
+         items * (key_bytes + val_bytes + (segmented ? 4 : 0)) <= item_budget;
+}
+
 // digit_count: this pass's counts per digit (from CountDigits); unused when
 // kSegmented (seg gives the placement instead).
 template <typename K, typename V, int kBlock, int kItems, typename Source, bool kSegmented = false>
@@ -387,8 +414,12 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint64_t bucket_start[kRadix];
   __shared__ std::uint32_t scan_scratch[kBlock / kWave];
   __shared__ std::uint64_t scan64[kBlock / kWave];
-  __shared__ K s_keys[kTile];
-  __shared__ V s_vals[kTile];
+  constexpr bool kSplit = OnesweepSplitStage(kBlock, kItems, sizeof(K), sizeof(V), kSegmented);
+  constexpr int kStageBytes = kSplit ? kTile * static_cast<int>(sizeof(K) > sizeof(V) ? sizeof(K) : sizeof(V))
+                                     : kTile * static_cast<int>(sizeof(K) + sizeof(V));
+  __shared__ alignas(16) unsigned char s_stage[kStageBytes];
+  K* const s_keys = reinterpret_cast<K*>(s_stage);
+  V* const s_vals = reinterpret_cast<V*>(kSplit ? s_stage : s_stage + kTile * sizeof(K));
   __shared__ std::uint16_t s_seg[kSegmented ? kTile : 1];  // segment - first segment of the tile
   __shared__ std::uint32_t s_seg_range[2];
 
@@ -509,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       const std::uint32_t d = digit[j];
       const std::uint32_t pos = local_start[d] + wave_cnt[wave][d] + rank[j];
       s_keys[pos] = key[j];
-      s_vals[pos] = val[j];
+      if constexpr (!kSplit) s_vals[pos] = val[j];
       if constexpr (kSegmented) {
         // Segment of the input slot (the tile usually lies in one segment).
         const std::uint32_t sg = seg_first == seg_last ? seg_first : SegmentOfSlot(seg.offsets, seg_first, seg_last, idx);
@@ -519,17 +550,57 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   }
   __syncthreads();
   const std::uint32_t tile_n = static_cast<std::uint32_t>(end > base ? end - base : 0);
-  for (std::uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
-    const K k = s_keys[p];
-    const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
+  auto place = [&](std::uint32_t p, std::uint32_t d) {
     std::uint64_t out = global_base[d] + (p - local_start[d]);
     if constexpr (kSegmented) {
       const std::uint64_t sg = seg_first + s_seg[p];
       out += static_cast<std::uint64_t>(seg.adj[sg * kRadix + d]);
     }
-    if (out < m) {
-      keys_out[out] = k;
-      vals_out[out] = s_vals[p];
+    return out;
+  };
+  if constexpr (!kSplit) {
+    for (std::uint32_t p = threadIdx.x; p < tile_n; p += kBlock) {
+      const K k = s_keys[p];
+      const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
+      const std::uint64_t out = place(p, d);
+      if (out < m) {
+        keys_out[out] = k;
+        vals_out[out] = s_vals[p];
+      }
+    }
+  } else {
+    // Keys out first, remembering each staged slot's digit; then the values
+    // through the same buffer, to the same places.
+    std::uint32_t slot_digit[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const std::uint32_t p = threadIdx.x + static_cast<std::uint32_t>(i) * kBlock;
+      slot_digit[i] = 0;
+      if (p < tile_n) {
+        const K k = s_keys[p];
+        const std::uint32_t d = static_cast<std::uint32_t>(k >> shift) & (kRadix - 1);
+        slot_digit[i] = d;
+        const std::uint64_t out = place(p, d);
+        if (out < m) keys_out[out] = k;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
+      if (idx < end) {
+        const std::uint32_t d = digit[j];
+        s_vals[local_start[d] + wave_cnt[wave][d] + rank[j]] = val[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const std::uint32_t p = threadIdx.x + static_cast<std::uint32_t>(i) * kBlock;
+      if (p < tile_n) {
+        const std::uint64_t out = place(p, slot_digit[i]);
+        if (out < m) vals_out[out] = s_vals[p];
+      }
     }
   }
 }

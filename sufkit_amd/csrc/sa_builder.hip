@@ -227,25 +227,17 @@ constexpr int SortKind() {
 }
 
 // Static LDS of RadixOnesweep (its __shared__ arrays) against the 160 KiB a
-// gfx950 workgroup may hold.
+// gfx950 workgroup may hold (OnesweepFits).
 template <typename K, typename V, int kBlock, int kItems, bool kSeg>
 constexpr bool ShapeFits() {
-  constexpr std::uint64_t tile = static_cast<std::uint64_t>(kBlock) * kItems;
-  constexpr std::uint64_t lds = tile * (sizeof(K) + sizeof(V) + (kSeg ? 2 : 0)) +
-                                static_cast<std::uint64_t>(kBlock / kWave) * RadixTile::kRadix * 4 +
-                                RadixTile::kRadix * (4 + 8 + 8) + (kBlock / kWave) * 12 + 64;
-  return lds <= 160 * 1024;
+  return OnesweepFits(kBlock, kItems, sizeof(K), sizeof(V), kSeg);
 }
 
 constexpr bool KindShapeFits(int kind, int shape) {
   const ShapeDesc d = kShapes[shape];
-  const std::uint64_t key = kind == 1 || kind == 3 ? 8 : 4;
-  const std::uint64_t val = kind == 2 || kind == 3 || kind == 5 ? 8 : 4;
-  const std::uint64_t seg = kind >= 4 ? 2 : 0;
-  const std::uint64_t lds = static_cast<std::uint64_t>(d.block) * d.items * (key + val + seg) +
-                            static_cast<std::uint64_t>(d.block / kWave) * RadixTile::kRadix * 4 +
-                            RadixTile::kRadix * (4 + 8 + 8) + (d.block / kWave) * 12 + 64;
-  return lds <= 160 * 1024;
+  const int key = kind == 1 || kind == 3 ? 8 : 4;
+  const int val = kind == 2 || kind == 3 || kind == 5 ? 8 : 4;
+  return OnesweepFits(d.block, d.items, key, val, kind >= 4);
 }
 
 // Defaults (measured on MI355X): large tiles write longer runs per bucket and

@@ -524,7 +524,7 @@ __global__ __launch_bounds__(RadixTile::kRadix) void TuneAdjKernel(const unsigne
 // value = i, generated from the packed text instead of read.
 template <typename Index>
 struct TextFillHiSource {
-  static constexpr int kFetchGroup = 8;  // two 12-byte entry loads per item
+  static constexpr int kFetchGroup = 4;  // two 12-byte entry loads per item
   PackedText t;
   struct Raw {
     uint4 e0, e1;  // the packed entries holding the window
