@@ -202,8 +202,9 @@ struct ShapeDesc {
   int block;
   int items;
 };
-constexpr ShapeDesc kShapes[] = {{256, 16}, {512, 8}, {512, 16}, {1024, 8}, {1024, 12}, {1024, 16}};
+constexpr ShapeDesc kShapes[] = {{256, 16}, {512, 8}, {512, 16}, {1024, 8}, {1024, 12}, {1024, 16}, {512, 24}};
 constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
+static_assert(kShapeCount == 7, "SortPairs dispatches shapes 0..6");
 constexpr std::uint64_t kMinOnesweepTile = 256 * 16;
 
 // Kinds: 0 u32/u32, 1 u64/u32, 2 u32/u64, 3 u64/u64 (key/value), 4 and 5
@@ -388,6 +389,7 @@ void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_
     case 3: OnesweepPassesShape<K, V, 3, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
     case 4: OnesweepPassesShape<K, V, 4, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
     case 5: OnesweepPassesShape<K, V, 5, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
+    case 6: OnesweepPassesShape<K, V, 6, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
     default: Fail(SFK_HIP_BUILD_FAILURE, "unknown radix shape");
   }
 }

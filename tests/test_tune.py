@@ -9,7 +9,7 @@ from tests import synth
 
 KINDS = range(6)
 CLASSES = range(7)
-SHAPES = [(256, 16), (512, 8), (512, 16), (1024, 8), (1024, 12), (1024, 16)]
+SHAPES = [(256, 16), (512, 8), (512, 16), (1024, 8), (1024, 12), (1024, 16), (512, 24)]
 
 
 def test_defaults_cover_every_kind_and_class():
