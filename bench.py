@@ -352,6 +352,8 @@ def run_served(args) -> int:
         "max_concurrent_builds": summary["max_concurrent"],
         "peak_reserved_GB": round(summary["peak_reserved_bytes"] / 1e9, 2),
         "requests_failed": summary["requests_failed"], "sampled_count_mismatches": bad,
+        "first_error": (summary["first_error"].decode(errors="replace")
+                        if isinstance(summary["first_error"], bytes) else summary["first_error"]) or None,
         "wall_seconds": round(wall, 3), "latency_max_s": round(max(lat), 4),
         "radix_tuning": tuning,
     }

@@ -355,7 +355,7 @@ class ServeSummary(ctypes.Structure):
                 ("end_to_end_p99", ctypes.c_double), ("build_mbps", ctypes.c_double),
                 ("queries_per_second", ctypes.c_double), ("peak_reserved_bytes", ctypes.c_uint64),
                 ("requests_failed", ctypes.c_uint64), ("max_concurrent", ctypes.c_uint64),
-                ("evicted", ctypes.c_uint64)]
+                ("evicted", ctypes.c_uint64), ("first_error", ctypes.c_char * 256)]
 
 
 def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,

@@ -42,10 +42,27 @@ constexpr std::uint64_t kCountMask = (1ull << 48) - 1;
 constexpr std::uint32_t kEpochMask = (1u << 14) - 1;
 }  // namespace onesweep
 
-// Decoupled look-back of one value over tiles in block order (tile =
-// blockIdx.x; workgroups are dispatched in increasing order, so every tile a
-// tile waits on is resident or finished): the status word of tile t is
-// status[t * stride].  Publishes this tile's value,
+// Tile number of this workgroup in a look-back pass of epoch `epoch`, taken
+// from a ticket counter when the workgroup starts: every tile with a lower
+// number has started, so every tile a tile waits on is resident or
+// finished.  Numbering by block index would not give that: the 8 XCDs take
+// workgroups round-robin and each dispatches its share only as it has room,
+// so while other kernels (another build's stream) fill an XCD, a tile could
+// wait on a predecessor that its XCD has not dispatched.  The pass of epoch
+// e counts on tickets[e & 1]; its tile 0 clears the other counter, which the
+// next pass uses.  Call from every thread; a barrier follows.
+__device__ inline std::uint64_t LookBackTile(unsigned int* tickets, std::uint32_t epoch, std::uint32_t* s_tile) {
+  if (threadIdx.x == 0) {
+    const unsigned int t = atomicAdd(&tickets[epoch & 1u], 1u);
+    if (t == 0) tickets[(epoch + 1u) & 1u] = 0u;
+    *s_tile = t;
+  }
+  __syncthreads();
+  return *s_tile;
+}
+
+// Decoupled look-back of one value over tiles in ticket order
+// (LookBackTile): the status word of tile t is status[t * stride].  Publishes this tile's value,
 // walks back to the nearest tile whose inclusive value is published, then
 // publishes its own inclusive value; returns the exclusive prefix.  Tile 0's
 // prefix is `first`.  kMax: values combine by max, else by sum.  The wait is
@@ -401,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
     const unsigned long long* __restrict__ digit_count, unsigned long long* status,
     std::uint32_t epoch, unsigned int* error_flags,
-    std::uint32_t error_bit, SegmentedPass seg = SegmentedPass{nullptr, 0, nullptr}) {
+    std::uint32_t error_bit, unsigned int* tickets, SegmentedPass seg = SegmentedPass{nullptr, 0, nullptr}) {
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
   constexpr int kTile = kBlock * kItems;
@@ -423,9 +440,10 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint16_t s_seg[kSegmented ? kTile : 1];  // segment - first segment of the tile
   __shared__ std::uint32_t s_seg_range[2];
 
+  __shared__ std::uint32_t s_tile;
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  const std::uint64_t tile = blockIdx.x;
+  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile);
   const std::uint64_t base = tile * kTile;
   const std::uint64_t end = base + kTile < m ? base + kTile : m;
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;

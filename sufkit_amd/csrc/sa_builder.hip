@@ -119,6 +119,9 @@ struct sfk_hip_builder {
   sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
       vals0, vals1, seg_delta0, seg_delta1, seg_start, radix_counts, radix_status,
       partial, counters, narrow, ref_offsets, ref_counts, ref_adj, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
+  // Tile tickets of the look-back kernels: two counters, the pass of epoch e
+  // numbering its tiles from tickets[e & 1] (see LookBackTile).
+  sfk::DeviceBuffer tickets;
   std::uint32_t sort_epoch = 0;  // epoch of the last onesweep pass
   int radix_set = 0;             // radix count set of the next sort
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
@@ -278,8 +281,11 @@ std::uint64_t OnesweepStatusBytes(std::uint64_t n) {
 // Reserves the look-back status array; a new array is cleared (its words
 // then read as "not published" in every epoch).
 void ReserveOnesweep(sfk_hip_builder* b, std::uint64_t n) {
-  if (b->radix_status.Reserve(OnesweepStatusBytes(n))) {
+  const bool new_status = b->radix_status.Reserve(OnesweepStatusBytes(n));
+  const bool new_tickets = b->tickets.Reserve(2 * sizeof(unsigned int));
+  if (new_status || new_tickets) {
     SFK_HIP_CHECK(hipMemsetAsync(b->radix_status.As<void>(), 0, b->radix_status.Bytes(), b->stream));
+    SFK_HIP_CHECK(hipMemsetAsync(b->tickets.As<void>(), 0, b->tickets.Bytes(), b->stream));
     b->sort_epoch = 0;
   }
 }
@@ -289,7 +295,9 @@ void ReserveOnesweep(sfk_hip_builder* b, std::uint64_t n) {
 std::uint32_t NextEpoch(sfk_hip_builder* b) {
   b->sort_epoch += 1;
   if (b->sort_epoch > onesweep::kEpochMask) {
+    // Epochs restart at 1, so the ticket parity restarts too: clear both.
     SFK_HIP_CHECK(hipMemsetAsync(b->radix_status.As<void>(), 0, b->radix_status.Bytes(), b->stream));
+    SFK_HIP_CHECK(hipMemsetAsync(b->tickets.As<void>(), 0, b->tickets.Bytes(), b->stream));
     b->sort_epoch = 1;
   }
   return b->sort_epoch;
@@ -328,6 +336,7 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
     Fail(SFK_HIP_BUILD_FAILURE, "radix status array smaller than the sort");
   }
   unsigned long long* status = b->radix_status.As<unsigned long long>();
+  if (b->tickets.Bytes() < 2 * sizeof(unsigned int)) Fail(SFK_HIP_BUILD_FAILURE, "look-back tickets not reserved");
   unsigned int* error_flags = &b->counters.As<DeviceCounters>()->error_flags;
   for (int p = 0; p < passes.count; ++p) {
     const std::uint32_t epoch = NextEpoch(b);
@@ -342,12 +351,13 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
     if (p == 0) {
       RadixOnesweep<K, V, kBlock, kItems, Source, kSeg><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
           first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status, epoch,
-          error_flags, kFlagLookback, seg);
+          error_flags, kFlagLookback, b->tickets.As<unsigned int>(), seg);
     } else {
       RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
-              set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback, seg);
+              set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback,
+              b->tickets.As<unsigned int>(), seg);
     }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
@@ -886,11 +896,12 @@ class Doubling {
       if ((tiles + 1) * 3 * 8 > b_->radix_status.Bytes()) {
         Fail(SFK_HIP_BUILD_FAILURE, "look-back status array smaller than the list");
       }
+      if (b_->tickets.Bytes() < 2 * sizeof(unsigned int)) Fail(SFK_HIP_BUILD_FAILURE, "look-back tickets not reserved");
       RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
           b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
-          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), dc, segmented ? ctx_.d_offsets : nullptr,
-          segmented ? ctx_.refs : 0u);
+          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), dc,
+          segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u);
     } else {
       const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
       RefinePartial* partial = b_->partial.As<RefinePartial>();

@@ -68,6 +68,7 @@ struct sfk_serve_summary {
   uint64_t requests_failed;
   uint64_t max_concurrent;
   uint64_t evicted;
+  char first_error[256];  // message of the first request that failed ("" if none)
 };
 
 }  // extern "C"
@@ -119,6 +120,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
   const auto t0 = Clock::now();
   const auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   std::mutex mu;
+  std::string first_error;  // under mu
   std::condition_variable cv;
   std::deque<uint64_t> waiting;
   std::deque<uint64_t> resident_order;  // finished indexes, oldest first
@@ -148,7 +150,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
 
   auto worker = [&](uint32_t slot) {
     sfk_hip_builder* builder = nullptr;
-    char err[256];
+    char err[256] = {0};
     if (sfk_hip_builder_create(0, &builder, err, sizeof(err)) != SFK_HIP_OK) builder = nullptr;
     while (true) {
       uint64_t i = 0;
@@ -291,6 +293,9 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
       res.status = status;
       {
         std::lock_guard<std::mutex> lock(mu);
+        if (status != SFK_HIP_OK && first_error.empty()) {
+          first_error = "request " + std::to_string(i) + " (status " + std::to_string(status) + "): " + err;
+        }
         --concurrent;
         ++finished;
         // Query buffers are gone; a resident index keeps the rest.
@@ -342,6 +347,8 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         qt > 0 ? static_cast<double>(config->queries_per_index * (count - failed)) / qt : 0;
     summary->peak_reserved_bytes = peak;
     summary->requests_failed = failed;
+    std::memset(summary->first_error, 0, sizeof(summary->first_error));
+    std::strncpy(summary->first_error, first_error.c_str(), sizeof(summary->first_error) - 1);
     summary->max_concurrent = max_concurrent;
     summary->evicted = evicted;
   }
