@@ -135,6 +135,19 @@ namespace {
 
 constexpr int kScanItems = 8;
 
+// Builds (and tuning runs) in progress in this process.  While more than
+// one is, look-back passes number their tiles by ticket (LookBackTile).
+std::atomic<int> g_active_builds{0};
+
+struct ActiveBuild {
+  ActiveBuild() { g_active_builds.fetch_add(1); }
+  ~ActiveBuild() { g_active_builds.fetch_sub(1); }
+  ActiveBuild(const ActiveBuild&) = delete;
+  ActiveBuild& operator=(const ActiveBuild&) = delete;
+};
+
+bool TilesByTicket() { return g_active_builds.load() > 1; }
+
 struct BuildContext {
   sfk_hip_builder* b;
   std::uint64_t n;
@@ -351,13 +364,13 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
     if (p == 0) {
       RadixOnesweep<K, V, kBlock, kItems, Source, kSeg><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
           first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status, epoch,
-          error_flags, kFlagLookback, b->tickets.As<unsigned int>(), seg);
+          error_flags, kFlagLookback, b->tickets.As<unsigned int>(), TilesByTicket(), seg);
     } else {
       RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
               set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback,
-              b->tickets.As<unsigned int>(), seg);
+              b->tickets.As<unsigned int>(), TilesByTicket(), seg);
     }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
@@ -900,7 +913,7 @@ class Doubling {
       RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
           b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
-          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), dc,
+          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), TilesByTicket(), dc,
           segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u);
     } else {
       const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
@@ -1202,6 +1215,7 @@ int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t
                    std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits, void* d_lcp,
                    std::uint32_t lcp_bits, sfk_hip_stats* stats, std::string& message,
                    const std::uint64_t* offsets = nullptr, std::uint32_t refs = 0) {
+  const ActiveBuild active;
   BuildContext ctx{b, n, (flags & SFK_HIP_PROFILE_KERNELS) != 0, sfk_hip_stats{}};
   b->events_used = 0;
   SFK_HIP_CHECK(hipSetDevice(b->device));
@@ -1936,6 +1950,7 @@ int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entr
     b->radix_set = 0;
     sfk::ReserveOnesweep(b, top);
     sfk::ResetCounters(b);
+    const sfk::ActiveBuild active;  // tuning passes overlap builds on other streams like a build
     SFK_HIP_CHECK(hipEventCreate(&e0));
     SFK_HIP_CHECK(hipEventCreate(&e1));
     std::vector<sfk::TuneResult> results;

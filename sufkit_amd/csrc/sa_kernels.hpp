@@ -770,15 +770,15 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     const Index* __restrict__ seg_delta, std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank,
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
     Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
-    unsigned int* tickets, DeviceCounters* counters, const std::uint64_t* __restrict__ ref_offsets,
-    std::uint32_t refs) {
+    unsigned int* tickets, bool by_ticket, DeviceCounters* counters,
+    const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
   constexpr int kBlock = kRefineThreads;
   constexpr int kTile = kRefineTile;
   __shared__ RefineTileLds<K, kTile> lds;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   __shared__ std::uint64_t s_prefix[3];
   __shared__ std::uint32_t s_tile;
-  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile);
+  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile, by_ticket);
   const std::uint64_t base = tile * kTile;
   const int len = static_cast<int>(m - base < static_cast<std::uint64_t>(kTile) ? m - base : kTile);
   Index v[kRefineItems];
