@@ -290,7 +290,9 @@ int sfk_hip_tune_reset(int device);
  *   LOW32     the low 32 bits (first plane of split40 / split48);
  *   HIGH8     bits 32..39 (second plane of split40); values < 2^40;
  *   HIGH16    bits 32..47 (second plane of split48); values < 2^48;
- *   NATIVE64  64-bit values (native64 codec, raw64 LCP).
+ *   NATIVE64  64-bit values (native64 codec, raw64 LCP);
+ *   CAPPED8   min(value, 255) in one byte (the primary plane of a
+ *             byte-coded LCP, lcp_storage.cpp:233-300).
  * Device values are 32- or 64-bit (value_bits) for every kind but BYTES.
  * The plane's bytes are handed to `sink` in order, in chunks of at most the
  * writer's chunk size (pinned host memory, valid during the call); a
@@ -306,6 +308,7 @@ int sfk_hip_tune_reset(int device);
 #define SFK_HIP_PLANE_HIGH8 3u
 #define SFK_HIP_PLANE_HIGH16 4u
 #define SFK_HIP_PLANE_NATIVE64 5u
+#define SFK_HIP_PLANE_CAPPED8 6u
 typedef struct sfk_hip_plane_writer sfk_hip_plane_writer;
 typedef int (*sfk_hip_sink)(void* context, const void* data, uint64_t bytes);
 /* chunk_bytes 0 selects 64 MiB; two pinned host and two device buffers of

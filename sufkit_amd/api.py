@@ -406,8 +406,9 @@ def save_device_index(reference: GenomeReference, path: str | Path, options: Bui
     if text.size == 0 or text[-1] != 0:
         text = np.append(text, np.uint8(0))
     n = int(text.size)
-    keep_isa = o.acceleration == ACCEL_LCP_SUFFIX_LINK
-    keep_lcp = o.acceleration != ACCEL_NONE
+    low_memory = o.resource_profile == 1  # byte-coded LCP made from the device ISA and raw LCP
+    keep_isa = low_memory or o.acceleration == ACCEL_LCP_SUFFIX_LINK
+    keep_lcp = low_memory or o.acceleration != ACCEL_NONE
     d_text = DeviceArray(n + 64)
     d_text.upload(text)
     d_sa = DeviceArray(n * sa_bits // 8)

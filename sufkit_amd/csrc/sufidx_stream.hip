@@ -7,7 +7,8 @@
 // the file is reread for validation.  Here a section payload that lives in
 // HBM (text, SA, ISA, LCP) goes out in chunks: a kernel lays the chunk out
 // in its stored plane form (native32, the low 32 bits, the high 8 or 16
-// bits of split40/split48, or native64) when that differs from the device
+// bits of split40/split48, native64, or a byte-coded LCP's capped bytes)
+// when that differs from the device
 // form, the CRC kernels checksum the chunk where it lies, and the chunk is
 // copied to one of two pinned host buffers while the host hands the other
 // one to the sink (a file write).  The checksum of a plane continues from
@@ -57,6 +58,7 @@ std::uint32_t PlaneElementBytes(std::uint32_t plane) {
     case SFK_HIP_PLANE_HIGH8: return 1;
     case SFK_HIP_PLANE_HIGH16: return 2;
     case SFK_HIP_PLANE_NATIVE64: return 8;
+    case SFK_HIP_PLANE_CAPPED8: return 1;
     default: return 0;
   }
 }
@@ -80,6 +82,8 @@ __global__ __launch_bounds__(256) void PlaneKernel(const SrcT* __restrict__ src,
   } else if constexpr (kPlane == SFK_HIP_PLANE_HIGH16) {
     if ((v >> 48) != 0) atomicOr(error, kPlaneRange);
     static_cast<std::uint16_t*>(dst)[i] = static_cast<std::uint16_t>(v >> 32);
+  } else if constexpr (kPlane == SFK_HIP_PLANE_CAPPED8) {
+    static_cast<std::uint8_t*>(dst)[i] = static_cast<std::uint8_t>(v < 255 ? v : 255);
   } else {
     static_cast<std::uint64_t*>(dst)[i] = v;
   }
@@ -94,6 +98,7 @@ void LaunchPlane(std::uint32_t plane, const SrcT* src, std::uint64_t count, void
     case SFK_HIP_PLANE_LOW32: PlaneKernel<SrcT, SFK_HIP_PLANE_LOW32><<<grid, 256, 0, stream>>>(src, count, dst, error); break;
     case SFK_HIP_PLANE_HIGH8: PlaneKernel<SrcT, SFK_HIP_PLANE_HIGH8><<<grid, 256, 0, stream>>>(src, count, dst, error); break;
     case SFK_HIP_PLANE_HIGH16: PlaneKernel<SrcT, SFK_HIP_PLANE_HIGH16><<<grid, 256, 0, stream>>>(src, count, dst, error); break;
+    case SFK_HIP_PLANE_CAPPED8: PlaneKernel<SrcT, SFK_HIP_PLANE_CAPPED8><<<grid, 256, 0, stream>>>(src, count, dst, error); break;
     default: PlaneKernel<SrcT, SFK_HIP_PLANE_NATIVE64><<<grid, 256, 0, stream>>>(src, count, dst, error); break;
   }
   Check(hipGetLastError(), "plane kernel launch");

@@ -6,7 +6,10 @@
 // (ResolveProfileStorageWidth, ResolveAuxiliaryStorageWidth, raw LCP of 32
 // bits unless n - 1 needs 64), the codec headers are built on the host, and
 // every payload plane is streamed from device memory by
-// sfk_hip_write_plane, which checksums it on the device.
+// sfk_hip_write_plane, which checksums it on the device.  A Low-memory
+// index (byte-coded LCP, no ISA) streams its LCP's capped bytes the same
+// way; its few overflow anchors are found on the device and written from
+// the host.
 #include <chrono>
 #include <limits>
 #include <sstream>
@@ -81,11 +84,11 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
   if (options.sampling_rate != 1) {
     detail::Throw(ErrorCode::kInvalidInput, "saving from device memory needs sampling rate 1");
   }
-  if (options.resource_profile == SaResourceProfile::kLowMemory) {
-    detail::Throw(ErrorCode::kUnsupportedBackend,
-                  "the Low-memory profile (byte-coded LCP) is not part of this build");
-  }
-  if (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull) {
+  // The Low-memory profile stores a byte-coded LCP and no ISA
+  // (suffix_array.cpp:4142-4151); its anchors are found from the device ISA.
+  const bool low_memory = options.resource_profile == SaResourceProfile::kLowMemory;
+  if (!low_memory &&
+      (options.acceleration == SaAcceleration::kLcpChild || options.acceleration == SaAcceleration::kFull)) {
     detail::Throw(ErrorCode::kUnsupportedBackend, "CHILD arrays are not part of this build");
   }
   if (static_cast<std::uint8_t>(options.acceleration) > static_cast<std::uint8_t>(SaAcceleration::kFull)) {
@@ -98,14 +101,14 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
   if (!detail::SaConstructionCanRepresent(SaBackend::kHip, width, n)) {
     detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
   }
-  const bool retain_lcp = options.acceleration != SaAcceleration::kNone;
-  const bool retain_isa = options.acceleration == SaAcceleration::kLcpSuffixLink;
-  if (buffers.text == nullptr || buffers.sa == nullptr || (retain_isa && buffers.isa == nullptr) ||
+  const bool retain_lcp = low_memory || options.acceleration != SaAcceleration::kNone;
+  const bool retain_isa = !low_memory && options.acceleration == SaAcceleration::kLcpSuffixLink;
+  if (buffers.text == nullptr || buffers.sa == nullptr || ((retain_isa || low_memory) && buffers.isa == nullptr) ||
       (retain_lcp && buffers.lcp == nullptr)) {
     detail::Throw(ErrorCode::kInvalidInput, "device text, SA and the retained ISA / LCP are required");
   }
   CheckBits(buffers.sa_bits, "SA");
-  if (retain_isa) CheckBits(buffers.isa_bits, "ISA");
+  if (retain_isa || low_memory) CheckBits(buffers.isa_bits, "ISA");
   if (retain_lcp) CheckBits(buffers.lcp_bits, "LCP");
   const auto stored = detail::ResolveProfileStorageWidth(options.storage_width, options.resource_profile, n);
 
@@ -136,7 +139,36 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
     sections.push_back(
         CoordinateSection(detail::SectionType::kInverseSuffixArray, aux, buffers.isa, buffers.isa_bits, n, n));
   }
-  if (retain_lcp) {
+  if (low_memory) {
+    // SKL1 codec 1: primary bytes streamed from the raw LCP, anchors found
+    // on the device and written from the host (sa_codec.cpp:569-631).
+    const std::uint32_t anchor_width = n - 1U > std::numeric_limits<std::uint32_t>::max() ? 64 : 32;
+    char err[512] = {0};
+    std::uint64_t anchors = 0;
+    int status = sfk_hip_byte_coded_lcp(buffers.device, buffers.lcp, buffers.lcp_bits, buffers.isa, buffers.isa_bits,
+                                        n, nullptr, &anchors, nullptr, nullptr, 0, err, sizeof(err));
+    std::vector<std::uint64_t> positions(anchors), values(anchors);
+    if (status == SFK_HIP_OK && anchors != 0) {
+      status = sfk_hip_byte_coded_lcp(buffers.device, buffers.lcp, buffers.lcp_bits, buffers.isa, buffers.isa_bits,
+                                      n, nullptr, &anchors, positions.data(), values.data(), positions.size(), err,
+                                      sizeof(err));
+    }
+    if (status != SFK_HIP_OK) {
+      detail::Throw(static_cast<ErrorCode>(status - 1), std::string("byte-coded LCP anchors: ") + err);
+    }
+    std::vector<std::uint8_t> planes;
+    planes.reserve(2 * anchors * (anchor_width / 8));
+    for (const auto* plane : {&positions, &values}) {
+      for (const std::uint64_t v : *plane) {
+        for (std::uint32_t b = 0; b < anchor_width / 8; ++b) planes.push_back(static_cast<std::uint8_t>(v >> (8 * b)));
+      }
+    }
+    StreamSection s{detail::SectionType::kLcp, {}};
+    s.parts.push_back(HostPart(detail::LcpSectionHeader(anchor_width, 1, n, n, true, anchors)));
+    s.parts.push_back(DevicePart(buffers.lcp, n, buffers.lcp_bits, SFK_HIP_PLANE_CAPPED8));
+    s.parts.push_back(HostPart(std::move(planes)));
+    sections.push_back(std::move(s));
+  } else if (retain_lcp) {
     const std::uint32_t lcp_width = n - 1U > std::numeric_limits<std::uint32_t>::max() ? 64 : 32;
     StreamSection s{detail::SectionType::kLcp, {}};
     s.parts.push_back(HostPart(detail::LcpSectionHeader(lcp_width, 1, n, n)));

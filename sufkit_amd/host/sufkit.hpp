@@ -228,7 +228,8 @@ struct DeviceSaveStatistics {
 // arrays: each section is streamed from HBM through pinned buffers with its
 // CRC32 computed on the device, then the file is reread and checked and
 // published as Save does (suffix_array.cpp:4853-4916,
-// serialization.cpp:529-619).  Sampling rate 1 only.
+// serialization.cpp:529-619).  Sampling rate 1 only.  A Low-memory index
+// needs the device ISA and raw LCP too: its byte-coded LCP is made from them.
 void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers& buffers,
                      const std::filesystem::path& path, const SaveOptions& save = {},
                      const SuffixArrayBuildOptions& options = FastSuffixArrayBuildOptions(),

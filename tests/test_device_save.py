@@ -40,9 +40,7 @@ def contigs(rng, total: int, count: int) -> list[bytes]:
 
 
 def golden_items():
-    """The Fast-profile golden files (the device writer saves Fast indexes)."""
-    items = json.loads((GOLDEN / "sufidx_index.json").read_text()).items()
-    return sorted((k, v) for k, v in items if v.get("resource_profile", 0) == 0)
+    return sorted(json.loads((GOLDEN / "sufidx_index.json").read_text()).items())
 
 
 # ----------------------------------------------------------------- CPU ---
@@ -97,7 +95,8 @@ def test_device_save_matches_the_reference_save(tmp_path, hip, fname, meta):
     records = golden_references()[meta["reference"]]
     ref = api.GenomeReference.from_records(records)
     opts = api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=meta["coordinate_width"],
-                            storage_width=meta["storage_width"], acceleration=meta["acceleration"])
+                            storage_width=meta["storage_width"], acceleration=meta["acceleration"],
+                            resource_profile=meta.get("resource_profile", 0))
     bits = meta["coordinate_width"] or 32
     out = tmp_path / "device.sufidx"
     stats = api.save_device_index(ref, out, opts, construction_bits=bits)
@@ -219,9 +218,24 @@ def test_a_corrupted_section_is_detected_on_load(tmp_path, hip):
 
 
 @pytest.mark.gpu
-def test_device_save_of_a_low_memory_index_is_unsupported(tmp_path, hip):
-    ref = api.GenomeReference.from_records([("a", "ACGT" * 40)])
-    with pytest.raises(SufkitError) as e:
-        api.save_device_index(ref, tmp_path / "l.sufidx", api.BuildOptions(backend=api.BACKEND_HIP,
-                                                                           resource_profile=1))
-    assert e.value.status == 3
+@pytest.mark.parametrize("construction,storage", [(32, 0), (64, 40)])
+def test_device_save_of_a_low_memory_index(tmp_path, hip, construction, storage):
+    """Low-memory: byte-coded LCP (capped bytes streamed from the device raw
+    LCP, anchors found on the device), no ISA section; the file equals the
+    host path's on a multi-chunk reference whose repeats need overflow
+    anchors."""
+    rng = np.random.default_rng(7 + construction)
+    seqs = contigs(rng, 17_500_000, 4)
+    unit = seqs[0][1000:4000]
+    seqs = [s + unit * 3 + b"A" * 900 for s in seqs]
+    ref = api.GenomeReference.from_records([(f"c{i}", s) for i, s in enumerate(seqs)])
+    opts = api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=construction, storage_width=storage,
+                            resource_profile=1)
+    dev, host = tmp_path / "device.sufidx", tmp_path / "host.sufidx"
+    api.save_device_index(ref, dev, opts, construction_bits=construction)
+    index = api.SuffixArray.build(ref, opts)
+    index.save(host)
+    assert dev.read_bytes() == host.read_bytes()
+    back = api.SuffixArray.load(dev)
+    assert back.info()["lcp_encoding"] == 2 and back.isa() is None
+    assert np.array_equal(back.lcp(), index.lcp())
