@@ -1,20 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: SA + ISA + raw LCP construction (Fast profile) on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3]
 
 A step is one complete device build (pack, prefix doubling, ISA validation,
 Φ/PLCP/LCP) of the workload's logical text, already resident in HBM, into
-device SA/ISA/LCP buffers.  The default workload is the 64 Mbp multi-contig
-reference (200 contigs, GC 45%, 1% N runs, 30% diverged repeats, 1 Mbp
-poly-A/period-2/period-7 block), 32-bit construction, uint32 SA/ISA/LCP.
+device SA/ISA/LCP buffers.  The default workload is the 3.1 Gbp human-scale
+genome (24 contigs of 50-250 Mbp, GC 41%, 45% repeats, 3% satellites, N gaps
+up to 10 Mbp) with 64-bit construction and validated 32-bit storage.
 
 Prints one JSON line.  `value` is bases / 1e6 / seconds-per-step.  The
 `roofline` object is for the dominant kernel (the LSD radix scatter pass),
-timed with HIP events on the build stream during the timed steps;
+timed with HIP events on the build stream during the timed steps.  After
+the timed steps the device digests its SA/ISA/LCP and compares them with the
+reference's own digests of the same text (tests/golden/workload_digests.json),
+and the device checker proves order, inverse and LCP of every row.
 `cpu_baseline` is the reference's own divsufsort backend (SA + ISA + Kasai
-LCP, 1 thread) timed on this host on the same text, and its output is
-compared with the GPU's element by element.
+LCP, 1 thread; divsufsort64 for 64-bit construction) timed on this host on a
+bounded prefix of the same text, with CaPS-SA on all host threads beside it.
+`host_build` times the whole host SuffixArray::Build (host text to validated
+host storage plus the Fast prefix directory), the reference's build_seconds
+scope.
 """
 from __future__ import annotations
 
@@ -51,9 +57,9 @@ WORKLOADS = {
 def parse_args():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=3, choices=sorted(WORKLOADS))
     ap.add_argument("--scale", type=float, default=1.0, help="workload size multiplier")
     ap.add_argument("--construction-bits", type=int, default=None, choices=(32, 64))
     ap.add_argument("--seed", type=int, default=20260822)
@@ -65,6 +71,10 @@ def parse_args():
     ap.add_argument("--sa-count", action="store_true",
                     help="served: count by SA binary search instead of FM backward search")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-build", action="store_true",
+                    help="skip the timed host SuffixArray::Build leg")
+    ap.add_argument("--no-extra-leg", action="store_true",
+                    help="skip the 64 Mbp 32-bit leg that the 3.1 Gbp run adds")
     ap.add_argument("--cpu-sample-symbols", type=int, default=0,
                     help="symbols of the text timed on the CPU (0 = whole text)")
     ap.add_argument("--no-profile", action="store_true",
@@ -114,21 +124,63 @@ def radix_tuning(args, max_elements: int):
     return {"source": source, "seconds": round(seconds, 3), "max_elements": max_elements, "chosen": chosen}
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "sa_build_64mbp_pmc_traffic.md"
+# HBM traffic per RadixOnesweep launch, measured by two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE; calibrated against a device copy of known size)
+# and committed with their commands, per (workload, construction bits).
+TRAFFIC_PROFILES = {
+    (3, 64): ROOT / "profiles" / "sa_build_3100mbp_pmc_traffic.md",
+    (2, 32): ROOT / "profiles" / "sa_build_64mbp_pmc_traffic.md",
+}
+
+GOLDEN_DIGESTS = ROOT / "tests" / "golden" / "workload_digests.json"
 
 
-def measured_traffic() -> dict:
-    """HBM bytes per RadixOnesweep launch of the default workload, as measured
-    by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; calibrated against a
-    device copy of known size) and committed with their commands."""
+def measured_traffic(workload: int, bits: int) -> dict:
+    path = TRAFFIC_PROFILES.get((workload, bits))
+    if path is None:
+        return {}
     try:
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in path.read_text().splitlines():
             if line.startswith("onesweep_traffic_bytes_per_launch:"):
                 return {"traffic": int(line.split(":", 1)[1]),
-                        "traffic_source": "profiles/sa_build_64mbp_pmc_traffic.md"}
+                        "traffic_source": str(path.relative_to(ROOT))}
     except OSError:
         pass
     return {}
+
+
+def golden_digests(workload: int, seed: int, scale: float) -> dict | None:
+    """The reference's SA/ISA/LCP digests of this workload's text, when the
+    golden file holds them (workloads 1-3 at the default seed and scale)."""
+    key = {1: "c1", 2: "c2", 3: "c3"}.get(workload)
+    if key is None or seed != 20260822 or scale != 1.0:
+        return None
+    try:
+        return json.loads(GOLDEN_DIGESTS.read_text()).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def device_parity(d_text, n, d_sa, d_isa, d_lcp, out_bits, want, fingerprint) -> dict:
+    """Device digests of the built arrays against the reference's digests,
+    and the device's full order/inverse/LCP check of every row."""
+    import sufkit_amd
+
+    got = {k: f"0x{sufkit_amd.device_digest(ptr, n, out_bits):016x}"
+           for k, ptr in (("sa", d_sa), ("isa", d_isa), ("lcp", d_lcp))}
+    out = {"digests": got}
+    if want is not None:
+        same = want.get("fingerprint") == fingerprint and all(got[k] == want[k] for k in got)
+        out["parity_vs_reference_digests"] = "bit-exact" if same else "MISMATCH"
+        out["reference_digest_method"] = want.get("method")
+    rep = sufkit_amd.check_index(d_text, n, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits,
+                                 raise_on_failure=False)
+    out["full_check"] = {"rows": rep["rows"], "failed": rep["status"] != 0,
+                         "order_failures": rep["order_failures"], "inverse_failures": rep["inverse_failures"],
+                         "range_failures": rep["range_failures"],
+                         "lcp_failures": rep["lcp_failures"] + rep["lcp_prefix_failures"],
+                         "seconds": round(rep["seconds"], 3)}
+    return out
 
 
 def cpu_info() -> str:
@@ -204,29 +256,6 @@ def cpu_baseline_caps(sample: np.ndarray, width_bits: int, whole: bool):
                    f"{threads} threads on {cpu_info()}"),
         "seconds": round(dt, 3), "sa_lcp_seconds": round(t_sa_lcp, 3), "isa_seconds": round(t_isa, 3),
     }, (sa, isa, lcp)
-
-
-def sample_check(text: np.ndarray, sa_rows: np.ndarray, sa_prev: np.ndarray, lcp_rows: np.ndarray) -> int:
-    """Size-independent parity on sampled rows: suffix SA[r-1] < suffix SA[r]
-    and LCP[r] == lcp(SA[r-1], SA[r]), by direct byte comparison.  Returns
-    the number of failing rows."""
-    bad = 0
-    n = text.size
-    for a, b, l in zip(sa_prev.tolist(), sa_rows.tolist(), lcp_rows.tolist()):
-        k, step = 0, 64
-        while True:  # chunked compare; the unique sentinel ends every run
-            m = min(step, n - a - k, n - b - k)
-            if m <= 0:
-                break
-            diff = np.flatnonzero(text[a + k:a + k + m] != text[b + k:b + k + m])
-            if diff.size:
-                k += int(diff[0])
-                break
-            k += m
-            step = min(step * 4, 1 << 22)
-        if k != l or not (b + k < n and (a + k >= n or text[a + k] < text[b + k])):
-            bad += 1
-    return bad
 
 
 def run_batched(args) -> int:
@@ -482,6 +511,74 @@ def low_memory_leg(hip, d_sa, d_isa, d_lcp, n, out_bits, sa, lcp) -> dict:
     return out
 
 
+def host_build_leg(text: np.ndarray, contigs: np.ndarray, bits: int) -> dict:
+    """The whole host SuffixArray::Build with the HIP backend, timed as the
+    reference times build_seconds (sa_build_benchmark.cpp:548-553): host
+    text to validated host SA/ISA/LCP storage plus the Fast prefix
+    directory; the reference model is made before the clock starts."""
+    from sufkit_amd import api
+
+    bases = int(contigs.sum())
+    ref = api.GenomeReference.from_encoded(text, contigs)
+    opts = api.BuildOptions(backend=api.BACKEND_HIP, coordinate_width=bits)
+    t0 = time.perf_counter()
+    index = api.SuffixArray.build(ref, opts)
+    wall = time.perf_counter() - t0
+    st = index.stats
+    info = index.info()
+    out = {
+        "t_build_seconds": round(wall, 3), "mbps": round(bases / 1e6 / wall, 2),
+        "backend_call_seconds": round(st["call_seconds"], 3),
+        "device_seconds": round(st["device_seconds"], 3),
+        "pcie_seconds": round(st["transfer_seconds"], 3),
+        "prefix_directory_seconds": round(st["prefix_directory_seconds"], 4),
+        "host_storage_seconds": round(st["storage_compaction_seconds"] + st["lcp_finalize_seconds"], 3),
+        "prefix_directory_k": info["prefix_directory_k"],
+        "scope": ("SuffixArray::Build (HIP backend): text prepare, host->device through pinned staging, "
+                  "device build, Fast prefix directory on the device, device->host, host storage"),
+    }
+    del index, ref
+    return out
+
+
+def timed_device_build(builder, d_text, n, bits, d_sa, d_isa, d_lcp, out_bits, flags, steps, warmup):
+    def step():
+        return builder.build(d_text, n, bits, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits, flags)
+
+    for _ in range(warmup):
+        step()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(step())   # synchronous: returns after the build stream drains
+    return stats, (time.perf_counter() - t0) / steps
+
+
+def extra_leg(args, hip) -> dict:
+    """The 64 Mbp multi-contig reference with 32-bit construction: device
+    time, digests against the reference's, and the full check."""
+    import oracle
+    import sufkit_amd
+
+    text, contigs = sufkit_amd.synthetic_text(2, args.seed, 0, 1.0)
+    n, bases = int(text.size), int(contigs.sum())
+    arrays = [sufkit_amd.DeviceArray(n + 64)] + [sufkit_amd.DeviceArray(n * 4) for _ in range(3)]
+    arrays[0].upload(text)
+    d_text, d_sa, d_isa, d_lcp = (a.ptr for a in arrays)
+    with sufkit_amd.DeviceBuilder(0) as builder:
+        flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
+        stats, sec = timed_device_build(builder, d_text, n, 32, d_sa, d_isa, d_lcp, 32, flags,
+                                        max(args.steps, 5), 1)
+    out = {"workload": WORKLOADS[2], "construction_bits": 32, "text_symbols": n, "bases": bases,
+           "ms_per_step": round(sec * 1e3, 3), "mbps": round(bases / 1e6 / sec, 2),
+           "device_seconds": round(sum(s["device_seconds"] for s in stats) / len(stats), 6)}
+    out.update(device_parity(d_text, n, d_sa, d_isa, d_lcp, 32, golden_digests(2, args.seed, 1.0),
+                             f"0x{oracle.fingerprint(text[:-1]):016x}"))
+    for a in arrays:
+        a.free()
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -492,12 +589,15 @@ def main() -> int:
         return run_batched(args)
     if args.workload == 5:
         return run_served(args)
+    import oracle
     import sufkit_amd
 
     text, contigs = sufkit_amd.synthetic_text(args.workload, args.seed, 0, args.scale)
     n = int(text.size)
     bases = int(contigs.sum())
-    bits = args.construction_bits or (32 if n - 1 <= 0xFFFFFFFF else 64)
+    # The 3.1 Gbp genome is the 64-bit-construction workload (n > INT32_MAX,
+    # the reference's divsufsort64 path); the others construct in 32 bits.
+    bits = args.construction_bits or (64 if args.workload == 3 or n - 1 > 0xFFFFFFFF else 32)
     out_bits = 32 if n - 1 <= 0xFFFFFFFF else 64
     hip = sufkit_amd.load_hip()
 
@@ -527,20 +627,13 @@ def main() -> int:
     flags = sufkit_amd.SFK_HIP_RETAIN_ISA | sufkit_amd.SFK_HIP_RETAIN_LCP
     if not args.no_profile:
         flags |= sufkit_amd.SFK_HIP_PROFILE_KERNELS
+    stats, sec_per_step = timed_device_build(builder, d_text, n, bits, d_sa, d_isa, d_lcp, out_bits, flags,
+                                             args.steps, args.warmup)
+    ms_per_step = sec_per_step * 1e3
+    builder.close()  # its workspace is not needed by the checks and legs below
 
-    def step():
-        return builder.build(d_text, n, bits, d_sa, out_bits, d_isa, out_bits, d_lcp, out_bits, flags)
-
-    for _ in range(args.warmup):
-        step()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())   # synchronous: returns after the build stream drains
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-
-    # Results back to the host once (PCIe, outside the timed region).
+    # Results back to the host once (PCIe, outside the timed region), for
+    # the element-by-element comparison with the CPU baselines.
     dt = np.uint32 if out_bits == 32 else np.uint64
     full_copy = n <= (1 << 28)
     sa = isa = lcp = None
@@ -553,22 +646,9 @@ def main() -> int:
         for arr, ptr in ((sa, d_sa), (isa, d_isa), (lcp, d_lcp)):
             hip.sfk_hip_memcpy_d2h(arr.ctypes.data, ptr, arr.nbytes)
         d2h = time.perf_counter() - t0
-    # Sampled adjacent-row check (order + LCP) on the full-size result.
-    rng = np.random.default_rng(1)
-    rows = np.unique(rng.integers(1, n, size=4000))
-    item = np.dtype(dt).itemsize
-    def fetch(ptr, idx):
-        out = np.empty(idx.size, dt)
-        one = np.empty(1, dt)
-        for j, r in enumerate(idx.tolist()):
-            hip.sfk_hip_memcpy_d2h(one.ctypes.data, ptr + r * item, item)
-            out[j] = one[0]
-        return out.astype(np.uint64)
-    if full_copy:
-        s_rows, s_prev, s_lcp = sa[rows].astype(np.uint64), sa[rows - 1].astype(np.uint64), lcp[rows].astype(np.uint64)
-    else:
-        s_rows, s_prev, s_lcp = fetch(d_sa, rows), fetch(d_sa, rows - 1), fetch(d_lcp, rows)
-    sample_bad = sample_check(text, s_rows, s_prev, s_lcp)
+    fingerprint = f"0x{oracle.fingerprint(text[:-1]):016x}"
+    parity = device_parity(d_text, n, d_sa, d_isa, d_lcp, out_bits,
+                           golden_digests(args.workload, args.seed, args.scale), fingerprint)
 
     last = stats[-1]
     scatter_s = sum(s["sort_scatter_seconds"] for s in stats)
@@ -587,12 +667,12 @@ def main() -> int:
             "avg_launch_us": round(scatter_s / scatter_n * 1e6, 2),
             "bytes_per_launch": scatter_b // scatter_n,
         }
-        if args.workload == 2 and args.scale == 1.0 and bits == 32:
-            roofline.update(measured_traffic())
+        if args.scale == 1.0:
+            roofline.update(measured_traffic(args.workload, bits))
 
     result = {
         "metric": "SA+ISA+LCP build Mbp/s + HBM roofline frac, 1 MI355X vs divsufsort64 on host",
-        "value": round(bases / 1e6 / (ms_per_step / 1e3), 2),
+        "value": round(bases / 1e6 / sec_per_step, 2),
         "unit": "Mbp/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -607,7 +687,7 @@ def main() -> int:
             "workload": WORKLOADS[args.workload] + (f" x{args.scale}" if args.scale != 1.0 else ""),
             "text_symbols": n, "bases": bases, "contigs": int(contigs.size),
             "construction_bits": bits, "stored_bits": out_bits,
-            "profile": "fast (SA+ISA+raw LCP)",
+            "profile": "fast (SA+ISA+raw LCP)", "fingerprint": fingerprint,
         },
         "roofline": roofline,
         "phases": {
@@ -633,10 +713,8 @@ def main() -> int:
             "active_total": last["active_total"], "sort_passes": last["sort_passes"],
             "lce_long_items": last["lce_long_items"],
         },
-        "end_to_end_mbps_incl_pcie": (round(bases / 1e6 / (ms_per_step / 1e3 + h2d + d2h), 2)
-                                      if full_copy else None),
-        "pcie_seconds": {"h2d": round(h2d, 4), "d2h": round(d2h, 4) if full_copy else None},
-        "sampled_rows_checked": int(rows.size), "sampled_rows_failed": sample_bad,
+        "parity": parity,
+        "pcie_seconds": {"h2d_text": round(h2d, 4), "d2h_results": round(d2h, 4) if full_copy else None},
         "radix_tuning": tuning,
     }
 
@@ -649,7 +727,6 @@ def main() -> int:
                   and np.array_equal(lcp.astype(np.uint64), rlcp))
             result["parity_vs_cpu_baseline"] = "bit-exact" if ok else "MISMATCH"
         del rsa, risa, rlcp
-        import oracle
         if oracle.ref_caps_available() and sample.size >= 16:
             caps, (csa, cisa, clcp) = cpu_baseline_caps(sample, bits, whole)
             result["cpu_baseline_caps"] = caps
@@ -657,23 +734,31 @@ def main() -> int:
                 ok = (np.array_equal(sa.astype(np.uint64), csa) and np.array_equal(isa.astype(np.uint64), cisa)
                       and np.array_equal(lcp.astype(np.uint64), clcp))
                 result["parity_vs_cpu_baseline_caps"] = "bit-exact" if ok else "MISMATCH"
+            del csa, cisa, clcp
     else:
         result["cpu_baseline"] = None
 
-    builder.close()  # its workspace is not needed by the legs below
     if args.save:
         result["save"] = save_leg(args, text, contigs, d_text, d_sa, d_isa, d_lcp, bits, out_bits)
     if args.low_memory:
         result["low_memory"] = low_memory_leg(hip, d_sa, d_isa, d_lcp, n, out_bits, sa, lcp)
     for ptr in (d_text, d_sa, d_isa, d_lcp):
         hip.sfk_hip_free(ptr)
+    del sa, isa, lcp
+    if not args.no_host_build:
+        result["host_build"] = host_build_leg(text, contigs, bits)
+    if args.workload == 3 and args.scale == 1.0 and not args.no_extra_leg:
+        result["extra_leg_64mbp"] = extra_leg(args, hip)
     print(json.dumps(result), flush=True)
     if args.save and result["save"].get("parity_vs_reference_save", "sections identical") != "sections identical":
         return 1
     if args.low_memory and result["low_memory"].get("decodes_to_raw_lcp", True) is not True:
         return 1
+    legs = [parity] + ([result["extra_leg_64mbp"]] if "extra_leg_64mbp" in result else [])
     ok = (result.get("parity_vs_cpu_baseline", "bit-exact") == "bit-exact"
-          and result.get("parity_vs_cpu_baseline_caps", "bit-exact") == "bit-exact" and sample_bad == 0)
+          and result.get("parity_vs_cpu_baseline_caps", "bit-exact") == "bit-exact"
+          and all(leg.get("parity_vs_reference_digests", "bit-exact") == "bit-exact"
+                  and not leg["full_check"]["failed"] for leg in legs))
     return 0 if ok else 1
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SFK_HIP_ABI_VERSION 2
+#define SFK_HIP_ABI_VERSION 3
 
 enum sfk_hip_status {
   SFK_HIP_OK = 0,
@@ -39,6 +39,14 @@ enum sfk_hip_status {
   SFK_HIP_VERSION_MISMATCH = 5,    /* ErrorCode::kVersionMismatch */
   SFK_HIP_BUILD_FAILURE = 6        /* ErrorCode::kBuildFailure */
 };
+
+/* Longest text (symbols, sentinel included) any entry point accepts:
+ * 2^32 - 2^16.  Kernels that take one element per work-item launch at most
+ * this many work-items, within the 32-bit grid size of a dispatch, and the
+ * workspace of a 64-bit construction of a longer text (over 100 bytes per
+ * symbol) would not fit one MI355X's 288 GB anyway.  Longer texts give
+ * SFK_HIP_UNSUPPORTED_BACKEND. */
+#define SFK_HIP_MAX_TEXT_SYMBOLS 4294901760ull
 
 /* Build flags (retain_isa / retain_lcp of divsufsort_backend.hpp:27-33). */
 #define SFK_HIP_RETAIN_ISA 1u
@@ -68,6 +76,8 @@ typedef struct sfk_hip_stats {
   uint64_t sort_scatter_launches;
   uint64_t sort_scatter_bytes;
   double prefix_directory_seconds; /* sfk_hip_build_fast's directory pass */
+  double call_seconds;      /* whole host call: allocation, copies, build,
+                               directory (host entry points only) */
 } sfk_hip_stats;
 
 typedef struct sfk_hip_builder sfk_hip_builder;
@@ -93,20 +103,27 @@ uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits);
 
 /*
  * Device-resident build.  d_text: n device bytes (values 0..6, a single 0 at
- * position n-1).  Outputs are device buffers of n elements:
- *   d_sa  of sa_bits  (32 or 64), required
- *   d_isa of isa_bits (32 or 64), or NULL
- *   d_lcp of lcp_bits (32 or 64), or NULL
+ * position n-1).  sampling_rate K >= 1 (sampling_rate of
+ * BuildDivsufsort32/64, divsufsort_backend.hpp:27-33): the outputs keep the
+ * rows whose text position is a multiple of K, rows = ceil(n / K) = (n + K -
+ * 1) / K elements each (n when K = 1):
+ *   d_sa  of sa_bits  (32 or 64), required: the kept SA values in SA order
+ *   d_isa of isa_bits (32 or 64), or NULL: d_isa[p / K] = kept row of p
+ *   d_lcp of lcp_bits (32 or 64), or NULL: LCP of each kept row and the
+ *         previous kept row (0 for the first), the minimum complete LCP
+ *         over the rows between them
+ * exactly the arrays the reference's adapter returns for K
+ * (divsufsort_backend.cpp:62-150); K = 0 gives SFK_HIP_INVALID_INPUT.
  * construction_bits: 32 (n <= 2^32 - 1) or 64.  A stored width narrower than
  * the construction width is checked on the device (range error ->
  * SFK_HIP_BUILD_FAILURE).  Synchronous: returns after the stream drains.
  */
 int sfk_hip_build_device(sfk_hip_builder* builder, const uint8_t* d_text,
-                         uint64_t n, uint32_t construction_bits, uint32_t flags,
-                         void* d_sa, uint32_t sa_bits, void* d_isa,
-                         uint32_t isa_bits, void* d_lcp, uint32_t lcp_bits,
-                         sfk_hip_stats* stats, char* error,
-                         size_t error_capacity);
+                         uint64_t n, uint32_t construction_bits,
+                         uint32_t sampling_rate, uint32_t flags, void* d_sa,
+                         uint32_t sa_bits, void* d_isa, uint32_t isa_bits,
+                         void* d_lcp, uint32_t lcp_bits, sfk_hip_stats* stats,
+                         char* error, size_t error_capacity);
 
 /*
  * Batched device build of `refs` independent references laid out back to
@@ -129,15 +146,19 @@ int sfk_hip_build_batch_device(sfk_hip_builder* builder, const uint8_t* d_text,
                                size_t error_capacity);
 
 /*
- * Host build: the same, from host text into host output buffers (each n
- * elements of the given width; isa/lcp may be NULL).  This is the entry
- * point a SuffixArray::Build backend branch calls in place of
- * detail::BuildDivsufsort32/64 (divsufsort_backend.cpp:24-156).
+ * Host build: the same, from host text into host output buffers (each
+ * ceil(n / K) elements of the given width; isa/lcp may be NULL).  This is
+ * the entry point a SuffixArray::Build backend branch calls in place of
+ * detail::BuildDivsufsort32/64(text, sampling_rate, retain_lcp, retain_isa,
+ * threads) (divsufsort_backend.hpp:27-33, .cpp:24-156).  The text goes up
+ * and the results come back through pinned staging buffers; stats get the
+ * copy times and the whole call's time.
  */
 int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits,
-                  uint32_t flags, void* sa, uint32_t sa_bits, void* isa,
-                  uint32_t isa_bits, void* lcp, uint32_t lcp_bits,
-                  sfk_hip_stats* stats, char* error, size_t error_capacity);
+                  uint32_t sampling_rate, uint32_t flags, void* sa,
+                  uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
+                  uint32_t lcp_bits, sfk_hip_stats* stats, char* error,
+                  size_t error_capacity);
 
 /*
  * Fast prefix directory: the exact k-mer -> SA-interval table that
@@ -157,7 +178,9 @@ int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits,
  *     (counts[0] = indexed k-mers, counts[1] = non-empty entries);
  *   sfk_hip_build_fast: sfk_hip_build plus the directory of its SA into a
  *     host table, computed while the SA is still on the device
- *     (prefix_k = 0: none; sfk_hip_build is this call with 0).
+ *     (prefix_k = 0: none; sfk_hip_build is this call with 0; k > 0 needs
+ *     a complete index, sampling_rate 1, as Build derives the directory
+ *     only then).
  */
 uint32_t sfk_prefix_directory_select_k(uint64_t suffix_count, uint64_t resident_core_bytes);
 uint64_t sfk_prefix_directory_words(uint32_t k, uint64_t suffix_count);
@@ -165,7 +188,7 @@ int sfk_hip_prefix_directory(sfk_hip_builder* builder, const uint8_t* d_text, ui
                              const void* d_sa, uint32_t sa_bits, uint32_t k, uint64_t* d_table,
                              uint64_t counts[2], char* error, size_t error_capacity);
 int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bits,
-                       uint32_t flags, void* sa, uint32_t sa_bits, void* isa,
+                       uint32_t sampling_rate, uint32_t flags, void* sa, uint32_t sa_bits, void* isa,
                        uint32_t isa_bits, void* lcp, uint32_t lcp_bits, uint32_t prefix_k,
                        uint64_t* prefix_table, uint64_t prefix_counts[2],
                        sfk_hip_stats* stats, char* error, size_t error_capacity);
@@ -345,6 +368,56 @@ int sfk_hip_byte_coded_lcp(int device, const void* d_lcp, uint32_t lcp_bits,
                            uint64_t* anchor_positions, uint64_t* anchor_values,
                            uint64_t capacity, char* error,
                            size_t error_capacity);
+
+/*
+ * Full-size checks of a built index in device memory.
+ *
+ * sfk_hip_digest: the position-keyed sum digest
+ *   count * G + sum_i fmix64(values[i] + G * (i + 1))  (mod 2^64)
+ * (G = 0x9E3779B97F4A7C15, fmix64 the splitmix64 finaliser) of `count`
+ * device values of value_bits 8, 32 or 64; equal arrays give equal digests
+ * whatever their width.  No counterpart in the reference; its benchmark
+ * compares whole SAs by checksum the same way (sa_build_benchmark.cpp:
+ * 455-465,1130-1140).
+ *
+ * sfk_hip_check_index: proves SA, ISA and (when d_lcp is given) LCP of the
+ * n-symbol device text without a second construction:
+ *   - every SA value < n and ISA[SA[r]] == r (SA a permutation, ISA its
+ *     inverse: the checks of RepackSuffixArray, suffix_array.cpp:852-881, and
+ *     of Load, :4999-5010,5120-5131);
+ *   - SA[0] == n - 1 and every adjacent pair a = SA[r-1], b = SA[r] ordered:
+ *     T[a] < T[b], or T[a] == T[b] and ISA[a+1] < ISA[b+1] (Burkhardt and
+ *     Kaerkkaeinen's suffix array checker);
+ *   - LCP[0] == 0 and, for l = LCP[r], T[a+l] != T[b+l] and the first l
+ *     symbols of a and b equal under a Karp-Rabin hash modulo 2^61 - 1 whose
+ *     base comes from hash_seed (a wrong value passes with probability at
+ *     most n / 2^61).
+ * Corrupted arrays give failure counts, never an access outside the arrays.
+ * Returns SFK_HIP_OK when every check passes, SFK_HIP_BUILD_FAILURE when one
+ * fails (the reference's code for a suffix array that is not a permutation,
+ * suffix_array.cpp:852-881), with the counts in *report either way.  Holds
+ * 8 (n + 1) bytes of device memory for the prefix hashes while it runs.
+ */
+typedef struct sfk_hip_check_report {
+  uint64_t rows;
+  uint64_t range_failures;      /* SA or ISA value >= n */
+  uint64_t inverse_failures;    /* ISA[SA[r]] != r */
+  uint64_t order_failures;      /* SA[0] != n - 1, or an adjacent pair out of order */
+  uint64_t lcp_failures;        /* LCP[0] != 0, or T[a+l] == T[b+l] / past the end */
+  uint64_t lcp_prefix_failures; /* the first LCP[r] symbols differ */
+  uint64_t first_failed_row;    /* n when every row passed */
+  uint32_t lcp_checked;         /* 1 when d_lcp was given */
+  uint32_t reserved;
+  double seconds;
+} sfk_hip_check_report;
+int sfk_hip_digest(int device, const void* d_values, uint64_t count,
+                   uint32_t value_bits, uint64_t* out, char* error,
+                   size_t error_capacity);
+int sfk_hip_check_index(int device, const uint8_t* d_text, uint64_t n,
+                        const void* d_sa, uint32_t sa_bits, const void* d_isa,
+                        uint32_t isa_bits, const void* d_lcp, uint32_t lcp_bits,
+                        uint64_t hash_seed, sfk_hip_check_report* report,
+                        char* error, size_t error_capacity);
 
 /* Device memory helpers so callers without a device allocator of their own
  * can stage buffers (tests, benchmarks). */

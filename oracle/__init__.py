@@ -69,6 +69,12 @@ def lib() -> ctypes.CDLL:
         L.sfk_oracle_crc32.restype = ctypes.c_uint32
         L.sfk_oracle_digest_u64.argtypes = [U64P, ctypes.c_uint64]
         L.sfk_oracle_digest_u64.restype = ctypes.c_uint64
+        L.sfk_oracle_sum_digest_u64.argtypes = [U64P, ctypes.c_uint64]
+        L.sfk_oracle_sum_digest_u64.restype = ctypes.c_uint64
+        L.sfk_oracle_sum_digest_u32.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]
+        L.sfk_oracle_sum_digest_u32.restype = ctypes.c_uint64
+        L.sfk_oracle_lean_digests.argtypes = [U8P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32), U64P]
+        L.sfk_oracle_lean_digests.restype = ctypes.c_int
         L.sfk_oracle_prefix_directory.argtypes = [U8P, ctypes.c_uint64, U64P, ctypes.c_uint32, U64P, U64P, U64P]
         L.sfk_oracle_prefix_directory.restype = ctypes.c_uint64
         _LIB = L
@@ -273,6 +279,47 @@ def digest(values: np.ndarray) -> int:
     return int(lib().sfk_oracle_digest_u64(_u64(values), values.size))
 
 
+def sum_digest(values: np.ndarray) -> int:
+    """Position-keyed sum digest (oracle.c sfk_oracle_sum_digest_u64): the
+    form a device reduction can compute."""
+    if values.dtype == np.uint32:
+        values = np.ascontiguousarray(values)
+        return int(lib().sfk_oracle_sum_digest_u32(
+            values.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), values.size))
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    return int(lib().sfk_oracle_sum_digest_u64(_u64(values), values.size))
+
+
+def lean_digests(text: np.ndarray, sa32: np.ndarray) -> dict:
+    """Sum digests of SA, ISA and LCP from a complete 32-bit SA (ISA scatter
+    and Phi/PLCP on the CPU, 8n bytes of work arrays)."""
+    text, p = _u8(text)
+    sa32 = np.ascontiguousarray(sa32, dtype=np.uint32)
+    out = np.zeros(4, np.uint64)
+    rc = lib().sfk_oracle_lean_digests(p, text.size, sa32.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                       _u64(out))
+    if rc != 0:
+        raise RuntimeError("lean digests: SA is not a permutation or memory ran out")
+    return {"sa": int(out[0]), "isa": int(out[1]), "lcp": int(out[2]), "lcp_max": int(out[3])}
+
+
+def ref_divsufsort64_u32(text: np.ndarray) -> tuple[np.ndarray, float]:
+    """The compiled reference's divsufsort64 SA, narrowed to 32 bits (texts
+    of up to 2^32 - 1 symbols); returns (SA, seconds of the divsufsort64 call)."""
+    text, p = _u8(text)
+    sa = np.empty(text.size, np.uint32)
+    secs = (ctypes.c_double * 1)()
+    R = ref()
+    if not hasattr(R, "_narrow_set"):
+        R.sfk_ref_divsufsort64_narrow.argtypes = [U8P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
+                                                  ctypes.POINTER(ctypes.c_double)]
+        R.sfk_ref_divsufsort64_narrow.restype = ctypes.c_int
+        R._narrow_set = True
+    if R.sfk_ref_divsufsort64_narrow(p, text.size, sa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), secs) != 0:
+        raise RuntimeError("reference divsufsort64 failed")
+    return sa, float(secs[0])
+
+
 def fast_build(text: np.ndarray, method: str = "auto"):
     """SA, ISA, LCP (uint64) of the Fast profile.
 
@@ -297,6 +344,22 @@ def fast_build(text: np.ndarray, method: str = "auto"):
     sa = sa_naive(text) if method == "naive" else sa_doubling(text)
     isa_ = isa(sa)
     return sa, isa_, lcp_kasai(text, sa, isa_)
+
+
+def ref_build_sampled(text: np.ndarray, sampling_rate: int, width_bits: int = 32):
+    """The compiled reference's BuildDivsufsort32/64 with sampling rate K:
+    (sampled SA, ISA over positions / K, LCP) as uint64."""
+    text, p = _u8(text)
+    n = text.size
+    rows = (n + sampling_rate - 1) // sampling_rate
+    sa = np.empty(n, np.uint64)
+    isa_ = np.empty(n, np.uint64)
+    lcp = np.empty(n, np.uint64)
+    secs = (ctypes.c_double * 3)()
+    got = ref().sfk_ref_build_fast(p, n, width_bits, sampling_rate, 1, _u64(sa), _u64(isa_), _u64(lcp), secs)
+    if got != rows:
+        raise RuntimeError("reference sampled build failed")
+    return sa[:rows].copy(), isa_[:rows].copy(), lcp[:rows].copy()
 
 
 def ref_writer_available() -> bool:

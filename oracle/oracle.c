@@ -22,6 +22,9 @@
  *                              LCP[0] = 0)
  *   sfk_oracle_crc32           zlib crc32 (IEEE 802.3, reflected 0xEDB88320)
  *                              as used by serialization.cpp:60-141
+ *   sfk_oracle_lean_digests    ISA scatter as above and the LCP through
+ *                              Phi/PLCP (same values as the Kasai above) for
+ *                              texts beyond the 64-bit-array entry points
  *   sfk_oracle_occurrences,    exact pattern occurrences and the first SA
  *   sfk_oracle_suffixes_below  row of a pattern by definition (the FM-index
  *                              queries, fm_index.cpp:298-309,547-677)
@@ -32,6 +35,8 @@
  *     run-heavy texts, and the compiled reference (oracle/_ref) where built;
  *   - ISA: closed form (inverse permutation, checked both ways);
  *   - Kasai: brute-force adjacent compare on small texts + fixture LCPs;
+ *   - lean digests: the digests of the compiled reference's full SA/ISA/LCP
+ *     and of the Kasai entry point on the same texts;
  *   - CRC32: the standard check value crc32("123456789") = 0xCBF43926 and
  *     the header/section CRCs stored in the legacy fixtures;
  *   - FNV-1a-64: the fixture fingerprint 0x0e8d89af19be03e1;
@@ -247,6 +252,77 @@ uint64_t sfk_oracle_digest_u64(const uint64_t* values, uint64_t count) {
     h = (h ^ z) * 0x100000001B3ULL + (h >> 29);
   }
   return h;
+}
+
+/* Position-keyed sum digest of an integer array:
+ *   count * G + sum_i fmix(values[i] + G * (i + 1))   (mod 2^64)
+ * with G the golden-ratio constant and fmix the splitmix64 finaliser.  The
+ * terms commute, so a device reduction computes the same number in any
+ * order; a changed value or two swapped values change it.  Used for the
+ * full-size golden digests (SA/ISA/LCP of whole workloads). */
+static uint64_t fmix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t sfk_oracle_sum_digest_u64(const uint64_t* values, uint64_t count) {
+  uint64_t h = count * 0x9E3779B97F4A7C15ULL;
+  for (uint64_t i = 0; i < count; ++i) h += fmix64(values[i] + 0x9E3779B97F4A7C15ULL * (i + 1));
+  return h;
+}
+
+uint64_t sfk_oracle_sum_digest_u32(const uint32_t* values, uint64_t count) {
+  uint64_t h = count * 0x9E3779B97F4A7C15ULL;
+  for (uint64_t i = 0; i < count; ++i) h += fmix64((uint64_t)values[i] + 0x9E3779B97F4A7C15ULL * (i + 1));
+  return h;
+}
+
+/* Sum digests of the ISA and LCP of a complete 32-bit SA, for texts too
+ * large for the 64-bit arrays of the other entry points (n <= 2^32 - 1):
+ * ISA by the scatter isa[SA[r]] = r (divsufsort_backend.cpp:81-119) and the
+ * LCP through Phi/PLCP (Karkkainen, Manzini and Puglisi, CPM 2009): Phi[SA[r]]
+ * = SA[r-1], PLCP[i] = lcp(i, Phi[i]) with PLCP[i+1] >= PLCP[i] - 1, byte
+ * equality as LongestCommonPrefixBytes compares (sequence_compare.hpp:
+ * 153-157), LCP[r] = PLCP[SA[r]], LCP[0] = 0.  The same LCP as the
+ * generalised Kasai of divsufsort_backend.cpp:121-150 at K = 1.  Holds two
+ * 4n-byte arrays.  out = {sa, isa, lcp} digests, out[3] = max LCP.
+ * Returns 0, or -1 when SA is not a permutation or memory runs out. */
+int sfk_oracle_lean_digests(const uint8_t* text, uint64_t n, const uint32_t* sa, uint64_t out[4]) {
+  if (n == 0 || n > 0xFFFFFFFFull) return -1;
+  uint32_t* isa = (uint32_t*)malloc(n * sizeof(uint32_t));
+  uint32_t* plcp = (uint32_t*)malloc(n * sizeof(uint32_t));
+  if (!isa || !plcp) { free(isa); free(plcp); return -1; }
+  const uint32_t none = 0xFFFFFFFFu;
+  memset(isa, 0xFF, n * sizeof(uint32_t));
+  for (uint64_t r = 0; r < n; ++r) {
+    if (sa[r] >= n || isa[sa[r]] != none) { free(isa); free(plcp); return -1; }
+    isa[sa[r]] = (uint32_t)r;
+  }
+  out[0] = sfk_oracle_sum_digest_u32(sa, n);
+  out[1] = sfk_oracle_sum_digest_u32(isa, n);
+  free(isa);
+  /* plcp first holds Phi, then is overwritten in text order. */
+  plcp[sa[0]] = none;
+  for (uint64_t r = 1; r < n; ++r) plcp[sa[r]] = sa[r - 1];
+  uint64_t l = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t j = plcp[i];
+    if (j == none) { plcp[i] = 0; l = 0; continue; }
+    while (i + l < n && j + l < n && text[i + l] == text[j + l]) ++l;
+    plcp[i] = (uint32_t)l;
+    if (l > 0) --l;
+  }
+  uint64_t h = n * 0x9E3779B97F4A7C15ULL, lmax = 0;
+  for (uint64_t r = 0; r < n; ++r) {
+    const uint64_t v = r == 0 ? 0 : plcp[sa[r]];
+    if (v > lmax) lmax = v;
+    h += fmix64(v + 0x9E3779B97F4A7C15ULL * (r + 1));
+  }
+  out[2] = h;
+  out[3] = lmax;
+  free(plcp);
+  return 0;
 }
 
 /* Fast prefix directory of a complete SA, restated from

@@ -18,6 +18,8 @@
 #include <exception>
 #include <vector>
 
+#include <sys/mman.h>
+
 #include "divsufsort_backend.hpp"
 
 extern "C" {
@@ -88,6 +90,30 @@ int sfk_ref_divsufsort(const std::uint8_t* text, std::uint64_t n,
   if (divsufsort64(text, sa.data(), static_cast<saidx64_t>(n)) != 0) return -1;
   for (std::uint64_t r = 0; r < n; ++r) sa_out[r] = static_cast<std::uint64_t>(sa[r]);
   return 0;
+}
+
+// The reference's divsufsort64 (the SA its 64-bit construction computes,
+// divsufsort_backend.cpp:46-60) for texts of up to 2^32 - 1 symbols whose
+// full 64-bit ISA/LCP would not fit this host's memory: the 8n-byte SA is
+// built in an anonymous mapping, narrowed into the caller's 32-bit array and
+// unmapped.  Returns 0, or -1 on a divsufsort error or failed mapping.
+int sfk_ref_divsufsort64_narrow(const std::uint8_t* text, std::uint64_t n,
+                                std::uint32_t* sa_out, double* seconds) {
+  if (n == 0 || n > 0xFFFFFFFFull) return -1;
+  const std::size_t bytes = n * sizeof(saidx64_t);
+  void* mem = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+  if (mem == MAP_FAILED) return -1;
+  auto* sa = static_cast<saidx64_t*>(mem);
+  const auto t0 = std::chrono::steady_clock::now();
+  const int status = divsufsort64(text, sa, static_cast<saidx64_t>(n));
+  if (seconds != nullptr) {
+    seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (status == 0) {
+    for (std::uint64_t r = 0; r < n; ++r) sa_out[r] = static_cast<std::uint32_t>(sa[r]);
+  }
+  munmap(mem, bytes);
+  return status == 0 ? 0 : -1;
 }
 
 }  // extern "C"

@@ -62,6 +62,7 @@ class HipStats(ctypes.Structure):
         ("sort_scatter_launches", ctypes.c_uint64),
         ("sort_scatter_bytes", ctypes.c_uint64),
         ("prefix_directory_seconds", ctypes.c_double),
+        ("call_seconds", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -99,7 +100,7 @@ def load_hip() -> ctypes.CDLL:
     lib.sfk_hip_workspace_bytes.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
     lib.sfk_hip_workspace_bytes.restype = ctypes.c_uint64
     lib.sfk_hip_build_device.argtypes = [
-        vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
         vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
         ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_build_device.restype = ctypes.c_int
@@ -109,7 +110,7 @@ def load_hip() -> ctypes.CDLL:
         ctypes.c_uint32, ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_build_batch_device.restype = ctypes.c_int
     lib.sfk_hip_build.argtypes = [
-        u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        u8p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
         vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, ctypes.c_uint32,
         ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_build.restype = ctypes.c_int
@@ -131,10 +132,13 @@ def _dtype(bits: int):
 
 def hip_build(text: np.ndarray, construction_bits: int = 32, *, sa_bits: int | None = None,
               isa_bits: int | None = None, lcp_bits: int | None = None,
-              retain_isa: bool = True, retain_lcp: bool = True, profile: bool = False):
+              retain_isa: bool = True, retain_lcp: bool = True, profile: bool = False,
+              sampling_rate: int = 1):
     """Host-buffer build through sfk_hip_build.
 
     `text` is the logical text (encoded bytes 0..6 with the sentinel 0 last).
+    With sampling_rate K > 1 the arrays keep the rows of text positions that
+    are multiples of K (ceil(n / K) entries each).
     Returns (sa, isa or None, lcp or None, stats dict).
     """
     lib = load_hip()
@@ -144,16 +148,17 @@ def hip_build(text: np.ndarray, construction_bits: int = 32, *, sa_bits: int | N
     sa_bits = sa_bits or (32 if narrow else 64)
     isa_bits = isa_bits or (32 if narrow else 64)
     lcp_bits = lcp_bits or (32 if narrow else 64)
-    sa = np.empty(n, dtype=_dtype(sa_bits))
-    isa = np.empty(n, dtype=_dtype(isa_bits)) if retain_isa else None
-    lcp = np.empty(n, dtype=_dtype(lcp_bits)) if retain_lcp else None
+    rows = (n + sampling_rate - 1) // sampling_rate if sampling_rate > 0 else n
+    sa = np.empty(rows, dtype=_dtype(sa_bits))
+    isa = np.empty(rows, dtype=_dtype(isa_bits)) if retain_isa else None
+    lcp = np.empty(rows, dtype=_dtype(lcp_bits)) if retain_lcp else None
     flags = (SFK_HIP_RETAIN_ISA if retain_isa else 0) | (SFK_HIP_RETAIN_LCP if retain_lcp else 0)
     if profile:
         flags |= SFK_HIP_PROFILE_KERNELS
     stats = HipStats()
     err = ctypes.create_string_buffer(512)
     status = lib.sfk_hip_build(
-        text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits, flags,
+        text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits, sampling_rate, flags,
         sa.ctypes.data, sa_bits,
         isa.ctypes.data if isa is not None else None, isa_bits,
         lcp.ctypes.data if lcp is not None else None, lcp_bits,
@@ -181,11 +186,11 @@ class DeviceBuilder:
 
     def build(self, d_text: int, n: int, construction_bits: int, d_sa: int, sa_bits: int,
               d_isa: int | None, isa_bits: int, d_lcp: int | None, lcp_bits: int,
-              flags: int = SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP) -> dict:
+              flags: int = SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, sampling_rate: int = 1) -> dict:
         stats = HipStats()
         err = ctypes.create_string_buffer(512)
         status = self.lib.sfk_hip_build_device(
-            self.handle, d_text, n, construction_bits, flags, d_sa, sa_bits,
+            self.handle, d_text, n, construction_bits, sampling_rate, flags, d_sa, sa_bits,
             d_isa, isa_bits, d_lcp, lcp_bits, ctypes.byref(stats), err, len(err))
         if status != 0:
             raise SufkitError(status, err.value.decode(errors="replace"))
@@ -303,6 +308,61 @@ class DeviceArray:
             pass
 
 
+class CheckReport(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "rows", "range_failures", "inverse_failures", "order_failures", "lcp_failures",
+        "lcp_prefix_failures", "first_failed_row")] + [
+        ("lcp_checked", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("seconds", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+def _check_lib():
+    lib = load_hip()
+    if not hasattr(lib, "_check_bound"):
+        vp = ctypes.c_void_p
+        lib.sfk_hip_digest.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
+        lib.sfk_hip_digest.restype = ctypes.c_int
+        lib.sfk_hip_check_index.argtypes = [ctypes.c_int, vp, ctypes.c_uint64, vp, ctypes.c_uint32, vp,
+                                            ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.c_uint64,
+                                            ctypes.POINTER(CheckReport), ctypes.c_char_p, ctypes.c_size_t]
+        lib.sfk_hip_check_index.restype = ctypes.c_int
+        lib._check_bound = True
+    return lib
+
+
+def device_digest(ptr: int, count: int, value_bits: int, device: int = 0) -> int:
+    """sfk_hip_digest of `count` device values (the oracle's sum digest)."""
+    lib = _check_lib()
+    out = ctypes.c_uint64()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_digest(device, ptr, count, value_bits, ctypes.byref(out), err, len(err))
+    if status != 0:
+        raise SufkitError(status, err.value.decode(errors="replace"))
+    return int(out.value)
+
+
+def check_index(d_text: int, n: int, d_sa: int, sa_bits: int, d_isa: int, isa_bits: int,
+                d_lcp: int | None = None, lcp_bits: int = 32, seed: int = 20260822,
+                device: int = 0, raise_on_failure: bool = True) -> dict:
+    """sfk_hip_check_index: order, permutation/inverse and LCP of a device
+    index.  Returns the report; raises SufkitError on a failed check unless
+    raise_on_failure is False (the report then carries status and message)."""
+    lib = _check_lib()
+    rep = CheckReport()
+    err = ctypes.create_string_buffer(512)
+    status = lib.sfk_hip_check_index(device, d_text, n, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits,
+                                     seed, ctypes.byref(rep), err, len(err))
+    out = rep.as_dict()
+    out["status"] = status
+    out["message"] = err.value.decode(errors="replace")
+    if status != 0 and raise_on_failure:
+        raise SufkitError(status, out["message"])
+    return out
+
+
 def hip_build_batch(texts: list[np.ndarray], construction_bits: int = 32, out_bits: int = 32,
                     builder: "DeviceBuilder | None" = None):
     """Batched build of several logical texts; returns [(sa, isa, lcp)] in
@@ -406,7 +466,7 @@ def _prefix_lib():
                                              ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_prefix_directory.restype = ctypes.c_int
     lib.sfk_hip_build_fast.argtypes = [
-        ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+        ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
         ctypes.c_uint32, u64p, u64p, ctypes.POINTER(HipStats), ctypes.c_char_p, ctypes.c_size_t]
     lib.sfk_hip_build_fast.restype = ctypes.c_int
@@ -466,7 +526,7 @@ def hip_build_fast(text: np.ndarray, prefix_k: int, construction_bits: int = 32)
     counts = (ctypes.c_uint64 * 2)()
     stats = HipStats()
     err = ctypes.create_string_buffer(512)
-    status = lib.sfk_hip_build_fast(text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits,
+    status = lib.sfk_hip_build_fast(text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n, construction_bits, 1,
                                     SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, sa.ctypes.data, 32, isa.ctypes.data,
                                     32, lcp.ctypes.data, 32, prefix_k,
                                     table.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), counts,

@@ -31,7 +31,7 @@ class _BuildStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in (
         "sa_seconds", "storage_compaction_seconds", "isa_seconds", "lcp_seconds",
         "text_prepare_seconds", "lcp_finalize_seconds", "total_seconds", "device_seconds",
-        "transfer_seconds", "prefix_directory_seconds")]
+        "transfer_seconds", "prefix_directory_seconds", "call_seconds")]
 
 
 class _IndexInfo(ctypes.Structure):

@@ -113,17 +113,34 @@ __device__ inline std::uint32_t WindowMismatch(std::uint64_t ca, std::uint32_t s
 __device__ inline std::uint32_t LaneId() { return __lane_id(); }
 
 // a != b for 64-bit values, decided by a 32-bit compare of the folded
-// difference.  On gfx950 a data-dependent 64-bit compare whose mask feeds
-// scalar exec/mask arithmetic directly was observed to select wrong lanes
-// (the prefix directory lost and misplaced entries; the same code with this
-// form was exact in every run), so per-lane predicates that gate stores use
-// this form.  The empty asm statement only keeps the optimiser from widening
-// the compare back to 64 bits; it emits no instruction.
+// difference.  Background: the prefix-directory kernel once wrote wrong
+// table entries at random on large inputs.  The change that removed the
+// fault rewrote that kernel in several ways at once (this compare form, a
+// compile-time table layout, unconditional stores, a halo computed by two
+// lanes) and also ordered the host's uploads before the device work, so
+// which of those removed it is not known; a miscompiled 64-bit compare is
+// one unconfirmed candidate.  Since the form costs one 32-bit OR, every
+// per-lane 64-bit compare that decides a store (group boundaries of the
+// refine steps, segment starts of the small-list step, the directory and
+// FM search) uses it, and the full-size order/LCP check of the tests
+// (sfk_hip_check_index) guards the result.  The empty asm statement only
+// keeps the optimiser from widening the compare back to 64 bits; it emits
+// no instruction.
 __device__ __forceinline__ bool Differ64(std::uint64_t a, std::uint64_t b) {
   const std::uint64_t x = a ^ b;
   std::uint32_t folded = static_cast<std::uint32_t>(x) | static_cast<std::uint32_t>(x >> 32);
   asm volatile("" : "+v"(folded));
   return folded != 0u;
+}
+
+// a != b for keys of either width (64-bit ones through Differ64).
+template <typename T>
+__device__ __forceinline__ bool Differ(T a, T b) {
+  if constexpr (sizeof(T) == 8) {
+    return Differ64(static_cast<std::uint64_t>(a), static_cast<std::uint64_t>(b));
+  } else {
+    return a != b;
+  }
 }
 
 __device__ inline std::uint64_t LaneMaskLt() {

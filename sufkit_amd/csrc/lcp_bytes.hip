@@ -144,8 +144,11 @@ int sfk_hip_byte_coded_lcp(int device, const void* d_lcp, uint32_t lcp_bits, con
       (lcp_bits != 32 && lcp_bits != 64) || (isa_bits != 32 && isa_bits != 64)) {
     return fail(SFK_HIP_INVALID_INPUT, "LCP and ISA (32 or 64 bits), n > 0 and a count word are required");
   }
+  if (n > SFK_HIP_MAX_TEXT_SYMBOLS) {
+    return fail(SFK_HIP_UNSUPPORTED_BACKEND, "texts of more than " + std::to_string(SFK_HIP_MAX_TEXT_SYMBOLS) +
+                                                 " symbols are not supported by the HIP backend");
+  }
   const std::uint64_t tiles = (n + sfk::kAnchorTile - 1) / sfk::kAnchorTile;
-  if (tiles > 0xFFFFFFFFull) return fail(SFK_HIP_INVALID_INPUT, "text too large");
   void* d_primary = nullptr;
   unsigned long long* d_counts = nullptr;
   std::uint64_t* d_anchors = nullptr;

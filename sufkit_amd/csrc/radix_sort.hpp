@@ -42,24 +42,17 @@ constexpr std::uint64_t kCountMask = (1ull << 48) - 1;
 constexpr std::uint32_t kEpochMask = (1u << 14) - 1;
 }  // namespace onesweep
 
-// Tile number of this workgroup in a look-back pass of epoch `epoch`.  By
-// ticket (by_ticket): taken from a counter when the workgroup starts, so
-// every tile with a lower number has started and every tile a tile waits on
-// is resident or finished.  By block index otherwise: the 8 XCDs take
-// workgroups round-robin and each dispatches its share only as it has room,
-// so a tile can wait on a predecessor its XCD has not dispatched yet.  That
-// only delays a pass while other kernels fill the XCD, but two such passes
-// in flight at once (two builds on two streams) can wait on each other, so
-// the host asks for tickets whenever more than one build is active.  The
-// pass of epoch e counts on tickets[e & 1]; its tile 0 clears the other
-// counter, which the next pass uses.  Call from every thread; by ticket, a
-// barrier follows.
-__device__ inline std::uint64_t LookBackTile(unsigned int* tickets, std::uint32_t epoch, std::uint32_t* s_tile,
-                                             bool by_ticket) {
-  if (!by_ticket) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) tickets[(epoch + 1u) & 1u] = 0u;
-    return blockIdx.x;
-  }
+// Tile number of this workgroup in a look-back pass of epoch `epoch`, taken
+// from a counter when the workgroup starts: every tile with a lower number
+// has started, so every tile a tile waits on is resident or finished,
+// whatever else runs on the device.  Numbering by block index is not safe
+// on MI355X: the 8 XCDs take workgroups round-robin and each dispatches its
+// share only as it has room, so a tile could wait on a predecessor its XCD
+// has not dispatched (two such passes in flight, from two builds on one GPU
+// in one process or in two, could wait on each other).  The pass of epoch e
+// counts on tickets[e & 1]; the tile with ticket 0 clears the other counter,
+// which the next pass uses.  Call from every thread; a barrier follows.
+__device__ inline std::uint64_t LookBackTile(unsigned int* tickets, std::uint32_t epoch, std::uint32_t* s_tile) {
   if (threadIdx.x == 0) {
     const unsigned int t = atomicAdd(&tickets[epoch & 1u], 1u);
     if (t == 0) tickets[(epoch + 1u) & 1u] = 0u;
@@ -426,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
     const unsigned long long* __restrict__ digit_count, unsigned long long* status,
     std::uint32_t epoch, unsigned int* error_flags,
-    std::uint32_t error_bit, unsigned int* tickets, bool by_ticket,
+    std::uint32_t error_bit, unsigned int* tickets,
     SegmentedPass seg = SegmentedPass{nullptr, 0, nullptr}) {
   constexpr int kRadix = RadixTile::kRadix;
   constexpr int kWaves = kBlock / kWave;
@@ -452,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __shared__ std::uint32_t s_tile;
   const int wave = threadIdx.x / kWave;
   const std::uint32_t lane = LaneId();
-  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile, by_ticket);
+  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile);
   const std::uint64_t base = tile * kTile;
   const std::uint64_t end = base + kTile < m ? base + kTile : m;
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wave_cnt[0][0])[i] = 0;

@@ -29,9 +29,11 @@
 #include <utility>
 #include <vector>
 
+#include "host_staging.hpp"
 #include "query_kernels.hpp"
 #include "radix_sort.hpp"
 #include "sa_kernels.hpp"
+#include "sample_kernels.hpp"
 #include "fm_kernels.hpp"  // after sa_kernels.hpp: uses its error flags
 #include "sufkit_hip.h"
 
@@ -135,19 +137,6 @@ namespace {
 
 constexpr int kScanItems = 8;
 
-// Builds (and tuning runs) in progress in this process.  While more than
-// one is, look-back passes number their tiles by ticket (LookBackTile).
-std::atomic<int> g_active_builds{0};
-
-struct ActiveBuild {
-  ActiveBuild() { g_active_builds.fetch_add(1); }
-  ~ActiveBuild() { g_active_builds.fetch_sub(1); }
-  ActiveBuild(const ActiveBuild&) = delete;
-  ActiveBuild& operator=(const ActiveBuild&) = delete;
-};
-
-bool TilesByTicket() { return g_active_builds.load() > 1; }
-
 struct BuildContext {
   sfk_hip_builder* b;
   std::uint64_t n;
@@ -156,6 +145,8 @@ struct BuildContext {
   // Batch build: reference r owns [offsets[r], offsets[r+1]) (device copy).
   const std::uint64_t* d_offsets = nullptr;
   std::uint32_t refs = 0;
+  // Text-position sampling rate K of the outputs (1: complete arrays).
+  std::uint32_t sampling = 1;
 };
 
 void ResetCounters(sfk_hip_builder* b) {
@@ -364,13 +355,13 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
     if (p == 0) {
       RadixOnesweep<K, V, kBlock, kItems, Source, kSeg><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
           first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status, epoch,
-          error_flags, kFlagLookback, b->tickets.As<unsigned int>(), TilesByTicket(), seg);
+          error_flags, kFlagLookback, b->tickets.As<unsigned int>(), seg);
     } else {
       RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
               set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback,
-              b->tickets.As<unsigned int>(), TilesByTicket(), seg);
+              b->tickets.As<unsigned int>(), seg);
     }
     EndProfiled(ctx, ev);
     SFK_HIP_CHECK(hipGetLastError());
@@ -913,7 +904,7 @@ class Doubling {
       RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
           b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
-          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), TilesByTicket(), dc,
+          b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), dc,
           segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u);
     } else {
       const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
@@ -1023,6 +1014,41 @@ void ConvertTo(sfk_hip_builder* b, const In* in, std::uint64_t n, void* out, std
   }
 }
 
+// Rows a sampled index keeps: the text positions that are multiples of K.
+std::uint64_t SampledRows(std::uint64_t n, std::uint32_t k) { return (n + k - 1) / k; }
+
+// Sampled SA, ISA and LCP (K > 1) from the complete SA and LCP: written in
+// the construction width to free workspace (vals1, act_idx, keys0), then
+// narrowed into the caller's buffers with the range check.
+template <typename Index>
+void SampleOutputs(BuildContext& ctx, const Index* sa, const Index* lcp, void* d_sa, std::uint32_t sa_bits,
+                   void* d_isa, std::uint32_t isa_bits, void* d_lcp, std::uint32_t lcp_bits) {
+  sfk_hip_builder* b = ctx.b;
+  const std::uint64_t n = ctx.n;
+  const std::uint64_t k = ctx.sampling;
+  const std::uint64_t rows = SampledRows(n, ctx.sampling);
+  Index* s_sa = b->vals1.As<Index>();
+  Index* s_isa = b->act_idx.As<Index>();
+  Index* s_lcp = lcp != nullptr ? b->keys0.As<Index>() : nullptr;
+  static_assert(sizeof(SampleSummary) * 1024 <= kRefineMaxChunks * sizeof(RefinePartial) ||
+                    sizeof(SampleSummary) * 1024 <= 1024 * sizeof(SplitPartial),
+                "the builder's partial buffer holds the sampling summaries");
+  const SortGeometry g = Geometry(n, kSampleTile, 1024);
+  SampleSummary* partial = b->partial.As<SampleSummary>();
+  SampleReduceKernel<Index><<<g.grid, kSampleThreads, 0, b->stream>>>(sa, lcp, n, k, g.chunk, partial);
+  SampleSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
+  SampleApplyKernel<Index><<<g.grid, kSampleThreads, 0, b->stream>>>(sa, lcp, n, k, g.chunk, partial, s_sa, s_isa,
+                                                                     s_lcp);
+  SFK_HIP_CHECK(hipGetLastError());
+  constexpr std::uint64_t w = sizeof(Index);
+  ctx.stats.algorithmic_bytes_isa += 2 * n * (w + (lcp != nullptr ? w : 0)) + rows * (2 * w + (lcp ? w : 0));
+  ConvertTo<Index>(b, s_sa, rows, d_sa, sa_bits);
+  if (d_isa != nullptr) ConvertTo<Index>(b, s_isa, rows, d_isa, isa_bits);
+  if (d_lcp != nullptr && s_lcp != nullptr) ConvertTo<Index>(b, s_lcp, rows, d_lcp, lcp_bits);
+  ctx.stats.algorithmic_bytes_isa += rows * (w + sa_bits / 8) + (d_isa ? rows * (w + isa_bits / 8) : 0) +
+                                     (d_lcp ? rows * (w + lcp_bits / 8) : 0);
+}
+
 template <typename Index>
 void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t flags,
                 void* d_sa, std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits,
@@ -1034,8 +1060,11 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
 
   // Workspace (grow-only).
   b->packed.Reserve(words * 16);
-  const bool sa_direct = sa_bits == kIndexBits;
-  const bool isa_direct = d_isa != nullptr && isa_bits == kIndexBits;
+  // Sampled outputs (K > 1) are shorter than the complete arrays the build
+  // works on, so those stay in the workspace.
+  const bool sampled = ctx.sampling > 1;
+  const bool sa_direct = !sampled && sa_bits == kIndexBits;
+  const bool isa_direct = !sampled && d_isa != nullptr && isa_bits == kIndexBits;
   if (!sa_direct) b->sa.Reserve(n * sizeof(Index));
   if (!isa_direct) b->rank.Reserve(n * sizeof(Index));
   b->act_idx.Reserve(n * sizeof(Index));
@@ -1087,7 +1116,11 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   }
   SFK_HIP_CHECK(hipEventRecord(b->phase[2], b->stream));
   if (want_lcp) {
-    if (lcp_bits == 32) {
+    if (sampled) {
+      // The complete LCP into the workspace (vals0 is free after the
+      // doubling), for the interval minima of the sampled rows.
+      LaunchLcp<Index, Index>(ctx, text, sa, rank, b->vals0.As<Index>());
+    } else if (lcp_bits == 32) {
       LaunchLcp<Index, std::uint32_t>(ctx, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
     } else {
       LaunchLcp<Index, std::uint64_t>(ctx, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
@@ -1121,6 +1154,9 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     }
     SFK_HIP_CHECK(hipGetLastError());
     ctx.stats.algorithmic_bytes_isa += n * (2 * sizeof(Index) + sa_bits / 8 + isa_bits / 8);
+  } else if (sampled) {
+    SampleOutputs<Index>(ctx, sa, want_lcp ? b->vals0.As<Index>() : nullptr, d_sa, sa_bits, d_isa, isa_bits,
+                         want_lcp ? d_lcp : nullptr, lcp_bits);
   } else {
     if (d_isa != nullptr && !isa_direct) {
       ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
@@ -1171,22 +1207,28 @@ void CopyError(const std::string& message, char* error, size_t capacity) {
 
 int ValidateArgs(std::uint64_t n, std::uint32_t construction_bits, std::uint32_t sa_bits,
                  const void* isa, std::uint32_t isa_bits, const void* lcp,
-                 std::uint32_t lcp_bits, std::uint32_t flags, std::string& message) {
+                 std::uint32_t lcp_bits, std::uint32_t flags, std::string& message,
+                 std::uint32_t sampling = 1) {
   if (construction_bits != 32 && construction_bits != 64) {
     message = "construction width must be 32 or 64 bits";
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (sampling == 0) {
+    message = "sampling rate must be greater than zero";
     return SFK_HIP_INVALID_INPUT;
   }
   if (n < 1) {
     message = "reference text is empty";
     return SFK_HIP_INVALID_INPUT;
   }
-  if (n >= (1ull << 33)) {
-    message = "texts of 2^33 symbols or more are not supported by the HIP backend";
-    return SFK_HIP_UNSUPPORTED_BACKEND;
-  }
   if (construction_bits == 32 && n > 0xFFFFFFFFull) {
     message = "reference is too large for the 32-bit HIP backend";
     return SFK_HIP_INVALID_INPUT;
+  }
+  if (n > SFK_HIP_MAX_TEXT_SYMBOLS) {
+    message = "texts of more than " + std::to_string(SFK_HIP_MAX_TEXT_SYMBOLS) +
+              " symbols are not supported by the HIP backend";
+    return SFK_HIP_UNSUPPORTED_BACKEND;
   }
   if (sa_bits != 32 && sa_bits != 64) {
     message = "SA output width must be 32 or 64 bits";
@@ -1214,9 +1256,10 @@ int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t
                    std::uint32_t construction_bits, std::uint32_t flags, void* d_sa,
                    std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits, void* d_lcp,
                    std::uint32_t lcp_bits, sfk_hip_stats* stats, std::string& message,
-                   const std::uint64_t* offsets = nullptr, std::uint32_t refs = 0) {
-  const ActiveBuild active;
+                   const std::uint64_t* offsets = nullptr, std::uint32_t refs = 0,
+                   std::uint32_t sampling = 1) {
   BuildContext ctx{b, n, (flags & SFK_HIP_PROFILE_KERNELS) != 0, sfk_hip_stats{}};
+  ctx.sampling = sampling;
   b->events_used = 0;
   SFK_HIP_CHECK(hipSetDevice(b->device));
   if (refs > 0) {
@@ -1320,9 +1363,9 @@ uint64_t sfk_hip_workspace_bytes(uint64_t n, uint32_t construction_bits) {
 }
 
 int sfk_hip_build_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,
-                         uint32_t construction_bits, uint32_t flags, void* d_sa,
-                         uint32_t sa_bits, void* d_isa, uint32_t isa_bits, void* d_lcp,
-                         uint32_t lcp_bits, sfk_hip_stats* stats, char* error,
+                         uint32_t construction_bits, uint32_t sampling_rate, uint32_t flags,
+                         void* d_sa, uint32_t sa_bits, void* d_isa, uint32_t isa_bits,
+                         void* d_lcp, uint32_t lcp_bits, sfk_hip_stats* stats, char* error,
                          size_t error_capacity) {
   std::string message;
   if (b == nullptr || d_text == nullptr || d_sa == nullptr) {
@@ -1330,14 +1373,14 @@ int sfk_hip_build_device(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n,
     return SFK_HIP_INVALID_INPUT;
   }
   int status = sfk::ValidateArgs(n, construction_bits, sa_bits, d_isa, isa_bits, d_lcp,
-                                 lcp_bits, flags, message);
+                                 lcp_bits, flags, message, sampling_rate);
   if (status != SFK_HIP_OK) {
     sfk::CopyError(message, error, error_capacity);
     return status;
   }
   try {
     return sfk::RunDeviceBuild(b, d_text, n, construction_bits, flags, d_sa, sa_bits, d_isa,
-                               isa_bits, d_lcp, lcp_bits, stats, message);
+                               isa_bits, d_lcp, lcp_bits, stats, message, nullptr, 0, sampling_rate);
   } catch (const sfk::Failure& f) {
     (void)hipStreamSynchronize(b->stream);
     sfk::CopyError(f.message, error, error_capacity);
@@ -1458,15 +1501,15 @@ void LaunchPrefixDirectory(hipStream_t stream, PackedText t, const void* d_sa, s
 }  // namespace
 }  // namespace sfk
 
-int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t flags,
-                  void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
+int sfk_hip_build(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t sampling_rate,
+                  uint32_t flags, void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
                   uint32_t lcp_bits, sfk_hip_stats* stats, char* error, size_t error_capacity) {
-  return sfk_hip_build_fast(text, n, construction_bits, flags, sa, sa_bits, isa, isa_bits, lcp, lcp_bits, 0,
-                            nullptr, nullptr, stats, error, error_capacity);
+  return sfk_hip_build_fast(text, n, construction_bits, sampling_rate, flags, sa, sa_bits, isa, isa_bits, lcp,
+                            lcp_bits, 0, nullptr, nullptr, stats, error, error_capacity);
 }
 
-int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t flags,
-                       void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
+int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bits, uint32_t sampling_rate,
+                       uint32_t flags, void* sa, uint32_t sa_bits, void* isa, uint32_t isa_bits, void* lcp,
                        uint32_t lcp_bits, uint32_t prefix_k, uint64_t* prefix_table, uint64_t prefix_counts[2],
                        sfk_hip_stats* stats, char* error, size_t error_capacity) {
   std::string message;
@@ -1474,39 +1517,41 @@ int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bi
     sfk::CopyError("text and SA buffers are required", error, error_capacity);
     return SFK_HIP_INVALID_INPUT;
   }
-  if (prefix_k > sfk::kDirectoryMaxK || (prefix_k > 0 && prefix_table == nullptr)) {
-    sfk::CopyError("prefix directory k must be 0..10 with a table for k > 0", error, error_capacity);
+  if (prefix_k > sfk::kDirectoryMaxK || (prefix_k > 0 && prefix_table == nullptr) ||
+      (prefix_k > 0 && sampling_rate != 1)) {
+    sfk::CopyError("prefix directory k must be 0..10 with a table for k > 0, of a complete (K = 1) index", error,
+                   error_capacity);
     return SFK_HIP_INVALID_INPUT;
   }
   int status = sfk::ValidateArgs(n, construction_bits, sa_bits, isa, isa_bits, lcp, lcp_bits,
-                                 flags, message);
+                                 flags, message, sampling_rate);
   if (status != SFK_HIP_OK) {
     sfk::CopyError(message, error, error_capacity);
     return status;
   }
+  const auto t_begin = std::chrono::steady_clock::now();
   sfk_hip_builder* b = nullptr;
   status = sfk_hip_builder_create(0, &b, error, error_capacity);
   if (status != SFK_HIP_OK) return status;
   sfk::DeviceBuffer d_text, d_sa, d_isa, d_lcp;
   const bool want_isa = isa != nullptr;
   const bool want_lcp = lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
+  const std::uint64_t rows = sfk::SampledRows(n, sampling_rate);
   try {
     d_text.Reserve(n + 64);
-    d_sa.Reserve(n * (sa_bits / 8));
-    if (want_isa) d_isa.Reserve(n * (isa_bits / 8));
-    if (want_lcp) d_lcp.Reserve(n * (lcp_bits / 8));
+    d_sa.Reserve(rows * (sa_bits / 8));
+    if (want_isa) d_isa.Reserve(rows * (isa_bits / 8));
+    if (want_lcp) d_lcp.Reserve(rows * (lcp_bits / 8));
+    // Copies go through pinned staging on the builder's stream; each one
+    // returns after its data has landed, so the times are the copies'.
     const auto t0 = std::chrono::steady_clock::now();
-    // Ordered on the builder's stream: a copy from pageable memory through
-    // hipMemcpy may return before its last chunk lands, and the builder's
-    // stream does not wait for the null stream.
-    SFK_HIP_CHECK(hipMemcpyAsync(d_text.As<void>(), text, n, hipMemcpyHostToDevice, b->stream));
-    const auto t1 = std::chrono::steady_clock::now();
+    SFK_HIP_CHECK(sfk::StagedCopy(b->device, d_text.As<void>(), text, n, hipMemcpyHostToDevice, b->stream));
     sfk_hip_stats local{};
-    local.h2d_seconds = std::chrono::duration<double>(t1 - t0).count();
+    local.h2d_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     status = sfk::RunDeviceBuild(b, d_text.As<uint8_t>(), n, construction_bits, flags,
                                  d_sa.As<void>(), sa_bits, want_isa ? d_isa.As<void>() : nullptr,
                                  isa_bits, want_lcp ? d_lcp.As<void>() : nullptr, lcp_bits,
-                                 &local, message);
+                                 &local, message, nullptr, 0, sampling_rate);
     sfk::DeviceBuffer d_table;
     if (status == SFK_HIP_OK && prefix_k > 0) {
       // The directory from the device SA while it is resident (the packed
@@ -1522,18 +1567,26 @@ int sfk_hip_build_fast(const uint8_t* text, uint64_t n, uint32_t construction_bi
       SFK_HIP_CHECK(hipMemcpyAsync(host, dcounts, sizeof(host), hipMemcpyDeviceToHost, b->stream));
       SFK_HIP_CHECK(hipStreamSynchronize(b->stream));
       local.prefix_directory_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
-      SFK_HIP_CHECK(hipMemcpy(prefix_table, d_table.As<void>(), words * 8, hipMemcpyDeviceToHost));
+      SFK_HIP_CHECK(sfk::StagedCopy(b->device, prefix_table, d_table.As<void>(), words * 8, hipMemcpyDeviceToHost,
+                                    b->stream));
       if (prefix_counts != nullptr) {
         prefix_counts[0] = host[0];
         prefix_counts[1] = host[1];
       }
     }
     const auto t2 = std::chrono::steady_clock::now();
-    SFK_HIP_CHECK(hipMemcpy(sa, d_sa.As<void>(), n * (sa_bits / 8), hipMemcpyDeviceToHost));
-    if (want_isa) SFK_HIP_CHECK(hipMemcpy(isa, d_isa.As<void>(), n * (isa_bits / 8), hipMemcpyDeviceToHost));
-    if (want_lcp) SFK_HIP_CHECK(hipMemcpy(lcp, d_lcp.As<void>(), n * (lcp_bits / 8), hipMemcpyDeviceToHost));
-    local.d2h_seconds =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
+    SFK_HIP_CHECK(sfk::StagedCopy(b->device, sa, d_sa.As<void>(), rows * (sa_bits / 8), hipMemcpyDeviceToHost,
+                                  b->stream));
+    if (want_isa) {
+      SFK_HIP_CHECK(sfk::StagedCopy(b->device, isa, d_isa.As<void>(), rows * (isa_bits / 8), hipMemcpyDeviceToHost,
+                                    b->stream));
+    }
+    if (want_lcp) {
+      SFK_HIP_CHECK(sfk::StagedCopy(b->device, lcp, d_lcp.As<void>(), rows * (lcp_bits / 8), hipMemcpyDeviceToHost,
+                                    b->stream));
+    }
+    local.d2h_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
+    local.call_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
     if (stats != nullptr) *stats = local;
   } catch (const sfk::Failure& f) {
     sfk::CopyError(f.message, error, error_capacity);
@@ -1559,7 +1612,7 @@ uint64_t sfk_prefix_directory_words(uint32_t k, uint64_t suffix_count) {
 int sfk_hip_prefix_directory(sfk_hip_builder* b, const uint8_t* d_text, uint64_t n, const void* d_sa,
                              uint32_t sa_bits, uint32_t k, uint64_t* d_table, uint64_t counts[2], char* error,
                              size_t error_capacity) {
-  if (b == nullptr || d_text == nullptr || d_sa == nullptr || d_table == nullptr || n < 2 || k < 1 ||
+  if (b == nullptr || d_text == nullptr || d_sa == nullptr || d_table == nullptr || n < 2 || n > SFK_HIP_MAX_TEXT_SYMBOLS || k < 1 ||
       k > sfk::kDirectoryMaxK || (sa_bits != 32 && sa_bits != 64)) {
     sfk::CopyError("prefix directory needs a builder, a text, its SA (32/64-bit), a table and 1 <= k <= 10",
                    error, error_capacity);
@@ -1623,7 +1676,7 @@ void sfk_hip_index_destroy(sfk_hip_index* index) {
 int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const void* d_sa,
                          uint32_t sa_bits, sfk_hip_index** out, char* error,
                          size_t error_capacity) {
-  if (out == nullptr || d_text == nullptr || d_sa == nullptr || n < 1 ||
+  if (out == nullptr || d_text == nullptr || d_sa == nullptr || n < 1 || n > SFK_HIP_MAX_TEXT_SYMBOLS ||
       (sa_bits != 32 && sa_bits != 64)) {
     sfk::CopyError("index needs a text, its SA and a 32/64-bit SA width", error, error_capacity);
     return SFK_HIP_INVALID_INPUT;
@@ -1891,17 +1944,23 @@ int sfk_hip_free(void* ptr) {
   return hipFree(ptr) == hipSuccess ? SFK_HIP_OK : SFK_HIP_BUILD_FAILURE;
 }
 
+// Synchronous copies between pageable host memory and the device, through
+// the device's pinned stager when it is free.  Both return after the data
+// has landed, so work on any stream launched after the call sees it.
 int sfk_hip_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
-  // A copy from pageable memory may return before its last chunk lands on
-  // the device; the null stream's synchronisation waits for it, so work on
-  // any stream launched after this call sees the data.
-  if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
-  return hipStreamSynchronize(nullptr) == hipSuccess ? SFK_HIP_OK : SFK_HIP_BUILD_FAILURE;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return sfk::StagedCopy(device, dst, src, bytes, hipMemcpyHostToDevice, nullptr) == hipSuccess
+             ? SFK_HIP_OK
+             : SFK_HIP_BUILD_FAILURE;
 }
 
 int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
-  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? SFK_HIP_OK
-                                                                         : SFK_HIP_BUILD_FAILURE;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return SFK_HIP_BUILD_FAILURE;
+  return sfk::StagedCopy(device, dst, src, bytes, hipMemcpyDeviceToHost, nullptr) == hipSuccess
+             ? SFK_HIP_OK
+             : SFK_HIP_BUILD_FAILURE;
 }
 
 }  // extern "C"
@@ -1950,7 +2009,6 @@ int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entr
     b->radix_set = 0;
     sfk::ReserveOnesweep(b, top);
     sfk::ResetCounters(b);
-    const sfk::ActiveBuild active;  // tuning passes overlap builds on other streams like a build
     SFK_HIP_CHECK(hipEventCreate(&e0));
     SFK_HIP_CHECK(hipEventCreate(&e1));
     std::vector<sfk::TuneResult> results;

@@ -686,8 +686,8 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
       const K mine = lds.key[i + 1];
       const K before = lds.key[i];
       const K after = lds.key[i + 2];
-      bool starts = (k == 0) | (before != mine);
-      bool ends = (k + 1 == m) | (after != mine);
+      bool starts = (k == 0) | Differ(before, mine);
+      bool ends = (k + 1 == m) | Differ(after, mine);
       if (ref_offsets) {
         starts |= (lds.bound[i >> 5] >> (i & 31)) & 1u;
         ends |= (lds.bound[(i + 1) >> 5] >> ((i + 1) & 31)) & 1u;
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     const Index* __restrict__ seg_delta, std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank,
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
     Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
-    unsigned int* tickets, bool by_ticket, DeviceCounters* counters,
+    unsigned int* tickets, DeviceCounters* counters,
     const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
   constexpr int kBlock = kRefineThreads;
   constexpr int kTile = kRefineTile;
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
   __shared__ std::uint64_t scratch[kBlock / kWave];
   __shared__ std::uint64_t s_prefix[3];
   __shared__ std::uint32_t s_tile;
-  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile, by_ticket);
+  const std::uint64_t tile = LookBackTile(tickets, epoch, &s_tile);
   const std::uint64_t base = tile * kTile;
   const int len = static_cast<int>(m - base < static_cast<std::uint64_t>(kTile) ? m - base : kTile);
   Index v[kRefineItems];
@@ -837,8 +837,8 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineReduceKernel(const 
     for (int q = 0; q < kVec; ++q) {
       const std::uint64_t k = k0 + q;
       const bool in = k < end;
-      const bool starts = (k == 0) | (w[q] != w[q + 1]);
-      const bool ends = (k + 1 == m) | (w[q + 2] != w[q + 1]);
+      const bool starts = (k == 0) | Differ(w[q], w[q + 1]);
+      const bool ends = (k + 1 == m) | Differ(w[q + 2], w[q + 1]);
       const bool single = starts & ends;
       surv += in & !single;
       segs += in & starts & !single;
@@ -1421,7 +1421,7 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     const Index here = s_row0[k + 1];
     const Index before = s_row0[k];
     const bool first_slot = (k == 0) & (s0 == 0);
-    const bool starts = (k < valid_len) & (first_slot | (here != before));
+    const bool starts = (k < valid_len) & (first_slot | Differ(here, before));
     run = starts ? static_cast<std::uint64_t>(k) + 1 : run;
     mark[i] = run;
   }

@@ -56,7 +56,8 @@ struct sfk_build_options {
 
 struct sfk_build_stats {
   double sa_seconds, storage_compaction_seconds, isa_seconds, lcp_seconds, text_prepare_seconds,
-      lcp_finalize_seconds, total_seconds, device_seconds, transfer_seconds, prefix_directory_seconds;
+      lcp_finalize_seconds, total_seconds, device_seconds, transfer_seconds, prefix_directory_seconds,
+      call_seconds;
 };
 
 struct sfk_index_info {
@@ -142,7 +143,7 @@ int sfk_sa_build(const sfk_reference* r, const sfk_build_options* o, sfk_suffix_
       *stats = sfk_build_stats{s.sa_seconds, s.storage_compaction_seconds, s.isa_seconds,
                                s.lcp_seconds, s.text_prepare_seconds, s.lcp_finalize_seconds,
                                s.total_seconds, s.device_seconds, s.transfer_seconds,
-                               s.prefix_directory_seconds};
+                               s.prefix_directory_seconds, s.call_seconds};
     }
   });
 }

@@ -251,7 +251,7 @@ FmIndex FmIndex::Build(const GenomeReference& reference, const FmIndexBuildOptio
   CheckHip(sfk_hip_builder_create(0, &builder, error, sizeof(error)), error);
   sfk_hip_stats stats{};
   const int status = sfk_hip_build_device(builder, static_cast<const std::uint8_t*>(impl->text.p), n,
-                                          impl->sa_bits, 0, impl->sa.p, impl->sa_bits, nullptr, 0, nullptr, 0,
+                                          impl->sa_bits, 1, 0, impl->sa.p, impl->sa_bits, nullptr, 0, nullptr, 0,
                                           &stats, error, sizeof(error));
   sfk_hip_builder_destroy(builder);
   CheckHip(status, error);

@@ -221,7 +221,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
       }
       if (status == SFK_HIP_OK) {
         sfk_hip_stats stats{};
-        status = sfk_hip_build_device(builder, static_cast<const uint8_t*>(rs.text), n, bits,
+        status = sfk_hip_build_device(builder, static_cast<const uint8_t*>(rs.text), n, bits, 1,
                                       SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, rs.sa, bits, rs.isa,
                                       bits, rs.lcp, bits, &stats, err, sizeof(err));
         res.device_build_seconds = stats.device_seconds;

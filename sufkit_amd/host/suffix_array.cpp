@@ -134,39 +134,24 @@ detail::CoordinateStorage Store(std::vector<T>&& values, CoordinateStorageWidth 
   }
 }
 
-// Sampled SA (K > 1) from the complete SA and LCP: keep rows whose text
-// position is a multiple of K; the LCP of adjacent kept rows is the minimum
-// complete-SA LCP over the interval between them (suffix_array.cpp:933-966).
-template <class S, class L>
-void SampleSaLcp(std::vector<S>& sa, std::vector<L>* lcp, std::uint32_t k) {
-  std::size_t out = 0;
-  std::uint64_t interval_min = std::numeric_limits<std::uint64_t>::max();
-  for (std::size_t row = 0; row < sa.size(); ++row) {
-    if (lcp != nullptr && row != 0) interval_min = std::min<std::uint64_t>(interval_min, (*lcp)[row]);
-    if (static_cast<std::uint64_t>(sa[row]) % k != 0) continue;
-    sa[out] = sa[row];
-    if (lcp != nullptr) (*lcp)[out] = static_cast<L>(out == 0 ? 0 : interval_min);
-    ++out;
-    interval_min = std::numeric_limits<std::uint64_t>::max();
-  }
-  sa.resize(out);
-  if (lcp != nullptr) lcp->resize(out);
-}
-
 template <class SaT, class IsaT, class LcpT>
 void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& options,
                   CoordinateStorageWidth stored_width, bool retain_isa, bool retain_lcp, std::uint32_t prefix_k,
                   SuffixArrayBuildStatistics& stats, const std::function<void(const char*)>& notify) {
   const std::uint64_t n = impl.text.size();
-  std::vector<SaT> sa(n);
-  std::vector<IsaT> isa(retain_isa || options.sampling_rate > 1 ? n : 0);
-  std::vector<LcpT> lcp(retain_lcp ? n : 0);
+  // K > 1: the device keeps the rows of positions that are multiples of K
+  // (suffix_array.cpp:912-974), with their ISA and interval-minimum LCP.
+  const std::uint32_t k = options.sampling_rate;
+  const std::uint64_t rows = 1 + (n - 1) / k;
+  std::vector<SaT> sa(rows);
+  std::vector<IsaT> isa(retain_isa ? rows : 0);
+  std::vector<LcpT> lcp(retain_lcp ? rows : 0);
   std::vector<std::uint64_t> table(prefix_k > 0 ? sfk_prefix_directory_words(prefix_k, n) : 0);
   sfk_hip_stats hs{};
   char err[512] = {0};
   const std::uint32_t flags = (retain_isa ? SFK_HIP_RETAIN_ISA : 0u) | (retain_lcp ? SFK_HIP_RETAIN_LCP : 0u);
   std::uint64_t prefix_counts[2] = {0, 0};
-  const int status = sfk_hip_build_fast(impl.text.data(), n, impl.construction_width, flags, sa.data(),
+  const int status = sfk_hip_build_fast(impl.text.data(), n, impl.construction_width, k, flags, sa.data(),
                                         8 * sizeof(SaT), isa.empty() ? nullptr : isa.data(), 8 * sizeof(IsaT),
                                         lcp.empty() ? nullptr : lcp.data(), 8 * sizeof(LcpT), prefix_k,
                                         table.empty() ? nullptr : table.data(), prefix_counts, &hs, err,
@@ -181,16 +166,9 @@ void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& option
   stats.device_seconds = hs.device_seconds;
   stats.transfer_seconds = hs.h2d_seconds + hs.d2h_seconds;
 
-  const std::uint32_t k = options.sampling_rate;
-  if (k > 1) {
-    SampleSaLcp(sa, retain_lcp ? &lcp : nullptr, k);
-    isa.assign(sa.size(), 0);
-    for (std::size_t r = 0; r < sa.size(); ++r) isa[static_cast<std::size_t>(sa[r] / k)] = static_cast<IsaT>(r);
-    if (!retain_isa) std::vector<IsaT>().swap(isa);
-  }
+  stats.call_seconds = hs.call_seconds;
   const auto compaction = Clock::now();
   notify("index-storage-layout");
-  const std::uint64_t rows = sa.size();
   impl.sa = Store(std::move(sa), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
   stats.storage_compaction_seconds = Since(compaction);
   if (retain_isa) {
@@ -246,7 +224,7 @@ void BuildLowMemoryWithHip(SuffixArray::Impl& impl, CoordinateStorageWidth store
   sfk_hip_builder* builder = nullptr;
   check(sfk_hip_builder_create(0, &builder, err, sizeof(err)), "builder");
   sfk_hip_stats hs{};
-  const int built = sfk_hip_build_device(builder, static_cast<const std::uint8_t*>(text.p), n, bits,
+  const int built = sfk_hip_build_device(builder, static_cast<const std::uint8_t*>(text.p), n, bits, 1,
                                          SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, sa.p, sa_bits, isa.p, wide, lcp.p,
                                          wide, &hs, err, sizeof(err));
   sfk_hip_builder_destroy(builder);

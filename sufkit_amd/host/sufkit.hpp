@@ -86,6 +86,7 @@ struct SuffixArrayBuildStatistics {
   // Device-side detail of the HIP backend.
   double device_seconds = 0.0;
   double transfer_seconds = 0.0;
+  double call_seconds = 0.0;  // the backend call: allocation, copies, build
 };
 
 struct SuffixArrayBuildOptions {

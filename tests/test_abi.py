@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_device_count():
     lib = sufkit_amd.load_hip()
-    assert lib.sfk_hip_abi_version() == 2
+    assert lib.sfk_hip_abi_version() == 3
     assert lib.sfk_hip_device_count() >= 0
 
 
@@ -42,10 +42,10 @@ def _status(**kw):
     text = np.array([2, 3, 0], np.uint8)
     sa = np.zeros(3, np.uint64)
     err = ctypes.create_string_buffer(256)
-    args = dict(n=3, bits=32, flags=3, sa_bits=64, isa=None, isa_bits=64, lcp=None, lcp_bits=64)
+    args = dict(n=3, bits=32, k=1, flags=3, sa_bits=64, isa=None, isa_bits=64, lcp=None, lcp_bits=64)
     args.update(kw)
     status = lib.sfk_hip_build(text.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), args["n"],
-                               args["bits"], args["flags"], sa.ctypes.data, args["sa_bits"],
+                               args["bits"], args["k"], args["flags"], sa.ctypes.data, args["sa_bits"],
                                args["isa"], args["isa_bits"], args["lcp"], args["lcp_bits"],
                                None, err, len(err))
     return status, err.value.decode()
@@ -55,6 +55,7 @@ def _status(**kw):
     (dict(bits=16), "construction width"),
     (dict(n=0), "empty"),
     (dict(sa_bits=40), "SA output width"),
+    (dict(k=0), "sampling rate"),
 ])
 def test_argument_validation_before_any_device_work(kw, needle):
     status, message = _status(**kw)
@@ -65,6 +66,15 @@ def test_argument_validation_before_any_device_work(kw, needle):
 def test_too_large_for_32bit_construction():
     status, message = _status(n=(1 << 32) + 5)
     assert status == 1 and "too large" in message
+
+
+@pytest.mark.parametrize("n", [(1 << 32) + 5, (1 << 32) - (1 << 16) + 1, 1 << 40])
+def test_longer_than_device_limit_is_unsupported(n):
+    # 64-bit construction beyond SFK_HIP_MAX_TEXT_SYMBOLS: rejected before
+    # any device work (one-element-per-work-item launches stay within the
+    # 32-bit dispatch grid).
+    status, message = _status(n=n, bits=64)
+    assert status == 3 and "4294901760" in message  # kUnsupportedBackend + 1
 
 
 def test_workspace_estimate_scales():
