@@ -126,6 +126,38 @@ int sfk_hip_build_device(sfk_hip_builder* builder, const uint8_t* d_text,
                          char* error, size_t error_capacity);
 
 /*
+ * Allocation-free variant of sfk_hip_build_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  It calls no hipMalloc/hipFree or other allocating routine:
+ * its workspace is a fixed array of SFK_HIP_NOALLOC_SCRATCH_BYTES in device
+ * memory that each GPU the library runs on holds from the first use of the
+ * library until the process ends.  Limits: texts of at most
+ * SFK_HIP_NOALLOC_MAX_SYMBOLS symbols, 32-bit construction, sampling rate
+ * 1, 32-bit SA/ISA/LCP, no SFK_HIP_PROFILE_KERNELS; anything else gives
+ * SFK_HIP_INVALID_INPUT.  The caller passes the device, its stream (a
+ * hipStream_t, NULL for the null stream) and its input and output buffers,
+ * as for sfk_hip_build_device; results are identical.  Every call on one
+ * GPU shares that GPU's array, so calls that overlap on one GPU (two host
+ * threads, or two streams) race and must be serialized by the caller.
+ * Untimed: stats get the counts and call_seconds (host wall time of the
+ * synchronous call), not the per-phase event times.
+ * sfk_hip_noalloc_workspace_bytes: bytes of the array a build of n symbols
+ * uses (with_isa: an ISA output is given).
+ */
+#define SFK_HIP_NOALLOC_MAX_SYMBOLS 1000000ull
+#define SFK_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sfk_hip_build_device_noalloc(int device, const uint8_t* d_text, uint64_t n,
+                                 uint32_t construction_bits,
+                                 uint32_t sampling_rate, uint32_t flags,
+                                 void* d_sa, uint32_t sa_bits, void* d_isa,
+                                 uint32_t isa_bits, void* d_lcp,
+                                 uint32_t lcp_bits, void* stream,
+                                 sfk_hip_stats* stats, char* error,
+                                 size_t error_capacity);
+uint64_t sfk_hip_noalloc_workspace_bytes(uint64_t n, int with_isa);
+
+/*
  * Batched device build of `refs` independent references laid out back to
  * back in one device text: reference r is the logical text (its encoded
  * bytes followed by its own sentinel 0) at [offsets[r], offsets[r+1]);

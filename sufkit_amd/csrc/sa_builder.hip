@@ -76,8 +76,13 @@ class DeviceBuffer {
   ~DeviceBuffer() { Release(); }
 
   // Returns true when the buffer was (re)allocated, i.e. its contents are new.
+  // A borrowed buffer (a slice of a caller's workspace) never reallocates.
   bool Reserve(std::uint64_t bytes) {
     if (bytes <= bytes_) return false;
+    if (borrowed_) {
+      Fail(SFK_HIP_BUILD_FAILURE, "workspace slice of " + std::to_string(bytes_) + " bytes is smaller than the " +
+                                      std::to_string(bytes) + " bytes the build needs");
+    }
     Release();
     void* ptr = nullptr;
     const hipError_t err = hipMalloc(&ptr, bytes);
@@ -91,9 +96,17 @@ class DeviceBuffer {
     return true;
   }
   void Release() {
-    if (ptr_ != nullptr) (void)hipFree(ptr_);
+    if (ptr_ != nullptr && !borrowed_) (void)hipFree(ptr_);
     ptr_ = nullptr;
     bytes_ = 0;
+    borrowed_ = false;
+  }
+  // Uses [ptr, ptr + bytes) of memory the buffer does not own.
+  void Borrow(void* ptr, std::uint64_t bytes) {
+    Release();
+    ptr_ = ptr;
+    bytes_ = bytes;
+    borrowed_ = true;
   }
   template <typename T>
   T* As() const {
@@ -104,6 +117,7 @@ class DeviceBuffer {
  private:
   void* ptr_ = nullptr;
   std::uint64_t bytes_ = 0;
+  bool borrowed_ = false;
 };
 
 struct EventPair {
@@ -129,7 +143,11 @@ struct sfk_hip_builder {
   sfk::DeviceCounters* host_counters = nullptr;  // pinned mirror
   std::vector<sfk::EventPair> events;             // per-kernel profiling
   std::size_t events_used = 0;
+  // Phase timing events (all null: the build is not timed).
   hipEvent_t phase[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Workspace borrowed from a fixed region instead of allocated: buffers
+  // never grow and the small lists take their slice as it is.
+  bool borrowed_workspace = false;
 };
 
 namespace sfk {
@@ -626,7 +644,9 @@ class Doubling {
     constexpr std::uint64_t kArrays = 5;
     const std::uint64_t held = b_->q_idx0.Bytes() / sizeof(Index);
     std::uint64_t cap = std::max<std::uint64_t>(want, 1);
-    if (cap > held) {
+    if (b_->borrowed_workspace) {
+      cap = std::min(cap, held);  // the slices are sized for the whole text
+    } else if (cap > held) {
       std::size_t free_bytes = 0, total_bytes = 0;
       SFK_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
       const std::uint64_t margin = 2ull << 30;
@@ -1014,6 +1034,10 @@ void ConvertTo(sfk_hip_builder* b, const In* in, std::uint64_t n, void* out, std
   }
 }
 
+void RecordPhase(sfk_hip_builder* b, int k) {
+  if (b->phase[k] != nullptr) SFK_HIP_CHECK(hipEventRecord(b->phase[k], b->stream));
+}
+
 // Rows a sampled index keeps: the text positions that are multiples of K.
 std::uint64_t SampledRows(std::uint64_t n, std::uint32_t k) { return (n + k - 1) / k; }
 
@@ -1089,7 +1113,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   // Both radix count sets start cleared (each sort then clears the other).
   SFK_HIP_CHECK(hipMemsetAsync(b->radix_counts.As<void>(), 0, 2 * kRadixSetStride, b->stream));
   b->radix_set = 0;
-  SFK_HIP_CHECK(hipEventRecord(b->phase[0], b->stream));
+  RecordPhase(b, 0);
   PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
       d_text, n, words, b->packed.As<uint4>(),
       (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, ctx.refs > 0,
@@ -1103,7 +1127,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   ctx.stats.algorithmic_bytes_sa += n + words * 16;
 
   Doubling<Index>(ctx, sa, rank, text).Run();
-  SFK_HIP_CHECK(hipEventRecord(b->phase[1], b->stream));
+  RecordPhase(b, 1);
 
   // K9: permutation + inverse check on the device (fused into the PLCP pass
   // when the LCP is built).
@@ -1114,7 +1138,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     SFK_HIP_CHECK(hipGetLastError());
     ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
   }
-  SFK_HIP_CHECK(hipEventRecord(b->phase[2], b->stream));
+  RecordPhase(b, 2);
   if (want_lcp) {
     if (sampled) {
       // The complete LCP into the workspace (vals0 is free after the
@@ -1126,7 +1150,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
       LaunchLcp<Index, std::uint64_t>(ctx, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
     }
   }
-  SFK_HIP_CHECK(hipEventRecord(b->phase[3], b->stream));
+  RecordPhase(b, 3);
   // SA and ISA in their stored widths (and, for a batch, per-reference
   // coordinates).
   if (ctx.refs > 0) {
@@ -1167,7 +1191,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
       ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + sa_bits / 8);
     }
   }
-  SFK_HIP_CHECK(hipEventRecord(b->phase[4], b->stream));
+  RecordPhase(b, 4);
   const DeviceCounters& c = ReadCounters(b);
   if (c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) {
     Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
@@ -1177,6 +1201,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   }
   ctx.stats.lce_long_items = c.long_items;
   ctx.stats.lce_long_words = c.long_words;
+  if (b->phase[0] == nullptr) return;  // untimed build
   float ms = 0.f;
   SFK_HIP_CHECK(hipEventElapsedTime(&ms, b->phase[0], b->phase[1]));
   ctx.stats.sa_seconds = ms * 1e-3;
@@ -2117,4 +2142,145 @@ int sfk_hip_tune_reset(int device) {
   for (auto& k : sfk::g_shape_table[device])
     for (auto& c : k) c.store(-1);
   return SFK_HIP_OK;
+}
+
+// ------------------------------------------------ allocation-free build
+//
+// sfk_hip_build_device_noalloc: sfk_hip_build_device for callers that cannot
+// allocate device memory.  Its workspace is `scratch`, a fixed array in
+// device memory that every GPU the library runs on holds from module load
+// until the process ends; each buffer of the build borrows a slice of it.
+// Host code never uses the array's name as an address: the device address
+// comes from hipGetSymbolAddress, once per device with that device current,
+// and only that pointer is handed to kernels and copies.  Every call on a
+// GPU shares that GPU's array (and one host word block for the step
+// counters), so calls that overlap on one GPU race; the call clears what a
+// build needs cleared (counters, radix digit counts, look-back status and
+// tickets) before it starts.
+
+static __device__ __align__(256) unsigned char scratch[SFK_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace sfk {
+namespace {
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxTuneDevices] = {};
+DeviceCounters g_noalloc_counters[kMaxTuneDevices];
+
+// The device address of `scratch` on `device` (which must be current).
+void* ScratchAddress(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      Fail(SFK_HIP_BUILD_FAILURE, std::string("address of the allocation-free workspace: ") + hipGetErrorString(err));
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+// Slices of the workspace for a 32-bit build of n symbols, in the sizes
+// BuildTyped and Doubling reserve (the small lists sized for every suffix).
+// Returns the bytes used; borrows the slices into `b` when base is given.
+std::uint64_t NoAllocLayout(sfk_hip_builder* b, std::uint64_t n, bool need_rank, char* base) {
+  constexpr std::uint64_t w = sizeof(std::uint32_t);
+  std::uint64_t used = 0;
+  auto take = [&](DeviceBuffer& buf, std::uint64_t bytes) {
+    bytes = std::max<std::uint64_t>(bytes, 1);
+    if (base != nullptr) buf.Borrow(base + used, bytes);
+    used += (bytes + 255) / 256 * 256;
+  };
+  const std::uint64_t seg_cap = n / 2 + 2;
+  take(b->counters, sizeof(DeviceCounters));
+  take(b->packed, PackedWords(n) * 16);
+  if (need_rank) take(b->rank, n * w);
+  take(b->act_idx, n * w);
+  take(b->act_seg, n * sizeof(std::uint32_t));
+  take(b->keys0, std::max<std::uint64_t>(n * 8, 64));
+  take(b->keys1, std::max<std::uint64_t>(n * 8, 64));
+  take(b->vals0, n * w);
+  take(b->vals1, n * w);
+  take(b->seg_delta0, seg_cap * w);
+  take(b->seg_delta1, seg_cap * w);
+  take(b->seg_start, seg_cap * w);
+  take(b->radix_counts, 2 * kRadixSetStride);
+  take(b->radix_status, OnesweepStatusBytes(n));
+  take(b->tickets, 2 * sizeof(unsigned int));
+  take(b->partial, std::max(kRefineMaxChunks * sizeof(RefinePartial), 1024 * sizeof(SplitPartial)));
+  for (DeviceBuffer* q : {&b->q_idx0, &b->q_idx1, &b->q_row0_0, &b->q_row0_1, &b->q_key}) take(*q, n * w);
+  return used;
+}
+
+}  // namespace
+}  // namespace sfk
+
+int sfk_hip_build_device_noalloc(int device, const uint8_t* d_text, uint64_t n, uint32_t construction_bits,
+                                 uint32_t sampling_rate, uint32_t flags, void* d_sa, uint32_t sa_bits, void* d_isa,
+                                 uint32_t isa_bits, void* d_lcp, uint32_t lcp_bits, void* stream,
+                                 sfk_hip_stats* stats, char* error, size_t error_capacity) {
+  std::string message;
+  if (d_text == nullptr || d_sa == nullptr || device < 0 || device >= sfk::kMaxTuneDevices) {
+    sfk::CopyError("device, text and SA buffers are required", error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (construction_bits != 32 || sampling_rate != 1 || sa_bits != 32 || (d_isa != nullptr && isa_bits != 32) ||
+      (d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP) && lcp_bits != 32) || (flags & SFK_HIP_PROFILE_KERNELS)) {
+    sfk::CopyError("the allocation-free build takes 32-bit construction, sampling rate 1, 32-bit outputs and no "
+                   "kernel profiling",
+                   error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  if (n > SFK_HIP_NOALLOC_MAX_SYMBOLS) {
+    sfk::CopyError("the allocation-free build takes at most " + std::to_string(SFK_HIP_NOALLOC_MAX_SYMBOLS) +
+                       " symbols (its workspace is a fixed " + std::to_string(SFK_HIP_NOALLOC_SCRATCH_BYTES) +
+                       "-byte array); " + std::to_string(n) + " given",
+                   error, error_capacity);
+    return SFK_HIP_INVALID_INPUT;
+  }
+  int status = sfk::ValidateArgs(n, construction_bits, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits, flags, message);
+  if (status != SFK_HIP_OK) {
+    sfk::CopyError(message, error, error_capacity);
+    return status;
+  }
+  sfk_hip_builder b;
+  b.device = device;
+  b.stream = static_cast<hipStream_t>(stream);
+  b.borrowed_workspace = true;
+  b.host_counters = &sfk::g_noalloc_counters[device];
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    SFK_HIP_CHECK(hipSetDevice(device));
+    char* base = static_cast<char*>(sfk::ScratchAddress(device));
+    const bool need_rank = d_isa == nullptr;
+    if (sfk::NoAllocLayout(&b, n, need_rank, nullptr) > SFK_HIP_NOALLOC_SCRATCH_BYTES) {
+      sfk::Fail(SFK_HIP_BUILD_FAILURE, "allocation-free workspace layout exceeds the fixed array");
+    }
+    sfk::NoAllocLayout(&b, n, need_rank, base);
+    // The look-back status words and tile tickets of the last call are
+    // stale for this one's epochs.
+    SFK_HIP_CHECK(hipMemsetAsync(b.radix_status.As<void>(), 0, b.radix_status.Bytes(), b.stream));
+    SFK_HIP_CHECK(hipMemsetAsync(b.tickets.As<void>(), 0, b.tickets.Bytes(), b.stream));
+    b.sort_epoch = 0;
+    status = sfk::RunDeviceBuild(&b, d_text, n, construction_bits, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp,
+                                 lcp_bits, stats, message);
+    if (stats != nullptr) {
+      stats->call_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  } catch (const sfk::Failure& f) {
+    (void)hipStreamSynchronize(b.stream);
+    sfk::CopyError(f.message, error, error_capacity);
+    status = f.status;
+  } catch (const std::bad_alloc&) {
+    sfk::CopyError("host allocation failed", error, error_capacity);
+    status = SFK_HIP_BUILD_FAILURE;
+  }
+  return status;
+}
+
+uint64_t sfk_hip_noalloc_workspace_bytes(uint64_t n, int with_isa) {
+  sfk_hip_builder b;
+  return sfk::NoAllocLayout(&b, n, with_isa == 0, nullptr);
 }
