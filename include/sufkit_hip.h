@@ -85,6 +85,13 @@ typedef struct sfk_hip_builder sfk_hip_builder;
 /* ABI version of the loaded library (SFK_HIP_ABI_VERSION). */
 int sfk_hip_abi_version(void);
 
+/* Fault injection for tests of the allocation-failure path (the kBuildFailure
+ * of a failed allocation, caps_backend.cpp:68-76): after `after` more device
+ * workspace allocations succeed, the next one fails as an out-of-memory
+ * hipMalloc would (SFK_HIP_BUILD_FAILURE), once; a negative value disarms.
+ * Process-wide.  No counterpart in the reference. */
+void sfk_hip_inject_allocation_failure(long long after);
+
 /* Number of visible HIP devices (0 when none); never fails. */
 int sfk_hip_device_count(void);
 

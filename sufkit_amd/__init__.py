@@ -9,6 +9,7 @@ native library is missing every call raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
@@ -232,7 +233,10 @@ _HOST = None
 
 
 def host_library_path() -> Path:
-    return LIB_DIR / "libsufkit.so"
+    """libsufkit.so in the package, or the file SUFKIT_AMD_HOST_LIBRARY names
+    (e.g. a sanitizer build of the same sources)."""
+    override = os.environ.get("SUFKIT_AMD_HOST_LIBRARY")
+    return Path(override) if override else LIB_DIR / "libsufkit.so"
 
 
 def load_host() -> ctypes.CDLL:
@@ -398,7 +402,8 @@ class ServeConfig(ctypes.Structure):
                 ("queries_per_index", ctypes.c_uint64), ("min_query_len", ctypes.c_uint32),
                 ("max_query_len", ctypes.c_uint32), ("query_seed", ctypes.c_uint64),
                 ("keep_resident", ctypes.c_uint32), ("sample_queries", ctypes.c_uint32),
-                ("digest_outputs", ctypes.c_uint32), ("fm_queries", ctypes.c_uint32)]
+                ("digest_outputs", ctypes.c_uint32), ("fm_queries", ctypes.c_uint32),
+                ("strict_budget", ctypes.c_uint32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -422,10 +427,12 @@ def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
           hbm_budget_bytes: int = 200 << 30, queries_per_index: int = 1_000_000,
           min_query_len: int = 20, max_query_len: int = 32, query_seed: int = 7,
           keep_resident: bool = True, sample_queries: int = 0, digest_outputs: bool = False,
-          fm_queries: bool = True):
+          fm_queries: bool = True, strict_budget: bool = False):
     """Run the serving loop (sufkit_amd/host/serve.cpp) over host texts with
     the given arrival times; count queries by FM backward search (fm_queries)
-    or SA binary search.  Returns (results, summary, samples)."""
+    or SA binary search.  strict_budget: refuse (status 1, invalid input) a
+    request whose footprint alone exceeds the budget instead of running it
+    alone.  Returns (results, summary, samples)."""
     lib = load_host()
     lib.sfk_serve_run.argtypes = [ctypes.POINTER(ServeRequest), ctypes.c_uint64, ctypes.POINTER(ServeConfig),
                                   ctypes.POINTER(ServeResult), ctypes.c_void_p, ctypes.c_void_p,
@@ -436,7 +443,8 @@ def serve(texts: list[np.ndarray], arrivals: list[float], *, slots: int = 4,
     count = len(texts)
     reqs = (ServeRequest * count)(*[ServeRequest(a, t.ctypes.data, t.size) for a, t in zip(arrivals, texts)])
     cfg = ServeConfig(hbm_budget_bytes, slots, queries_per_index, min_query_len, max_query_len,
-                      query_seed, int(keep_resident), sample_queries, int(digest_outputs), int(fm_queries))
+                      query_seed, int(keep_resident), sample_queries, int(digest_outputs), int(fm_queries),
+                      int(strict_budget))
     results = (ServeResult * count)()
     summary = ServeSummary()
     sp = np.zeros(max(1, count * sample_queries), np.uint64)

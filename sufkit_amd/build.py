@@ -55,15 +55,30 @@ def build_hip() -> Path:
     return out
 
 
-def build_host() -> Path | None:
+# Host-code sanitizers (AddressSanitizer + UndefinedBehaviorSanitizer, every
+# report fatal), for a CPU build of libsufkit.so only; the HIP library is
+# never built with them.
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                  "-fno-sanitize-recover=all"]
+
+
+def build_host(out_dir: Path | None = None, sanitize: bool = False) -> Path | None:
+    """libsufkit.so into sufkit_amd/lib (or out_dir, which then also gets a
+    link to the HIP library it loads).  sanitize: ASan + UBSan build."""
     sources = sorted(HOST.glob("*.cpp")) if HOST.exists() else []
     if not sources:
         return None
-    LIB.mkdir(exist_ok=True)
-    out = LIB / "libsufkit.so"
-    tmp = LIB / "libsufkit.so.tmp"
+    out_dir = Path(out_dir) if out_dir else LIB
+    out_dir.mkdir(parents=True, exist_ok=True)
+    if out_dir != LIB:
+        hip_lib = out_dir / "libsufkit_hip.so"
+        if not hip_lib.exists():
+            hip_lib.symlink_to(LIB / "libsufkit_hip.so")
+    out = out_dir / "libsufkit.so"
+    tmp = out_dir / "libsufkit.so.tmp"
+    opt = SANITIZE_FLAGS if sanitize else ["-O2"]
     _run([
-        CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+        CXX, *opt, "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
         f"-I{INCLUDE}", f"-I{HOST}",
         *map(str, sources),
         f"-L{LIB}", "-lsufkit_hip", "-Wl,-rpath,$ORIGIN", "-lz", "-lpthread",

@@ -68,6 +68,19 @@ int BitsFor(std::uint64_t max_value) {
 
 
 
+// Fault injection for tests of the allocation-failure path: when armed with
+// k >= 0, the (k+1)-th device workspace allocation from then on fails as an
+// out-of-memory hipMalloc would, once, and the hook disarms.
+std::atomic<long long> g_fail_allocation{-1};
+
+bool InjectedAllocationFailure() {
+  long long left = g_fail_allocation.load();
+  while (left >= 0) {
+    if (g_fail_allocation.compare_exchange_weak(left, left - 1)) return left == 0;
+  }
+  return false;
+}
+
 class DeviceBuffer {
  public:
   DeviceBuffer() = default;
@@ -84,6 +97,9 @@ class DeviceBuffer {
                                       std::to_string(bytes) + " bytes the build needs");
     }
     Release();
+    if (InjectedAllocationFailure()) {
+      Fail(SFK_HIP_BUILD_FAILURE, "device allocation of " + std::to_string(bytes) + " bytes failed (injected)");
+    }
     void* ptr = nullptr;
     const hipError_t err = hipMalloc(&ptr, bytes);
     if (err != hipSuccess) {
@@ -1316,6 +1332,10 @@ int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t
 extern "C" {
 
 int sfk_hip_abi_version(void) { return SFK_HIP_ABI_VERSION; }
+
+void sfk_hip_inject_allocation_failure(long long after) {
+  sfk::g_fail_allocation.store(after < 0 ? -1 : after);
+}
 
 int sfk_hip_device_count(void) {
   int count = 0;

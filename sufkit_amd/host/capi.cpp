@@ -399,7 +399,7 @@ int sfk_resolve_build_policy(int backend, int coordinate_width, int storage_widt
     const auto w = detail::ResolveSaCoordinateWidth(b, static_cast<CoordinateWidth>(coordinate_width),
                                                     text_symbols);
     if (!detail::SaConstructionCanRepresent(b, w, text_symbols)) {
-      detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
+      detail::Throw(ErrorCode::kInvalidInput, "text is longer than the chosen construction width can index");
     }
     const auto stored = detail::ResolveProfileStorageWidth(
         static_cast<CoordinateStorageWidth>(storage_width), static_cast<SaResourceProfile>(profile),

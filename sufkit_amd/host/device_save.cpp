@@ -62,7 +62,7 @@ StreamSection CoordinateSection(detail::SectionType type, CoordinateStorageWidth
       s.parts.push_back(DevicePart(values, count, bits, SFK_HIP_PLANE_NATIVE64));
       break;
     default:
-      detail::Throw(ErrorCode::kBuildFailure, "coordinate storage width is not resolved");
+      detail::Throw(ErrorCode::kBuildFailure, "storage width was not resolved before packing");
   }
   return s;
 }
@@ -80,7 +80,7 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
                      const SuffixArrayBuildOptions& options, DeviceSaveStatistics* statistics) {
   const auto begin = std::chrono::steady_clock::now();
   const std::uint64_t n = reference.Encoded().size() + 1;
-  if (n < 2) detail::Throw(ErrorCode::kInvalidInput, "reference text is empty");
+  if (n < 2) detail::Throw(ErrorCode::kInvalidInput, "the reference has no text");
   if (options.sampling_rate != 1) {
     detail::Throw(ErrorCode::kInvalidInput, "saving from device memory needs sampling rate 1");
   }
@@ -92,14 +92,14 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
     detail::Throw(ErrorCode::kUnsupportedBackend, "CHILD arrays are not part of this build");
   }
   if (static_cast<std::uint8_t>(options.acceleration) > static_cast<std::uint8_t>(SaAcceleration::kFull)) {
-    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array acceleration");
+    detail::Throw(ErrorCode::kInvalidInput, "acceleration must be none, LCP, LCP+CHILD, LCP+suffix links or full");
   }
   const auto width = buffers.construction_width;
   if (width != CoordinateWidth::kBits32 && width != CoordinateWidth::kBits64) {
-    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array coordinate width");
+    detail::Throw(ErrorCode::kInvalidInput, "construction width must be auto, 32 or 64 bits");
   }
   if (!detail::SaConstructionCanRepresent(SaBackend::kHip, width, n)) {
-    detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
+    detail::Throw(ErrorCode::kInvalidInput, "text is longer than the chosen construction width can index");
   }
   const bool retain_lcp = low_memory || options.acceleration != SaAcceleration::kNone;
   const bool retain_isa = !low_memory && options.acceleration == SaAcceleration::kLcpSuffixLink;
@@ -144,6 +144,20 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
     // on the device and written from the host (sa_codec.cpp:569-631).
     const std::uint32_t anchor_width = n - 1U > std::numeric_limits<std::uint32_t>::max() ? 64 : 32;
     char err[512] = {0};
+    // The checks BuildByteCoded makes before coding (lcp_storage.cpp:237-297):
+    // LCP[0] is 0, the ISA is the inverse of a permutation SA (here with the
+    // order of every adjacent pair as well), anchors fit their coordinates.
+    std::uint64_t first = 0;
+    if (sfk_hip_memcpy_d2h(&first, buffers.lcp, buffers.lcp_bits / 8) != SFK_HIP_OK) {
+      detail::Throw(ErrorCode::kBuildFailure, "cannot read the first LCP row from the device");
+    }
+    if (first != 0) detail::Throw(ErrorCode::kBuildFailure, "byte-coded LCP: row 0 of the raw LCP is not 0");
+    sfk_hip_check_report report{};
+    if (sfk_hip_check_index(buffers.device, static_cast<const std::uint8_t*>(buffers.text), n, buffers.sa,
+                            buffers.sa_bits, buffers.isa, buffers.isa_bits, nullptr, 0, 1, &report, err,
+                            sizeof(err)) != SFK_HIP_OK) {
+      detail::Throw(ErrorCode::kBuildFailure, std::string("byte-coded LCP: ") + err);
+    }
     std::uint64_t anchors = 0;
     int status = sfk_hip_byte_coded_lcp(buffers.device, buffers.lcp, buffers.lcp_bits, buffers.isa, buffers.isa_bits,
                                         n, nullptr, &anchors, nullptr, nullptr, 0, err, sizeof(err));
@@ -155,6 +169,16 @@ void SaveDeviceIndex(const GenomeReference& reference, const DeviceIndexBuffers&
     }
     if (status != SFK_HIP_OK) {
       detail::Throw(static_cast<ErrorCode>(status - 1), std::string("byte-coded LCP anchors: ") + err);
+    }
+    const std::uint64_t value_limit = anchor_width == 64 ? std::numeric_limits<std::uint64_t>::max()
+                                                         : std::numeric_limits<std::uint32_t>::max();
+    for (std::uint64_t a = 0; a < anchors; ++a) {
+      if (positions[a] >= n || values[a] > value_limit || values[a] < detail::LcpStorage::kByteLimit) {
+        detail::Throw(ErrorCode::kBuildFailure,
+                      "byte-coded LCP: anchor " + std::to_string(a) + " at position " + std::to_string(positions[a]) +
+                          " holds " + std::to_string(values[a]) + ", outside the " +
+                          std::to_string(anchor_width) + "-bit anchor range");
+      }
     }
     std::vector<std::uint8_t> planes;
     planes.reserve(2 * anchors * (anchor_width / 8));

@@ -53,7 +53,7 @@ void CheckHip(int status, const char* what) {
 
 // query.cpp:47-65: A/C/G/T in either case, as codes A=0 .. T=3.
 std::vector<std::uint8_t> EncodePattern(std::string_view pattern) {
-  if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "pattern must not be empty");
+  if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "an empty pattern was given");
   std::vector<std::uint8_t> out(pattern.size());
   for (std::size_t i = 0; i < pattern.size(); ++i) {
     switch (pattern[i]) {
@@ -61,7 +61,7 @@ std::vector<std::uint8_t> EncodePattern(std::string_view pattern) {
       case 'C': case 'c': out[i] = 1; break;
       case 'G': case 'g': out[i] = 2; break;
       case 'T': case 't': out[i] = 3; break;
-      default: detail::Throw(ErrorCode::kInvalidInput, "pattern contains a non-ACGT character");
+      default: detail::Throw(ErrorCode::kInvalidInput, "patterns may hold only A, C, G and T");
     }
   }
   return out;
@@ -79,7 +79,7 @@ bool IsReverseComplementPalindrome(const std::vector<std::uint8_t>& p) {
 std::uint32_t EffectiveBatchWidth(std::uint32_t requested) {
   if (requested == 0) return kAutomaticBatchWidth;
   if (requested > kMaximumBatchWidth) {
-    detail::Throw(ErrorCode::kInvalidInput, "FM batch width must be in [1,256] or zero for automatic");
+    detail::Throw(ErrorCode::kInvalidInput, "FM batch width is 0 (automatic) or 1 to 256");
   }
   return requested;
 }
@@ -186,7 +186,7 @@ struct FmIndex::Impl {
     }
     hits.resize(size);
     if (!hits.empty() && (contig_starts.empty() || contig_starts.size() != contig_lengths.size())) {
-      detail::Throw(ErrorCode::kCorruptIndex, "FM-index contig coordinate metadata is invalid");
+      detail::Throw(ErrorCode::kCorruptIndex, "contig table of the FM-index is inconsistent");
     }
     QueryResult result;
     result.total_hits = total;
@@ -215,7 +215,7 @@ FmIndex::~FmIndex() = default;
 
 FmIndex FmIndex::Build(const GenomeReference& reference, const FmIndexBuildOptions& options) {
   const auto& encoded = reference.Encoded();
-  if (encoded.empty()) detail::Throw(ErrorCode::kInvalidInput, "reference text is empty");
+  if (encoded.empty()) detail::Throw(ErrorCode::kInvalidInput, "the reference has no text");
   if (std::find(encoded.begin(), encoded.end(), detail::kSentinel) != encoded.end()) {
     detail::Throw(ErrorCode::kInvalidInput, "FM construction input contains a zero symbol");
   }
@@ -225,9 +225,9 @@ FmIndex FmIndex::Build(const GenomeReference& reference, const FmIndexBuildOptio
     case FmBackend::kSdslCsaWtEpr:
       break;
     case FmBackend::kSdslCsaSada:
-      detail::Throw(ErrorCode::kUnsupportedBackend, "sdsl-csa-sada is not implemented");
+      detail::Throw(ErrorCode::kUnsupportedBackend, "the sdsl-csa-sada FM backend is not available");
     default:
-      detail::Throw(ErrorCode::kUnsupportedBackend, "unknown FM-index backend");
+      detail::Throw(ErrorCode::kUnsupportedBackend, "FM backend id is not one this library knows");
   }
   if (!HipBackendAvailable()) detail::Throw(ErrorCode::kUnsupportedBackend, "no HIP device is visible");
 
@@ -315,7 +315,7 @@ std::uint64_t FmIndex::Count(std::string_view pattern, StrandMode strands) const
       return r[0].Size() + r[1].Size();
     }
   }
-  detail::Throw(ErrorCode::kInvalidInput, "invalid strand mode for FM count query");
+  detail::Throw(ErrorCode::kInvalidInput, "count: strand mode must be forward, reverse complement or both");
 }
 
 std::vector<std::uint64_t> FmIndex::CountBatch(const std::vector<std::string_view>& patterns,
@@ -323,7 +323,7 @@ std::vector<std::uint64_t> FmIndex::CountBatch(const std::vector<std::string_vie
   (void)EffectiveBatchWidth(options.batch_width);
   if (options.strands != StrandMode::kForward && options.strands != StrandMode::kReverseComplement &&
       options.strands != StrandMode::kBoth) {
-    detail::Throw(ErrorCode::kInvalidInput, "invalid strand mode for FM batch query");
+    detail::Throw(ErrorCode::kInvalidInput, "batch count: strand mode must be forward, reverse complement or both");
   }
   std::vector<std::vector<std::uint8_t>> encoded;
   encoded.reserve(patterns.size());
@@ -374,7 +374,7 @@ QueryResult FmIndex::Locate(std::string_view pattern, const LocateOptions& optio
       }
       break;
     default:
-      detail::Throw(ErrorCode::kInvalidInput, "invalid strand mode for FM locate query");
+      detail::Throw(ErrorCode::kInvalidInput, "locate: strand mode must be forward, reverse complement or both");
   }
   const std::uint64_t total = first.Size() + (has_second ? second.Size() : 0);
   // max_hits = 0 asks for the exact count without coordinates.
@@ -390,7 +390,7 @@ QueryResult FmIndex::Locate(std::string_view pattern, const LocateOptions& optio
 
 SequenceInfo FmIndex::GetSequenceInfo(SequenceId id) const {
   if (static_cast<std::size_t>(id) >= impl_->sequences.size()) {
-    detail::Throw(ErrorCode::kInvalidInput, "sequence id is out of range");
+    detail::Throw(ErrorCode::kInvalidInput, "no contig has this id");
   }
   return impl_->sequences[id];
 }

@@ -39,13 +39,13 @@ constexpr auto kEncoding = MakeEncodingTable();
 class Builder {
  public:
   void Add(std::string name, std::string description, std::string_view sequence) {
-    if (name.empty()) detail::Throw(ErrorCode::kInvalidInput, "FASTA record name must not be empty");
+    if (name.empty()) detail::Throw(ErrorCode::kInvalidInput, "a record has no name");
     if (!names_.insert(name).second) {
-      detail::Throw(ErrorCode::kInvalidInput, "duplicate FASTA record name: " + name);
+      detail::Throw(ErrorCode::kInvalidInput, "two records share the name " + name);
     }
-    if (sequence.empty()) detail::Throw(ErrorCode::kInvalidInput, "FASTA record must not be empty: " + name);
+    if (sequence.empty()) detail::Throw(ErrorCode::kInvalidInput, "record holds no bases: " + name);
     if (impl_->sequences.size() > std::numeric_limits<SequenceId>::max()) {
-      detail::Throw(ErrorCode::kInvalidInput, "too many FASTA records");
+      detail::Throw(ErrorCode::kInvalidInput, "more records than a reference can hold");
     }
     SequenceInfo info;
     info.id = static_cast<SequenceId>(impl_->sequences.size());
@@ -75,7 +75,7 @@ class Builder {
 
   std::unique_ptr<GenomeReference::Impl> Finish() {
     if (impl_->sequences.empty()) {
-      detail::Throw(ErrorCode::kInvalidInput, "reference must contain at least one FASTA record");
+      detail::Throw(ErrorCode::kInvalidInput, "a reference needs one record or more");
     }
     return std::move(impl_);
   }
@@ -95,7 +95,7 @@ GenomeReference::~GenomeReference() = default;
 
 GenomeReference GenomeReference::FromRecords(std::vector<SequenceRecord> records) {
   if (records.empty()) {
-    detail::Throw(ErrorCode::kInvalidInput, "reference must contain at least one FASTA record");
+    detail::Throw(ErrorCode::kInvalidInput, "a reference needs one record or more");
   }
   Builder builder;
   for (auto& r : records) builder.Add(std::move(r.name), std::move(r.description), r.sequence);
@@ -105,12 +105,12 @@ GenomeReference GenomeReference::FromRecords(std::vector<SequenceRecord> records
 GenomeReference GenomeReference::FromEncoded(const std::uint8_t* bases, const std::uint64_t* lengths,
                                              std::uint64_t count) {
   if (count == 0) {
-    detail::Throw(ErrorCode::kInvalidInput, "reference must contain at least one FASTA record");
+    detail::Throw(ErrorCode::kInvalidInput, "a reference needs one record or more");
   }
   auto impl = std::make_unique<Impl>();
   std::uint64_t offset = 0, in = 0;
   for (std::uint64_t c = 0; c < count; ++c) {
-    if (lengths[c] == 0) detail::Throw(ErrorCode::kInvalidInput, "FASTA record must not be empty");
+    if (lengths[c] == 0) detail::Throw(ErrorCode::kInvalidInput, "a record holds no bases");
     SequenceInfo info;
     info.id = static_cast<SequenceId>(c);
     info.name = "contig" + std::to_string(c);
@@ -136,7 +136,7 @@ GenomeReference GenomeReference::FromEncoded(const std::uint8_t* bases, const st
 
 GenomeReference GenomeReference::FromFasta(const std::filesystem::path& path) {
   gzFile file = gzopen(path.c_str(), "rb");
-  if (file == nullptr) detail::Throw(ErrorCode::kIoError, "cannot open FASTA: " + path.string());
+  if (file == nullptr) detail::Throw(ErrorCode::kIoError, "FASTA file could not be opened: " + path.string());
   std::unique_ptr<gzFile_s, int (*)(gzFile)> guard(file, gzclose);
   Builder builder;
   std::string name, description, sequence;
@@ -158,7 +158,7 @@ GenomeReference GenomeReference::FromFasta(const std::filesystem::path& path) {
         int err = 0;
         gzerror(file, &err);
         if (err != Z_OK && err != Z_STREAM_END) {
-          detail::Throw(ErrorCode::kInvalidInput, "malformed FASTA input: " + path.string());
+          detail::Throw(ErrorCode::kInvalidInput, "not FASTA (no '>' header before the bases): " + path.string());
         }
         eof = true;
         break;
@@ -180,7 +180,7 @@ GenomeReference GenomeReference::FromFasta(const std::filesystem::path& path) {
       }
       continue;
     }
-    if (!in_record) detail::Throw(ErrorCode::kInvalidInput, "malformed FASTA input: " + path.string());
+    if (!in_record) detail::Throw(ErrorCode::kInvalidInput, "not FASTA (no '>' header before the bases): " + path.string());
     for (const char ch : line) {
       if (ch != ' ' && ch != '\t') sequence.push_back(ch);
     }

@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
 #include <deque>
 #include <mutex>
 #include <string>
@@ -44,6 +45,8 @@ struct sfk_serve_config {
   uint32_t sample_queries;  // queries per request copied back for checking
   uint32_t digest_outputs;  // copy SA/ISA/LCP back and hash them (checking)
   uint32_t fm_queries;      // answer by FM backward search (FM-index built on the resident SA)
+  uint32_t strict_budget;   // refuse a request whose footprint alone exceeds the budget
+                            // (status SFK_HIP_INVALID_INPUT) instead of running it alone
 };
 
 struct sfk_serve_result {
@@ -154,7 +157,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
     if (sfk_hip_builder_create(0, &builder, err, sizeof(err)) != SFK_HIP_OK) builder = nullptr;
     while (true) {
       uint64_t i = 0;
-      uint64_t need = 0, ws_growth = 0;
+      uint64_t need = 0, ws_growth = 0, own = 0;
       {
         std::unique_lock<std::mutex> lock(mu);
         bool admitted = false;
@@ -180,7 +183,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
             FreeResident(resident[victim]);
             ++evicted;
           } else if (concurrent == 0) {
-            admitted = true;  // nothing in flight: an oversized request still runs
+            admitted = true;  // nothing in flight: an oversized request runs alone, or is refused
           } else {
             cv.wait(lock);
           }
@@ -195,6 +198,7 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         const uint64_t w = r.n - 1 <= 0xFFFFFFFFull ? 4 : 8;
         need = (r.n + 64) + 3 * r.n * w + r.n / 2 + 64 + (config->fm_queries ? r.n + 64 : 0) +
                config->queries_per_index * 13;
+        own = ws + need;  // this request's footprint alone
         reserved += ws_growth + need;
         peak = std::max(peak, reserved);
         max_concurrent = std::max(max_concurrent, ++concurrent);
@@ -205,25 +209,45 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
       const auto& req = requests[i];
       auto& res = results[i];
       Resident& rs = resident[i];
-      int status = builder == nullptr ? SFK_HIP_BUILD_FAILURE : SFK_HIP_OK;
+      // This request's message: the failing call's own text, or the name of
+      // the call when it leaves none.
+      err[0] = '\0';
+      int status = SFK_HIP_OK;
+      auto step = [&](int st, const std::string& what) {
+        if (st != SFK_HIP_OK && status == SFK_HIP_OK) {
+          status = st;
+          if (err[0] == '\0') std::snprintf(err, sizeof(err), "%s failed", what.c_str());
+        }
+        return status == SFK_HIP_OK;
+      };
+      if (builder == nullptr) step(SFK_HIP_BUILD_FAILURE, "builder creation for slot " + std::to_string(slot));
       const uint64_t n = req.n;
       const uint32_t bits = n - 1 <= 0xFFFFFFFFull ? 32 : 64;
       const uint64_t w = bits / 8;
       void* patterns = nullptr;
       void* lengths = nullptr;
       void* counts = nullptr;
+      const bool refused = status == SFK_HIP_OK && config->strict_budget && own > config->hbm_budget_bytes;
+      if (refused) {
+        std::snprintf(err, sizeof(err), "footprint of %llu bytes exceeds the HBM budget of %llu bytes",
+                      static_cast<unsigned long long>(own),
+                      static_cast<unsigned long long>(config->hbm_budget_bytes));
+        step(SFK_HIP_INVALID_INPUT, "admission");
+      }
       if (status == SFK_HIP_OK) {
-        status = sfk_hip_malloc(0, n + 64, &rs.text);
-        if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, n * w, &rs.sa);
-        if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, n * w, &rs.isa);
-        if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, n * w, &rs.lcp);
-        if (status == SFK_HIP_OK) status = sfk_hip_memcpy_h2d(rs.text, req.text, n);
+        const std::string bytes = std::to_string(n * w);
+        step(sfk_hip_malloc(0, n + 64, &rs.text), "device allocation of " + std::to_string(n + 64) + " text bytes");
+        if (status == SFK_HIP_OK) step(sfk_hip_malloc(0, n * w, &rs.sa), "device allocation of " + bytes + " SA bytes");
+        if (status == SFK_HIP_OK) step(sfk_hip_malloc(0, n * w, &rs.isa), "device allocation of " + bytes + " ISA bytes");
+        if (status == SFK_HIP_OK) step(sfk_hip_malloc(0, n * w, &rs.lcp), "device allocation of " + bytes + " LCP bytes");
+        if (status == SFK_HIP_OK) step(sfk_hip_memcpy_h2d(rs.text, req.text, n), "text upload");
       }
       if (status == SFK_HIP_OK) {
         sfk_hip_stats stats{};
-        status = sfk_hip_build_device(builder, static_cast<const uint8_t*>(rs.text), n, bits, 1,
-                                      SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, rs.sa, bits, rs.isa,
-                                      bits, rs.lcp, bits, &stats, err, sizeof(err));
+        step(sfk_hip_build_device(builder, static_cast<const uint8_t*>(rs.text), n, bits, 1,
+                                  SFK_HIP_RETAIN_ISA | SFK_HIP_RETAIN_LCP, rs.sa, bits, rs.isa, bits, rs.lcp, bits,
+                                  &stats, err, sizeof(err)),
+             "device build");
         res.device_build_seconds = stats.device_seconds;
       }
       res.built = now();
@@ -231,43 +255,45 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         void* planes[3] = {rs.sa, rs.isa, rs.lcp};
         std::vector<uint8_t> host(n * w);
         for (int k = 0; k < 3 && status == SFK_HIP_OK; ++k) {
-          status = sfk_hip_memcpy_d2h(host.data(), planes[k], n * w);
+          step(sfk_hip_memcpy_d2h(host.data(), planes[k], n * w), "output download");
           uint64_t hash = 1469598103934665603ULL;
           for (uint8_t byte : host) hash = (hash ^ byte) * 1099511628211ULL;
           res.output_hash[k] = hash;
         }
       }
       if (status == SFK_HIP_OK) {
-        status = sfk_hip_index_create(0, static_cast<const uint8_t*>(rs.text), n, rs.sa, bits,
-                                      &rs.index, err, sizeof(err));
+        step(sfk_hip_index_create(0, static_cast<const uint8_t*>(rs.text), n, rs.sa, bits, &rs.index, err,
+                                  sizeof(err)),
+             "resident index creation");
       }
       if (status == SFK_HIP_OK && config->fm_queries) {
-        status = sfk_hip_index_build_fm(rs.index, err, sizeof(err));
+        step(sfk_hip_index_build_fm(rs.index, err, sizeof(err)), "FM-index build");
       }
       const uint64_t q = config->queries_per_index;
       if (status == SFK_HIP_OK && q > 0) {
         const double qs = now();
-        status = sfk_hip_malloc(0, q * 8, &patterns);
-        if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, q, &lengths);
-        if (status == SFK_HIP_OK) status = sfk_hip_malloc(0, q * 4, &counts);
+        step(sfk_hip_malloc(0, q * 8, &patterns), "device allocation of " + std::to_string(q * 8) + " query bytes");
+        if (status == SFK_HIP_OK) step(sfk_hip_malloc(0, q, &lengths), "device allocation of query lengths");
+        if (status == SFK_HIP_OK) step(sfk_hip_malloc(0, q * 4, &counts), "device allocation of query counts");
         if (status == SFK_HIP_OK) {
-          status = sfk_hip_index_generate_queries(rs.index, config->query_seed ^ (i * 0x9E3779B97F4A7C15ULL),
-                                                  q, config->min_query_len, config->max_query_len,
-                                                  static_cast<uint64_t*>(patterns),
-                                                  static_cast<uint8_t*>(lengths), nullptr);
+          step(sfk_hip_index_generate_queries(rs.index, config->query_seed ^ (i * 0x9E3779B97F4A7C15ULL), q,
+                                              config->min_query_len, config->max_query_len,
+                                              static_cast<uint64_t*>(patterns), static_cast<uint8_t*>(lengths),
+                                              nullptr),
+               "query generation");
         }
         if (status == SFK_HIP_OK) {
-          status = config->fm_queries
-                       ? sfk_hip_index_fm_count(rs.index, static_cast<uint64_t*>(patterns),
-                                                static_cast<uint8_t*>(lengths), q,
-                                                static_cast<uint32_t*>(counts), nullptr)
-                       : sfk_hip_index_count(rs.index, static_cast<uint64_t*>(patterns),
-                                             static_cast<uint8_t*>(lengths), q,
-                                             static_cast<uint32_t*>(counts), nullptr);
+          step(config->fm_queries
+                   ? sfk_hip_index_fm_count(rs.index, static_cast<uint64_t*>(patterns),
+                                            static_cast<uint8_t*>(lengths), q, static_cast<uint32_t*>(counts),
+                                            nullptr)
+                   : sfk_hip_index_count(rs.index, static_cast<uint64_t*>(patterns), static_cast<uint8_t*>(lengths),
+                                         q, static_cast<uint32_t*>(counts), nullptr),
+               "count queries");
         }
         if (status == SFK_HIP_OK) {
           std::vector<uint32_t> host(q);
-          status = sfk_hip_memcpy_d2h(host.data(), counts, q * 4);
+          step(sfk_hip_memcpy_d2h(host.data(), counts, q * 4), "count download");
           uint64_t sum = 0, hash = 1469598103934665603ULL;
           for (uint64_t k = 0; k < q; ++k) {
             sum += host[k];
@@ -277,9 +303,11 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
           res.count_hash = hash;
           const uint64_t s = std::min<uint64_t>(config->sample_queries, q);
           if (status == SFK_HIP_OK && s > 0 && sample_patterns != nullptr) {
-            status = sfk_hip_memcpy_d2h(sample_patterns + i * config->sample_queries, patterns, s * 8);
+            step(sfk_hip_memcpy_d2h(sample_patterns + i * config->sample_queries, patterns, s * 8),
+                 "sample query download");
             if (status == SFK_HIP_OK) {
-              status = sfk_hip_memcpy_d2h(sample_lengths + i * config->sample_queries, lengths, s);
+              step(sfk_hip_memcpy_d2h(sample_lengths + i * config->sample_queries, lengths, s),
+                   "sample length download");
             }
             std::memcpy(sample_counts + i * config->sample_queries, host.data(), s * 4);
           }
@@ -298,6 +326,10 @@ extern "C" int sfk_serve_run(const sfk_serve_request* requests, uint64_t count,
         }
         --concurrent;
         ++finished;
+        if (refused) {  // its workspace growth never happened
+          workspace_reserved[slot] -= ws_growth;
+          reserved -= ws_growth;
+        }
         // Query buffers are gone; a resident index keeps the rest.
         reserved -= config->queries_per_index * 13;
         if (!config->keep_resident || status != SFK_HIP_OK) {

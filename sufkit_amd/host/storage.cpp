@@ -79,7 +79,7 @@ std::uint64_t MaxCoordinateForWidth(CoordinateStorageWidth width, ErrorCode erro
     case CoordinateStorageWidth::kAutoSelect:
       break;
   }
-  Throw(error_code, "invalid coordinate storage width");
+  Throw(error_code, "storage width must be auto, 32, 40, 48 or 64 bits");
 }
 
 CoordinateStorageWidth SelectCoordinateStorageWidth(std::uint64_t symbol_count,
@@ -100,7 +100,7 @@ CoordinateStorageWidth ResolveCoordinateStorageWidth(CoordinateStorageWidth requ
     return SelectCoordinateStorageWidth(symbol_count, error_code);
   }
   if (symbol_count - 1U > MaxCoordinateForWidth(requested, error_code)) {
-    Throw(error_code, "logical text exceeds the selected coordinate storage width");
+    Throw(error_code, "text positions do not fit the chosen storage width");
   }
   return requested;
 }
@@ -109,7 +109,7 @@ CoordinateStorageWidth ResolveProfileStorageWidth(CoordinateStorageWidth request
                                                   SaResourceProfile profile,
                                                   std::uint64_t symbol_count) {
   if (profile != SaResourceProfile::kFast && profile != SaResourceProfile::kLowMemory) {
-    Throw(ErrorCode::kInvalidInput, "invalid suffix-array resource profile");
+    Throw(ErrorCode::kInvalidInput, "resource profile must be Fast or Low-memory");
   }
   if (requested != CoordinateStorageWidth::kAutoSelect) {
     return ResolveCoordinateStorageWidth(requested, symbol_count);
@@ -304,30 +304,30 @@ LcpStorage LcpStorage::FromByteCoded(std::vector<std::uint8_t>&& primary, std::v
   if (sampling_rate == 0) Throw(error_code, "LCP storage: sampling rate must be positive");
   if (coordinate_width != 32 && coordinate_width != 64) Throw(error_code, "LCP storage: invalid coordinate width");
   if (positions.size() != values.size()) {
-    Throw(error_code, "LCP storage: overflow anchor planes have different lengths");
+    Throw(error_code, "LCP storage: anchor positions and values differ in number");
   }
   const std::uint64_t max_coordinate =
       coordinate_width == 32 ? std::numeric_limits<std::uint32_t>::max() : std::numeric_limits<std::uint64_t>::max();
   if (text_symbols != 0 && text_symbols - 1 > max_coordinate) {
-    Throw(error_code, "LCP storage: text does not fit the overflow coordinate width");
+    Throw(error_code, "LCP storage: anchor width too narrow for the text");
   }
   const std::uint64_t rows = text_symbols == 0 ? 0 : 1 + (text_symbols - 1) / sampling_rate;
-  if (primary.size() != rows) Throw(error_code, "LCP storage: LCP row count disagrees with sampling metadata");
+  if (primary.size() != rows) Throw(error_code, "LCP storage: row count does not follow from the sampling rate");
   if (positions.size() > (primary.empty() ? 0 : primary.size() - 1)) {
-    Throw(error_code, "LCP storage: too many overflow anchors for the LCP rows");
+    Throw(error_code, "LCP storage: more anchors than rows");
   }
   if (!primary.empty() && primary.front() != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
   for (std::size_t i = 0; i < positions.size(); ++i) {
     const std::uint64_t p = positions[i], v = values[i];
-    if (p >= text_symbols) Throw(error_code, "LCP storage: overflow anchor position is out of range");
-    if (p % sampling_rate != 0) Throw(error_code, "LCP storage: overflow anchor is not sampling-rate aligned");
-    if (v < kByteLimit) Throw(error_code, "LCP storage: overflow anchor value is smaller than 255");
-    if (v > max_coordinate) Throw(error_code, "LCP storage: overflow anchor value exceeds its width");
+    if (p >= text_symbols) Throw(error_code, "LCP storage: an anchor position lies past the text");
+    if (p % sampling_rate != 0) Throw(error_code, "LCP storage: an anchor position is not a multiple of the sampling rate");
+    if (v < kByteLimit) Throw(error_code, "LCP storage: an anchor holds a value below 255");
+    if (v > max_coordinate) Throw(error_code, "LCP storage: an anchor value does not fit the anchor width");
     if (i == 0) continue;
-    if (p <= positions[i - 1]) Throw(error_code, "LCP storage: overflow anchor positions are not increasing");
+    if (p <= positions[i - 1]) Throw(error_code, "LCP storage: anchor positions are not in increasing order");
     const std::uint64_t delta = p - positions[i - 1];
     if (delta == sampling_rate && values[i - 1] >= delta && values[i - 1] - delta == v) {
-      Throw(error_code, "LCP storage: overflow anchor redundantly continues a run");
+      Throw(error_code, "LCP storage: an anchor sits where the previous run continues");
     }
   }
   LcpStorage s;
@@ -371,9 +371,9 @@ std::uint64_t LcpStorage::Decode(std::uint64_t row, std::uint64_t suffix_positio
   const std::uint8_t primary = primary_[row];
   if (primary < kByteLimit) return primary;
   if (suffix_position % sampling_rate_ != 0) Throw(error_code, "LCP storage: suffix position is not aligned");
-  if (anchor_positions_.empty()) Throw(error_code, "LCP storage: byte overflow has no anchor");
+  if (anchor_positions_.empty()) Throw(error_code, "LCP storage: a 255 row has no anchor to decode it");
   const std::uint64_t block = suffix_position >> kGuideBlockShift;
-  if (block + 1 >= guide_.size()) Throw(error_code, "LCP storage: suffix position exceeds the overflow guide");
+  if (block + 1 >= guide_.size()) Throw(error_code, "LCP storage: suffix position lies beyond the anchor guide");
   const auto first = anchor_positions_.begin() + static_cast<std::ptrdiff_t>(guide_[block]);
   const auto last = anchor_positions_.begin() + static_cast<std::ptrdiff_t>(guide_[block + 1]);
   const auto found = std::upper_bound(first, last, suffix_position);
@@ -381,32 +381,32 @@ std::uint64_t LcpStorage::Decode(std::uint64_t row, std::uint64_t suffix_positio
   if (found != first) {
     anchor = static_cast<std::size_t>(found - anchor_positions_.begin()) - 1;
   } else {
-    if (guide_[block] == 0) Throw(error_code, "LCP storage: byte overflow precedes its first anchor");
+    if (guide_[block] == 0) Throw(error_code, "LCP storage: a 255 row comes before the first anchor");
     anchor = static_cast<std::size_t>(guide_[block]) - 1;
   }
   const std::uint64_t at = anchor_positions_[anchor], value = anchor_values_[anchor];
   const std::uint64_t delta = suffix_position - at;
   if (at > suffix_position || delta % sampling_rate_ != 0 || value < delta || value - delta < kByteLimit) {
-    Throw(error_code, "LCP storage: overflow anchor does not cover the suffix position");
+    Throw(error_code, "LCP storage: no anchor run reaches this suffix position");
   }
   return value - delta;
 }
 
 void LcpStorage::Validate(const CoordinateStorage& sa, std::uint64_t text_symbols,
                           ErrorCode error_code) const {
-  if (Size() != sa.Size()) Throw(error_code, "LCP storage: LCP and suffix-array lengths differ");
+  if (Size() != sa.Size()) Throw(error_code, "LCP storage: LCP rows and SA rows differ in number");
   if (Size() == 0) return;
   if (text_symbols == 0 || Size() != 1 + (text_symbols - 1) / sampling_rate_) {
-    Throw(error_code, "LCP storage: row count disagrees with sampling metadata");
+    Throw(error_code, "LCP storage: rows do not follow from the sampling rate");
   }
   if (Decode(0, sa.At(0), error_code) != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
   for (std::uint64_t row = 1; row < Size(); ++row) {
     const std::uint64_t suffix = sa.At(row), previous = sa.At(row - 1);
     if (suffix >= text_symbols || previous >= text_symbols) {
-      Throw(error_code, "LCP storage: suffix-array position is invalid");
+      Throw(error_code, "LCP storage: an SA value lies past the text");
     }
     if (Decode(row, suffix, error_code) > std::min(text_symbols - suffix, text_symbols - previous)) {
-      Throw(error_code, "LCP storage: value exceeds adjacent suffix bounds");
+      Throw(error_code, "LCP storage: an LCP runs past the end of its suffixes");
     }
   }
 }

@@ -275,13 +275,13 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
   };
   notify("index-text-prepare");
   if (options.threads == 0) {
-    detail::Throw(ErrorCode::kInvalidInput, "suffix-array thread count must be greater than zero");
+    detail::Throw(ErrorCode::kInvalidInput, "threads must be 1 or more");
   }
   if (options.sampling_rate == 0) {
-    detail::Throw(ErrorCode::kInvalidInput, "SA sampling rate must be greater than zero");
+    detail::Throw(ErrorCode::kInvalidInput, "sampling rate must be 1 or more");
   }
   if (static_cast<std::uint8_t>(options.acceleration) > static_cast<std::uint8_t>(SaAcceleration::kFull)) {
-    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array acceleration");
+    detail::Throw(ErrorCode::kInvalidInput, "acceleration must be none, LCP, LCP+CHILD, LCP+suffix links or full");
   }
   SuffixArrayBuildStatistics local_stats;
   SuffixArrayBuildStatistics& stats = options.statistics ? *options.statistics : local_stats;
@@ -295,7 +295,7 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
   impl->text.reserve(encoded.size() + 1);
   impl->text.insert(impl->text.end(), encoded.begin(), encoded.end());
   impl->text.push_back(detail::kSentinel);
-  if (impl->text.size() < 2) detail::Throw(ErrorCode::kInvalidInput, "reference text is empty");
+  if (impl->text.size() < 2) detail::Throw(ErrorCode::kInvalidInput, "the reference has no text");
   const std::uint64_t n = impl->text.size();
 
   const auto backend = detail::ResolveSaBackend(options.backend, n, options.threads, HipBackendAvailable());
@@ -303,19 +303,19 @@ SuffixArray SuffixArray::Build(const GenomeReference& reference, const SuffixArr
     detail::Throw(ErrorCode::kUnsupportedBackend,
                   "the divsufsort and CaPS-SA engines are not part of this build; use SaBackend::kHip");
   }
-  if (backend != SaBackend::kHip) detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array backend");
+  if (backend != SaBackend::kHip) detail::Throw(ErrorCode::kInvalidInput, "backend must be auto, divsufsort, CaPS or HIP");
   if (!HipBackendAvailable()) detail::Throw(ErrorCode::kUnsupportedBackend, "no HIP device is visible");
   const auto width = detail::ResolveSaCoordinateWidth(backend, options.coordinate_width, n);
   if (width != CoordinateWidth::kBits32 && width != CoordinateWidth::kBits64) {
-    detail::Throw(ErrorCode::kInvalidInput, "invalid suffix-array coordinate width");
+    detail::Throw(ErrorCode::kInvalidInput, "construction width must be auto, 32 or 64 bits");
   }
   if (!detail::SaConstructionCanRepresent(backend, width, n)) {
-    detail::Throw(ErrorCode::kInvalidInput, "reference exceeds the requested SA construction width");
+    detail::Throw(ErrorCode::kInvalidInput, "text is longer than the chosen construction width can index");
   }
   const auto stored_width = detail::ResolveProfileStorageWidth(options.storage_width, options.resource_profile, n);
   const bool low_memory = options.resource_profile == SaResourceProfile::kLowMemory;
   if (low_memory && options.sampling_rate != 1) {
-    detail::Throw(ErrorCode::kInvalidInput, "low-memory profile requires a complete suffix array");
+    detail::Throw(ErrorCode::kInvalidInput, "the Low-memory profile keeps every suffix (sampling rate 1)");
   }
   // The Low-memory profile is authoritative (suffix_array.cpp:4142-4151):
   // byte-coded LCP, and no ISA or CHILD stays resident.
@@ -470,7 +470,7 @@ std::vector<std::uint64_t> ReadLegacyValues(const std::vector<std::uint8_t>& byt
   }
   if (width != 32 && width != 64) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " has an invalid width");
   const std::size_t w = width / 8;
-  if ((bytes.size() - r.pos) != count * w) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " size mismatch");
+  if ((bytes.size() - r.pos) != count * w) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " has the wrong number of entries");
   std::vector<std::uint64_t> out(count);
   for (std::uint64_t i = 0; i < count; ++i) {
     std::uint64_t v = 0;
@@ -484,10 +484,10 @@ std::vector<std::uint64_t> ReadLegacyValues(const std::vector<std::uint8_t>& byt
 
 SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
   const auto container = detail::ReadContainer(path);
-  if (container.spec.kind != 1) detail::Throw(ErrorCode::kCorruptIndex, "index does not contain a suffix array");
+  if (container.spec.kind != 1) detail::Throw(ErrorCode::kCorruptIndex, "file holds no suffix-array payload");
   const auto backend_id = static_cast<std::uint8_t>(container.spec.backend);
   if (backend_id < 1 || backend_id > 6) {
-    detail::Throw(ErrorCode::kUnsupportedBackend, "unsupported suffix-array payload");
+    detail::Throw(ErrorCode::kUnsupportedBackend, "file holds a payload kind this library does not read");
   }
   auto impl = std::make_unique<Impl>();
   impl->backend = container.spec.backend;
@@ -498,33 +498,33 @@ SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
   {
     const auto bytes = detail::ReadSection(container, detail::SectionType::kMetadata);
     ByteReader r{bytes};
-    const auto count = r.U32("metadata sequence count");
+    const auto count = r.U32("metadata contig count");
     if (count == 0 || count != container.spec.sequence_count) {
-      detail::Throw(ErrorCode::kCorruptIndex, "metadata sequence count mismatch");
+      detail::Throw(ErrorCode::kCorruptIndex, "contig count of the metadata and of the header differ");
     }
     std::uint64_t expected_offset = 0;
     for (std::uint32_t i = 0; i < count; ++i) {
       SequenceInfo s;
       s.id = r.U32("sequence id");
-      s.global_offset = r.U64("sequence global offset");
-      s.length = r.U64("sequence length");
-      s.ambiguous_bases = r.U64("sequence ambiguous bases");
+      s.global_offset = r.U64("contig offset");
+      s.length = r.U64("contig length");
+      s.ambiguous_bases = r.U64("contig N count");
       s.name = r.Str("sequence name");
-      s.description = r.Str("sequence description");
+      s.description = r.Str("contig description");
       if (s.id != i || s.name.empty() || s.length == 0 || s.global_offset != expected_offset ||
           s.ambiguous_bases > s.length) {
-        detail::Throw(ErrorCode::kCorruptIndex, "invalid sequence metadata");
+        detail::Throw(ErrorCode::kCorruptIndex, "contig records are inconsistent");
       }
       expected_offset += s.length + 1;
       impl->total_bases += s.length;
       impl->ambiguous_bases += s.ambiguous_bases;
       impl->sequences.push_back(std::move(s));
     }
-    if (r.pos != bytes.size()) detail::Throw(ErrorCode::kCorruptIndex, "metadata section has trailing bytes");
+    if (r.pos != bytes.size()) detail::Throw(ErrorCode::kCorruptIndex, "bytes remain after the metadata records");
     if (impl->total_bases != container.spec.total_bases ||
         impl->ambiguous_bases != container.spec.ambiguous_bases ||
         expected_offset + 1 != container.spec.text_symbols) {
-      detail::Throw(ErrorCode::kCorruptIndex, "metadata totals do not match the index header");
+      detail::Throw(ErrorCode::kCorruptIndex, "metadata totals and header totals differ");
     }
   }
   // Text: sentinel, separators, fingerprint (suffix_array.cpp:4945-4969).
@@ -532,32 +532,32 @@ SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
   const std::uint64_t n = impl->text.size();
   if (n != container.spec.text_symbols || n < 2 || impl->text.back() != detail::kSentinel ||
       std::find(impl->text.begin(), impl->text.end() - 1, detail::kSentinel) != impl->text.end() - 1) {
-    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text has an invalid sentinel");
+    detail::Throw(ErrorCode::kCorruptIndex, "text section does not end with its only 0");
   }
   for (const auto& s : impl->sequences) {
     const auto sep = s.global_offset + s.length;
     if (sep >= n || impl->text[sep] != detail::kSeparator) {
-      detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text has an invalid contig separator");
+      detail::Throw(ErrorCode::kCorruptIndex, "text section has separators where the contigs do not end");
     }
   }
   if (detail::ContentFingerprint(impl->text.data(), n - 1) != impl->fingerprint) {
-    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array text fingerprint mismatch");
+    detail::Throw(ErrorCode::kCorruptIndex, "text section does not hash to the header's fingerprint");
   }
   if (const auto* sampling = container.Find(detail::SectionType::kSaSampling)) {
     (void)sampling;
     const auto bytes = detail::ReadSection(container, detail::SectionType::kSaSampling);
     ByteReader r{bytes};
     impl->sampling_rate = r.U32("SA sampling rate");
-    const auto recorded = r.U64("sampled suffix count");
+    const auto recorded = r.U64("kept suffix count");
     if (impl->sampling_rate <= 1 || recorded != 1 + (n - 1) / impl->sampling_rate || r.pos != bytes.size()) {
-      detail::Throw(ErrorCode::kCorruptIndex, "invalid sampled SA metadata");
+      detail::Throw(ErrorCode::kCorruptIndex, "sampling record is inconsistent");
     }
   }
   const std::uint32_t k = impl->sampling_rate;
   const std::uint64_t count = 1 + (n - 1) / k;
   const bool backend_is_32 = backend_id == 1 || backend_id == 3 || backend_id == 5;
   if (container.spec.coordinate_width != (backend_is_32 ? 32 : 64)) {
-    detail::Throw(ErrorCode::kCorruptIndex, "suffix-array backend and construction width disagree");
+    detail::Throw(ErrorCode::kCorruptIndex, "stored backend does not match the construction width");
   }
   const bool v14 = container.spec.format_minor >= 4;
   // SA section + sampled-permutation check (suffix_array.cpp:4999-5010).
@@ -578,8 +578,8 @@ SuffixArray SuffixArray::Load(const std::filesystem::path& path) {
     std::vector<bool> seen(count, false);
     for (std::uint64_t r = 0; r < count; ++r) {
       const auto v = impl->sa.At(r);
-      if (v >= n || v % k != 0) detail::Throw(ErrorCode::kCorruptIndex, "suffix array contains a non-sampled position");
-      if (seen[v / k]) detail::Throw(ErrorCode::kCorruptIndex, "suffix array is not a sampled permutation");
+      if (v >= n || v % k != 0) detail::Throw(ErrorCode::kCorruptIndex, "a stored suffix is not at a multiple of the sampling rate");
+      if (seen[v / k]) detail::Throw(ErrorCode::kCorruptIndex, "stored suffixes repeat a position");
       seen[v / k] = true;
     }
   }
@@ -716,7 +716,7 @@ std::pair<std::uint64_t, std::uint64_t> SuffixArray::PrefixDirectoryInterval(std
 }
 
 std::uint64_t SuffixArray::Count(std::string_view pattern) const {
-  if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "pattern must not be empty");
+  if (pattern.empty()) detail::Throw(ErrorCode::kInvalidInput, "an empty pattern was given");
   if (impl_->sampling_rate != 1) {
     detail::Throw(ErrorCode::kUnsupportedBackend, "count over a sampled suffix array is not part of this build");
   }
@@ -727,7 +727,7 @@ std::uint64_t SuffixArray::Count(std::string_view pattern) const {
       case 'C': case 'c': p[i] = detail::kC; break;
       case 'G': case 'g': p[i] = detail::kG; break;
       case 'T': case 't': p[i] = detail::kT; break;
-      default: detail::Throw(ErrorCode::kInvalidInput, "pattern contains a non-ACGT character");
+      default: detail::Throw(ErrorCode::kInvalidInput, "patterns may hold only A, C, G and T");
     }
   }
   const auto& text = impl_->text;

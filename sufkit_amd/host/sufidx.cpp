@@ -3,6 +3,8 @@
 // no-clobber publication (serialization.cpp:246-362, 529-765).  All integers
 // are little-endian.
 #include <fcntl.h>
+#include <stdio.h>
+#include <sys/stat.h>
 #include <unistd.h>
 #include <zlib.h>
 
@@ -39,11 +41,9 @@ constexpr std::uint32_t kMaxSections = 16;
 
 enum class CoordinateCodec : std::uint32_t { kNative32 = 0, kNative64 = 1, kSplit40 = 2, kSplit48 = 3 };
 
-std::atomic<std::uint64_t> g_temp_counter{0};
-
 void WriteBytes(std::ostream& out, const void* data, std::size_t size) {
   out.write(static_cast<const char*>(data), static_cast<std::streamsize>(size));
-  if (!out) Throw(ErrorCode::kIoError, "cannot write index section");
+  if (!out) Throw(ErrorCode::kIoError, "write to the index file failed");
 }
 
 template <class T>
@@ -84,14 +84,14 @@ std::vector<T> ReadPlane(std::istream& in, std::uint64_t count, const char* what
   try {
     values.resize(static_cast<std::size_t>(count));
   } catch (const std::exception&) {
-    Throw(ErrorCode::kCorruptIndex, std::string("cannot allocate ") + what);
+    Throw(ErrorCode::kCorruptIndex, std::string("no memory for ") + what);
   }
   if (count) ReadExact(in, values.data(), static_cast<std::size_t>(count) * sizeof(T), what);
   return values;
 }
 
 std::uint64_t CheckedMul(std::uint64_t a, std::uint64_t b, ErrorCode code) {
-  if (b != 0 && a > std::numeric_limits<std::uint64_t>::max() / b) Throw(code, "byte count overflows");
+  if (b != 0 && a > std::numeric_limits<std::uint64_t>::max() / b) Throw(code, "section size overflows 64 bits");
   return a * b;
 }
 
@@ -158,41 +158,60 @@ class CrcBuffer : public std::streambuf {
   std::uint32_t crc_ = 0;
 };
 
-std::filesystem::path TemporaryPathFor(const std::filesystem::path& target) {
-  for (unsigned attempt = 0; attempt < 100; ++attempt) {
-    const auto counter = g_temp_counter.fetch_add(1, std::memory_order_relaxed);
-    const auto ticks = static_cast<std::uint64_t>(
-        std::chrono::steady_clock::now().time_since_epoch().count());
-    const std::filesystem::path temp(target.string() + ".partial." +
-                                     std::to_string(static_cast<long long>(getpid())) + "." +
-                                     std::to_string(ticks ^ counter));
-    const int fd = open(temp.c_str(), O_CREAT | O_EXCL | O_RDWR | O_CLOEXEC, 0666);
-    if (fd >= 0) {
-      close(fd);
-      return temp;
-    }
-    if (errno != EEXIST) {
-      Throw(ErrorCode::kIoError, "cannot create temporary index: " +
-                                     std::error_code(errno, std::generic_category()).message());
-    }
-  }
-  Throw(ErrorCode::kIoError, "cannot allocate a unique temporary index path");
+// The process's file-creation mask, read once (umask() can only be read by
+// setting it, so this happens before any file is written).
+mode_t CreationMask() {
+  static const mode_t mask = [] {
+    const mode_t old = umask(0);
+    umask(old);
+    return old;
+  }();
+  return mask;
 }
 
-void Publish(const std::filesystem::path& temp, const std::filesystem::path& target, bool overwrite) {
+// A new empty file beside `target` for the index to be written into, named
+// by mkostemp from a "<name>.sfk-XXXXXX" template, so its directory (and
+// hence its file system) is the target's.  It gets the permissions a plain
+// create would give (0666 less the creation mask), not mkostemp's 0600.
+std::filesystem::path StagingFileFor(const std::filesystem::path& target) {
+  std::string name = target.string() + ".sfk-XXXXXX";
+  const mode_t mask = CreationMask();
+  const int fd = mkostemp(name.data(), O_CLOEXEC);
+  if (fd < 0) {
+    Throw(ErrorCode::kIoError, "no staging file could be made beside " + target.string() + " (" +
+                                   std::error_code(errno, std::generic_category()).message() + ")");
+  }
+  const bool ok = fchmod(fd, 0666 & ~mask) == 0;
+  const int chmod_errno = errno;
+  close(fd);
+  if (!ok) {
+    unlink(name.c_str());
+    Throw(ErrorCode::kIoError, "staging file " + name + " keeps mkostemp's permissions (" +
+                                   std::error_code(chmod_errno, std::generic_category()).message() + ")");
+  }
+  return name;
+}
+
+// Moves the finished staging file to `target` in one step.  Without
+// overwrite the move must not replace a file that appeared meanwhile:
+// renameat2 with RENAME_NOREPLACE, or, where the file system lacks it, a
+// hard link (which fails on an existing name) and the staging name removed.
+void MoveIntoPlace(const std::filesystem::path& staging, const std::filesystem::path& target, bool overwrite) {
+  auto fail = [&](int err) {
+    if (err == EEXIST) {
+      Throw(ErrorCode::kIoError, target.string() + " appeared while the index was written; it was left as it is");
+    }
+    Throw(ErrorCode::kIoError, "the written index could not be moved to " + target.string() + " (" +
+                                   std::error_code(err, std::generic_category()).message() + ")");
+  };
   if (overwrite) {
-    std::filesystem::rename(temp, target);
+    if (rename(staging.c_str(), target.c_str()) != 0) fail(errno);
     return;
   }
-  // Same-directory hard link: atomic publication that never replaces a
-  // target created concurrently by another writer.
-  if (link(temp.c_str(), target.c_str()) != 0) {
-    if (errno == EEXIST) Throw(ErrorCode::kIoError, "index already exists: " + target.string());
-    Throw(ErrorCode::kIoError, "cannot publish index: " +
-                                   std::error_code(errno, std::generic_category()).message());
-  }
-  std::error_code ignored;
-  std::filesystem::remove(temp, ignored);
+  if (renameat2(AT_FDCWD, staging.c_str(), AT_FDCWD, target.c_str(), RENAME_NOREPLACE) == 0) return;
+  if (errno != EINVAL && errno != ENOSYS) fail(errno);
+  if (link(staging.c_str(), target.c_str()) != 0) fail(errno);
+  unlink(staging.c_str());
 }
 
 }  // namespace
@@ -233,7 +252,7 @@ std::vector<std::uint8_t> CoordinateSectionHeader(CoordinateStorageWidth width, 
     case CoordinateStorageWidth::kBits64: codec = CoordinateCodec::kNative64; low = count * 8; break;
     case CoordinateStorageWidth::kBits40: codec = CoordinateCodec::kSplit40; low = count * 4; high = count; break;
     case CoordinateStorageWidth::kBits48: codec = CoordinateCodec::kSplit48; low = count * 4; high = count * 2; break;
-    default: Throw(ErrorCode::kBuildFailure, "coordinate storage width is not resolved");
+    default: Throw(ErrorCode::kBuildFailure, "storage width was not resolved before packing");
   }
   std::vector<std::uint8_t> h;
   h.reserve(kCoordinateHeaderBytes);
@@ -301,20 +320,20 @@ void WriteCoordinateSectionV14(std::ostream& out, const CoordinateStorage& stora
 CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t section_size,
                                            std::uint64_t expected_elements,
                                            std::uint64_t expected_symbols, const char* label) {
-  if (ReadU32(in, "coordinate codec magic") != kCoordinateMagic) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: wrong coordinate codec magic");
+  if (ReadU32(in, "SKC1 magic") != kCoordinateMagic) {
+    Throw(ErrorCode::kCorruptIndex, "SKC1 section: magic is not SKC1");
   }
-  if (ReadU32(in, "coordinate codec version") != kCodecVersion) {
-    Throw(ErrorCode::kVersionMismatch, "SA section codec: unsupported coordinate codec header version");
+  if (ReadU32(in, "SKC1 version") != kCodecVersion) {
+    Throw(ErrorCode::kVersionMismatch, "SKC1 section: header version is not 1");
   }
-  const std::uint32_t raw_codec = ReadU32(in, "coordinate codec identifier");
-  if (raw_codec > 3) Throw(ErrorCode::kVersionMismatch, "SA section codec: unknown coordinate codec identifier");
-  if (ReadU32(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: reserved field is non-zero");
-  const std::uint64_t symbols = ReadU64(in, "coordinate symbol count");
-  const std::uint64_t count = ReadU64(in, "coordinate element count");
-  const std::uint64_t low = ReadU64(in, "coordinate low-plane size");
-  const std::uint64_t high = ReadU64(in, "coordinate high-plane size");
-  if (ReadU64(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: reserved field is non-zero");
+  const std::uint32_t raw_codec = ReadU32(in, "SKC1 codec id");
+  if (raw_codec > 3) Throw(ErrorCode::kVersionMismatch, "SKC1 section: codec id is not native32/native64/split40/split48");
+  if (ReadU32(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SKC1 section: a reserved field is set");
+  const std::uint64_t symbols = ReadU64(in, "SKC1 symbol count");
+  const std::uint64_t count = ReadU64(in, "SKC1 element count");
+  const std::uint64_t low = ReadU64(in, "SKC1 low plane bytes");
+  const std::uint64_t high = ReadU64(in, "SKC1 high plane bytes");
+  if (ReadU64(in, "coordinate reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SKC1 section: a reserved field is set");
   const auto codec = static_cast<CoordinateCodec>(raw_codec);
   std::uint64_t want_low = 0, want_high = 0;
   CoordinateStorageWidth width = CoordinateStorageWidth::kBits32;
@@ -326,11 +345,11 @@ CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t secti
   }
   if (symbols == 0 || low != want_low || high != want_high ||
       kCoordinateHeaderBytes + low + high != section_size) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: coordinate plane lengths disagree with the codec");
+    Throw(ErrorCode::kCorruptIndex, "SKC1 section: plane byte counts do not follow from the codec");
   }
   if (count != expected_elements || symbols != expected_symbols) {
-    Throw(ErrorCode::kCorruptIndex, std::string("SA section codec: ") + label +
-                                        " metadata disagrees with the container");
+    Throw(ErrorCode::kCorruptIndex, std::string("SKC1 section (") + label + ")" +
+                                        " fields disagree with the file header");
   }
   switch (codec) {
     case CoordinateCodec::kNative32:
@@ -352,13 +371,13 @@ CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t secti
       return CoordinateStorage::FromPacked48(std::move(v), symbols, ErrorCode::kCorruptIndex, label);
     }
   }
-  Throw(ErrorCode::kCorruptIndex, "SA section codec: unreachable");
+  Throw(ErrorCode::kCorruptIndex, "SKC1 section: codec id not handled");
 }
 
 void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t text_symbols) {
-  if (lcp.Empty()) Throw(ErrorCode::kBuildFailure, "SA section codec: cannot serialize an empty LCP");
+  if (lcp.Empty()) Throw(ErrorCode::kBuildFailure, "SKL1 section: there is no LCP to write");
   if (text_symbols == 0 || lcp.Size() != 1 + (text_symbols - 1) / lcp.SamplingRate()) {
-    Throw(ErrorCode::kBuildFailure, "SA section codec: LCP metadata is inconsistent before serialization");
+    Throw(ErrorCode::kBuildFailure, "SKL1 section: LCP fields disagree before writing");
   }
   const auto header = LcpSectionHeader(lcp.Width(), lcp.SamplingRate(), text_symbols, lcp.Size(), lcp.ByteCoded(),
                                        lcp.AnchorPositions().size());
@@ -382,57 +401,57 @@ void WriteLcpSectionV14(std::ostream& out, const LcpStorage& lcp, std::uint64_t 
 
 LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size, std::uint64_t expected_rows,
                              std::uint32_t sampling_rate, std::uint64_t text_symbols) {
-  if (ReadU32(in, "LCP codec magic") != kLcpMagic) Throw(ErrorCode::kCorruptIndex, "SA section codec: wrong LCP codec magic");
-  if (ReadU32(in, "LCP codec version") != kCodecVersion) {
-    Throw(ErrorCode::kVersionMismatch, "SA section codec: unsupported LCP codec header version");
+  if (ReadU32(in, "SKL1 magic") != kLcpMagic) Throw(ErrorCode::kCorruptIndex, "SKL1 section: magic is not SKL1");
+  if (ReadU32(in, "SKL1 version") != kCodecVersion) {
+    Throw(ErrorCode::kVersionMismatch, "SKL1 section: header version is not 1");
   }
-  const std::uint32_t codec = ReadU32(in, "LCP codec identifier");
-  if (codec > 1) Throw(ErrorCode::kVersionMismatch, "SA section codec: unknown LCP codec identifier");
-  const std::uint32_t width = ReadU32(in, "LCP coordinate width");
-  const std::uint32_t rate = ReadU32(in, "LCP sampling rate");
-  if (ReadU32(in, "LCP guide block shift") != kLcpGuideBlockShift) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: invalid LCP guide block shift");
+  const std::uint32_t codec = ReadU32(in, "SKL1 codec id");
+  if (codec > 1) Throw(ErrorCode::kVersionMismatch, "SKL1 section: codec id is not raw or byte-coded");
+  const std::uint32_t width = ReadU32(in, "SKL1 value width");
+  const std::uint32_t rate = ReadU32(in, "SKL1 sampling rate");
+  if (ReadU32(in, "SKL1 guide shift") != kLcpGuideBlockShift) {
+    Throw(ErrorCode::kCorruptIndex, "SKL1 section: guide shift is not 12");
   }
-  const std::uint64_t symbols = ReadU64(in, "LCP text-symbol count");
+  const std::uint64_t symbols = ReadU64(in, "SKL1 symbol count");
   const std::uint64_t rows = ReadU64(in, "LCP row count");
-  const std::uint64_t primary = ReadU64(in, "LCP primary-plane size");
-  const std::uint64_t anchors = ReadU64(in, "LCP anchor count");
-  const std::uint64_t anchor_pos = ReadU64(in, "LCP anchor-position size");
-  const std::uint64_t anchor_val = ReadU64(in, "LCP anchor-value size");
-  if (ReadU64(in, "LCP reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP codec reserved field is non-zero");
+  const std::uint64_t primary = ReadU64(in, "SKL1 primary bytes");
+  const std::uint64_t anchors = ReadU64(in, "SKL1 anchor count");
+  const std::uint64_t anchor_pos = ReadU64(in, "SKL1 anchor position bytes");
+  const std::uint64_t anchor_val = ReadU64(in, "SKL1 anchor value bytes");
+  if (ReadU64(in, "LCP reserved") != 0) Throw(ErrorCode::kCorruptIndex, "SKL1 section: a reserved field is set");
   if (rate == 0 || (width != 32 && width != 64) || symbols == 0) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: invalid LCP header");
+    Throw(ErrorCode::kCorruptIndex, "SKL1 section: header fields are inconsistent");
   }
   if (rows != expected_rows || rate != sampling_rate || symbols != text_symbols) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP metadata disagrees with the container");
+    Throw(ErrorCode::kCorruptIndex, "SKL1 section: fields disagree with the file header");
   }
   if (codec == 1) {  // byte-coded: primary bytes, anchor positions, anchor values (sa_codec.cpp:353-370, 655-680)
     if (width == 32 && symbols - 1 > std::numeric_limits<std::uint32_t>::max()) {
-      Throw(ErrorCode::kCorruptIndex, "SA section codec: LCP logical text exceeds its coordinate width");
+      Throw(ErrorCode::kCorruptIndex, "SKL1 section: text positions do not fit the value width");
     }
     if (rows != 1 + (symbols - 1) / rate || primary != rows || anchors >= rows ||
         anchor_pos != CheckedMul(anchors, width / 8u, ErrorCode::kCorruptIndex) || anchor_val != anchor_pos ||
         kLcpHeaderBytes + primary + anchor_pos + anchor_val != section_size) {
-      Throw(ErrorCode::kCorruptIndex, "SA section codec: byte-coded LCP payload lengths are inconsistent");
+      Throw(ErrorCode::kCorruptIndex, "SKL1 section: byte-coded plane sizes are inconsistent");
     }
     std::vector<std::uint8_t> bytes(static_cast<std::size_t>(rows));
-    ReadExact(in, bytes.data(), bytes.size(), "byte-coded LCP primary");
+    ReadExact(in, bytes.data(), bytes.size(), "SKL1 primary plane");
     std::vector<std::uint64_t> positions, values;
     if (width == 32) {
-      auto p32 = ReadPlane<std::uint32_t>(in, anchors, "LCP anchor positions");
-      auto v32 = ReadPlane<std::uint32_t>(in, anchors, "LCP anchor values");
+      auto p32 = ReadPlane<std::uint32_t>(in, anchors, "SKL1 anchor positions");
+      auto v32 = ReadPlane<std::uint32_t>(in, anchors, "SKL1 anchor values");
       positions.assign(p32.begin(), p32.end());
       values.assign(v32.begin(), v32.end());
     } else {
-      positions = ReadPlane<std::uint64_t>(in, anchors, "LCP anchor positions");
-      values = ReadPlane<std::uint64_t>(in, anchors, "LCP anchor values");
+      positions = ReadPlane<std::uint64_t>(in, anchors, "SKL1 anchor positions");
+      values = ReadPlane<std::uint64_t>(in, anchors, "SKL1 anchor values");
     }
     return LcpStorage::FromByteCoded(std::move(bytes), std::move(positions), std::move(values), rate, symbols,
                                      static_cast<std::uint8_t>(width), ErrorCode::kCorruptIndex);
   }
   if (rows != 1 + (symbols - 1) / rate || primary != CheckedMul(rows, width / 8u, ErrorCode::kCorruptIndex) ||
       anchors != 0 || anchor_pos != 0 || anchor_val != 0 || kLcpHeaderBytes + primary != section_size) {
-    Throw(ErrorCode::kCorruptIndex, "SA section codec: raw LCP payload lengths are inconsistent");
+    Throw(ErrorCode::kCorruptIndex, "SKL1 section: raw plane size is inconsistent");
   }
   if (width == 32) return LcpStorage::FromRaw32(ReadPlane<std::uint32_t>(in, rows, "raw32 LCP"), rate, ErrorCode::kCorruptIndex);
   return LcpStorage::FromRaw64(ReadPlane<std::uint64_t>(in, rows, "raw64 LCP"), rate, ErrorCode::kCorruptIndex);
@@ -449,7 +468,7 @@ const SectionDescriptor* ParsedContainer::Find(SectionType type) const noexcept 
 
 const SectionDescriptor& ParsedContainer::Require(SectionType type) const {
   const auto* s = Find(type);
-  if (s == nullptr) Throw(ErrorCode::kCorruptIndex, "required index section is missing");
+  if (s == nullptr) Throw(ErrorCode::kCorruptIndex, "a section the index needs is absent");
   return *s;
 }
 
@@ -496,29 +515,29 @@ std::vector<std::uint8_t> BuildHeader(const ContainerSpec& spec,
 void WriteContainer(const std::filesystem::path& path, const SaveOptions& options,
                     const ContainerSpec& spec, const std::vector<SectionWriter>& writers) {
   if (writers.empty() || writers.size() > kMaxSections) {
-    Throw(ErrorCode::kBuildFailure, "invalid number of index sections");
+    Throw(ErrorCode::kBuildFailure, "an index has 1 to 16 sections");
   }
-  if (path.empty()) Throw(ErrorCode::kInvalidInput, "index output path must not be empty");
+  if (path.empty()) Throw(ErrorCode::kInvalidInput, "no output path was given");
   if (!options.overwrite && std::filesystem::exists(path)) {
-    Throw(ErrorCode::kIoError, "index already exists: " + path.string());
+    Throw(ErrorCode::kIoError, path.string() + " exists; saving over it needs the overwrite option");
   }
-  const auto temp = TemporaryPathFor(path);
+  const auto temp = StagingFileFor(path);
   try {
     const std::size_t header_size = kBaseHeaderSize + writers.size() * kSectionEntrySize;
     std::fstream out(temp, std::ios::binary | std::ios::in | std::ios::out | std::ios::trunc);
-    if (!out) Throw(ErrorCode::kIoError, "cannot create temporary index: " + temp.string());
+    if (!out) Throw(ErrorCode::kIoError, "staging file could not be opened: " + temp.string());
     std::vector<char> placeholder(header_size, '\0');
     out.write(placeholder.data(), static_cast<std::streamsize>(placeholder.size()));
     std::vector<SectionDescriptor> sections;
     std::set<SectionType> types;
     for (const auto& w : writers) {
-      if (!types.insert(w.type).second) Throw(ErrorCode::kBuildFailure, "duplicate index section type");
+      if (!types.insert(w.type).second) Throw(ErrorCode::kBuildFailure, "a section type was written twice");
       const auto begin = out.tellp();
       CrcBuffer crc(out.rdbuf());
       std::ostream section(&crc);
       w.write(section);
       section.flush();
-      if (!section || !crc.Finish() || !out) Throw(ErrorCode::kIoError, "failed while writing index section");
+      if (!section || !crc.Finish() || !out) Throw(ErrorCode::kIoError, "writing a section failed");
       const auto end = out.tellp();
       SectionDescriptor d;
       d.type = w.type;
@@ -534,9 +553,9 @@ void WriteContainer(const std::filesystem::path& path, const SaveOptions& option
     out.write(reinterpret_cast<const char*>(header.data()), static_cast<std::streamsize>(header.size()));
     out.flush();
     out.close();
-    if (!out) Throw(ErrorCode::kIoError, "failed to finalize temporary index");
+    if (!out) Throw(ErrorCode::kIoError, "closing the staging file failed");
     (void)ReadContainer(temp);  // self-validate before publication
-    Publish(temp, path, options.overwrite);
+    MoveIntoPlace(temp, path, options.overwrite);
   } catch (...) {
     std::error_code ignored;
     std::filesystem::remove(temp, ignored);
@@ -547,18 +566,18 @@ void WriteContainer(const std::filesystem::path& path, const SaveOptions& option
 ParsedContainer ReadContainer(const std::filesystem::path& path) {
   std::error_code ec;
   const auto file_size = std::filesystem::file_size(path, ec);
-  if (ec) Throw(ErrorCode::kIoError, "cannot stat index: " + path.string());
-  if (file_size < kBaseHeaderSize) Throw(ErrorCode::kCorruptIndex, "index is smaller than its fixed header");
+  if (ec) Throw(ErrorCode::kIoError, "index file cannot be examined: " + path.string());
+  if (file_size < kBaseHeaderSize) Throw(ErrorCode::kCorruptIndex, "file is shorter than an index header");
   std::ifstream in(path, std::ios::binary);
-  if (!in) Throw(ErrorCode::kIoError, "cannot open index: " + path.string());
+  if (!in) Throw(ErrorCode::kIoError, "index file could not be opened: " + path.string());
   std::vector<std::uint8_t> head(kBaseHeaderSize);
   ReadExact(in, head.data(), head.size(), "index header");
   if (!std::equal(kMagic.begin(), kMagic.end(), head.begin())) {
-    Throw(ErrorCode::kCorruptIndex, "index magic is invalid");
+    Throw(ErrorCode::kCorruptIndex, "file does not start with the index magic");
   }
   const auto major = GetU16(&head[8]);
   const auto minor = GetU16(&head[10]);
-  if (major != 1 || minor > 4) Throw(ErrorCode::kVersionMismatch, "unsupported sufidx format version");
+  if (major != 1 || minor > 4) Throw(ErrorCode::kVersionMismatch, "index format version is newer than 1.4");
   if (head[12] != 1) Throw(ErrorCode::kVersionMismatch, "unsupported index endianness");
   const std::uint32_t count = GetU32(&head[24]);
   const std::uint32_t header_size = GetU32(&head[28]);
@@ -603,7 +622,7 @@ ParsedContainer ReadContainer(const std::filesystem::path& path) {
     d.offset = GetU64(e + 8);
     d.size = GetU64(e + 16);
     d.crc32 = GetU32(e + 24);
-    if (!types.insert(GetU32(e)).second) Throw(ErrorCode::kCorruptIndex, "duplicate index section");
+    if (!types.insert(GetU32(e)).second) Throw(ErrorCode::kCorruptIndex, "two sections share a type");
     if (d.offset < header_size || d.offset > file_size || d.size > file_size - d.offset) {
       Throw(ErrorCode::kCorruptIndex, "index section lies outside the file");
     }
@@ -613,7 +632,7 @@ ParsedContainer ReadContainer(const std::filesystem::path& path) {
   std::sort(sorted.begin(), sorted.end(), [](const auto& a, const auto& b) { return a.offset < b.offset; });
   for (std::size_t i = 1; i < sorted.size(); ++i) {
     if (sorted[i - 1].offset + sorted[i - 1].size > sorted[i].offset) {
-      Throw(ErrorCode::kCorruptIndex, "index sections overlap");
+      Throw(ErrorCode::kCorruptIndex, "two sections share bytes");
     }
   }
   // Every section's CRC32.
@@ -633,7 +652,7 @@ std::uint32_t CrcFileRange(const std::filesystem::path& path, std::uint64_t offs
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const unsigned threads = static_cast<unsigned>(std::min<std::uint64_t>({pieces, hw, 16}));
   const int fd = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
-  if (fd < 0) Throw(ErrorCode::kIoError, "cannot open index: " + path.string());
+  if (fd < 0) Throw(ErrorCode::kIoError, "index file could not be opened: " + path.string());
   std::vector<std::uint32_t> crcs(pieces, 0);
   std::atomic<std::uint64_t> next{0};
   std::atomic<bool> failed{false};
@@ -737,18 +756,18 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
   using Clock = std::chrono::steady_clock;
   const auto begin = Clock::now();
   if (sections.empty() || sections.size() > kMaxSections) {
-    Throw(ErrorCode::kBuildFailure, "invalid number of index sections");
+    Throw(ErrorCode::kBuildFailure, "an index has 1 to 16 sections");
   }
-  if (path.empty()) Throw(ErrorCode::kInvalidInput, "index output path must not be empty");
+  if (path.empty()) Throw(ErrorCode::kInvalidInput, "no output path was given");
   if (!options.overwrite && std::filesystem::exists(path)) {
-    Throw(ErrorCode::kIoError, "index already exists: " + path.string());
+    Throw(ErrorCode::kIoError, path.string() + " exists; saving over it needs the overwrite option");
   }
   PlaneWriter writer(device);
-  const auto temp = TemporaryPathFor(path);
+  const auto temp = StagingFileFor(path);
   FileSink sink;
   try {
     sink.fd = ::open(temp.c_str(), O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
-    if (sink.fd < 0) Throw(ErrorCode::kIoError, "cannot create temporary index: " + temp.string());
+    if (sink.fd < 0) Throw(ErrorCode::kIoError, "staging file could not be opened: " + temp.string());
     const std::size_t header_size = kBaseHeaderSize + sections.size() * kSectionEntrySize;
     const std::vector<std::uint8_t> placeholder(header_size, 0);
     if (FileSink::Write(&sink, placeholder.data(), placeholder.size()) != 0) {
@@ -757,7 +776,7 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
     std::vector<SectionDescriptor> written;
     std::set<SectionType> types;
     for (const auto& section : sections) {
-      if (!types.insert(section.type).second) Throw(ErrorCode::kBuildFailure, "duplicate index section type");
+      if (!types.insert(section.type).second) Throw(ErrorCode::kBuildFailure, "a section type was written twice");
       SectionDescriptor d;
       d.type = section.type;
       d.flags = 1;
@@ -766,7 +785,7 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
       for (const auto& part : section.parts) {
         if (!part.on_device) {
           if (FileSink::Write(&sink, part.host.data(), part.host.size()) != 0) {
-            Throw(ErrorCode::kIoError, "failed while writing index section");
+            Throw(ErrorCode::kIoError, "writing a section failed");
           }
           crc = Crc32(crc, part.host.data(), part.host.size());
           continue;
@@ -788,7 +807,7 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
     }
     const int closed = ::close(sink.fd);
     sink.fd = -1;
-    if (closed != 0) Throw(ErrorCode::kIoError, "failed to finalize temporary index");
+    if (closed != 0) Throw(ErrorCode::kIoError, "closing the staging file failed");
     const auto validate = Clock::now();
     if (stats != nullptr) {
       stats->write_seconds = std::chrono::duration<double>(validate - begin).count();
@@ -796,7 +815,7 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
     }
     (void)ReadContainer(temp);  // self-validate before publication
     if (stats != nullptr) stats->validate_seconds = std::chrono::duration<double>(Clock::now() - validate).count();
-    Publish(temp, path, options.overwrite);
+    MoveIntoPlace(temp, path, options.overwrite);
   } catch (...) {
     if (sink.fd >= 0) ::close(sink.fd);
     std::error_code ignored;
@@ -808,7 +827,7 @@ void WriteContainerStreamed(const std::filesystem::path& path, const SaveOptions
 std::vector<std::uint8_t> ReadSection(const ParsedContainer& container, SectionType type) {
   const auto& d = container.Require(type);
   std::ifstream in(container.path, std::ios::binary);
-  if (!in) Throw(ErrorCode::kIoError, "cannot open index: " + container.path.string());
+  if (!in) Throw(ErrorCode::kIoError, "index file could not be opened: " + container.path.string());
   in.seekg(static_cast<std::streamoff>(d.offset));
   std::vector<std::uint8_t> bytes(static_cast<std::size_t>(d.size));
   if (d.size) ReadExact(in, bytes.data(), bytes.size(), "index section");

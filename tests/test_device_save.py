@@ -239,3 +239,39 @@ def test_device_save_of_a_low_memory_index(tmp_path, hip, construction, storage)
     back = api.SuffixArray.load(dev)
     assert back.info()["lcp_encoding"] == 2 and back.isa() is None
     assert np.array_equal(back.lcp(), index.lcp())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fault", ["lcp0", "anchor", "isa"])
+def test_low_memory_device_save_rejects_bad_arrays(tmp_path, hip, fault):
+    """The Low-memory save checks what BuildByteCoded checks before coding
+    (lcp_storage.cpp:237-297): LCP[0] == 0, an ISA inverse to the SA, anchor
+    values within the anchor coordinates; a failure throws kBuildFailure and
+    publishes no file.  The arrays are the oracle's, altered on the CPU."""
+    import oracle
+    from sufkit_amd import DeviceArray
+
+    ref = api.GenomeReference.from_records([("a", "ACGTTGCA" * 300 + "A" * 700), ("b", "TTGACCA" * 200)])
+    text = ref.text()
+    n = int(text.size)
+    sa, isa, lcp = oracle.fast_build(text)
+    lcp = lcp.astype(np.uint64)
+    if fault == "lcp0":
+        lcp[0] = 3
+    elif fault == "anchor":
+        # a run of overflow values beyond 32-bit anchors (n < 2^32)
+        lcp[int(isa[5])] = (1 << 33) + 7
+    else:
+        isa = isa.copy()
+        isa[[1, 2]] = isa[[2, 1]]
+    arrays = [DeviceArray(n + 64)] + [DeviceArray(n * 8) for _ in range(3)]
+    arrays[0].upload(text)
+    for d, a in zip(arrays[1:], (sa, isa, lcp)):
+        d.upload(np.ascontiguousarray(a, dtype=np.uint64))
+    out = tmp_path / "bad.sufidx"
+    opts = api.BuildOptions(backend=api.BACKEND_HIP, resource_profile=1)
+    with pytest.raises(SufkitError) as e:
+        api.save_device_arrays(ref, out, arrays[0].ptr, arrays[1].ptr, 64, arrays[2].ptr, 64, arrays[3].ptr, 64,
+                               32, opts)
+    assert e.value.status == 6  # kBuildFailure
+    assert not out.exists()

@@ -164,3 +164,40 @@ def test_invalid_arguments(hip):
     with pytest.raises(sufkit_amd.SufkitError) as e:
         sufkit_amd.check_index(0, 10, 0, 32, 0, 32)
     assert e.value.status == 1
+
+
+def test_allocation_failure_is_a_build_failure_and_recoverable(hip):
+    # An out-of-memory device allocation in the middle of a build surfaces as
+    # ErrorCode::kBuildFailure (as a failed CaPS allocation does,
+    # caps_backend.cpp:68-76); the same builder then builds correctly.
+    import ctypes
+    from sufkit_amd import api
+
+    lib = sufkit_amd.load_hip()
+    lib.sfk_hip_inject_allocation_failure.argtypes = [ctypes.c_longlong]
+    lib.sfk_hip_inject_allocation_failure.restype = None
+    text = synth.repeat_text(41, 50000, cover=0.5, n_runs=3, contigs=2)
+    n = text.size
+    d_text = sufkit_amd.DeviceArray(n + 64)
+    d_text.upload(text)
+    outs = [sufkit_amd.DeviceArray(n * 4) for _ in range(3)]
+    osa, oisa, olcp = oracle.fast_build(text)
+    with sufkit_amd.DeviceBuilder(0) as b:
+        for after in (0, 3, 7):
+            lib.sfk_hip_inject_allocation_failure(after)
+            with pytest.raises(sufkit_amd.SufkitError) as e:
+                b.build(d_text.ptr, n, 32, outs[0].ptr, 32, outs[1].ptr, 32, outs[2].ptr, 32)
+            assert e.value.status == BUILD_FAILURE and "allocation" in str(e.value)
+        lib.sfk_hip_inject_allocation_failure(-1)
+        b.build(d_text.ptr, n, 32, outs[0].ptr, 32, outs[1].ptr, 32, outs[2].ptr, 32)
+    assert np.array_equal(outs[0].download(np.uint32, n).astype(np.uint64), osa)
+    assert np.array_equal(outs[2].download(np.uint32, n).astype(np.uint64), olcp)
+    # the host Build maps it to ErrorCode::kBuildFailure
+    ref = api.GenomeReference.from_encoded(text, synth.contig_lengths(text))
+    lib.sfk_hip_inject_allocation_failure(1)
+    try:
+        with pytest.raises(sufkit_amd.SufkitError) as e:
+            api.SuffixArray.build(ref, api.BuildOptions(backend=api.BACKEND_HIP))
+        assert e.value.code == "build_failure"
+    finally:
+        lib.sfk_hip_inject_allocation_failure(-1)

@@ -194,3 +194,19 @@ def test_runs_of_one_base_are_counted(hip, fm):
     got = hip.index_count(text, sa, pats, fm=fm)
     for p, g in zip(pats, got):
         assert g == naive_count(s, p), (p, g)
+
+
+def test_strict_budget_refuses_an_oversized_request(hip):
+    # Under strict_budget a request whose footprint alone exceeds the HBM
+    # budget is refused with ErrorCode::kInvalidInput (status 1) and a
+    # message naming the budget; the other requests are served.
+    small = [synth.random_text(80 + i, 20_000) for i in range(3)]
+    big = synth.random_text(90, 3_000_000)
+    texts = [small[0], big, small[1], small[2]]
+    budget = 64 << 20  # holds a 20 k request, not a 3 M one (about 100 bytes per symbol)
+    res, summ, _ = hip.serve(texts, [0.0, 0.0005, 0.001, 0.0015], slots=2, queries_per_index=1000,
+                             hbm_budget_bytes=budget, strict_budget=True)
+    assert [r["status"] for r in res] == [0, 1, 0, 0]
+    assert summ["requests_failed"] == 1
+    msg = summ["first_error"].decode()
+    assert msg.startswith("request 1 (status 1)") and "exceeds the HBM budget" in msg

@@ -57,10 +57,10 @@ def test_lowercase_and_iupac_normalize_like_oracle():
 
 
 @pytest.mark.parametrize("records,needle", [
-    ([], "at least one"),
-    ([("a", b"ACGT"), ("a", b"GG")], "duplicate"),
-    ([("", b"ACGT")], "name must not be empty"),
-    ([("a", b"")], "must not be empty"),
+    ([], "one record or more"),
+    ([("a", b"ACGT"), ("a", b"GG")], "share the name"),
+    ([("", b"ACGT")], "has no name"),
+    ([("a", b"")], "holds no bases"),
 ])
 def test_reference_validation_errors(records, needle):
     with pytest.raises(SufkitError) as e:
