@@ -104,7 +104,11 @@ def radix_tuning(args, max_elements: int):
 
     sufkit_amd.tune_reset()
     if args.load_tuning:
-        table = json.loads(Path(args.load_tuning).read_text())
+        raw = Path(args.load_tuning).read_text()
+        if args.load_tuning.endswith(".md"):
+            # a profile summary holding the table in a ```json block
+            raw = raw.split("```json", 1)[1].split("```", 1)[0]
+        table = json.loads(raw)
         sufkit_amd.tune_set(table)
         source, seconds = "loaded", 0.0
     elif args.no_autotune:

@@ -1,0 +1,311 @@
+// LCP of a complete suffix array on the device: Φ/PLCP (Kärkkäinen,
+// Manzini and Puglisi, "Permuted longest-common-prefix array", CPM 2009)
+// with 64-bit XOR+clz window compares, then LCP[r] = PLCP[SA[r]].
+//
+//   BucketStarts   the suffixes that start a first-symbol bucket (at most 7
+//                  per reference, found by a binary search per reference
+//                  and symbol over the SA), marked in a bitmap over text
+//                  positions.  Their PLCP is 0: the predecessor starts with
+//                  another symbol (or there is none: the sentinel).
+//   Plcp           text positions in order, one tile of consecutive
+//                  positions per block.  Φ(i) = SA[ISA[i] - 1] is gathered
+//                  (one random SA line, which also holds SA[ISA[i]] for the
+//                  inverse check); the block shares its Φ values in LDS.
+//                  Position i is reducible when Φ(i-1) + 1 == Φ(i) and
+//                  i - 1 does not start a bucket: then the suffixes i-1 and
+//                  Φ(i) - 1 are adjacent and start with the same symbol, so
+//                  T[i-1] == T[Φ(i)-1] and PLCP[i] = PLCP[i-1] - 1.  This is
+//                  Kärkkäinen's reducibility test decided without reading the
+//                  text at random; only irreducible positions compare
+//                  32-symbol windows at i and Φ(i).
+//   PlcpScan       PLCP[i] + i never decreases along the text, so the
+//                  inclusive max over irreducible j <= i of PLCP[j] + j is
+//                  PLCP[i] + i at every i (three phases over chunks).
+//   LcpGather      LCP[r] = PLCP[SA[r]].
+//
+// Random writes cost far more than random reads on this memory system (a
+// Φ built by scattering Φ[SA[r]] = SA[r-1] took as long as the whole PLCP
+// pass), so every random access here is a read.  PLCP values are 32-bit:
+// every text the backend takes has fewer than 2^32 symbols.  The same pass
+// checks ISA[i] < n and SA[ISA[i]] == i for every i, which makes ISA a
+// bijection and SA its inverse (the permutation check of RepackSuffixArray,
+// suffix_array.cpp:852-881).
+#pragma once
+
+#include "block_scan.hpp"
+#include "device_common.hpp"
+#include "sa_kernels.hpp"
+
+namespace sfk {
+
+constexpr std::uint32_t kPlcpReducible = 0xFFFFFFFFu;
+constexpr int kPlcpThreads = 256;
+
+struct LongLce {
+  unsigned long long i;
+  unsigned long long j;
+  unsigned long long l;
+};
+
+template <int kBlock>
+__device__ __forceinline__ void BlockAddTo(std::uint64_t v, std::uint64_t* scratch, unsigned long long* out) {
+  std::uint64_t total;
+  BlockExclusiveSum<kBlock>(v, scratch, total);
+  if (threadIdx.x == 0 && total != 0) atomicAdd(out, static_cast<unsigned long long>(total));
+}
+
+__device__ __forceinline__ bool Bit(const std::uint32_t* bits, std::uint64_t p) {
+  return (bits[p >> 5] >> (p & 31)) & 1u;
+}
+
+// One thread per (reference, symbol 0..6): the first row of the reference's
+// row block whose suffix starts with a symbol >= c starts a bucket (c = 0:
+// the block's first row, its sentinel); its suffix gets bit SA[row] in the
+// position bitmap.  refs == 0: one text of n symbols.
+template <typename Index>
+__global__ __launch_bounds__(64) void BucketStartsKernel(const std::uint8_t* __restrict__ text,
+                                                         const Index* __restrict__ sa, std::uint64_t n,
+                                                         const std::uint64_t* __restrict__ offsets,
+                                                         std::uint32_t refs, std::uint32_t* __restrict__ bits) {
+  const std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t blocks = refs == 0 ? 1 : refs;
+  if (t >= blocks * 7) return;
+  const std::uint64_t ref = t / 7;
+  const std::uint32_t c = static_cast<std::uint32_t>(t % 7);
+  const std::uint64_t lo0 = refs == 0 ? 0 : offsets[ref];
+  const std::uint64_t hi0 = refs == 0 ? n : offsets[ref + 1];
+  std::uint64_t lo = lo0;
+  if (c > 0) {
+    std::uint64_t hi = hi0;
+    while (lo < hi) {
+      const std::uint64_t mid = lo + (hi - lo) / 2;
+      const std::uint64_t p = static_cast<std::uint64_t>(sa[mid]);
+      const std::uint32_t sym = p < n ? text[p] : 0xFFu;  // an out-of-range SA value fails the checks later
+      if (sym >= c) {
+        hi = mid;
+      } else {
+        lo = mid + 1;
+      }
+    }
+  }
+  if (lo < hi0) {
+    const std::uint64_t p = static_cast<std::uint64_t>(sa[lo]);
+    if (p < n) atomicOr(&bits[p >> 5], 1u << (p & 31));
+  }
+}
+
+// Φ(i), or kPhiNone when suffix i starts a bucket; flags a row outside the
+// text or an SA that does not invert the ISA.
+constexpr std::uint64_t kPhiNone = ~0ull;
+
+template <typename Index>
+__device__ __forceinline__ std::uint64_t PhiOf(const Index* __restrict__ sa, const Index* __restrict__ isa,
+                                               const std::uint32_t* __restrict__ starts, std::uint64_t n,
+                                               std::uint64_t i, bool& bad) {
+  const std::uint64_t r = static_cast<std::uint64_t>(__builtin_nontemporal_load(&isa[i]));
+  if (r >= n) {
+    bad = true;
+    return kPhiNone;
+  }
+  const std::uint64_t self = static_cast<std::uint64_t>(sa[r]);
+  if (self != i) bad = true;
+  if (Bit(starts, i) || r == 0) return kPhiNone;
+  const std::uint64_t j = static_cast<std::uint64_t>(sa[r - 1]);
+  if (j >= n) {
+    bad = true;
+    return kPhiNone;
+  }
+  return j;
+}
+
+// PLCP of irreducible positions (kPlcpReducible at reducible ones) with the
+// permutation/inverse check.  LCEs longer than word_limit windows go to the
+// long queue.
+template <typename Index>
+__global__ __launch_bounds__(kPlcpThreads) void PlcpKernel(PackedText t, const Index* __restrict__ sa,
+                                                           const Index* __restrict__ isa,
+                                                           const std::uint32_t* __restrict__ starts, std::uint64_t n,
+                                                           std::uint32_t word_limit, std::uint32_t* __restrict__ plcp,
+                                                           LongLce* __restrict__ queue, std::uint64_t queue_capacity,
+                                                           DeviceCounters* counters) {
+  __shared__ std::uint64_t s_phi[kPlcpThreads + 1];
+  __shared__ std::uint64_t scratch[kPlcpThreads / kWave];
+  const std::uint64_t tiles = (n + kPlcpThreads - 1) / kPlcpThreads;
+  std::uint64_t irreducible = 0;
+  bool bad = false;
+  for (std::uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const std::uint64_t i = tile * kPlcpThreads + threadIdx.x;
+    std::uint64_t phi = kPhiNone;
+    if (i < n) phi = PhiOf(sa, isa, starts, n, i, bad);
+    s_phi[threadIdx.x + 1] = phi;
+    if (threadIdx.x == 0) {
+      bool ignored = false;  // position i - 1 is checked by its own thread
+      s_phi[0] = i > 0 && i - 1 < n ? PhiOf(sa, isa, starts, n, i - 1, ignored) : kPhiNone;
+    }
+    __syncthreads();
+    const std::uint64_t before = s_phi[threadIdx.x];
+    __syncthreads();
+    if (i >= n) continue;
+    if (phi == kPhiNone) {
+      plcp[i] = 0;
+      ++irreducible;
+      continue;
+    }
+    if (before != kPhiNone && before + 1 == phi) {
+      plcp[i] = kPlcpReducible;
+      continue;
+    }
+    ++irreducible;
+    std::uint64_t l = 0;
+    bool done = false;
+    for (std::uint32_t w = 0; w < word_limit; ++w) {
+      std::uint64_t ca, cb;
+      std::uint32_t sa_, sb;
+      LoadWindow(t, i + l, ca, sa_);
+      LoadWindow(t, phi + l, cb, sb);
+      const std::uint32_t mm = WindowMismatch(ca, sa_, cb, sb);
+      l += mm;
+      if (mm < 32) {
+        done = true;
+        break;
+      }
+    }
+    if (done) {
+      plcp[i] = static_cast<std::uint32_t>(l);
+      continue;
+    }
+    const unsigned long long slot = atomicAdd(&counters->long_items, 1ull);
+    if (slot < queue_capacity) {
+      queue[slot].i = i;
+      queue[slot].j = phi;
+      queue[slot].l = l;
+    }
+    plcp[i] = 0;  // the long-LCE pass writes the value
+  }
+  if (bad) atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotInverse));
+  BlockAddTo<kPlcpThreads>(irreducible, scratch, &counters->irreducible);
+}
+
+// One wave per queued long LCE: 64 windows (2048 symbols) compared per step.
+__global__ __launch_bounds__(256) void PlcpLongKernel(PackedText t, std::uint64_t n,
+                                                      const LongLce* __restrict__ queue, std::uint64_t count,
+                                                      std::uint32_t* __restrict__ plcp, DeviceCounters* counters) {
+  const std::uint64_t item = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  if (item >= count) return;  // wave-uniform
+  const std::uint32_t lane = LaneId();
+  const std::uint64_t i = queue[item].i;
+  const std::uint64_t j = queue[item].j;
+  std::uint64_t l = queue[item].l;
+  unsigned long long steps = 0;
+  while (true) {
+    const std::uint64_t pa = i + l + 32ull * lane;
+    const std::uint64_t pb = j + l + 32ull * lane;
+    std::uint32_t mm = 0;
+    if (pa < n && pb < n) {
+      std::uint64_t ca, cb;
+      std::uint32_t sa_, sb;
+      LoadWindow(t, pa, ca, sa_);
+      LoadWindow(t, pb, cb, sb);
+      mm = WindowMismatch(ca, sa_, cb, sb);
+    }
+    const std::uint64_t vote = __ballot(mm < 32);
+    ++steps;
+    if (vote) {
+      const int f = __ffsll(static_cast<unsigned long long>(vote)) - 1;
+      const std::uint32_t mf = __shfl(mm, f, kWave);
+      l += 32ull * f + mf;
+      break;
+    }
+    l += 32ull * kWave;
+  }
+  if (lane == 0) {
+    plcp[i] = static_cast<std::uint32_t>(l);
+    atomicAdd(&counters->long_words, steps * kWave);
+  }
+}
+
+// The scan key of position k: PLCP[k] + k + 1 at irreducible positions, 0
+// at reducible ones (every key is at least 1 where it counts, so 0 is the
+// identity of the max).
+__device__ __forceinline__ std::uint64_t PlcpKey(std::uint32_t v, std::uint64_t k) {
+  return v == kPlcpReducible ? 0ull : static_cast<std::uint64_t>(v) + k + 1;
+}
+
+__global__ __launch_bounds__(256) void PlcpScanReduceKernel(const std::uint32_t* __restrict__ plcp,
+                                                            std::uint64_t n, std::uint64_t chunk,
+                                                            std::uint64_t* __restrict__ partial) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
+  std::uint64_t best = 0;
+  for (std::uint64_t k = begin + threadIdx.x; k < end; k += kBlock) {
+    const std::uint64_t v = PlcpKey(plcp[k], k);
+    best = v > best ? v : best;
+  }
+  std::uint64_t total;
+  BlockExclusiveMax<kBlock>(best, scratch, total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void PlcpScanSpineKernel(std::uint64_t* partial, int grid) {
+  constexpr int kBlock = 1024;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  std::uint64_t total;
+  const std::uint64_t excl = BlockExclusiveMax<kBlock>(valid ? partial[threadIdx.x] : 0ull, scratch, total);
+  if (valid) partial[threadIdx.x] = excl;
+}
+
+// Inclusive max-scan of the keys, written back as PLCP[k] = scan - k - 1.
+template <int kItems>
+__global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __restrict__ plcp, std::uint64_t n,
+                                                           std::uint64_t chunk,
+                                                           const std::uint64_t* __restrict__ partial) {
+  constexpr int kBlock = 256;
+  constexpr int kTile = kBlock * kItems;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
+  std::uint64_t run = partial[blockIdx.x];
+  for (std::uint64_t base = begin; base < end; base += kTile) {
+    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
+    std::uint64_t keys[kItems];
+    std::uint64_t local = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const std::uint64_t k = first + i;
+      keys[i] = k < end ? PlcpKey(plcp[k], k) : 0ull;
+      local = keys[i] > local ? keys[i] : local;
+    }
+    std::uint64_t total;
+    std::uint64_t cur = BlockExclusiveMax<kBlock>(local, scratch, total);
+    if (run > cur) cur = run;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const std::uint64_t k = first + i;
+      if (k >= end) break;
+      cur = keys[i] > cur ? keys[i] : cur;
+      plcp[k] = static_cast<std::uint32_t>(cur - k - 1);
+    }
+    if (total > run) run = total;
+  }
+}
+
+// LCP[r] = PLCP[SA[r]] in the requested output width; LCP[0] = 0.
+template <typename Index, typename Out>
+__global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __restrict__ plcp,
+                                                       const Index* __restrict__ sa, std::uint64_t n,
+                                                       Out* __restrict__ lcp) {
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * 256;
+  for (std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x; r < n; r += stride) {
+    // SA and LCP stream through once: non-temporal, so that they do not
+    // evict the randomly read PLCP lines.  An out-of-range SA value (the
+    // checks have flagged it) reads nothing.
+    const std::uint64_t v = static_cast<std::uint64_t>(__builtin_nontemporal_load(&sa[r]));
+    const Out out = (r == 0 || v >= n) ? Out(0) : static_cast<Out>(plcp[v]);
+    __builtin_nontemporal_store(out, &lcp[r]);
+  }
+}
+
+}  // namespace sfk

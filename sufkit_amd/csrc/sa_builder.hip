@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -33,6 +34,7 @@
 #include "query_kernels.hpp"
 #include "radix_sort.hpp"
 #include "sa_kernels.hpp"
+#include "lcp_kernels.hpp"
 #include "sample_kernels.hpp"
 #include "fm_kernels.hpp"  // after sa_kernels.hpp: uses its error flags
 #include "sufkit_hip.h"
@@ -847,6 +849,20 @@ class Doubling {
     ctx_.stats.refine_steps += 1;
     ctx_.stats.active_total += qm_;
     ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * w + (kExact ? 32 : w) + 2 * w + 2 * w);
+    StepLog(kExact ? "local-exact" : "local", qm_, 0, 0);
+  }
+
+  // Diagnostics: with SFK_HIP_STEP_LOG set, one line per refine step on
+  // stderr (kind, suffixes, segments, key bits).
+  static bool StepLogOn() {
+    static const bool on = std::getenv("SFK_HIP_STEP_LOG") != nullptr;
+    return on;
+  }
+  void StepLog(const char* kind, std::uint64_t m, std::uint64_t segs, int key_bits) const {
+    if (!StepLogOn()) return;
+    std::fprintf(stderr, "sfk step %-12s m=%llu segs=%llu key_bits=%d small=%llu\n", kind,
+                 static_cast<unsigned long long>(m), static_cast<unsigned long long>(segs), key_bits,
+                 static_cast<unsigned long long>(qm_));
   }
 
   // readback: read the new list's sizes back right after RefineApply (the
@@ -862,6 +878,8 @@ class Doubling {
     }
     ctx_.stats.refine_steps += 1;
     ctx_.stats.active_total += lm_;
+    static const char* const kKindNames[] = {"fill-hi", "fill-lo", "exact0", "exact1", "exact2", "rank"};
+    StepLog(kKindNames[sec_kind], lm_, l_segs_, total_bits);
     if (total_bits <= 32) {
       GlobalStepTyped<std::uint32_t>(sec_kind, sec_bits, seg_bits, h, iota, local_between, survivor_rank);
     } else {
@@ -993,45 +1011,64 @@ class Doubling {
   Index* q_row0_next_ = nullptr;
 };
 
+// Grid of a grid-stride kernel over m items: enough blocks to fill the
+// device, few enough that each thread takes several items.
+unsigned StrideGrid(std::uint64_t m) {
+  return static_cast<unsigned>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, 256), 1u << 16)));
+}
+
+// LCP pass (lcp_kernels.hpp): bucket-start suffixes, PLCP of irreducible
+// positions (with the permutation/inverse check; long LCEs by whole waves),
+// PLCP max-scan, LCP gather.  keys0 holds PLCP (32-bit), keys1 the
+// bucket-start position bitmap and the long-LCE queue.
 template <typename Index, typename Out>
-void LaunchLcp(BuildContext& ctx, PackedText text, const Index* sa, const Index* isa,
+void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const Index* sa, const Index* isa,
                Out* lcp) {
   sfk_hip_builder* b = ctx.b;
   const std::uint64_t n = ctx.n;
   DeviceCounters* dc = b->counters.As<DeviceCounters>();
-  Index* val = b->keys0.As<Index>();
-  LongLce* queue = b->keys1.As<LongLce>();
-  const std::uint64_t capacity = b->keys1.Bytes() / sizeof(LongLce);
+  std::uint32_t* plcp = b->keys0.As<std::uint32_t>();
+  if (b->keys0.Bytes() < n * 4) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace smaller than the PLCP");
+  std::uint32_t* starts = b->keys1.As<std::uint32_t>();
+  const std::uint64_t bit_words = n / 32 + 1;
+  const std::uint64_t bit_bytes = (bit_words * 4 + 63) / 64 * 64;
+  if (b->keys1.Bytes() < bit_bytes) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the bitmap");
+  LongLce* queue = reinterpret_cast<LongLce*>(b->keys1.As<char>() + bit_bytes);
+  const std::uint64_t capacity = (b->keys1.Bytes() - bit_bytes) / sizeof(LongLce);
   constexpr std::uint64_t w = sizeof(Index);
   constexpr std::uint32_t kWordLimit = 8;
-  PlcpIrreducibleKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
-      text, sa, isa, n, kWordLimit, val, queue, capacity, dc);
+  SFK_HIP_CHECK(hipMemsetAsync(starts, 0, bit_words * sizeof(std::uint32_t), b->stream));
+  const std::uint64_t bucket_threads = 7ull * (ctx.refs > 0 ? ctx.refs : 1);
+  BucketStartsKernel<Index><<<static_cast<unsigned>(CeilDiv(bucket_threads, 64)), 64, 0, b->stream>>>(
+      d_text, sa, n, ctx.d_offsets, ctx.refs, starts);
+  PlcpKernel<Index><<<StrideGrid(n), kPlcpThreads, 0, b->stream>>>(text, sa, isa, starts, n, kWordLimit, plcp, queue,
+                                                                   capacity, dc);
   SFK_HIP_CHECK(hipGetLastError());
   const DeviceCounters& c = ReadCounters(b);
   if (c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) {
     Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
   }
   const std::uint64_t items = c.long_items;
-  if (items > capacity) {
-    Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
-  }
+  const std::uint64_t irreducible = c.irreducible;
+  if (items > capacity) Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
   if (items > 0) {
-    PlcpLongKernel<Index><<<static_cast<unsigned>(CeilDiv(items * kWave, 256)), 256, 0, b->stream>>>(
-        text, n, queue, items, val, dc);
+    PlcpLongKernel<<<static_cast<unsigned>(CeilDiv(items * kWave, 256)), 256, 0, b->stream>>>(text, n, queue, items,
+                                                                                              plcp, dc);
     SFK_HIP_CHECK(hipGetLastError());
   }
   const SortGeometry g = Geometry(n, 256 * kScanItems, 1024);
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
-  MaxReduceKernel<Index><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
-  MaxSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
-  MaxScanToPlcpKernel<Index, kScanItems><<<g.grid, 256, 0, b->stream>>>(val, n, g.chunk, partial);
-  LcpPermuteKernel<Index, Out><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
-      val, sa, n, lcp);
+  PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
+  PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
+  PlcpScanApplyKernel<kScanItems><<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
+  LcpGatherKernel<Index, Out><<<StrideGrid(n), 256, 0, b->stream>>>(plcp, sa, n, lcp);
   SFK_HIP_CHECK(hipGetLastError());
   ctx.stats.algorithmic_bytes_lcp +=
-      n * (w + w + 8 + 24 + w)        // ISA, SA[r-1], two symbols, first LCE windows, val
-      + n * (3 * w)                   // max-scan: two reads, one write
-      + n * (w + w + sizeof(Out));    // SA, PLCP[SA], LCP
+      n / 8                            // bucket-start bitmap
+      + n * (w + w + 4) + n / 2        // PLCP: ISA read, SA[ISA[i] - 1] (with SA[ISA[i]]), PLCP write, text at i
+      + irreducible * 32               // the window pair at Φ(i) of each irreducible position
+      + n * (3 * 4)                    // max-scan: two reads, one write
+      + n * (w + 4 + sizeof(Out));     // LCP = PLCP[SA]: SA, PLCP, LCP
 }
 
 template <typename In, typename Out>
@@ -1159,11 +1196,11 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     if (sampled) {
       // The complete LCP into the workspace (vals0 is free after the
       // doubling), for the interval minima of the sampled rows.
-      LaunchLcp<Index, Index>(ctx, text, sa, rank, b->vals0.As<Index>());
+      LaunchLcp<Index, Index>(ctx, d_text, text, sa, rank, b->vals0.As<Index>());
     } else if (lcp_bits == 32) {
-      LaunchLcp<Index, std::uint32_t>(ctx, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
+      LaunchLcp<Index, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
     } else {
-      LaunchLcp<Index, std::uint64_t>(ctx, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
+      LaunchLcp<Index, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
     }
   }
   RecordPhase(b, 3);

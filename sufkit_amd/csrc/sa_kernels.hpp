@@ -77,6 +77,7 @@ struct DeviceCounters {
   unsigned long long q_next;      // suffixes appended to the next small list
   unsigned long long l_next;      // suffixes in the next large list
   unsigned long long l_next_segs; // segments in the next large list
+  unsigned long long irreducible;  // PLCP positions computed directly
   unsigned int error_flags;
   unsigned int pad;
 };
@@ -1532,194 +1533,6 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
       out_row0[at] = head[i];
     }
   }
-}
-
-// ------------------------------------------------------------ LCP ---------
-
-struct LongLce {
-  unsigned long long i;
-  unsigned long long j;
-  unsigned long long l;
-};
-
-// PLCP by irreducible positions.  Position i (with phi = SA[ISA[i]-1]) is
-// reducible when T[i-1] == T[phi-1]; then PLCP[i] = PLCP[i-1] - 1.  For every
-// other position the LCE is computed directly with 32-symbol word compares.
-// Output val[i] = PLCP[i] + i for irreducible i, 0 for reducible i; the
-// inclusive max-scan of val then gives PLCP[i] + i everywhere, because
-// PLCP[i] + i never decreases along the text.
-template <typename Index>
-__global__ __launch_bounds__(256) void PlcpIrreducibleKernel(
-    PackedText t, const Index* __restrict__ sa, const Index* __restrict__ isa,
-    std::uint64_t n, std::uint32_t word_limit, Index* __restrict__ val,
-    LongLce* __restrict__ queue, std::uint64_t queue_capacity,
-    DeviceCounters* counters) {
-  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  // ISA streams through once (non-temporal: it should not evict the SA and
-  // text lines read at random).
-  const std::uint64_t r = static_cast<std::uint64_t>(__builtin_nontemporal_load(&isa[i]));
-  // Validation fused here: ISA[i] < n and SA[ISA[i]] == i for every i makes
-  // ISA a bijection and SA its inverse (the permutation check of the SA).
-  if (r >= n || static_cast<std::uint64_t>(sa[r]) != i) {
-    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotInverse));
-    val[i] = 0;
-    return;
-  }
-  if (SymbolAt(t, i) == 4u) {  // the sentinel: first row of its reference, LCP 0
-    val[i] = static_cast<Index>(i);
-    return;
-  }
-  const std::uint64_t j = sa[r - 1];
-  if (j >= n) {  // only possible for a corrupted SA; keeps every load in bounds
-    atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotPermutation));
-    val[i] = 0;
-    return;
-  }
-  if (i > 0 && j > 0 && SymbolAt(t, i - 1) == SymbolAt(t, j - 1)) {
-    val[i] = 0;
-    return;
-  }
-  std::uint64_t l = 0;
-  for (std::uint32_t w = 0; w < word_limit; ++w) {
-    std::uint64_t ca, cb;
-    std::uint32_t sa_, sb;
-    LoadWindow(t, i + l, ca, sa_);
-    LoadWindow(t, j + l, cb, sb);
-    const std::uint32_t mm = WindowMismatch(ca, sa_, cb, sb);
-    l += mm;
-    if (mm < 32) {
-      val[i] = static_cast<Index>(l + i);
-      return;
-    }
-  }
-  const unsigned long long slot = atomicAdd(&counters->long_items, 1ull);
-  if (slot < queue_capacity) {
-    queue[slot].i = i;
-    queue[slot].j = j;
-    queue[slot].l = l;
-  }
-  val[i] = 0;
-}
-
-// One wave per queued long LCE: 64 windows (2048 symbols) compared per step.
-template <typename Index>
-__global__ __launch_bounds__(256) void PlcpLongKernel(
-    PackedText t, std::uint64_t n, const LongLce* __restrict__ queue,
-    std::uint64_t count, Index* __restrict__ val, DeviceCounters* counters) {
-  const std::uint64_t item =
-      (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  if (item >= count) return;  // wave-uniform
-  const std::uint32_t lane = LaneId();
-  const std::uint64_t i = queue[item].i;
-  const std::uint64_t j = queue[item].j;
-  std::uint64_t l = queue[item].l;
-  unsigned long long steps = 0;
-  while (true) {
-    const std::uint64_t pa = i + l + 32ull * lane;
-    const std::uint64_t pb = j + l + 32ull * lane;
-    std::uint32_t mm = 0;
-    if (pa < n && pb < n) {
-      std::uint64_t ca, cb;
-      std::uint32_t sa_, sb;
-      LoadWindow(t, pa, ca, sa_);
-      LoadWindow(t, pb, cb, sb);
-      mm = WindowMismatch(ca, sa_, cb, sb);
-    }
-    const std::uint64_t vote = __ballot(mm < 32);
-    ++steps;
-    if (vote) {
-      const int f = __ffsll(static_cast<unsigned long long>(vote)) - 1;
-      const std::uint32_t mf = __shfl(mm, f, kWave);
-      l += 32ull * f + mf;
-      break;
-    }
-    l += 32ull * kWave;
-  }
-  if (lane == 0) {
-    val[i] = static_cast<Index>(l + i);
-    atomicAdd(&counters->long_words, steps * kWave);
-  }
-}
-
-// Inclusive max-scan, reduce phase: per-block maximum of its chunk.
-template <typename Index>
-__global__ __launch_bounds__(256) void MaxReduceKernel(
-    const Index* __restrict__ in, std::uint64_t m, std::uint64_t chunk,
-    std::uint64_t* __restrict__ partial) {
-  constexpr int kBlock = 256;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
-  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
-  std::uint64_t best = 0;
-  for (std::uint64_t k = begin + threadIdx.x; k < end; k += kBlock) {
-    const std::uint64_t v = in[k];
-    best = v > best ? v : best;
-  }
-  std::uint64_t total;
-  BlockExclusiveMax<kBlock>(best, scratch, total);
-  if (threadIdx.x == 0) partial[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(1024) void MaxSpineKernel(std::uint64_t* partial, int grid) {
-  constexpr int kBlock = 1024;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  const bool valid = static_cast<int>(threadIdx.x) < grid;
-  std::uint64_t total;
-  const std::uint64_t excl =
-      BlockExclusiveMax<kBlock>(valid ? partial[threadIdx.x] : 0ull, scratch, total);
-  if (valid) partial[threadIdx.x] = excl;
-}
-
-// Inclusive max-scan apply phase; each thread owns kItems consecutive items,
-// and the result is turned into PLCP in place: out[i] = scan[i] - i.
-template <typename Index, int kItems>
-__global__ __launch_bounds__(256) void MaxScanToPlcpKernel(
-    Index* __restrict__ data, std::uint64_t m, std::uint64_t chunk,
-    const std::uint64_t* __restrict__ partial) {
-  constexpr int kBlock = 256;
-  constexpr int kTile = kBlock * kItems;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
-  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
-  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
-  std::uint64_t run = partial[blockIdx.x];
-  for (std::uint64_t base = begin; base < end; base += kTile) {
-    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
-    std::uint64_t vals[kItems];
-    std::uint64_t local = 0;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      vals[i] = k < end ? static_cast<std::uint64_t>(data[k]) : 0ull;
-      local = vals[i] > local ? vals[i] : local;
-    }
-    std::uint64_t total;
-    std::uint64_t cur = BlockExclusiveMax<kBlock>(local, scratch, total);
-    if (run > cur) cur = run;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      if (k >= end) break;
-      cur = vals[i] > cur ? vals[i] : cur;
-      data[k] = static_cast<Index>(cur - k);
-    }
-    if (total > run) run = total;
-  }
-}
-
-// LCP[r] = PLCP[SA[r]], written in the requested output width.
-template <typename Index, typename Out>
-__global__ __launch_bounds__(256) void LcpPermuteKernel(
-    const Index* __restrict__ plcp, const Index* __restrict__ sa, std::uint64_t n,
-    Out* __restrict__ lcp) {
-  const std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  // SA and LCP stream through once: non-temporal, so that they do not evict
-  // the randomly read PLCP lines from the caches.  A row whose SA value is
-  // out of range (the validation has flagged it) reads nothing.
-  const std::uint64_t v = static_cast<std::uint64_t>(__builtin_nontemporal_load(&sa[r]));
-  const Out out = (r == 0 || v >= n) ? Out(0) : static_cast<Out>(plcp[v]);
-  __builtin_nontemporal_store(out, &lcp[r]);
 }
 
 // ------------------------------------------------------------ K9/K10 ------
