@@ -7,10 +7,10 @@
 //                  and symbol over the SA), marked in a bitmap over text
 //                  positions.  Their PLCP is 0: the predecessor starts with
 //                  another symbol (or there is none: the sentinel).
-//   Plcp           text positions in order, one tile of consecutive
-//                  positions per block.  Φ(i) = SA[ISA[i] - 1] is gathered
+//   Plcp           text positions in order, one tile of 63 consecutive
+//                  positions per wave.  Φ(i) = SA[ISA[i] - 1] is gathered
 //                  (one random SA line, which also holds SA[ISA[i]] for the
-//                  inverse check); the block shares its Φ values in LDS.
+//                  inverse check); a lane reads Φ(i-1) from its neighbour.
 //                  Position i is reducible when Φ(i-1) + 1 == Φ(i) and
 //                  i - 1 does not start a bucket: then the suffixes i-1 and
 //                  Φ(i) - 1 are adjacent and start with the same symbol, so
@@ -119,8 +119,13 @@ __device__ __forceinline__ std::uint64_t PhiOf(const Index* __restrict__ sa, con
 }
 
 // PLCP of irreducible positions (kPlcpReducible at reducible ones) with the
-// permutation/inverse check.  LCEs longer than word_limit windows go to the
-// long queue.
+// permutation/inverse check.  One wave per tile of kPlcpTile = 63 positions:
+// lane 0 gathers Φ of the position before the tile (its checks belong to the
+// previous tile's last lane), lanes 1..63 those of the tile, and each lane
+// takes its predecessor's Φ by a lane shuffle, so no block barrier holds the
+// waves together.  LCEs longer than word_limit windows go to the long queue.
+constexpr std::uint64_t kPlcpTile = kWave - 1;
+
 template <typename Index>
 __global__ __launch_bounds__(kPlcpThreads) void PlcpKernel(PackedText t, const Index* __restrict__ sa,
                                                            const Index* __restrict__ isa,
@@ -128,34 +133,30 @@ __global__ __launch_bounds__(kPlcpThreads) void PlcpKernel(PackedText t, const I
                                                            std::uint32_t word_limit, std::uint32_t* __restrict__ plcp,
                                                            LongLce* __restrict__ queue, std::uint64_t queue_capacity,
                                                            DeviceCounters* counters) {
-  __shared__ std::uint64_t s_phi[kPlcpThreads + 1];
-  __shared__ std::uint64_t scratch[kPlcpThreads / kWave];
-  const std::uint64_t tiles = (n + kPlcpThreads - 1) / kPlcpThreads;
-  std::uint64_t irreducible = 0;
+  constexpr std::uint64_t kWavesPerBlock = kPlcpThreads / kWave;
+  const std::uint64_t tiles = (n + kPlcpTile - 1) / kPlcpTile;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kWavesPerBlock;
+  const std::uint32_t lane = LaneId();
   bool bad = false;
-  for (std::uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const std::uint64_t i = tile * kPlcpThreads + threadIdx.x;
+  for (std::uint64_t tile = static_cast<std::uint64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+       tile < tiles; tile += stride) {  // wave-uniform
+    const std::uint64_t first = tile * kPlcpTile;
+    const std::uint64_t i = first + lane - 1;  // lane 0: the position before the tile
+    const bool own = lane > 0 && i < n;
     std::uint64_t phi = kPhiNone;
-    if (i < n) phi = PhiOf(sa, isa, starts, n, i, bad);
-    s_phi[threadIdx.x + 1] = phi;
-    if (threadIdx.x == 0) {
-      bool ignored = false;  // position i - 1 is checked by its own thread
-      s_phi[0] = i > 0 && i - 1 < n ? PhiOf(sa, isa, starts, n, i - 1, ignored) : kPhiNone;
-    }
-    __syncthreads();
-    const std::uint64_t before = s_phi[threadIdx.x];
-    __syncthreads();
-    if (i >= n) continue;
+    bool checked = false;
+    if (own || (lane == 0 && first > 0)) phi = PhiOf(sa, isa, starts, n, i, checked);
+    if (own) bad |= checked;
+    const std::uint64_t before = WaveShfl(phi, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
+    if (!own) continue;
     if (phi == kPhiNone) {
       plcp[i] = 0;
-      ++irreducible;
       continue;
     }
     if (before != kPhiNone && before + 1 == phi) {
       plcp[i] = kPlcpReducible;
       continue;
     }
-    ++irreducible;
     std::uint64_t l = 0;
     bool done = false;
     for (std::uint32_t w = 0; w < word_limit; ++w) {
@@ -183,7 +184,6 @@ __global__ __launch_bounds__(kPlcpThreads) void PlcpKernel(PackedText t, const I
     plcp[i] = 0;  // the long-LCE pass writes the value
   }
   if (bad) atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagNotInverse));
-  BlockAddTo<kPlcpThreads>(irreducible, scratch, &counters->irreducible);
 }
 
 // One wave per queued long LCE: 64 windows (2048 symbols) compared per step.
@@ -231,21 +231,27 @@ __device__ __forceinline__ std::uint64_t PlcpKey(std::uint32_t v, std::uint64_t 
   return v == kPlcpReducible ? 0ull : static_cast<std::uint64_t>(v) + k + 1;
 }
 
+// Per-chunk maximum key; also counts the irreducible positions (nonzero
+// keys) into counters->irreducible.
 __global__ __launch_bounds__(256) void PlcpScanReduceKernel(const std::uint32_t* __restrict__ plcp,
                                                             std::uint64_t n, std::uint64_t chunk,
-                                                            std::uint64_t* __restrict__ partial) {
+                                                            std::uint64_t* __restrict__ partial,
+                                                            DeviceCounters* counters) {
   constexpr int kBlock = 256;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
   std::uint64_t best = 0;
+  std::uint64_t irreducible = 0;
   for (std::uint64_t k = begin + threadIdx.x; k < end; k += kBlock) {
     const std::uint64_t v = PlcpKey(plcp[k], k);
     best = v > best ? v : best;
+    irreducible += v != 0;
   }
   std::uint64_t total;
   BlockExclusiveMax<kBlock>(best, scratch, total);
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
+  BlockAddTo<kBlock>(irreducible, scratch, &counters->irreducible);
 }
 
 __global__ __launch_bounds__(1024) void PlcpScanSpineKernel(std::uint64_t* partial, int grid) {
@@ -292,7 +298,8 @@ __global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __rest
   }
 }
 
-// LCP[r] = PLCP[SA[r]] in the requested output width; LCP[0] = 0.
+// LCP[r] = PLCP[SA[r]] in the requested output width; LCP[0] = 0.  One row
+// per thread (a grid-stride loop only past 2^32 threads).
 template <typename Index, typename Out>
 __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __restrict__ plcp,
                                                        const Index* __restrict__ sa, std::uint64_t n,

@@ -1011,10 +1011,12 @@ class Doubling {
   Index* q_row0_next_ = nullptr;
 };
 
-// Grid of a grid-stride kernel over m items: enough blocks to fill the
-// device, few enough that each thread takes several items.
-unsigned StrideGrid(std::uint64_t m) {
-  return static_cast<unsigned>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, 256), 1u << 16)));
+// Grid of a kernel that takes `per_block` items per 256-thread block: one
+// block per group of items, capped where the dispatch would pass 2^32
+// work-items (the kernel's grid-stride loop takes the rest).
+unsigned FlatGrid(std::uint64_t m, std::uint64_t per_block) {
+  constexpr std::uint64_t kMaxBlocks = 0xFFFFFFFFull / 256;
+  return static_cast<unsigned>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, per_block), kMaxBlocks)));
 }
 
 // LCP pass (lcp_kernels.hpp): bucket-start suffixes, PLCP of irreducible
@@ -1041,15 +1043,15 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
   const std::uint64_t bucket_threads = 7ull * (ctx.refs > 0 ? ctx.refs : 1);
   BucketStartsKernel<Index><<<static_cast<unsigned>(CeilDiv(bucket_threads, 64)), 64, 0, b->stream>>>(
       d_text, sa, n, ctx.d_offsets, ctx.refs, starts);
-  PlcpKernel<Index><<<StrideGrid(n), kPlcpThreads, 0, b->stream>>>(text, sa, isa, starts, n, kWordLimit, plcp, queue,
-                                                                   capacity, dc);
+  const std::uint64_t plcp_tiles = CeilDiv(n, kPlcpTile);
+  PlcpKernel<Index><<<FlatGrid(plcp_tiles, kPlcpThreads / kWave), kPlcpThreads, 0, b->stream>>>(
+      text, sa, isa, starts, n, kWordLimit, plcp, queue, capacity, dc);
   SFK_HIP_CHECK(hipGetLastError());
   const DeviceCounters& c = ReadCounters(b);
   if (c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) {
     Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
   }
   const std::uint64_t items = c.long_items;
-  const std::uint64_t irreducible = c.irreducible;
   if (items > capacity) Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
   if (items > 0) {
     PlcpLongKernel<<<static_cast<unsigned>(CeilDiv(items * kWave, 256)), 256, 0, b->stream>>>(text, n, queue, items,
@@ -1058,15 +1060,14 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
   }
   const SortGeometry g = Geometry(n, 256 * kScanItems, 1024);
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
-  PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
+  PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial, dc);
   PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
   PlcpScanApplyKernel<kScanItems><<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
-  LcpGatherKernel<Index, Out><<<StrideGrid(n), 256, 0, b->stream>>>(plcp, sa, n, lcp);
+  LcpGatherKernel<Index, Out><<<FlatGrid(n, 256), 256, 0, b->stream>>>(plcp, sa, n, lcp);
   SFK_HIP_CHECK(hipGetLastError());
   ctx.stats.algorithmic_bytes_lcp +=
       n / 8                            // bucket-start bitmap
       + n * (w + w + 4) + n / 2        // PLCP: ISA read, SA[ISA[i] - 1] (with SA[ISA[i]]), PLCP write, text at i
-      + irreducible * 32               // the window pair at Φ(i) of each irreducible position
       + n * (3 * 4)                    // max-scan: two reads, one write
       + n * (w + 4 + sizeof(Out));     // LCP = PLCP[SA]: SA, PLCP, LCP
 }
@@ -1253,6 +1254,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     Fail(SFK_HIP_BUILD_FAILURE, "value does not fit the requested stored width");
   }
   ctx.stats.lce_long_items = c.long_items;
+  if (want_lcp) ctx.stats.algorithmic_bytes_lcp += c.irreducible * 32;  // window pair at Φ(i) per irreducible i
   ctx.stats.lce_long_words = c.long_words;
   if (b->phase[0] == nullptr) return;  // untimed build
   float ms = 0.f;
