@@ -552,6 +552,14 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const std::uint32_t ex = BlockExclusiveSum<kBlock>(v, scan_scratch, total);
     if (threadIdx.x < kRadix) local_start[threadIdx.x] = ex;
   }
+  if constexpr (kSegmented) {
+    // A tile within one segment (the usual case) folds the segment's
+    // placement into its digit bases once, instead of per element.
+    if (seg_first == seg_last && threadIdx.x < kRadix) {
+      global_base[threadIdx.x] +=
+          static_cast<std::uint64_t>(seg.adj[static_cast<std::uint64_t>(seg_first) * kRadix + threadIdx.x]);
+    }
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -562,9 +570,11 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       s_keys[pos] = key[j];
       if constexpr (!kSplit) s_vals[pos] = val[j];
       if constexpr (kSegmented) {
-        // Segment of the input slot (the tile usually lies in one segment).
-        const std::uint32_t sg = seg_first == seg_last ? seg_first : SegmentOfSlot(seg.offsets, seg_first, seg_last, idx);
-        s_seg[pos] = static_cast<std::uint16_t>(sg - seg_first);
+        // Segment of the input slot, for a tile that spans segments.
+        if (seg_first != seg_last) {
+          const std::uint32_t sg = SegmentOfSlot(seg.offsets, seg_first, seg_last, idx);
+          s_seg[pos] = static_cast<std::uint16_t>(sg - seg_first);
+        }
       }
     }
   }
@@ -573,8 +583,10 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   auto place = [&](std::uint32_t p, std::uint32_t d) {
     std::uint64_t out = global_base[d] + (p - local_start[d]);
     if constexpr (kSegmented) {
-      const std::uint64_t sg = seg_first + s_seg[p];
-      out += static_cast<std::uint64_t>(seg.adj[sg * kRadix + d]);
+      if (seg_first != seg_last) {
+        const std::uint64_t sg = seg_first + s_seg[p];
+        out += static_cast<std::uint64_t>(seg.adj[sg * kRadix + d]);
+      }
     }
     return out;
   };

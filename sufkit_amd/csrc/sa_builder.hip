@@ -557,10 +557,13 @@ __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
   c->l_next_segs = 0;
 }
 
-template <typename Index>
+// Index: suffix positions and rows in the lists; R: the SA and rank (ISA)
+// arrays, which may be narrower (the stored width of a 64-bit build whose
+// values fit 32 bits).
+template <typename Index, typename R>
 class Doubling {
  public:
-  Doubling(BuildContext& ctx, Index* sa, Index* rank, PackedText text)
+  Doubling(BuildContext& ctx, R* sa, R* rank, PackedText text)
       : ctx_(ctx), b_(ctx.b), n_(ctx.n), sa_(sa), rank_(rank), text_(text) {}
 
   void Run() {
@@ -639,7 +642,7 @@ class Doubling {
         QKeygenKernel<Index><<<static_cast<unsigned>(CeilDiv(qm_, 256)), 256, 0, b_->stream>>>(
             q_idx_cur_, qm_, rank_, n_, h, b_->q_key.As<Index>(), dc);
         SFK_HIP_CHECK(hipGetLastError());
-        ctx_.stats.algorithmic_bytes_sa += qm_ * 3 * sizeof(Index);
+        ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * sizeof(Index) + sizeof(R));
       }
       split = false;
       if (lm_ > 0) {
@@ -713,10 +716,10 @@ class Doubling {
   const DeviceCounters& ApplySizes(const DeviceCounters& c) {
     CheckFlags(c);
     if (pending_apply_) {
-      constexpr std::uint64_t w = sizeof(Index);
+      constexpr std::uint64_t w = sizeof(Index), wr = sizeof(R);
       const std::uint64_t m = pending_m_, survivors = c.list.survivors;
       ctx_.stats.algorithmic_bytes_sa +=
-          m * (2 * pending_key_bytes_ + w) + (m - survivors) * 2 * w + survivors * (3 * w + 4);
+          m * (2 * pending_key_bytes_ + w) + (m - survivors) * 2 * wr + survivors * (2 * w + wr + 4);
       pending_apply_ = false;
     }
     lm_ = c.list.survivors;
@@ -848,7 +851,7 @@ class Doubling {
     constexpr std::uint64_t w = sizeof(Index);
     ctx_.stats.refine_steps += 1;
     ctx_.stats.active_total += qm_;
-    ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * w + (kExact ? 32 : w) + 2 * w + 2 * w);
+    ctx_.stats.algorithmic_bytes_sa += qm_ * (2 * w + (kExact ? 32 : w) + 2 * sizeof(R) + 2 * w);
     StepLog(kExact ? "local-exact" : "local", qm_, 0, 0);
   }
 
@@ -940,7 +943,7 @@ class Doubling {
       }
       SFK_HIP_CHECK(hipGetLastError());
       ctx_.stats.algorithmic_bytes_sa +=
-          m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? w : 24) + sizeof(K) + w);
+          m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? sizeof(R) : 24) + sizeof(K) + w);
       if (local_between) LocalRefine<false>();
       SortPairs<K, Index>(ctx_, ArraySource<K, Index>{keys, vals}, m * (sizeof(K) + w), keys, vals, keys_alt,
                           vals_alt, m, passes, set);
@@ -982,8 +985,8 @@ class Doubling {
   BuildContext& ctx_;
   sfk_hip_builder* b_;
   std::uint64_t n_;
-  Index* sa_;
-  Index* rank_;
+  R* sa_;
+  R* rank_;
   PackedText text_;
   // Large list: suffixes, segment ids, per-segment deltas.  After a
   // RefineApply whose sizes are not read back yet, lm_bound_ bounds lm_.
@@ -1127,13 +1130,21 @@ void SampleOutputs(BuildContext& ctx, const Index* sa, const Index* lcp, void* d
                                      (d_lcp ? rows * (w + lcp_bits / 8) : 0);
 }
 
-template <typename Index>
+// Index: the construction width of the lists; R: that of the SA and rank
+// arrays (Index, or 32 bits for a 64-bit build of a single text whose SA and
+// ISA are stored in 32 bits: the doubling then writes them in their stored
+// width, and the values, rows and positions below n <= 2^32, fit).
+template <typename Index, typename R>
 void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t flags,
                 void* d_sa, std::uint32_t sa_bits, void* d_isa, std::uint32_t isa_bits,
                 void* d_lcp, std::uint32_t lcp_bits) {
   sfk_hip_builder* b = ctx.b;
   const std::uint64_t n = ctx.n;
-  constexpr std::uint32_t kIndexBits = 8 * sizeof(Index);
+  constexpr std::uint32_t kIndexBits = 8 * sizeof(R);
+  constexpr bool kNarrowRank = sizeof(R) < sizeof(Index);
+  if (kNarrowRank && (ctx.sampling > 1 || ctx.refs > 0 || n > (1ull << 32))) {
+    Fail(SFK_HIP_BUILD_FAILURE, "stored-width SA and rank arrays taken for a build that cannot use them");
+  }
   const std::uint64_t words = PackedWords(n);
 
   // Workspace (grow-only).
@@ -1143,8 +1154,8 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   const bool sampled = ctx.sampling > 1;
   const bool sa_direct = !sampled && sa_bits == kIndexBits;
   const bool isa_direct = !sampled && d_isa != nullptr && isa_bits == kIndexBits;
-  if (!sa_direct) b->sa.Reserve(n * sizeof(Index));
-  if (!isa_direct) b->rank.Reserve(n * sizeof(Index));
+  if (!sa_direct) b->sa.Reserve(n * sizeof(R));
+  if (!isa_direct) b->rank.Reserve(n * sizeof(R));
   b->act_idx.Reserve(n * sizeof(Index));
   b->act_seg.Reserve(n * sizeof(std::uint32_t));
   b->keys0.Reserve(std::max<std::uint64_t>(n * 8, 64));
@@ -1159,8 +1170,8 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   ReserveOnesweep(b, n);
   b->partial.Reserve(std::max(kRefineMaxChunks * sizeof(RefinePartial), 1024 * sizeof(SplitPartial)));
 
-  Index* sa = sa_direct ? static_cast<Index*>(d_sa) : b->sa.As<Index>();
-  Index* rank = isa_direct ? static_cast<Index*>(d_isa) : b->rank.As<Index>();
+  R* sa = sa_direct ? static_cast<R*>(d_sa) : b->sa.As<R>();
+  R* rank = isa_direct ? static_cast<R*>(d_isa) : b->rank.As<R>();
   PackedText text{b->packed.As<uint4>(), n};
 
   ResetCounters(b);
@@ -1180,69 +1191,72 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   }
   ctx.stats.algorithmic_bytes_sa += n + words * 16;
 
-  Doubling<Index>(ctx, sa, rank, text).Run();
+  Doubling<Index, R>(ctx, sa, rank, text).Run();
   RecordPhase(b, 1);
 
   // K9: permutation + inverse check on the device (fused into the PLCP pass
   // when the LCP is built).
   const bool want_lcp = d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
   if (!want_lcp) {
-    ValidateKernel<Index><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
+    ValidateKernel<R><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
         sa, rank, n, b->counters.As<DeviceCounters>());
     SFK_HIP_CHECK(hipGetLastError());
-    ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(Index);
+    ctx.stats.algorithmic_bytes_isa += n * 2 * sizeof(R);
   }
   RecordPhase(b, 2);
   if (want_lcp) {
     if (sampled) {
       // The complete LCP into the workspace (vals0 is free after the
       // doubling), for the interval minima of the sampled rows.
-      LaunchLcp<Index, Index>(ctx, d_text, text, sa, rank, b->vals0.As<Index>());
+      if constexpr (!kNarrowRank) LaunchLcp<Index, Index>(ctx, d_text, text, sa, rank, b->vals0.As<Index>());
     } else if (lcp_bits == 32) {
-      LaunchLcp<Index, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
+      LaunchLcp<R, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
     } else {
-      LaunchLcp<Index, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
+      LaunchLcp<R, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
     }
   }
   RecordPhase(b, 3);
   // SA and ISA in their stored widths (and, for a batch, per-reference
   // coordinates).
-  if (ctx.refs > 0) {
-    const unsigned grid = static_cast<unsigned>(CeilDiv(n, kLocalizeTile));
-    auto localize = [&](auto* out_sa) {
-      using OutSa = std::remove_pointer_t<decltype(out_sa)>;
-      if (d_isa == nullptr) {
-        LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
-            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(nullptr),
-            b->counters.As<DeviceCounters>());
-      } else if (isa_bits == 32) {
-        LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
-            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(d_isa),
-            b->counters.As<DeviceCounters>());
+  if constexpr (!kNarrowRank) {
+    if (ctx.refs > 0) {
+      const unsigned grid = static_cast<unsigned>(CeilDiv(n, kLocalizeTile));
+      auto localize = [&](auto* out_sa) {
+        using OutSa = std::remove_pointer_t<decltype(out_sa)>;
+        if (d_isa == nullptr) {
+          LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
+              sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(nullptr),
+              b->counters.As<DeviceCounters>());
+        } else if (isa_bits == 32) {
+          LocalizeKernel<Index, OutSa, std::uint32_t><<<grid, 256, 0, b->stream>>>(
+              sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint32_t*>(d_isa),
+              b->counters.As<DeviceCounters>());
+        } else {
+          LocalizeKernel<Index, OutSa, std::uint64_t><<<grid, 256, 0, b->stream>>>(
+              sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint64_t*>(d_isa),
+              b->counters.As<DeviceCounters>());
+        }
+      };
+      if (sa_bits == 32) {
+        localize(static_cast<std::uint32_t*>(d_sa));
       } else {
-        LocalizeKernel<Index, OutSa, std::uint64_t><<<grid, 256, 0, b->stream>>>(
-            sa, rank, n, ctx.d_offsets, ctx.refs, out_sa, static_cast<std::uint64_t*>(d_isa),
-            b->counters.As<DeviceCounters>());
+        localize(static_cast<std::uint64_t*>(d_sa));
       }
-    };
-    if (sa_bits == 32) {
-      localize(static_cast<std::uint32_t*>(d_sa));
-    } else {
-      localize(static_cast<std::uint64_t*>(d_sa));
+      SFK_HIP_CHECK(hipGetLastError());
+      ctx.stats.algorithmic_bytes_isa += n * (2 * sizeof(Index) + sa_bits / 8 + isa_bits / 8);
+    } else if (sampled) {
+      SampleOutputs<Index>(ctx, sa, want_lcp ? b->vals0.As<Index>() : nullptr, d_sa, sa_bits, d_isa, isa_bits,
+                           want_lcp ? d_lcp : nullptr, lcp_bits);
     }
-    SFK_HIP_CHECK(hipGetLastError());
-    ctx.stats.algorithmic_bytes_isa += n * (2 * sizeof(Index) + sa_bits / 8 + isa_bits / 8);
-  } else if (sampled) {
-    SampleOutputs<Index>(ctx, sa, want_lcp ? b->vals0.As<Index>() : nullptr, d_sa, sa_bits, d_isa, isa_bits,
-                         want_lcp ? d_lcp : nullptr, lcp_bits);
-  } else {
+  }
+  if (ctx.refs == 0 && !sampled) {
     if (d_isa != nullptr && !isa_direct) {
-      ConvertTo<Index>(b, rank, n, d_isa, isa_bits);
-      ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + isa_bits / 8);
+      ConvertTo<R>(b, rank, n, d_isa, isa_bits);
+      ctx.stats.algorithmic_bytes_isa += n * (sizeof(R) + isa_bits / 8);
     }
     if (!sa_direct) {
-      ConvertTo<Index>(b, sa, n, d_sa, sa_bits);
-      ctx.stats.algorithmic_bytes_isa += n * (sizeof(Index) + sa_bits / 8);
+      ConvertTo<R>(b, sa, n, d_sa, sa_bits);
+      ctx.stats.algorithmic_bytes_isa += n * (sizeof(R) + sa_bits / 8);
     }
   }
   RecordPhase(b, 4);
@@ -1350,10 +1364,17 @@ int RunDeviceBuild(sfk_hip_builder* b, const std::uint8_t* d_text, std::uint64_t
     ctx.d_offsets = b->ref_offsets.As<std::uint64_t>();
     ctx.refs = refs;
   }
+  // A 64-bit build of one text whose SA and ISA are stored in 32 bits
+  // writes them in 32 bits during the doubling (no 64-bit copies to narrow
+  // afterwards, and the memory they would take goes to the small lists).
+  const bool narrow_rank = refs == 0 && sampling == 1 && n <= (1ull << 32) && sa_bits == 32 &&
+                           (d_isa == nullptr || isa_bits == 32);
   if (construction_bits == 32) {
-    BuildTyped<std::uint32_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
+    BuildTyped<std::uint32_t, std::uint32_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
+  } else if (narrow_rank) {
+    BuildTyped<std::uint64_t, std::uint32_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
   } else {
-    BuildTyped<std::uint64_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
+    BuildTyped<std::uint64_t, std::uint64_t>(ctx, d_text, flags, d_sa, sa_bits, d_isa, isa_bits, d_lcp, lcp_bits);
   }
   if (stats != nullptr) {
     const double h2d = stats->h2d_seconds, d2h = stats->d2h_seconds;

@@ -226,8 +226,8 @@ __global__ __launch_bounds__(256) void LocalizeKernel(
 // ------------------------------------------------------------ keys --------
 
 // Secondary key of suffix v for a refine step (see SecondaryKind).
-template <typename Index>
-__device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const Index* __restrict__ rank,
+template <typename R>
+__device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const R* __restrict__ rank,
                                              std::uint64_t v, std::uint64_t h, DeviceCounters* counters) {
   if (sec_kind == kSecRank) {
     // An unsorted suffix never reaches the sentinel within h symbols, so
@@ -251,11 +251,11 @@ __device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const I
 // written, by the same thread (hence no __restrict__ on either).  kRank: the
 // secondary key is rank[suffix + h] (sec_kind is kSecRank); otherwise it
 // comes from the suffix's 32-symbol window (FillHi, FillLo, Exact0..2).
-template <typename Index, typename K, bool kRank>
+template <typename Index, typename K, bool kRank, typename R = Index>
 __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
     const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
-    const Index* __restrict__ rank, std::uint64_t h, bool iota,
+    const R* __restrict__ rank, std::uint64_t h, bool iota,
     K* __restrict__ keys, Index* vals, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
   constexpr int kWaves = kKeygenBlock / kWave;
   __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
@@ -723,10 +723,10 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
 
 // Writes of one tile, given the survivors, segments and 1 + last group start
 // of every slot before the tile.
-template <int kBlock, int kItems, typename Index, typename K>
+template <int kBlock, int kItems, typename Index, typename K, typename R>
 __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std::uint64_t base, int len, int sec_bits,
                                        const Index* __restrict__ seg_delta, std::uint64_t n,
-                                       Index* __restrict__ sa, Index* __restrict__ rank,
+                                       R* __restrict__ sa, R* __restrict__ rank,
                                        Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg,
                                        Index* __restrict__ out_delta, Index* __restrict__ out_start,
                                        bool survivor_rank, std::uint64_t run_surv, std::uint64_t run_segs,
@@ -744,14 +744,14 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
     const Index row = static_cast<Index>(delta + static_cast<Index>(k));
     if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
     if (f & 2) {
-      sa[row] = v;
-      rank[v] = row;
+      sa[row] = static_cast<R>(v);
+      rank[v] = static_cast<R>(row);
     } else {
       // The group start lies in the same segment: every segment start is a
       // key change, hence a group start.
       const int g = lds.group[i];
       const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
-      if (survivor_rank) rank[v] = static_cast<Index>(delta + static_cast<Index>(group_slot));
+      if (survivor_rank) rank[v] = static_cast<R>(delta + static_cast<Index>(group_slot));
       const std::uint64_t at = run_surv + lds.pos[i];
       const std::uint64_t sc = run_segs + lds.segc[i];  // segments opened through this slot
       if (f & 1) {
@@ -765,10 +765,10 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
 }
 
 // Single pass: one block per tile (tiles numbered by LookBackTile).
-template <typename Index, typename K>
+template <typename Index, typename K, typename R = Index>
 __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m, int sec_bits,
-    const Index* __restrict__ seg_delta, std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank,
+    const Index* __restrict__ seg_delta, std::uint64_t n, R* __restrict__ sa, R* __restrict__ rank,
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
     Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
     unsigned int* tickets, DeviceCounters* counters,
@@ -900,11 +900,11 @@ __global__ __launch_bounds__(1024) void RefineSpineKernel(RefinePartial* partial
 
 // Apply: block b walks the tiles of its chunk (a multiple of the tile) in
 // order, starting from the spine's exclusive totals.
-template <typename Index, typename K>
+template <typename Index, typename K, typename R = Index>
 __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
     const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m, std::uint64_t chunk,
     int sec_bits, const Index* __restrict__ seg_delta, const RefinePartial* __restrict__ partial,
-    std::uint64_t n, Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
+    std::uint64_t n, R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
     std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta, Index* __restrict__ out_start,
     bool survivor_rank, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
   constexpr int kBlock = kRefineChunkThreads;
@@ -1280,16 +1280,16 @@ __global__ __launch_bounds__(256) void PartitionMoveKernel(
 
 // rank[suffix + h] of every small-list suffix, gathered before any rank of
 // the round is rewritten.
-template <typename Index>
+template <typename Index, typename R = Index>
 __global__ __launch_bounds__(256) void QKeygenKernel(
-    const Index* __restrict__ q_idx, std::uint64_t qm, const Index* __restrict__ rank,
+    const Index* __restrict__ q_idx, std::uint64_t qm, const R* __restrict__ rank,
     std::uint64_t n, std::uint64_t h, Index* __restrict__ q_key, DeviceCounters* counters) {
   const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= qm) return;
   const std::uint64_t at = static_cast<std::uint64_t>(q_idx[p]) + h;
   Index key = 0;
   if (at < n) {
-    key = rank[at];
+    key = static_cast<Index>(rank[at]);
   } else {
     atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
   }
@@ -1345,11 +1345,11 @@ __device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
 // FillKey high halves): the low half of FillKey, which orders windows without
 // a separator, N or sentinel exactly; a segment holding such a window compares
 // the exact 96-bit keys instead.  Otherwise q_key (rank[suffix + h]).
-template <typename Index, bool kExact>
+template <typename Index, bool kExact, typename R = Index>
 __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     PackedText t, const Index* __restrict__ q_idx, const Index* __restrict__ q_row0,
     const Index* __restrict__ q_key, std::uint64_t qm, std::uint64_t n,
-    Index* __restrict__ sa, Index* __restrict__ rank, Index* __restrict__ out_idx,
+    R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
     Index* __restrict__ out_row0, DeviceCounters* counters) {
   using KeyT = typename std::conditional<kExact, std::uint32_t, Index>::type;
   constexpr int kItems = kLocalItems;
@@ -1503,10 +1503,10 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     if (static_cast<std::uint64_t>(v[i]) >= n || static_cast<std::uint64_t>(head[i]) >= n) {
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
     } else if (equal[i] == 1) {
-      sa[head[i]] = v[i];
-      rank[v[i]] = head[i];
+      sa[head[i]] = static_cast<R>(v[i]);
+      rank[v[i]] = static_cast<R>(head[i]);
     } else {
-      rank[v[i]] = head[i];
+      rank[v[i]] = static_cast<R>(head[i]);
       survivor[i] = true;
       if (tie[i] == 0) leader_total += equal[i];
     }
