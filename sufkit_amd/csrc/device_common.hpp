@@ -148,17 +148,24 @@ __device__ inline std::uint64_t LaneMaskLt() {
 }
 
 // Lanes of this wave whose `bits`-bit digit equals this lane's, among lanes
-// with `valid` set.  Every lane of the wave must call this together.
+// with `valid` set.  Every lane of the wave must call this together.  One
+// ballot per digit bit; a lane collects, for every other lane, whether any
+// bit of that lane's digit differs from its own (the ballot XOR this lane's
+// bit spread over 32 bits), so each bit costs a few 32-bit operations and no
+// per-lane select.
 template <int kBits>
 __device__ inline std::uint64_t MatchDigit(std::uint32_t digit, bool valid) {
-  std::uint64_t peers = __ballot(valid);
+  const std::uint64_t invalid = ~__ballot(valid);
+  std::uint32_t diff_lo = static_cast<std::uint32_t>(invalid);
+  std::uint32_t diff_hi = static_cast<std::uint32_t>(invalid >> 32);
 #pragma unroll
   for (int b = 0; b < kBits; ++b) {
-    const bool bit = (digit >> b) & 1u;
-    const std::uint64_t vote = __ballot(bit);
-    peers &= bit ? vote : ~vote;
+    const std::uint32_t spread = static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(digit), b, 1));
+    const std::uint64_t vote = __ballot(spread != 0u);
+    diff_lo |= static_cast<std::uint32_t>(vote) ^ spread;
+    diff_hi |= static_cast<std::uint32_t>(vote >> 32) ^ spread;
   }
-  return peers;
+  return ~((static_cast<std::uint64_t>(diff_hi) << 32) | diff_lo);
 }
 
 template <typename T>

@@ -503,6 +503,10 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       val[g + j] = valid ? vv : V(0);
     }
   }
+  // Stable rank of every item among the wave's items of its digit so far:
+  // every lane reads its digit's count, and the last lane of each group of
+  // equal digits writes the count back raised by the group's size.
+  const std::uint64_t lanes_below = LaneMaskLt();
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
@@ -510,15 +514,13 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
     digit[j] = d;
     const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
-    const int leader = peers ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : static_cast<int>(lane);
-    std::uint32_t old = 0;
-    if (valid && static_cast<int>(lane) == leader) {
-      old = wave_cnt[wave][d];
-      wave_cnt[wave][d] = old + static_cast<std::uint32_t>(__popcll(peers));
-    }
+    const std::uint32_t below = static_cast<std::uint32_t>(__popcll(peers & lanes_below));
+    const std::uint32_t count = static_cast<std::uint32_t>(__popcll(peers));
+    const std::uint32_t old = wave_cnt[wave][d];
     __builtin_amdgcn_wave_barrier();
-    old = __shfl(old, leader, kWave);
-    rank[j] = old + static_cast<std::uint32_t>(__popcll(peers & LaneMaskLt()));
+    if (valid && below + 1 == count) wave_cnt[wave][d] = old + count;
+    __builtin_amdgcn_wave_barrier();
+    rank[j] = old + below;
   }
   __syncthreads();
   std::uint32_t my_count = 0;
