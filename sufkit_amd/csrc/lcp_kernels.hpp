@@ -231,8 +231,22 @@ __device__ __forceinline__ std::uint64_t PlcpKey(std::uint32_t v, std::uint64_t 
   return v == kPlcpReducible ? 0ull : static_cast<std::uint64_t>(v) + k + 1;
 }
 
+// Eight consecutive PLCP values from slot k (k a multiple of 8), as two
+// 16-byte loads when all eight lie below end.
+__device__ __forceinline__ void LoadPlcp8(const std::uint32_t* __restrict__ plcp, std::uint64_t k, std::uint64_t end,
+                                          std::uint32_t (&v)[8]) {
+  if (k + 8 <= end) {
+    const uint4 a = *reinterpret_cast<const uint4*>(plcp + k);
+    const uint4 b = *reinterpret_cast<const uint4*>(plcp + k + 4);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = k + i < end ? plcp[k + i] : kPlcpReducible;
+  }
+}
+
 // Per-chunk maximum key; also counts the irreducible positions (nonzero
-// keys) into counters->irreducible.
+// keys) into counters->irreducible.  Chunks start at multiples of 8 slots.
 __global__ __launch_bounds__(256) void PlcpScanReduceKernel(const std::uint32_t* __restrict__ plcp,
                                                             std::uint64_t n, std::uint64_t chunk,
                                                             std::uint64_t* __restrict__ partial,
@@ -243,10 +257,15 @@ __global__ __launch_bounds__(256) void PlcpScanReduceKernel(const std::uint32_t*
   const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
   std::uint64_t best = 0;
   std::uint64_t irreducible = 0;
-  for (std::uint64_t k = begin + threadIdx.x; k < end; k += kBlock) {
-    const std::uint64_t v = PlcpKey(plcp[k], k);
-    best = v > best ? v : best;
-    irreducible += v != 0;
+  for (std::uint64_t k = begin + threadIdx.x * 8ull; k < end; k += kBlock * 8ull) {
+    std::uint32_t v[8];
+    LoadPlcp8(plcp, k, end, v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const std::uint64_t key = PlcpKey(v[i], k + i);
+      best = key > best ? key : best;
+      irreducible += key != 0;
+    }
   }
   std::uint64_t total;
   BlockExclusiveMax<kBlock>(best, scratch, total);
@@ -264,11 +283,12 @@ __global__ __launch_bounds__(1024) void PlcpScanSpineKernel(std::uint64_t* parti
 }
 
 // Inclusive max-scan of the keys, written back as PLCP[k] = scan - k - 1.
-template <int kItems>
+// Each thread owns 8 consecutive slots (two 16-byte loads and stores).
 __global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __restrict__ plcp, std::uint64_t n,
                                                            std::uint64_t chunk,
                                                            const std::uint64_t* __restrict__ partial) {
   constexpr int kBlock = 256;
+  constexpr int kItems = 8;
   constexpr int kTile = kBlock * kItems;
   __shared__ std::uint64_t scratch[kBlock / kWave];
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
@@ -276,12 +296,13 @@ __global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __rest
   std::uint64_t run = partial[blockIdx.x];
   for (std::uint64_t base = begin; base < end; base += kTile) {
     const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
+    std::uint32_t v[kItems];
+    LoadPlcp8(plcp, first, end, v);
     std::uint64_t keys[kItems];
     std::uint64_t local = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      keys[i] = k < end ? PlcpKey(plcp[k], k) : 0ull;
+      keys[i] = first + i < end ? PlcpKey(v[i], first + i) : 0ull;
       local = keys[i] > local ? keys[i] : local;
     }
     std::uint64_t total;
@@ -289,10 +310,17 @@ __global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __rest
     if (run > cur) cur = run;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t k = first + i;
-      if (k >= end) break;
       cur = keys[i] > cur ? keys[i] : cur;
-      plcp[k] = static_cast<std::uint32_t>(cur - k - 1);
+      v[i] = static_cast<std::uint32_t>(cur - (first + i) - 1);
+    }
+    if (first + kItems <= end) {
+      *reinterpret_cast<uint4*>(plcp + first) = make_uint4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<uint4*>(plcp + first + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kItems; ++i) {
+        if (first + i < end) plcp[first + i] = v[i];
+      }
     }
     if (total > run) run = total;
   }

@@ -1065,7 +1065,7 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
   PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial, dc);
   PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
-  PlcpScanApplyKernel<kScanItems><<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
+  PlcpScanApplyKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
   LcpGatherKernel<Index, Out><<<FlatGrid(n, 256), 256, 0, b->stream>>>(plcp, sa, n, lcp);
   SFK_HIP_CHECK(hipGetLastError());
   ctx.stats.algorithmic_bytes_lcp +=
