@@ -9,13 +9,50 @@
 #include <functional>
 #include <iosfwd>
 #include <limits>
+#include <memory>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <variant>
 #include <vector>
 
 #include "sufkit.hpp"
 
 namespace sufkit::detail {
+
+// Allocator that leaves the elements of a vector sized without values
+// uninitialised: the coordinate and LCP vectors a build fills from the
+// device are not zero-filled first (tens of gigabytes of single-threaded
+// stores); their pages are first touched by the multi-threaded copy that
+// fills them.  Construction with a value still stores it.
+template <class T>
+struct NoInitAllocator : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAllocator<U>;
+  };
+  NoInitAllocator() noexcept = default;
+  template <class U>
+  NoInitAllocator(const NoInitAllocator<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible_v<U>) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+template <class T, class U>
+bool operator==(const NoInitAllocator<T>&, const NoInitAllocator<U>&) noexcept { return true; }
+template <class T, class U>
+bool operator!=(const NoInitAllocator<T>&, const NoInitAllocator<U>&) noexcept { return false; }
+
+// Coordinates and raw LCP values (see NoInitAllocator).
+template <class T>
+using ValueVector = std::vector<T, NoInitAllocator<T>>;
+using U32Values = ValueVector<std::uint32_t>;
+using U64Values = ValueVector<std::uint64_t>;
 
 // reference_data.hpp:20-27
 constexpr std::uint8_t kSentinel = 0;
@@ -102,13 +139,12 @@ struct Coordinate48 {
 // 439-598 semantics).
 class CoordinateStorage {
  public:
-  using Storage = std::variant<std::monostate, std::vector<std::uint32_t>, std::vector<std::uint64_t>,
-                               Coordinate40, Coordinate48>;
+  using Storage = std::variant<std::monostate, U32Values, U64Values, Coordinate40, Coordinate48>;
   CoordinateStorage() = default;
-  static CoordinateStorage FromUInt32(std::vector<std::uint32_t>&& values,
+  static CoordinateStorage FromUInt32(U32Values&& values,
                                       CoordinateStorageWidth width, std::uint64_t symbol_count,
                                       ErrorCode error_code, const char* label);
-  static CoordinateStorage FromUInt64(std::vector<std::uint64_t>&& values,
+  static CoordinateStorage FromUInt64(U64Values&& values,
                                       CoordinateStorageWidth width, std::uint64_t symbol_count,
                                       ErrorCode error_code, const char* label);
   static CoordinateStorage FromPacked40(Coordinate40&& values, std::uint64_t symbol_count,
@@ -140,9 +176,9 @@ class LcpStorage {
   static constexpr std::uint32_t kGuideBlockShift = 12;
 
   LcpStorage() = default;
-  static LcpStorage FromRaw32(std::vector<std::uint32_t>&& values, std::uint32_t sampling_rate,
+  static LcpStorage FromRaw32(U32Values&& values, std::uint32_t sampling_rate,
                               ErrorCode error_code);
-  static LcpStorage FromRaw64(std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
+  static LcpStorage FromRaw64(U64Values&& values, std::uint32_t sampling_rate,
                               ErrorCode error_code);
   // Validates as ValidateByteStructure (lcp_storage.cpp:97-150) and builds
   // the guide (BuildGuide, :55-95).
@@ -160,8 +196,8 @@ class LcpStorage {
   // suffix_position (DecodeByte, lcp_storage.cpp:186-215).
   std::uint64_t Decode(std::uint64_t row, std::uint64_t suffix_position, ErrorCode error_code) const;
   std::uint64_t Bytes() const noexcept;
-  const std::vector<std::uint32_t>& Raw32() const noexcept { return raw32_; }
-  const std::vector<std::uint64_t>& Raw64() const noexcept { return raw64_; }
+  const U32Values& Raw32() const noexcept { return raw32_; }
+  const U64Values& Raw64() const noexcept { return raw64_; }
   const std::vector<std::uint8_t>& Primary() const noexcept { return primary_; }
   const std::vector<std::uint64_t>& AnchorPositions() const noexcept { return anchor_positions_; }
   const std::vector<std::uint64_t>& AnchorValues() const noexcept { return anchor_values_; }
@@ -170,8 +206,8 @@ class LcpStorage {
                 ErrorCode error_code) const;
 
  private:
-  std::vector<std::uint32_t> raw32_;
-  std::vector<std::uint64_t> raw64_;
+  U32Values raw32_;
+  U64Values raw64_;
   std::vector<std::uint8_t> primary_;
   std::vector<std::uint64_t> anchor_positions_;
   std::vector<std::uint64_t> anchor_values_;

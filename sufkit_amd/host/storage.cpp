@@ -149,7 +149,7 @@ CoordinateStorage::Storage Pack(Values&& values, CoordinateStorageWidth width) {
     if constexpr (std::is_same_v<T, std::uint32_t>) {
       return std::move(values);
     } else {
-      std::vector<std::uint32_t> out(values.size());
+      U32Values out(values.size());
       for (std::size_t i = 0; i < values.size(); ++i) out[i] = static_cast<std::uint32_t>(values[i]);
       return out;
     }
@@ -158,7 +158,7 @@ CoordinateStorage::Storage Pack(Values&& values, CoordinateStorageWidth width) {
     if constexpr (std::is_same_v<T, std::uint64_t>) {
       return std::move(values);
     } else {
-      return std::vector<std::uint64_t>(values.begin(), values.end());
+      return U64Values(values.begin(), values.end());
     }
   }
   if (width == CoordinateStorageWidth::kBits40) {
@@ -185,7 +185,7 @@ CoordinateStorage::Storage Pack(Values&& values, CoordinateStorageWidth width) {
 
 }  // namespace
 
-CoordinateStorage CoordinateStorage::FromUInt32(std::vector<std::uint32_t>&& values,
+CoordinateStorage CoordinateStorage::FromUInt32(U32Values&& values,
                                                 CoordinateStorageWidth width,
                                                 std::uint64_t symbol_count, ErrorCode error_code,
                                                 const char* label) {
@@ -197,7 +197,7 @@ CoordinateStorage CoordinateStorage::FromUInt32(std::vector<std::uint32_t>&& val
   return s;
 }
 
-CoordinateStorage CoordinateStorage::FromUInt64(std::vector<std::uint64_t>&& values,
+CoordinateStorage CoordinateStorage::FromUInt64(U64Values&& values,
                                                 CoordinateStorageWidth width,
                                                 std::uint64_t symbol_count, ErrorCode error_code,
                                                 const char* label) {
@@ -275,7 +275,7 @@ std::uint64_t CoordinateStorage::At(std::uint64_t i) const noexcept {
 
 // ------------------------------------------------------------ LCP ---
 
-LcpStorage LcpStorage::FromRaw32(std::vector<std::uint32_t>&& values, std::uint32_t sampling_rate,
+LcpStorage LcpStorage::FromRaw32(U32Values&& values, std::uint32_t sampling_rate,
                                  ErrorCode error_code) {
   if (sampling_rate == 0) Throw(error_code, "LCP storage: sampling rate must be positive");
   if (!values.empty() && values.front() != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");
@@ -286,7 +286,7 @@ LcpStorage LcpStorage::FromRaw32(std::vector<std::uint32_t>&& values, std::uint3
   return s;
 }
 
-LcpStorage LcpStorage::FromRaw64(std::vector<std::uint64_t>&& values, std::uint32_t sampling_rate,
+LcpStorage LcpStorage::FromRaw64(U64Values&& values, std::uint32_t sampling_rate,
                                  ErrorCode error_code) {
   if (sampling_rate == 0) Throw(error_code, "LCP storage: sampling rate must be positive");
   if (!values.empty() && values.front() != 0) Throw(error_code, "LCP storage: LCP[0] must be zero");

@@ -125,7 +125,7 @@ ErrorCode FromStatus(int status) {
 }
 
 template <class T>
-detail::CoordinateStorage Store(std::vector<T>&& values, CoordinateStorageWidth width,
+detail::CoordinateStorage Store(detail::ValueVector<T>&& values, CoordinateStorageWidth width,
                                 std::uint64_t domain, ErrorCode code, const char* label) {
   if constexpr (std::is_same_v<T, std::uint32_t>) {
     return detail::CoordinateStorage::FromUInt32(std::move(values), width, domain, code, label);
@@ -143,9 +143,10 @@ void BuildWithHip(SuffixArray::Impl& impl, const SuffixArrayBuildOptions& option
   // (suffix_array.cpp:912-974), with their ISA and interval-minimum LCP.
   const std::uint32_t k = options.sampling_rate;
   const std::uint64_t rows = 1 + (n - 1) / k;
-  std::vector<SaT> sa(rows);
-  std::vector<IsaT> isa(retain_isa ? rows : 0);
-  std::vector<LcpT> lcp(retain_lcp ? rows : 0);
+  // Filled by the device build (never read before it writes them).
+  detail::ValueVector<SaT> sa(rows);
+  detail::ValueVector<IsaT> isa(retain_isa ? rows : 0);
+  detail::ValueVector<LcpT> lcp(retain_lcp ? rows : 0);
   std::vector<std::uint64_t> table(prefix_k > 0 ? sfk_prefix_directory_words(prefix_k, n) : 0);
   sfk_hip_stats hs{};
   char err[512] = {0};
@@ -250,11 +251,11 @@ void BuildLowMemoryWithHip(SuffixArray::Impl& impl, CoordinateStorageWidth store
   stats.lcp_finalize_seconds = Since(fin);
   const auto compaction = Clock::now();
   if (sa_bits == 32) {
-    std::vector<std::uint32_t> values32(n);
+    detail::U32Values values32(n);
     check(sfk_hip_memcpy_d2h(values32.data(), sa.p, n * 4), "SA download");
     impl.sa = Store(std::move(values32), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
   } else {
-    std::vector<std::uint64_t> values64(n);
+    detail::U64Values values64(n);
     check(sfk_hip_memcpy_d2h(values64.data(), sa.p, n * 8), "SA download");
     impl.sa = Store(std::move(values64), stored_width, n, ErrorCode::kBuildFailure, "suffix array");
   }
@@ -459,7 +460,7 @@ struct ByteReader {
 
 // Legacy (format 1.0-1.3) coordinate sections: u64 count [+ u8 width] +
 // values (suffix_array.cpp:1778-1830).
-std::vector<std::uint64_t> ReadLegacyValues(const std::vector<std::uint8_t>& bytes, bool has_width,
+detail::U64Values ReadLegacyValues(const std::vector<std::uint8_t>& bytes, bool has_width,
                                             std::uint8_t default_width, const char* what) {
   ByteReader r{bytes};
   const std::uint64_t count = r.U64(what);
@@ -471,7 +472,7 @@ std::vector<std::uint64_t> ReadLegacyValues(const std::vector<std::uint8_t>& byt
   if (width != 32 && width != 64) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " has an invalid width");
   const std::size_t w = width / 8;
   if ((bytes.size() - r.pos) != count * w) detail::Throw(ErrorCode::kCorruptIndex, std::string(what) + " has the wrong number of entries");
-  std::vector<std::uint64_t> out(count);
+  detail::U64Values out(count);
   for (std::uint64_t i = 0; i < count; ++i) {
     std::uint64_t v = 0;
     for (std::size_t b = 0; b < w; ++b) v |= static_cast<std::uint64_t>(bytes[r.pos + i * w + b]) << (8 * b);

@@ -46,8 +46,8 @@ void WriteBytes(std::ostream& out, const void* data, std::size_t size) {
   if (!out) Throw(ErrorCode::kIoError, "write to the index file failed");
 }
 
-template <class T>
-void WritePlane(std::ostream& out, const std::vector<T>& values) {
+template <class T, class A>
+void WritePlane(std::ostream& out, const std::vector<T, A>& values) {
   // Little-endian host: the in-memory layout is the file layout.
   constexpr std::size_t kBlock = 1u << 18;
   for (std::size_t begin = 0; begin < values.size(); begin += kBlock) {
@@ -78,9 +78,11 @@ std::uint64_t ReadU64(std::istream& in, const char* what) {
   return v;
 }
 
-template <class T>
-std::vector<T> ReadPlane(std::istream& in, std::uint64_t count, const char* what) {
-  std::vector<T> values;
+// Vec: std::vector<T>, or ValueVector<T> for coordinates and raw LCP (not
+// zero-filled before the read fills it).
+template <class T, class Vec = std::vector<T>>
+Vec ReadPlane(std::istream& in, std::uint64_t count, const char* what) {
+  Vec values;
   try {
     values.resize(static_cast<std::size_t>(count));
   } catch (const std::exception&) {
@@ -353,10 +355,10 @@ CoordinateStorage ReadCoordinateSectionV14(std::istream& in, std::uint64_t secti
   }
   switch (codec) {
     case CoordinateCodec::kNative32:
-      return CoordinateStorage::FromUInt32(ReadPlane<std::uint32_t>(in, count, label), width, symbols,
+      return CoordinateStorage::FromUInt32(ReadPlane<std::uint32_t, U32Values>(in, count, label), width, symbols,
                                            ErrorCode::kCorruptIndex, label);
     case CoordinateCodec::kNative64:
-      return CoordinateStorage::FromUInt64(ReadPlane<std::uint64_t>(in, count, label), width, symbols,
+      return CoordinateStorage::FromUInt64(ReadPlane<std::uint64_t, U64Values>(in, count, label), width, symbols,
                                            ErrorCode::kCorruptIndex, label);
     case CoordinateCodec::kSplit40: {
       Coordinate40 v;
@@ -453,8 +455,8 @@ LcpStorage ReadLcpSectionV14(std::istream& in, std::uint64_t section_size, std::
       anchors != 0 || anchor_pos != 0 || anchor_val != 0 || kLcpHeaderBytes + primary != section_size) {
     Throw(ErrorCode::kCorruptIndex, "SKL1 section: raw plane size is inconsistent");
   }
-  if (width == 32) return LcpStorage::FromRaw32(ReadPlane<std::uint32_t>(in, rows, "raw32 LCP"), rate, ErrorCode::kCorruptIndex);
-  return LcpStorage::FromRaw64(ReadPlane<std::uint64_t>(in, rows, "raw64 LCP"), rate, ErrorCode::kCorruptIndex);
+  if (width == 32) return LcpStorage::FromRaw32(ReadPlane<std::uint32_t, U32Values>(in, rows, "raw32 LCP"), rate, ErrorCode::kCorruptIndex);
+  return LcpStorage::FromRaw64(ReadPlane<std::uint64_t, U64Values>(in, rows, "raw64 LCP"), rate, ErrorCode::kCorruptIndex);
 }
 
 // ---------------------------------------------------------- container ---
