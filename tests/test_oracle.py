@@ -142,3 +142,71 @@ def test_reference_caps_matches_the_oracle(bits):
         sa, isa_, lcp, _ = oracle.ref_caps_fast(text, bits, 4)
         osa, oisa, olcp = oracle.fast_build(text, "doubling")
         assert np.array_equal(sa, osa) and np.array_equal(isa_, oisa) and np.array_equal(lcp, olcp)
+
+
+def test_sum_digest_sees_changed_swapped_and_shifted_values():
+    # The digest the device reduction computes: a changed value, two swapped
+    # rows and a shifted run each change it; the value width does not.
+    values = np.arange(7, 40007, dtype=np.uint64) * 3
+    base = oracle.sum_digest(values)
+    changed = values.copy()
+    changed[1234] += 1
+    swapped = values.copy()
+    swapped[[5, 30000]] = swapped[[30000, 5]]
+    shifted = np.concatenate([values[1:], values[:1]])
+    digests = {base, oracle.sum_digest(changed), oracle.sum_digest(swapped), oracle.sum_digest(shifted),
+               oracle.sum_digest(values[:-1])}
+    assert len(digests) == 5
+    assert oracle.sum_digest(values.astype(np.uint32)) == base
+    assert oracle.sum_digest(np.zeros(0, np.uint64)) == 0
+
+
+def _lean_cases():
+    return [
+        ("random", synth.random_text(61, 20000)),
+        ("symbols", synth.random_symbols(62, 15000)),
+        ("runs", synth.run_text(63, [("rand", 300), ("A", 2500), ("tandem2", 900), ("N", 400), ("sep", 1),
+                                     ("tandem7", 1200), ("rand", 500)])),
+        ("repeats", synth.repeat_text(64, 30000)),
+    ]
+
+
+@pytest.mark.parametrize("name,text", _lean_cases())
+def test_lean_digests_equal_the_full_oracle(name, text):
+    # The lean digests (ISA scatter and Phi/PLCP from a 32-bit SA, the form
+    # the 3.1 Gbp golden digests use) equal the digests of the oracle's
+    # pinned ISA and Kasai LCP arrays, and LCP max.
+    sa, isa_, lcp = oracle.fast_build(text, "doubling")
+    got = oracle.lean_digests(text, sa.astype(np.uint32))
+    assert got["sa"] == oracle.sum_digest(sa)
+    assert got["isa"] == oracle.sum_digest(isa_)
+    assert got["lcp"] == oracle.sum_digest(lcp)
+    assert got["lcp_max"] == int(lcp.max())
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built")
+@pytest.mark.parametrize("name,text", _lean_cases()[2:])
+def test_lean_digests_equal_the_compiled_reference(name, text):
+    sa, isa_, lcp = oracle.fast_build(text, "ref64")
+    got = oracle.lean_digests(text, sa.astype(np.uint32))
+    assert (got["sa"], got["isa"], got["lcp"]) == tuple(oracle.sum_digest(a) for a in (sa, isa_, lcp))
+    ref_sa, secs = oracle.ref_divsufsort64_u32(text)
+    assert np.array_equal(ref_sa, sa.astype(np.uint32)) and secs >= 0
+
+
+def test_lean_digests_catch_planted_mistakes():
+    text = synth.repeat_text(65, 20000)
+    sa, isa_, lcp = oracle.fast_build(text, "doubling")
+    sa32 = sa.astype(np.uint32)
+    want = oracle.lean_digests(text, sa32)
+    # Two adjacent rows whose LCPs with their neighbours differ, so that the
+    # swap changes the LCP as well as the order.
+    r = int(np.flatnonzero((lcp[1:-1] != lcp[2:]) & (lcp[1:-1] > 0))[10]) + 1
+    swapped = sa32.copy()
+    swapped[[r, r + 1]] = swapped[[r + 1, r]]
+    got = oracle.lean_digests(text, swapped)
+    assert got["sa"] != want["sa"] and got["isa"] != want["isa"] and got["lcp"] != want["lcp"]
+    repeated = sa32.copy()
+    repeated[7] = repeated[8]
+    with pytest.raises(RuntimeError):
+        oracle.lean_digests(text, repeated)
