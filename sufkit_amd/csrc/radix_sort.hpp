@@ -183,13 +183,30 @@ __device__ __forceinline__ std::uint64_t LookBackWave(unsigned long long* status
   return prefix;
 }
 
+// Publishes digit d's count of a tile (the first of its kW lanes stores):
+// its inclusive value first + value for tile 0, its aggregate otherwise.  A
+// tile publishes as soon as its counts are known, so that later tiles can
+// look past it while it does other work before its own look-back.
+template <int kW>
+__device__ inline void PublishDigit(unsigned long long* status, std::uint64_t tile, int d, std::uint64_t value,
+                                    std::uint64_t first, std::uint32_t epoch) {
+  constexpr int kRadix = 256;
+  if (static_cast<int>(threadIdx.x) % kW != 0) return;
+  const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
+  unsigned long long* slot = status + tile * kRadix + static_cast<std::uint64_t>(d);
+  const std::uint64_t word = tile == 0 ? (onesweep::kFlagInclusive | ep | (first + value))
+                                       : (onesweep::kFlagAggregate | ep | value);
+  __hip_atomic_store(slot, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The onesweep look-back of all kRadix digits of a tile by a whole block of
-// kW * kRadix threads: the kW consecutive lanes of digit d = threadIdx.x / kW
-// each read the status word of a different predecessor tile (lane w reads
-// tile hi - w), so one round trip covers kW tiles; a ballot finds the nearest
-// inclusive word within the group and the group sums the counts up to it.
-// Every thread calls it; returns the digit's exclusive prefix (in all kW
-// lanes).  Publication, epochs and the bounded wait are those of LookBack.
+// kW * kRadix threads, after PublishDigit: the kW consecutive lanes of digit
+// d = threadIdx.x / kW each read the status word of a different predecessor
+// tile (lane w reads tile hi - w), so one round trip covers kW tiles; a
+// ballot finds the nearest inclusive word within the group and the group
+// sums the counts up to it.  Every thread calls it; returns the digit's
+// exclusive prefix (in all kW lanes) and publishes the inclusive value.
+// Epochs and the bounded wait are those of LookBack.
 template <int kW>
 __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::uint64_t tile, int d,
                                                std::uint64_t value, std::uint64_t first, std::uint32_t epoch,
@@ -203,16 +220,7 @@ __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::
   const std::uint32_t group_base = lane - static_cast<std::uint32_t>(w);
   constexpr std::uint64_t kGroupMask = kW == 64 ? ~0ull : ((1ull << kW) - 1);
   unsigned long long* slot = status + tile * kRadix + static_cast<std::uint64_t>(d);
-  if (tile == 0) {
-    if (w == 0) {
-      __hip_atomic_store(slot, onesweep::kFlagInclusive | ep | (first + value), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return first;
-  }
-  if (w == 0) {
-    __hip_atomic_store(slot, onesweep::kFlagAggregate | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (tile == 0) return first;  // published inclusive by PublishDigit
   std::uint64_t prefix = 0;
   std::int64_t hi = static_cast<std::int64_t>(tile) - 1;
   std::uint32_t spins = 0;
@@ -537,38 +545,48 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     tile_cnt[d] = s;
   }
   __syncthreads();
-  {
-    // Publish this tile's counts, then look back for the exclusive prefixes
-    // (the whole block, kBlock / kRadix lanes per digit).
-    constexpr int kW = kBlock / kRadix;
-    const int d = static_cast<int>(threadIdx.x) / kW;
-    const std::uint64_t first = (!kSegmented && tile == 0) ? bucket_start[d] : 0;
-    const std::uint64_t p = LookBackDigits<kW>(status, tile, d, tile_cnt[d], first, epoch, error_flags, error_bit);
-    if (static_cast<int>(threadIdx.x) % kW == 0) global_base[d] = p;
-  }
-  __syncthreads();
+  // Publish this tile's counts, then (while predecessors finish theirs) scan
+  // them into the tile's digit starts and fold those into every wave's
+  // offsets, then look back for the exclusive prefixes (the whole block,
+  // kBlock / kRadix lanes per digit).
+  constexpr int kW = kBlock / kRadix;
+  const int dg = static_cast<int>(threadIdx.x) / kW;
+  const std::uint64_t first = (!kSegmented && tile == 0) ? bucket_start[dg] : 0;
+  PublishDigit<kW>(status, tile, dg, tile_cnt[dg], first, epoch);
   {
     // Exclusive scan of the tile's digit counts (threads >= kRadix add 0).
     std::uint32_t total;
     const std::uint32_t v = threadIdx.x < kRadix ? my_count : 0u;
     const std::uint32_t ex = BlockExclusiveSum<kBlock>(v, scan_scratch, total);
-    if (threadIdx.x < kRadix) local_start[threadIdx.x] = ex;
-  }
-  if constexpr (kSegmented) {
-    // A tile within one segment (the usual case) folds the segment's
-    // placement into its digit bases once, instead of per element.
-    if (seg_first == seg_last && threadIdx.x < kRadix) {
-      global_base[threadIdx.x] +=
-          static_cast<std::uint64_t>(seg.adj[static_cast<std::uint64_t>(seg_first) * kRadix + threadIdx.x]);
+    if (threadIdx.x < kRadix) {
+      local_start[threadIdx.x] = ex;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] += ex;
     }
   }
+  {
+    const std::uint64_t p = LookBackDigits<kW>(status, tile, dg, tile_cnt[dg], first, epoch, error_flags, error_bit);
+    if (static_cast<int>(threadIdx.x) % kW == 0) global_base[dg] = p;
+  }
   __syncthreads();
+  if (threadIdx.x < kRadix) {
+    // Output slot of staged slot p with digit d: global_base[d] + p from now on.
+    std::uint64_t b = global_base[threadIdx.x] - local_start[threadIdx.x];
+    if constexpr (kSegmented) {
+      // A tile within one segment (the usual case) folds the segment's
+      // placement into its digit bases once, instead of per element.
+      if (seg_first == seg_last) {
+        b += static_cast<std::uint64_t>(seg.adj[static_cast<std::uint64_t>(seg_first) * kRadix + threadIdx.x]);
+      }
+    }
+    global_base[threadIdx.x] = b;
+  }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
     if (idx < end) {
       const std::uint32_t d = digit[j];
-      const std::uint32_t pos = local_start[d] + wave_cnt[wave][d] + rank[j];
+      const std::uint32_t pos = wave_cnt[wave][d] + rank[j];
       s_keys[pos] = key[j];
       if constexpr (!kSplit) s_vals[pos] = val[j];
       if constexpr (kSegmented) {
@@ -583,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   __syncthreads();
   const std::uint32_t tile_n = static_cast<std::uint32_t>(end > base ? end - base : 0);
   auto place = [&](std::uint32_t p, std::uint32_t d) {
-    std::uint64_t out = global_base[d] + (p - local_start[d]);
+    std::uint64_t out = global_base[d] + p;
     if constexpr (kSegmented) {
       if (seg_first != seg_last) {
         const std::uint64_t sg = seg_first + s_seg[p];
@@ -624,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
       const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
       if (idx < end) {
         const std::uint32_t d = digit[j];
-        s_vals[local_start[d] + wave_cnt[wave][d] + rank[j]] = val[j];
+        s_vals[wave_cnt[wave][d] + rank[j]] = val[j];
       }
     }
     __syncthreads();
