@@ -201,19 +201,25 @@ __device__ inline void PublishDigit(unsigned long long* status, std::uint64_t ti
 
 // The onesweep look-back of all kRadix digits of a tile by a whole block of
 // kW * kRadix threads, after PublishDigit: the kW consecutive lanes of digit
-// d = threadIdx.x / kW each read the status word of a different predecessor
-// tile (lane w reads tile hi - w), so one round trip covers kW tiles; a
-// ballot finds the nearest inclusive word within the group and the group
-// sums the counts up to it.  Every thread calls it; returns the digit's
-// exclusive prefix (in all kW lanes) and publishes the inclusive value.
-// Epochs and the bounded wait are those of LookBack.
-template <int kW>
+// d = threadIdx.x / kW each read kReach status words of different
+// predecessor tiles (lane w's q-th word is tile hi - (q * kW + w)), so one
+// round trip covers kW * kReach tiles; ballots find the nearest inclusive
+// word and the group sums the counts up to it.  Every thread calls it;
+// returns the digit's exclusive prefix (in all kW lanes) and publishes the
+// inclusive value.  Epochs and the bounded wait are those of LookBack.  A
+// probe of the 1024 x 16 tile found the nearest inclusive tile about 13
+// tiles back: 3.3 round trips with one word per lane, 2.0 with four, but the
+// four-word form took 5.7 instead of 4.1 us per tile (the agent-scope loads
+// themselves are the cost), so one word per lane is the default.
+template <int kW, int kReach = 1>
 __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::uint64_t tile, int d,
                                                std::uint64_t value, std::uint64_t first, std::uint32_t epoch,
                                                unsigned int* error_flags, std::uint32_t error_bit) {
   constexpr int kRadix = 256;
   constexpr std::uint32_t kSpinLimit = 1u << 20;
   static_assert(kW >= 1 && kW <= 64 && (kW & (kW - 1)) == 0, "digit groups within a wave");
+  static_assert(kReach >= 1 && kReach <= 8, "status words per lane and round trip");
+  constexpr int kSpan = kW * kReach;
   const std::uint64_t ep = static_cast<std::uint64_t>(epoch & onesweep::kEpochMask) << 48;
   const int w = static_cast<int>(threadIdx.x) % kW;
   const std::uint32_t lane = LaneId();
@@ -227,20 +233,36 @@ __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::
   bool done = false;
   while (true) {
     if (!done) {
-      const std::int64_t t = hi - w;
-      std::uint64_t word = onesweep::kFlagInclusive | ep;  // before tile 0: an inclusive zero
-      if (t >= 0) {
-        word = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * kRadix + static_cast<std::uint64_t>(d),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      std::uint64_t word[kReach];
+#pragma unroll
+      for (int q = 0; q < kReach; ++q) {
+        const std::int64_t t = hi - (q * kW + w);
+        word[q] = onesweep::kFlagInclusive | ep;  // before tile 0: an inclusive zero
+        if (t >= 0) {
+          word[q] = __hip_atomic_load(status + static_cast<std::uint64_t>(t) * kRadix + static_cast<std::uint64_t>(d),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
-      const std::uint64_t flag = word & (3ull << 62);
-      const bool ready = flag != 0 && ((word >> 48) & onesweep::kEpochMask) == (epoch & onesweep::kEpochMask);
-      const bool incl = ready && flag == onesweep::kFlagInclusive;
-      const std::uint64_t rmask = (__ballot(ready) >> group_base) & kGroupMask;
-      const std::uint64_t imask = (__ballot(incl) >> group_base) & kGroupMask;
-      const int first_incl = imask ? __ffsll(static_cast<unsigned long long>(imask)) - 1 : kW;
-      const std::uint64_t need = first_incl < kW ? ((2ull << first_incl) - 1) : kGroupMask;
-      if ((rmask & need) != need) {
+      // Nearest inclusive word in distance order q * kW + w, and whether
+      // every word up to it (or the whole span when there is none) is ready.
+      int limit = kSpan;  // distance of the nearest inclusive word, kSpan: none
+      bool all_ready = true;
+#pragma unroll
+      for (int q = 0; q < kReach; ++q) {
+        const std::uint64_t flag = word[q] & (3ull << 62);
+        const bool ready =
+            flag != 0 && ((word[q] >> 48) & onesweep::kEpochMask) == (epoch & onesweep::kEpochMask);
+        const bool incl = ready && flag == onesweep::kFlagInclusive;
+        const std::uint64_t rmask = (__ballot(ready) >> group_base) & kGroupMask;
+        const std::uint64_t imask = (__ballot(incl) >> group_base) & kGroupMask;
+        if (limit == kSpan) {
+          const int fi = imask ? __ffsll(static_cast<unsigned long long>(imask)) - 1 : kW;
+          const std::uint64_t need = fi < kW ? ((2ull << fi) - 1) : kGroupMask;
+          if ((rmask & need) != need) all_ready = false;
+          if (fi < kW) limit = q * kW + fi;
+        }
+      }
+      if (!all_ready) {
         if (++spins >= kSpinLimit) {
           atomicOr(error_flags, error_bit);
           done = true;
@@ -252,14 +274,18 @@ __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::
         }
       } else {
         spins = 0;
-        std::uint64_t v = w <= first_incl ? (word & onesweep::kCountMask) : 0ull;
+        std::uint64_t v = 0;
+#pragma unroll
+        for (int q = 0; q < kReach; ++q) {
+          if (q * kW + w <= limit) v += word[q] & onesweep::kCountMask;
+        }
 #pragma unroll
         for (int o = 1; o < kW; o <<= 1) v += WaveShfl(v, static_cast<int>(lane ^ static_cast<std::uint32_t>(o)));
         prefix += v;
-        if (first_incl < kW) {
+        if (limit < kSpan) {
           done = true;
         } else {
-          hi -= kW;
+          hi -= kSpan;
         }
       }
     }
