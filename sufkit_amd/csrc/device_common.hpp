@@ -161,7 +161,7 @@ __device__ inline std::uint64_t MatchDigit(std::uint32_t digit, bool valid) {
 #pragma unroll
   for (int b = 0; b < kBits; ++b) {
     const std::uint32_t spread = static_cast<std::uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(digit), b, 1));
-    const std::uint64_t vote = __ballot(spread != 0u);
+    const std::uint64_t vote = __builtin_amdgcn_uicmp(spread, 0u, 33);
     diff_lo |= static_cast<std::uint32_t>(vote) ^ spread;
     diff_hi |= static_cast<std::uint32_t>(vote >> 32) ^ spread;
   }

@@ -540,7 +540,6 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
   // Stable rank of every item among the wave's items of its digit so far:
   // every lane reads its digit's count, and the last lane of each group of
   // equal digits writes the count back raised by the group's size.
-  const std::uint64_t lanes_below = LaneMaskLt();
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const std::uint64_t idx = wave_base + static_cast<std::uint64_t>(j) * kWave + lane;
@@ -548,8 +547,9 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
     digit[j] = d;
     const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
-    const std::uint32_t below = static_cast<std::uint32_t>(__popcll(peers & lanes_below));
-    const std::uint32_t count = static_cast<std::uint32_t>(__popcll(peers));
+    const std::uint32_t p_lo = static_cast<std::uint32_t>(peers), p_hi = static_cast<std::uint32_t>(peers >> 32);
+    const std::uint32_t below = __builtin_amdgcn_mbcnt_hi(p_hi, __builtin_amdgcn_mbcnt_lo(p_lo, 0u));
+    const std::uint32_t count = static_cast<std::uint32_t>(__popc(p_lo) + __popc(p_hi));
     const std::uint32_t old = wave_cnt[wave][d];
     __builtin_amdgcn_wave_barrier();
     if (valid && below + 1 == count) wave_cnt[wave][d] = old + count;
