@@ -716,6 +716,7 @@ def main() -> int:
             "refine_steps": last["refine_steps"], "doubling_rounds": last["doubling_rounds"],
             "active_total": last["active_total"], "sort_passes": last["sort_passes"],
             "lce_long_items": last["lce_long_items"],
+            "lcp_gathered_rows": last["lcp_gathered_rows"],
         },
         "parity": parity,
         "pcie_seconds": {"h2d_text": round(h2d, 4), "d2h_results": round(d2h, 4) if full_copy else None},

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SFK_HIP_ABI_VERSION 3
+#define SFK_HIP_ABI_VERSION 4
 
 enum sfk_hip_status {
   SFK_HIP_OK = 0,
@@ -78,6 +78,9 @@ typedef struct sfk_hip_stats {
   double prefix_directory_seconds; /* sfk_hip_build_fast's directory pass */
   double call_seconds;      /* whole host call: allocation, copies, build,
                                directory (host entry points only) */
+  uint64_t lcp_gathered_rows; /* rows whose LCP the LCP pass gathered from
+                                 PLCP; the others were fixed when the
+                                 doubling separated them */
 } sfk_hip_stats;
 
 typedef struct sfk_hip_builder sfk_hip_builder;

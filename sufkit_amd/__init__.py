@@ -64,6 +64,7 @@ class HipStats(ctypes.Structure):
         ("sort_scatter_bytes", ctypes.c_uint64),
         ("prefix_directory_seconds", ctypes.c_double),
         ("call_seconds", ctypes.c_double),
+        ("lcp_gathered_rows", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

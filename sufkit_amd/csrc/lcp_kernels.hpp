@@ -326,20 +326,60 @@ __global__ __launch_bounds__(256) void PlcpScanApplyKernel(std::uint32_t* __rest
   }
 }
 
-// LCP[r] = PLCP[SA[r]] in the requested output width; LCP[0] = 0.  One row
-// per thread (a grid-stride loop only past 2^32 threads).
+// LCP[r] = PLCP[SA[r]] in the requested output width; LCP[0] = 0.  Each
+// thread takes kGatherRows consecutive rows (a grid-stride loop only past
+// 2^32 threads): their SA (and LCP) values first, then every PLCP gather its
+// rows need, so that several random reads are in flight per thread.  early:
+// the LCP array already holds the rows the doubling fixed (EarlyLcp) and the
+// marker elsewhere; only marked rows are gathered, and their number is
+// counted.
+constexpr int kGatherRows = 8;
+
 template <typename Index, typename Out>
 __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __restrict__ plcp,
                                                        const Index* __restrict__ sa, std::uint64_t n,
-                                                       Out* __restrict__ lcp) {
-  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * 256;
-  for (std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x; r < n; r += stride) {
+                                                       Out* __restrict__ lcp, bool early,
+                                                       DeviceCounters* counters) {
+  constexpr Out kUnknown = sizeof(Out) == 4 ? static_cast<Out>(kLcpUnknown32) : static_cast<Out>(kLcpUnknown64);
+  __shared__ std::uint32_t s_gathered;
+  if (threadIdx.x == 0) s_gathered = 0;
+  __syncthreads();
+  std::uint32_t gathered = 0;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * 256 * kGatherRows;
+  for (std::uint64_t r0 = (static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x) * kGatherRows; r0 < n;
+       r0 += stride) {
     // SA and LCP stream through once: non-temporal, so that they do not
     // evict the randomly read PLCP lines.  An out-of-range SA value (the
     // checks have flagged it) reads nothing.
-    const std::uint64_t v = static_cast<std::uint64_t>(__builtin_nontemporal_load(&sa[r]));
-    const Out out = (r == 0 || v >= n) ? Out(0) : static_cast<Out>(plcp[v]);
-    __builtin_nontemporal_store(out, &lcp[r]);
+    std::uint64_t v[kGatherRows];
+    Out out[kGatherRows];
+    bool need[kGatherRows];
+#pragma unroll
+    for (int i = 0; i < kGatherRows; ++i) {
+      const std::uint64_t r = r0 + i;
+      v[i] = r < n ? static_cast<std::uint64_t>(__builtin_nontemporal_load(&sa[r])) : n;
+      out[i] = (r < n && early) ? __builtin_nontemporal_load(&lcp[r]) : kUnknown;
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherRows; ++i) {
+      const std::uint64_t r = r0 + i;
+      need[i] = r < n && out[i] == kUnknown;
+      if (need[i]) out[i] = (r == 0 || v[i] >= n) ? Out(0) : static_cast<Out>(plcp[v[i]]);
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherRows; ++i) {
+      if (need[i]) {
+        __builtin_nontemporal_store(out[i], &lcp[r0 + i]);
+        ++gathered;
+      }
+    }
+  }
+  if (early) {
+    if (gathered != 0) atomicAdd(&s_gathered, gathered);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_gathered != 0) {
+      atomicAdd(&counters->lcp_gathered[blockIdx.x & 15u], static_cast<unsigned long long>(s_gathered));
+    }
   }
 }
 

@@ -563,8 +563,10 @@ __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
 template <typename Index, typename R>
 class Doubling {
  public:
-  Doubling(BuildContext& ctx, R* sa, R* rank, PackedText text)
-      : ctx_(ctx), b_(ctx.b), n_(ctx.n), sa_(sa), rank_(rank), text_(text) {}
+  // early: where the first step and the exact local step write early LCP
+  // values (the caller's LCP buffer), or off.
+  Doubling(BuildContext& ctx, R* sa, R* rank, PackedText text, EarlyLcp early)
+      : ctx_(ctx), b_(ctx.b), n_(ctx.n), sa_(sa), rank_(rank), text_(text), early_(early) {}
 
   void Run() {
     DeviceCounters* dc = b_->counters.As<DeviceCounters>();
@@ -846,7 +848,7 @@ class Doubling {
     const unsigned grid = static_cast<unsigned>(CeilDiv(qm_, kLocalOwn));
     LocalRefineKernel<Index, kExact><<<grid, kLocalThreads, 0, b_->stream>>>(
         text_, q_idx_cur_, q_row0_cur_, b_->q_key.As<Index>(), qm_, n_, sa_, rank_, q_idx_next_,
-        q_row0_next_, dc);
+        q_row0_next_, dc, kExact ? early_ : EarlyLcp{nullptr, nullptr});
     SFK_HIP_CHECK(hipGetLastError());
     constexpr std::uint64_t w = sizeof(Index);
     ctx_.stats.refine_steps += 1;
@@ -962,7 +964,8 @@ class Doubling {
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
           b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
           b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), dc,
-          segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u);
+          segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u,
+          sec_kind == kSecFillHi ? early_ : EarlyLcp{nullptr, nullptr});
     } else {
       const SortGeometry g = Geometry(m, kRefineChunkTile, kRefineMaxChunks);
       RefinePartial* partial = b_->partial.As<RefinePartial>();
@@ -970,9 +973,13 @@ class Doubling {
       RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
       RefineApplyChunkedKernel<Index, K><<<g.grid, kRefineChunkThreads, 0, b_->stream>>>(
           keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_, b_->act_idx.As<Index>(),
-          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, nullptr, 0u);
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, nullptr, 0u,
+          sec_kind == kSecFillHi ? early_ : EarlyLcp{nullptr, nullptr});
     }
     SFK_HIP_CHECK(hipGetLastError());
+    if (sec_kind == kSecFillHi) {
+      ctx_.stats.algorithmic_bytes_sa += m * (early_.lcp32 ? 4 : early_.lcp64 ? 8 : 0);  // early LCP or marker per row
+    }
     pending_apply_ = true;
     pending_m_ = m;
     pending_key_bytes_ = sizeof(K);
@@ -988,6 +995,7 @@ class Doubling {
   R* sa_;
   R* rank_;
   PackedText text_;
+  EarlyLcp early_;
   // Large list: suffixes, segment ids, per-segment deltas.  After a
   // RefineApply whose sizes are not read back yet, lm_bound_ bounds lm_.
   std::uint64_t lm_ = 0;
@@ -1028,7 +1036,7 @@ unsigned FlatGrid(std::uint64_t m, std::uint64_t per_block) {
 // bucket-start position bitmap and the long-LCE queue.
 template <typename Index, typename Out>
 void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const Index* sa, const Index* isa,
-               Out* lcp) {
+               Out* lcp, bool early = false) {
   sfk_hip_builder* b = ctx.b;
   const std::uint64_t n = ctx.n;
   DeviceCounters* dc = b->counters.As<DeviceCounters>();
@@ -1066,13 +1074,14 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
   PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial, dc);
   PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
   PlcpScanApplyKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
-  LcpGatherKernel<Index, Out><<<FlatGrid(n, 256), 256, 0, b->stream>>>(plcp, sa, n, lcp);
+  LcpGatherKernel<Index, Out><<<FlatGrid(n, 256 * kGatherRows), 256, 0, b->stream>>>(plcp, sa, n, lcp, early, dc);
   SFK_HIP_CHECK(hipGetLastError());
   ctx.stats.algorithmic_bytes_lcp +=
       n / 8                            // bucket-start bitmap
       + n * (w + w + 4) + n / 2        // PLCP: ISA read, SA[ISA[i] - 1] (with SA[ISA[i]]), PLCP write, text at i
       + n * (3 * 4)                    // max-scan: two reads, one write
-      + n * (w + 4 + sizeof(Out));     // LCP = PLCP[SA]: SA, PLCP, LCP
+      + (early ? n * sizeof(Out)       // early LCP read (the gathered rows are booked after the build)
+               : n * (w + 4 + sizeof(Out)));  // LCP = PLCP[SA]: SA, PLCP, LCP
 }
 
 template <typename In, typename Out>
@@ -1191,12 +1200,22 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   }
   ctx.stats.algorithmic_bytes_sa += n + words * 16;
 
-  Doubling<Index, R>(ctx, sa, rank, text).Run();
+  // Early LCP into the caller's LCP buffer (complete builds only: a sampled
+  // build computes its LCP in the workspace).
+  const bool want_lcp = d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
+  EarlyLcp early{nullptr, nullptr};
+  if (want_lcp && !sampled) {
+    if (lcp_bits == 32) {
+      early.lcp32 = static_cast<std::uint32_t*>(d_lcp);
+    } else {
+      early.lcp64 = static_cast<std::uint64_t*>(d_lcp);
+    }
+  }
+  Doubling<Index, R>(ctx, sa, rank, text, early).Run();
   RecordPhase(b, 1);
 
   // K9: permutation + inverse check on the device (fused into the PLCP pass
   // when the LCP is built).
-  const bool want_lcp = d_lcp != nullptr && (flags & SFK_HIP_RETAIN_LCP);
   if (!want_lcp) {
     ValidateKernel<R><<<static_cast<unsigned>(CeilDiv(n, 256)), 256, 0, b->stream>>>(
         sa, rank, n, b->counters.As<DeviceCounters>());
@@ -1210,9 +1229,9 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
       // doubling), for the interval minima of the sampled rows.
       if constexpr (!kNarrowRank) LaunchLcp<Index, Index>(ctx, d_text, text, sa, rank, b->vals0.As<Index>());
     } else if (lcp_bits == 32) {
-      LaunchLcp<R, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
+      LaunchLcp<R, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp), true);
     } else {
-      LaunchLcp<R, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
+      LaunchLcp<R, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp), true);
     }
   }
   RecordPhase(b, 3);
@@ -1269,6 +1288,14 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   }
   ctx.stats.lce_long_items = c.long_items;
   if (want_lcp) ctx.stats.algorithmic_bytes_lcp += c.irreducible * 32;  // window pair at Φ(i) per irreducible i
+  if (early.lcp32 != nullptr || early.lcp64 != nullptr) {
+    std::uint64_t gathered = 0;
+    for (unsigned long long g : c.lcp_gathered) gathered += g;
+    ctx.stats.lcp_gathered_rows = gathered;
+    ctx.stats.algorithmic_bytes_lcp += gathered * (sizeof(R) + 4 + lcp_bits / 8);  // SA, PLCP, LCP per gathered row
+  } else if (want_lcp) {
+    ctx.stats.lcp_gathered_rows = n;
+  }
   ctx.stats.lce_long_words = c.long_words;
   if (b->phase[0] == nullptr) return;  // untimed build
   float ms = 0.f;

@@ -78,6 +78,7 @@ struct DeviceCounters {
   unsigned long long l_next;      // suffixes in the next large list
   unsigned long long l_next_segs; // segments in the next large list
   unsigned long long irreducible;  // PLCP positions computed directly
+  unsigned long long lcp_gathered[16];  // LCP rows gathered from PLCP, over 16 words (block % 16)
   unsigned int error_flags;
   unsigned int pad;
 };
@@ -603,6 +604,48 @@ __global__ __launch_bounds__(256) void MarkSpecialSegmentsKernel(
 // suffixes to their group's first row (a step whose successor rewrites every
 // survivor's rank before any rank is gathered skips it).
 
+// Early LCP values.  The LCP of a row is fixed when the row becomes the
+// first of its group: its predecessor row then belongs to the group just
+// before, and any two suffixes of the two groups share exactly the prefix
+// their sort keys share.  The first step (16-symbol keys) writes that value
+// for every group start whose keys prove it (see EarlyKeyLcp) and a marker
+// in every other row; the exact local step writes its group starts (LCP 16
+// to 31, or the exact mismatch in segments with special symbols).  The LCP
+// pass then reads PLCP only for rows that still hold the marker.  Both
+// pointers null: off.
+struct EarlyLcp {
+  std::uint32_t* lcp32;
+  std::uint64_t* lcp64;
+};
+constexpr std::uint32_t kLcpUnknown32 = 0xFFFFFFFFu;
+constexpr std::uint64_t kLcpUnknown64 = ~0ull;
+
+__device__ __forceinline__ void WriteEarlyLcp(const EarlyLcp& e, std::uint64_t row, std::uint32_t value,
+                                              bool known) {
+  if (e.lcp32 != nullptr) {
+    e.lcp32[row] = known ? value : kLcpUnknown32;
+  } else if (e.lcp64 != nullptr) {
+    e.lcp64[row] = known ? static_cast<std::uint64_t>(value) : kLcpUnknown64;
+  }
+}
+
+// LCP of two suffixes from their 16-symbol FillKey high halves a != b (2-bit
+// codes, first symbol in the top bits), when the keys prove it: the first
+// differing symbol m, provided neither window holds a special symbol at or
+// before m.  FillKey fills a window from its first special symbol on with
+// all-0 or all-1 codes, so a key whose symbols m..15 are not all 00 and not
+// all 11 has no special symbol at or before m; then symbols 0..m-1 are equal
+// bases and symbol m is a base in both, so the LCP is m.  Returns 32 when the
+// keys do not prove it.
+__device__ __forceinline__ std::uint32_t EarlyKeyLcp(std::uint32_t a, std::uint32_t b) {
+  const std::uint32_t m = static_cast<std::uint32_t>(__clz(a ^ b)) >> 1;
+  const std::uint32_t tail_bits = 32u - 2u * m;                   // symbols m..15
+  const std::uint32_t mask = tail_bits >= 32u ? 0xFFFFFFFFu : ((1u << tail_bits) - 1u);
+  const std::uint32_t ta = a & mask, tb = b & mask;
+  const bool proven = ta != 0u && ta != mask && tb != 0u && tb != mask;
+  return proven ? m : 32u;
+}
+
 struct RefinePartial {
   std::uint64_t survivors;
   std::uint64_t newsegs;
@@ -730,7 +773,8 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
                                        Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg,
                                        Index* __restrict__ out_delta, Index* __restrict__ out_start,
                                        bool survivor_rank, std::uint64_t run_surv, std::uint64_t run_segs,
-                                       std::uint64_t run_last, const RefineTileLds<K, kBlock * kItems>& lds) {
+                                       std::uint64_t run_last, const RefineTileLds<K, kBlock * kItems>& lds,
+                                       const EarlyLcp early) {
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int i = j * kBlock + static_cast<int>(threadIdx.x);
@@ -743,6 +787,17 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
     const Index delta = seg_delta[seg];
     const Index row = static_cast<Index>(delta + static_cast<Index>(k));
     if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
+    if (early.lcp32 != nullptr || early.lcp64 != nullptr) {
+      // The first step: every row gets its early LCP or the marker.
+      std::uint32_t value = 32u;
+      if ((f & 1) && k > 0) {
+        const K before = lds.key[i];
+        if ((static_cast<std::uint64_t>(before) >> sec_bits) == seg) {
+          value = EarlyKeyLcp(static_cast<std::uint32_t>(before), static_cast<std::uint32_t>(key));
+        }
+      }
+      WriteEarlyLcp(early, static_cast<std::uint64_t>(row), value, value < 32u);
+    }
     if (f & 2) {
       sa[row] = static_cast<R>(v);
       rank[v] = static_cast<R>(row);
@@ -772,7 +827,7 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
     Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
     unsigned int* tickets, DeviceCounters* counters,
-    const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
+    const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs, const EarlyLcp early) {
   constexpr int kBlock = kRefineThreads;
   constexpr int kTile = kRefineTile;
   __shared__ RefineTileLds<K, kTile> lds;
@@ -804,7 +859,7 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
   const std::uint64_t run_segs = s_prefix[1];
   RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
                                         out_delta, out_start, survivor_rank, run_surv, run_segs, s_prefix[2],
-                                        lds);
+                                        lds, early);
   if (threadIdx.x == 0 && base + len == m) {
     counters->list.survivors = run_surv + tot.surv;
     counters->list.segments = run_segs + tot.segs;
@@ -906,7 +961,8 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
     int sec_bits, const Index* __restrict__ seg_delta, const RefinePartial* __restrict__ partial,
     std::uint64_t n, R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
     std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta, Index* __restrict__ out_start,
-    bool survivor_rank, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs) {
+    bool survivor_rank, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs,
+    const EarlyLcp early) {
   constexpr int kBlock = kRefineChunkThreads;
   constexpr int kTile = kRefineChunkTile;
   __shared__ RefineTileLds<K, kTile> lds;
@@ -923,7 +979,8 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
         RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, ref_offsets, refs, lds, scratch, v);
     __syncthreads();
     RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
-                                          out_delta, out_start, survivor_rank, run_surv, run_segs, run_last, lds);
+                                          out_delta, out_start, survivor_rank, run_surv, run_segs, run_last, lds,
+                                          early);
     run_surv += tot.surv;
     run_segs += tot.segs;
     if (tot.last > 0) run_last = base + tot.last;
@@ -1317,6 +1374,20 @@ __device__ inline bool KeyEqual(T a, T b) {
   return a == b;
 }
 
+// First differing symbol of two different exact 96-bit keys (3 bits per
+// symbol, first symbol in the top bits of w0).
+__device__ inline std::uint32_t ExactKeyMismatch(const ExactKey& a, const ExactKey& b) {
+  std::uint32_t bit;
+  if (a.w0 != b.w0) {
+    bit = static_cast<std::uint32_t>(__clz(a.w0 ^ b.w0));
+  } else if (a.w1 != b.w1) {
+    bit = 32u + static_cast<std::uint32_t>(__clz(a.w1 ^ b.w1));
+  } else {
+    bit = 64u + static_cast<std::uint32_t>(__clz(a.w2 ^ b.w2));
+  }
+  return bit / 3u;
+}
+
 // All three words of the exact 96-bit key of a 32-symbol window.
 __device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
   std::uint64_t hi = 0;
@@ -1350,7 +1421,7 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     PackedText t, const Index* __restrict__ q_idx, const Index* __restrict__ q_row0,
     const Index* __restrict__ q_key, std::uint64_t qm, std::uint64_t n,
     R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
-    Index* __restrict__ out_row0, DeviceCounters* counters) {
+    Index* __restrict__ out_row0, DeviceCounters* counters, const EarlyLcp early) {
   using KeyT = typename std::conditional<kExact, std::uint32_t, Index>::type;
   constexpr int kItems = kLocalItems;
   constexpr std::uint32_t kSizeMask = 0xFFFFu;  // s_count: size | (specials << 16)
@@ -1474,11 +1545,16 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     }
     const int en = st[i] + static_cast<int>(size);
     std::uint32_t less = 0;
+    // The exact step also finds the largest smaller key (the group just
+    // before this one in the segment), for the early LCP of a group start.
+    std::uint32_t early_lcp = 32u;
     if (kExact && (packed >> 16) != 0) {
       const ExactKey key = s_exact[k];
+      ExactKey before{0u, 0u, 0u};
       for (int j = st[i]; j < en; ++j) {
         const ExactKey kj = s_exact[j];
         if (KeyLess(kj, key)) {
+          if (less == 0 || KeyLess(before, kj)) before = kj;
           ++less;
         } else if (KeyEqual(kj, key)) {
           if (equal[i] == 0) first_equal[i] = j;
@@ -1486,11 +1562,14 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
           tie[i] += j < k;
         }
       }
+      if (less > 0) early_lcp = ExactKeyMismatch(before, key);
     } else {
       const KeyT key = s_key[k];
+      KeyT before = 0;
       for (int j = st[i]; j < en; ++j) {
         const KeyT kj = s_key[j];
         if (kj < key) {
+          if (less == 0 || kj > before) before = kj;
           ++less;
         } else if (kj == key) {
           if (equal[i] == 0) first_equal[i] = j;
@@ -1498,8 +1577,16 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
           tie[i] += j < k;
         }
       }
+      // Segments without a special window: 16 equal symbols, then the low
+      // halves of FillKey, exact 2-bit codes.
+      if (kExact && less > 0) {
+        early_lcp = 16u + (static_cast<std::uint32_t>(__clz(static_cast<std::uint32_t>(before ^ key))) >> 1);
+      }
     }
     head[i] = static_cast<Index>(r0[i] + static_cast<Index>(less));
+    if (kExact && less > 0 && tie[i] == 0 && early_lcp < 32u && static_cast<std::uint64_t>(head[i]) < n) {
+      WriteEarlyLcp(early, static_cast<std::uint64_t>(head[i]), early_lcp, true);
+    }
     if (static_cast<std::uint64_t>(v[i]) >= n || static_cast<std::uint64_t>(head[i]) >= n) {
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
     } else if (equal[i] == 1) {
