@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SFK_HIP_ABI_VERSION 4
+#define SFK_HIP_ABI_VERSION 5
 
 enum sfk_hip_status {
   SFK_HIP_OK = 0,
@@ -94,6 +94,13 @@ int sfk_hip_abi_version(void);
  * hipMalloc would (SFK_HIP_BUILD_FAILURE), once; a negative value disarms.
  * Process-wide.  No counterpart in the reference. */
 void sfk_hip_inject_allocation_failure(long long after);
+
+/* Fault injection for tests of the device permutation check (the
+ * kBuildFailure of RepackSuffixArray's check, suffix_array.cpp:852-881): when
+ * armed (non-zero), the next build of at least 3 symbols swaps SA rows 1 and
+ * 2 after the doubling, before the check runs, and the hook disarms; zero
+ * disarms.  Process-wide.  No counterpart in the reference. */
+void sfk_hip_inject_sa_swap(int armed);
 
 /* Number of visible HIP devices (0 when none); never fails. */
 int sfk_hip_device_count(void);

@@ -23,6 +23,10 @@
 //                  PLCP[i] + i at every i (three phases over chunks).
 //   LcpGather      LCP[r] = PLCP[SA[r]].
 //
+// That full pass serves builds without early LCP (sampled ones).  A build
+// with early LCP computes PLCP only for the rows its doubling left marked
+// (marked-row LCP, at the end of this file).
+//
 // Random writes cost far more than random reads on this memory system (a
 // Φ built by scattering Φ[SA[r]] = SA[r-1] took as long as the whole PLCP
 // pass), so every random access here is a read.  PLCP values are 32-bit:
@@ -41,10 +45,13 @@ namespace sfk {
 constexpr std::uint32_t kPlcpReducible = 0xFFFFFFFFu;
 constexpr int kPlcpThreads = 256;
 
+// A long LCE left to PlcpLongKernel: the suffixes at i and j agree on their
+// first l symbols; the result goes to plcp[dst].
 struct LongLce {
   unsigned long long i;
   unsigned long long j;
   unsigned long long l;
+  unsigned long long dst;
 };
 
 template <int kBlock>
@@ -180,6 +187,7 @@ __global__ __launch_bounds__(kPlcpThreads) void PlcpKernel(PackedText t, const I
       queue[slot].i = i;
       queue[slot].j = phi;
       queue[slot].l = l;
+      queue[slot].dst = i;
     }
     plcp[i] = 0;  // the long-LCE pass writes the value
   }
@@ -196,6 +204,7 @@ __global__ __launch_bounds__(256) void PlcpLongKernel(PackedText t, std::uint64_
   const std::uint64_t i = queue[item].i;
   const std::uint64_t j = queue[item].j;
   std::uint64_t l = queue[item].l;
+  const std::uint64_t dst = queue[item].dst;
   unsigned long long steps = 0;
   while (true) {
     const std::uint64_t pa = i + l + 32ull * lane;
@@ -219,7 +228,7 @@ __global__ __launch_bounds__(256) void PlcpLongKernel(PackedText t, std::uint64_
     l += 32ull * kWave;
   }
   if (lane == 0) {
-    plcp[i] = static_cast<std::uint32_t>(l);
+    plcp[dst] = static_cast<std::uint32_t>(l);
     atomicAdd(&counters->long_words, steps * kWave);
   }
 }
@@ -380,6 +389,332 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
     if (threadIdx.x == 0 && s_gathered != 0) {
       atomicAdd(&counters->lcp_gathered[blockIdx.x & 15u], static_cast<unsigned long long>(s_gathered));
     }
+  }
+}
+
+
+// ------------------------------------------------------- marked-row LCP --
+//
+// After a build with early LCP, a row holds the marker only where the
+// doubling's keys did not prove its LCP; every row whose LCP is 32 or more is
+// such a row.  Only the marked rows need Φ/PLCP, so instead of a PLCP pass
+// over every text position (one random SA line per position):
+//   LcpMarkCount / LcpMarkWrite   rows in order: the (position SA[r],
+//                  Φ = SA[r-1]) pair of every marked row r >= 1 goes to a
+//                  list, whose position digits are counted for the radix
+//                  sort.  The same pass checks that ISA inverts SA (below).
+//   (radix sort of the list by position)
+//   SparsePlcp     the list in text order: Kasai's reducibility test between
+//                  neighbouring entries.  Entry k (position p) is reducible
+//                  when entry k-1 is position p-1, p-1 does not start a
+//                  bucket, and Φ(p-1) + 1 == Φ(p); then PLCP[p] = PLCP[p-1]
+//                  - 1.  Otherwise its LCE is computed from its windows (a
+//                  reducible position whose predecessor is unmarked is
+//                  computed too: its predecessor's LCP was proven below 32,
+//                  so its own is below 31, one window).
+//   SparseScan     PLCP[p] + p never decreases along the text, and the
+//                  entries form a subsequence of it, so the inclusive max of
+//                  PLCP + p + 1 over the irreducible entries up to k is
+//                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply writes
+//                  PLCP[p_k] into the position-indexed array.
+//   LcpGather      (early mode) reads PLCP[SA[r]] for the marked rows.
+//
+// The check that replaces the inverse check of the full PLCP pass: the pairs
+// (r, SA[r]) over rows and (ISA[i], i) over positions are the same multiset
+// exactly when SA is a permutation of [0, n) and ISA its inverse (each side
+// holds one pair per row and one per position).  Each pair is hashed by two
+// independent 64-bit finalizers of its packed 64-bit value (row and position
+// both lie below 2^32), and the pass accumulates, per finalizer, the sum over
+// rows minus the sum over positions, mod 2^64; the host requires both to be
+// zero, and the pass also flags any value at or past n.  A wrong SA or ISA
+// passes only if two sums of distinct pseudo-random 64-bit values collide in
+// both finalizers.
+
+constexpr int kMarkThreads = 256;
+constexpr int kMarkRows = 8;
+constexpr int kMarkTile = kMarkThreads * kMarkRows;
+
+template <typename T>
+__device__ __forceinline__ T LcpMarker() {
+  return sizeof(T) == 4 ? static_cast<T>(kLcpUnknown32) : static_cast<T>(kLcpUnknown64);
+}
+
+// Eight consecutive elements from k (a multiple of 8) of an array read once:
+// 16-byte loads when the array is 16-byte aligned and all eight lie below
+// end, else one load each; elements at or past end read as `fill`.
+template <typename T>
+__device__ __forceinline__ void LoadRun8(const T* __restrict__ a, std::uint64_t k, std::uint64_t end, bool aligned,
+                                         T fill, T (&v)[8]) {
+  if (aligned && k + 8 <= end) {
+    constexpr int kVec = static_cast<int>(8 * sizeof(T) / 16);
+    const uint4* q = reinterpret_cast<const uint4*>(a + k);
+    uint4 w[kVec];
+#pragma unroll
+    for (int i = 0; i < kVec; ++i) w[i] = q[i];
+    __builtin_memcpy(v, w, sizeof(v));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = k + i < end ? a[k + i] : fill;
+  }
+}
+
+// The two pair finalizers: MurmurHash3's fmix64 and SplitMix64's output
+// function, applied to (row << 32 | position) and (position << 32 | row).
+__device__ __forceinline__ std::uint64_t PairHashA(std::uint64_t row, std::uint64_t pos) {
+  std::uint64_t x = ((row << 32) | pos) ^ 0x6a09e667f3bcc909ull;
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+__device__ __forceinline__ std::uint64_t PairHashB(std::uint64_t row, std::uint64_t pos) {
+  std::uint64_t x = ((pos << 32) | row) + 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Marked rows r >= 1 of each chunk (chunks are whole kMarkTile tiles).
+template <typename Out>
+__global__ __launch_bounds__(kMarkThreads) void LcpMarkCountKernel(const Out* __restrict__ lcp, std::uint64_t n,
+                                                                   std::uint64_t chunk, bool aligned,
+                                                                   std::uint64_t* __restrict__ partial) {
+  __shared__ std::uint64_t scratch[kMarkThreads / kWave];
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
+  const Out marker = LcpMarker<Out>();
+  std::uint64_t count = 0;
+  for (std::uint64_t r0 = begin + static_cast<std::uint64_t>(threadIdx.x) * kMarkRows; r0 < end;
+       r0 += kMarkTile) {
+    Out v[8];
+    LoadRun8(lcp, r0, end, aligned, Out(0), v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) count += (v[i] == marker) & (r0 + i > 0);
+  }
+  std::uint64_t total;
+  BlockExclusiveSum<kMarkThreads>(count, scratch, total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Exclusive sum of the chunk counts (grid <= 1024), the total into *total.
+__global__ __launch_bounds__(1024) void MarkSpineKernel(std::uint64_t* partial, int grid,
+                                                        unsigned long long* total) {
+  __shared__ std::uint64_t scratch[1024 / kWave];
+  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  std::uint64_t sum;
+  const std::uint64_t excl = BlockExclusiveSum<1024>(valid ? partial[threadIdx.x] : std::uint64_t{0}, scratch, sum);
+  if (valid) partial[threadIdx.x] = excl;
+  if (threadIdx.x == 0) *total = sum;
+}
+
+// The list of marked rows (position SA[r], Φ = SA[r-1]) at the chunk's
+// offset, in row order, with the position digits counted for the sort and
+// the inverse check of SA and ISA (rows and positions share the index).
+template <typename R, typename Out>
+__global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
+    const Out* __restrict__ lcp, const R* __restrict__ sa, const R* __restrict__ isa, std::uint64_t n,
+    std::uint64_t chunk, bool lcp_aligned, bool sa_aligned, bool isa_aligned,
+    const std::uint64_t* __restrict__ partial, std::uint32_t* __restrict__ out_pos,
+    std::uint32_t* __restrict__ out_phi, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
+  constexpr int kWaves = kMarkThreads / kWave;
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t scratch32[kWaves];
+  __shared__ std::uint64_t scratch[kWaves];
+  ClearNextRadixSet(set);
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kMarkThreads) {
+    (&local[0][0][0])[i] = 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < n ? begin + chunk : n;
+  const Out marker = LcpMarker<Out>();
+  std::uint64_t at = partial[blockIdx.x];
+  std::uint64_t hash_a = 0, hash_b = 0;
+  std::uint32_t flags = 0;
+  // Every thread runs the same number of tile iterations (block scans).
+  for (std::uint64_t base = begin; base < end; base += kMarkTile) {
+    const std::uint64_t r0 = base + static_cast<std::uint64_t>(threadIdx.x) * kMarkRows;
+    Out l[8];
+    R s[8], q[8];
+    LoadRun8(lcp, r0, end, lcp_aligned, Out(0), l);
+    LoadRun8(sa, r0, end, sa_aligned, R(0), s);
+    LoadRun8(isa, r0, end, isa_aligned, R(0), q);
+    std::uint32_t marks = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const std::uint64_t r = r0 + i;
+      if (r < end) {
+        const std::uint64_t pos = static_cast<std::uint64_t>(s[i]);
+        const std::uint64_t row = static_cast<std::uint64_t>(q[i]);
+        flags |= pos >= n ? static_cast<std::uint32_t>(kFlagNotPermutation) : 0u;
+        flags |= row >= n ? static_cast<std::uint32_t>(kFlagNotInverse) : 0u;
+        hash_a += PairHashA(r, pos) - PairHashA(row, r);
+        hash_b += PairHashB(r, pos) - PairHashB(row, r);
+        if (l[i] == marker && r > 0) marks |= 1u << i;
+      }
+    }
+    // Φ of the thread's first row: SA[r0 - 1] (read only when that row is
+    // marked; the other rows take their predecessor from s[]).
+    const std::uint64_t before = (marks & 1u) ? static_cast<std::uint64_t>(sa[r0 - 1]) : 0ull;
+    std::uint32_t total;
+    const std::uint32_t off = BlockExclusiveSum<kMarkThreads>(static_cast<std::uint32_t>(__popc(marks)), scratch32,
+                                                              total);
+    std::uint64_t slot = at + off;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bool m = (marks >> i) & 1u;
+      const std::uint32_t pos = static_cast<std::uint32_t>(s[i]);
+      if (m) {
+        out_pos[slot] = pos;
+        out_phi[slot] = static_cast<std::uint32_t>(i == 0 ? before : static_cast<std::uint64_t>(s[i - 1]));
+        ++slot;
+      }
+      CountDigits<std::uint32_t>(local[wave], passes, pos, m, false);
+    }
+    at += total;
+  }
+  __syncthreads();
+  FlushDigits<kWaves>(local, passes, set.hist);
+  std::uint64_t sum_a, sum_b;
+  BlockExclusiveSum<kMarkThreads>(hash_a, scratch, sum_a);
+  BlockExclusiveSum<kMarkThreads>(hash_b, scratch, sum_b);
+  if (threadIdx.x == 0) {
+    if (sum_a) atomicAdd(&counters->inverse_hash[0], static_cast<unsigned long long>(sum_a));
+    if (sum_b) atomicAdd(&counters->inverse_hash[1], static_cast<unsigned long long>(sum_b));
+  }
+  if (flags) atomicOr(&counters->error_flags, flags);
+}
+
+// PLCP of the sorted list's entries (kPlcpReducible at reducible ones), one
+// wave per tile of kPlcpTile entries: lane 0 reads the entry before the tile
+// and each lane takes its predecessor by a lane shuffle.  LCEs longer than
+// word_limit windows go to the long queue (dst: the entry).
+__global__ __launch_bounds__(kPlcpThreads) void SparsePlcpKernel(
+    PackedText t, const std::uint32_t* __restrict__ pos, const std::uint32_t* __restrict__ phi, std::uint64_t m,
+    const std::uint32_t* __restrict__ starts, std::uint32_t word_limit, std::uint32_t* __restrict__ out,
+    LongLce* __restrict__ queue, std::uint64_t queue_capacity, DeviceCounters* counters) {
+  constexpr std::uint64_t kWavesPerBlock = kPlcpThreads / kWave;
+  const std::uint64_t tiles = (m + kPlcpTile - 1) / kPlcpTile;
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kWavesPerBlock;
+  const std::uint32_t lane = LaneId();
+  for (std::uint64_t tile = static_cast<std::uint64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+       tile < tiles; tile += stride) {  // wave-uniform
+    const std::uint64_t first = tile * kPlcpTile;
+    const std::uint64_t k = first + lane - 1;  // lane 0: the entry before the tile
+    const bool own = lane > 0 && k < m;
+    std::uint32_t p = 0xFFFFFFFFu, f = 0xFFFFFFFFu;
+    if (own || (lane == 0 && first > 0)) {
+      p = pos[k];
+      f = phi[k];
+    }
+    const std::uint32_t pb = WaveShfl(p, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
+    const std::uint32_t fb = WaveShfl(f, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
+    if (!own) continue;
+    if (Bit(starts, p)) {
+      out[k] = 0;
+      continue;
+    }
+    if (k > 0 && pb + 1u == p && fb + 1u == f && !Bit(starts, pb)) {
+      out[k] = kPlcpReducible;
+      continue;
+    }
+    std::uint64_t l = 0;
+    bool done = false;
+    for (std::uint32_t w = 0; w < word_limit; ++w) {
+      std::uint64_t ca, cb;
+      std::uint32_t sa_, sb;
+      LoadWindow(t, static_cast<std::uint64_t>(p) + l, ca, sa_);
+      LoadWindow(t, static_cast<std::uint64_t>(f) + l, cb, sb);
+      const std::uint32_t mm = WindowMismatch(ca, sa_, cb, sb);
+      l += mm;
+      if (mm < 32) {
+        done = true;
+        break;
+      }
+    }
+    if (done) {
+      out[k] = static_cast<std::uint32_t>(l);
+      continue;
+    }
+    const unsigned long long slot = atomicAdd(&counters->long_items, 1ull);
+    if (slot < queue_capacity) {
+      queue[slot].i = p;
+      queue[slot].j = f;
+      queue[slot].l = l;
+      queue[slot].dst = k;
+    }
+    out[k] = 0;  // the long-LCE pass writes the value
+  }
+}
+
+// Per-chunk maximum scan key of the list (PLCP + position + 1 at irreducible
+// entries, 0 at reducible ones); counts the irreducible entries.
+__global__ __launch_bounds__(256) void SparseScanReduceKernel(const std::uint32_t* __restrict__ plcp,
+                                                              const std::uint32_t* __restrict__ pos, std::uint64_t m,
+                                                              std::uint64_t chunk, std::uint64_t* __restrict__ partial,
+                                                              DeviceCounters* counters) {
+  constexpr int kBlock = 256;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
+  std::uint64_t best = 0;
+  std::uint64_t irreducible = 0;
+  for (std::uint64_t k = begin + threadIdx.x * 8ull; k < end; k += kBlock * 8ull) {
+    std::uint32_t v[8], p[8];
+    LoadRun8(plcp, k, end, true, kPlcpReducible, v);
+    LoadRun8(pos, k, end, true, 0u, p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const std::uint64_t key = PlcpKey(v[i], p[i]);
+      best = key > best ? key : best;
+      irreducible += key != 0;
+    }
+  }
+  std::uint64_t total;
+  BlockExclusiveMax<kBlock>(best, scratch, total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+  BlockAddTo<kBlock>(irreducible, scratch, &counters->irreducible);
+}
+
+// Inclusive max-scan of the list's keys; PLCP[pos[k]] = scan - pos[k] - 1
+// into the position-indexed array (the list is in text order, so the writes
+// advance through it).
+__global__ __launch_bounds__(256) void SparseScanApplyKernel(const std::uint32_t* __restrict__ plcp,
+                                                             const std::uint32_t* __restrict__ pos, std::uint64_t m,
+                                                             std::uint64_t chunk,
+                                                             const std::uint64_t* __restrict__ partial,
+                                                             std::uint32_t* __restrict__ dense) {
+  constexpr int kBlock = 256;
+  constexpr int kItems = 8;
+  constexpr int kTile = kBlock * kItems;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
+  const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
+  std::uint64_t run = partial[blockIdx.x];
+  for (std::uint64_t base = begin; base < end; base += kTile) {
+    const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
+    std::uint32_t v[kItems], p[kItems];
+    LoadRun8(plcp, first, end, true, kPlcpReducible, v);
+    LoadRun8(pos, first, end, true, 0u, p);
+    std::uint64_t keys[kItems];
+    std::uint64_t local = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      keys[i] = first + i < end ? PlcpKey(v[i], p[i]) : 0ull;
+      local = keys[i] > local ? keys[i] : local;
+    }
+    std::uint64_t total;
+    std::uint64_t cur = BlockExclusiveMax<kBlock>(local, scratch, total);
+    if (run > cur) cur = run;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      cur = keys[i] > cur ? keys[i] : cur;
+      if (first + i < end) dense[p[i]] = static_cast<std::uint32_t>(cur - p[i] - 1);
+    }
+    if (total > run) run = total;
   }
 }
 

@@ -83,6 +83,17 @@ bool InjectedAllocationFailure() {
   return false;
 }
 
+// Fault injection for tests of the permutation check: when armed, the next
+// build swaps SA rows 1 and 2 after the doubling (once).
+std::atomic<bool> g_swap_sa_rows{false};
+
+template <typename R>
+__global__ void SwapSaRowsKernel(R* sa) {
+  const R a = sa[1];
+  sa[1] = sa[2];
+  sa[2] = a;
+}
+
 class DeviceBuffer {
  public:
   DeviceBuffer() = default;
@@ -1030,13 +1041,14 @@ unsigned FlatGrid(std::uint64_t m, std::uint64_t per_block) {
   return static_cast<unsigned>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, per_block), kMaxBlocks)));
 }
 
-// LCP pass (lcp_kernels.hpp): bucket-start suffixes, PLCP of irreducible
+// LCP pass over every text position (lcp_kernels.hpp), for builds without
+// early LCP (sampled ones): bucket-start suffixes, PLCP of irreducible
 // positions (with the permutation/inverse check; long LCEs by whole waves),
 // PLCP max-scan, LCP gather.  keys0 holds PLCP (32-bit), keys1 the
 // bucket-start position bitmap and the long-LCE queue.
 template <typename Index, typename Out>
 void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const Index* sa, const Index* isa,
-               Out* lcp, bool early = false) {
+               Out* lcp) {
   sfk_hip_builder* b = ctx.b;
   const std::uint64_t n = ctx.n;
   DeviceCounters* dc = b->counters.As<DeviceCounters>();
@@ -1074,14 +1086,98 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
   PlcpScanReduceKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial, dc);
   PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid);
   PlcpScanApplyKernel<<<g.grid, 256, 0, b->stream>>>(plcp, n, g.chunk, partial);
-  LcpGatherKernel<Index, Out><<<FlatGrid(n, 256 * kGatherRows), 256, 0, b->stream>>>(plcp, sa, n, lcp, early, dc);
+  LcpGatherKernel<Index, Out><<<FlatGrid(n, 256 * kGatherRows), 256, 0, b->stream>>>(plcp, sa, n, lcp, false, dc);
   SFK_HIP_CHECK(hipGetLastError());
   ctx.stats.algorithmic_bytes_lcp +=
       n / 8                            // bucket-start bitmap
       + n * (w + w + 4) + n / 2        // PLCP: ISA read, SA[ISA[i] - 1] (with SA[ISA[i]]), PLCP write, text at i
       + n * (3 * 4)                    // max-scan: two reads, one write
-      + (early ? n * sizeof(Out)       // early LCP read (the gathered rows are booked after the build)
-               : n * (w + 4 + sizeof(Out)));  // LCP = PLCP[SA]: SA, PLCP, LCP
+      + n * (w + 4 + sizeof(Out));     // LCP = PLCP[SA]: SA, PLCP, LCP
+}
+
+// LCP of a build with early LCP (lcp_kernels.hpp, marked-row LCP): the
+// rows the doubling left marked get Φ/PLCP through a list sorted by
+// position; the pass that builds the list checks that ISA inverts SA.
+// Workspace: keys0 holds the position-indexed PLCP (first n words) and the
+// list's PLCP (next n); keys1 the bucket-start bitmap and the long-LCE
+// queue; vals0/vals1 and act_idx/act_seg the list's (position, Φ) pairs and
+// their radix ping-pong buffers (each holds n 32-bit words).
+template <typename R, typename Out>
+void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const R* sa, const R* isa,
+                     Out* lcp) {
+  sfk_hip_builder* b = ctx.b;
+  const std::uint64_t n = ctx.n;
+  DeviceCounters* dc = b->counters.As<DeviceCounters>();
+  if (b->keys0.Bytes() < n * 8) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace smaller than two PLCP arrays");
+  std::uint32_t* dense = b->keys0.As<std::uint32_t>();
+  std::uint32_t* list_plcp = dense + n;
+  std::uint32_t* starts = b->keys1.As<std::uint32_t>();
+  const std::uint64_t bit_words = n / 32 + 1;
+  const std::uint64_t bit_bytes = (bit_words * 4 + 63) / 64 * 64;
+  if (b->keys1.Bytes() < bit_bytes) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the bitmap");
+  LongLce* queue = reinterpret_cast<LongLce*>(b->keys1.As<char>() + bit_bytes);
+  const std::uint64_t capacity = (b->keys1.Bytes() - bit_bytes) / sizeof(LongLce);
+  for (const DeviceBuffer* buf : {&b->vals0, &b->vals1, &b->act_idx, &b->act_seg}) {
+    if (buf->Bytes() < n * 4) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the marked-row list");
+  }
+  std::uint32_t* pos = b->vals0.As<std::uint32_t>();
+  std::uint32_t* phi = b->vals1.As<std::uint32_t>();
+  std::uint32_t* pos_alt = b->act_idx.As<std::uint32_t>();
+  std::uint32_t* phi_alt = b->act_seg.As<std::uint32_t>();
+  constexpr std::uint32_t kWordLimit = 8;
+  auto aligned = [](const void* p) { return reinterpret_cast<std::uintptr_t>(p) % 16 == 0; };
+
+  SFK_HIP_CHECK(hipMemsetAsync(starts, 0, bit_words * sizeof(std::uint32_t), b->stream));
+  const std::uint64_t bucket_threads = 7ull * (ctx.refs > 0 ? ctx.refs : 1);
+  BucketStartsKernel<R><<<static_cast<unsigned>(CeilDiv(bucket_threads, 64)), 64, 0, b->stream>>>(
+      d_text, sa, n, ctx.d_offsets, ctx.refs, starts);
+  const SortGeometry g = Geometry(n, kMarkTile, 1024);
+  std::uint64_t* partial = b->partial.As<std::uint64_t>();
+  LcpMarkCountKernel<Out><<<g.grid, kMarkThreads, 0, b->stream>>>(lcp, n, g.chunk, aligned(lcp), partial);
+  MarkSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid, &dc->lcp_marked);
+  const RadixPasses passes = AllDigits(std::max(1, BitsFor(n - 1)));
+  const RadixSet set = TakeRadixSet(b);
+  LcpMarkWriteKernel<R, Out><<<g.grid, kMarkThreads, 0, b->stream>>>(
+      lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, phi, passes, set, dc);
+  SFK_HIP_CHECK(hipGetLastError());
+  constexpr std::uint64_t r = sizeof(R);
+  std::uint64_t bytes = n / 8 + n * sizeof(Out) + n * (sizeof(Out) + 2 * r);  // bitmap; two LCP reads, SA and ISA
+  const DeviceCounters& c = ReadCounters(b);
+  if ((c.error_flags & (kFlagNotPermutation | kFlagNotInverse)) || c.inverse_hash[0] != 0 || c.inverse_hash[1] != 0) {
+    Fail(SFK_HIP_BUILD_FAILURE, "suffix array is not a permutation with an inverse");
+  }
+  const std::uint64_t m = c.lcp_marked;
+  if (m > 0) {
+    bytes += m * 8;  // the list written
+    const std::uint64_t sort_before = ctx.stats.algorithmic_bytes_sa;
+    SortPairs<std::uint32_t, std::uint32_t>(ctx, ArraySource<std::uint32_t, std::uint32_t>{pos, phi}, m * 8, pos,
+                                            phi, pos_alt, phi_alt, m, passes, set);
+    bytes += ctx.stats.algorithmic_bytes_sa - sort_before;
+    ctx.stats.algorithmic_bytes_sa = sort_before;
+    const std::uint64_t tiles = CeilDiv(m, kPlcpTile);
+    SparsePlcpKernel<<<FlatGrid(tiles, kPlcpThreads / kWave), kPlcpThreads, 0, b->stream>>>(
+        text, pos, phi, m, starts, kWordLimit, list_plcp, queue, capacity, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    const DeviceCounters& c2 = ReadCounters(b);
+    const std::uint64_t items = c2.long_items;
+    if (items > capacity) Fail(SFK_HIP_BUILD_FAILURE, "long-LCE queue overflow");
+    if (items > 0) {
+      PlcpLongKernel<<<static_cast<unsigned>(CeilDiv(items * kWave, 256)), 256, 0, b->stream>>>(text, n, queue, items,
+                                                                                                list_plcp, dc);
+      SFK_HIP_CHECK(hipGetLastError());
+    }
+    const SortGeometry gs = Geometry(m, 256 * kScanItems, 1024);
+    SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);
+    PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, gs.grid);
+    SparseScanApplyKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dense);
+    SFK_HIP_CHECK(hipGetLastError());
+    bytes += m * (8 + 4 + 1)     // PLCP pass: the pair read, PLCP written, text window at the position
+             + m * (8 + 8 + 4);  // scan: pairs of (PLCP, position) read twice, PLCP written at the position
+  }
+  LcpGatherKernel<R, Out><<<FlatGrid(n, 256 * kGatherRows), 256, 0, b->stream>>>(dense, sa, n, lcp, true, dc);
+  SFK_HIP_CHECK(hipGetLastError());
+  bytes += n * (r + sizeof(Out));  // gather: SA and LCP read (the gathered rows are booked after the build)
+  ctx.stats.algorithmic_bytes_lcp += bytes;
 }
 
 template <typename In, typename Out>
@@ -1212,6 +1308,10 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
     }
   }
   Doubling<Index, R>(ctx, sa, rank, text, early).Run();
+  if (n >= 3 && g_swap_sa_rows.exchange(false)) {
+    SwapSaRowsKernel<R><<<1, 1, 0, b->stream>>>(sa);
+    SFK_HIP_CHECK(hipGetLastError());
+  }
   RecordPhase(b, 1);
 
   // K9: permutation + inverse check on the device (fused into the PLCP pass
@@ -1229,9 +1329,9 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
       // doubling), for the interval minima of the sampled rows.
       if constexpr (!kNarrowRank) LaunchLcp<Index, Index>(ctx, d_text, text, sa, rank, b->vals0.As<Index>());
     } else if (lcp_bits == 32) {
-      LaunchLcp<R, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp), true);
+      LaunchLcpMarked<R, std::uint32_t>(ctx, d_text, text, sa, rank, static_cast<std::uint32_t*>(d_lcp));
     } else {
-      LaunchLcp<R, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp), true);
+      LaunchLcpMarked<R, std::uint64_t>(ctx, d_text, text, sa, rank, static_cast<std::uint64_t*>(d_lcp));
     }
   }
   RecordPhase(b, 3);
@@ -1423,6 +1523,8 @@ int sfk_hip_abi_version(void) { return SFK_HIP_ABI_VERSION; }
 void sfk_hip_inject_allocation_failure(long long after) {
   sfk::g_fail_allocation.store(after < 0 ? -1 : after);
 }
+
+void sfk_hip_inject_sa_swap(int armed) { sfk::g_swap_sa_rows.store(armed != 0); }
 
 int sfk_hip_device_count(void) {
   int count = 0;

@@ -79,6 +79,8 @@ struct DeviceCounters {
   unsigned long long l_next_segs; // segments in the next large list
   unsigned long long irreducible;  // PLCP positions computed directly
   unsigned long long lcp_gathered[16];  // LCP rows gathered from PLCP, over 16 words (block % 16)
+  unsigned long long lcp_marked;        // rows the marked-row LCP pass computes
+  unsigned long long inverse_hash[2];   // SA/ISA pair-multiset hash difference (0 when ISA inverts SA)
   unsigned int error_flags;
   unsigned int pad;
 };

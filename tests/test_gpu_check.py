@@ -201,3 +201,41 @@ def test_allocation_failure_is_a_build_failure_and_recoverable(hip):
         assert e.value.code == "build_failure"
     finally:
         lib.sfk_hip_inject_allocation_failure(-1)
+
+
+@pytest.mark.parametrize("bits,flags,k,path", [
+    (32, 3, 1, "marked-row LCP (pair-multiset hash)"),
+    (64, 3, 1, "marked-row LCP, 64-bit construction"),
+    (32, 1, 1, "no LCP (ValidateKernel)"),
+    (32, 3, 2, "sampled K=2 (full PLCP pass)"),
+])
+def test_swapped_sa_rows_fail_the_build(hip, bits, flags, k, path):
+    # Two SA rows swapped after the doubling (the ISA left as built): every
+    # build path's own permutation check must refuse it with kBuildFailure,
+    # as RepackSuffixArray's check does (suffix_array.cpp:852-881); the next
+    # build, unarmed, is exact.
+    import ctypes
+    lib = sufkit_amd.load_hip()
+    lib.sfk_hip_inject_sa_swap.argtypes = [ctypes.c_int]
+    lib.sfk_hip_inject_sa_swap.restype = None
+    text = synth.repeat_text(43, 40000, cover=0.6, n_runs=4, contigs=3)
+    n = text.size
+    rows = (n + k - 1) // k
+    d_text = sufkit_amd.DeviceArray(n + 64)
+    d_text.upload(text)
+    outs = [sufkit_amd.DeviceArray(rows * 4) for _ in range(3)]
+    lcp = outs[2].ptr if flags & 2 else None
+    with sufkit_amd.DeviceBuilder(0) as b:
+        lib.sfk_hip_inject_sa_swap(1)
+        try:
+            with pytest.raises(sufkit_amd.SufkitError) as e:
+                b.build(d_text.ptr, n, bits, outs[0].ptr, 32, outs[1].ptr, 32, lcp, 32, flags, k)
+        finally:
+            lib.sfk_hip_inject_sa_swap(0)
+        assert e.value.status == BUILD_FAILURE and "permutation" in str(e.value), path
+        b.build(d_text.ptr, n, bits, outs[0].ptr, 32, outs[1].ptr, 32, lcp, 32, flags, k)
+    osa, oisa, olcp = oracle.fast_build(text)
+    if k == 1:
+        assert np.array_equal(outs[0].download(np.uint32, n).astype(np.uint64), osa)
+        if flags & 2:
+            assert np.array_equal(outs[2].download(np.uint32, n).astype(np.uint64), olcp)
