@@ -78,7 +78,7 @@ typedef struct sfk_hip_stats {
   double prefix_directory_seconds; /* sfk_hip_build_fast's directory pass */
   double call_seconds;      /* whole host call: allocation, copies, build,
                                directory (host entry points only) */
-  uint64_t lcp_gathered_rows; /* rows whose LCP the LCP pass gathered from
+  uint64_t lcp_gathered_rows; /* rows whose LCP the LCP pass took from
                                  PLCP; the others were fixed when the
                                  doubling separated them */
 } sfk_hip_stats;

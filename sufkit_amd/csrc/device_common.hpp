@@ -100,6 +100,36 @@ __device__ inline std::uint32_t ExactKeyWord(std::uint64_t codes,
   return lo;
 }
 
+// Position held by a list value.  Every text the backend takes is shorter
+// than 2^32 symbols, so a position fits the low 32 bits, and a 64-bit list
+// value may carry a 32-bit payload above it (the first step of a 64-bit
+// build carries the low FillKey half of the suffix's window there).
+template <typename Index>
+__device__ __forceinline__ std::uint64_t PosOf(Index v) {
+  if constexpr (sizeof(Index) == 8) {
+    return static_cast<std::uint64_t>(v) & 0xFFFFFFFFull;
+  } else {
+    return static_cast<std::uint64_t>(v);
+  }
+}
+
+// Special blocks: one bit per kSpecialBlock text positions, set when the
+// block holds a separator, an N or the sentinel (written by PackTextKernel).
+// The 32-symbol window at p can hold a special symbol only when the block of
+// p or of p + 31 is special; the bitmap has a spare word past the last block.
+constexpr int kSpecialBlockShift = 12;
+constexpr std::uint64_t kSpecialBlock = 1ull << kSpecialBlockShift;
+
+__host__ __device__ inline std::uint64_t SpecialBlockWords(std::uint64_t n) {
+  return (n >> kSpecialBlockShift) / 32 + 2;
+}
+
+__device__ __forceinline__ bool MaybeSpecialWindow(const std::uint32_t* __restrict__ blocks, std::uint64_t p) {
+  const std::uint64_t b0 = p >> kSpecialBlockShift;
+  const std::uint64_t b1 = (p + 31) >> kSpecialBlockShift;
+  return ((blocks[b0 >> 5] >> (b0 & 31)) | (blocks[b1 >> 5] >> (b1 & 31))) & 1u;
+}
+
 // Length of the common prefix of two 32-symbol windows (32 if equal).
 __device__ inline std::uint32_t WindowMismatch(std::uint64_t ca, std::uint32_t sa,
                                                std::uint64_t cb, std::uint32_t sb) {

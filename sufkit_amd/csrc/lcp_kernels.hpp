@@ -397,12 +397,13 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 //
 // After a build with early LCP, a row holds the marker only where the
 // doubling's keys did not prove its LCP; every row whose LCP is 32 or more is
-// such a row.  Only the marked rows need Φ/PLCP, so instead of a PLCP pass
-// over every text position (one random SA line per position):
-//   LcpMarkCount / LcpMarkWrite   rows in order: the (position SA[r],
-//                  Φ = SA[r-1]) pair of every marked row r >= 1 goes to a
-//                  list, whose position digits are counted for the radix
-//                  sort.  The same pass checks that ISA inverts SA (below).
+// such a row, and every other row already holds its final value.  Only the
+// marked rows need Φ/PLCP, so instead of a PLCP pass over every text
+// position (one random SA line per position) and a gather over every row:
+//   LcpMarkCount / LcpMarkWrite   rows in order: the marked rows r >= 1 go to
+//                  a list of (position SA[r], Φ = SA[r-1] and r), whose
+//                  position digits are counted for the radix sort.  The same
+//                  pass checks that ISA inverts SA (below) and writes LCP[0].
 //   (radix sort of the list by position)
 //   SparsePlcp     the list in text order: Kasai's reducibility test between
 //                  neighbouring entries.  Entry k (position p) is reducible
@@ -415,9 +416,13 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 //   SparseScan     PLCP[p] + p never decreases along the text, and the
 //                  entries form a subsequence of it, so the inclusive max of
 //                  PLCP + p + 1 over the irreducible entries up to k is
-//                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply writes
-//                  PLCP[p_k] into the position-indexed array.
-//   LcpGather      (early mode) reads PLCP[SA[r]] for the marked rows.
+//                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply leaves
+//                  (r, LCP[r]) pairs and counts the high digits of r.
+//   (radix passes over the digits of r from bit 16 up)
+//   LcpScatter     LCP[r] = value, pairs in order: the passes grouped them by
+//                  r >> 16, so the stores in flight fall in a few windows of
+//                  2^16 rows, which the L2 combines into whole lines, instead
+//                  of one partial line per store at random.
 //
 // The check that replaces the inverse check of the full PLCP pass: the pairs
 // (r, SA[r]) over rows and (ISA[i], i) over positions are the same multiset
@@ -433,6 +438,8 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 constexpr int kMarkThreads = 256;
 constexpr int kMarkRows = 8;
 constexpr int kMarkTile = kMarkThreads * kMarkRows;
+// Row digits below this bit are left unsorted before the LCP scatter.
+constexpr int kScatterWindowBits = 16;
 
 template <typename T>
 __device__ __forceinline__ T LcpMarker() {
@@ -509,15 +516,16 @@ __global__ __launch_bounds__(1024) void MarkSpineKernel(std::uint64_t* partial, 
   if (threadIdx.x == 0) *total = sum;
 }
 
-// The list of marked rows (position SA[r], Φ = SA[r-1]) at the chunk's
-// offset, in row order, with the position digits counted for the sort and
-// the inverse check of SA and ISA (rows and positions share the index).
+// The list of marked rows at the chunk's offset, in row order: position
+// SA[r] (out_pos) and Φ = SA[r-1] above r (out_val), with the position
+// digits counted for the sort, the inverse check of SA and ISA (rows and
+// positions share the index), and LCP[0] = 0 when row 0 is marked.
 template <typename R, typename Out>
 __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
-    const Out* __restrict__ lcp, const R* __restrict__ sa, const R* __restrict__ isa, std::uint64_t n,
+    Out* __restrict__ lcp, const R* __restrict__ sa, const R* __restrict__ isa, std::uint64_t n,
     std::uint64_t chunk, bool lcp_aligned, bool sa_aligned, bool isa_aligned,
     const std::uint64_t* __restrict__ partial, std::uint32_t* __restrict__ out_pos,
-    std::uint32_t* __restrict__ out_phi, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
+    std::uint64_t* __restrict__ out_val, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
   constexpr int kWaves = kMarkThreads / kWave;
   __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
   __shared__ std::uint32_t scratch32[kWaves];
@@ -553,7 +561,13 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
         flags |= row >= n ? static_cast<std::uint32_t>(kFlagNotInverse) : 0u;
         hash_a += PairHashA(r, pos) - PairHashA(row, r);
         hash_b += PairHashB(r, pos) - PairHashB(row, r);
-        if (l[i] == marker && r > 0) marks |= 1u << i;
+        if (l[i] == marker) {
+          if (r > 0) {
+            marks |= 1u << i;
+          } else {
+            lcp[0] = Out(0);
+          }
+        }
       }
     }
     // Φ of the thread's first row: SA[r0 - 1] (read only when that row is
@@ -568,8 +582,9 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
       const bool m = (marks >> i) & 1u;
       const std::uint32_t pos = static_cast<std::uint32_t>(s[i]);
       if (m) {
+        const std::uint64_t phi = i == 0 ? before : static_cast<std::uint64_t>(s[i - 1]);
         out_pos[slot] = pos;
-        out_phi[slot] = static_cast<std::uint32_t>(i == 0 ? before : static_cast<std::uint64_t>(s[i - 1]));
+        out_val[slot] = (phi << 32) | (r0 + i);
         ++slot;
       }
       CountDigits<std::uint32_t>(local[wave], passes, pos, m, false);
@@ -593,7 +608,7 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
 // and each lane takes its predecessor by a lane shuffle.  LCEs longer than
 // word_limit windows go to the long queue (dst: the entry).
 __global__ __launch_bounds__(kPlcpThreads) void SparsePlcpKernel(
-    PackedText t, const std::uint32_t* __restrict__ pos, const std::uint32_t* __restrict__ phi, std::uint64_t m,
+    PackedText t, const std::uint32_t* __restrict__ pos, const std::uint64_t* __restrict__ val, std::uint64_t m,
     const std::uint32_t* __restrict__ starts, std::uint32_t word_limit, std::uint32_t* __restrict__ out,
     LongLce* __restrict__ queue, std::uint64_t queue_capacity, DeviceCounters* counters) {
   constexpr std::uint64_t kWavesPerBlock = kPlcpThreads / kWave;
@@ -608,7 +623,7 @@ __global__ __launch_bounds__(kPlcpThreads) void SparsePlcpKernel(
     std::uint32_t p = 0xFFFFFFFFu, f = 0xFFFFFFFFu;
     if (own || (lane == 0 && first > 0)) {
       p = pos[k];
-      f = phi[k];
+      f = static_cast<std::uint32_t>(val[k] >> 32);
     }
     const std::uint32_t pb = WaveShfl(p, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
     const std::uint32_t fb = WaveShfl(f, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
@@ -679,43 +694,107 @@ __global__ __launch_bounds__(256) void SparseScanReduceKernel(const std::uint32_
   BlockAddTo<kBlock>(irreducible, scratch, &counters->irreducible);
 }
 
-// Inclusive max-scan of the list's keys; PLCP[pos[k]] = scan - pos[k] - 1
-// into the position-indexed array (the list is in text order, so the writes
-// advance through it).
-__global__ __launch_bounds__(256) void SparseScanApplyKernel(const std::uint32_t* __restrict__ plcp,
-                                                             const std::uint32_t* __restrict__ pos, std::uint64_t m,
+// Inclusive max-scan of the list's keys: entry k's LCP row r_k (from val)
+// goes to rows[k] and its value, scan - p_k - 1, back to plcp[k]; the row
+// digits of the scatter's passes are counted.
+__global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __restrict__ plcp,
+                                                             const std::uint32_t* __restrict__ pos,
+                                                             const std::uint64_t* __restrict__ val, std::uint64_t m,
                                                              std::uint64_t chunk,
                                                              const std::uint64_t* __restrict__ partial,
-                                                             std::uint32_t* __restrict__ dense) {
+                                                             std::uint32_t* __restrict__ rows, RadixPasses passes,
+                                                             RadixSet set) {
   constexpr int kBlock = 256;
   constexpr int kItems = 8;
   constexpr int kTile = kBlock * kItems;
-  __shared__ std::uint64_t scratch[kBlock / kWave];
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ std::uint64_t scratch[kWaves];
+  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  ClearNextRadixSet(set);
+  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kBlock) {
+    (&local[0][0][0])[i] = 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
   const std::uint64_t begin = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t end = begin + chunk < m ? begin + chunk : m;
   std::uint64_t run = partial[blockIdx.x];
   for (std::uint64_t base = begin; base < end; base += kTile) {
     const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
     std::uint32_t v[kItems], p[kItems];
+    std::uint64_t rv[kItems];
     LoadRun8(plcp, first, end, true, kPlcpReducible, v);
     LoadRun8(pos, first, end, true, 0u, p);
+    LoadRun8(val, first, end, true, std::uint64_t{0}, rv);
     std::uint64_t keys[kItems];
-    std::uint64_t local = 0;
+    std::uint64_t local_max = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       keys[i] = first + i < end ? PlcpKey(v[i], p[i]) : 0ull;
-      local = keys[i] > local ? keys[i] : local;
+      local_max = keys[i] > local_max ? keys[i] : local_max;
     }
     std::uint64_t total;
-    std::uint64_t cur = BlockExclusiveMax<kBlock>(local, scratch, total);
+    std::uint64_t cur = BlockExclusiveMax<kBlock>(local_max, scratch, total);
     if (run > cur) cur = run;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       cur = keys[i] > cur ? keys[i] : cur;
-      if (first + i < end) dense[p[i]] = static_cast<std::uint32_t>(cur - p[i] - 1);
+      v[i] = static_cast<std::uint32_t>(cur - p[i] - 1);
+    }
+    if (first + kItems <= end) {
+      *reinterpret_cast<uint4*>(plcp + first) = make_uint4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<uint4*>(plcp + first + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+      *reinterpret_cast<uint4*>(rows + first) = make_uint4(static_cast<std::uint32_t>(rv[0]),
+          static_cast<std::uint32_t>(rv[1]), static_cast<std::uint32_t>(rv[2]), static_cast<std::uint32_t>(rv[3]));
+      *reinterpret_cast<uint4*>(rows + first + 4) = make_uint4(static_cast<std::uint32_t>(rv[4]),
+          static_cast<std::uint32_t>(rv[5]), static_cast<std::uint32_t>(rv[6]), static_cast<std::uint32_t>(rv[7]));
+    } else {
+#pragma unroll
+      for (int i = 0; i < kItems; ++i) {
+        if (first + i < end) {
+          plcp[first + i] = v[i];
+          rows[first + i] = static_cast<std::uint32_t>(rv[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      CountDigits<std::uint32_t>(local[wave], passes, static_cast<std::uint32_t>(rv[i]), first + i < end, false);
     }
     if (total > run) run = total;
   }
+  __syncthreads();
+  FlushDigits<kWaves>(local, passes, set.hist);
+}
+
+// LCP[rows[k]] = values[k] for the m pairs, in order.
+template <typename Out>
+__global__ __launch_bounds__(256) void LcpScatterKernel(const std::uint32_t* __restrict__ rows,
+                                                        const std::uint32_t* __restrict__ values, std::uint64_t m,
+                                                        std::uint64_t n, Out* __restrict__ lcp,
+                                                        DeviceCounters* counters) {
+  const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * 256 * 4;
+  bool bad = false;
+  for (std::uint64_t k0 = static_cast<std::uint64_t>(blockIdx.x) * 256 * 4 + threadIdx.x; k0 < m; k0 += stride) {
+    std::uint32_t r[4], v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const std::uint64_t k = k0 + static_cast<std::uint64_t>(j) * 256;
+      r[j] = k < m ? rows[k] : 0xFFFFFFFFu;
+      v[j] = k < m ? values[k] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const std::uint64_t k = k0 + static_cast<std::uint64_t>(j) * 256;
+      if (k >= m) continue;
+      if (r[j] < n) {
+        lcp[r[j]] = static_cast<Out>(v[j]);
+      } else {
+        bad = true;
+      }
+    }
+  }
+  if (bad) atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
 }
 
 }  // namespace sfk

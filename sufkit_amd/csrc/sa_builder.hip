@@ -163,7 +163,8 @@ struct sfk_hip_builder {
   hipStream_t stream = nullptr;
   sfk::DeviceBuffer packed, rank, sa, act_idx, act_seg, keys0, keys1,
       vals0, vals1, seg_delta0, seg_delta1, seg_start, radix_counts, radix_status,
-      partial, counters, narrow, ref_offsets, ref_counts, ref_adj, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key;
+      partial, counters, narrow, ref_offsets, ref_counts, ref_adj, q_idx0, q_idx1, q_row0_0, q_row0_1, q_key,
+      special_blocks;
   // Tile tickets of the look-back kernels: two counters, the pass of epoch e
   // numbering its tiles from tickets[e & 1] (see LookBackTile).
   sfk::DeviceBuffer tickets;
@@ -183,6 +184,13 @@ namespace sfk {
 namespace {
 
 constexpr int kScanItems = 8;
+
+// The large list's positions; after the doubling the same buffer holds the
+// LCP pass's bucket-start bitmap (n / 8 bytes) and long-LCE queue, hence the
+// floor for short texts.
+std::uint64_t ActIdxBytes(std::uint64_t n, std::uint64_t index_bytes) {
+  return std::max<std::uint64_t>(n * index_bytes, 4096);
+}
 
 struct BuildContext {
   sfk_hip_builder* b;
@@ -624,7 +632,8 @@ class Doubling {
       SFK_HIP_CHECK(hipMemsetAsync(flags, 0, (lm_bound_ / 2 + 1) * sizeof(std::uint32_t), b_->stream));
       ResetSpecialCountKernel<<<1, 1, 0, b_->stream>>>(dc);
       MarkSpecialSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256)), 256, 0, b_->stream>>>(
-          b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), text_, flags, dc);
+          b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), text_, b_->special_blocks.As<std::uint32_t>(),
+          flags, dc);
       SFK_HIP_CHECK(hipGetLastError());
       ctx_.stats.algorithmic_bytes_sa += lm_bound_ * (sizeof(Index) + 4 + 16);
       const bool special = ApplySizes(ReadCounters(b_)).special_active > 0;
@@ -645,6 +654,7 @@ class Doubling {
         split = true;
       }
     }
+    packed_lo_ = false;  // every list holds plain positions from here on
     EndRound(split);
 
     const int rank_bits = std::max(1, BitsFor(n_ - 1));
@@ -859,7 +869,8 @@ class Doubling {
     const unsigned grid = static_cast<unsigned>(CeilDiv(qm_, kLocalOwn));
     LocalRefineKernel<Index, kExact><<<grid, kLocalThreads, 0, b_->stream>>>(
         text_, q_idx_cur_, q_row0_cur_, b_->q_key.As<Index>(), qm_, n_, sa_, rank_, q_idx_next_,
-        q_row0_next_, dc, kExact ? early_ : EarlyLcp{nullptr, nullptr});
+        q_row0_next_, dc, kExact ? early_ : EarlyLcp{nullptr, nullptr}, kExact && packed_lo_,
+        b_->special_blocks.As<std::uint32_t>());
     SFK_HIP_CHECK(hipGetLastError());
     constexpr std::uint64_t w = sizeof(Index);
     ctx_.stats.refine_steps += 1;
@@ -941,22 +952,29 @@ class Doubling {
       const std::uint64_t text_bytes = PackedWords(n_) * 16;
       ctx_.stats.algorithmic_bytes_sa += text_bytes;
       if constexpr (kNarrowKey) {
-        SortPairs<K, Index>(ctx_, TextFillHiSource<Index>{text_}, text_bytes, keys, vals, keys_alt, vals_alt, m,
-                            passes, set);
+        // A 64-bit list carries each suffix's low FillKey half above its
+        // position until the exact steps have used it.
+        packed_lo_ = sizeof(Index) == 8;
+        SortPairs<K, Index>(ctx_, TextFillHiSource<Index>{text_, packed_lo_}, text_bytes, keys, vals, keys_alt,
+                            vals_alt, m, passes, set);
       }
     } else {
       // The large list may sit in vals0/vals1 (after a split): Keygen reads
       // slot k of it before writing slot k of vals0, in the same thread.
+      const bool lo_from_value = packed_lo_ && sec_kind == kSecFillLo;
       if (rank_step) {
         KeygenKernel<Index, K, true><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
-            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc,
+            false);
       } else {
         KeygenKernel<Index, K, false><<<keygen_grid, kKeygenBlock, 0, b_->stream>>>(
-            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc);
+            l_idx_, l_seg_, m, sec_kind, sec_bits, seg_bits, text_, rank_, h, iota, keys, vals, passes, set, dc,
+            lo_from_value);
       }
       SFK_HIP_CHECK(hipGetLastError());
       ctx_.stats.algorithmic_bytes_sa +=
-          m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? sizeof(R) : 24) + sizeof(K) + w);
+          m * ((iota ? 0 : w) + (seg_bits ? w : 0) + (rank_step ? sizeof(R) : lo_from_value ? 0 : 24) + sizeof(K) +
+               w);
       if (local_between) LocalRefine<false>();
       SortPairs<K, Index>(ctx_, ArraySource<K, Index>{keys, vals}, m * (sizeof(K) + w), keys, vals, keys_alt,
                           vals_alt, m, passes, set);
@@ -1007,6 +1025,9 @@ class Doubling {
   R* rank_;
   PackedText text_;
   EarlyLcp early_;
+  // The large and small lists hold 64-bit values with the low FillKey half
+  // above the position (from the first step until the exact steps).
+  bool packed_lo_ = false;
   // Large list: suffixes, segment ids, per-segment deltas.  After a
   // RefineApply whose sizes are not read back yet, lm_bound_ bounds lm_.
   std::uint64_t lm_ = 0;
@@ -1097,33 +1118,36 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
 
 // LCP of a build with early LCP (lcp_kernels.hpp, marked-row LCP): the
 // rows the doubling left marked get Φ/PLCP through a list sorted by
-// position; the pass that builds the list checks that ISA inverts SA.
-// Workspace: keys0 holds the position-indexed PLCP (first n words) and the
-// list's PLCP (next n); keys1 the bucket-start bitmap and the long-LCE
-// queue; vals0/vals1 and act_idx/act_seg the list's (position, Φ) pairs and
-// their radix ping-pong buffers (each holds n 32-bit words).
+// position, and their values go back through a scatter grouped by row; the
+// pass that builds the list checks that ISA inverts SA.  Workspace (every
+// buffer holds n 32-bit words, keys0/keys1 n 64-bit ones): vals0/vals1 the
+// list's positions and their radix ping-pong, keys0/keys1 its (Φ, row)
+// values and theirs, act_idx the bucket-start bitmap and the long-LCE
+// queue, act_seg the list's PLCP; the scatter's (row, value) pairs reuse
+// the buffers the sort left free.
 template <typename R, typename Out>
 void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const R* sa, const R* isa,
                      Out* lcp) {
   sfk_hip_builder* b = ctx.b;
   const std::uint64_t n = ctx.n;
   DeviceCounters* dc = b->counters.As<DeviceCounters>();
-  if (b->keys0.Bytes() < n * 8) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace smaller than two PLCP arrays");
-  std::uint32_t* dense = b->keys0.As<std::uint32_t>();
-  std::uint32_t* list_plcp = dense + n;
-  std::uint32_t* starts = b->keys1.As<std::uint32_t>();
-  const std::uint64_t bit_words = n / 32 + 1;
-  const std::uint64_t bit_bytes = (bit_words * 4 + 63) / 64 * 64;
-  if (b->keys1.Bytes() < bit_bytes) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the bitmap");
-  LongLce* queue = reinterpret_cast<LongLce*>(b->keys1.As<char>() + bit_bytes);
-  const std::uint64_t capacity = (b->keys1.Bytes() - bit_bytes) / sizeof(LongLce);
   for (const DeviceBuffer* buf : {&b->vals0, &b->vals1, &b->act_idx, &b->act_seg}) {
     if (buf->Bytes() < n * 4) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the marked-row list");
   }
+  for (const DeviceBuffer* buf : {&b->keys0, &b->keys1}) {
+    if (buf->Bytes() < n * 8) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the marked-row list");
+  }
   std::uint32_t* pos = b->vals0.As<std::uint32_t>();
-  std::uint32_t* phi = b->vals1.As<std::uint32_t>();
-  std::uint32_t* pos_alt = b->act_idx.As<std::uint32_t>();
-  std::uint32_t* phi_alt = b->act_seg.As<std::uint32_t>();
+  std::uint32_t* pos_alt = b->vals1.As<std::uint32_t>();
+  std::uint64_t* val = b->keys0.As<std::uint64_t>();
+  std::uint64_t* val_alt = b->keys1.As<std::uint64_t>();
+  std::uint32_t* list_plcp = b->act_seg.As<std::uint32_t>();
+  std::uint32_t* starts = b->act_idx.As<std::uint32_t>();
+  const std::uint64_t bit_words = n / 32 + 1;
+  const std::uint64_t bit_bytes = (bit_words * 4 + 63) / 64 * 64;
+  if (b->act_idx.Bytes() < bit_bytes) Fail(SFK_HIP_BUILD_FAILURE, "LCP pass workspace too small for the bitmap");
+  LongLce* queue = reinterpret_cast<LongLce*>(b->act_idx.As<char>() + bit_bytes);
+  const std::uint64_t capacity = (b->act_idx.Bytes() - bit_bytes) / sizeof(LongLce);
   constexpr std::uint32_t kWordLimit = 8;
   auto aligned = [](const void* p) { return reinterpret_cast<std::uintptr_t>(p) % 16 == 0; };
 
@@ -1135,10 +1159,11 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
   LcpMarkCountKernel<Out><<<g.grid, kMarkThreads, 0, b->stream>>>(lcp, n, g.chunk, aligned(lcp), partial);
   MarkSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid, &dc->lcp_marked);
-  const RadixPasses passes = AllDigits(std::max(1, BitsFor(n - 1)));
+  const int bits = std::max(1, BitsFor(n - 1));
+  const RadixPasses passes = AllDigits(bits);
   const RadixSet set = TakeRadixSet(b);
   LcpMarkWriteKernel<R, Out><<<g.grid, kMarkThreads, 0, b->stream>>>(
-      lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, phi, passes, set, dc);
+      lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, val, passes, set, dc);
   SFK_HIP_CHECK(hipGetLastError());
   constexpr std::uint64_t r = sizeof(R);
   std::uint64_t bytes = n / 8 + n * sizeof(Out) + n * (sizeof(Out) + 2 * r);  // bitmap; two LCP reads, SA and ISA
@@ -1148,15 +1173,13 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   }
   const std::uint64_t m = c.lcp_marked;
   if (m > 0) {
-    bytes += m * 8;  // the list written
+    bytes += m * 12;  // the list written
     const std::uint64_t sort_before = ctx.stats.algorithmic_bytes_sa;
-    SortPairs<std::uint32_t, std::uint32_t>(ctx, ArraySource<std::uint32_t, std::uint32_t>{pos, phi}, m * 8, pos,
-                                            phi, pos_alt, phi_alt, m, passes, set);
-    bytes += ctx.stats.algorithmic_bytes_sa - sort_before;
-    ctx.stats.algorithmic_bytes_sa = sort_before;
+    SortPairs<std::uint32_t, std::uint64_t>(ctx, ArraySource<std::uint32_t, std::uint64_t>{pos, val}, m * 12, pos,
+                                            val, pos_alt, val_alt, m, passes, set);
     const std::uint64_t tiles = CeilDiv(m, kPlcpTile);
     SparsePlcpKernel<<<FlatGrid(tiles, kPlcpThreads / kWave), kPlcpThreads, 0, b->stream>>>(
-        text, pos, phi, m, starts, kWordLimit, list_plcp, queue, capacity, dc);
+        text, pos, val, m, starts, kWordLimit, list_plcp, queue, capacity, dc);
     SFK_HIP_CHECK(hipGetLastError());
     const DeviceCounters& c2 = ReadCounters(b);
     const std::uint64_t items = c2.long_items;
@@ -1166,17 +1189,36 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
                                                                                                 list_plcp, dc);
       SFK_HIP_CHECK(hipGetLastError());
     }
+    // The scatter's pairs: rows into the position buffer the sort left free
+    // (pos_alt is now the other one), values in place.
+    std::uint32_t* rows = pos == b->vals0.As<std::uint32_t>() ? b->vals1.As<std::uint32_t>()
+                                                              : b->vals0.As<std::uint32_t>();
+    RadixPasses scatter_passes{};
+    for (int shift = kScatterWindowBits; shift < bits; shift += RadixTile::kDigitBits) {
+      scatter_passes.shift[scatter_passes.count++] = shift;
+    }
+    const RadixSet scatter_set = TakeRadixSet(b);
     const SortGeometry gs = Geometry(m, 256 * kScanItems, 1024);
     SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);
     PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, gs.grid);
-    SparseScanApplyKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dense);
+    SparseScanApplyKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, val, m, gs.chunk, partial, rows,
+                                                          scatter_passes, scatter_set);
     SFK_HIP_CHECK(hipGetLastError());
-    bytes += m * (8 + 4 + 1)     // PLCP pass: the pair read, PLCP written, text window at the position
-             + m * (8 + 8 + 4);  // scan: pairs of (PLCP, position) read twice, PLCP written at the position
+    std::uint32_t* values = list_plcp;
+    std::uint32_t* rows_alt = pos;  // free once the scan has read it
+    std::uint32_t* values_alt = reinterpret_cast<std::uint32_t*>(val == b->keys0.As<std::uint64_t>()
+                                                                     ? b->keys1.As<std::uint64_t>()
+                                                                     : b->keys0.As<std::uint64_t>());
+    SortPairs<std::uint32_t, std::uint32_t>(ctx, ArraySource<std::uint32_t, std::uint32_t>{rows, values}, m * 8,
+                                            rows, values, rows_alt, values_alt, m, scatter_passes, scatter_set);
+    bytes += ctx.stats.algorithmic_bytes_sa - sort_before;  // both sorts belong to the LCP phase
+    ctx.stats.algorithmic_bytes_sa = sort_before;
+    LcpScatterKernel<Out><<<FlatGrid(m, 256 * 4), 256, 0, b->stream>>>(rows, values, m, n, lcp, dc);
+    SFK_HIP_CHECK(hipGetLastError());
+    bytes += m * (12 + 4 + 1)         // PLCP pass: the entry read, PLCP written, text window at the position
+             + m * (8 + 16 + 8)       // scan: PLCP and position read twice, the row read, (row, value) written
+             + m * (8 + sizeof(Out));  // scatter: the pair read, LCP written
   }
-  LcpGatherKernel<R, Out><<<FlatGrid(n, 256 * kGatherRows), 256, 0, b->stream>>>(dense, sa, n, lcp, true, dc);
-  SFK_HIP_CHECK(hipGetLastError());
-  bytes += n * (r + sizeof(Out));  // gather: SA and LCP read (the gathered rows are booked after the build)
   ctx.stats.algorithmic_bytes_lcp += bytes;
 }
 
@@ -1254,6 +1296,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
 
   // Workspace (grow-only).
   b->packed.Reserve(words * 16);
+  b->special_blocks.Reserve(SpecialBlockWords(n) * sizeof(std::uint32_t));
   // Sampled outputs (K > 1) are shorter than the complete arrays the build
   // works on, so those stay in the workspace.
   const bool sampled = ctx.sampling > 1;
@@ -1261,7 +1304,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   const bool isa_direct = !sampled && d_isa != nullptr && isa_bits == kIndexBits;
   if (!sa_direct) b->sa.Reserve(n * sizeof(R));
   if (!isa_direct) b->rank.Reserve(n * sizeof(R));
-  b->act_idx.Reserve(n * sizeof(Index));
+  b->act_idx.Reserve(ActIdxBytes(n, sizeof(Index)));
   b->act_seg.Reserve(n * sizeof(std::uint32_t));
   b->keys0.Reserve(std::max<std::uint64_t>(n * 8, 64));
   b->keys1.Reserve(std::max<std::uint64_t>(n * 8, 64));
@@ -1284,10 +1327,12 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   SFK_HIP_CHECK(hipMemsetAsync(b->radix_counts.As<void>(), 0, 2 * kRadixSetStride, b->stream));
   b->radix_set = 0;
   RecordPhase(b, 0);
+  SFK_HIP_CHECK(hipMemsetAsync(b->special_blocks.As<void>(), 0, SpecialBlockWords(n) * sizeof(std::uint32_t),
+                               b->stream));
   PackTextKernel<<<static_cast<unsigned>(CeilDiv(words, 256)), 256, 0, b->stream>>>(
       d_text, n, words, b->packed.As<uint4>(),
       (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, ctx.refs > 0,
-      b->counters.As<DeviceCounters>());
+      b->counters.As<DeviceCounters>(), b->special_blocks.As<std::uint32_t>());
   SFK_HIP_CHECK(hipGetLastError());
   if (ctx.refs > 0) {
     CheckReferenceEndsKernel<<<static_cast<unsigned>(CeilDiv(ctx.refs, 256)), 256, 0, b->stream>>>(
@@ -1389,10 +1434,7 @@ void BuildTyped(BuildContext& ctx, const std::uint8_t* d_text, std::uint32_t fla
   ctx.stats.lce_long_items = c.long_items;
   if (want_lcp) ctx.stats.algorithmic_bytes_lcp += c.irreducible * 32;  // window pair at Φ(i) per irreducible i
   if (early.lcp32 != nullptr || early.lcp64 != nullptr) {
-    std::uint64_t gathered = 0;
-    for (unsigned long long g : c.lcp_gathered) gathered += g;
-    ctx.stats.lcp_gathered_rows = gathered;
-    ctx.stats.algorithmic_bytes_lcp += gathered * (sizeof(R) + 4 + lcp_bits / 8);  // SA, PLCP, LCP per gathered row
+    ctx.stats.lcp_gathered_rows = c.lcp_marked;  // rows whose LCP came from PLCP (the rest were early)
   } else if (want_lcp) {
     ctx.stats.lcp_gathered_rows = n;
   }
@@ -1860,7 +1902,7 @@ int sfk_hip_prefix_directory(sfk_hip_builder* b, const uint8_t* d_text, uint64_t
     sfk::ResetCounters(b);
     sfk::PackTextKernel<<<static_cast<unsigned>(sfk::CeilDiv(words, 256)), 256, 0, b->stream>>>(
         d_text, n, words, b->packed.As<uint4>(), (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, false,
-        b->counters.As<sfk::DeviceCounters>());
+        b->counters.As<sfk::DeviceCounters>(), nullptr);
     SFK_HIP_CHECK(hipGetLastError());
     b->partial.Reserve(1024 * sizeof(sfk::SplitPartial));
     auto* dcounts = b->partial.As<unsigned long long>();
@@ -1937,7 +1979,7 @@ int sfk_hip_index_create(int device, const uint8_t* d_text, uint64_t n, const vo
     sfk::PackTextKernel<<<static_cast<unsigned>(sfk::CeilDiv(words, 256)), 256, 0, index->stream>>>(
         d_text, n, words, index->packed.As<uint4>(),
         (reinterpret_cast<std::uintptr_t>(d_text) % 16) == 0, false,
-        index->counters.As<sfk::DeviceCounters>());
+        index->counters.As<sfk::DeviceCounters>(), nullptr);
     SFK_HIP_CHECK(hipGetLastError());
     SFK_HIP_CHECK(hipMemcpyAsync(index->host_counters, index->counters.As<void>(),
                                  sizeof(sfk::DeviceCounters), hipMemcpyDeviceToHost, index->stream));
@@ -2405,8 +2447,9 @@ std::uint64_t NoAllocLayout(sfk_hip_builder* b, std::uint64_t n, bool need_rank,
   const std::uint64_t seg_cap = n / 2 + 2;
   take(b->counters, sizeof(DeviceCounters));
   take(b->packed, PackedWords(n) * 16);
+  take(b->special_blocks, SpecialBlockWords(n) * sizeof(std::uint32_t));
   if (need_rank) take(b->rank, n * w);
-  take(b->act_idx, n * w);
+  take(b->act_idx, ActIdxBytes(n, w));
   take(b->act_seg, n * sizeof(std::uint32_t));
   take(b->keys0, std::max<std::uint64_t>(n * 8, 64));
   take(b->keys1, std::max<std::uint64_t>(n * 8, 64));

@@ -100,9 +100,12 @@ constexpr std::uint32_t kSmallSegment = 0xFFFFFFFFu;
 // ------------------------------------------------------------ K0 pack -----
 
 // One thread per 32-symbol entry.  Encoded bytes 0..6 -> (code, spec).
+// special_blocks (cleared before, or null): the entry's special-block bit
+// (device_common.hpp) when it holds a special symbol.
 __global__ __launch_bounds__(256) void PackTextKernel(
     const std::uint8_t* __restrict__ text, std::uint64_t n, std::uint64_t words,
-    uint4* __restrict__ packed, bool aligned, bool batch, DeviceCounters* counters) {
+    uint4* __restrict__ packed, bool aligned, bool batch, DeviceCounters* counters,
+    std::uint32_t* __restrict__ special_blocks) {
   const std::uint64_t w = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w >= words) return;
   std::uint64_t c = 0;
@@ -153,6 +156,12 @@ __global__ __launch_bounds__(256) void PackTextKernel(
     s |= sp << (31 - j);
   }
   packed[w] = make_uint4(static_cast<std::uint32_t>(c), static_cast<std::uint32_t>(c >> 32), s, 0u);
+  // A wave's 64 entries (entry-aligned: w0 a multiple of 64) lie in one
+  // special block of 128 entries, so one lane sets the block's bit.
+  static_assert(kSpecialBlockShift == 12, "an entry of 32 symbols is 1/128 of a special block");
+  if (special_blocks != nullptr && __ballot(s != 0) != 0 && LaneId() == 0) {
+    atomicOr(&special_blocks[w >> 12], 1u << ((w >> 7) & 31u));
+  }
   if (flags) atomicOr(&counters->error_flags, flags);
   if (zeros) atomicAdd(&counters->zero_count, static_cast<unsigned long long>(zeros));
 }
@@ -254,12 +263,16 @@ __device__ inline std::uint64_t SecondaryKey(int sec_kind, PackedText t, const R
 // written, by the same thread (hence no __restrict__ on either).  kRank: the
 // secondary key is rank[suffix + h] (sec_kind is kSecRank); otherwise it
 // comes from the suffix's 32-symbol window (FillHi, FillLo, Exact0..2).
+// lo_from_value (FillLo of a 64-bit list whose values carry it above the
+// position): the key is that payload and the text is not read.  vals receive
+// the positions alone.
 template <typename Index, typename K, bool kRank, typename R = Index>
 __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
     const Index* act_idx, const std::uint32_t* __restrict__ act_seg,
     std::uint64_t m, int sec_kind, int sec_bits, int seg_bits, PackedText t,
     const R* __restrict__ rank, std::uint64_t h, bool iota,
-    K* __restrict__ keys, Index* vals, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
+    K* __restrict__ keys, Index* vals, RadixPasses passes, RadixSet set, DeviceCounters* counters,
+    bool lo_from_value) {
   constexpr int kWaves = kKeygenBlock / kWave;
   __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
   ClearNextRadixSet(set);
@@ -292,17 +305,20 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const bool valid = wbase + u * stride + LaneId() < m;
-        const std::uint64_t p = static_cast<std::uint64_t>(v[u]) + h;
+        const std::uint64_t p = PosOf(v[u]) + h;
         const bool ok = p < t.n;
         bad |= valid & !ok;
         sec[u] = (valid && ok) ? static_cast<std::uint64_t>(rank[ok ? p : 0]) : 0ull;
       }
       if (bad) atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
+    } else if (lo_from_value) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) sec[u] = static_cast<std::uint64_t>(v[u]) >> 32;
     } else {
       uint4 e0[kUnroll], e1[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const std::uint64_t w = static_cast<std::uint64_t>(v[u]) >> 5;
+        const std::uint64_t w = PosOf(v[u]) >> 5;
         e0[u] = t.words[w];
         e1[u] = t.words[w + 1];
       }
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
       const K key = valid ? static_cast<K>((seg[u] << sec_bits) | sec[u]) : K(0);
       if (valid) {
         keys[k] = key;
-        vals[k] = v[u];
+        vals[k] = static_cast<Index>(PosOf(v[u]));
       }
       CountDigits<K>(local[wave], passes, key, valid, kbase + kWave <= m);
     }
@@ -525,11 +541,14 @@ __global__ __launch_bounds__(RadixTile::kRadix) void TuneAdjKernel(const unsigne
 }
 
 // First pass of the first step: key = high half of FillKey of suffix i,
-// value = i, generated from the packed text instead of read.
+// value = i, generated from the packed text instead of read.  pack_lo (64-bit
+// values only): the value also carries the low FillKey half above the
+// position (PosOf), for the steps that order the first step's groups by it.
 template <typename Index>
 struct TextFillHiSource {
   static constexpr int kFetchGroup = 4;  // two 12-byte entry loads per item
   PackedText t;
+  bool pack_lo = false;
   struct Raw {
     uint4 e0, e1;  // the packed entries holding the window
   };
@@ -541,8 +560,13 @@ struct TextFillHiSource {
     std::uint64_t c;
     std::uint32_t s;
     WindowOf(r.e0, r.e1, static_cast<std::uint32_t>(i & 31), c, s);
-    key = static_cast<std::uint32_t>(FillKey(c, s) >> 32);
-    val = static_cast<Index>(i);
+    const std::uint64_t fill = FillKey(c, s);
+    key = static_cast<std::uint32_t>(fill >> 32);
+    if constexpr (sizeof(Index) == 8) {
+      val = static_cast<Index>(pack_lo ? (fill << 32) | i : i);
+    } else {
+      val = static_cast<Index>(i);
+    }
   }
 };
 
@@ -552,20 +576,24 @@ struct TextFillHiSource {
 // only value ever stored, so racing stores agree).  The list is in segment
 // order, so one lane per run of special suffixes of one segment in a wave
 // stores, and only while the flag still reads 0.
+// Windows outside special blocks are not read.
 template <typename Index>
 __global__ __launch_bounds__(256) void MarkSpecialSegmentsKernel(
     const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg, PackedText t,
-    std::uint32_t* __restrict__ flags, DeviceCounters* counters) {
+    const std::uint32_t* __restrict__ special_blocks, std::uint32_t* __restrict__ flags, DeviceCounters* counters) {
   const std::uint64_t m = counters->list.survivors;
   const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool special = false;
   std::uint32_t seg = 0;
   if (k < m) {
-    std::uint64_t c;
-    std::uint32_t s;
     seg = act_seg[k];
-    LoadWindow(t, act_idx[k], c, s);
-    special = s != 0;
+    const std::uint64_t p = PosOf(act_idx[k]);
+    if (MaybeSpecialWindow(special_blocks, p)) {
+      std::uint64_t c;
+      std::uint32_t s;
+      LoadWindow(t, p, c, s);
+      special = s != 0;
+    }
   }
   const std::uint64_t vote = __ballot(special);
   const std::uint32_t lane = LaneId();
@@ -788,7 +816,8 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
     const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
     const Index delta = seg_delta[seg];
     const Index row = static_cast<Index>(delta + static_cast<Index>(k));
-    if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(v) >= n) continue;  // corrupted state only
+    const std::uint64_t pos = PosOf(v);
+    if (static_cast<std::uint64_t>(row) >= n || pos >= n) continue;  // corrupted state only
     if (early.lcp32 != nullptr || early.lcp64 != nullptr) {
       // The first step: every row gets its early LCP or the marker.
       std::uint32_t value = 32u;
@@ -801,14 +830,14 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
       WriteEarlyLcp(early, static_cast<std::uint64_t>(row), value, value < 32u);
     }
     if (f & 2) {
-      sa[row] = static_cast<R>(v);
-      rank[v] = static_cast<R>(row);
+      sa[row] = static_cast<R>(pos);
+      rank[pos] = static_cast<R>(row);
     } else {
       // The group start lies in the same segment: every segment start is a
       // key change, hence a group start.
       const int g = lds.group[i];
       const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
-      if (survivor_rank) rank[v] = static_cast<R>(delta + static_cast<Index>(group_slot));
+      if (survivor_rank) rank[pos] = static_cast<R>(delta + static_cast<Index>(group_slot));
       const std::uint64_t at = run_surv + lds.pos[i];
       const std::uint64_t sc = run_segs + lds.segc[i];  // segments opened through this slot
       if (f & 1) {
@@ -1345,7 +1374,7 @@ __global__ __launch_bounds__(256) void QKeygenKernel(
     std::uint64_t n, std::uint64_t h, Index* __restrict__ q_key, DeviceCounters* counters) {
   const std::uint64_t p = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= qm) return;
-  const std::uint64_t at = static_cast<std::uint64_t>(q_idx[p]) + h;
+  const std::uint64_t at = PosOf(q_idx[p]) + h;
   Index key = 0;
   if (at < n) {
     key = static_cast<Index>(rank[at]);
@@ -1418,12 +1447,18 @@ __device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
 // FillKey high halves): the low half of FillKey, which orders windows without
 // a separator, N or sentinel exactly; a segment holding such a window compares
 // the exact 96-bit keys instead.  Otherwise q_key (rank[suffix + h]).
+// packed_lo (kExact, 64-bit values): the values carry the low FillKey half
+// above the position, and only windows in special blocks are read; the other
+// members of a segment holding a special window rebuild their window codes
+// from the high half every member of the segment shares and their own low
+// half (their windows hold no special symbol, so FillKey is their codes).
 template <typename Index, bool kExact, typename R = Index>
 __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     PackedText t, const Index* __restrict__ q_idx, const Index* __restrict__ q_row0,
     const Index* __restrict__ q_key, std::uint64_t qm, std::uint64_t n,
     R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
-    Index* __restrict__ out_row0, DeviceCounters* counters, const EarlyLcp early) {
+    Index* __restrict__ out_row0, DeviceCounters* counters, const EarlyLcp early,
+    bool packed_lo, const std::uint32_t* __restrict__ special_blocks) {
   using KeyT = typename std::conditional<kExact, std::uint32_t, Index>::type;
   constexpr int kItems = kLocalItems;
   constexpr std::uint32_t kSizeMask = 0xFFFFu;  // s_count: size | (specials << 16)
@@ -1457,20 +1492,29 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     spec[i] = 0;
     if constexpr (!kExact) key[i] = in ? q_key[at] : KeyT(0);
   }
+  bool loaded[kItems];  // kExact: the window was read from the text
   if constexpr (kExact) {
     uint4 e0[kItems], e1[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-      const std::uint64_t w = static_cast<std::uint64_t>(v[i]) >> 5;
-      e0[i] = t.words[w];
-      e1[i] = t.words[w + 1];
+      const int k = tid * kItems + i;
+      const std::uint64_t p = PosOf(v[i]);
+      loaded[i] = k < valid_len && (!packed_lo || MaybeSpecialWindow(special_blocks, p));
+      if (loaded[i]) {
+        e0[i] = t.words[p >> 5];
+        e1[i] = t.words[(p >> 5) + 1];
+      }
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       const int k = tid * kItems + i;
-      WindowOf(e0[i], e1[i], static_cast<std::uint32_t>(v[i] & 31), codes[i], spec[i]);
-      if (k >= valid_len) codes[i] = 0, spec[i] = 0;
-      key[i] = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]));
+      if (loaded[i]) {
+        WindowOf(e0[i], e1[i], static_cast<std::uint32_t>(v[i] & 31), codes[i], spec[i]);
+        key[i] = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]));
+      } else if (k < valid_len) {
+        key[i] = static_cast<std::uint32_t>(static_cast<std::uint64_t>(v[i]) >> 32);
+        codes[i] = key[i];  // the high half follows if the segment needs it
+      }
     }
   }
 #pragma unroll
@@ -1516,6 +1560,23 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
   __syncthreads();
   if constexpr (kExact) {
     // Exact keys only where a segment holds a special window.
+    if (packed_lo) {
+      // The segment's shared high half, from a member whose window was read
+      // (the special one is); s_off is free until the survivors' offsets.
+#pragma unroll
+      for (int i = 0; i < kItems; ++i) {
+        if (st[i] >= 0 && loaded[i] && (s_count[st[i]] >> 16) != 0) {
+          s_off[st[i]] = static_cast<std::uint32_t>(FillKey(codes[i], spec[i]) >> 32);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kItems; ++i) {
+        if (st[i] >= 0 && !loaded[i] && (s_count[st[i]] >> 16) != 0) {
+          codes[i] = (static_cast<std::uint64_t>(s_off[st[i]]) << 32) | (codes[i] & 0xFFFFFFFFull);
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       if (st[i] >= 0 && (s_count[st[i]] >> 16) != 0) {
@@ -1589,13 +1650,15 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     if (kExact && less > 0 && tie[i] == 0 && early_lcp < 32u && static_cast<std::uint64_t>(head[i]) < n) {
       WriteEarlyLcp(early, static_cast<std::uint64_t>(head[i]), early_lcp, true);
     }
-    if (static_cast<std::uint64_t>(v[i]) >= n || static_cast<std::uint64_t>(head[i]) >= n) {
+    const std::uint64_t pos = PosOf(v[i]);
+    v[i] = static_cast<Index>(pos);  // survivors go on as plain positions
+    if (pos >= n || static_cast<std::uint64_t>(head[i]) >= n) {
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
     } else if (equal[i] == 1) {
-      sa[head[i]] = static_cast<R>(v[i]);
-      rank[v[i]] = static_cast<R>(head[i]);
+      sa[head[i]] = static_cast<R>(pos);
+      rank[pos] = static_cast<R>(head[i]);
     } else {
-      rank[v[i]] = static_cast<R>(head[i]);
+      rank[pos] = static_cast<R>(head[i]);
       survivor[i] = true;
       if (tie[i] == 0) leader_total += equal[i];
     }
