@@ -418,11 +418,11 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 //                  PLCP + p + 1 over the irreducible entries up to k is
 //                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply leaves
 //                  (r, LCP[r]) pairs and counts the high digits of r.
-//   (radix passes over the digits of r from bit 16 up)
-//   LcpScatter     LCP[r] = value, pairs in order: the passes grouped them by
-//                  r >> 16, so the stores in flight fall in a few windows of
-//                  2^16 rows, which the L2 combines into whole lines, instead
-//                  of one partial line per store at random.
+//   (radix sort of the pairs by r)
+//   LcpScatter     LCP[r] = value, pairs in row order: the stores of one
+//                  instruction fall in a few lines instead of one partial
+//                  line each at random (a scatter of pairs grouped only by
+//                  r >> 16 left every store its own 32-byte write request).
 //
 // The check that replaces the inverse check of the full PLCP pass: the pairs
 // (r, SA[r]) over rows and (ISA[i], i) over positions are the same multiset
@@ -438,8 +438,6 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 constexpr int kMarkThreads = 256;
 constexpr int kMarkRows = 8;
 constexpr int kMarkTile = kMarkThreads * kMarkRows;
-// Row digits below this bit are left unsorted before the LCP scatter.
-constexpr int kScatterWindowBits = 16;
 
 template <typename T>
 __device__ __forceinline__ T LcpMarker() {
@@ -767,7 +765,8 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
   FlushDigits<kWaves>(local, passes, set.hist);
 }
 
-// LCP[rows[k]] = values[k] for the m pairs, in order.
+// LCP[rows[k]] = values[k] for the m pairs, in order (sorted by row, so the
+// stores of one instruction fall in a few lines).
 template <typename Out>
 __global__ __launch_bounds__(256) void LcpScatterKernel(const std::uint32_t* __restrict__ rows,
                                                         const std::uint32_t* __restrict__ values, std::uint64_t m,
@@ -780,8 +779,8 @@ __global__ __launch_bounds__(256) void LcpScatterKernel(const std::uint32_t* __r
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const std::uint64_t k = k0 + static_cast<std::uint64_t>(j) * 256;
-      r[j] = k < m ? rows[k] : 0xFFFFFFFFu;
-      v[j] = k < m ? values[k] : 0u;
+      r[j] = k < m ? __builtin_nontemporal_load(&rows[k]) : 0xFFFFFFFFu;
+      v[j] = k < m ? __builtin_nontemporal_load(&values[k]) : 0u;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {

@@ -1118,13 +1118,13 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
 
 // LCP of a build with early LCP (lcp_kernels.hpp, marked-row LCP): the
 // rows the doubling left marked get Φ/PLCP through a list sorted by
-// position, and their values go back through a scatter grouped by row; the
+// position, and their values go back through a scatter sorted by row; the
 // pass that builds the list checks that ISA inverts SA.  Workspace (every
 // buffer holds n 32-bit words, keys0/keys1 n 64-bit ones): vals0/vals1 the
 // list's positions and their radix ping-pong, keys0/keys1 its (Φ, row)
 // values and theirs, act_idx the bucket-start bitmap and the long-LCE
-// queue, act_seg the list's PLCP; the scatter's (row, value) pairs reuse
-// the buffers the sort left free.
+// queue, act_seg the list's PLCP; the (row, value) pairs, sorted by row for
+// the scatter, reuse the buffers the first sort left free.
 template <typename R, typename Out>
 void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const R* sa, const R* isa,
                      Out* lcp) {
@@ -1193,10 +1193,7 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
     // (pos_alt is now the other one), values in place.
     std::uint32_t* rows = pos == b->vals0.As<std::uint32_t>() ? b->vals1.As<std::uint32_t>()
                                                               : b->vals0.As<std::uint32_t>();
-    RadixPasses scatter_passes{};
-    for (int shift = kScatterWindowBits; shift < bits; shift += RadixTile::kDigitBits) {
-      scatter_passes.shift[scatter_passes.count++] = shift;
-    }
+    const RadixPasses scatter_passes = passes;  // rows lie below n, as positions do
     const RadixSet scatter_set = TakeRadixSet(b);
     const SortGeometry gs = Geometry(m, 256 * kScanItems, 1024);
     SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);

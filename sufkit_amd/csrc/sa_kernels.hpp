@@ -816,8 +816,8 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
     const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
     const Index delta = seg_delta[seg];
     const Index row = static_cast<Index>(delta + static_cast<Index>(k));
-    const std::uint64_t pos = PosOf(v);
-    if (static_cast<std::uint64_t>(row) >= n || pos >= n) continue;  // corrupted state only
+    const Index pos = static_cast<Index>(PosOf(v));  // in the list's own width
+    if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(pos) >= n) continue;  // corrupted state only
     if (early.lcp32 != nullptr || early.lcp64 != nullptr) {
       // The first step: every row gets its early LCP or the marker.
       std::uint32_t value = 32u;
@@ -1396,6 +1396,13 @@ __device__ inline bool KeyLess(const ExactKey& a, const ExactKey& b) {
 __device__ inline bool KeyEqual(const ExactKey& a, const ExactKey& b) {
   return a.w0 == b.w0 && a.w1 == b.w1 && a.w2 == b.w2;
 }
+// The same orders without short-circuit evaluation.
+__device__ __forceinline__ bool KeyLessFlat(const ExactKey& a, const ExactKey& b) {
+  return (a.w0 < b.w0) | ((a.w0 == b.w0) & ((a.w1 < b.w1) | ((a.w1 == b.w1) & (a.w2 < b.w2))));
+}
+__device__ __forceinline__ bool KeyEqualFlat(const ExactKey& a, const ExactKey& b) {
+  return (a.w0 == b.w0) & (a.w1 == b.w1) & (a.w2 == b.w2);
+}
 template <typename T>
 __device__ inline bool KeyLess(T a, T b) {
   return a < b;
@@ -1611,33 +1618,52 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     // The exact step also finds the largest smaller key (the group just
     // before this one in the segment), for the early LCP of a group start.
     std::uint32_t early_lcp = 32u;
+    // The scans of the segment: kScanUnroll keys per step, their LDS loads
+    // issued together (the index clamped to the segment's last slot, the
+    // excess ones not counted), compares and updates without branches.  The
+    // largest smaller key starts from 0: every key is at least 0, and it is
+    // used only when some key is smaller.
+    constexpr int kScanUnroll = 4;
     if (kExact && (packed >> 16) != 0) {
       const ExactKey key = s_exact[k];
       ExactKey before{0u, 0u, 0u};
-      for (int j = st[i]; j < en; ++j) {
-        const ExactKey kj = s_exact[j];
-        if (KeyLess(kj, key)) {
-          if (less == 0 || KeyLess(before, kj)) before = kj;
-          ++less;
-        } else if (KeyEqual(kj, key)) {
-          if (equal[i] == 0) first_equal[i] = j;
-          ++equal[i];
-          tie[i] += j < k;
+      for (int j = st[i]; j < en; j += kScanUnroll) {
+        ExactKey kj[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) kj[u] = s_exact[j + u < en ? j + u : en - 1];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+          const bool ok = j + u < en;
+          const bool lt = ok & KeyLessFlat(kj[u], key);
+          const bool eq = ok & KeyEqualFlat(kj[u], key);
+          const bool up = lt & KeyLessFlat(before, kj[u]);
+          before.w0 = up ? kj[u].w0 : before.w0;
+          before.w1 = up ? kj[u].w1 : before.w1;
+          before.w2 = up ? kj[u].w2 : before.w2;
+          less += lt;
+          first_equal[i] = (eq & (equal[i] == 0)) ? j + u : first_equal[i];
+          equal[i] += eq;
+          tie[i] += eq & (j + u < k);
         }
       }
       if (less > 0) early_lcp = ExactKeyMismatch(before, key);
     } else {
       const KeyT key = s_key[k];
       KeyT before = 0;
-      for (int j = st[i]; j < en; ++j) {
-        const KeyT kj = s_key[j];
-        if (kj < key) {
-          if (less == 0 || kj > before) before = kj;
-          ++less;
-        } else if (kj == key) {
-          if (equal[i] == 0) first_equal[i] = j;
-          ++equal[i];
-          tie[i] += j < k;
+      for (int j = st[i]; j < en; j += kScanUnroll) {
+        KeyT kj[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) kj[u] = s_key[j + u < en ? j + u : en - 1];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+          const bool ok = j + u < en;
+          const bool lt = ok & (kj[u] < key);
+          const bool eq = ok & (kj[u] == key);
+          before = (lt & (kj[u] > before)) ? kj[u] : before;
+          less += lt;
+          first_equal[i] = (eq & (equal[i] == 0)) ? j + u : first_equal[i];
+          equal[i] += eq;
+          tie[i] += eq & (j + u < k);
         }
       }
       // Segments without a special window: 16 equal symbols, then the low
