@@ -1469,10 +1469,13 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
   using KeyT = typename std::conditional<kExact, std::uint32_t, Index>::type;
   constexpr int kItems = kLocalItems;
   constexpr std::uint32_t kSizeMask = 0xFFFFu;  // s_count: size | (specials << 16)
-  __shared__ Index s_row0[kLocalSpan + 1];  // s_row0[k + 1] = slot k, s_row0[0] = slot -1
-  __shared__ KeyT s_key[kLocalSpan];
+  // Rows (below 2^32 for every text) as 32-bit values: s_row0[k + 1] =
+  // slot k, s_row0[0] = slot -1.
+  __shared__ std::uint32_t s_row0[kLocalSpan + 1];
+  __shared__ KeyT s_key[kLocalSpan];  // slot order, then sorted order in each segment
   __shared__ ExactKey s_exact[kExact ? kLocalSpan : 1];
-  __shared__ std::uint32_t s_count[kLocalSpan];
+  __shared__ std::uint32_t s_count[kLocalSpan];  // per segment start: size | (specials << 16)
+  __shared__ std::uint32_t s_group[kLocalSpan];  // per group: members counted so far
   __shared__ std::uint32_t s_off[kLocalSpan];
   __shared__ std::uint64_t scratch[kLocalThreads / kWave];
   __shared__ std::uint32_t scratch32[kLocalThreads / kWave];
@@ -1527,11 +1530,12 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const int k = tid * kItems + i;
-    s_row0[k + 1] = r0[i];
+    s_row0[k + 1] = static_cast<std::uint32_t>(r0[i]);
     s_key[k] = key[i];
     s_count[k] = 0;
+    s_group[k] = 0;
   }
-  if (tid == 0) s_row0[0] = s0 > 0 ? q_row0[s0 - 1] : Index(0);
+  if (tid == 0) s_row0[0] = s0 > 0 ? static_cast<std::uint32_t>(q_row0[s0 - 1]) : 0u;
   __syncthreads();
 
   // st + 1 of every slot (0: its segment starts before the span).
@@ -1543,8 +1547,8 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     // gfx950 dropped the update of `run` when this was written with
     // short-circuit && / || around the LDS reads.
     const int k = tid * kItems + i;
-    const Index here = s_row0[k + 1];
-    const Index before = s_row0[k];
+    const std::uint32_t here = s_row0[k + 1];
+    const std::uint32_t before = s_row0[k];
     const bool first_slot = (k == 0) & (s0 == 0);
     const bool starts = (k < valid_len) & (first_slot | Differ(here, before));
     run = starts ? static_cast<std::uint64_t>(k) + 1 : run;
@@ -1593,19 +1597,26 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
     __syncthreads();
   }
 
-  bool survivor[kItems];
-  std::uint32_t equal[kItems], tie[kItems];
-  int first_equal[kItems];
-  Index head[kItems];
-  std::uint32_t leader_total = 0;
+  // Ranks.  A suffix's rank in its segment is the number of smaller keys
+  // (less), scanned kScanUnroll keys per step with their LDS loads issued
+  // together (the index clamped to the segment's last slot, the excess keys
+  // not counted).  The members of a group share less, so slot st + less of
+  // the segment is the group's own: an LDS counter there gives each member
+  // its index in the group (tie) and, after a barrier, the group's size; and
+  // each member writing its key to slot st + less + tie leaves the segment's
+  // keys in sorted order, where slot st + less - 1 holds the largest smaller
+  // key (the early LCP of a group start).
+  constexpr int kScanUnroll = 4;
+  bool owned[kItems], special[kItems];
+  std::uint32_t less[kItems], tie[kItems], equal[kItems];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const int k = tid * kItems + i;
-    survivor[i] = false;
-    equal[i] = 0;
+    owned[i] = false;
+    special[i] = false;
+    less[i] = 0;
     tie[i] = 0;
-    first_equal[i] = k;
-    head[i] = r0[i];
+    equal[i] = 0;
     if (st[i] < 0 || st[i] >= kLocalOwn) continue;  // not owned by this block
     const std::uint32_t packed = s_count[st[i]];
     const std::uint32_t size = packed & kSizeMask;
@@ -1613,68 +1624,83 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagLocalSegment));
       continue;
     }
+    owned[i] = true;
     const int en = st[i] + static_cast<int>(size);
-    std::uint32_t less = 0;
-    // The exact step also finds the largest smaller key (the group just
-    // before this one in the segment), for the early LCP of a group start.
-    std::uint32_t early_lcp = 32u;
-    // The scans of the segment: kScanUnroll keys per step, their LDS loads
-    // issued together (the index clamped to the segment's last slot, the
-    // excess ones not counted), compares and updates without branches.  The
-    // largest smaller key starts from 0: every key is at least 0, and it is
-    // used only when some key is smaller.
-    constexpr int kScanUnroll = 4;
+    std::uint32_t count = 0;
     if (kExact && (packed >> 16) != 0) {
+      special[i] = true;
       const ExactKey key = s_exact[k];
-      ExactKey before{0u, 0u, 0u};
       for (int j = st[i]; j < en; j += kScanUnroll) {
         ExactKey kj[kScanUnroll];
 #pragma unroll
         for (int u = 0; u < kScanUnroll; ++u) kj[u] = s_exact[j + u < en ? j + u : en - 1];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
-          const bool ok = j + u < en;
-          const bool lt = ok & KeyLessFlat(kj[u], key);
-          const bool eq = ok & KeyEqualFlat(kj[u], key);
-          const bool up = lt & KeyLessFlat(before, kj[u]);
-          before.w0 = up ? kj[u].w0 : before.w0;
-          before.w1 = up ? kj[u].w1 : before.w1;
-          before.w2 = up ? kj[u].w2 : before.w2;
-          less += lt;
-          first_equal[i] = (eq & (equal[i] == 0)) ? j + u : first_equal[i];
-          equal[i] += eq;
-          tie[i] += eq & (j + u < k);
-        }
+        for (int u = 0; u < kScanUnroll; ++u) count += (j + u < en) & KeyLessFlat(kj[u], key);
       }
-      if (less > 0) early_lcp = ExactKeyMismatch(before, key);
     } else {
       const KeyT key = s_key[k];
-      KeyT before = 0;
       for (int j = st[i]; j < en; j += kScanUnroll) {
         KeyT kj[kScanUnroll];
 #pragma unroll
         for (int u = 0; u < kScanUnroll; ++u) kj[u] = s_key[j + u < en ? j + u : en - 1];
 #pragma unroll
-        for (int u = 0; u < kScanUnroll; ++u) {
-          const bool ok = j + u < en;
-          const bool lt = ok & (kj[u] < key);
-          const bool eq = ok & (kj[u] == key);
-          before = (lt & (kj[u] > before)) ? kj[u] : before;
-          less += lt;
-          first_equal[i] = (eq & (equal[i] == 0)) ? j + u : first_equal[i];
-          equal[i] += eq;
-          tie[i] += eq & (j + u < k);
-        }
-      }
-      // Segments without a special window: 16 equal symbols, then the low
-      // halves of FillKey, exact 2-bit codes.
-      if (kExact && less > 0) {
-        early_lcp = 16u + (static_cast<std::uint32_t>(__clz(static_cast<std::uint32_t>(before ^ key))) >> 1);
+        for (int u = 0; u < kScanUnroll; ++u) count += (j + u < en) & (kj[u] < key);
       }
     }
-    head[i] = static_cast<Index>(r0[i] + static_cast<Index>(less));
-    if (kExact && less > 0 && tie[i] == 0 && early_lcp < 32u && static_cast<std::uint64_t>(head[i]) < n) {
-      WriteEarlyLcp(early, static_cast<std::uint64_t>(head[i]), early_lcp, true);
+    less[i] = count;
+    tie[i] = atomicAdd(&s_group[st[i] + static_cast<int>(count)], 1u);
+  }
+  __syncthreads();  // every scan read its keys and every group is counted
+  ExactKey my_exact[kItems];
+  KeyT my_key[kItems];
+  if constexpr (kExact) {
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int k = tid * kItems + i;
+      if (special[i]) my_exact[i] = s_exact[k];
+      my_key[i] = s_key[k];
+    }
+    __syncthreads();  // keys read before any slot is overwritten
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if (!owned[i]) continue;
+    const int g = st[i] + static_cast<int>(less[i]);
+    equal[i] = s_group[g];
+    if constexpr (kExact) {
+      if (special[i]) {
+        s_exact[g + static_cast<int>(tie[i])] = my_exact[i];
+      } else {
+        s_key[g + static_cast<int>(tie[i])] = my_key[i];
+      }
+    }
+  }
+  if constexpr (kExact) __syncthreads();  // sorted keys in place
+
+  bool survivor[kItems];
+  Index head[kItems];
+  std::uint32_t leader_total = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    survivor[i] = false;
+    head[i] = r0[i];
+    if (!owned[i]) continue;
+    const int g = st[i] + static_cast<int>(less[i]);
+    head[i] = static_cast<Index>(r0[i] + static_cast<Index>(less[i]));
+    if constexpr (kExact) {
+      // The early LCP of a group start against the group before it: 16 equal
+      // symbols and the low halves of FillKey (exact 2-bit codes) in a segment
+      // without a special window, the exact 96-bit keys otherwise.
+      if (less[i] > 0 && tie[i] == 0 && static_cast<std::uint64_t>(head[i]) < n) {
+        std::uint32_t early_lcp;
+        if (special[i]) {
+          early_lcp = ExactKeyMismatch(s_exact[g - 1], my_exact[i]);
+        } else {
+          early_lcp = 16u + (static_cast<std::uint32_t>(
+                                 __clz(static_cast<std::uint32_t>(s_key[g - 1] ^ my_key[i]))) >> 1);
+        }
+        if (early_lcp < 32u) WriteEarlyLcp(early, static_cast<std::uint64_t>(head[i]), early_lcp, true);
+      }
     }
     const std::uint64_t pos = PosOf(v[i]);
     v[i] = static_cast<Index>(pos);  // survivors go on as plain positions
@@ -1693,9 +1719,8 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
   std::uint32_t off = BlockExclusiveSum<kLocalThreads>(leader_total, scratch32, total);
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int k = tid * kItems + i;
     if (survivor[i] && tie[i] == 0) {
-      s_off[k] = off;
+      s_off[st[i] + static_cast<int>(less[i])] = off;
       off += equal[i];
     }
   }
@@ -1706,7 +1731,7 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     if (survivor[i]) {
-      const std::uint64_t at = s_base + s_off[first_equal[i]] + tie[i];
+      const std::uint64_t at = s_base + s_off[st[i] + static_cast<int>(less[i])] + tie[i];
       out_idx[at] = v[i];
       out_row0[at] = head[i];
     }
