@@ -418,11 +418,12 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 //                  PLCP + p + 1 over the irreducible entries up to k is
 //                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply leaves
 //                  (r, LCP[r]) pairs and counts the high digits of r.
-//   (radix sort of the pairs by r)
-//   LcpScatter     LCP[r] = value, pairs in row order: the stores of one
+//   (radix passes over the digits of r from bit 8 up)
+//   LcpScatter     LCP[r] = value, pairs grouped by r >> 8: the pairs of a
+//                  window of 256 rows are contiguous, so the stores of one
 //                  instruction fall in a few lines instead of one partial
-//                  line each at random (a scatter of pairs grouped only by
-//                  r >> 16 left every store its own 32-byte write request).
+//                  line each at random (pairs grouped only by r >> 16 left
+//                  every store its own 32-byte write request).
 //
 // The check that replaces the inverse check of the full PLCP pass: the pairs
 // (r, SA[r]) over rows and (ISA[i], i) over positions are the same multiset
@@ -438,6 +439,8 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 constexpr int kMarkThreads = 256;
 constexpr int kMarkRows = 8;
 constexpr int kMarkTile = kMarkThreads * kMarkRows;
+// Row digits below this bit stay unsorted before the LCP scatter.
+constexpr int kScatterWindowBits = 8;
 
 template <typename T>
 __device__ __forceinline__ T LcpMarker() {
@@ -765,8 +768,8 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
   FlushDigits<kWaves>(local, passes, set.hist);
 }
 
-// LCP[rows[k]] = values[k] for the m pairs, in order (sorted by row, so the
-// stores of one instruction fall in a few lines).
+// LCP[rows[k]] = values[k] for the m pairs, in order (grouped by r >> 8, so
+// the stores of one instruction fall in a few lines).
 template <typename Out>
 __global__ __launch_bounds__(256) void LcpScatterKernel(const std::uint32_t* __restrict__ rows,
                                                         const std::uint32_t* __restrict__ values, std::uint64_t m,

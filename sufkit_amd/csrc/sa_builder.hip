@@ -1193,7 +1193,13 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
     // (pos_alt is now the other one), values in place.
     std::uint32_t* rows = pos == b->vals0.As<std::uint32_t>() ? b->vals1.As<std::uint32_t>()
                                                               : b->vals0.As<std::uint32_t>();
-    const RadixPasses scatter_passes = passes;  // rows lie below n, as positions do
+    // Rows sorted down to windows of 2^kScatterWindowBits rows: the pairs of
+    // one window are contiguous, so the stores of one instruction fall in a
+    // few lines, for one radix pass fewer than a full sort.
+    RadixPasses scatter_passes{};
+    for (int shift = kScatterWindowBits; shift < bits; shift += RadixTile::kDigitBits) {
+      scatter_passes.shift[scatter_passes.count++] = shift;
+    }
     const RadixSet scatter_set = TakeRadixSet(b);
     const SortGeometry gs = Geometry(m, 256 * kScanItems, 1024);
     SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);
