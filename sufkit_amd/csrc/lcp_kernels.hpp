@@ -282,13 +282,29 @@ __global__ __launch_bounds__(256) void PlcpScanReduceKernel(const std::uint32_t*
   BlockAddTo<kBlock>(irreducible, scratch, &counters->irreducible);
 }
 
+// Exclusive max of the chunk maxima, grid <= kSpineChunks (four per thread).
+constexpr int kSpineChunks = 4096;
+
 __global__ __launch_bounds__(1024) void PlcpScanSpineKernel(std::uint64_t* partial, int grid) {
   constexpr int kBlock = 1024;
+  constexpr int kPer = kSpineChunks / kBlock;
   __shared__ std::uint64_t scratch[kBlock / kWave];
-  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  std::uint64_t v[kPer];
+  std::uint64_t local = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = static_cast<int>(threadIdx.x) * kPer + j;
+    v[j] = c < grid ? partial[c] : 0ull;
+    local = v[j] > local ? v[j] : local;
+  }
   std::uint64_t total;
-  const std::uint64_t excl = BlockExclusiveMax<kBlock>(valid ? partial[threadIdx.x] : 0ull, scratch, total);
-  if (valid) partial[threadIdx.x] = excl;
+  std::uint64_t run = BlockExclusiveMax<kBlock>(local, scratch, total);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = static_cast<int>(threadIdx.x) * kPer + j;
+    if (c < grid) partial[c] = run;
+    run = v[j] > run ? v[j] : run;
+  }
 }
 
 // Inclusive max-scan of the keys, written back as PLCP[k] = scan - k - 1.
@@ -506,14 +522,29 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkCountKernel(const Out* __
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// Exclusive sum of the chunk counts (grid <= 1024), the total into *total.
+// Exclusive sum of the chunk counts (grid <= kSpineChunks, four per thread),
+// the total into *total.
 __global__ __launch_bounds__(1024) void MarkSpineKernel(std::uint64_t* partial, int grid,
                                                         unsigned long long* total) {
-  __shared__ std::uint64_t scratch[1024 / kWave];
-  const bool valid = static_cast<int>(threadIdx.x) < grid;
+  constexpr int kBlock = 1024;
+  constexpr int kPer = kSpineChunks / kBlock;
+  __shared__ std::uint64_t scratch[kBlock / kWave];
+  std::uint64_t v[kPer];
+  std::uint64_t local = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = static_cast<int>(threadIdx.x) * kPer + j;
+    v[j] = c < grid ? partial[c] : std::uint64_t{0};
+    local += v[j];
+  }
   std::uint64_t sum;
-  const std::uint64_t excl = BlockExclusiveSum<1024>(valid ? partial[threadIdx.x] : std::uint64_t{0}, scratch, sum);
-  if (valid) partial[threadIdx.x] = excl;
+  std::uint64_t run = BlockExclusiveSum<kBlock>(local, scratch, sum);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int c = static_cast<int>(threadIdx.x) * kPer + j;
+    if (c < grid) partial[c] = run;
+    run += v[j];
+  }
   if (threadIdx.x == 0) *total = sum;
 }
 

@@ -631,7 +631,8 @@ class Doubling {
       std::uint32_t* flags = SpecialFlags();
       SFK_HIP_CHECK(hipMemsetAsync(flags, 0, (lm_bound_ / 2 + 1) * sizeof(std::uint32_t), b_->stream));
       ResetSpecialCountKernel<<<1, 1, 0, b_->stream>>>(dc);
-      MarkSpecialSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256)), 256, 0, b_->stream>>>(
+      MarkSpecialSegmentsKernel<Index><<<static_cast<unsigned>(CeilDiv(lm_bound_, 256 * kMarkSpans)), 256, 0,
+                                         b_->stream>>>(
           b_->act_idx.As<Index>(), b_->act_seg.As<std::uint32_t>(), text_, b_->special_blocks.As<std::uint32_t>(),
           flags, dc);
       SFK_HIP_CHECK(hipGetLastError());
@@ -1125,6 +1126,9 @@ void LaunchLcp(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, c
 // values and theirs, act_idx the bucket-start bitmap and the long-LCE
 // queue, act_seg the list's PLCP; the (row, value) pairs, sorted by row for
 // the scatter, reuse the buffers the first sort left free.
+static_assert(kSpineChunks * sizeof(std::uint64_t) <= kRefineMaxChunks * sizeof(RefinePartial),
+              "the builder's partial buffer holds every chunk total of the LCP scans");
+
 template <typename R, typename Out>
 void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText text, const R* sa, const R* isa,
                      Out* lcp) {
@@ -1155,7 +1159,7 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   const std::uint64_t bucket_threads = 7ull * (ctx.refs > 0 ? ctx.refs : 1);
   BucketStartsKernel<R><<<static_cast<unsigned>(CeilDiv(bucket_threads, 64)), 64, 0, b->stream>>>(
       d_text, sa, n, ctx.d_offsets, ctx.refs, starts);
-  const SortGeometry g = Geometry(n, kMarkTile, 1024);
+  const SortGeometry g = Geometry(n, kMarkTile, kSpineChunks);
   std::uint64_t* partial = b->partial.As<std::uint64_t>();
   LcpMarkCountKernel<Out><<<g.grid, kMarkThreads, 0, b->stream>>>(lcp, n, g.chunk, aligned(lcp), partial);
   MarkSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid, &dc->lcp_marked);
@@ -1201,7 +1205,7 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
       scatter_passes.shift[scatter_passes.count++] = shift;
     }
     const RadixSet scatter_set = TakeRadixSet(b);
-    const SortGeometry gs = Geometry(m, 256 * kScanItems, 1024);
+    const SortGeometry gs = Geometry(m, 256 * kScanItems, kSpineChunks);
     SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);
     PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, gs.grid);
     SparseScanApplyKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, val, m, gs.chunk, partial, rows,

@@ -575,34 +575,58 @@ struct TextFillHiSource {
 // holds a special symbol and marks their segments (flags[segment] = 1, the
 // only value ever stored, so racing stores agree).  The list is in segment
 // order, so one lane per run of special suffixes of one segment in a wave
-// stores, and only while the flag still reads 0.
-// Windows outside special blocks are not read.
+// stores, and only while the flag still reads 0.  Each wave takes
+// kMarkSpans consecutive spans of 64 slots, all their loads issued before
+// any is used; windows outside special blocks are not read.
+constexpr int kMarkSpans = 4;
+
 template <typename Index>
 __global__ __launch_bounds__(256) void MarkSpecialSegmentsKernel(
     const Index* __restrict__ act_idx, const std::uint32_t* __restrict__ act_seg, PackedText t,
     const std::uint32_t* __restrict__ special_blocks, std::uint32_t* __restrict__ flags, DeviceCounters* counters) {
   const std::uint64_t m = counters->list.survivors;
-  const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool special = false;
-  std::uint32_t seg = 0;
-  if (k < m) {
-    seg = act_seg[k];
-    const std::uint64_t p = PosOf(act_idx[k]);
-    if (MaybeSpecialWindow(special_blocks, p)) {
-      std::uint64_t c;
-      std::uint32_t s;
-      LoadWindow(t, p, c, s);
-      special = s != 0;
+  const std::uint32_t lane = LaneId();
+  const std::uint64_t base =
+      (static_cast<std::uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave * kMarkSpans;
+  std::uint64_t pos[kMarkSpans];
+  std::uint32_t seg[kMarkSpans];
+  bool maybe[kMarkSpans];
+#pragma unroll
+  for (int u = 0; u < kMarkSpans; ++u) {
+    const std::uint64_t k = base + static_cast<std::uint64_t>(u) * kWave + lane;
+    pos[u] = k < m ? PosOf(act_idx[k]) : 0;
+    seg[u] = k < m ? act_seg[k] : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < kMarkSpans; ++u) {
+    const std::uint64_t k = base + static_cast<std::uint64_t>(u) * kWave + lane;
+    maybe[u] = k < m && MaybeSpecialWindow(special_blocks, pos[u]);
+  }
+  uint4 e0[kMarkSpans], e1[kMarkSpans];
+#pragma unroll
+  for (int u = 0; u < kMarkSpans; ++u) {
+    if (maybe[u]) {
+      e0[u] = t.words[pos[u] >> 5];
+      e1[u] = t.words[(pos[u] >> 5) + 1];
     }
   }
-  const std::uint64_t vote = __ballot(special);
-  const std::uint32_t lane = LaneId();
-  const std::uint32_t prev_seg = __shfl_up(seg, 1, kWave);
-  const bool prev_special = lane > 0 && ((vote >> (lane - 1)) & 1ull) != 0;
-  if (special && !(prev_special && prev_seg == seg) && flags[seg] == 0) flags[seg] = 1;
-  if (lane == 0 && vote) {
-    atomicAdd(&counters->special_active, static_cast<unsigned long long>(__popcll(vote)));
+  unsigned long long specials = 0;
+#pragma unroll
+  for (int u = 0; u < kMarkSpans; ++u) {
+    bool special = false;
+    if (maybe[u]) {
+      std::uint64_t c;
+      std::uint32_t sp;
+      WindowOf(e0[u], e1[u], static_cast<std::uint32_t>(pos[u] & 31), c, sp);
+      special = sp != 0;
+    }
+    const std::uint64_t vote = __ballot(special);
+    const std::uint32_t prev_seg = __shfl_up(seg[u], 1, kWave);
+    const bool prev_special = lane > 0 && ((vote >> (lane - 1)) & 1ull) != 0;
+    if (special && !(prev_special && prev_seg == seg[u]) && flags[seg[u]] == 0) flags[seg[u]] = 1;
+    specials += static_cast<unsigned long long>(__popcll(vote));
   }
+  if (lane == 0 && specials) atomicAdd(&counters->special_active, specials);
 }
 
 // ------------------------------------------------------------ refine ------
