@@ -120,11 +120,12 @@ def radix_tuning(args, max_elements: int):
         source = "sfk_hip_autotune"
         if args.save_tuning:
             Path(args.save_tuning).write_text(json.dumps([e for e in measured if e["chosen"]]))
+    # per kind and class: the shape of each digit width with its time per pass
     chosen = {}
     for e in sufkit_amd.tune_get():
         if e["tuned"]:
-            chosen.setdefault(sufkit_amd.TUNE_KINDS[e["kind"]], {})[f"class{e['size_class']}"] = \
-                f"{e['block']}x{e['items']}"
+            cell = chosen.setdefault(sufkit_amd.TUNE_KINDS[e["kind"]], {}).setdefault(f"class{e['size_class']}", [])
+            cell.append(f"{e['digit_bits']}b {e['block']}x{e['items']} {e['us_per_pass']:.1f}us")
     return {"source": source, "seconds": round(seconds, 3), "max_elements": max_elements, "chosen": chosen}
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SFK_HIP_ABI_VERSION 5
+#define SFK_HIP_ABI_VERSION 6
 
 enum sfk_hip_status {
   SFK_HIP_OK = 0,
@@ -310,9 +310,10 @@ int sfk_hip_index_sa_rows(sfk_hip_index* index, uint64_t begin, uint64_t end,
                           uint64_t* out);
 
 /*
- * Radix shape autotuner.  Every onesweep pass of a build runs one of a set
- * of compiled shapes (threads per block x keys per thread), chosen per kind
- * of sort and size class of the list from a per-device table:
+ * Radix autotuner.  Every onesweep pass of a build runs one of a set of
+ * compiled shapes (threads per block x keys per thread) at a digit width of
+ * 8 or 7 bits, chosen per kind of sort and size class of the list from a
+ * per-device table:
  *   kind 0 u32 key/u32 value, 1 u64/u32, 2 u32/u64, 3 u64/u64,
  *        4 u32/u32 and 5 u32/u64 sorted in place per reference (the first
  *        step of a batch)
@@ -330,6 +331,11 @@ int sfk_hip_index_sa_rows(sfk_hip_index* index, uint64_t begin, uint64_t end,
  * chosen = 1 (a saved table) and rejects shapes that are not compiled for
  * their kind; sfk_hip_tune_reset restores the defaults.  No counterpart in
  * the reference, whose CPU engines have no device shapes to pick.
+ * Digit widths: the table holds a shape and its time per pass for each
+ * width (kinds 4 and 5 only 8-bit); the tuner times both, and a sort of
+ * b-bit keys takes 7-bit digits when ceil(b/7) 7-bit passes of its kind and
+ * class take less time than ceil(b/8) 8-bit ones (8 when either width has
+ * no time, so the defaults and tables saved without times sort 8-bit).
  */
 typedef struct sfk_hip_tune_entry {
   uint32_t kind;
@@ -339,6 +345,7 @@ typedef struct sfk_hip_tune_entry {
   double us_per_pass;
   uint32_t chosen;
   uint32_t tuned;
+  uint32_t digit_bits; /* 8 or 7 (0 reads as 8 in sfk_hip_tune_set) */
 } sfk_hip_tune_entry;
 int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entries,
                      uint32_t capacity, uint32_t* count, char* error,

@@ -549,7 +549,7 @@ def hip_build_fast(text: np.ndarray, prefix_k: int, construction_bits: int = 32)
 class TuneEntry(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("size_class", ctypes.c_uint32), ("block", ctypes.c_uint32),
                 ("items", ctypes.c_uint32), ("us_per_pass", ctypes.c_double), ("chosen", ctypes.c_uint32),
-                ("tuned", ctypes.c_uint32)]
+                ("tuned", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32)]
 
 
 TUNE_KINDS = ("u32/u32", "u64/u32", "u32/u64", "u64/u64", "u32/u32 per reference", "u32/u64 per reference")
@@ -591,23 +591,27 @@ def autotune(max_elements: int, device: int = 0) -> list[dict]:
 
 
 def tune_get(device: int = 0) -> list[dict]:
-    """The radix shape in use for every (kind, size class) of the device."""
+    """The radix shape in use for every (kind, size class, digit width) of
+    the device."""
     lib = _tune_lib()
-    arr = (TuneEntry * 64)()
+    cap = 128
+    arr = (TuneEntry * cap)()
     count = ctypes.c_uint32(0)
-    status = lib.sfk_hip_tune_get(device, arr, 64, ctypes.byref(count))
+    status = lib.sfk_hip_tune_get(device, arr, cap, ctypes.byref(count))
     if status != 0:
         raise SufkitError(status, "tune_get failed")
-    return _entries(arr, min(count.value, 64))
+    return _entries(arr, min(count.value, cap))
 
 
 def tune_set(entries: list[dict], device: int = 0) -> None:
-    """Install radix shapes (entries with chosen = 1), e.g. a saved table."""
+    """Install radix shapes (entries with chosen = 1), e.g. a saved table.
+    digit_bits defaults to 8; us_per_pass, when given for both widths of a
+    (kind, class), decides which width the sorts take."""
     lib = _tune_lib()
     arr = (TuneEntry * max(1, len(entries)))()
     for i, e in enumerate(entries):
         arr[i] = TuneEntry(e["kind"], e["size_class"], e["block"], e["items"], e.get("us_per_pass", 0.0),
-                           e.get("chosen", 1), e.get("tuned", 1))
+                           e.get("chosen", 1), e.get("tuned", 1), e.get("digit_bits", 8))
     err = ctypes.create_string_buffer(512)
     status = lib.sfk_hip_tune_set(device, arr, len(entries), err, len(err))
     if status != 0:

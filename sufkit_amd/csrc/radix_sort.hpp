@@ -298,11 +298,14 @@ __device__ inline std::uint64_t LookBackDigits(unsigned long long* status, std::
   return prefix;
 }
 
-// Digit passes of one sort: the bit offset of each 8-bit digit that varies.
+// Digit passes of one sort: the bit offset of each digit that varies, and
+// the digit width (8 bits, or 7 where the tuner measured 7-bit passes to be
+// cheaper for the key's width).
 struct RadixPasses {
   static constexpr int kMax = 8;
   int count;
   int shift[kMax];
+  int bits = RadixTile::kDigitBits;
 };
 
 // Digit counts of one sort, hist[pass][digit], and the tile counters of its
@@ -334,9 +337,9 @@ __device__ inline void ClearNextRadixSet(const RadixSet& set) {
 template <typename K>
 __device__ inline void CountDigits(std::uint32_t (*local)[RadixTile::kRadix], const RadixPasses& passes,
                                    K key, bool valid, bool full) {
-  constexpr int kRadix = RadixTile::kRadix;
+  const std::uint32_t mask = (1u << passes.bits) - 1u;
   for (int p = 0; p < passes.count; ++p) {
-    const std::uint32_t d = static_cast<std::uint32_t>(key >> passes.shift[p]) & (kRadix - 1);
+    const std::uint32_t d = static_cast<std::uint32_t>(key >> passes.shift[p]) & mask;
     if (full) {
       const std::uint32_t first = __builtin_amdgcn_readfirstlane(d);
       if (__all(d == first)) {
@@ -447,17 +450,21 @@ constexpr bool OnesweepFits(int block, int items, int key_bytes, int val_bytes, 
 }
 
 // digit_count: this pass's counts per digit (from CountDigits); unused when
-// kSegmented (seg gives the placement instead).
-template <typename K, typename V, int kBlock, int kItems, typename Source, bool kSegmented = false>
+// kSegmented (seg gives the placement instead).  kBits: the digit width (the
+// status words keep a stride of 256 digits per tile).
+template <typename K, typename V, int kBlock, int kItems, typename Source, bool kSegmented = false,
+          int kBits = RadixTile::kDigitBits>
 __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     Source src, K* __restrict__ keys_out, V* __restrict__ vals_out, std::uint64_t m, int shift,
     const unsigned long long* __restrict__ digit_count, unsigned long long* status,
     std::uint32_t epoch, unsigned int* error_flags,
     std::uint32_t error_bit, unsigned int* tickets,
     SegmentedPass seg = SegmentedPass{nullptr, 0, nullptr}) {
-  constexpr int kRadix = RadixTile::kRadix;
+  constexpr int kRadix = 1 << kBits;
   constexpr int kWaves = kBlock / kWave;
   constexpr int kTile = kBlock * kItems;
+  static_assert(kBits == 7 || kBits == 8, "7- or 8-bit digits");
+  static_assert(!kSegmented || kBits == RadixTile::kDigitBits, "per-reference placement tables hold 8-bit digits");
   static_assert(kBlock >= kRadix, "one digit per thread for the look-back");
   static_assert(!kSegmented || kTile <= 65536, "segment offsets within a tile are 16-bit");
   __shared__ std::uint32_t wave_cnt[kWaves][kRadix];
@@ -546,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void RadixOnesweep(
     const bool valid = idx < end;
     const std::uint32_t d = static_cast<std::uint32_t>(key[j] >> shift) & (kRadix - 1);
     digit[j] = d;
-    const std::uint64_t peers = MatchDigit<RadixTile::kDigitBits>(d, valid);
+    const std::uint64_t peers = MatchDigit<kBits>(d, valid);
     const std::uint32_t p_lo = static_cast<std::uint32_t>(peers), p_hi = static_cast<std::uint32_t>(peers >> 32);
     const std::uint32_t below = __builtin_amdgcn_mbcnt_hi(p_hi, __builtin_amdgcn_mbcnt_lo(p_lo, 0u));
     const std::uint32_t count = static_cast<std::uint32_t>(__popc(p_lo) + __popc(p_hi));

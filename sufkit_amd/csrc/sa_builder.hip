@@ -313,24 +313,51 @@ int DefaultShape(int kind, std::uint64_t m) {
 }
 
 constexpr int kMaxTuneDevices = 64;
-// -1: the default rule; otherwise an index into kShapes.
-std::atomic<int> g_shape_table[kMaxTuneDevices][kSortKinds][kSizeClasses];
+// Digit widths a sort may use: index 0 for 8-bit digits, 1 for 7-bit ones
+// (the per-reference kinds only have 8-bit placement tables).
+constexpr int kDigitWidths = 2;
+constexpr int DigitBitsOf(int width) { return width == 0 ? 8 : 7; }
+constexpr int WidthOf(int bits) { return bits == 8 ? 0 : 1; }
+// -1: the default rule; otherwise an index into kShapes.  Beside each shape,
+// its measured time per pass in microseconds (0: not measured), from which a
+// sort picks its digit width.
+std::atomic<int> g_shape_table[kMaxTuneDevices][kSortKinds][kSizeClasses][kDigitWidths];
+std::atomic<double> g_pass_us[kMaxTuneDevices][kSortKinds][kSizeClasses][kDigitWidths];
 std::once_flag g_shape_table_once;
 
 void InitShapeTable() {
   std::call_once(g_shape_table_once, [] {
     for (auto& dev : g_shape_table)
       for (auto& kind : dev)
-        for (auto& c : kind) c.store(-1);
+        for (auto& c : kind)
+          for (auto& w : c) w.store(-1);
+    for (auto& dev : g_pass_us)
+      for (auto& kind : dev)
+        for (auto& c : kind)
+          for (auto& w : c) w.store(0.0);
   });
 }
 
-int ShapeFor(int device, int kind, std::uint64_t m) {
+int ShapeFor(int device, int kind, std::uint64_t m, int bits = RadixTile::kDigitBits) {
   InitShapeTable();
   const int tuned = device >= 0 && device < kMaxTuneDevices
-                        ? g_shape_table[device][kind][SizeClass(m)].load(std::memory_order_relaxed)
+                        ? g_shape_table[device][kind][SizeClass(m)][WidthOf(bits)].load(std::memory_order_relaxed)
                         : -1;
   return tuned >= 0 ? tuned : DefaultShape(kind, m);
+}
+
+// Digit width of a sort of m keys of key_bits bits: 7 when both widths were
+// measured for this kind and size class and ceil(key_bits / 7) 7-bit passes
+// take less time than ceil(key_bits / 8) 8-bit ones, else 8.
+int DigitBitsFor(int device, int kind, std::uint64_t m, int key_bits) {
+  InitShapeTable();
+  if (device < 0 || device >= kMaxTuneDevices || kind >= 4) return RadixTile::kDigitBits;
+  const int c = SizeClass(m);
+  const double us8 = g_pass_us[device][kind][c][0].load(std::memory_order_relaxed);
+  const double us7 = g_pass_us[device][kind][c][1].load(std::memory_order_relaxed);
+  const int passes8 = (key_bits + 7) / 8, passes7 = (key_bits + 6) / 7;
+  if (us8 <= 0.0 || us7 <= 0.0 || passes7 > RadixPasses::kMax) return RadixTile::kDigitBits;
+  return passes7 * us7 < passes8 * us8 ? 7 : RadixTile::kDigitBits;
 }
 
 std::uint64_t OnesweepStatusBytes(std::uint64_t n) {
@@ -375,15 +402,16 @@ RadixSet TakeRadixSet(sfk_hip_builder* b) {
                   reinterpret_cast<unsigned long long*>(next), reinterpret_cast<unsigned int*>(next + hist_bytes)};
 }
 
-// Every 8-bit digit below total_bits (keys are not inspected for constant
-// digits: that would take a host round trip per step).
-RadixPasses AllDigits(int total_bits) {
+// Every digit of `bits` bits below total_bits (keys are not inspected for
+// constant digits: that would take a host round trip per step).
+RadixPasses AllDigits(int total_bits, int bits = RadixTile::kDigitBits) {
   RadixPasses passes{};
-  for (int shift = 0; shift < total_bits; shift += RadixTile::kDigitBits) passes.shift[passes.count++] = shift;
+  passes.bits = bits;
+  for (int shift = 0; shift < total_bits; shift += bits) passes.shift[passes.count++] = shift;
   return passes;
 }
 
-template <typename K, typename V, int kBlock, int kItems, bool kSeg, typename Source>
+template <typename K, typename V, int kBlock, int kItems, bool kSeg, int kBits, typename Source>
 void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys, V*& vals,
                      K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set,
                      const long long* adj) {
@@ -408,11 +436,11 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
     }
     EventPair* ev = BeginProfiled(ctx, bytes);
     if (p == 0) {
-      RadixOnesweep<K, V, kBlock, kItems, Source, kSeg><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
+      RadixOnesweep<K, V, kBlock, kItems, Source, kSeg, kBits><<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
           first, keys_alt, vals_alt, m, passes.shift[p], set.hist + p * RadixTile::kRadix, status, epoch,
           error_flags, kFlagLookback, b->tickets.As<unsigned int>(), seg);
     } else {
-      RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg>
+      RadixOnesweep<K, V, kBlock, kItems, ArraySource<K, V>, kSeg, kBits>
           <<<static_cast<unsigned>(tiles), kBlock, 0, b->stream>>>(
               ArraySource<K, V>{keys, vals}, keys_alt, vals_alt, m, passes.shift[p],
               set.hist + p * RadixTile::kRadix, status, epoch, error_flags, kFlagLookback,
@@ -427,14 +455,31 @@ void OnesweepPassesT(BuildContext& ctx, const Source& first, std::uint64_t first
   }
 }
 
+// 7-bit passes exist for the sorts of arrays (not for the per-reference
+// kinds, whose placement tables hold 8-bit digits, nor for a first pass that
+// generates its input from the text, whose keys are 32 bits wide).
+template <typename K, typename V, bool kSeg, typename Source>
+constexpr bool kSevenBitSort = !kSeg && std::is_same<Source, ArraySource<K, V>>::value;
+
 template <typename K, typename V, int kShape, bool kSeg, typename Source>
 void OnesweepPassesShape(BuildContext& ctx, const Source& first, std::uint64_t first_read_bytes, K*& keys,
                          V*& vals, K*& keys_alt, V*& vals_alt, std::uint64_t m, const RadixPasses& passes,
                          const RadixSet& set, const long long* adj) {
   constexpr ShapeDesc d = kShapes[kShape];
   if constexpr (ShapeFits<K, V, d.block, d.items, kSeg>()) {
-    OnesweepPassesT<K, V, d.block, d.items, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m,
-                                                  passes, set, adj);
+    if (passes.bits == RadixTile::kDigitBits) {
+      OnesweepPassesT<K, V, d.block, d.items, kSeg, RadixTile::kDigitBits>(ctx, first, first_read_bytes, keys, vals,
+                                                                          keys_alt, vals_alt, m, passes, set, adj);
+      return;
+    }
+    if constexpr (kSevenBitSort<K, V, kSeg, Source>) {
+      if (passes.bits == 7) {
+        OnesweepPassesT<K, V, d.block, d.items, kSeg, 7>(ctx, first, first_read_bytes, keys, vals, keys_alt,
+                                                         vals_alt, m, passes, set, adj);
+        return;
+      }
+    }
+    Fail(SFK_HIP_BUILD_FAILURE, "radix digit width not compiled for this sort");
   } else {
     Fail(SFK_HIP_BUILD_FAILURE, "radix shape does not fit the LDS of a workgroup");
   }
@@ -450,7 +495,7 @@ void SortPairs(BuildContext& ctx, const Source& first, std::uint64_t first_read_
                K* keys_alt, V* vals_alt, std::uint64_t m, const RadixPasses& passes, const RadixSet& set,
                const long long* adj = nullptr, int shape = -1) {
   if (passes.count == 0 || m == 0) return;
-  if (shape < 0) shape = ShapeFor(ctx.b->device, SortKind<K, V, kSeg>(), m);
+  if (shape < 0) shape = ShapeFor(ctx.b->device, SortKind<K, V, kSeg>(), m, passes.bits);
   switch (shape) {
     case 0: OnesweepPassesShape<K, V, 0, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
     case 1: OnesweepPassesShape<K, V, 1, kSeg>(ctx, first, first_read_bytes, keys, vals, keys_alt, vals_alt, m, passes, set, adj); break;
@@ -506,12 +551,14 @@ struct TuneResult {
   int kind;
   int size_class;
   int shape;
+  int bits;
   double us_per_pass;
 };
 
-// Times every compiled shape of one sort kind on uniformly random keys, for
-// each size class up to max_elements: two passes per sort, the best of two
-// timed sorts after one warm-up, on the tuning builder's stream.
+// Times every compiled shape of one sort kind at every digit width it has on
+// uniformly random keys, for each size class up to max_elements: two passes
+// per sort, the best of two timed sorts after one warm-up, on the tuning
+// builder's stream.
 template <typename K, typename V, bool kSeg>
 void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hipEvent_t e1,
               std::vector<TuneResult>& out) {
@@ -519,10 +566,6 @@ void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hip
   BuildContext ctx{};
   ctx.b = b;
   ctx.profile = false;
-  RadixPasses passes{};
-  passes.count = 2;
-  passes.shift[0] = 0;
-  passes.shift[1] = RadixTile::kDigitBits;
   long long* adj = b->ref_adj.As<long long>();
   if constexpr (kSeg) {
     ctx.d_offsets = b->ref_offsets.As<std::uint64_t>();
@@ -542,8 +585,14 @@ void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hip
     }
     const unsigned grid = static_cast<unsigned>(
         std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 4), 1024)));
+    for (int width = 0; width < (kSeg ? 1 : kDigitWidths); ++width)
     for (int shape = 0; shape < kShapeCount; ++shape) {
       if (!KindShapeFits(kind, shape)) continue;
+      RadixPasses passes{};
+      passes.bits = DigitBitsOf(width);
+      passes.count = 2;
+      passes.shift[0] = 0;
+      passes.shift[1] = passes.bits;
       float best = 0;
       for (int rep = 0; rep < 3; ++rep) {
         const RadixSet set = TakeRadixSet(b);
@@ -562,7 +611,7 @@ void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hip
         SFK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         if (rep == 1 || (rep > 1 && ms < best)) best = ms;
       }
-      out.push_back(TuneResult{kind, c, shape, best * 1000.0 / passes.count});
+      out.push_back(TuneResult{kind, c, shape, passes.bits, best * 1000.0 / passes.count});
     }
   }
 }
@@ -932,7 +981,6 @@ class Doubling {
 
     const bool rank_step = sec_kind == kSecRank;
     const RadixSet set = TakeRadixSet(b_);
-    const RadixPasses passes = AllDigits(total_bits);
     const unsigned keygen_grid = static_cast<unsigned>(
         std::max<std::uint64_t>(1, std::min<std::uint64_t>(CeilDiv(m, kKeygenBlock * 4), 1024)));
     // The first step of a single text generates its first pass's input from
@@ -940,6 +988,13 @@ class Doubling {
     // the text for the digit counts.
     constexpr bool kNarrowKey = sizeof(K) == 4;
     const bool segmented = segmented_first_ && sec_kind == kSecFillHi;
+    // Sorts of keygen arrays take the digit width the tuner's times favour
+    // for their key width; the first step's (from the text, or in place per
+    // reference) takes 8-bit digits.
+    const bool from_text = kNarrowKey && sec_kind == kSecFillHi && (segmented || (iota && seg_bits == 0));
+    const RadixPasses passes =
+        AllDigits(total_bits, from_text ? RadixTile::kDigitBits
+                                        : DigitBitsFor(b_->device, SortKind<K, Index, false>(), m, total_bits));
     if (kNarrowKey && segmented) {
       if (m != n_ || !iota || seg_bits != 0) Fail(SFK_HIP_BUILD_FAILURE, "segmented first step out of place");
       if constexpr (kNarrowKey) {
@@ -2250,7 +2305,8 @@ int sfk_hip_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
 
 namespace {
 
-void TuneEntryOf(int kind, int size_class, int shape, double us, bool chosen, bool tuned, sfk_hip_tune_entry* e) {
+void TuneEntryOf(int kind, int size_class, int shape, int bits, double us, bool chosen, bool tuned,
+                 sfk_hip_tune_entry* e) {
   e->kind = static_cast<uint32_t>(kind);
   e->size_class = static_cast<uint32_t>(size_class);
   e->block = static_cast<uint32_t>(sfk::kShapes[shape].block);
@@ -2258,6 +2314,7 @@ void TuneEntryOf(int kind, int size_class, int shape, double us, bool chosen, bo
   e->us_per_pass = us;
   e->chosen = chosen ? 1u : 0u;
   e->tuned = tuned ? 1u : 0u;
+  e->digit_bits = static_cast<uint32_t>(bits);
 }
 
 }  // namespace
@@ -2303,27 +2360,34 @@ int sfk_hip_autotune(int device, uint64_t max_elements, sfk_hip_tune_entry* entr
     sfk::TuneKind<std::uint32_t, std::uint64_t, true>(b, top, e0, e1, results);
     const sfk::DeviceCounters& c = sfk::ReadCounters(b);
     if (c.error_flags != 0) sfk::Fail(SFK_HIP_BUILD_FAILURE, "radix autotuning failed (look-back did not converge)");
-    // Fastest shape per (kind, class); classes above max_elements keep the
-    // table's current entry.
-    int best[sfk::kSortKinds][sfk::kSizeClasses];
-    double best_us[sfk::kSortKinds][sfk::kSizeClasses];
+    // Fastest shape per (kind, class, digit width), with its time per pass;
+    // classes above max_elements keep the table's current entries.
+    constexpr int W = sfk::kDigitWidths;
+    int best[sfk::kSortKinds][sfk::kSizeClasses][W];
+    double best_us[sfk::kSortKinds][sfk::kSizeClasses][W];
     for (int k = 0; k < sfk::kSortKinds; ++k)
-      for (int cl = 0; cl < sfk::kSizeClasses; ++cl) best[k][cl] = -1, best_us[k][cl] = 0;
+      for (int cl = 0; cl < sfk::kSizeClasses; ++cl)
+        for (int w = 0; w < W; ++w) best[k][cl][w] = -1, best_us[k][cl][w] = 0;
     for (const auto& r : results) {
-      if (best[r.kind][r.size_class] < 0 || r.us_per_pass < best_us[r.kind][r.size_class]) {
-        best[r.kind][r.size_class] = r.shape;
-        best_us[r.kind][r.size_class] = r.us_per_pass;
+      const int w = sfk::WidthOf(r.bits);
+      if (best[r.kind][r.size_class][w] < 0 || r.us_per_pass < best_us[r.kind][r.size_class][w]) {
+        best[r.kind][r.size_class][w] = r.shape;
+        best_us[r.kind][r.size_class][w] = r.us_per_pass;
       }
     }
     sfk::InitShapeTable();
     for (int k = 0; k < sfk::kSortKinds; ++k)
       for (int cl = 0; cl < sfk::kSizeClasses; ++cl)
-        if (best[k][cl] >= 0) sfk::g_shape_table[device][k][cl].store(best[k][cl]);
+        for (int w = 0; w < W; ++w)
+          if (best[k][cl][w] >= 0) {
+            sfk::g_shape_table[device][k][cl][w].store(best[k][cl][w]);
+            sfk::g_pass_us[device][k][cl][w].store(best_us[k][cl][w]);
+          }
     std::uint32_t written = 0;
     for (const auto& r : results) {
       if (entries != nullptr && written < capacity) {
-        TuneEntryOf(r.kind, r.size_class, r.shape, r.us_per_pass, best[r.kind][r.size_class] == r.shape, true,
-                    &entries[written]);
+        TuneEntryOf(r.kind, r.size_class, r.shape, r.bits, r.us_per_pass,
+                    best[r.kind][r.size_class][sfk::WidthOf(r.bits)] == r.shape, true, &entries[written]);
         ++written;
       }
     }
@@ -2348,11 +2412,17 @@ int sfk_hip_tune_get(int device, sfk_hip_tune_entry* entries, uint32_t capacity,
   std::uint32_t written = 0;
   for (int k = 0; k < sfk::kSortKinds; ++k) {
     for (int cl = 0; cl < sfk::kSizeClasses; ++cl) {
-      const int tuned = sfk::g_shape_table[device][k][cl].load();
-      const std::uint64_t rep = 1ull << (17 + 2 * cl);
-      const int shape = tuned >= 0 ? tuned : sfk::DefaultShape(k, rep);
-      if (entries != nullptr && written < capacity) TuneEntryOf(k, cl, shape, 0.0, true, tuned >= 0, &entries[written]);
-      ++written;
+      for (int w = 0; w < sfk::kDigitWidths; ++w) {
+        if (k >= 4 && w > 0) continue;  // per-reference kinds: 8-bit digits only
+        const int tuned = sfk::g_shape_table[device][k][cl][w].load();
+        const std::uint64_t rep = 1ull << (17 + 2 * cl);
+        const int shape = tuned >= 0 ? tuned : sfk::DefaultShape(k, rep);
+        if (entries != nullptr && written < capacity) {
+          TuneEntryOf(k, cl, shape, sfk::DigitBitsOf(w), sfk::g_pass_us[device][k][cl][w].load(), true, tuned >= 0,
+                      &entries[written]);
+        }
+        ++written;
+      }
     }
   }
   if (count != nullptr) *count = written;
@@ -2365,9 +2435,12 @@ int sfk_hip_tune_set(int device, const sfk_hip_tune_entry* entries, uint32_t cou
     sfk::CopyError("device index out of range or no entries", error, error_capacity);
     return SFK_HIP_INVALID_INPUT;
   }
-  int shapes[sfk::kSortKinds][sfk::kSizeClasses];
+  constexpr int W = sfk::kDigitWidths;
+  int shapes[sfk::kSortKinds][sfk::kSizeClasses][W];
+  double us[sfk::kSortKinds][sfk::kSizeClasses][W];
   for (auto& k : shapes)
-    for (auto& c : k) c = -2;  // untouched
+    for (auto& c : k)
+      for (auto& w : c) w = -2;  // untouched
   for (uint32_t i = 0; i < count; ++i) {
     const sfk_hip_tune_entry& e = entries[i];
     if (e.chosen == 0) continue;
@@ -2377,20 +2450,28 @@ int sfk_hip_tune_set(int device, const sfk_hip_tune_entry* entries, uint32_t cou
         shape = j;
       }
     }
+    const std::uint32_t bits = e.digit_bits == 0 ? 8u : e.digit_bits;  // 0: a table saved before widths
     if (e.kind >= static_cast<uint32_t>(sfk::kSortKinds) || e.size_class >= static_cast<uint32_t>(sfk::kSizeClasses) ||
-        shape < 0 || !sfk::KindShapeFits(static_cast<int>(e.kind), shape)) {
+        shape < 0 || !sfk::KindShapeFits(static_cast<int>(e.kind), shape) || (bits != 8 && bits != 7) ||
+        (bits == 7 && e.kind >= 4)) {
       sfk::CopyError("entry " + std::to_string(i) + ": no compiled radix shape " + std::to_string(e.block) + "x" +
-                         std::to_string(e.items) + " for kind " + std::to_string(e.kind) + ", class " +
-                         std::to_string(e.size_class),
+                         std::to_string(e.items) + " with " + std::to_string(bits) + "-bit digits for kind " +
+                         std::to_string(e.kind) + ", class " + std::to_string(e.size_class),
                      error, error_capacity);
       return SFK_HIP_INVALID_INPUT;
     }
-    shapes[e.kind][e.size_class] = shape;
+    const int w = sfk::WidthOf(static_cast<int>(bits));
+    shapes[e.kind][e.size_class][w] = shape;
+    us[e.kind][e.size_class][w] = e.us_per_pass;
   }
   sfk::InitShapeTable();
   for (int k = 0; k < sfk::kSortKinds; ++k)
     for (int cl = 0; cl < sfk::kSizeClasses; ++cl)
-      if (shapes[k][cl] != -2) sfk::g_shape_table[device][k][cl].store(shapes[k][cl]);
+      for (int w = 0; w < W; ++w)
+        if (shapes[k][cl][w] != -2) {
+          sfk::g_shape_table[device][k][cl][w].store(shapes[k][cl][w]);
+          sfk::g_pass_us[device][k][cl][w].store(us[k][cl][w]);
+        }
   return SFK_HIP_OK;
 }
 
@@ -2398,7 +2479,11 @@ int sfk_hip_tune_reset(int device) {
   if (device < 0 || device >= sfk::kMaxTuneDevices) return SFK_HIP_INVALID_INPUT;
   sfk::InitShapeTable();
   for (auto& k : sfk::g_shape_table[device])
-    for (auto& c : k) c.store(-1);
+    for (auto& c : k)
+      for (auto& w : c) w.store(-1);
+  for (auto& k : sfk::g_pass_us[device])
+    for (auto& c : k)
+      for (auto& w : c) w.store(0.0);
   return SFK_HIP_OK;
 }
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_device_count():
     lib = sufkit_amd.load_hip()
-    assert lib.sfk_hip_abi_version() == 5
+    assert lib.sfk_hip_abi_version() == 6
     assert lib.sfk_hip_device_count() >= 0
 
 
