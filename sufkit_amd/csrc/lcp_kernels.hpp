@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 constexpr int kMarkThreads = 256;
 constexpr int kMarkRows = 8;
 constexpr int kMarkTile = kMarkThreads * kMarkRows;
-// Row digits below this bit stay unsorted before the LCP scatter.
+// At most this many low row bits stay unsorted before the LCP scatter.
 constexpr int kScatterWindowBits = 8;
 
 template <typename T>
@@ -799,8 +799,9 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
   FlushDigits<kWaves>(local, passes, set.hist);
 }
 
-// LCP[rows[k]] = values[k] for the m pairs, in order (grouped by r >> 8, so
-// the stores of one instruction fall in a few lines).
+// LCP[rows[k]] = values[k] for the m pairs, in order (grouped by windows of
+// at most 2^kScatterWindowBits rows, so the stores of one instruction fall in
+// a few lines).
 template <typename Out>
 __global__ __launch_bounds__(256) void LcpScatterKernel(const std::uint32_t* __restrict__ rows,
                                                         const std::uint32_t* __restrict__ values, std::uint64_t m,

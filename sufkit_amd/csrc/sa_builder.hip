@@ -617,7 +617,6 @@ void TuneKind(sfk_hip_builder* b, std::uint64_t max_elements, hipEvent_t e0, hip
 }
 
 // Reset of the per-round list counters (small-list appends, large-list
-// allocation).// Reset of the per-round list counters (small-list appends, large-list
 // allocation).
 __global__ void ResetRoundCountersKernel(DeviceCounters* c) {
   c->q_next = 0;
@@ -1219,7 +1218,10 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   LcpMarkCountKernel<Out><<<g.grid, kMarkThreads, 0, b->stream>>>(lcp, n, g.chunk, aligned(lcp), partial);
   MarkSpineKernel<<<1, 1024, 0, b->stream>>>(partial, g.grid, &dc->lcp_marked);
   const int bits = std::max(1, BitsFor(n - 1));
-  const RadixPasses passes = AllDigits(bits);
+  // The marked count is known only once the write kernel has run; the digit
+  // width of the position sort takes its size class from n (m <= n).
+  const RadixPasses passes =
+      AllDigits(bits, DigitBitsFor(b->device, SortKind<std::uint32_t, std::uint64_t, false>(), n, bits));
   const RadixSet set = TakeRadixSet(b);
   LcpMarkWriteKernel<R, Out><<<g.grid, kMarkThreads, 0, b->stream>>>(
       lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, val, passes, set, dc);
@@ -1252,12 +1254,18 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
     // (pos_alt is now the other one), values in place.
     std::uint32_t* rows = pos == b->vals0.As<std::uint32_t>() ? b->vals1.As<std::uint32_t>()
                                                               : b->vals0.As<std::uint32_t>();
-    // Rows sorted down to windows of 2^kScatterWindowBits rows: the pairs of
-    // one window are contiguous, so the stores of one instruction fall in a
-    // few lines, for one radix pass fewer than a full sort.
+    // Rows sorted down to windows of at most 2^kScatterWindowBits rows: the
+    // pairs of one window are contiguous, so the stores of one instruction
+    // fall in a few lines, for one radix pass fewer than a full sort.  The
+    // passes end at the row's top bit, so the spare bits of their digits
+    // narrow the windows further.
+    const int span = std::max(0, bits - kScatterWindowBits);
     RadixPasses scatter_passes{};
-    for (int shift = kScatterWindowBits; shift < bits; shift += RadixTile::kDigitBits) {
-      scatter_passes.shift[scatter_passes.count++] = shift;
+    scatter_passes.bits = DigitBitsFor(b->device, SortKind<std::uint32_t, std::uint32_t, false>(), m, span);
+    const int scatter_count = (span + scatter_passes.bits - 1) / scatter_passes.bits;
+    for (int p = 0; p < scatter_count; ++p) {
+      scatter_passes.shift[scatter_passes.count++] = std::max(0, bits - scatter_count * scatter_passes.bits) +
+                                                      p * scatter_passes.bits;
     }
     const RadixSet scatter_set = TakeRadixSet(b);
     const SortGeometry gs = Geometry(m, 256 * kScanItems, kSpineChunks);
