@@ -1036,7 +1036,11 @@ class Doubling {
     }
 
     // Boundaries, singletons, compaction: reduce / spine / apply for a long
-    // one-segment list, one look-back kernel otherwise.
+    // one-segment list, one look-back kernel otherwise.  In a doubling round
+    // every suffix's rank is its segment's first row: the step before wrote
+    // the ranks of all survivors (the exact steps before the first round end
+    // with one that does), so unchanged ranks are not stored again.
+    const bool ranks_current = rank_step;
     Index* delta_next = OtherDelta(l_delta_);
     const std::uint64_t tiles = CeilDiv(m, kRefineTile);
     if (seg_bits > 0 || segmented || tiles <= static_cast<std::uint64_t>(kRefineResidentTiles)) {
@@ -1046,7 +1050,7 @@ class Doubling {
       if (b_->tickets.Bytes() < 2 * sizeof(unsigned int)) Fail(SFK_HIP_BUILD_FAILURE, "look-back tickets not reserved");
       RefineApplyKernel<Index, K><<<static_cast<unsigned>(tiles), kRefineThreads, 0, b_->stream>>>(
           keys, vals, m, sec_bits, l_delta_, n_, sa_, rank_, b_->act_idx.As<Index>(),
-          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank,
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, ranks_current,
           b_->radix_status.As<unsigned long long>(), NextEpoch(b_), b_->tickets.As<unsigned int>(), dc,
           segmented ? ctx_.d_offsets : nullptr, segmented ? ctx_.refs : 0u,
           sec_kind == kSecFillHi ? early_ : EarlyLcp{nullptr, nullptr});
@@ -1057,7 +1061,8 @@ class Doubling {
       RefineSpineKernel<<<1, 1024, 0, b_->stream>>>(partial, g.grid, dc);
       RefineApplyChunkedKernel<Index, K><<<g.grid, kRefineChunkThreads, 0, b_->stream>>>(
           keys, vals, m, g.chunk, sec_bits, l_delta_, partial, n_, sa_, rank_, b_->act_idx.As<Index>(),
-          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, nullptr, 0u,
+          b_->act_seg.As<std::uint32_t>(), delta_next, b_->seg_start.As<Index>(), survivor_rank, ranks_current,
+          nullptr, 0u,
           sec_kind == kSecFillHi ? early_ : EarlyLcp{nullptr, nullptr});
     }
     SFK_HIP_CHECK(hipGetLastError());

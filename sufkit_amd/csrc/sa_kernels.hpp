@@ -821,14 +821,29 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
 // Writes of one tile, given the survivors, segments and 1 + last group start
 // of every slot before the tile.
 template <int kBlock, int kItems, typename Index, typename K, typename R>
-__device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std::uint64_t base, int len, int sec_bits,
+__device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], const K* __restrict__ keys,
+                                       std::uint64_t base, int len, int sec_bits,
                                        const Index* __restrict__ seg_delta, std::uint64_t n,
                                        R* __restrict__ sa, R* __restrict__ rank,
                                        Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg,
                                        Index* __restrict__ out_delta, Index* __restrict__ out_start,
-                                       bool survivor_rank, std::uint64_t run_surv, std::uint64_t run_segs,
-                                       std::uint64_t run_last, const RefineTileLds<K, kBlock * kItems>& lds,
-                                       const EarlyLcp early) {
+                                       bool survivor_rank, bool ranks_current, std::uint64_t run_surv,
+                                       std::uint64_t run_segs, std::uint64_t run_last,
+                                       const RefineTileLds<K, kBlock * kItems>& lds, const EarlyLcp early) {
+  // ranks_current: every suffix's rank is already its segment's first row
+  // (the step that formed the segments wrote it, and rows do not move), so
+  // the suffixes of a group that opens its segment keep their rank and their
+  // rank stores are skipped.  A group opens its segment when its first slot
+  // is the list's first or follows a slot of another segment.
+  auto opens_segment = [sec_bits](std::uint64_t slot, K at, K before) {
+    return slot == 0 || (static_cast<std::uint64_t>(at) >> sec_bits) != (static_cast<std::uint64_t>(before) >> sec_bits);
+  };
+  // The group running in from the tiles before (its first slot run_last - 1).
+  bool carried_opens = false;
+  if (ranks_current && run_last > 0) {
+    const std::uint64_t s = run_last - 1;
+    carried_opens = opens_segment(s, keys[s], s > 0 ? keys[s - 1] : K(0));
+  }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int i = j * kBlock + static_cast<int>(threadIdx.x);
@@ -855,13 +870,15 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], std
     }
     if (f & 2) {
       sa[row] = static_cast<R>(pos);
-      rank[pos] = static_cast<R>(row);
+      if (!(ranks_current && opens_segment(k, key, lds.key[i]))) rank[pos] = static_cast<R>(row);
     } else {
       // The group start lies in the same segment: every segment start is a
       // key change, hence a group start.
       const int g = lds.group[i];
       const std::uint64_t group_slot = g >= 0 ? base + static_cast<std::uint64_t>(g) : run_last - 1;
-      if (survivor_rank) rank[pos] = static_cast<R>(delta + static_cast<Index>(group_slot));
+      const bool kept = ranks_current && (g >= 0 ? opens_segment(group_slot, lds.key[g + 1], lds.key[g])
+                                                 : carried_opens);
+      if (survivor_rank && !kept) rank[pos] = static_cast<R>(delta + static_cast<Index>(group_slot));
       const std::uint64_t at = run_surv + lds.pos[i];
       const std::uint64_t sc = run_segs + lds.segc[i];  // segments opened through this slot
       if (f & 1) {
@@ -880,7 +897,8 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
     const K* __restrict__ keys, const Index* __restrict__ vals, std::uint64_t m, int sec_bits,
     const Index* __restrict__ seg_delta, std::uint64_t n, R* __restrict__ sa, R* __restrict__ rank,
     Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta,
-    Index* __restrict__ out_start, bool survivor_rank, unsigned long long* status, std::uint32_t epoch,
+    Index* __restrict__ out_start, bool survivor_rank, bool ranks_current, unsigned long long* status,
+    std::uint32_t epoch,
     unsigned int* tickets, DeviceCounters* counters,
     const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs, const EarlyLcp early) {
   constexpr int kBlock = kRefineThreads;
@@ -912,9 +930,9 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
   __syncthreads();
   const std::uint64_t run_surv = s_prefix[0];
   const std::uint64_t run_segs = s_prefix[1];
-  RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
-                                        out_delta, out_start, survivor_rank, run_surv, run_segs, s_prefix[2],
-                                        lds, early);
+  RefineTileWrite<kBlock, kRefineItems>(v, keys, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
+                                        out_delta, out_start, survivor_rank, ranks_current, run_surv, run_segs,
+                                        s_prefix[2], lds, early);
   if (threadIdx.x == 0 && base + len == m) {
     counters->list.survivors = run_surv + tot.surv;
     counters->list.segments = run_segs + tot.segs;
@@ -1016,7 +1034,7 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
     int sec_bits, const Index* __restrict__ seg_delta, const RefinePartial* __restrict__ partial,
     std::uint64_t n, R* __restrict__ sa, R* __restrict__ rank, Index* __restrict__ out_idx,
     std::uint32_t* __restrict__ out_seg, Index* __restrict__ out_delta, Index* __restrict__ out_start,
-    bool survivor_rank, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs,
+    bool survivor_rank, bool ranks_current, const std::uint64_t* __restrict__ ref_offsets, std::uint32_t refs,
     const EarlyLcp early) {
   constexpr int kBlock = kRefineChunkThreads;
   constexpr int kTile = kRefineChunkTile;
@@ -1033,9 +1051,9 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
     const RefineTotals tot =
         RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, ref_offsets, refs, lds, scratch, v);
     __syncthreads();
-    RefineTileWrite<kBlock, kRefineItems>(v, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
-                                          out_delta, out_start, survivor_rank, run_surv, run_segs, run_last, lds,
-                                          early);
+    RefineTileWrite<kBlock, kRefineItems>(v, keys, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
+                                          out_delta, out_start, survivor_rank, ranks_current, run_surv, run_segs,
+                                          run_last, lds, early);
     run_surv += tot.surv;
     run_segs += tot.segs;
     if (tot.last > 0) run_last = base + tot.last;
@@ -1472,7 +1490,10 @@ __device__ inline ExactKey ExactKeyOf(std::uint64_t codes, std::uint32_t spec) {
 // inside its segment, the keys below its own (its group's first row is row0 +
 // that count) and the keys equal to it (its group's size); the first suffix
 // of each surviving group reserves the group's slots in the next small list
-// (one atomic per block).
+// (one atomic per block).  After the exact step (not kExact) every suffix's
+// rank is already its segment's row0 (the step that formed the segment wrote
+// it), so the suffixes with no smaller key in their segment keep their rank
+// and only the others store one.
 //
 // Keys.  kExact (the small list's first step, on segments of equal 16-symbol
 // FillKey high halves): the low half of FillKey, which orders windows without
@@ -1732,9 +1753,9 @@ __global__ __launch_bounds__(kLocalThreads) void LocalRefineKernel(
       atomicOr(&counters->error_flags, static_cast<unsigned int>(kFlagRange));
     } else if (equal[i] == 1) {
       sa[head[i]] = static_cast<R>(pos);
-      rank[pos] = static_cast<R>(head[i]);
+      if (kExact || less[i] != 0) rank[pos] = static_cast<R>(head[i]);
     } else {
-      rank[pos] = static_cast<R>(head[i]);
+      if (kExact || less[i] != 0) rank[pos] = static_cast<R>(head[i]);
       survivor[i] = true;
       if (tie[i] == 0) leader_total += equal[i];
     }
