@@ -559,11 +559,11 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
     const std::uint64_t* __restrict__ partial, std::uint32_t* __restrict__ out_pos,
     std::uint64_t* __restrict__ out_val, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
   constexpr int kWaves = kMarkThreads / kWave;
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   __shared__ std::uint32_t scratch32[kWaves];
   __shared__ std::uint64_t scratch[kWaves];
   ClearNextRadixSet(set);
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kMarkThreads) {
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * RadixTile::kRadix; i += kMarkThreads) {
     (&local[0][0][0])[i] = 0;
   }
   __syncthreads();
@@ -619,12 +619,12 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
         out_val[slot] = (phi << 32) | (r0 + i);
         ++slot;
       }
-      CountDigits<std::uint32_t>(local[wave], passes, pos, m, false);
+      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, pos, m, false);
     }
     at += total;
   }
   __syncthreads();
-  FlushDigits<kWaves>(local, passes, set.hist);
+  FlushDigits<kDigitCopies>(local, passes, set.hist);
   std::uint64_t sum_a, sum_b;
   BlockExclusiveSum<kMarkThreads>(hash_a, scratch, sum_a);
   BlockExclusiveSum<kMarkThreads>(hash_b, scratch, sum_b);
@@ -741,9 +741,9 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
   constexpr int kTile = kBlock * kItems;
   constexpr int kWaves = kBlock / kWave;
   __shared__ std::uint64_t scratch[kWaves];
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   ClearNextRadixSet(set);
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kBlock) {
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * RadixTile::kRadix; i += kBlock) {
     (&local[0][0][0])[i] = 0;
   }
   __syncthreads();
@@ -791,12 +791,12 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-      CountDigits<std::uint32_t>(local[wave], passes, static_cast<std::uint32_t>(rv[i]), first + i < end, false);
+      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, static_cast<std::uint32_t>(rv[i]), first + i < end, false);
     }
     if (total > run) run = total;
   }
   __syncthreads();
-  FlushDigits<kWaves>(local, passes, set.hist);
+  FlushDigits<kDigitCopies>(local, passes, set.hist);
 }
 
 // LCP[rows[k]] = values[k] for the m pairs, in order (grouped by windows of

@@ -353,9 +353,16 @@ __device__ inline void CountDigits(std::uint32_t (*local)[RadixTile::kRadix], co
   }
 }
 
-// Sums the block's wave counts into the sort's digit counts (call after a
-// __syncthreads that follows the last CountDigits).
-template <int kWaves>
+// Copies of the digit counters a counting block keeps in LDS.  LDS atomics
+// execute one wave instruction at a time, so waves sharing one copy collide
+// no more than one wave alone, and a single copy (8 KiB at eight passes)
+// leaves the LDS to occupancy: per-wave copies held a 512-thread counting
+// block to two blocks per CU.
+constexpr int kDigitCopies = 1;
+
+// Sums the block's copies of the counts into the sort's digit counts (call
+// after a __syncthreads that follows the last CountDigits).
+template <int kCopies>
 __device__ inline void FlushDigits(std::uint32_t (*local)[RadixPasses::kMax][RadixTile::kRadix],
                                    const RadixPasses& passes, unsigned long long* hist) {
   constexpr int kRadix = RadixTile::kRadix;
@@ -363,7 +370,7 @@ __device__ inline void FlushDigits(std::uint32_t (*local)[RadixPasses::kMax][Rad
     const int p = i / kRadix, d = i % kRadix;
     std::uint32_t sum = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) sum += local[w][p][d];
+    for (int w = 0; w < kCopies; ++w) sum += local[w][p][d];
     if (sum) atomicAdd(&hist[p * kRadix + d], static_cast<unsigned long long>(sum));
   }
 }

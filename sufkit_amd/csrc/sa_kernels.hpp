@@ -274,9 +274,9 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
     K* __restrict__ keys, Index* vals, RadixPasses passes, RadixSet set, DeviceCounters* counters,
     bool lo_from_value) {
   constexpr int kWaves = kKeygenBlock / kWave;
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   ClearNextRadixSet(set);
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
     (&local[0][0][0])[i] = 0;
   }
   __syncthreads();
@@ -342,11 +342,11 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
         keys[k] = key;
         vals[k] = static_cast<Index>(PosOf(v[u]));
       }
-      CountDigits<K>(local[wave], passes, key, valid, kbase + kWave <= m);
+      CountDigits<K>(local[wave % kDigitCopies], passes, key, valid, kbase + kWave <= m);
     }
   }
   __syncthreads();
-  FlushDigits<kWaves>(local, passes, set.hist);
+  FlushDigits<kDigitCopies>(local, passes, set.hist);
 }
 
 // Digit counts of the first step's keys (high half of FillKey of every
@@ -355,9 +355,9 @@ __global__ __launch_bounds__(kKeygenBlock) void KeygenKernel(
 __global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText t, RadixPasses passes,
                                                                       RadixSet set) {
   constexpr int kWaves = kKeygenBlock / kWave;
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   ClearNextRadixSet(set);
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
     (&local[0][0][0])[i] = 0;
   }
   __syncthreads();
@@ -381,11 +381,11 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText
       std::uint32_t sp;
       WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
       const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
-      CountDigits<std::uint32_t>(local[wave], passes, key, p < m, full);
+      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, p < m, full);
     }
   }
   __syncthreads();
-  FlushDigits<kWaves>(local, passes, set.hist);
+  FlushDigits<kDigitCopies>(local, passes, set.hist);
 }
 
 // Digit counts of the first step's keys per reference of a batch:
@@ -402,8 +402,8 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedT
                                                                          unsigned int* __restrict__ counts) {
   constexpr int kWaves = kKeygenBlock / kWave;
   constexpr int kRadix = RadixTile::kRadix;
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][kRadix];
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * kRadix; i += kKeygenBlock) (&local[0][0][0])[i] = 0;
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][kRadix];
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * kRadix; i += kKeygenBlock) (&local[0][0][0])[i] = 0;
   const std::uint64_t c0 = static_cast<std::uint64_t>(blockIdx.x) * chunk;
   const std::uint64_t c1 = c0 + chunk < t.n ? c0 + chunk : t.n;
   if (c0 >= c1) return;
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedT
         std::uint32_t sp;
         WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
         const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
-        CountDigits<std::uint32_t>(local[wave], passes, key, g < g1 && p >= lo && p < hi, full);
+        CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, g < g1 && p >= lo && p < hi, full);
       }
     }
     __syncthreads();
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedT
       const int p = i / kRadix, d = i % kRadix;
       std::uint32_t sum = 0;
 #pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
+      for (int w = 0; w < kDigitCopies; ++w) {
         sum += local[w][p][d];
         local[w][p][d] = 0;
       }
@@ -504,9 +504,9 @@ __global__ __launch_bounds__(kKeygenBlock) void TuneKeysKernel(K* __restrict__ k
                                                                std::uint64_t m, std::uint64_t seed,
                                                                RadixPasses passes, RadixSet set) {
   constexpr int kWaves = kKeygenBlock / kWave;
-  __shared__ std::uint32_t local[kWaves][RadixPasses::kMax][RadixTile::kRadix];
+  __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   ClearNextRadixSet(set);
-  for (int i = threadIdx.x; i < kWaves * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
+  for (int i = threadIdx.x; i < kDigitCopies * RadixPasses::kMax * RadixTile::kRadix; i += kKeygenBlock) {
     (&local[0][0][0])[i] = 0;
   }
   __syncthreads();
@@ -521,10 +521,10 @@ __global__ __launch_bounds__(kKeygenBlock) void TuneKeysKernel(K* __restrict__ k
       keys[i] = key;
       vals[i] = static_cast<V>(i);
     }
-    CountDigits<K>(local[wave], passes, key, valid, wbase + kWave <= m);
+    CountDigits<K>(local[wave % kDigitCopies], passes, key, valid, wbase + kWave <= m);
   }
   __syncthreads();
-  FlushDigits<kWaves>(local, passes, set.hist);
+  FlushDigits<kDigitCopies>(local, passes, set.hist);
 }
 
 // Placement tables of a one-segment list (bucket starts of each pass), for
