@@ -730,6 +730,34 @@ struct RefineTotals {
   std::uint64_t surv, segs, last;
 };
 
+// One tile's keys and values in registers (item j of thread t is slot
+// j * kBlock + t) and, in thread 0, the keys just before and after the tile:
+// loaded ahead of the tile's scan when a block walks tiles in turn.
+template <typename K, typename Index, int kItems>
+struct RefineTileRegs {
+  K key[kItems];
+  Index v[kItems];
+  K before, after;
+};
+
+template <int kBlock, int kItems, typename Index, typename K>
+__device__ __forceinline__ void LoadRefineTile(const K* __restrict__ keys, const Index* __restrict__ vals,
+                                               std::uint64_t m, std::uint64_t base, int len,
+                                               RefineTileRegs<K, Index, kItems>& r) {
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int i = j * kBlock + static_cast<int>(threadIdx.x);
+    if (i < len) {
+      r.key[j] = keys[base + i];
+      r.v[j] = vals[base + i];
+    }
+  }
+  if (threadIdx.x == 0) {
+    r.before = base > 0 ? keys[base - 1] : K(0);
+    r.after = base + len < m ? keys[base + len] : K(0);
+  }
+}
+
 // Stages tile [base, base + len): keys into LDS, values into registers
 // (v[j] = slot j * kBlock + thread, the striped order of the writes), and
 // fills the per-slot arrays.  ref_offsets (or null): the slots are a batch's
@@ -737,23 +765,23 @@ struct RefineTotals {
 // a group start whatever the keys.  The caller puts a barrier between this
 // and RefineTileWrite.
 template <int kBlock, int kItems, typename Index, typename K>
-__device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ keys, const Index* __restrict__ vals,
-                                                       std::uint64_t m, std::uint64_t base, int len,
-                                                       const std::uint64_t* __restrict__ ref_offsets,
-                                                       std::uint32_t refs, RefineTileLds<K, kBlock * kItems>& lds,
-                                                       std::uint64_t* scratch, Index (&v)[kItems]) {
+__device__ __forceinline__ RefineTotals RefineTileScanRegs(const RefineTileRegs<K, Index, kItems>& r,
+                                                           std::uint64_t m, std::uint64_t base, int len,
+                                                           const std::uint64_t* __restrict__ ref_offsets,
+                                                           std::uint32_t refs, RefineTileLds<K, kBlock * kItems>& lds,
+                                                           std::uint64_t* scratch, Index (&v)[kItems]) {
   constexpr int kTile = kBlock * kItems;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int i = j * kBlock + static_cast<int>(threadIdx.x);
     if (i < len) {
-      lds.key[i + 1] = keys[base + i];
-      v[j] = vals[base + i];
+      lds.key[i + 1] = r.key[j];
+      v[j] = r.v[j];
     }
   }
   if (threadIdx.x == 0) {
-    lds.key[0] = base > 0 ? keys[base - 1] : K(0);
-    lds.key[len + 1] = base + len < m ? keys[base + len] : K(0);
+    lds.key[0] = r.before;
+    lds.key[len + 1] = r.after;
   }
   if (ref_offsets) {
     for (int i = threadIdx.x; i < kTile / 32 + 1; i += kBlock) lds.bound[i] = 0;
@@ -818,12 +846,25 @@ __device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ key
   return tot;
 }
 
+template <int kBlock, int kItems, typename Index, typename K>
+__device__ __forceinline__ RefineTotals RefineTileScan(const K* __restrict__ keys, const Index* __restrict__ vals,
+                                                       std::uint64_t m, std::uint64_t base, int len,
+                                                       const std::uint64_t* __restrict__ ref_offsets,
+                                                       std::uint32_t refs, RefineTileLds<K, kBlock * kItems>& lds,
+                                                       std::uint64_t* scratch, Index (&v)[kItems]) {
+  RefineTileRegs<K, Index, kItems> r;
+  LoadRefineTile<kBlock, kItems>(keys, vals, m, base, len, r);
+  return RefineTileScanRegs<kBlock, kItems>(r, m, base, len, ref_offsets, refs, lds, scratch, v);
+}
+
 // Writes of one tile, given the survivors, segments and 1 + last group start
-// of every slot before the tile.
-template <int kBlock, int kItems, typename Index, typename K, typename R>
+// of every slot before the tile.  kOneSegment: the list is one segment whose
+// delta is one_delta (no per-item delta or key loads, so that a block's loads
+// of its next tile stay the only ones in flight).
+template <int kBlock, int kItems, bool kOneSegment, typename Index, typename K, typename R>
 __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], const K* __restrict__ keys,
                                        std::uint64_t base, int len, int sec_bits,
-                                       const Index* __restrict__ seg_delta, std::uint64_t n,
+                                       const Index* __restrict__ seg_delta, Index one_delta, std::uint64_t n,
                                        R* __restrict__ sa, R* __restrict__ rank,
                                        Index* __restrict__ out_idx, std::uint32_t* __restrict__ out_seg,
                                        Index* __restrict__ out_delta, Index* __restrict__ out_start,
@@ -842,7 +883,11 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], con
   bool carried_opens = false;
   if (ranks_current && run_last > 0) {
     const std::uint64_t s = run_last - 1;
-    carried_opens = opens_segment(s, keys[s], s > 0 ? keys[s - 1] : K(0));
+    if constexpr (kOneSegment) {
+      carried_opens = s == 0;
+    } else {
+      carried_opens = opens_segment(s, keys[s], s > 0 ? keys[s - 1] : K(0));
+    }
   }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -853,7 +898,7 @@ __device__ __forceinline__ void RefineTileWrite(const Index (&vals)[kItems], con
     const Index v = vals[j];
     const std::uint8_t f = lds.flags[i];
     const std::uint64_t seg = static_cast<std::uint64_t>(key) >> sec_bits;
-    const Index delta = seg_delta[seg];
+    const Index delta = kOneSegment ? one_delta : seg_delta[seg];
     const Index row = static_cast<Index>(delta + static_cast<Index>(k));
     const Index pos = static_cast<Index>(PosOf(v));  // in the list's own width
     if (static_cast<std::uint64_t>(row) >= n || static_cast<std::uint64_t>(pos) >= n) continue;  // corrupted state only
@@ -930,9 +975,9 @@ __global__ __launch_bounds__(kRefineThreads) void RefineApplyKernel(
   __syncthreads();
   const std::uint64_t run_surv = s_prefix[0];
   const std::uint64_t run_segs = s_prefix[1];
-  RefineTileWrite<kBlock, kRefineItems>(v, keys, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
-                                        out_delta, out_start, survivor_rank, ranks_current, run_surv, run_segs,
-                                        s_prefix[2], lds, early);
+  RefineTileWrite<kBlock, kRefineItems, false>(v, keys, base, len, sec_bits, seg_delta, Index(0), n, sa, rank,
+                                               out_idx, out_seg, out_delta, out_start, survivor_rank, ranks_current,
+                                               run_surv, run_segs, s_prefix[2], lds, early);
   if (threadIdx.x == 0 && base + len == m) {
     counters->list.survivors = run_surv + tot.surv;
     counters->list.segments = run_segs + tot.segs;
@@ -1045,15 +1090,27 @@ __global__ __launch_bounds__(kRefineChunkThreads) void RefineApplyChunkedKernel(
   std::uint64_t run_surv = partial[blockIdx.x].survivors;
   std::uint64_t run_segs = partial[blockIdx.x].newsegs;
   std::uint64_t run_last = partial[blockIdx.x].last_boundary;
+  // A one-segment list: one delta for every row.  The next tile's keys and
+  // values are loaded while this one is scanned and written.
+  const Index one_delta = seg_delta[0];
+  auto tile_len = [end](std::uint64_t b) {
+    return static_cast<int>(end - b < static_cast<std::uint64_t>(kTile) ? end - b : kTile);
+  };
+  RefineTileRegs<K, Index, kRefineItems> next;
+  if (begin < end) LoadRefineTile<kBlock, kRefineItems>(keys, vals, m, begin, tile_len(begin), next);
   for (std::uint64_t base = begin; base < end; base += kTile) {
-    const int len = static_cast<int>(end - base < static_cast<std::uint64_t>(kTile) ? end - base : kTile);
+    const int len = tile_len(base);
+    const RefineTileRegs<K, Index, kRefineItems> cur = next;
+    if (base + kTile < end) {
+      LoadRefineTile<kBlock, kRefineItems>(keys, vals, m, base + kTile, tile_len(base + kTile), next);
+    }
     Index v[kRefineItems];
     const RefineTotals tot =
-        RefineTileScan<kBlock, kRefineItems>(keys, vals, m, base, len, ref_offsets, refs, lds, scratch, v);
+        RefineTileScanRegs<kBlock, kRefineItems>(cur, m, base, len, ref_offsets, refs, lds, scratch, v);
     __syncthreads();
-    RefineTileWrite<kBlock, kRefineItems>(v, keys, base, len, sec_bits, seg_delta, n, sa, rank, out_idx, out_seg,
-                                          out_delta, out_start, survivor_rank, ranks_current, run_surv, run_segs,
-                                          run_last, lds, early);
+    RefineTileWrite<kBlock, kRefineItems, true>(v, keys, base, len, sec_bits, seg_delta, one_delta, n, sa, rank,
+                                                out_idx, out_seg, out_delta, out_start, survivor_rank, ranks_current,
+                                                run_surv, run_segs, run_last, lds, early);
     run_surv += tot.surv;
     run_segs += tot.segs;
     if (tot.last > 0) run_last = base + tot.last;
