@@ -549,15 +549,16 @@ __global__ __launch_bounds__(1024) void MarkSpineKernel(std::uint64_t* partial, 
 }
 
 // The list of marked rows at the chunk's offset, in row order: position
-// SA[r] (out_pos) and Φ = SA[r-1] above r (out_val), with the position
-// digits counted for the sort, the inverse check of SA and ISA (rows and
-// positions share the index), and LCP[0] = 0 when row 0 is marked.
+// SA[r] (out_pos) and Φ = SA[r-1] (out_phi), with the position digits
+// counted for the sort, the inverse check of SA and ISA (rows and positions
+// share the index), and LCP[0] = 0 when row 0 is marked.  The row itself is
+// not carried: after the sort by position it is ISA[position].
 template <typename R, typename Out>
 __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
     Out* __restrict__ lcp, const R* __restrict__ sa, const R* __restrict__ isa, std::uint64_t n,
     std::uint64_t chunk, bool lcp_aligned, bool sa_aligned, bool isa_aligned,
     const std::uint64_t* __restrict__ partial, std::uint32_t* __restrict__ out_pos,
-    std::uint64_t* __restrict__ out_val, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
+    std::uint32_t* __restrict__ out_phi, RadixPasses passes, RadixSet set, DeviceCounters* counters) {
   constexpr int kWaves = kMarkThreads / kWave;
   __shared__ std::uint32_t local[kDigitCopies][RadixPasses::kMax][RadixTile::kRadix];
   __shared__ std::uint32_t scratch32[kWaves];
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
       if (m) {
         const std::uint64_t phi = i == 0 ? before : static_cast<std::uint64_t>(s[i - 1]);
         out_pos[slot] = pos;
-        out_val[slot] = (phi << 32) | (r0 + i);
+        out_phi[slot] = static_cast<std::uint32_t>(phi);
         ++slot;
       }
       CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, pos, m, false);
@@ -640,7 +641,7 @@ __global__ __launch_bounds__(kMarkThreads) void LcpMarkWriteKernel(
 // and each lane takes its predecessor by a lane shuffle.  LCEs longer than
 // word_limit windows go to the long queue (dst: the entry).
 __global__ __launch_bounds__(kPlcpThreads) void SparsePlcpKernel(
-    PackedText t, const std::uint32_t* __restrict__ pos, const std::uint64_t* __restrict__ val, std::uint64_t m,
+    PackedText t, const std::uint32_t* __restrict__ pos, const std::uint32_t* __restrict__ phi, std::uint64_t m,
     const std::uint32_t* __restrict__ starts, std::uint32_t word_limit, std::uint32_t* __restrict__ out,
     LongLce* __restrict__ queue, std::uint64_t queue_capacity, DeviceCounters* counters) {
   constexpr std::uint64_t kWavesPerBlock = kPlcpThreads / kWave;
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(kPlcpThreads) void SparsePlcpKernel(
     std::uint32_t p = 0xFFFFFFFFu, f = 0xFFFFFFFFu;
     if (own || (lane == 0 && first > 0)) {
       p = pos[k];
-      f = static_cast<std::uint32_t>(val[k] >> 32);
+      f = phi[k];
     }
     const std::uint32_t pb = WaveShfl(p, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
     const std::uint32_t fb = WaveShfl(f, static_cast<int>((lane + kWave - 1) & (kWave - 1)));
@@ -726,12 +727,14 @@ __global__ __launch_bounds__(256) void SparseScanReduceKernel(const std::uint32_
   BlockAddTo<kBlock>(irreducible, scratch, &counters->irreducible);
 }
 
-// Inclusive max-scan of the list's keys: entry k's LCP row r_k (from val)
-// goes to rows[k] and its value, scan - p_k - 1, back to plcp[k]; the row
-// digits of the scatter's passes are counted.
+// Inclusive max-scan of the list's keys: entry k's LCP row r_k = ISA[p_k]
+// (the positions ascend, so these reads sweep the ISA once) goes to rows[k]
+// and its value, scan - p_k - 1, back to plcp[k]; the row digits of the
+// scatter's passes are counted.
+template <typename R>
 __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __restrict__ plcp,
                                                              const std::uint32_t* __restrict__ pos,
-                                                             const std::uint64_t* __restrict__ val, std::uint64_t m,
+                                                             const R* __restrict__ isa, std::uint64_t m,
                                                              std::uint64_t chunk,
                                                              const std::uint64_t* __restrict__ partial,
                                                              std::uint32_t* __restrict__ rows, RadixPasses passes,
@@ -753,11 +756,11 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
   std::uint64_t run = partial[blockIdx.x];
   for (std::uint64_t base = begin; base < end; base += kTile) {
     const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kItems;
-    std::uint32_t v[kItems], p[kItems];
-    std::uint64_t rv[kItems];
+    std::uint32_t v[kItems], p[kItems], rv[kItems];
     LoadRun8(plcp, first, end, true, kPlcpReducible, v);
     LoadRun8(pos, first, end, true, 0u, p);
-    LoadRun8(val, first, end, true, std::uint64_t{0}, rv);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) rv[i] = first + i < end ? static_cast<std::uint32_t>(isa[p[i]]) : 0u;
     std::uint64_t keys[kItems];
     std::uint64_t local_max = 0;
 #pragma unroll
@@ -776,22 +779,20 @@ __global__ __launch_bounds__(256) void SparseScanApplyKernel(std::uint32_t* __re
     if (first + kItems <= end) {
       *reinterpret_cast<uint4*>(plcp + first) = make_uint4(v[0], v[1], v[2], v[3]);
       *reinterpret_cast<uint4*>(plcp + first + 4) = make_uint4(v[4], v[5], v[6], v[7]);
-      *reinterpret_cast<uint4*>(rows + first) = make_uint4(static_cast<std::uint32_t>(rv[0]),
-          static_cast<std::uint32_t>(rv[1]), static_cast<std::uint32_t>(rv[2]), static_cast<std::uint32_t>(rv[3]));
-      *reinterpret_cast<uint4*>(rows + first + 4) = make_uint4(static_cast<std::uint32_t>(rv[4]),
-          static_cast<std::uint32_t>(rv[5]), static_cast<std::uint32_t>(rv[6]), static_cast<std::uint32_t>(rv[7]));
+      *reinterpret_cast<uint4*>(rows + first) = make_uint4(rv[0], rv[1], rv[2], rv[3]);
+      *reinterpret_cast<uint4*>(rows + first + 4) = make_uint4(rv[4], rv[5], rv[6], rv[7]);
     } else {
 #pragma unroll
       for (int i = 0; i < kItems; ++i) {
         if (first + i < end) {
           plcp[first + i] = v[i];
-          rows[first + i] = static_cast<std::uint32_t>(rv[i]);
+          rows[first + i] = rv[i];
         }
       }
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, static_cast<std::uint32_t>(rv[i]), first + i < end, false);
+      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, rv[i], first + i < end, false);
     }
     if (total > run) run = total;
   }

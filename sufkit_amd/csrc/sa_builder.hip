@@ -1202,8 +1202,8 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   }
   std::uint32_t* pos = b->vals0.As<std::uint32_t>();
   std::uint32_t* pos_alt = b->vals1.As<std::uint32_t>();
-  std::uint64_t* val = b->keys0.As<std::uint64_t>();
-  std::uint64_t* val_alt = b->keys1.As<std::uint64_t>();
+  std::uint32_t* phi = b->keys0.As<std::uint32_t>();
+  std::uint32_t* phi_alt = b->keys1.As<std::uint32_t>();
   std::uint32_t* list_plcp = b->act_seg.As<std::uint32_t>();
   std::uint32_t* starts = b->act_idx.As<std::uint32_t>();
   const std::uint64_t bit_words = n / 32 + 1;
@@ -1226,10 +1226,10 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   // The marked count is known only once the write kernel has run; the digit
   // width of the position sort takes its size class from n (m <= n).
   const RadixPasses passes =
-      AllDigits(bits, DigitBitsFor(b->device, SortKind<std::uint32_t, std::uint64_t, false>(), n, bits));
+      AllDigits(bits, DigitBitsFor(b->device, SortKind<std::uint32_t, std::uint32_t, false>(), n, bits));
   const RadixSet set = TakeRadixSet(b);
   LcpMarkWriteKernel<R, Out><<<g.grid, kMarkThreads, 0, b->stream>>>(
-      lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, val, passes, set, dc);
+      lcp, sa, isa, n, g.chunk, aligned(lcp), aligned(sa), aligned(isa), partial, pos, phi, passes, set, dc);
   SFK_HIP_CHECK(hipGetLastError());
   constexpr std::uint64_t r = sizeof(R);
   std::uint64_t bytes = n / 8 + n * sizeof(Out) + n * (sizeof(Out) + 2 * r);  // bitmap; two LCP reads, SA and ISA
@@ -1239,13 +1239,13 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
   }
   const std::uint64_t m = c.lcp_marked;
   if (m > 0) {
-    bytes += m * 12;  // the list written
+    bytes += m * 8;  // the list written
     const std::uint64_t sort_before = ctx.stats.algorithmic_bytes_sa;
-    SortPairs<std::uint32_t, std::uint64_t>(ctx, ArraySource<std::uint32_t, std::uint64_t>{pos, val}, m * 12, pos,
-                                            val, pos_alt, val_alt, m, passes, set);
+    SortPairs<std::uint32_t, std::uint32_t>(ctx, ArraySource<std::uint32_t, std::uint32_t>{pos, phi}, m * 8, pos,
+                                            phi, pos_alt, phi_alt, m, passes, set);
     const std::uint64_t tiles = CeilDiv(m, kPlcpTile);
     SparsePlcpKernel<<<FlatGrid(tiles, kPlcpThreads / kWave), kPlcpThreads, 0, b->stream>>>(
-        text, pos, val, m, starts, kWordLimit, list_plcp, queue, capacity, dc);
+        text, pos, phi, m, starts, kWordLimit, list_plcp, queue, capacity, dc);
     SFK_HIP_CHECK(hipGetLastError());
     const DeviceCounters& c2 = ReadCounters(b);
     const std::uint64_t items = c2.long_items;
@@ -1276,22 +1276,22 @@ void LaunchLcpMarked(BuildContext& ctx, const std::uint8_t* d_text, PackedText t
     const SortGeometry gs = Geometry(m, 256 * kScanItems, kSpineChunks);
     SparseScanReduceKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, m, gs.chunk, partial, dc);
     PlcpScanSpineKernel<<<1, 1024, 0, b->stream>>>(partial, gs.grid);
-    SparseScanApplyKernel<<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, val, m, gs.chunk, partial, rows,
-                                                          scatter_passes, scatter_set);
+    SparseScanApplyKernel<R><<<gs.grid, 256, 0, b->stream>>>(list_plcp, pos, isa, m, gs.chunk, partial, rows,
+                                                             scatter_passes, scatter_set);
     SFK_HIP_CHECK(hipGetLastError());
     std::uint32_t* values = list_plcp;
     std::uint32_t* rows_alt = pos;  // free once the scan has read it
-    std::uint32_t* values_alt = reinterpret_cast<std::uint32_t*>(val == b->keys0.As<std::uint64_t>()
-                                                                     ? b->keys1.As<std::uint64_t>()
-                                                                     : b->keys0.As<std::uint64_t>());
+    std::uint32_t* values_alt = phi == b->keys0.As<std::uint32_t>() ? b->keys1.As<std::uint32_t>()
+                                                                 : b->keys0.As<std::uint32_t>();
     SortPairs<std::uint32_t, std::uint32_t>(ctx, ArraySource<std::uint32_t, std::uint32_t>{rows, values}, m * 8,
                                             rows, values, rows_alt, values_alt, m, scatter_passes, scatter_set);
     bytes += ctx.stats.algorithmic_bytes_sa - sort_before;  // both sorts belong to the LCP phase
     ctx.stats.algorithmic_bytes_sa = sort_before;
     LcpScatterKernel<Out><<<FlatGrid(m, 256 * 4), 256, 0, b->stream>>>(rows, values, m, n, lcp, dc);
     SFK_HIP_CHECK(hipGetLastError());
-    bytes += m * (12 + 4 + 1)         // PLCP pass: the entry read, PLCP written, text window at the position
-             + m * (8 + 16 + 8)       // scan: PLCP and position read twice, the row read, (row, value) written
+    bytes += m * (8 + 4 + 1)          // PLCP pass: the entry read, PLCP written, text window at the position
+             + m * (8 + 8 + sizeof(R) + 8)  // scan: PLCP and position read twice, the row ISA[position] read,
+                                            // (row, value) written
              + m * (8 + sizeof(Out));  // scatter: the pair read, LCP written
   }
   ctx.stats.algorithmic_bytes_lcp += bytes;
