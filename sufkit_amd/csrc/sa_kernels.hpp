@@ -546,7 +546,7 @@ __global__ __launch_bounds__(RadixTile::kRadix) void TuneAdjKernel(const unsigne
 // position (PosOf), for the steps that order the first step's groups by it.
 template <typename Index>
 struct TextFillHiSource {
-  static constexpr int kFetchGroup = 4;  // two 12-byte entry loads per item
+  static constexpr int kFetchGroup = 8;  // two 12-byte entry loads per item
   PackedText t;
   bool pack_lo = false;
   struct Raw {
