@@ -368,7 +368,6 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText
   for (std::uint64_t wbase = (static_cast<std::uint64_t>(blockIdx.x) * kWaves + wave) * kWave; wbase < groups;
        wbase += stride) {
     const std::uint64_t g = wbase + LaneId();
-    const bool full = (wbase + kWave) * 4 <= m;
     uint4 e0 = make_uint4(0, 0, 0, 0), e1 = e0;
     if (g < groups) {
       e0 = t.words[g >> 3];
@@ -381,7 +380,9 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiCountKernel(PackedText
       std::uint32_t sp;
       WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
       const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
-      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, p < m, full);
+      // No test for a digit the whole wave shares: text windows rarely do,
+      // and the test cost more than the colliding atomics it saves.
+      CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, p < m, false);
     }
   }
   __syncthreads();
