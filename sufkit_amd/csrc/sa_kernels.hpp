@@ -424,7 +424,6 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedT
         e0 = t.words[g >> 3];
         e1 = t.words[(g >> 3) + 1];
       }
-      const bool full = wbase + kWave <= g1 && wbase * 4 >= lo && (wbase + kWave) * 4 <= hi;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const std::uint64_t p = g * 4 + j;
@@ -432,7 +431,8 @@ __global__ __launch_bounds__(kKeygenBlock) void TextFillHiRefCountKernel(PackedT
         std::uint32_t sp;
         WindowOf(e0, e1, static_cast<std::uint32_t>(p & 31), c, sp);
         const std::uint32_t key = static_cast<std::uint32_t>(FillKey(c, sp) >> 32);
-        CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, g < g1 && p >= lo && p < hi, full);
+        // No shared-digit test, as in TextFillHiCountKernel.
+        CountDigits<std::uint32_t>(local[wave % kDigitCopies], passes, key, g < g1 && p >= lo && p < hi, false);
       }
     }
     __syncthreads();
