@@ -417,9 +417,10 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 // marked rows need Φ/PLCP, so instead of a PLCP pass over every text
 // position (one random SA line per position) and a gather over every row:
 //   LcpMarkCount / LcpMarkWrite   rows in order: the marked rows r >= 1 go to
-//                  a list of (position SA[r], Φ = SA[r-1] and r), whose
-//                  position digits are counted for the radix sort.  The same
-//                  pass checks that ISA inverts SA (below) and writes LCP[0].
+//                  a list of (position SA[r], Φ = SA[r-1]), whose position
+//                  digits are counted for the radix sort (the row is read
+//                  back as ISA[position] after the sort).  The same pass
+//                  checks that ISA inverts SA (below) and writes LCP[0].
 //   (radix sort of the list by position)
 //   SparsePlcp     the list in text order: Kasai's reducibility test between
 //                  neighbouring entries.  Entry k (position p) is reducible
@@ -432,11 +433,13 @@ __global__ __launch_bounds__(256) void LcpGatherKernel(const std::uint32_t* __re
 //   SparseScan     PLCP[p] + p never decreases along the text, and the
 //                  entries form a subsequence of it, so the inclusive max of
 //                  PLCP + p + 1 over the irreducible entries up to k is
-//                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply leaves
-//                  (r, LCP[r]) pairs and counts the high digits of r.
-//   (radix passes over the digits of r from bit 8 up)
-//   LcpScatter     LCP[r] = value, pairs grouped by r >> 8: the pairs of a
-//                  window of 256 rows are contiguous, so the stores of one
+//                  PLCP[p_k] + p_k + 1 (as in PlcpScan).  The apply reads
+//                  r = ISA[p_k] (the positions ascend: one sweep over the
+//                  ISA), leaves (r, LCP[r]) pairs and counts the high digits
+//                  of r.
+//   (radix passes over the high digits of r, ending at its top bit)
+//   LcpScatter     LCP[r] = value, pairs grouped by windows of at most 256
+//                  rows: the pairs of a window are contiguous, so the stores of one
 //                  instruction fall in a few lines instead of one partial
 //                  line each at random (pairs grouped only by r >> 16 left
 //                  every store its own 32-byte write request).
