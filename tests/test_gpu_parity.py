@@ -68,6 +68,17 @@ def test_only_runs(hip):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+def test_long_run_one_segment_rounds(hip, bits):
+    # The run's suffixes stay one group for many doubling rounds, so those
+    # rounds sort and apply a one-segment list of more than 512 tiles: the
+    # reduce / spine / apply schedule, with groups that run in from earlier
+    # tiles and ranks that the round does not change.
+    text = synth.run_text(7, [("rand", 300000), ("N", 2_600_000), ("rand", 200000)])
+    stats = check_against_oracle(hip, text, bits)
+    assert stats["doubling_rounds"] >= 15
+
+
+@pytest.mark.parametrize("bits", [32, 64])
 def test_repeats(hip, bits):
     text = synth.repeat_text(5, 200_000, cover=0.6, divergence=(0.0, 0.02), n_runs=10)
     check_against_oracle(hip, text, construction_bits=bits)
