@@ -1239,12 +1239,20 @@ __global__ __launch_bounds__(256) void SplitAssignKernel(
   std::uint64_t run_q = part.q_base, run_lsegs = part.large_segs, run_lelems = part.large_elems;
   for (std::uint64_t base = begin; base < end; base += kSplitTile) {
     const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kSplitItems;
+    // The first slots of the thread's segments and of the one after them
+    // (m past the last segment), loaded once for both loops.
+    std::uint64_t st[kSplitItems + 1];
+#pragma unroll
+    for (int i = 0; i <= kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      st[i] = s < segs ? static_cast<std::uint64_t>(seg_start[s]) : m;
+    }
     std::uint64_t q_elems = 0, l_segs = 0, l_elems = 0;
 #pragma unroll
     for (int i = 0; i < kSplitItems; ++i) {
       const std::uint64_t s = first + i;
       if (s < end) {
-        const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+        const std::uint64_t size = st[i + 1] - st[i];
         if (fits && size <= small_max) {
           q_elems += size;
         } else {
@@ -1261,8 +1269,8 @@ __global__ __launch_bounds__(256) void SplitAssignKernel(
     for (int i = 0; i < kSplitItems; ++i) {
       const std::uint64_t s = first + i;
       if (s >= end) break;
-      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
-      const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+      const std::uint64_t start = st[i];
+      const std::uint64_t size = st[i + 1] - st[i];
       if (fits && size <= small_max) {
         dest[s] = static_cast<Index>(q_off);
         new_id[s] = kSmallSegment;
