@@ -1400,13 +1400,22 @@ __global__ __launch_bounds__(256) void PartitionAssignKernel(
   std::uint64_t run_xs = part.x_segs, run_xe = part.x_elems, run_ys = part.y_segs, run_ye = part.y_elems;
   for (std::uint64_t base = begin; base < end; base += kSplitTile) {
     const std::uint64_t first = base + static_cast<std::uint64_t>(threadIdx.x) * kSplitItems;
+    // As in SplitAssignKernel: first slots loaded once for both loops.
+    std::uint64_t st[kSplitItems + 1];
+    std::uint32_t fl[kSplitItems];
+#pragma unroll
+    for (int i = 0; i <= kSplitItems; ++i) {
+      const std::uint64_t s = first + i;
+      st[i] = s < segs ? static_cast<std::uint64_t>(seg_start[s]) : m;
+      if (i < kSplitItems) fl[i] = s < end ? flags[s] : 0u;
+    }
     std::uint64_t xs = 0, xe = 0, ys = 0, ye = 0;
 #pragma unroll
     for (int i = 0; i < kSplitItems; ++i) {
       const std::uint64_t s = first + i;
       if (s < end) {
-        const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
-        if (flags[s] != 0) {
+        const std::uint64_t size = st[i + 1] - st[i];
+        if (fl[i] != 0) {
           xe += size;
           ++xs;
         } else {
@@ -1424,10 +1433,10 @@ __global__ __launch_bounds__(256) void PartitionAssignKernel(
     for (int i = 0; i < kSplitItems; ++i) {
       const std::uint64_t s = first + i;
       if (s >= end) break;
-      const std::uint64_t start = static_cast<std::uint64_t>(seg_start[s]);
-      const std::uint64_t size = SegmentSize(seg_start, s, segs, m);
+      const std::uint64_t start = st[i];
+      const std::uint64_t size = st[i + 1] - st[i];
       const Index row0 = static_cast<Index>(seg_delta[s] + static_cast<Index>(start));
-      if (flags[s] != 0) {
+      if (fl[i] != 0) {
         dest[s] = static_cast<Index>(xe_off);
         new_id[s] = static_cast<std::uint32_t>(xs_off);
         x_delta[xs_off] = static_cast<Index>(row0 - static_cast<Index>(xe_off));
